@@ -1,0 +1,195 @@
+// KoalaBear field (p = 2^31 - 2^24 + 1) in Montgomery form, and the degree-6
+// extension E6 = E2[v]/(v^3 - (u+1)), E2 = F_p[u]/(u^2 - 3).
+//
+// Elements are one u32 in Montgomery form (x*2^32 mod p), the in-memory layout
+// of gnark-crypto's koalabear.Element that the reference prover uses.  E6
+// limbs follow the reference order (B0.A0, B0.A1, B1.A0, B1.A1, B2.A0, B2.A1)
+// (internal/hash/hash.go:59-68); the multiplication is the schoolbook product
+// restated in recursion/e6.go:21-53.
+//
+// Usable from host code (g++ / hipcc host pass), device code and hiprtc.
+#pragma once
+#ifndef LG_HIPRTC
+#include <cstdint>
+#endif
+
+#if defined(__HIPCC__) || defined(__HIPCC_RTC__)
+#define LG_HD __host__ __device__ __forceinline__
+#else
+#define LG_HD inline
+#endif
+
+namespace lg {
+
+constexpr uint32_t KB_P = 0x7f000001u;
+constexpr uint32_t KB_PINV = 0x81000001u;   // p^-1 mod 2^32
+constexpr uint32_t KB_R = 0x01fffffeu;      // 2^32 mod p  (Montgomery one)
+constexpr uint32_t KB_R2 = 0x17f7efe4u;     // 2^64 mod p
+
+LG_HD uint32_t kb_add(uint32_t a, uint32_t b) {
+  uint32_t r = a + b;
+  return r >= KB_P ? r - KB_P : r;
+}
+
+LG_HD uint32_t kb_sub(uint32_t a, uint32_t b) {
+  uint32_t r = a - b;
+  return a < b ? r + KB_P : r;
+}
+
+LG_HD uint32_t kb_neg(uint32_t a) { return a ? KB_P - a : 0u; }
+
+// Montgomery product a*b*2^-32 mod p for a, b < p.
+LG_HD uint32_t kb_mul(uint32_t a, uint32_t b) {
+  uint64_t t = (uint64_t)a * b;
+  uint32_t m = (uint32_t)t * KB_PINV;
+  uint32_t th = (uint32_t)(t >> 32);
+  uint32_t mh = (uint32_t)(((uint64_t)m * KB_P) >> 32);
+  uint32_t r = th - mh;
+  return th < mh ? r + KB_P : r;
+}
+
+LG_HD uint32_t kb_dbl(uint32_t a) { return kb_add(a, a); }
+
+LG_HD uint32_t kb_to_mont(uint32_t x) { return kb_mul(x % KB_P, KB_R2); }
+LG_HD uint32_t kb_from_mont(uint32_t x) { return kb_mul(x, 1u); }
+
+LG_HD uint32_t kb_pow(uint32_t a, uint64_t e) {
+  uint32_t r = KB_R;
+  while (e) {
+    if (e & 1) r = kb_mul(r, a);
+    a = kb_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+LG_HD uint32_t kb_inv(uint32_t a) { return kb_pow(a, KB_P - 2); }
+
+// ------------------------------------------------------------------ E2 / E6
+struct E2 {
+  uint32_t a0, a1;
+};
+
+struct E6 {
+  uint32_t v[6];
+};
+
+LG_HD E2 e2_add(E2 x, E2 y) { return {kb_add(x.a0, y.a0), kb_add(x.a1, y.a1)}; }
+LG_HD E2 e2_sub(E2 x, E2 y) { return {kb_sub(x.a0, y.a0), kb_sub(x.a1, y.a1)}; }
+LG_HD uint32_t kb_mul3(uint32_t a) { return kb_add(kb_dbl(a), a); }
+// (a0 + a1 u)(b0 + b1 u) = (a0 b0 + 3 a1 b1) + (a0 b1 + a1 b0) u
+LG_HD E2 e2_mul(E2 x, E2 y) {
+  uint32_t t = kb_mul(x.a1, y.a1);
+  return {kb_add(kb_mul(x.a0, y.a0), kb_mul3(t)), kb_add(kb_mul(x.a0, y.a1), kb_mul(x.a1, y.a0))};
+}
+// times xi = 1 + u: (p + q u)(1 + u) = (p + 3q) + (p + q) u
+LG_HD E2 e2_mul_xi(E2 x) { return {kb_add(x.a0, kb_mul3(x.a1)), kb_add(x.a0, x.a1)}; }
+LG_HD E2 e2_scale(E2 x, uint32_t s) { return {kb_mul(x.a0, s), kb_mul(x.a1, s)}; }
+
+LG_HD E2 e6_c(const E6& x, int i) { return {x.v[2 * i], x.v[2 * i + 1]}; }
+
+LG_HD E6 e6_make(E2 c0, E2 c1, E2 c2) {
+  E6 r;
+  r.v[0] = c0.a0; r.v[1] = c0.a1; r.v[2] = c1.a0; r.v[3] = c1.a1; r.v[4] = c2.a0; r.v[5] = c2.a1;
+  return r;
+}
+
+LG_HD E6 e6_zero() {
+  E6 r;
+  for (int i = 0; i < 6; ++i) r.v[i] = 0;
+  return r;
+}
+
+LG_HD E6 e6_one() {
+  E6 r = e6_zero();
+  r.v[0] = KB_R;
+  return r;
+}
+
+LG_HD E6 e6_from_base(uint32_t a) {
+  E6 r = e6_zero();
+  r.v[0] = a;
+  return r;
+}
+
+LG_HD E6 e6_add(const E6& x, const E6& y) {
+  E6 r;
+  for (int i = 0; i < 6; ++i) r.v[i] = kb_add(x.v[i], y.v[i]);
+  return r;
+}
+
+LG_HD E6 e6_sub(const E6& x, const E6& y) {
+  E6 r;
+  for (int i = 0; i < 6; ++i) r.v[i] = kb_sub(x.v[i], y.v[i]);
+  return r;
+}
+
+LG_HD E6 e6_neg(const E6& x) {
+  E6 r;
+  for (int i = 0; i < 6; ++i) r.v[i] = kb_neg(x.v[i]);
+  return r;
+}
+
+LG_HD E6 e6_scale(const E6& x, uint32_t s) {
+  E6 r;
+  for (int i = 0; i < 6; ++i) r.v[i] = kb_mul(x.v[i], s);
+  return r;
+}
+
+LG_HD E6 e6_add_base(const E6& x, uint32_t b) {
+  E6 r = x;
+  r.v[0] = kb_add(r.v[0], b);
+  return r;
+}
+
+LG_HD E6 e6_mul(const E6& x, const E6& y) {
+  E2 x0 = e6_c(x, 0), x1 = e6_c(x, 1), x2 = e6_c(x, 2);
+  E2 y0 = e6_c(y, 0), y1 = e6_c(y, 1), y2 = e6_c(y, 2);
+  E2 p00 = e2_mul(x0, y0), p01 = e2_mul(x0, y1), p02 = e2_mul(x0, y2);
+  E2 p10 = e2_mul(x1, y0), p11 = e2_mul(x1, y1), p12 = e2_mul(x1, y2);
+  E2 p20 = e2_mul(x2, y0), p21 = e2_mul(x2, y1), p22 = e2_mul(x2, y2);
+  E2 z0 = e2_add(p00, e2_mul_xi(e2_add(p12, p21)));
+  E2 z1 = e2_add(e2_add(p01, p10), e2_mul_xi(p22));
+  E2 z2 = e2_add(e2_add(p02, p11), p20);
+  return e6_make(z0, z1, z2);
+}
+
+LG_HD E2 e2_inv(E2 x) {
+  uint32_t n = kb_sub(kb_mul(x.a0, x.a0), kb_mul3(kb_mul(x.a1, x.a1)));
+  uint32_t ni = kb_inv(n);
+  return {kb_mul(x.a0, ni), kb_mul(kb_neg(x.a1), ni)};
+}
+
+LG_HD E6 e6_inv(const E6& x) {
+  E2 a0 = e6_c(x, 0), a1 = e6_c(x, 1), a2 = e6_c(x, 2);
+  E2 c0 = e2_sub(e2_mul(a0, a0), e2_mul_xi(e2_mul(a1, a2)));
+  E2 c1 = e2_sub(e2_mul_xi(e2_mul(a2, a2)), e2_mul(a0, a1));
+  E2 c2 = e2_sub(e2_mul(a1, a1), e2_mul(a0, a2));
+  E2 t = e2_add(e2_mul(a0, c0), e2_mul_xi(e2_add(e2_mul(a2, c1), e2_mul(a1, c2))));
+  E2 ti = e2_inv(t);
+  return e6_make(e2_mul(c0, ti), e2_mul(c1, ti), e2_mul(c2, ti));
+}
+
+LG_HD bool e6_is_zero(const E6& x) {
+  uint32_t acc = 0;
+  for (int i = 0; i < 6; ++i) acc |= x.v[i];
+  return acc == 0;
+}
+
+LG_HD bool e6_eq(const E6& x, const E6& y) {
+  uint32_t acc = 0;
+  for (int i = 0; i < 6; ++i) acc |= x.v[i] ^ y.v[i];
+  return acc == 0;
+}
+
+LG_HD E6 e6_pow(E6 a, uint64_t e) {
+  E6 r = e6_one();
+  while (e) {
+    if (e & 1) r = e6_mul(r, a);
+    a = e6_mul(a, a);
+    e >>= 1;
+  }
+  return r;
+}
+
+}  // namespace lg

@@ -1,0 +1,129 @@
+// Poseidon2 over KoalaBear (x^3 S-box, 6 full + 21 partial rounds) on a state
+// held in registers, one state per lane.  Structure: external layer, 3 full
+// rounds, 21 partial rounds, 3 full rounds (gnark-crypto Permutation, restated
+// in recursion/p2_air.go:78-117).  External layer = M4 on each 4-lane chunk
+// followed by adding the chunk-wise sums (circ(2*M4, M4, ...)); internal layer
+// s_i <- sum(s) + d_i * s_i with the diagonal of recursion/poseidon2.go:48-90.
+//
+// Also the loom hashing formats built on the width-24 permutation:
+//   node  = P([NODE, 0 x 7, L[0..8], R[0..8]])[0..8]   (internal/hash/poseidon2.go:409-422)
+//   leaf  = sponge([LEAF, 2nb, 2ne, lo.base, hi.base, lo.ext, hi.ext])   (internal/fri/hash.go:39-56)
+//   sponge: rate 16, overwrite mode, partial last block overwrites only its
+//   lanes, digest = state[0..8] (internal/hash/poseidon2.go:165-255).
+#pragma once
+#include "kb.h"
+#include "poseidon2_constants.h"
+
+namespace lg {
+
+constexpr uint32_t TAG_LEAF = 0x4c454146u;
+constexpr uint32_t TAG_NODE = 0x4e4f4445u;
+constexpr uint32_t TAG_LEAF_MONT = 0x6d420179u;  // Montgomery form of TAG_LEAF
+constexpr uint32_t TAG_NODE_MONT = 0x6425c2feu;  // Montgomery form of TAG_NODE
+
+template <int W>
+LG_HD void p2_external(uint32_t* s) {
+#pragma unroll
+  for (int c = 0; c < W / 4; ++c) {
+    uint32_t x0 = s[4 * c], x1 = s[4 * c + 1], x2 = s[4 * c + 2], x3 = s[4 * c + 3];
+    uint32_t t01 = kb_add(x0, x1);
+    uint32_t t23 = kb_add(x2, x3);
+    uint32_t t0123 = kb_add(t01, t23);
+    uint32_t t01123 = kb_add(t0123, x1);
+    uint32_t t01233 = kb_add(t0123, x3);
+    s[4 * c + 3] = kb_add(kb_dbl(x0), t01233);
+    s[4 * c + 1] = kb_add(kb_dbl(x2), t01123);
+    s[4 * c + 0] = kb_add(t01, t01123);
+    s[4 * c + 2] = kb_add(t23, t01233);
+  }
+  uint32_t sum[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint32_t a = s[j];
+#pragma unroll
+    for (int c = 1; c < W / 4; ++c) a = kb_add(a, s[4 * c + j]);
+    sum[j] = a;
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], sum[i & 3]);
+}
+
+template <int W>
+struct P2Consts;
+
+template <>
+struct P2Consts<24> {
+  static LG_HD uint32_t full(int r, int i) { return P2_W24_FULL_RK[r][i]; }
+  static LG_HD uint32_t partial(int r) { return P2_W24_PARTIAL_RK[r]; }
+  static LG_HD uint32_t diag(int i) { return P2_W24_DIAG[i]; }
+};
+
+template <>
+struct P2Consts<16> {
+  static LG_HD uint32_t full(int r, int i) { return P2_W16_FULL_RK[r][i]; }
+  static LG_HD uint32_t partial(int r) { return P2_W16_PARTIAL_RK[r]; }
+  static LG_HD uint32_t diag(int i) { return P2_W16_DIAG[i]; }
+};
+
+LG_HD uint32_t kb_cube(uint32_t x) { return kb_mul(kb_mul(x, x), x); }
+
+template <int W>
+LG_HD void p2_full_round(uint32_t* s, int r) {
+#pragma unroll
+  for (int i = 0; i < W; ++i) s[i] = kb_cube(kb_add(s[i], P2Consts<W>::full(r, i)));
+  p2_external<W>(s);
+}
+
+template <int W>
+LG_HD void p2_partial_round(uint32_t* s, int r) {
+  s[0] = kb_cube(kb_add(s[0], P2Consts<W>::partial(r)));
+  uint32_t sum = s[0];
+#pragma unroll
+  for (int i = 1; i < W; ++i) sum = kb_add(sum, s[i]);
+#pragma unroll
+  for (int i = 0; i < W; ++i) s[i] = kb_add(sum, kb_mul(s[i], P2Consts<W>::diag(i)));
+}
+
+template <int W>
+LG_HD void p2_permute(uint32_t* s) {
+  p2_external<W>(s);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) p2_full_round<W>(s, r);
+#pragma unroll
+  for (int r = 0; r < 21; ++r) p2_partial_round<W>(s, r);
+#pragma unroll
+  for (int r = 3; r < 6; ++r) p2_full_round<W>(s, r);
+}
+
+// node = P([NODE, 0 x 7, L, R])[0..8]; l, r, out: 8 Montgomery elements each.
+LG_HD void p2_node(const uint32_t* l, const uint32_t* r, uint32_t* out) {
+  uint32_t s[24];
+  s[0] = TAG_NODE_MONT;
+#pragma unroll
+  for (int i = 1; i < 8; ++i) s[i] = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    s[8 + i] = l[i];
+    s[16 + i] = r[i];
+  }
+  p2_permute<24>(s);
+#pragma unroll
+  for (int i = 0; i < 8; ++i) out[i] = s[i];
+}
+
+// Host-side sponge over an explicit message (transcript, tests).
+inline void p2_sponge_host(const uint32_t* msg, size_t len, uint32_t* digest) {
+  uint32_t s[24] = {0};
+  if (len == 0) {
+    for (int i = 0; i < 8; ++i) digest[i] = 0;
+    return;
+  }
+  for (size_t off = 0; off < len; off += 16) {
+    size_t k = len - off < 16 ? len - off : 16;
+    for (size_t i = 0; i < k; ++i) s[i] = msg[off + i];
+    p2_permute<24>(s);
+  }
+  for (int i = 0; i < 8; ++i) digest[i] = s[i];
+}
+
+}  // namespace lg

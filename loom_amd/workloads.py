@@ -1,0 +1,87 @@
+"""Seeded synthetic statements for the prove-path workloads.
+
+``plonk_instance(log_n, seed)``: one plonk-like module of N = 2^log_n rows over
+KoalaBear, built like the reference's gnark PLONK modules
+(integration_test/gnark_plonk/plonk_gnark_utils.go:221-232, bench/main.go:93-104):
+gate qL*L + qR*R + qM*L*R + qO*O + qK committed as columns (no setup), and a
+copy constraint over (L, R, O) (arguments.CopyConstraint).
+
+Generator (numpy PCG64 seeded with `seed`):
+* the copy permutation first: 10% of the 3N wire cells, in random order, are
+  cut into cycles of length 2-4; sigma maps each cycle cell to the next one,
+  every other cell to itself;
+* L, R, O uniform in [0, p); each cycle then copies its first cell's value to
+  its other cells, so tied cells are equal;
+* per row a gate kind: 40% add (qL=qR=1, qO=-1, qM=0), 40% mul (qM=1, qO=-1,
+  qL=qR=0), 20% constant (qL=1, others 0);
+* qK = -(qL*L + qR*R + qM*L*R + qO*O), so every row satisfies the gate.
+"""
+import numpy as np
+
+from . import board as B
+from . import expr as E
+
+P = E.P
+
+
+def _names(i):
+    s = lambda x: "%s_%d" % (x, i)
+    return dict(module=s("plonk"), L=s("L"), R=s("R"), O=s("O"), QL=s("QL"), QR=s("QR"),
+                QM=s("QM"), QO=s("QO"), QK=s("QK"), perm=s("plonk.S"))
+
+
+def plonk_program(log_n, idx=0):
+    """The compiled program of one plonk-like module (same for every seed)."""
+    n = 1 << log_n
+    nm = _names(idx)
+    b = B.Builder()
+    m = B.Module(nm["module"], n)
+    qll = E.Col(nm["QL"]) * E.Col(nm["L"])
+    qrr = E.Col(nm["QR"]) * E.Col(nm["R"])
+    qmlr = E.Col(nm["QM"]) * E.Col(nm["L"]) * E.Col(nm["R"])
+    qoo = E.Col(nm["QO"]) * E.Col(nm["O"])
+    qk = E.Col(nm["QK"])
+    m.assert_zero(qll + qrr + qmlr + qoo + qk)
+    b.add_module(m)
+    return b, nm
+
+
+def plonk_instance(log_n, seed, idx=0):
+    """Returns (program, trace) with trace: name -> canonical uint64 array."""
+    n = 1 << log_n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    cells = 3 * n
+    sigma = np.arange(cells, dtype=np.int64)
+    tied = rng.permutation(cells)[: cells // 10]
+    cycles = []
+    pos = 0
+    while pos + 2 <= len(tied):
+        ln = int(rng.integers(2, 5))
+        ln = min(ln, len(tied) - pos)
+        cyc = tied[pos:pos + ln]
+        cycles.append(cyc)
+        for k in range(ln):
+            sigma[cyc[k]] = cyc[(k + 1) % ln]
+        pos += ln
+    w = rng.integers(0, P, size=cells, dtype=np.uint64)
+    for cyc in cycles:
+        w[cyc[1:]] = w[cyc[0]]
+    L, R, O = w[:n], w[n:2 * n], w[2 * n:]
+    kind = rng.choice(3, size=n, p=[0.4, 0.4, 0.2])
+    one, mone = np.uint64(1), np.uint64(P - 1)
+    zero = np.uint64(0)
+    ql = np.where(kind == 0, one, np.where(kind == 2, one, zero)).astype(np.uint64)
+    qr = np.where(kind == 0, one, zero).astype(np.uint64)
+    qm = np.where(kind == 1, one, zero).astype(np.uint64)
+    qo = np.where(kind == 2, zero, mone).astype(np.uint64)
+    pp = np.uint64(P)
+    acc = (ql * L % pp + qr * R % pp + qm * (L * R % pp) % pp + qo * O % pp) % pp
+    qk = (pp - acc) % pp
+
+    b, nm = plonk_program(log_n, idx)
+    S = B.PermutationGen(sigma, nm["perm"])
+    B.copy_constraint(b, nm["module"], [E.Col(nm["L"]), E.Col(nm["R"]), E.Col(nm["O"])], S)
+    prog = B.compile_program(b)
+    trace = {nm["L"]: L.copy(), nm["R"]: R.copy(), nm["O"]: O.copy(), nm["QL"]: ql, nm["QR"]: qr,
+             nm["QM"]: qm, nm["QO"]: qo, nm["QK"]: qk}
+    return prog, trace

@@ -1,0 +1,588 @@
+"""Whole prove phase on the CPU (numpy): the oracle the HIP prover is checked
+against, bit for bit.
+
+Follows prover/prover.go:142-719 and the PCS it drives:
+* transcript set-up and bindings -- prover.go:199-234 (HASH string, module
+  sizes), :321-358 (trace-round roots), :528-558 (AIR root, zeta);
+* GenCol -- board/module.go:77-101 (Lagrange), :229-293 (permutation support
+  ID_k = g^k * w^j and sigma columns);
+* grand product step -- board/step.go:301-327, poly/iop_utils.go:170-192,
+  :315-322 (gp[0] = 1, gp[i] = gp[i-1] * Num[i-1] / Den[i-1]);
+* commit -- fri/commitment.go:259-386 (RS encode, row-pair leaves, groups by
+  decreasing size, smaller groups as level injections);
+* AIR quotient -- poly/compute_quotient.go:223-424 (E/(X^N - 1) on the coset
+  g*<w_{4N}>), :484-499 (back to coefficients), prover.go:422-440 (chunks);
+* claimed values -- fri/values.go:46-110, poly/utils.go:66-104;
+* alpha_DEEP bindings -- fri/open.go:151-167, 223-285;
+* DEEP quotient -- fri/deep.go:71-266 (per module class, on the trace domain,
+  then RS-encoded);
+* multi-degree FRI -- fri/fri.go:294-736, 1024-1272;
+* query openings -- fri/open.go:198-208, 290-416.
+
+Everything here is computed from the mathematics (interpolation, evaluation at
+points, Horner/barycentric forms), not from a transcription of the reference's
+FFT loops; where the reference's value is mathematically unique (quotient
+chunks, claimed values, DEEP codeword) any correct method gives the same bits.
+"""
+import numpy as np
+
+from . import field as F
+from .field import P, U64
+from . import ntt
+from . import poseidon2 as p2
+from .merkle import Tree
+from .transcript import Transcript, string_to_elements
+
+from loom_amd import expr as E
+from loom_amd import board as B
+from loom_amd import protocol as PR
+
+HASH_TAG = 0x48415348
+EXTP_TAG = 0x45585450
+FRI_N = PR.RATE << PR.MAX_LOG_MODULE_SIZE
+FRI_D = 1 << PR.MAX_LOG_MODULE_SIZE
+FRI_ROUNDS = PR.MAX_LOG_MODULE_SIZE
+INV2 = (P + 1) // 2
+
+
+def log2(n):
+    return n.bit_length() - 1
+
+
+# ------------------------------------------------------------------ commit
+class Committed:
+    """One committed tree: per group (in decreasing-size order) the LDE
+    columns, and the Merkle tree."""
+
+    def __init__(self, groups, rate=PR.RATE):
+        # groups: list of (N, [base (N,)], [ext (N,6)]) in declaration order
+        order = sorted(range(len(groups)), key=lambda k: -groups[k][0])
+        self.order = order
+        self.sources = []
+        leaves = []
+        for gi in order:
+            n, bases, exts = groups[gi]
+            lb = [ntt.encode(np.asarray(c, dtype=np.uint64), rate) for c in bases]
+            le = [ntt.encode_ext(np.asarray(c, dtype=np.uint64), rate) for c in exts]
+            self.sources.append((lb, le))
+            leaves.append(p2.hash_leaf_pairs(lb, le))
+        self.tree = Tree(leaves[0], leaves[1:])
+
+    def root(self):
+        return self.tree.root()
+
+
+# ------------------------------------------------------------------ eval
+def eval_expr(e, leaf_value, m):
+    """Evaluate an expression on m points, everything lifted to E6 (m, 6)."""
+    memo = {}
+
+    def ev(x):
+        k = id(x)
+        if k in memo:
+            return memo[k]
+        if isinstance(x, E.Leaf):
+            if x.type == E.CONSTANT:
+                r = np.zeros((m, 6), dtype=np.uint64)
+                r[:, 0] = x.value % P
+            else:
+                r = leaf_value(x)
+        elif isinstance(x, E.Pow):
+            r = F.e6_pow(ev(x.base), x.exp)
+        elif isinstance(x, E.Add):
+            r = F.e6_add(ev(x.left), ev(x.right))
+        elif isinstance(x, E.Sub):
+            r = F.e6_sub(ev(x.left), ev(x.right))
+        else:
+            r = F.e6_mul(ev(x.left), ev(x.right))
+        memo[k] = r
+        return r
+
+    return ev(e)
+
+
+def as_e6_rows(v, m):
+    v = np.asarray(v, dtype=np.uint64)
+    if v.ndim == 1 and v.shape[0] == 6 and m != 6:
+        return np.broadcast_to(v, (m, 6)).copy()
+    if v.ndim == 1:
+        return F.e6_from_base(v)
+    return v
+
+
+# ------------------------------------------------------------------ prover
+class Prover:
+    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT):
+        self.prog = program
+        self.layout = PR.build_layout(program)
+        self.nq = num_queries
+        self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items()}
+        self.ext = {}
+        self.air_ext = {}
+        self.committed = [None] * len(self.layout.trees)
+        self.commitments = []
+        self.log = []   # (challenge name, digest)
+
+    # -- transcript helpers
+    def compute(self, name):
+        v = self.ts.compute(name)
+        self.log.append((name, list(v)))
+        return v
+
+    def gencol(self):
+        for mn in sorted(self.prog.modules):
+            m = self.prog.modules[mn]
+            n = m.N
+            for g in m.gencol:
+                if isinstance(g, B.LagrangeGen):
+                    c = np.zeros(n, dtype=np.uint64)
+                    c[g.i] = 1
+                    self.base.setdefault(B.lagrange_name(mn, g.i), c)
+                elif isinstance(g, B.LagrangeRelativeGen):
+                    c = np.zeros(n, dtype=np.uint64)
+                    c[n - 1 - g.i] = 1
+                    self.base.setdefault(B.lagrange_name_relative(mn, g.i), c)
+                elif isinstance(g, B.PermutationGen):
+                    S = np.asarray(g.S, dtype=np.int64)
+                    k = len(S) // n
+                    sup = [F.powers(F.omega(n), n)]
+                    for _ in range(1, k):
+                        sup.append(sup[-1] * U64(F.MULT_GEN) % U64(P))
+                    sup = np.stack(sup)
+                    for i in range(k):
+                        self.base["%s.ID_%d_%d" % (mn, i, n)] = sup[i]
+                    flat = sup.reshape(-1)
+                    for i in range(k):
+                        self.base[g.chunk_name(i)] = flat[S[i * n:(i + 1) * n]]
+                else:
+                    raise NotImplementedError(type(g))
+
+    def leaf_values_lagrange(self, x, n):
+        if x.type == E.CHALLENGE:
+            return np.broadcast_to(self.ext[x.name][0], (n, 6)).copy()
+        if x.name in self.ext:
+            v = self.ext[x.name]
+        else:
+            v = F.e6_from_base(self.base[x.name])
+        if x.shift:
+            v = np.roll(v, -x.shift, axis=0)
+        return v
+
+    def run_steps(self, rd):
+        for st in rd.steps:
+            if st.kind != B.GRAND_PRODUCT:
+                raise NotImplementedError(st.kind)
+            n = self.prog.modules[st.module].N
+            num = eval_expr(st.ins[0], lambda x: self.leaf_values_lagrange(x, n), n)
+            den = eval_expr(st.ins[1], lambda x: self.leaf_values_lagrange(x, n), n)
+            ratio = F.e6_mul(num, F.e6_inv(den))
+            gp = np.zeros((n, 6), dtype=np.uint64)
+            gp[0, 0] = 1
+            for i in range(1, n):
+                gp[i] = F.e6_mul(gp[i - 1], ratio[i - 1])
+            self.ext[st.outs[0]] = gp
+
+    def group_columns(self, g, air=False):
+        if air:
+            return (g.N, [], [self.air_ext[nm] for nm in g.ext])
+        bases = [self.base[nm] for nm in g.base]
+        exts = []
+        for nm in g.ext:
+            exts.append(self.ext[nm] if nm in self.ext else F.e6_from_base(self.base[nm]))
+        return (g.N, bases, exts)
+
+    def commit_tree(self, t_idx, air=False):
+        t = self.layout.trees[t_idx]
+        c = Committed([self.group_columns(g, air) for g in t.groups])
+        self.committed[t_idx] = c
+        root = c.root()
+        self.commitments.append(root)
+        return root
+
+    # -- quotient
+    def quotient_chunks(self, mn):
+        m = self.prog.modules[mn]
+        n = m.N
+        big = PR.quotient_domain_size(m.degree, n)
+        rho = big // n
+        g = F.MULT_GEN
+        cache = {}
+
+        def coset_vals(name, is_ext):
+            if name not in cache:
+                if is_ext:
+                    vals = self.ext[name] if name in self.ext else F.e6_from_base(self.base[name])
+                    coeffs = ntt.interpolate(vals.T)
+                    cache[name] = ntt.coset_evaluate(coeffs, g, big).T.copy()
+                else:
+                    coeffs = ntt.interpolate(self.base[name])
+                    cache[name] = F.e6_from_base(ntt.coset_evaluate(coeffs, g, big))
+            return cache[name]
+
+        def leaf(x):
+            if x.type == E.CHALLENGE:
+                return np.broadcast_to(self.ext[x.name][0], (big, 6)).copy()
+            v = coset_vals(x.name, x.name in self.ext)
+            if x.shift:
+                v = np.roll(v, -rho * x.shift, axis=0)
+            return v
+
+        num = eval_expr(m.relation, leaf, big)
+        xs = F.powers(F.omega(big), big) * U64(g) % U64(P)
+        zh = (F.vpow(xs, n) + U64(P) - U64(1)) % U64(P)
+        q = F.e6_scale(num, F.vinv(zh))
+        # coefficients of Q: values at g*w^k -> c_m g^m -> c_m
+        cg = ntt.interpolate(q.T)
+        c = cg * F.powers(F.inv(g), big) % U64(P)
+        chunks = []
+        for t in range(rho):
+            ct = c[:, t * n:(t + 1) * n]
+            chunks.append(ntt.evaluate(ct).T.copy())
+        return chunks
+
+    # -- claimed values
+    def lagrange_weights(self, zeta, n):
+        w = F.omega(n)
+        wi = F.powers(w, n)
+        den = F.e6_sub(np.broadcast_to(zeta, (n, 6)), F.e6_from_base(wi))
+        inv = F.e6_inv(den)
+        zn = F.e6_pow(zeta, n)
+        common = F.e6_scale(F.e6_sub(zn, F.e6_one()), F.inv(n))
+        return F.e6_mul(F.e6_scale(np.broadcast_to(common, (n, 6)), wi), inv)
+
+    def eval_at(self, col, lag, s):
+        n = lag.shape[0]
+        idx = (np.arange(n) - s) % n
+        wts = lag[idx]
+        if col.ndim == 1:
+            t = F.e6_scale(wts, col)
+        else:
+            t = F.e6_mul(wts, col)
+        return t.sum(axis=0) % U64(P)
+
+    # -- main
+    def prove(self):
+        prog, lay = self.prog, self.layout
+        nr = len(prog.rounds)
+        names = [B.challenge_name(r) for r in range(nr)] + ["__zeta"]
+        self.ts = Transcript(names)
+        init = names[0]
+        self.ts.bind(init, string_to_elements(HASH_TAG, "poseidon2"))
+        self.ts.bind(init, PR.module_sizes_transcript(prog))
+        self.gencol()
+        for r, rd in enumerate(prog.rounds):
+            self.run_steps(rd)
+            cn = B.challenge_name(r)
+            if r in lay.trace_tree_of_round:
+                root = self.commit_tree(lay.trace_tree_of_round[r])
+                self.ts.bind(cn, root)
+            if rd.fshook != B.NO_FS_HOOK:
+                raise NotImplementedError("exposed values")
+            v = self.compute(cn)
+            self.ext[cn] = np.array([v[:6]], dtype=np.uint64)
+        # AIR
+        for mn in sorted(prog.modules):
+            if prog.modules[mn].degree <= 0:
+                continue
+            for i, ch in enumerate(self.quotient_chunks(mn)):
+                self.air_ext[PR.quotient_chunk_name(mn, i)] = ch
+        air_idx = [i for i, t in enumerate(lay.trees) if t.kind == "air"]
+        for t in air_idx:
+            root = self.commit_tree(t, air=True)
+            self.ts.bind("__zeta", root)
+        zeta = np.array(self.compute("__zeta")[:6], dtype=np.uint64)
+        self.zeta = zeta
+        return self.open(zeta)
+
+    def tree_polys(self, t_idx, g):
+        t = self.layout.trees[t_idx]
+        air = t.kind == "air"
+        n, bases, exts = self.group_columns(g, air)
+        return bases, exts
+
+    def open(self, zeta):
+        lay = self.layout
+        # classes: one per module, in sorted module order
+        class_of = {m: i for i, m in enumerate(lay.modules)}
+        sizes = {class_of[m]: self.prog.modules[m].N for m in lay.modules}
+        lag = {}
+        claimed = []
+        for ti, t in enumerate(lay.trees):
+            bv = []
+            for g in t.groups:
+                if g.N not in lag:
+                    lag[g.N] = self.lagrange_weights(zeta, g.N)
+                bases, exts = self.tree_polys(ti, g)
+                gb = [[self.eval_at(c, lag[g.N], s) for s in ss] for c, ss in zip(bases, g.base_shifts)]
+                ge = [[self.eval_at(c, lag[g.N], s) for s in ss] for c, ss in zip(exts, g.ext_shifts)]
+                bv.append((gb, ge))
+            claimed.append(bv)
+        self.ts.new_challenge("alpha_DEEP")
+        present = sorted(set(class_of[g.module] for t in lay.trees for g in t.groups))
+        for cl in present:
+            for ti, t in enumerate(lay.trees):
+                for gi, g in enumerate(t.groups):
+                    if class_of[g.module] != cl:
+                        continue
+                    gb, ge = claimed[ti][gi]
+                    for vals in gb + ge:
+                        for v in vals:
+                            self.ts.bind("alpha_DEEP", v)
+        alpha = np.array(self.compute("alpha_DEEP")[:6], dtype=np.uint64)
+        # DEEP codeword per class
+        deep = {}
+        for cl in present:
+            n = sizes[cl]
+            w = F.omega(n)
+            xs = F.e6_from_base(F.powers(w, n))
+            acc = np.zeros((n, 6), dtype=np.uint64)
+            a_run = F.e6_one()
+            points = {}
+            for ti, t in enumerate(lay.trees):
+                for gi, g in enumerate(t.groups):
+                    if class_of[g.module] != cl:
+                        continue
+                    bases, exts = self.tree_polys(ti, g)
+                    gb, ge = claimed[ti][gi]
+                    polys = [(c, ss, vs) for c, ss, vs in zip(bases, g.base_shifts, gb)]
+                    polys += [(c, ss, vs) for c, ss, vs in zip(exts, g.ext_shifts, ge)]
+                    for c, ss, vs in polys:
+                        cv = F.e6_from_base(c) if c.ndim == 1 else c
+                        spx = F.e6_mul(np.broadcast_to(a_run, (n, 6)), cv)
+                        for s, v in zip(ss, vs):
+                            zs = F.e6_scale(zeta, pow(w, s % n, P))
+                            key = tuple(int(x) for x in zs)
+                            sv = F.e6_mul(a_run, v)
+                            term = F.e6_sub(np.broadcast_to(sv, (n, 6)), spx)
+                            points[key] = F.e6_add(points.get(key, np.zeros((n, 6), dtype=np.uint64)), term)
+                        a_run = F.e6_mul(a_run, alpha)
+            for key, numer in points.items():
+                zs = np.array(key, dtype=np.uint64)
+                inv = F.e6_inv(F.e6_sub(np.broadcast_to(zs, (n, 6)), xs))
+                acc = F.e6_add(acc, F.e6_mul(numer, inv))
+            deep[cl] = ntt.encode_ext(acc, PR.RATE)
+        fri = self.fri_prove([(sizes[cl], deep[cl]) for cl in present])
+        samplings = []
+        for s in fri["positions"]:
+            samplings.append([self.open_committed(c, s) for c in self.committed])
+        return dict(hash_id="poseidon2", commitments=self.commitments, claimed=claimed,
+                    fri=fri, samplings=samplings)
+
+    # -- FRI
+    def fri_prove(self, levels):
+        order = sorted(range(len(levels)), key=lambda i: -levels[i][0])
+        levels = [levels[i] for i in order]
+        nl = len(levels)
+        for l in range(1, nl):
+            self.ts.new_challenge("fri_level_%d_gamma" % l)
+        for j in range(FRI_ROUNDS):
+            self.ts.new_challenge("fri_fold_%d" % j)
+        for k in range(self.nq):
+            self.ts.new_challenge("fri_query_%d" % k)
+        first = log2(FRI_D // levels[0][0])
+        at_round = {}
+        for l in range(1, nl):
+            at_round.setdefault(log2(FRI_D // levels[l][0]), []).append(l)
+        leaves = [p2.hash_leaf_pairs([], [lv[1]]) for lv in levels]
+        ltree = Tree(leaves[0], leaves[1:])
+        gammas = [None] * nl
+        for l in range(1, nl):
+            nm = "fri_level_%d_gamma" % l
+            if l == 1:
+                self.ts.bind(nm, ltree.root())
+            gammas[l] = np.array(self.compute(nm)[:6], dtype=np.uint64)
+        for j in range(first):
+            self.ts.bind("fri_fold_%d" % j, [0] * 8)
+            self.compute("fri_fold_%d" % j)
+        running = levels[0][1].copy()
+        layers = {}
+        trees = {}
+        roots = []
+        for j in range(first, FRI_ROUNDS):
+            if j == first:
+                layers[j] = levels[0][1]
+            for l in at_round.get(j, []):
+                running = F.e6_add(running, F.e6_mul(levels[l][1], np.broadcast_to(gammas[l], levels[l][1].shape)))
+            if j > first:
+                layers[j] = running
+                tree = Tree(p2.hash_leaf_pairs([], [running]))
+            else:
+                tree = ltree
+            trees[j] = tree
+            root = tree.root()
+            self.ts.bind("fri_fold_%d" % j, root)
+            a = np.array(self.compute("fri_fold_%d" % j)[:6], dtype=np.uint64)
+            if j > first:
+                roots.append(root)
+            running = self.fold(running, a, FRI_N >> j)
+        final = running
+        self.ts.bind("fri_query_0", [EXTP_TAG, len(final)] + [int(x) for x in final.reshape(-1)])
+        positions = []
+        queries = []
+        level_queries = [[] for _ in range(nl - 1)]
+        for k in range(self.nq):
+            ch = self.compute("fri_query_%d" % k)
+            s = ((ch[0] << 31) ^ ch[1]) % FRI_N
+            positions.append(s)
+            if k < self.nq - 1:
+                self.ts.bind("fri_query_%d" % (k + 1), ch)
+            q = []
+            for j in range(first, FRI_ROUNDS):
+                row = s >> j
+                lo = row & ~1
+                idx, sibs, injl = trees[j].open(lo // 2)
+                if j == first:
+                    injl = []
+                q.append(dict(row=row, p=layers[j][lo], q=layers[j][lo + 1], path=(idx, sibs, injl)))
+            queries.append(q)
+            for l in range(1, nl):
+                jl = log2(FRI_D // levels[l][0])
+                row = s >> jl
+                lo = row & ~1
+                level_queries[l - 1].append(dict(row=row, p=levels[l][1][lo], q=levels[l][1][lo + 1]))
+        return dict(levels_root=ltree.root(), level_queries=level_queries, roots=roots,
+                    final=final, queries=queries, positions=positions)
+
+    @staticmethod
+    def fold(layer, alpha, n):
+        half = n // 2
+        ginv = F.inv(F.omega(n))
+        xinv = F.powers(ginv, half)[F.bitrev_indices(half)]
+        p, q = layer[0::2], layer[1::2]
+        s = F.e6_scale(F.e6_add(p, q), np.full(half, INV2, dtype=np.uint64))
+        d = F.e6_scale(F.e6_sub(p, q), xinv * U64(INV2) % U64(P))
+        d = F.e6_mul(d, np.broadcast_to(alpha, d.shape))
+        return F.e6_add(s, d)
+
+    def open_committed(self, c, s):
+        lb, le = c.sources[0]
+        rows = (lb[0] if lb else le[0]).shape[0]
+        top_row = s >> (log2(FRI_N) - log2(rows))
+        lo = top_row & ~1
+        top = self.raw_pair(c.sources[0], lo)
+        path = c.tree.open(lo // 2)
+        inj = []
+        for src in c.sources[1:]:
+            b2, e2 = src
+            r2 = (b2[0] if b2 else e2[0]).shape[0]
+            row = s >> (log2(FRI_N) - log2(r2))
+            inj.append(self.raw_pair(src, row & ~1))
+        return dict(top=top, path=path, inj=inj)
+
+    @staticmethod
+    def raw_pair(src, lo):
+        lb, le = src
+        return ([[int(c[lo]) for c in lb], [[int(x) for x in c[lo]] for c in le]],
+                [[int(c[lo + 1]) for c in lb], [[int(x) for x in c[lo + 1]] for c in le]])
+
+
+# ------------------------------------------------------------------ encoding
+def _m(x):
+    return F.to_mont(int(x))
+
+
+def encode_proof(pf):
+    """Canonical u32 word stream of a proof (field elements in Montgomery
+    form).  The layout is the one loom_amd.proof decodes."""
+    out = []
+
+    def u(x):
+        out.append(int(x) & 0xFFFFFFFF)
+
+    def e6v(v):
+        for x in v:
+            out.append(_m(x))
+
+    def dig(d):
+        for x in d:
+            out.append(_m(x))
+
+    def string(s):
+        b = s.encode()
+        u(len(b))
+        for i in range(0, len(b), 4):
+            out.append(int.from_bytes(b[i:i + 4].ljust(4, b"\0"), "little"))
+
+    def path(p):
+        idx, sibs, injl = p
+        u(idx)
+        u(len(sibs))
+        for d in sibs:
+            dig(d)
+        u(len(injl))
+        for d in injl:
+            dig(d)
+
+    def rawrow(r):
+        bases, exts = r
+        u(len(bases))
+        for x in bases:
+            out.append(_m(x))
+        u(len(exts))
+        for v in exts:
+            e6v(v)
+
+    string(pf["hash_id"])
+    u(0)  # exposed values
+    u(len(pf["commitments"]))
+    for d in pf["commitments"]:
+        dig(d)
+    u(len(pf["claimed"]))
+    for bv in pf["claimed"]:
+        u(len(bv))
+        for gb, ge in bv:
+            for rail in (gb, ge):
+                u(len(rail))
+                for vals in rail:
+                    u(len(vals))
+                    for v in vals:
+                        e6v(v)
+    fr = pf["fri"]
+    dig(fr["levels_root"])
+    u(len(fr["level_queries"]))
+    for lq in fr["level_queries"]:
+        u(len(lq))
+        for q in lq:
+            u(1)
+            u(q["row"])
+            e6v(q["p"])
+            e6v(q["q"])
+            u(0)
+            u(0)
+            u(0)
+    u(len(fr["roots"]))
+    for d in fr["roots"]:
+        dig(d)
+    u(1)
+    u(len(fr["final"]))
+    for v in fr["final"]:
+        e6v(v)
+    u(len(fr["queries"]))
+    for q in fr["queries"]:
+        u(len(q))
+        for lay in q:
+            u(1)
+            u(lay["row"])
+            e6v(lay["p"])
+            e6v(lay["q"])
+            path(lay["path"])
+    u(0)  # proofs of work (grinding off)
+    u(len(pf["samplings"]))
+    for row in pf["samplings"]:
+        u(len(row))
+        for wp in row:
+            rawrow(wp["top"][0])
+            rawrow(wp["top"][1])
+            path(wp["path"])
+            u(len(wp["inj"]))
+            for lo, hi in wp["inj"]:
+                rawrow(lo)
+                rawrow(hi)
+    return np.array(out, dtype=np.uint32)
+
+
+def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT):
+    """Returns (proof dict, canonical u32 encoding, transcript log)."""
+    pv = Prover(program, trace, num_queries)
+    pf = pv.prove()
+    return pf, encode_proof(pf), pv.log
