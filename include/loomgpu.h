@@ -1,0 +1,128 @@
+/*
+ * loomgpu -- C ABI of the MI355X prove phase for loom (KoalaBear / Poseidon2 /
+ * multi-degree FRI).
+ *
+ * The reference has no FFI layer: its prove path sits behind the Go function
+ *   prover.Prove(t trace.Trace, pk setup.ProvingKey, pi public.Inputs,
+ *                program board.Program, opts ...prover.Option) (proof.Proof, error)
+ * (prover/prover.go:655-719; loom.Prove wraps it, statement.go:91-97).  The entry
+ * points below are what a cgo shim of that function binds: the shim
+ * serialises the compiled program (board.Compile, program.go:45-89) and its
+ * commitment layout (protocol.BuildLayout / BuildCanonicalSchedule,
+ * internal/protocol/layout.go:165-241, schedule.go:92-235) into a plan once,
+ * then hands trace columns (Montgomery u32, the in-memory form of
+ * []koalabear.Element) to lg_prove and decodes the canonical proof words back
+ * into proof.Proof (proof/proof.go:31-51).
+ *
+ * Conventions: every function returns 0 on success and a negative code on
+ * failure; lg_last_error() returns the failure message of the calling thread.
+ * No entry point retains a caller pointer after it returns.  A context owns
+ * one HIP stream on one device; calls on one context are serialised by the
+ * library, several contexts may run concurrently.
+ *
+ * Plan word stream: see loom_amd/plan.py (module sizes, column table, rounds
+ * and prover steps, commitment trees with opening shifts, folded constraint
+ * DAGs).
+ *
+ * Proof word stream (all u32; field elements Montgomery; E6 = 6 words in the
+ * order B0.A0 B0.A1 B1.A0 B1.A1 B2.A0 B2.A1):
+ *   hash id (byte length, then bytes packed 4 per word, little endian)
+ *   #exposed (= 0)
+ *   #commitments, roots (8 words each; trace rounds then AIR)
+ *   claimed values: #trees { #groups { #base { #shifts E6... } #ext { #shifts E6... } } }
+ *   FRI: levels root; #levels-1 { #queries { 1, row, P, Q, 0, 0, 0 } };
+ *        #roots, roots; 1 (ext), #final, final E6...;
+ *        #queries { #layers { 1, row, P, Q, path } }; 0 (proofs of work)
+ *   samplings: #queries { #trees { lo row, hi row, path, #inj, lo row, hi row ... } }
+ *   path := leaf index, #siblings, siblings (8 each), #injection leaves, leaves (8 each)
+ *   row  := #base, base values, #ext, ext values (6 each)
+ */
+#ifndef LOOMGPU_H
+#define LOOMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lg_ctx lg_ctx;
+typedef struct lg_plan lg_plan;
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* lg_last_error(void);
+
+/* Context on HIP device `device` (one stream at default priority). */
+int lg_ctx_create(int device, lg_ctx** out);
+void lg_ctx_destroy(lg_ctx* ctx);
+
+/* Parses a plan and compiles its constraint DAGs (hiprtc, gfx950).  The
+ * compile step replaces the reference's per-row DAG interpreter
+ * (internal/dag/dag.go:1134-1446) and runs once per distinct DAG. */
+int lg_plan_create(lg_ctx* ctx, const uint32_t* words, size_t n_words, lg_plan** out);
+void lg_plan_destroy(lg_plan* plan);
+/* Parses a plan and compiles its DAG kernels with hiprtc without touching a
+ * device (no context needed): a validation step a caller can run at
+ * board.Compile time.  Returns the total code-object bytes in *code_bytes. */
+int lg_plan_check(const uint32_t* words, size_t n_words, size_t* code_bytes);
+
+/* Number of trace input columns / permutations an instance must provide. */
+int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms);
+
+/* Proves `batch` independent instances of one plan (same shapes, different
+ * witnesses), in lockstep: one launch sequence per phase for all instances.
+ * With batch = 1 this is prover.Prove for one statement.
+ *   columns[b * n_columns + k]: host pointer, N words (Montgomery), column k of
+ *                               instance b in plan input order
+ *   perms[b * n_perms + p]:     host pointer, permutation p of instance b
+ *                               (the board.PermutationGen S slice, as u32)
+ *   proofs[b], proof_caps[b]:   caller buffers for the proof words
+ *   proof_lens[b]:              receives the proof length in words
+ *   transcripts[b]:             optional (may be NULL): receives the challenge
+ *                               digests (8 words each) in computation order,
+ *                               capacity transcript_caps[b] words
+ * Returns -2 when a buffer is too small (lengths are still reported). */
+int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+             uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
+             const size_t* transcript_caps, size_t* transcript_lens);
+
+/* Phase timings of the last lg_prove on this plan (seconds): execute-steps,
+ * compute-air-quotients, pcs-open (the phases prover.WithPhaseCallback
+ * reports, prover.go:682-716). */
+int lg_plan_last_phases(const lg_plan* plan, double* out3);
+
+/* ---- building blocks (device pointers), the minimum slice of the path ---- */
+
+/* Poseidon2 row-pair leaf hashing of one committed group
+ * (Poseidon2LeafHasher.HashLeafPair, internal/fri/hash.go:39-56): for pair
+ * leaf k < rows/2, hashes [LEAF, 2nb, 2ne, lo base, hi base, lo ext, hi ext]
+ * where base column c is base_cols[c] and ext column e is the 6 limb arrays
+ * ext_cols[e] + l*rows (l < 6), rows 2k and 2k+1.  digests: rows/2 x 8 words. */
+int lg_hash_leaf_pairs(lg_ctx* ctx, const uint32_t* const* base_cols, int nb, const uint32_t* const* ext_cols, int ne,
+                       size_t rows, uint32_t* digests);
+
+/* Merkle tree over n_leaves leaf digests (merkle.Tree.Build,
+ * internal/merkle/tree.go:182-296, Poseidon2NodeCompress nodes): nodes is the
+ * 1-indexed flat array of 2*n_leaves digests whose leaves (nodes[n_leaves..])
+ * are already filled; on return nodes[1] is the root.  Level injections:
+ * inj[k] (inj_width[k] digests, non-increasing widths) fold into the level of
+ * that width (merkle.LevelInjection). */
+int lg_merkle_build(lg_ctx* ctx, uint32_t* nodes, size_t n_leaves, const uint32_t* const* inj, const size_t* inj_width,
+                    int n_inj);
+
+/* Grand product (board.GrandProductStep, poly/iop_utils.go:170-192, 315-322):
+ * gp[0] = 1, gp[i] = gp[i-1] * num[i-1] / den[i-1] over E6 columns given as
+ * 6 limb arrays of n words each (limb l at +l*n), n a power of two. */
+int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n);
+
+/* RS encoding (reedsolomon.Encoder.Encode, internal/reedsolomon/rs.go:52-70) of
+ * n_cols columns of n words each (Lagrange form on the size-n subgroup, one
+ * column after the other) to 4n bit-reversed evaluations each. */
+int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LOOMGPU_H */

@@ -1,0 +1,57 @@
+"""ctypes binding of libloomgpu.so (declarations: include/loomgpu.h).
+
+The library is built in-tree (``python -m loom_amd.build``); importing this
+module without it raises immediately -- there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libloomgpu.so")
+
+u32p = ctypes.POINTER(ctypes.c_uint32)
+u32pp = ctypes.POINTER(u32p)
+sizep = ctypes.POINTER(ctypes.c_size_t)
+
+EXPORTS = {
+    "lg_last_error": (ctypes.c_char_p, []),
+    "lg_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    "lg_ctx_destroy": (None, [ctypes.c_void_p]),
+    "lg_plan_create": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
+    "lg_plan_destroy": (None, [ctypes.c_void_p]),
+    "lg_plan_check": (ctypes.c_int, [u32p, ctypes.c_size_t, sizep]),
+    "lg_plan_num_inputs": (ctypes.c_int, [ctypes.c_void_p, sizep, sizep]),
+    "lg_prove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, u32pp, sizep, sizep, u32pp, sizep, sizep]),
+    "lg_plan_last_phases": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
+    "lg_hash_leaf_pairs": (ctypes.c_int, [ctypes.c_void_p, u32pp, ctypes.c_int, u32pp, ctypes.c_int, ctypes.c_size_t, u32p]),
+    "lg_merkle_build": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, u32pp, sizep, ctypes.c_int]),
+    "lg_grand_product": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_size_t]),
+    "lg_rs_encode": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, ctypes.c_int, u32p]),
+}
+
+_lib = None
+
+
+class LoomGpuError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LoomGpuError("libloomgpu.so is not built (run `python -m loom_amd.build`); "
+                               "the prove path has no CPU fallback")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in EXPORTS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc, what):
+    if rc != 0:
+        msg = lib().lg_last_error().decode(errors="replace")
+        raise LoomGpuError("%s failed (%d): %s" % (what, rc, msg))

@@ -1,0 +1,291 @@
+// extern "C" boundary of libloomgpu.so (declarations and contracts in
+// include/loomgpu.h).  Exceptions never cross it: they become a negative
+// return code and a thread-local message.
+#include "../../include/loomgpu.h"
+
+#include <cstring>
+#include <exception>
+#include <mutex>
+#include <string>
+
+#include "kernels.h"
+#include "p2.h"
+#include "prover.h"
+
+struct lg_ctx {
+  lg::Context c;
+  std::mutex mu;
+};
+
+struct lg_plan {
+  lg_ctx* ctx;
+  std::unique_ptr<lg::Plan> p;
+  double phases[3] = {0, 0, 0};
+};
+
+namespace {
+thread_local std::string g_err;
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -1;
+  }
+}
+
+lg::DevBuf stage_sync(const void* src, size_t bytes) {
+  lg::DevBuf b(bytes);
+  LG_HIP(hipMemcpy(b.as<void>(), src, bytes, hipMemcpyHostToDevice));
+  return b;
+}
+}  // namespace
+
+extern "C" {
+
+const char* lg_last_error(void) { return g_err.c_str(); }
+
+int lg_ctx_create(int device, lg_ctx** out) {
+  return guarded([&] {
+    if (!out) throw std::runtime_error("lg_ctx_create: null out");
+    int n = 0;
+    LG_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) throw std::runtime_error("lg_ctx_create: no such HIP device");
+    LG_HIP(hipSetDevice(device));
+    auto* c = new lg_ctx();
+    c->c.device = device;
+    if (hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess) {
+      delete c;
+      throw std::runtime_error("lg_ctx_create: stream creation failed");
+    }
+    *out = c;
+    return 0;
+  });
+}
+
+void lg_ctx_destroy(lg_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->c.device);
+  (void)hipStreamSynchronize(ctx->c.stream);
+  (void)hipStreamDestroy(ctx->c.stream);
+  delete ctx;
+}
+
+int lg_plan_create(lg_ctx* ctx, const uint32_t* words, size_t n_words, lg_plan** out) {
+  return guarded([&] {
+    if (!ctx || !words || !out) throw std::runtime_error("lg_plan_create: null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    auto* p = new lg_plan();
+    p->ctx = ctx;
+    try {
+      p->p = std::make_unique<lg::Plan>(&ctx->c, words, n_words);
+    } catch (...) {
+      delete p;
+      throw;
+    }
+    *out = p;
+    return 0;
+  });
+}
+
+void lg_plan_destroy(lg_plan* plan) { delete plan; }
+
+int lg_plan_check(const uint32_t* words, size_t n_words, size_t* code_bytes) {
+  return guarded([&] {
+    if (!words) throw std::runtime_error("lg_plan_check: null words");
+    lg::PlanData d = lg::parse_plan(words, n_words);
+    size_t total = 0;
+    auto one = [&](const lg::Dag& dag) {
+      std::vector<uint32_t> c, h;
+      total += lg::dag_kernel_compile_only(lg::dag_kernel_source(dag, d.cols, c, h));
+    };
+    for (auto& m : d.modules)
+      if (m.n_chunks) one(m.relation);
+    for (auto& r : d.rounds)
+      for (auto& st : r.steps)
+        for (auto& dag : st.ins) one(dag);
+    if (code_bytes) *code_bytes = total;
+    return 0;
+  });
+}
+
+int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms) {
+  return guarded([&] {
+    if (!plan) throw std::runtime_error("lg_plan_num_inputs: null plan");
+    if (n_columns) *n_columns = plan->p->num_inputs();
+    if (n_perms) *n_perms = plan->p->num_perms();
+    return 0;
+  });
+}
+
+int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+             uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
+             const size_t* transcript_caps, size_t* transcript_lens) {
+  return guarded([&] {
+    if (!plan || batch <= 0 || !proofs || !proof_caps || !proof_lens) throw std::runtime_error("lg_prove: bad arguments");
+    const size_t nc = plan->p->num_inputs(), np = plan->p->num_perms();
+    if ((nc && !columns) || (np && !perms)) throw std::runtime_error("lg_prove: missing inputs");
+    std::vector<lg::InstanceInput> in(batch);
+    for (int b = 0; b < batch; ++b) {
+      for (size_t k = 0; k < nc; ++k) {
+        if (!columns[b * nc + k]) throw std::runtime_error("lg_prove: null column pointer");
+        in[b].columns.push_back(columns[b * nc + k]);
+      }
+      for (size_t k = 0; k < np; ++k) {
+        if (!perms[b * np + k]) throw std::runtime_error("lg_prove: null permutation pointer");
+        in[b].perms.push_back(perms[b * np + k]);
+      }
+    }
+    std::vector<lg::ProveOutput> out;
+    {
+      std::lock_guard<std::mutex> lock(plan->ctx->mu);
+      int ph = 0;
+      double* phases = plan->phases;
+      plan->p->prove(in, out, [&](const char*, double dt) {
+        if (ph < 3) phases[ph++] = dt;
+      });
+    }
+    int rc = 0;
+    for (int b = 0; b < batch; ++b) {
+      proof_lens[b] = out[b].proof.size();
+      if (out[b].proof.size() > proof_caps[b]) {
+        rc = -2;
+      } else {
+        std::memcpy(proofs[b], out[b].proof.data(), out[b].proof.size() * 4);
+      }
+      if (transcripts && transcripts[b]) {
+        if (transcript_lens) transcript_lens[b] = out[b].transcript.size();
+        if (!transcript_caps || out[b].transcript.size() > transcript_caps[b]) rc = -2;
+        else std::memcpy(transcripts[b], out[b].transcript.data(), out[b].transcript.size() * 4);
+      }
+    }
+    if (rc) g_err = "lg_prove: output buffer too small";
+    return rc;
+  });
+}
+
+int lg_plan_last_phases(const lg_plan* plan, double* out3) {
+  return guarded([&] {
+    if (!plan || !out3) throw std::runtime_error("lg_plan_last_phases: null argument");
+    for (int i = 0; i < 3; ++i) out3[i] = plan->phases[i];
+    return 0;
+  });
+}
+
+int lg_hash_leaf_pairs(lg_ctx* ctx, const uint32_t* const* base_cols, int nb, const uint32_t* const* ext_cols, int ne,
+                       size_t rows, uint32_t* digests) {
+  return guarded([&] {
+    if (!ctx || !digests || rows < 2 || (rows & (rows - 1)) || nb < 0 || ne < 0 || (nb + ne) == 0)
+      throw std::runtime_error("lg_hash_leaf_pairs: bad arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    LG_HIP(hipSetDevice(ctx->c.device));
+    std::vector<const uint32_t*> cp;
+    for (int i = 0; i < nb; ++i) cp.push_back(base_cols[i]);
+    for (int e = 0; e < ne; ++e)
+      for (int l = 0; l < 6; ++l) cp.push_back(ext_cols[e] + (size_t)l * rows);
+    std::vector<int32_t> msg(3, 0);
+    for (int i = 0; i < nb; ++i) msg.push_back(2 * i);
+    for (int i = 0; i < nb; ++i) msg.push_back(2 * i + 1);
+    for (int i = 0; i < 6 * ne; ++i) msg.push_back(2 * (nb + i));
+    for (int i = 0; i < 6 * ne; ++i) msg.push_back(2 * (nb + i) + 1);
+    lg::DevBuf cpd = stage_sync(cp.data(), cp.size() * sizeof(void*));
+    lg::DevBuf md = stage_sync(msg.data(), msg.size() * 4);
+    uint32_t* o = digests;
+    lg::DevBuf od = stage_sync(&o, sizeof(void*));
+    lg::launch_leaf_hash(cpd.as<const uint32_t*>(), (int)cp.size(), md.as<int32_t>(), (int)msg.size(),
+                         lg::kb_to_mont(2 * nb), lg::kb_to_mont(2 * ne), od.as<uint32_t*>(), (int)(rows / 2), 1,
+                         ctx->c.stream);
+    LG_HIP(hipStreamSynchronize(ctx->c.stream));
+    return 0;
+  });
+}
+
+int lg_merkle_build(lg_ctx* ctx, uint32_t* nodes, size_t n_leaves, const uint32_t* const* inj, const size_t* inj_width,
+                    int n_inj) {
+  return guarded([&] {
+    if (!ctx || !nodes || n_leaves < 1 || (n_leaves & (n_leaves - 1)) || n_inj < 0)
+      throw std::runtime_error("lg_merkle_build: bad arguments");
+    size_t prev = n_leaves;
+    for (int k = 0; k < n_inj; ++k) {
+      if (!inj[k] || inj_width[k] == 0 || (inj_width[k] & (inj_width[k] - 1)) || inj_width[k] > prev)
+        throw std::runtime_error("lg_merkle_build: injection widths must be non-increasing powers of two <= n_leaves");
+      prev = inj_width[k];
+    }
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    LG_HIP(hipSetDevice(ctx->c.device));
+    hipStream_t s = ctx->c.stream;
+    uint32_t* tp = nodes;
+    lg::DevBuf tpd = stage_sync(&tp, sizeof(void*));
+    uint32_t* const* tp_d = tpd.as<uint32_t*>();
+    std::vector<lg::DevBuf> keep;
+    int next = 0;
+    auto fold = [&](size_t width) {
+      while (next < n_inj && inj_width[next] == width) {
+        const uint32_t* ip = inj[next];
+        keep.push_back(stage_sync(&ip, sizeof(void*)));
+        lg::launch_inject(tp_d, keep.back().as<const uint32_t*>(), (int)width, 1, s);
+        ++next;
+      }
+    };
+    fold(n_leaves);
+    for (size_t w = n_leaves / 2; w >= 1; w /= 2) {
+      lg::launch_merkle_level(tp_d, (int)w, 1, s);
+      fold(w);
+    }
+    LG_HIP(hipStreamSynchronize(s));
+    return 0;
+  });
+}
+
+int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n) {
+  return guarded([&] {
+    if (!ctx || !num || !den || !gp || n < 1 || (n & (n - 1))) throw std::runtime_error("lg_grand_product: bad arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    LG_HIP(hipSetDevice(ctx->c.device));
+    const uint32_t* np_ = num;
+    const uint32_t* dp_ = den;
+    uint32_t* gp_ = gp;
+    lg::DevBuf a = stage_sync(&np_, sizeof(void*)), b = stage_sync(&dp_, sizeof(void*)), c = stage_sync(&gp_, sizeof(void*));
+    lg::launch_grand_product(a.as<const uint32_t*>(), b.as<const uint32_t*>(), c.as<uint32_t*>(), (int)n, 1,
+                             ctx->c.stream);
+    LG_HIP(hipStreamSynchronize(ctx->c.stream));
+    return 0;
+  });
+}
+
+int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t* out) {
+  return guarded([&] {
+    if (!ctx || !in || !out || n < 2 || (n & (n - 1)) || n_cols <= 0) throw std::runtime_error("lg_rs_encode: bad arguments");
+    int logn = 0;
+    while (((size_t)1 << logn) < n) ++logn;
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    LG_HIP(hipSetDevice(ctx->c.device));
+    std::vector<lg::NttJob> jobs(n_cols);
+    for (int c = 0; c < n_cols; ++c) {
+      jobs[c] = lg::NttJob{};
+      jobs[c].in = in + (size_t)c * n;
+      for (int o = 0; o < 4; ++o) jobs[c].out[o] = out + (size_t)c * 4 * n + (size_t)o * n;
+    }
+    lg::NttCfg cfg{};
+    cfg.logn = logn;
+    cfg.inverse_first = 1;
+    cfg.n_out = 4;
+    for (int o = 0; o < 4; ++o) {
+      cfg.pre[o] = ctx->c.tables.lde_pre(logn, o, 2, true);
+      cfg.bitrev_out[o] = 1;
+    }
+    lg::DevBuf jd = stage_sync(jobs.data(), jobs.size() * sizeof(lg::NttJob));
+    lg::launch_ntt(jd.as<lg::NttJob>(), n_cols, cfg, ctx->c.tables.tw_fwd(logn), ctx->c.tables.tw_inv(logn),
+                   ctx->c.stream);
+    LG_HIP(hipStreamSynchronize(ctx->c.stream));
+    return 0;
+  });
+}
+
+}  // extern "C"

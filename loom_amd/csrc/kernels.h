@@ -1,0 +1,130 @@
+// Launchers of the fixed (not run-time generated) prover kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace lg {
+
+constexpr int NTT_MAX_LOG_LDS = 14;   // one workgroup holds a size-2^14 transform in LDS
+constexpr int NTT_MAX_OUT = 8;
+
+// One transform: input column, one destination per output, optional per-job
+// post-scale table applied after the inverse transform.
+struct NttJob {
+  const uint32_t* in;
+  const uint32_t* post;
+  uint32_t* out[NTT_MAX_OUT];
+};
+
+// out_o = DFT(pre_o * x) for o < n_out, where x = post * iDFT(in) when
+// inverse_first is set and x = in otherwise; with n_out == 0 the inverse
+// result itself is written.  Orders: inputs natural; outputs natural or
+// bit-reversed per output.  Scale tables are n-entry Montgomery arrays.
+struct NttCfg {
+  int logn;
+  int inverse_first;
+  int n_out;
+  const uint32_t* pre[NTT_MAX_OUT];
+  int bitrev_out[NTT_MAX_OUT];
+};
+
+void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* tw_fwd,
+                const uint32_t* tw_inv, hipStream_t s);
+
+// Leaf hashing: for instance b (grid y) and pair leaf k, hashes the message
+// [LEAF, hdr1, hdr2, ...] whose element m >= 3 is column msgmap[m] >> 1 at row
+// 2k + (msgmap[m] & 1); columns are colptrs[b * ncols + c].  Digest (8 words)
+// to outs[b] + 8k.
+void launch_leaf_hash(const uint32_t* const* colptrs, int ncols, const int32_t* msgmap, int msglen,
+                      uint32_t hdr1, uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch,
+                      hipStream_t s);
+
+// Merkle level: nodes[w + i] = H(nodes[2(w+i)], nodes[2(w+i)+1]) for i < w,
+// for every tree trees[b] (flat 1-indexed node array, 8 words per node).
+void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s);
+// Levels below `width_hi` down to `width_lo` (inclusive) in one pass over
+// workgroup-local subtrees (width_hi / width_lo <= 256).
+void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s);
+// roots[8b .. 8b+8) = trees[b][8 .. 16)  (node 1 of every tree)
+void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s);
+// Injection fold: nodes[w + j] = H(nodes[w + j], inj[b][j]).
+void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s);
+
+// Column fill: out[b][j] = (j == pos) ? one : 0.
+void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s);
+// Permutation support ID_k[j] = gk * w^j  (wpow: w^j table).
+void launch_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n, int batch, hipStream_t s);
+// sigma_k[j] = g^(t / n) * w^(t % n), t = S[b][k*n + j]   (gpow: g^c for c < chunks).
+void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_t* wpow, const uint32_t* gpow,
+                  int k, int n, int batch, hipStream_t s);
+
+// Grand product: gp[0] = 1, gp[i] = gp[i-1] * num[i-1] / den[i-1]; num, den,
+// gp are E6 columns as 6 limb arrays (limb stride n) per instance.
+void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* gp, int n,
+                          int batch, hipStream_t s);
+
+// Quotient coefficients from the per-coset inverse transforms:
+//   coef[b][t][l][a] = ginv[a + t n] * sum_i wr[(t i) mod rho] * Y[b][l][i n + a]
+// (wr[k] = w_rho^-k, ginv[m] = g^-m).
+void launch_quot_combine(const uint32_t* Y, uint32_t* coef, const uint32_t* ginv, const uint32_t* wr, int rho,
+                         int n, int batch, hipStream_t s);
+
+// Lagrange weights at zeta: W[i] = common * w^i / (zeta - w^i), E6 as 6 limb
+// arrays (stride n).  zeta, common: 6 words per instance.
+void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const uint32_t* wpow,
+                             uint32_t* const* W, int n, int batch, hipStream_t s);
+
+// Claimed value jobs: value = sum_i W[(i - shift) mod n] * P[i].
+struct ClaimJob {
+  const uint32_t* W;     // 6 limb arrays, stride n
+  const uint32_t* poly;  // 1 (base) or 6 (ext) limb arrays, stride n
+  uint32_t* out;         // 6 words
+  int32_t shift;
+  int32_t ext;
+};
+void launch_claims(const ClaimJob* jobs, int n_jobs, int n, hipStream_t s);
+
+// DEEP quotient on the trace domain for one class: per row x (natural order)
+//   DQ[x] = sum_p (c_p - sum_{bundles i at p} a_i * P_i(x)) / (z_p - w^x)
+struct DeepBundle {
+  int32_t ext;          // rail of P_i
+  int32_t col;          // column index into the per-instance column table
+  int32_t first_pt;     // index into pt_list
+  int32_t n_pts;
+};
+struct DeepDesc {
+  int n, n_bundles, n_points;
+  const DeepBundle* bundles;
+  const int32_t* pt_list;     // point indexes used by the bundles
+  const uint32_t* const* cols;  // [b * n_cols + col]
+  int n_cols;
+  const uint32_t* scale;      // [b][bundle][6]  alpha^i
+  const uint32_t* consts;     // [b][point][6]   c_p
+  const uint32_t* points;     // [b][point][6]   z_p
+  const uint32_t* wpow;       // w^x
+  uint32_t* const* out;       // [b] 6 limb arrays, stride n
+};
+void launch_deep(const DeepDesc& d, int batch, hipStream_t s);
+
+// FRI fold (fri.go:1177-1198): next[i] = (p+q)/2 + alpha*(p-q)/2*xinv[i],
+// p = layer[2i], q = layer[2i+1]; E6 layers as 6 limb arrays.
+void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
+                     int half, int batch, hipStream_t s);
+// running[k] += gamma * level[k]
+void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len,
+                        int batch, hipStream_t s);
+
+// Query gathers.  pos: [b * nq + q] full-domain positions.
+// Rows: row = pos >> shift, lo = row & ~1; writes cols[c][lo], cols[c][lo+1]
+// for every column c to out + (b * nq + q) * stride + off (lo values then hi).
+void launch_gather_rows(const uint32_t* pos, int nq, int shift, const uint32_t* const* cols, int ncols,
+                        uint32_t* out, int stride, int off, int batch, hipStream_t s);
+// Merkle path of leaf ((pos >> shift) & ~1) / 2: depth siblings (8 words each).
+void launch_gather_path(const uint32_t* pos, int nq, int shift, const uint32_t* const* trees, int nleaves,
+                        uint32_t* out, int stride, int off, int batch, hipStream_t s);
+// Injection leaves: digest inj[b][((pos >> shift) & ~1) / 2 >> d].
+void launch_gather_digest(const uint32_t* pos, int nq, int shift, int d, const uint32_t* const* inj,
+                          uint32_t* out, int stride, int off, int batch, hipStream_t s);
+
+}  // namespace lg

@@ -1,0 +1,607 @@
+// Fixed HIP kernels of the prove path (gfx950).  Every field value is a
+// KoalaBear element in Montgomery form; E6 columns are stored as 6 limb
+// arrays (structure of arrays) so that each limb streams like a base column.
+#include "kernels.h"
+
+#include <stdexcept>
+
+#include "gpu.h"
+#include "p2.h"
+
+namespace lg {
+
+__device__ __forceinline__ uint32_t brev(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
+
+// ------------------------------------------------------------------ NTT
+// Radix-2 decimation-in-time on LDS data already in bit-reversed order;
+// leaves natural order.  tw[k] = w_n^k for k < n/2.
+__device__ __forceinline__ void lds_dit(uint32_t* sh, int logn, const uint32_t* __restrict__ tw) {
+  const int n = 1 << logn;
+  const int T = blockDim.x;
+  for (int s = 0; s < logn; ++s) {
+    const int half = 1 << s;
+    const int tsh = logn - 1 - s;
+    for (int t = threadIdx.x; t < (n >> 1); t += T) {
+      int j = t & (half - 1);
+      int i0 = ((t >> s) << (s + 1)) | j;
+      int i1 = i0 + half;
+      uint32_t w = tw[j << tsh];
+      uint32_t u = sh[i0];
+      uint32_t v = kb_mul(sh[i1], w);
+      sh[i0] = kb_add(u, v);
+      sh[i1] = kb_sub(u, v);
+    }
+    __syncthreads();
+  }
+}
+
+constexpr int NTT_MAX_PER_THREAD = 16;
+
+__global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ jobs, NttCfg cfg,
+                                                    const uint32_t* __restrict__ tw_fwd,
+                                                    const uint32_t* __restrict__ tw_inv) {
+  extern __shared__ uint32_t sh[];
+  const NttJob job = jobs[blockIdx.x];
+  const int logn = cfg.logn;
+  const int n = 1 << logn;
+  const int T = blockDim.x;
+  const int per = n / T;
+  const int tid = threadIdx.x;
+  uint32_t reg[NTT_MAX_PER_THREAD];
+#pragma unroll
+  for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
+    if (r < per) reg[r] = job.in[tid + r * T];
+  if (cfg.inverse_first) {
+#pragma unroll
+    for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
+      if (r < per) sh[brev(tid + r * T, logn)] = reg[r];
+    __syncthreads();
+    lds_dit(sh, logn, tw_inv);
+#pragma unroll
+    for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
+      if (r < per) {
+        int k = tid + r * T;
+        uint32_t v = sh[k];
+        if (job.post) v = kb_mul(v, job.post[k]);
+        reg[r] = v;
+      }
+    __syncthreads();
+    if (cfg.n_out == 0) {
+#pragma unroll
+      for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
+        if (r < per) job.out[0][tid + r * T] = reg[r];
+      return;
+    }
+  }
+  for (int o = 0; o < cfg.n_out; ++o) {
+    const uint32_t* pre = cfg.pre[o];
+#pragma unroll
+    for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
+      if (r < per) {
+        int k = tid + r * T;
+        uint32_t v = reg[r];
+        if (pre) v = kb_mul(v, pre[k]);
+        sh[brev(k, logn)] = v;
+      }
+    __syncthreads();
+    lds_dit(sh, logn, tw_fwd);
+    uint32_t* out = job.out[o];
+    const bool br = cfg.bitrev_out[o] != 0;
+#pragma unroll
+    for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
+      if (r < per) {
+        int k = tid + r * T;
+        out[k] = sh[br ? brev(k, logn) : k];
+      }
+    __syncthreads();
+  }
+}
+
+void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* tw_fwd,
+                const uint32_t* tw_inv, hipStream_t s) {
+  if (n_jobs == 0) return;
+  if (cfg.logn > NTT_MAX_LOG_LDS || cfg.logn < 1) throw std::runtime_error("launch_ntt: unsupported size");
+  const int n = 1 << cfg.logn;
+  int T = n / 2;
+  if (T > 1024) T = 1024;
+  if (T < 1) T = 1;
+  if (n / T > NTT_MAX_PER_THREAD) throw std::runtime_error("launch_ntt: too many values per thread");
+  hipLaunchKernelGGL(k_ntt_multi, dim3(n_jobs), dim3(T), n * sizeof(uint32_t), s, jobs_dev, cfg, tw_fwd, tw_inv);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ hashing
+__global__ __launch_bounds__(256) void k_leaf_hash(const uint32_t* const* __restrict__ colptrs, int ncols,
+                                                   const int32_t* __restrict__ msgmap, int msglen, uint32_t hdr1,
+                                                   uint32_t hdr2, uint32_t* const* __restrict__ outs, int nleaves) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (k >= nleaves) return;
+  const uint32_t* const* cp = colptrs + (size_t)b * ncols;
+  uint32_t s[24];
+#pragma unroll
+  for (int i = 0; i < 24; ++i) s[i] = 0;
+  const int row0 = 2 * k;
+  for (int off = 0; off < msglen; off += 16) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int m = off + i;
+      if (m < msglen) {
+        uint32_t v;
+        if (m == 0) v = TAG_LEAF_MONT;
+        else if (m == 1) v = hdr1;
+        else if (m == 2) v = hdr2;
+        else {
+          const int code = msgmap[m];
+          v = cp[code >> 1][row0 + (code & 1)];
+        }
+        s[i] = v;
+      }
+    }
+    p2_permute<24>(s);
+  }
+  uint32_t* o = outs[b] + (size_t)8 * k;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) o[i] = s[i];
+}
+
+void launch_leaf_hash(const uint32_t* const* colptrs, int ncols, const int32_t* msgmap, int msglen, uint32_t hdr1,
+                      uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch, hipStream_t s) {
+  if (nleaves == 0 || batch == 0) return;
+  const int T = 256;
+  hipLaunchKernelGGL(k_leaf_hash, dim3((nleaves + T - 1) / T, batch), dim3(T), 0, s, colptrs, ncols, msgmap, msglen,
+                     hdr1, hdr2, outs, nleaves);
+  LG_HIP(hipGetLastError());
+}
+
+__device__ __forceinline__ void load8(const uint32_t* p, uint32_t* d) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint4 a = q[0], c = q[1];
+  d[0] = a.x; d[1] = a.y; d[2] = a.z; d[3] = a.w; d[4] = c.x; d[5] = c.y; d[6] = c.z; d[7] = c.w;
+}
+
+__device__ __forceinline__ void store8(uint32_t* p, const uint32_t* d) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+  q[0] = make_uint4(d[0], d[1], d[2], d[3]);
+  q[1] = make_uint4(d[4], d[5], d[6], d[7]);
+}
+
+__global__ __launch_bounds__(256) void k_merkle_level(uint32_t* const* __restrict__ trees, int width) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= width) return;
+  uint32_t* nodes = trees[blockIdx.y];
+  const size_t node = (size_t)width + i;
+  uint32_t l[8], r[8], d[8];
+  load8(nodes + 16 * node, l);
+  load8(nodes + 16 * node + 8, r);
+  p2_node(l, r, d);
+  store8(nodes + 8 * node, d);
+}
+
+void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s) {
+  const int T = width < 256 ? width : 256;
+  hipLaunchKernelGGL(k_merkle_level, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, width);
+  LG_HIP(hipGetLastError());
+}
+
+// Each workgroup owns G consecutive nodes of level width_hi and reduces them
+// to one node of level width_hi / G, writing every level it computes.
+__global__ __launch_bounds__(256) void k_merkle_fused(uint32_t* const* __restrict__ trees, int width_hi, int levels) {
+  __shared__ uint32_t sh[256 * 8];
+  uint32_t* nodes = trees[blockIdx.y];
+  const int G = blockDim.x;
+  const int tid = threadIdx.x;
+  const int base = blockIdx.x * G;
+  uint32_t l[8], r[8], d[8];
+  {
+    const size_t node = (size_t)width_hi + base + tid;
+    load8(nodes + 16 * node, l);
+    load8(nodes + 16 * node + 8, r);
+    p2_node(l, r, d);
+    store8(nodes + 8 * node, d);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) sh[8 * tid + k] = d[k];
+  }
+  int cnt = G;
+  for (int lev = 1; lev < levels; ++lev) {
+    __syncthreads();
+    cnt >>= 1;
+    const int w = width_hi >> lev;
+    if (tid < cnt) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        l[k] = sh[16 * tid + k];
+        r[k] = sh[16 * tid + 8 + k];
+      }
+      p2_node(l, r, d);
+      store8(nodes + 8 * ((size_t)w + (base >> lev) + tid), d);
+    }
+    __syncthreads();
+    if (tid < cnt) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) sh[8 * tid + k] = d[k];
+    }
+  }
+}
+
+void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s) {
+  int levels = 1;
+  while ((width_hi >> (levels - 1)) > width_lo) ++levels;
+  const int G = width_hi < 256 ? width_hi : 256;
+  if ((width_hi / width_lo) > G) throw std::runtime_error("merkle fused: span exceeds workgroup");
+  hipLaunchKernelGGL(k_merkle_fused, dim3(width_hi / G, batch), dim3(G), 0, s, trees, width_hi, levels);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_copy_roots(const uint32_t* const* trees, uint32_t* roots) {
+  const int b = blockIdx.x;
+  if (threadIdx.x < 8) roots[8 * b + threadIdx.x] = trees[b][8 + threadIdx.x];
+}
+
+void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_roots, dim3(batch), dim3(64), 0, s, trees, roots);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_inject(uint32_t* const* __restrict__ trees,
+                                                const uint32_t* const* __restrict__ inj, int width) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= width) return;
+  uint32_t* nodes = trees[blockIdx.y];
+  const uint32_t* in = inj[blockIdx.y];
+  uint32_t l[8], r[8], d[8];
+  load8(nodes + 8 * ((size_t)width + j), l);
+  load8(in + 8 * (size_t)j, r);
+  p2_node(l, r, d);
+  store8(nodes + 8 * ((size_t)width + j), d);
+}
+
+void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s) {
+  const int T = width < 256 ? width : 256;
+  hipLaunchKernelGGL(k_inject, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, inj, width);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ GenCol
+__global__ void k_lagrange_col(uint32_t* const* outs, int n, int pos) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  outs[blockIdx.y][j] = (j == pos) ? KB_R : 0u;
+}
+
+void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s) {
+  const int T = 256;
+  hipLaunchKernelGGL(k_lagrange_col, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, n, pos);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  outs[blockIdx.y][j] = kb_mul(gk, wpow[j]);
+}
+
+void launch_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n, int batch, hipStream_t s) {
+  const int T = 256;
+  hipLaunchKernelGGL(k_id_support, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, wpow, gk, n);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_t* wpow, const uint32_t* gpow,
+                        int k, int n, int logn) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t t = S[blockIdx.y][(size_t)k * n + j];
+  outs[blockIdx.y][j] = kb_mul(gpow[t >> logn], wpow[t & (n - 1)]);
+}
+
+void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_t* wpow, const uint32_t* gpow, int k,
+                  int n, int batch, hipStream_t s) {
+  int logn = 0;
+  while ((1 << logn) < n) ++logn;
+  const int T = 256;
+  hipLaunchKernelGGL(k_sigma, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, S, wpow, gpow, k, n, logn);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ E6 helpers
+__device__ __forceinline__ E6 ld6(const uint32_t* p, size_t stride, size_t i) {
+  E6 r;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) r.v[l] = p[l * stride + i];
+  return r;
+}
+
+__device__ __forceinline__ void st6(uint32_t* p, size_t stride, size_t i, const E6& x) {
+#pragma unroll
+  for (int l = 0; l < 6; ++l) p[l * stride + i] = x.v[l];
+}
+
+__device__ __forceinline__ E6 ldc6(const uint32_t* p) {
+  E6 r;
+#pragma unroll
+  for (int l = 0; l < 6; ++l) r.v[l] = p[l];
+  return r;
+}
+
+// ------------------------------------------------------------------ grand product
+// One workgroup per instance.  Thread t owns the contiguous chunk
+// [t*c, (t+1)*c) of the ratio vector; chunk products are scanned in LDS.
+__global__ __launch_bounds__(1024) void k_grand_product(const uint32_t* const* num, const uint32_t* const* den,
+                                                         uint32_t* const* gp, int n) {
+  __shared__ uint32_t sh[1024 * 6];
+  const int b = blockIdx.x;
+  const uint32_t* N_ = num[b];
+  const uint32_t* D_ = den[b];
+  uint32_t* G = gp[b];
+  const int T = blockDim.x;
+  const int tid = threadIdx.x;
+  const int c = n / T;
+  const int lo = tid * c;
+  // chunk product of ratios r_i = num_i / den_i for i in [lo, lo + c)
+  E6 acc = e6_one();
+  for (int i = lo; i < lo + c; ++i) acc = e6_mul(acc, e6_mul(ld6(N_, n, i), e6_inv(ld6(D_, n, i))));
+#pragma unroll
+  for (int l = 0; l < 6; ++l) sh[6 * tid + l] = acc.v[l];
+  __syncthreads();
+  // inclusive Hillis-Steele scan of chunk products
+  for (int off = 1; off < T; off <<= 1) {
+    E6 v = ldc6(sh + 6 * tid);
+    E6 u = e6_one();
+    if (tid >= off) u = ldc6(sh + 6 * (tid - off));
+    __syncthreads();
+    if (tid >= off) {
+      E6 w = e6_mul(u, v);
+#pragma unroll
+      for (int l = 0; l < 6; ++l) sh[6 * tid + l] = w.v[l];
+    }
+    __syncthreads();
+  }
+  // gp[i] = prod_{k < i} r_k : start from the exclusive prefix of this chunk
+  E6 run = tid == 0 ? e6_one() : ldc6(sh + 6 * (tid - 1));
+  for (int i = lo; i < lo + c; ++i) {
+    st6(G, n, i, run);
+    run = e6_mul(run, e6_mul(ld6(N_, n, i), e6_inv(ld6(D_, n, i))));
+  }
+}
+
+void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* gp, int n,
+                          int batch, hipStream_t s) {
+  int T = n < 1024 ? n : 1024;
+  hipLaunchKernelGGL(k_grand_product, dim3(batch), dim3(T), 0, s, num, den, gp, n);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ quotient
+__global__ void k_quot_combine(const uint32_t* Y, uint32_t* coef, const uint32_t* ginv, const uint32_t* wr, int rho,
+                               int n) {
+  const int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  const int l = blockIdx.y;
+  const int b = blockIdx.z;
+  const size_t rn = (size_t)rho * n;
+  const uint32_t* y = Y + ((size_t)b * 6 + l) * rn;
+  uint32_t yi[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+    if (i < rho) yi[i] = y[(size_t)i * n + a];
+  for (int t = 0; t < rho; ++t) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+      if (i < rho) acc = kb_add(acc, kb_mul(wr[(t * i) % rho], yi[i]));
+    coef[(((size_t)b * rho + t) * 6 + l) * n + a] = kb_mul(acc, ginv[(size_t)t * n + a]);
+  }
+}
+
+void launch_quot_combine(const uint32_t* Y, uint32_t* coef, const uint32_t* ginv, const uint32_t* wr, int rho, int n,
+                         int batch, hipStream_t s) {
+  if (rho > 8) throw std::runtime_error("quotient: more than 8 cosets");
+  const int T = n < 256 ? n : 256;
+  hipLaunchKernelGGL(k_quot_combine, dim3((n + T - 1) / T, 6, batch), dim3(T), 0, s, Y, coef, ginv, wr, rho, n);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ claimed values
+__global__ void k_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const uint32_t* wpow,
+                                   uint32_t* const* W, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = blockIdx.y;
+  E6 z = ldc6(zeta + 6 * b);
+  E6 c = ldc6(common + 6 * b);
+  const uint32_t wi = wpow[i];
+  E6 d = z;
+  d.v[0] = kb_sub(d.v[0], wi);
+  E6 r = e6_mul(e6_scale(c, wi), e6_inv(d));
+  st6(W[b], n, i, r);
+}
+
+void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const uint32_t* wpow, uint32_t* const* W,
+                             int n, int batch, hipStream_t s) {
+  const int T = 256;
+  hipLaunchKernelGGL(k_lagrange_weights, dim3((n + T - 1) / T, batch), dim3(T), 0, s, zeta, common, wpow, W, n);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void k_claims(const ClaimJob* jobs, int n) {
+  __shared__ uint32_t sh[256 * 6];
+  const ClaimJob job = jobs[blockIdx.x];
+  E6 acc = e6_zero();
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const int wi = (i - job.shift) & (n - 1);
+    E6 w = ld6(job.W, n, wi);
+    if (job.ext) acc = e6_add(acc, e6_mul(w, ld6(job.poly, n, i)));
+    else acc = e6_add(acc, e6_scale(w, job.poly[i]));
+  }
+#pragma unroll
+  for (int l = 0; l < 6; ++l) sh[6 * threadIdx.x + l] = acc.v[l];
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+#pragma unroll
+      for (int l = 0; l < 6; ++l)
+        sh[6 * threadIdx.x + l] = kb_add(sh[6 * threadIdx.x + l], sh[6 * (threadIdx.x + off) + l]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) job.out[threadIdx.x] = sh[threadIdx.x];
+}
+
+void launch_claims(const ClaimJob* jobs, int n_jobs, int n, hipStream_t s) {
+  if (n_jobs == 0) return;
+  hipLaunchKernelGGL(k_claims, dim3(n_jobs), dim3(256), 0, s, jobs, n);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ DEEP
+constexpr int DEEP_MAX_POINTS = 8;
+
+__global__ __launch_bounds__(256) void k_deep(DeepDesc d) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x;
+  const int b = blockIdx.y;
+  if (x >= d.n) return;
+  E6 num[DEEP_MAX_POINTS];
+#pragma unroll
+  for (int p = 0; p < DEEP_MAX_POINTS; ++p)
+    if (p < d.n_points) num[p] = ldc6(d.consts + ((size_t)b * d.n_points + p) * 6);
+  const uint32_t* const* cols = d.cols + (size_t)b * d.n_cols;
+  for (int i = 0; i < d.n_bundles; ++i) {
+    const DeepBundle bd = d.bundles[i];
+    E6 a = ldc6(d.scale + ((size_t)b * d.n_bundles + i) * 6);
+    E6 t = bd.ext ? e6_mul(a, ld6(cols[bd.col], d.n, x)) : e6_scale(a, cols[bd.col][x]);
+    for (int k = 0; k < bd.n_pts; ++k) {
+      const int p = d.pt_list[bd.first_pt + k];
+#pragma unroll
+      for (int q = 0; q < DEEP_MAX_POINTS; ++q)
+        if (q == p) num[q] = e6_sub(num[q], t);
+    }
+  }
+  const uint32_t wx = d.wpow[x];
+  E6 acc = e6_zero();
+#pragma unroll
+  for (int p = 0; p < DEEP_MAX_POINTS; ++p)
+    if (p < d.n_points) {
+      E6 z = ldc6(d.points + ((size_t)b * d.n_points + p) * 6);
+      z.v[0] = kb_sub(z.v[0], wx);
+      acc = e6_add(acc, e6_mul(num[p], e6_inv(z)));
+    }
+  st6(d.out[b], d.n, x, acc);
+}
+
+void launch_deep(const DeepDesc& d, int batch, hipStream_t s) {
+  if (d.n_points > DEEP_MAX_POINTS) throw std::runtime_error("deep: too many distinct opening points");
+  const int T = 256;
+  hipLaunchKernelGGL(k_deep, dim3((d.n + T - 1) / T, batch), dim3(T), 0, s, d);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ FRI
+__global__ void k_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha,
+                           const uint32_t* xinv2, int half) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= half) return;
+  const int b = blockIdx.y;
+  const uint32_t* L = in[b];
+  const size_t m = 2 * (size_t)half;
+  E6 p = ld6(L, m, 2 * i), q = ld6(L, m, 2 * i + 1);
+  // xinv2[i] = xinv_i / 2:  next = (p + q) / 2 + alpha * (p - q) * xinv_i / 2
+  constexpr uint32_t INV2_MONT = 0x00ffffffu;  // 1/2 in Montgomery form (2^31 mod p)
+  E6 s = e6_scale(e6_add(p, q), INV2_MONT);
+  E6 dlt = e6_mul(e6_scale(e6_sub(p, q), xinv2[i]), ldc6(alpha + 6 * b));
+  st6(out[b], half, i, e6_add(s, dlt));
+}
+
+void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
+                     int half, int batch, hipStream_t s) {
+  const int T = half < 256 ? half : 256;
+  hipLaunchKernelGGL(k_fri_fold, dim3((half + T - 1) / T, batch), dim3(T), 0, s, in, out, alpha, xinv2, half);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= len) return;
+  const int b = blockIdx.y;
+  E6 r = ld6(running[b], len, i);
+  r = e6_add(r, e6_mul(ld6(level[b], len, i), ldc6(gamma + 6 * b)));
+  st6(running[b], len, i, r);
+}
+
+void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len,
+                        int batch, hipStream_t s) {
+  const int T = 256;
+  hipLaunchKernelGGL(k_fri_combine, dim3((len + T - 1) / T, batch), dim3(T), 0, s, running, level, gamma, len);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ gathers
+__global__ void k_gather_rows(const uint32_t* pos, int nq, int shift, const uint32_t* const* cols, int ncols,
+                              uint32_t* out, int stride, int off) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int b = blockIdx.y;
+  const uint32_t row = pos[(size_t)b * nq + q] >> shift;
+  const uint32_t lo = row & ~1u;
+  uint32_t* o = out + ((size_t)b * nq + q) * stride + off;
+  const uint32_t* const* cp = cols + (size_t)b * ncols;
+  for (int c = 0; c < ncols; ++c) {
+    o[c] = cp[c][lo];
+    o[ncols + c] = cp[c][lo + 1];
+  }
+}
+
+void launch_gather_rows(const uint32_t* pos, int nq, int shift, const uint32_t* const* cols, int ncols, uint32_t* out,
+                        int stride, int off, int batch, hipStream_t s) {
+  const int T = 64;
+  hipLaunchKernelGGL(k_gather_rows, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, cols, ncols, out,
+                     stride, off);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_gather_path(const uint32_t* pos, int nq, int shift, const uint32_t* const* trees, int nleaves,
+                              uint32_t* out, int stride, int off) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int b = blockIdx.y;
+  const uint32_t row = pos[(size_t)b * nq + q] >> shift;
+  uint32_t node = (uint32_t)nleaves + ((row & ~1u) >> 1);
+  const uint32_t* nodes = trees[b];
+  uint32_t* o = out + ((size_t)b * nq + q) * stride + off;
+  for (int k = 0; node > 1; ++k, node >>= 1) {
+    const uint32_t* src = nodes + 8 * (size_t)(node ^ 1u);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) o[8 * k + l] = src[l];
+  }
+}
+
+void launch_gather_path(const uint32_t* pos, int nq, int shift, const uint32_t* const* trees, int nleaves,
+                        uint32_t* out, int stride, int off, int batch, hipStream_t s) {
+  const int T = 64;
+  hipLaunchKernelGGL(k_gather_path, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, trees, nleaves, out,
+                     stride, off);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_gather_digest(const uint32_t* pos, int nq, int shift, int d, const uint32_t* const* inj,
+                                uint32_t* out, int stride, int off) {
+  const int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= nq) return;
+  const int b = blockIdx.y;
+  const uint32_t row = pos[(size_t)b * nq + q] >> shift;
+  const uint32_t leaf = ((row & ~1u) >> 1) >> d;
+  const uint32_t* src = inj[b] + 8 * (size_t)leaf;
+  uint32_t* o = out + ((size_t)b * nq + q) * stride + off;
+#pragma unroll
+  for (int l = 0; l < 8; ++l) o[l] = src[l];
+}
+
+void launch_gather_digest(const uint32_t* pos, int nq, int shift, int d, const uint32_t* const* inj, uint32_t* out,
+                          int stride, int off, int batch, hipStream_t s) {
+  const int T = 64;
+  hipLaunchKernelGGL(k_gather_digest, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, d, inj, out,
+                     stride, off);
+  LG_HIP(hipGetLastError());
+}
+
+}  // namespace lg

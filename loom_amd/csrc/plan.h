@@ -1,0 +1,199 @@
+// Parsed prover plan: the program metadata a prove run needs (module sizes,
+// column sources, rounds and steps, commitment trees, opening shifts, folded
+// constraint DAGs).  The word format is documented in loom_amd/plan.py and in
+// include/loomgpu.h.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace lg {
+
+constexpr uint32_t PLAN_MAGIC = 0x31504C4Cu;
+constexpr uint32_t PLAN_NONE = 0xFFFFFFFFu;
+
+enum ColSrc : uint32_t { SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR };
+enum DagOp : uint32_t { OP_COL = 0, OP_CONST, OP_CHAL, OP_ADD, OP_SUB, OP_MUL, OP_POW };
+enum StepKind : uint32_t { STEP_GRAND_PRODUCT = 1 };
+
+struct DagNode {
+  uint32_t op, field, a, b;
+};
+
+struct Dag {
+  std::vector<DagNode> nodes;
+  uint32_t root = 0;
+};
+
+struct ModuleInfo {
+  uint32_t N, degree, n_chunks;
+  int log_n;
+  Dag relation;
+};
+
+struct ColInfo {
+  uint32_t field, module, src, a0, a1;
+  int limbs() const { return field ? 6 : 1; }
+};
+
+struct Perm {
+  uint32_t module;
+  uint32_t len;  // the permutation values are per-instance inputs
+};
+
+struct Step {
+  uint32_t kind, module, out_col;
+  std::vector<Dag> ins;
+};
+
+struct Round {
+  uint32_t challenge_col, tree, fs_hook;
+  std::vector<Step> steps;
+};
+
+struct Group {
+  uint32_t module, N;
+  std::vector<uint32_t> base, ext;
+  std::vector<std::vector<uint32_t>> base_shifts, ext_shifts;
+};
+
+struct TreeInfo {
+  uint32_t kind;  // 0 trace, 1 AIR
+  std::vector<Group> groups;
+};
+
+struct PlanData {
+  uint32_t num_queries = 0;
+  std::vector<ModuleInfo> modules;
+  std::vector<ColInfo> cols;
+  std::vector<uint32_t> inputs;
+  std::vector<Perm> perms;
+  std::vector<Round> rounds;
+  std::vector<TreeInfo> trees;
+};
+
+class WordReader {
+ public:
+  WordReader(const uint32_t* w, size_t n) : w_(w), n_(n) {}
+  uint32_t next() {
+    if (pos_ >= n_) throw std::runtime_error("plan: truncated word stream");
+    return w_[pos_++];
+  }
+  size_t pos() const { return pos_; }
+  size_t size() const { return n_; }
+
+ private:
+  const uint32_t* w_;
+  size_t n_;
+  size_t pos_ = 0;
+};
+
+inline Dag read_dag(WordReader& r, size_t n_cols) {
+  Dag d;
+  uint32_t n = r.next();
+  d.root = r.next();
+  if (n == 0 || d.root >= n) throw std::runtime_error("plan: bad dag root");
+  d.nodes.resize(n);
+  for (uint32_t i = 0; i < n; ++i) {
+    DagNode& x = d.nodes[i];
+    x.op = r.next(); x.field = r.next(); x.a = r.next(); x.b = r.next();
+    switch (x.op) {
+      case OP_COL:
+      case OP_CHAL:
+        if (x.a >= n_cols) throw std::runtime_error("plan: dag leaf column out of range");
+        break;
+      case OP_CONST:
+        break;
+      case OP_ADD: case OP_SUB: case OP_MUL:
+        if (x.a >= i || x.b >= i) throw std::runtime_error("plan: dag not topological");
+        break;
+      case OP_POW:
+        if (x.a >= i) throw std::runtime_error("plan: dag not topological");
+        break;
+      default:
+        throw std::runtime_error("plan: unknown dag op");
+    }
+  }
+  return d;
+}
+
+inline int ilog2(uint64_t n) {
+  int k = 0;
+  while ((1ull << k) < n) ++k;
+  return k;
+}
+
+inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
+  WordReader r(words, n_words);
+  if (r.next() != PLAN_MAGIC) throw std::runtime_error("plan: bad magic");
+  if (r.next() != 1) throw std::runtime_error("plan: unsupported version");
+  PlanData p;
+  p.num_queries = r.next();
+  uint32_t nm = r.next();
+  p.modules.resize(nm);
+  for (auto& m : p.modules) {
+    m.N = r.next(); m.degree = r.next(); m.n_chunks = r.next();
+    if (m.N < 2 || (m.N & (m.N - 1)) || m.N > (1u << 22)) throw std::runtime_error("plan: module size must be a power of two in [2, 2^22]");
+    m.log_n = ilog2(m.N);
+  }
+  uint32_t nc = r.next();
+  p.cols.resize(nc);
+  for (auto& c : p.cols) {
+    c.field = r.next(); c.module = r.next(); c.src = r.next(); c.a0 = r.next(); c.a1 = r.next();
+    if (c.module >= nm) throw std::runtime_error("plan: column module out of range");
+  }
+  uint32_t ni = r.next();
+  p.inputs.resize(ni);
+  for (auto& x : p.inputs) {
+    x = r.next();
+    if (x >= nc) throw std::runtime_error("plan: input column out of range");
+  }
+  uint32_t np = r.next();
+  p.perms.resize(np);
+  for (auto& pm : p.perms) {
+    pm.module = r.next();
+    pm.len = r.next();
+    if (pm.module >= nm || pm.len % p.modules[pm.module].N) throw std::runtime_error("plan: bad permutation shape");
+  }
+  uint32_t nr = r.next();
+  p.rounds.resize(nr);
+  for (auto& rd : p.rounds) {
+    rd.challenge_col = r.next(); rd.tree = r.next(); rd.fs_hook = r.next();
+    uint32_t ns = r.next();
+    rd.steps.resize(ns);
+    for (auto& st : rd.steps) {
+      st.kind = r.next(); st.module = r.next(); st.out_col = r.next();
+      uint32_t nin = r.next();
+      for (uint32_t k = 0; k < nin; ++k) st.ins.push_back(read_dag(r, nc));
+    }
+  }
+  uint32_t nt = r.next();
+  p.trees.resize(nt);
+  for (auto& t : p.trees) {
+    t.kind = r.next();
+    uint32_t ng = r.next();
+    t.groups.resize(ng);
+    for (auto& g : t.groups) {
+      g.module = r.next(); g.N = r.next();
+      uint32_t nb = r.next();
+      for (uint32_t k = 0; k < nb; ++k) g.base.push_back(r.next());
+      uint32_t ne = r.next();
+      for (uint32_t k = 0; k < ne; ++k) g.ext.push_back(r.next());
+      auto rd_shifts = [&](size_t cnt, std::vector<std::vector<uint32_t>>& out) {
+        out.resize(cnt);
+        for (auto& v : out) {
+          uint32_t k = r.next();
+          for (uint32_t j = 0; j < k; ++j) v.push_back(r.next());
+        }
+      };
+      rd_shifts(nb, g.base_shifts);
+      rd_shifts(ne, g.ext_shifts);
+    }
+  }
+  for (auto& m : p.modules) m.relation = read_dag(r, nc);
+  if (r.pos() != r.size()) throw std::runtime_error("plan: trailing words");
+  return p;
+}
+
+}  // namespace lg

@@ -1,0 +1,1280 @@
+// Host orchestration of the prove phase (prover/prover.go:655-719 and the PCS
+// it drives), for B instances of one plan in lockstep: every phase is one
+// launch sequence over all instances, the Fiat-Shamir transcripts (one per
+// instance) run on the host between phases, and only roots, claimed values,
+// the final polynomial and the query openings come back from the device.
+#include "prover.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <set>
+
+#include "kernels.h"
+#include "transcript.h"
+
+namespace lg {
+
+namespace {
+
+constexpr int RATE_LOG = 2;
+constexpr int FRI_LOG_N = 24;  // RATE * 2^MAX_LOG_MODULE_SIZE (internal/constants/const.go:24-32)
+constexpr int FRI_LOG_D = 22;
+constexpr int FRI_ROUNDS = FRI_LOG_D;
+constexpr uint32_t TAG_HASH = 0x48415348u;
+constexpr uint32_t TAG_SIZE = 0x53495a45u;
+constexpr uint32_t TAG_EXTP = 0x45585450u;
+constexpr uint32_t MULT_GEN = 3;  // fft.GeneratorFullMultiplicativeGroup (canonical)
+
+uint32_t M(uint32_t x) { return kb_to_mont(x); }
+
+uint32_t omega_max() {
+  static const uint32_t w = kb_pow(M(MULT_GEN), (KB_P - 1) >> 24);
+  return w;
+}
+
+uint32_t omega(int logn) { return kb_pow(omega_max(), 1ull << (24 - logn)); }
+
+uint32_t brev_host(uint32_t x, int bits) {
+  uint32_t r = 0;
+  for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
+  return r;
+}
+
+std::vector<uint32_t> powers(uint32_t g, size_t n, uint32_t scale = KB_R) {
+  std::vector<uint32_t> v(n);
+  uint32_t a = scale;
+  for (size_t i = 0; i < n; ++i) {
+    v[i] = a;
+    a = kb_mul(a, g);
+  }
+  return v;
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+std::string challenge_name(size_t r) { return "challenge@loom_" + std::to_string(r); }
+
+}  // namespace
+
+// ------------------------------------------------------------------ tables
+const uint32_t* Tables::get(const std::string& key, const std::function<std::vector<uint32_t>()>& make) {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto it = bufs_.find(key);
+  if (it != bufs_.end()) return it->second.as<uint32_t>();
+  std::vector<uint32_t> v = make();
+  DevBuf b = upload(v);
+  const uint32_t* p = b.as<uint32_t>();
+  bufs_.emplace(key, std::move(b));
+  return p;
+}
+
+const uint32_t* Tables::tw_fwd(int logn) {
+  return get("twf" + std::to_string(logn), [&] { return powers(omega(logn), std::max<size_t>(1, (1u << logn) / 2)); });
+}
+
+const uint32_t* Tables::tw_inv(int logn) {
+  return get("twi" + std::to_string(logn),
+             [&] { return powers(kb_inv(omega(logn)), std::max<size_t>(1, (1u << logn) / 2)); });
+}
+
+const uint32_t* Tables::wpow(int logn) {
+  return get("wp" + std::to_string(logn), [&] { return powers(omega(logn), 1u << logn); });
+}
+
+const uint32_t* Tables::lde_pre(int logn, int b, int rate_log, bool from_values) {
+  return get("lde" + std::to_string(logn) + "_" + std::to_string(b) + "_" + std::to_string(from_values), [&] {
+    uint32_t s = kb_pow(omega(logn + rate_log), brev_host(b, rate_log));
+    uint32_t sc = from_values ? kb_inv(M(1u << logn)) : KB_R;
+    return powers(s, 1u << logn, sc);
+  });
+}
+
+const uint32_t* Tables::quot_pre(int logn, int i, int rho_log) {
+  return get("qpre" + std::to_string(logn) + "_" + std::to_string(i) + "_" + std::to_string(rho_log), [&] {
+    uint32_t s = kb_mul(M(MULT_GEN), kb_pow(omega(logn + rho_log), i));
+    return powers(s, 1u << logn, kb_inv(M(1u << logn)));
+  });
+}
+
+const uint32_t* Tables::quot_post(int logn, int i, int rho_log) {
+  return get("qpost" + std::to_string(logn) + "_" + std::to_string(i) + "_" + std::to_string(rho_log), [&] {
+    uint32_t s = kb_inv(kb_pow(omega(logn + rho_log), i));
+    return powers(s, 1u << logn, kb_inv(M(1u << (logn + rho_log))));
+  });
+}
+
+const uint32_t* Tables::ginv_pow(int log_len) {
+  return get("ginv" + std::to_string(log_len), [&] { return powers(kb_inv(M(MULT_GEN)), 1u << log_len); });
+}
+
+const uint32_t* Tables::fri_xinv2(int log_m) {
+  return get("fx" + std::to_string(log_m), [&] {
+    const uint32_t half = 1u << (log_m - 1);
+    const int hb = log_m - 1;
+    std::vector<uint32_t> pw = powers(kb_inv(omega(log_m)), half);
+    std::vector<uint32_t> v(half);
+    const uint32_t inv2 = kb_inv(M(2));
+    for (uint32_t i = 0; i < half; ++i) v[i] = kb_mul(pw[brev_host(i, hb)], inv2);
+    return v;
+  });
+}
+
+const uint32_t* Tables::gpow_small(int count) {
+  return get("gs" + std::to_string(count), [&] { return powers(M(MULT_GEN), count); });
+}
+
+// ------------------------------------------------------------------ pool / staging
+DevBuf Pool::get(size_t bytes) {
+  std::lock_guard<std::mutex> lock(mu_);
+  auto it = free_.find(bytes);
+  if (it != free_.end()) {
+    DevBuf b = std::move(it->second);
+    free_.erase(it);
+    return b;
+  }
+  return DevBuf(bytes);
+}
+
+void Pool::put(DevBuf&& b) {
+  if (!b.bytes()) return;
+  std::lock_guard<std::mutex> lock(mu_);
+  size_t n = b.bytes();
+  free_.emplace(n, std::move(b));
+}
+
+Pool::~Pool() = default;
+
+Staging::Staging(size_t bytes) : cap_(bytes) {
+  LG_HIP(hipHostMalloc(reinterpret_cast<void**>(&host_), bytes, hipHostMallocDefault));
+  dev_.alloc(bytes);
+}
+
+Staging::~Staging() {
+  if (host_) (void)hipHostFree(host_);
+}
+
+void* Staging::put(const void* src, size_t n, hipStream_t s) {
+  size_t off = (off_ + 255) & ~size_t(255);
+  if (off + n > cap_) throw std::runtime_error("staging area exhausted");
+  if (n) {
+    std::memcpy(host_ + off, src, n);
+    LG_HIP(hipMemcpyAsync(dev_.as<uint8_t>() + off, host_ + off, n, hipMemcpyHostToDevice, s));
+  }
+  off_ = off + n;
+  return dev_.as<uint8_t>() + off;
+}
+
+std::unique_ptr<Staging> Context::take_staging() {
+  std::lock_guard<std::mutex> lock(staging_mu);
+  if (!staging_free.empty()) {
+    auto s = std::move(staging_free.back());
+    staging_free.pop_back();
+    s->reset();
+    return s;
+  }
+  return std::make_unique<Staging>(64u << 20);
+}
+
+void Context::give_staging(std::unique_ptr<Staging> s) {
+  std::lock_guard<std::mutex> lock(staging_mu);
+  staging_free.push_back(std::move(s));
+}
+
+// ------------------------------------------------------------------ plan
+Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_plan(words, n)) {
+  for (auto& m : d_.modules) {
+    if (m.log_n > NTT_MAX_LOG_LDS) throw std::runtime_error("module size above 2^14 is not supported by this build");
+    if (m.n_chunks > 8) throw std::runtime_error("constraint degree above 9 is not supported");
+  }
+  for (auto& pm : d_.perms)
+    if (pm.len / d_.modules[pm.module].N > 64) throw std::runtime_error("permutation with more than 64 chunks");
+  LG_HIP(hipSetDevice(ctx->device));
+  rel_.resize(d_.modules.size());
+  for (size_t m = 0; m < d_.modules.size(); ++m)
+    if (d_.modules[m].n_chunks) rel_[m] = dag_kernel_get(d_.modules[m].relation, d_.cols);
+  for (size_t r = 0; r < d_.rounds.size(); ++r)
+    for (size_t s = 0; s < d_.rounds[r].steps.size(); ++s) {
+      const Step& st = d_.rounds[r].steps[s];
+      if (st.kind != STEP_GRAND_PRODUCT || st.ins.size() != 2) throw std::runtime_error("unsupported prover step");
+      for (size_t k = 0; k < st.ins.size(); ++k) steps_[{r, s, k}] = dag_kernel_get(st.ins[k], d_.cols);
+    }
+}
+
+// ------------------------------------------------------------------ job
+namespace {
+
+struct ColBuf {
+  DevBuf buf;
+  bool committed = false;
+  bool lde_ready = false;
+  int limbs = 1;
+  uint32_t n = 0;
+  size_t blk = 0;  // words per instance
+  uint32_t* lag(int b, int l) const { return buf.as<uint32_t>() + b * blk + (size_t)l * n; }
+  uint32_t* lde(int b, int l) const { return buf.as<uint32_t>() + b * blk + (size_t)limbs * n + (size_t)l * 4 * n; }
+};
+
+struct TreeBuf {
+  int nleaves = 0;
+  DevBuf nodes;                       // B x 2 nleaves x 8
+  std::vector<size_t> order;          // group indexes, decreasing N (stable)
+  std::vector<DevBuf> inj;            // per order position >= 1: B x width x 8
+  std::vector<int> inj_width;
+  uint32_t* node_ptr(int b) const { return nodes.as<uint32_t>() + (size_t)b * 2 * nleaves * 8; }
+  uint32_t* inj_ptr(size_t k, int b) const { return inj[k].as<uint32_t>() + (size_t)b * inj_width[k] * 8; }
+};
+
+struct Layer {  // an E6 vector of len elements per instance, 6 limb arrays
+  DevBuf buf;
+  uint32_t len = 0;
+  uint32_t* ptr(int b) const { return buf.as<uint32_t>() + (size_t)b * 6 * len; }
+};
+
+class Job {
+ public:
+  Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb)
+      : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
+        B_((int)in.size()), s_(ctx_.stream) {
+    stg_ = ctx_.take_staging();
+  }
+  ~Job() {
+    for (auto& c : cols_) ctx_.pool.put(std::move(c.buf));
+    for (auto& t : trees_) {
+      ctx_.pool.put(std::move(t.nodes));
+      for (auto& x : t.inj) ctx_.pool.put(std::move(x));
+    }
+    for (auto& x : keep_) ctx_.pool.put(std::move(x));
+    if (stg_) ctx_.give_staging(std::move(stg_));
+  }
+
+  void run();
+
+ private:
+  DevBuf dev(size_t words) { return ctx_.pool.get(words * sizeof(uint32_t)); }
+  template <class T>
+  const T* stage(const std::vector<T>& v) { return stg_->put(v, s_); }
+  void sync() { LG_HIP(hipStreamSynchronize(s_)); }
+  void phase(const char* name, double& t0) {
+    if (cb_) {
+      sync();
+      double t = now_s();
+      cb_(name, t - t0);
+      t0 = t;
+    }
+  }
+
+  void alloc_columns();
+  void upload_inputs();
+  void gencol();
+  void run_steps(size_t r);
+  void lde_columns(const std::vector<uint32_t>& col_ids);
+  void commit_tree(size_t t);
+  std::vector<Digest> roots_of(TreeBuf& tb);
+  void build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>>& leaf_cols_per_group,
+                  const std::vector<std::pair<int, int>>& group_shape, const std::vector<int>& group_rows);
+  void quotient(size_t m);
+  void open();
+
+  Plan& plan_;
+  const PlanData& d_;
+  Context& ctx_;
+  Tables& tb_;
+  const std::vector<InstanceInput>& in_;
+  std::vector<ProveOutput>& out_;
+  PhaseCallback cb_;
+  int B_;
+  hipStream_t s_;
+  std::unique_ptr<Staging> stg_;
+
+  std::vector<ColBuf> cols_;
+  std::vector<DevBuf> perm_bufs_;   // [perm][b] flattened: perm_bufs_[p*B + b]
+  std::vector<TreeBuf> trees_;
+  std::vector<DevBuf> keep_;
+  std::vector<Transcript> ts_;
+  std::vector<std::vector<E6>> chal_;   // [b][col] challenge values (only challenge columns set)
+  std::vector<std::vector<Digest>> roots_;  // [tree][b]
+  std::vector<E6> zeta_;
+};
+
+void Job::alloc_columns() {
+  cols_.resize(d_.cols.size());
+  std::vector<char> committed(d_.cols.size(), 0);
+  for (auto& t : d_.trees)
+    for (auto& g : t.groups) {
+      for (auto c : g.base) committed[c] = 1;
+      for (auto c : g.ext) committed[c] = 1;
+    }
+  for (size_t c = 0; c < d_.cols.size(); ++c) {
+    const ColInfo& ci = d_.cols[c];
+    ColBuf& cb = cols_[c];
+    cb.limbs = ci.limbs();
+    cb.n = d_.modules[ci.module].N;
+    cb.committed = committed[c] != 0;
+    if (ci.src == SRC_CHALLENGE) continue;
+    cb.blk = (size_t)cb.limbs * cb.n * (cb.committed ? 5 : 1);
+    cb.buf = dev(cb.blk * B_);
+  }
+}
+
+void Job::upload_inputs() {
+  for (int b = 0; b < B_; ++b) {
+    if (in_[b].columns.size() != d_.inputs.size()) throw std::runtime_error("wrong number of input columns");
+    if (in_[b].perms.size() != d_.perms.size()) throw std::runtime_error("wrong number of permutations");
+    for (size_t k = 0; k < d_.inputs.size(); ++k) {
+      const ColBuf& cb = cols_[d_.inputs[k]];
+      if (cb.limbs != 1) throw std::runtime_error("extension-field input columns are not supported");
+      LG_HIP(hipMemcpyAsync(cb.lag(b, 0), in_[b].columns[k], cb.n * sizeof(uint32_t), hipMemcpyHostToDevice, s_));
+    }
+  }
+  perm_bufs_.clear();
+  for (size_t p = 0; p < d_.perms.size(); ++p)
+    for (int b = 0; b < B_; ++b) {
+      DevBuf x = dev(d_.perms[p].len);
+      LG_HIP(hipMemcpyAsync(x.as<void>(), in_[b].perms[p], d_.perms[p].len * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, s_));
+      perm_bufs_.push_back(std::move(x));
+    }
+}
+
+void Job::gencol() {
+  for (size_t c = 0; c < d_.cols.size(); ++c) {
+    const ColInfo& ci = d_.cols[c];
+    const ColBuf& cb = cols_[c];
+    if (ci.src != SRC_LAGRANGE && ci.src != SRC_ID && ci.src != SRC_SIGMA) continue;
+    const int logn = d_.modules[ci.module].log_n;
+    std::vector<uint32_t*> outs(B_);
+    for (int b = 0; b < B_; ++b) outs[b] = cb.lag(b, 0);
+    const auto* outs_d = stage(outs);
+    if (ci.src == SRC_LAGRANGE) {
+      if (ci.a0 >= cb.n) throw std::runtime_error("Lagrange column position out of range");
+      launch_lagrange_col(const_cast<uint32_t* const*>(outs_d), cb.n, ci.a0, B_, s_);
+    } else if (ci.src == SRC_ID) {
+      launch_id_support(const_cast<uint32_t* const*>(outs_d), tb_.wpow(logn), kb_pow(M(MULT_GEN), ci.a0), cb.n, B_, s_);
+    } else {
+      const Perm& pm = d_.perms[ci.a1];
+      std::vector<const uint32_t*> S(B_);
+      for (int b = 0; b < B_; ++b) S[b] = perm_bufs_[ci.a1 * B_ + b].as<uint32_t>();
+      launch_sigma(const_cast<uint32_t* const*>(outs_d), stage(S), tb_.wpow(logn), tb_.gpow_small(pm.len / cb.n),
+                   ci.a0, cb.n, B_, s_);
+    }
+  }
+}
+
+void Job::lde_columns(const std::vector<uint32_t>& col_ids) {
+  std::map<int, std::vector<NttJob>> by_log;
+  for (uint32_t c : col_ids) {
+    ColBuf& cb = cols_[c];
+    if (cb.lde_ready) continue;
+    const int logn = d_.modules[d_.cols[c].module].log_n;
+    for (int b = 0; b < B_; ++b)
+      for (int l = 0; l < cb.limbs; ++l) {
+        NttJob j{};
+        j.in = cb.lag(b, l);
+        j.post = nullptr;
+        for (int o = 0; o < 4; ++o) j.out[o] = cb.lde(b, l) + (size_t)o * cb.n;
+        by_log[logn].push_back(j);
+      }
+    cb.lde_ready = true;
+  }
+  for (auto& kv : by_log) {
+    NttCfg cfg{};
+    cfg.logn = kv.first;
+    cfg.inverse_first = 1;
+    cfg.n_out = 4;
+    for (int o = 0; o < 4; ++o) {
+      cfg.pre[o] = tb_.lde_pre(kv.first, o, RATE_LOG, true);
+      cfg.bitrev_out[o] = 1;
+    }
+    launch_ntt(stage(kv.second), (int)kv.second.size(), cfg, tb_.tw_fwd(kv.first), tb_.tw_inv(kv.first), s_);
+  }
+}
+
+// Hash every group, then build the tree with the smaller groups as level
+// injections (fri/commitment.go:259-386, merkle/tree.go:210-296).
+void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>>& gcols,
+                     const std::vector<std::pair<int, int>>& gshape, const std::vector<int>& grows) {
+  const size_t ng = gcols.size() / B_;
+  tb.order.resize(ng);
+  for (size_t k = 0; k < ng; ++k) tb.order[k] = k;
+  std::stable_sort(tb.order.begin(), tb.order.end(), [&](size_t a, size_t b) { return grows[a] > grows[b]; });
+  tb.nleaves = grows[tb.order[0]] / 2;
+  tb.nodes = dev((size_t)B_ * 2 * tb.nleaves * 8);
+  tb.inj.clear();
+  tb.inj_width.clear();
+  for (size_t k = 0; k < ng; ++k) {
+    const size_t g = tb.order[k];
+    const int nb = gshape[g].first, ne = gshape[g].second;
+    const int ncols = nb + 6 * ne;
+    const int width = grows[g] / 2;
+    std::vector<const uint32_t*> cp;
+    cp.reserve((size_t)B_ * ncols);
+    for (int b = 0; b < B_; ++b) cp.insert(cp.end(), gcols[g * B_ + b].begin(), gcols[g * B_ + b].end());
+    std::vector<int32_t> msg(3, 0);
+    for (int i = 0; i < nb; ++i) msg.push_back(2 * i);
+    for (int i = 0; i < nb; ++i) msg.push_back(2 * i + 1);
+    for (int i = 0; i < 6 * ne; ++i) msg.push_back(2 * (nb + i));
+    for (int i = 0; i < 6 * ne; ++i) msg.push_back(2 * (nb + i) + 1);
+    std::vector<uint32_t*> outs(B_);
+    if (k == 0) {
+      for (int b = 0; b < B_; ++b) outs[b] = tb.node_ptr(b) + (size_t)8 * tb.nleaves;
+    } else {
+      tb.inj.push_back(dev((size_t)B_ * width * 8));
+      tb.inj_width.push_back(width);
+      for (int b = 0; b < B_; ++b) outs[b] = tb.inj_ptr(tb.inj.size() - 1, b);
+    }
+    launch_leaf_hash(stage(cp), ncols, stage(msg), (int)msg.size(), M(2 * nb), M(2 * ne),
+                     const_cast<uint32_t* const*>(stage(outs)), width, B_, s_);
+  }
+  std::vector<uint32_t*> tp(B_);
+  for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
+  uint32_t* const* tp_d = const_cast<uint32_t* const*>(stage(tp));
+  size_t next = 0;
+  auto fold = [&](int width) {
+    while (next < tb.inj.size() && tb.inj_width[next] == width) {
+      std::vector<const uint32_t*> ip(B_);
+      for (int b = 0; b < B_; ++b) ip[b] = tb.inj_ptr(next, b);
+      launch_inject(tp_d, stage(ip), width, B_, s_);
+      ++next;
+    }
+  };
+  auto has_inj = [&](int width) {
+    for (size_t k = next; k < tb.inj.size(); ++k)
+      if (tb.inj_width[k] == width) return true;
+    return false;
+  };
+  fold(tb.nleaves);
+  int w = tb.nleaves / 2;
+  while (w >= 1) {
+    const int G = w < 256 ? w : 256;
+    int lo = w;
+    while (lo > 1 && !has_inj(lo) && (w / (lo / 2)) <= G) lo /= 2;
+    launch_merkle_levels_fused(tp_d, w, lo, B_, s_);
+    fold(lo);
+    w = lo / 2;
+  }
+  if (next != tb.inj.size()) throw std::runtime_error("merkle: injection width does not match a level");
+}
+
+std::vector<Digest> Job::roots_of(TreeBuf& tb) {
+  std::vector<const uint32_t*> tp(B_);
+  for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
+  DevBuf r = dev((size_t)B_ * 8);
+  launch_copy_roots(stage(tp), r.as<uint32_t>(), B_, s_);
+  std::vector<Digest> out(B_);
+  LG_HIP(hipMemcpyAsync(out.data(), r.as<void>(), (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
+  sync();
+  keep_.push_back(std::move(r));
+  return out;
+}
+
+void Job::commit_tree(size_t t) {
+  const TreeInfo& ti = d_.trees[t];
+  std::vector<uint32_t> ids;
+  for (auto& g : ti.groups) {
+    ids.insert(ids.end(), g.base.begin(), g.base.end());
+    ids.insert(ids.end(), g.ext.begin(), g.ext.end());
+  }
+  lde_columns(ids);
+  std::vector<std::vector<const uint32_t*>> gcols;
+  std::vector<std::pair<int, int>> shape;
+  std::vector<int> rows;
+  for (auto& g : ti.groups) {
+    for (int b = 0; b < B_; ++b) {
+      std::vector<const uint32_t*> cp;
+      for (auto c : g.base) cp.push_back(cols_[c].lde(b, 0));
+      for (auto c : g.ext)
+        for (int l = 0; l < 6; ++l) cp.push_back(cols_[c].lde(b, l));
+      gcols.push_back(std::move(cp));
+    }
+    shape.push_back({(int)g.base.size(), (int)g.ext.size()});
+    rows.push_back((int)g.N * 4);
+  }
+  build_tree(trees_[t], gcols, shape, rows);
+  roots_[t] = roots_of(trees_[t]);
+}
+
+void Job::run_steps(size_t r) {
+  const Round& rd = d_.rounds[r];
+  for (size_t s = 0; s < rd.steps.size(); ++s) {
+    const Step& st = rd.steps[s];
+    const ModuleInfo& m = d_.modules[st.module];
+    const uint32_t n = m.N;
+    Layer nd[2];
+    for (int k = 0; k < 2; ++k) {
+      const DagKernel& K = *plan_.step_kernel(r, s, k);
+      nd[k].len = n;
+      nd[k].buf = dev((size_t)B_ * 6 * n);
+      std::vector<const uint32_t*> cp;
+      std::vector<E6> ch;
+      for (int b = 0; b < B_; ++b) {
+        for (auto c : K.col_ids) cp.push_back(cols_[c].lag(b, 0));
+        for (auto c : K.chal_ids) ch.push_back(chal_[b][c]);
+      }
+      std::vector<uint32_t*> op(B_);
+      for (int b = 0; b < B_; ++b) op[b] = nd[k].ptr(b);
+      DagLaunch a{};
+      a.cols = stage(cp);
+      a.chal = reinterpret_cast<const uint32_t*>(stg_->put(ch.data(), ch.size() * sizeof(E6), s_));
+      a.rowscale = nullptr;
+      a.out = const_cast<uint32_t* const*>(stage(op));
+      a.col_stride = n;
+      a.out_stride = n;
+      a.logn = m.log_n;
+      a.ncos = 1;
+      a.batch = B_;
+      dag_kernel_launch(K, a, s_);
+    }
+    std::vector<const uint32_t*> np(B_), dp(B_);
+    std::vector<uint32_t*> gp(B_);
+    for (int b = 0; b < B_; ++b) {
+      np[b] = nd[0].ptr(b);
+      dp[b] = nd[1].ptr(b);
+      gp[b] = cols_[st.out_col].lag(b, 0);
+    }
+    launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(gp)), n, B_, s_);
+    keep_.push_back(std::move(nd[0].buf));
+    keep_.push_back(std::move(nd[1].buf));
+  }
+}
+
+// AIR quotient of one module (compute_quotient.go:223-424 and :484-499,
+// prover.go:422-440): leaf columns on rho cosets g*w_{rho n}^i <w_n>, the
+// generated DAG kernel divides by X^n - 1, per-coset inverse transforms and a
+// rho-point combination give the coefficients of Q, each n-chunk is then
+// evaluated on H (Lagrange form) and on the rate-4 LDE domain.
+void Job::quotient(size_t mi) {
+  const ModuleInfo& m = d_.modules[mi];
+  const DagKernel& K = *plan_.relation_kernel(mi);
+  const uint32_t n = m.N;
+  const int logn = m.log_n;
+  const int rho = (int)m.n_chunks;
+  int rho_log = 0;
+  while ((1 << rho_log) < rho) ++rho_log;
+  if ((1 << rho_log) != rho) throw std::runtime_error("quotient: chunk count is not a power of two");
+  const size_t rn = (size_t)rho * n;
+  // 1. coset evaluations of every leaf column
+  std::vector<DevBuf> cos(K.col_ids.size());
+  std::vector<NttJob> jobs;
+  for (size_t k = 0; k < K.col_ids.size(); ++k) {
+    const ColBuf& cb = cols_[K.col_ids[k]];
+    cos[k] = dev((size_t)B_ * cb.limbs * rn);
+    for (int b = 0; b < B_; ++b)
+      for (int l = 0; l < cb.limbs; ++l) {
+        NttJob j{};
+        j.in = cb.lag(b, l);
+        for (int i = 0; i < rho; ++i) j.out[i] = cos[k].as<uint32_t>() + ((size_t)b * cb.limbs + l) * rn + (size_t)i * n;
+        jobs.push_back(j);
+      }
+  }
+  NttCfg cfg{};
+  cfg.logn = logn;
+  cfg.inverse_first = 1;
+  cfg.n_out = rho;
+  for (int i = 0; i < rho; ++i) cfg.pre[i] = tb_.quot_pre(logn, i, rho_log);
+  launch_ntt(stage(jobs), (int)jobs.size(), cfg, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  // 2. numerator / Z_H on the cosets
+  DevBuf Q = dev((size_t)B_ * 6 * rn);
+  {
+    std::vector<const uint32_t*> cp;
+    std::vector<E6> ch;
+    for (int b = 0; b < B_; ++b) {
+      for (size_t k = 0; k < K.col_ids.size(); ++k)
+        cp.push_back(cos[k].as<uint32_t>() + (size_t)b * cols_[K.col_ids[k]].limbs * rn);
+      for (auto c : K.chal_ids) ch.push_back(chal_[b][c]);
+    }
+    // (g w_{rho n}^i)^n - 1 = g^n w_rho^i - 1
+    std::vector<uint32_t> zh(rho);
+    const uint32_t gn = kb_pow(M(MULT_GEN), n);
+    for (int i = 0; i < rho; ++i) zh[i] = kb_inv(kb_sub(kb_mul(gn, kb_pow(omega(rho_log), i)), KB_R));
+    std::vector<uint32_t*> op(B_);
+    for (int b = 0; b < B_; ++b) op[b] = Q.as<uint32_t>() + (size_t)b * 6 * rn;
+    DagLaunch a{};
+    a.cols = stage(cp);
+    a.chal = reinterpret_cast<const uint32_t*>(stg_->put(ch.data(), ch.size() * sizeof(E6), s_));
+    a.rowscale = stage(zh);
+    a.out = const_cast<uint32_t* const*>(stage(op));
+    a.col_stride = rn;
+    a.out_stride = rn;
+    a.logn = logn;
+    a.ncos = rho;
+    a.batch = B_;
+    dag_kernel_launch(K, a, s_);
+  }
+  for (auto& c : cos) keep_.push_back(std::move(c));
+  // 3. per-coset inverse transforms with the w_{rho n}^{-a i} / (rho n) twiddle
+  DevBuf Y = dev((size_t)B_ * 6 * rn);
+  jobs.clear();
+  for (int b = 0; b < B_; ++b)
+    for (int l = 0; l < 6; ++l)
+      for (int i = 0; i < rho; ++i) {
+        NttJob j{};
+        size_t off = ((size_t)b * 6 + l) * rn + (size_t)i * n;
+        j.in = Q.as<uint32_t>() + off;
+        j.post = tb_.quot_post(logn, i, rho_log);
+        j.out[0] = Y.as<uint32_t>() + off;
+        jobs.push_back(j);
+      }
+  NttCfg c2{};
+  c2.logn = logn;
+  c2.inverse_first = 1;
+  c2.n_out = 0;
+  launch_ntt(stage(jobs), (int)jobs.size(), c2, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  // 4. rho-point combination -> chunk coefficients [b][t][l][a]
+  DevBuf coef = dev((size_t)B_ * rho * 6 * n);
+  {
+    std::vector<uint32_t> wr(rho);
+    for (int k = 0; k < rho; ++k) wr[k] = kb_inv(kb_pow(omega(rho_log), k));
+    launch_quot_combine(Y.as<uint32_t>(), coef.as<uint32_t>(), tb_.ginv_pow(logn + rho_log), stage(wr), rho, n, B_, s_);
+  }
+  // 5. chunks: LDE (4 bit-reversed blocks) + Lagrange values on H
+  std::vector<uint32_t> chunk_col(rho, PLAN_NONE);
+  for (size_t c = 0; c < d_.cols.size(); ++c)
+    if (d_.cols[c].src == SRC_AIR && d_.cols[c].module == mi && d_.cols[c].a0 < (uint32_t)rho) chunk_col[d_.cols[c].a0] = c;
+  jobs.clear();
+  for (int t = 0; t < rho; ++t) {
+    if (chunk_col[t] == PLAN_NONE) throw std::runtime_error("quotient: missing chunk column");
+    ColBuf& cb = cols_[chunk_col[t]];
+    if (cb.limbs != 6 || !cb.committed) throw std::runtime_error("quotient: chunk column must be a committed E6 column");
+    for (int b = 0; b < B_; ++b)
+      for (int l = 0; l < 6; ++l) {
+        NttJob j{};
+        j.in = coef.as<uint32_t>() + (((size_t)b * rho + t) * 6 + l) * n;
+        for (int o = 0; o < 4; ++o) j.out[o] = cb.lde(b, l) + (size_t)o * n;
+        j.out[4] = cb.lag(b, l);
+        jobs.push_back(j);
+      }
+    cb.lde_ready = true;
+  }
+  NttCfg c3{};
+  c3.logn = logn;
+  c3.inverse_first = 0;
+  c3.n_out = 5;
+  for (int o = 0; o < 4; ++o) {
+    c3.pre[o] = tb_.lde_pre(logn, o, RATE_LOG, false);
+    c3.bitrev_out[o] = 1;
+  }
+  c3.pre[4] = nullptr;
+  c3.bitrev_out[4] = 0;
+  launch_ntt(stage(jobs), (int)jobs.size(), c3, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  keep_.push_back(std::move(Q));
+  keep_.push_back(std::move(Y));
+  keep_.push_back(std::move(coef));
+}
+
+void Job::run() {
+  double t0 = now_s();
+  LG_HIP(hipSetDevice(ctx_.device));
+  alloc_columns();
+  upload_inputs();
+  trees_.resize(d_.trees.size());
+  roots_.resize(d_.trees.size());
+  chal_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
+  ts_.assign(B_, Transcript());
+  const size_t nr = d_.rounds.size();
+  for (int b = 0; b < B_; ++b) {
+    for (size_t r = 0; r < nr; ++r) ts_[b].new_challenge(challenge_name(r));
+    ts_[b].new_challenge("__zeta");
+    const std::string init = nr ? challenge_name(0) : "__zeta";
+    ts_[b].bind(init, string_to_elements(TAG_HASH, "poseidon2"));
+    std::vector<uint32_t> sizes{M(TAG_SIZE), M((uint32_t)d_.modules.size())};
+    for (auto& m : d_.modules) sizes.push_back(M(m.N));
+    ts_[b].bind(init, sizes);
+  }
+  gencol();
+  // ExecuteSteps (prover.go:360-412)
+  for (size_t r = 0; r < nr; ++r) {
+    const Round& rd = d_.rounds[r];
+    run_steps(r);
+    if (rd.tree != PLAN_NONE) {
+      commit_tree(rd.tree);
+      for (int b = 0; b < B_; ++b) ts_[b].bind(challenge_name(r), roots_[rd.tree][b].v, 8);
+    }
+    if (rd.fs_hook != 0) throw std::runtime_error("exposed-value rounds are not supported by this build");
+    for (int b = 0; b < B_; ++b) {
+      const Digest& dg = ts_[b].compute(challenge_name(r));
+      E6 v;
+      for (int l = 0; l < 6; ++l) v.v[l] = dg.v[l];
+      chal_[b][rd.challenge_col] = v;
+    }
+  }
+  phase("execute-steps", t0);
+  // ComputeAIRQuotients (prover.go:461-561)
+  for (size_t mi = 0; mi < d_.modules.size(); ++mi)
+    if (d_.modules[mi].n_chunks) quotient(mi);
+  for (size_t t = 0; t < d_.trees.size(); ++t) {
+    if (d_.trees[t].kind != 1) continue;
+    commit_tree(t);
+    for (int b = 0; b < B_; ++b) ts_[b].bind("__zeta", roots_[t][b].v, 8);
+  }
+  zeta_.resize(B_);
+  for (int b = 0; b < B_; ++b) {
+    const Digest& dg = ts_[b].compute("__zeta");
+    for (int l = 0; l < 6; ++l) zeta_[b].v[l] = dg.v[l];
+  }
+  phase("compute-air-quotients", t0);
+  open();
+  phase("pcs-open", t0);
+}
+
+// PCS.Open (fri/open.go:98-215) + multi-degree FRI (fri/fri.go:352-736) +
+// query openings (open.go:290-416), then the canonical proof encoding.
+void Job::open() {
+  const size_t nt = d_.trees.size();
+  const size_t nm = d_.modules.size();
+  const int nq = (int)d_.num_queries;
+  // ---- claimed values
+  std::vector<DevBuf> W(nm);
+  for (size_t mi = 0; mi < nm; ++mi) {
+    bool used = false;
+    for (auto& t : d_.trees)
+      for (auto& g : t.groups) used |= g.module == mi;
+    if (!used) continue;
+    const ModuleInfo& m = d_.modules[mi];
+    W[mi] = dev((size_t)B_ * 6 * m.N);
+    std::vector<uint32_t> z(6 * B_), cm(6 * B_);
+    const uint32_t ninv = kb_inv(M(m.N));
+    for (int b = 0; b < B_; ++b) {
+      E6 zn = e6_pow(zeta_[b], m.N);
+      E6 c = e6_scale(e6_sub(zn, e6_one()), ninv);
+      for (int l = 0; l < 6; ++l) {
+        z[6 * b + l] = zeta_[b].v[l];
+        cm[6 * b + l] = c.v[l];
+      }
+    }
+    std::vector<uint32_t*> wp(B_);
+    for (int b = 0; b < B_; ++b) wp[b] = W[mi].as<uint32_t>() + (size_t)b * 6 * m.N;
+    launch_lagrange_weights(stage(z), stage(cm), tb_.wpow(m.log_n), const_cast<uint32_t* const*>(stage(wp)), m.N, B_, s_);
+  }
+  // value index per (tree, group, rail, poly, shift), in canonical order
+  struct CV {
+    size_t t, g;
+    int ext;
+    size_t poly;
+    uint32_t col, shift;
+  };
+  std::vector<CV> cvs;
+  for (size_t t = 0; t < nt; ++t)
+    for (size_t g = 0; g < d_.trees[t].groups.size(); ++g) {
+      const Group& gr = d_.trees[t].groups[g];
+      for (size_t i = 0; i < gr.base.size(); ++i)
+        for (auto s : gr.base_shifts[i]) cvs.push_back({t, g, 0, i, gr.base[i], s});
+      for (size_t i = 0; i < gr.ext.size(); ++i)
+        for (auto s : gr.ext_shifts[i]) cvs.push_back({t, g, 1, i, gr.ext[i], s});
+    }
+  const size_t ncv = cvs.size();
+  DevBuf cvd = dev((size_t)B_ * ncv * 6);
+  // claims are grouped by module size in one launch per size
+  std::map<uint32_t, std::vector<ClaimJob>> cj;
+  for (int b = 0; b < B_; ++b)
+    for (size_t k = 0; k < ncv; ++k) {
+      const CV& v = cvs[k];
+      const uint32_t mi = d_.cols[v.col].module;
+      const uint32_t n = d_.modules[mi].N;
+      ClaimJob j;
+      j.W = W[mi].as<uint32_t>() + (size_t)b * 6 * n;
+      j.poly = cols_[v.col].lag(b, 0);
+      j.out = cvd.as<uint32_t>() + ((size_t)b * ncv + k) * 6;
+      j.shift = (int32_t)v.shift;
+      j.ext = v.ext;
+      cj[n].push_back(j);
+    }
+  for (auto& kv : cj) launch_claims(stage(kv.second), (int)kv.second.size(), kv.first, s_);
+  std::vector<E6> cvals((size_t)B_ * ncv);
+  LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
+  sync();
+  keep_.push_back(std::move(cvd));
+  // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
+  for (int b = 0; b < B_; ++b) {
+    ts_[b].new_challenge("alpha_DEEP");
+    for (size_t mi = 0; mi < nm; ++mi)
+      for (size_t k = 0; k < ncv; ++k)
+        if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi)
+          ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
+  }
+  // FRI challenge registration (fri.go:427-437)
+  std::vector<size_t> classes;
+  for (size_t mi = 0; mi < nm; ++mi)
+    for (size_t k = 0; k < ncv; ++k)
+      if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi) {
+        classes.push_back(mi);
+        break;
+      }
+  std::vector<size_t> lv_order(classes.size());
+  for (size_t l = 0; l < lv_order.size(); ++l) lv_order[l] = l;
+  std::stable_sort(lv_order.begin(), lv_order.end(), [&](size_t a, size_t b) {
+    return d_.modules[classes[a]].N > d_.modules[classes[b]].N;
+  });
+  const size_t L = classes.size();
+  std::vector<E6> alpha(B_);
+  for (int b = 0; b < B_; ++b) {
+    const Digest& dg = ts_[b].compute("alpha_DEEP");
+    for (int l = 0; l < 6; ++l) alpha[b].v[l] = dg.v[l];
+    for (size_t l = 1; l < L; ++l) ts_[b].new_challenge("fri_level_" + std::to_string(l) + "_gamma");
+    for (int j = 0; j < FRI_ROUNDS; ++j) ts_[b].new_challenge("fri_fold_" + std::to_string(j));
+    for (int k = 0; k < nq; ++k) ts_[b].new_challenge("fri_query_" + std::to_string(k));
+  }
+  // ---- DEEP quotient per class, RS-encoded (deep.go:71-266)
+  std::vector<Layer> level(L);
+  for (size_t ci = 0; ci < L; ++ci) {
+    const size_t mi = classes[ci];
+    const ModuleInfo& m = d_.modules[mi];
+    std::vector<DeepBundle> bundles;
+    std::vector<int32_t> ptl;
+    std::vector<uint32_t> pt_shift;  // distinct shifts = distinct points
+    std::vector<std::vector<std::pair<size_t, int>>> bundle_vals;  // (cv index, point)
+    std::vector<uint32_t> bcols;
+    for (size_t t = 0; t < nt; ++t)
+      for (size_t g = 0; g < d_.trees[t].groups.size(); ++g) {
+        const Group& gr = d_.trees[t].groups[g];
+        if (gr.module != mi) continue;
+        for (int rail = 0; rail < 2; ++rail) {
+          const auto& polys = rail ? gr.ext : gr.base;
+          for (size_t i = 0; i < polys.size(); ++i) {
+            DeepBundle bd;
+            bd.ext = rail;
+            bd.col = (int32_t)bcols.size();
+            bcols.push_back(polys[i]);
+            bd.first_pt = (int32_t)ptl.size();
+            std::vector<std::pair<size_t, int>> vals;
+            for (size_t k = 0; k < ncv; ++k) {
+              const CV& v = cvs[k];
+              if (v.t != t || v.g != g || v.ext != rail || v.poly != i) continue;
+              auto it = std::find(pt_shift.begin(), pt_shift.end(), v.shift);
+              int p = (int)(it - pt_shift.begin());
+              if (it == pt_shift.end()) pt_shift.push_back(v.shift);
+              ptl.push_back(p);
+              vals.push_back({k, p});
+            }
+            bd.n_pts = (int32_t)vals.size();
+            bundles.push_back(bd);
+            bundle_vals.push_back(vals);
+          }
+        }
+      }
+    const int np = (int)pt_shift.size();
+    std::vector<E6> scale((size_t)B_ * bundles.size()), consts((size_t)B_ * np), points((size_t)B_ * np);
+    std::vector<const uint32_t*> cp;
+    for (int b = 0; b < B_; ++b) {
+      E6 a = e6_one();
+      for (int p = 0; p < np; ++p) {
+        consts[(size_t)b * np + p] = e6_zero();
+        points[(size_t)b * np + p] = e6_scale(zeta_[b], kb_pow(omega(m.log_n), pt_shift[p]));
+      }
+      for (size_t i = 0; i < bundles.size(); ++i) {
+        scale[(size_t)b * bundles.size() + i] = a;
+        for (auto& kv : bundle_vals[i]) {
+          E6& c = consts[(size_t)b * np + kv.second];
+          c = e6_add(c, e6_mul(a, cvals[(size_t)b * ncv + kv.first]));
+        }
+        a = e6_mul(a, alpha[b]);
+      }
+      for (auto c : bcols) cp.push_back(cols_[c].lag(b, 0));
+    }
+    Layer dq;
+    dq.len = m.N;
+    dq.buf = dev((size_t)B_ * 6 * m.N);
+    std::vector<uint32_t*> op(B_);
+    for (int b = 0; b < B_; ++b) op[b] = dq.ptr(b);
+    DeepDesc dd{};
+    dd.n = m.N;
+    dd.n_bundles = (int)bundles.size();
+    dd.n_points = np;
+    dd.bundles = stage(bundles);
+    dd.pt_list = stage(ptl);
+    dd.cols = stage(cp);
+    dd.n_cols = (int)bcols.size();
+    dd.scale = reinterpret_cast<const uint32_t*>(stg_->put(scale.data(), scale.size() * sizeof(E6), s_));
+    dd.consts = reinterpret_cast<const uint32_t*>(stg_->put(consts.data(), consts.size() * sizeof(E6), s_));
+    dd.points = reinterpret_cast<const uint32_t*>(stg_->put(points.data(), points.size() * sizeof(E6), s_));
+    dd.wpow = tb_.wpow(m.log_n);
+    dd.out = const_cast<uint32_t* const*>(stage(op));
+    launch_deep(dd, B_, s_);
+    level[ci].len = 4 * m.N;
+    level[ci].buf = dev((size_t)B_ * 6 * 4 * m.N);
+    std::vector<NttJob> jobs;
+    for (int b = 0; b < B_; ++b)
+      for (int l = 0; l < 6; ++l) {
+        NttJob j{};
+        j.in = dq.ptr(b) + (size_t)l * m.N;
+        for (int o = 0; o < 4; ++o) j.out[o] = level[ci].ptr(b) + (size_t)l * 4 * m.N + (size_t)o * m.N;
+        jobs.push_back(j);
+      }
+    NttCfg cfg{};
+    cfg.logn = m.log_n;
+    cfg.inverse_first = 1;
+    cfg.n_out = 4;
+    for (int o = 0; o < 4; ++o) {
+      cfg.pre[o] = tb_.lde_pre(m.log_n, o, RATE_LOG, true);
+      cfg.bitrev_out[o] = 1;
+    }
+    launch_ntt(stage(jobs), (int)jobs.size(), cfg, tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
+    keep_.push_back(std::move(dq.buf));
+  }
+  // ---- FRI (fri.go:606-736)
+  auto ext_tree = [&](TreeBuf& tb, const std::vector<const Layer*>& lv) {
+    std::vector<std::vector<const uint32_t*>> gcols;
+    std::vector<std::pair<int, int>> shape;
+    std::vector<int> rows;
+    for (const Layer* x : lv) {
+      for (int b = 0; b < B_; ++b) {
+        std::vector<const uint32_t*> cp;
+        for (int l = 0; l < 6; ++l) cp.push_back(x->ptr(b) + (size_t)l * x->len);
+        gcols.push_back(cp);
+      }
+      shape.push_back({0, 1});
+      rows.push_back((int)x->len);
+    }
+    build_tree(tb, gcols, shape, rows);
+  };
+  std::vector<const Layer*> lvp;
+  for (size_t l : lv_order) lvp.push_back(&level[l]);
+  const int first = FRI_LOG_D - d_.modules[classes[lv_order[0]]].log_n;
+  TreeBuf levels_tree;
+  ext_tree(levels_tree, lvp);
+  std::vector<Digest> levels_root = roots_of(levels_tree);
+  std::vector<std::vector<E6>> gamma(L, std::vector<E6>(B_));
+  for (size_t l = 1; l < L; ++l)
+    for (int b = 0; b < B_; ++b) {
+      const std::string nmg = "fri_level_" + std::to_string(l) + "_gamma";
+      if (l == 1) ts_[b].bind(nmg, levels_root[b].v, 8);
+      const Digest& dg = ts_[b].compute(nmg);
+      for (int k = 0; k < 6; ++k) gamma[l][b].v[k] = dg.v[k];
+    }
+  for (int j = 0; j < first; ++j)
+    for (int b = 0; b < B_; ++b) {
+      const std::string nf = "fri_fold_" + std::to_string(j);
+      uint32_t zero[8] = {0};
+      ts_[b].bind(nf, zero, 8);
+      ts_[b].compute(nf);
+    }
+  std::vector<Layer> layers(FRI_ROUNDS + 1);
+  std::vector<TreeBuf> ftrees(FRI_ROUNDS);
+  std::vector<std::vector<Digest>> fri_roots;
+  Layer running;
+  running.len = lvp[0]->len;
+  running.buf = dev((size_t)B_ * 6 * running.len);
+  LG_HIP(hipMemcpyAsync(running.buf.as<void>(), lvp[0]->buf.as<void>(), (size_t)B_ * 6 * running.len * 4,
+                        hipMemcpyDeviceToDevice, s_));
+  std::vector<const uint32_t*> layer_of(FRI_ROUNDS, nullptr);  // device base of layer j (instance 0)
+  for (int j = first; j < FRI_ROUNDS; ++j) {
+    for (size_t l = 1; l < L; ++l) {
+      const int jl = FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n;
+      if (jl != j) continue;
+      std::vector<uint32_t*> rp(B_);
+      std::vector<const uint32_t*> lp(B_);
+      std::vector<E6> gm(B_);
+      for (int b = 0; b < B_; ++b) {
+        rp[b] = running.ptr(b);
+        lp[b] = lvp[l]->ptr(b);
+        gm[b] = gamma[l][b];
+      }
+      launch_fri_combine(const_cast<uint32_t* const*>(stage(rp)), stage(lp),
+                         reinterpret_cast<const uint32_t*>(stg_->put(gm.data(), gm.size() * sizeof(E6), s_)),
+                         (int)running.len, B_, s_);
+    }
+    std::vector<Digest> root;
+    if (j == first) {
+      root = levels_root;
+    } else {
+      ext_tree(ftrees[j], {&running});
+      root = roots_of(ftrees[j]);
+      fri_roots.push_back(root);
+    }
+    std::vector<E6> al(B_);
+    for (int b = 0; b < B_; ++b) {
+      const std::string nf = "fri_fold_" + std::to_string(j);
+      ts_[b].bind(nf, root[b].v, 8);
+      const Digest& dg = ts_[b].compute(nf);
+      for (int k = 0; k < 6; ++k) al[b].v[k] = dg.v[k];
+    }
+    Layer next;
+    next.len = running.len / 2;
+    next.buf = dev((size_t)B_ * 6 * next.len);
+    std::vector<const uint32_t*> ip(B_);
+    std::vector<uint32_t*> op(B_);
+    for (int b = 0; b < B_; ++b) {
+      ip[b] = running.ptr(b);
+      op[b] = next.ptr(b);
+    }
+    int log_len = 0;
+    while ((1u << log_len) < running.len) ++log_len;
+    launch_fri_fold(stage(ip), const_cast<uint32_t* const*>(stage(op)),
+                    reinterpret_cast<const uint32_t*>(stg_->put(al.data(), al.size() * sizeof(E6), s_)),
+                    tb_.fri_xinv2(log_len), (int)next.len, B_, s_);
+    layers[j] = std::move(running);
+    running = std::move(next);
+  }
+  // final polynomial (running after the last round): 4 E6 values
+  const uint32_t flen = running.len;
+  std::vector<uint32_t> final_w((size_t)B_ * 6 * flen);
+  LG_HIP(hipMemcpyAsync(final_w.data(), running.buf.as<void>(), final_w.size() * 4, hipMemcpyDeviceToHost, s_));
+  sync();
+  std::vector<uint32_t> pos((size_t)B_ * nq);
+  for (int b = 0; b < B_; ++b) {
+    std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
+    for (uint32_t k = 0; k < flen; ++k)
+      for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
+    ts_[b].bind("fri_query_0", fp);
+    for (int k = 0; k < nq; ++k) {
+      const std::string qn = "fri_query_" + std::to_string(k);
+      Digest dg = ts_[b].compute(qn);
+      uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
+      pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
+      if (k < nq - 1) ts_[b].bind("fri_query_" + std::to_string(k + 1), dg.v, 8);
+    }
+  }
+  // ---- gathers: one record per (instance, query)
+  const uint32_t* pos_d = stage(pos);
+  size_t stride = 0;
+  struct TreeRec {
+    size_t rows_off, path_off, injl_off, injrows_off;
+    int ncols_top, depth, shift;
+    std::vector<int> inj_ncols, inj_shift;
+  };
+  std::vector<TreeRec> trec(nt);
+  auto cols_of_group = [&](const Group& g, int b) {
+    std::vector<const uint32_t*> cp;
+    for (auto c : g.base) cp.push_back(cols_[c].lde(b, 0));
+    for (auto c : g.ext)
+      for (int l = 0; l < 6; ++l) cp.push_back(cols_[c].lde(b, l));
+    return cp;
+  };
+  for (size_t t = 0; t < nt; ++t) {
+    const TreeInfo& ti = d_.trees[t];
+    const TreeBuf& tb = trees_[t];
+    TreeRec& r = trec[t];
+    const Group& top = ti.groups[tb.order[0]];
+    r.ncols_top = (int)(top.base.size() + 6 * top.ext.size());
+    r.depth = 0;
+    while ((1 << r.depth) < tb.nleaves) ++r.depth;
+    r.shift = FRI_LOG_N - (r.depth + 1);
+    r.rows_off = stride;
+    stride += 2 * r.ncols_top;
+    r.path_off = stride;
+    stride += 8 * r.depth;
+    r.injl_off = stride;
+    stride += 8 * tb.inj.size();
+    r.injrows_off = stride;
+    for (size_t k = 1; k < tb.order.size(); ++k) {
+      const Group& g = ti.groups[tb.order[k]];
+      int nc = (int)(g.base.size() + 6 * g.ext.size());
+      r.inj_ncols.push_back(nc);
+      int lr = 0;
+      while ((1u << lr) < g.N * 4) ++lr;
+      r.inj_shift.push_back(FRI_LOG_N - lr);
+      stride += 2 * nc;
+    }
+  }
+  struct FriRec {
+    size_t rows_off, path_off;
+    int depth;
+  };
+  std::vector<FriRec> frec(FRI_ROUNDS);
+  for (int j = first; j < FRI_ROUNDS; ++j) {
+    FriRec& r = frec[j];
+    const int len_log = FRI_LOG_N - j;
+    r.depth = len_log - 1;
+    r.rows_off = stride;
+    stride += 12;
+    r.path_off = stride;
+    stride += 8 * r.depth;
+  }
+  std::vector<size_t> lq_off(L);
+  for (size_t l = 1; l < L; ++l) {
+    lq_off[l] = stride;
+    stride += 12;
+  }
+  DevBuf gat = dev((size_t)B_ * nq * stride);
+  for (size_t t = 0; t < nt; ++t) {
+    const TreeInfo& ti = d_.trees[t];
+    const TreeBuf& tb = trees_[t];
+    const TreeRec& r = trec[t];
+    std::vector<const uint32_t*> cp, tp;
+    for (int b = 0; b < B_; ++b) {
+      auto c = cols_of_group(ti.groups[tb.order[0]], b);
+      cp.insert(cp.end(), c.begin(), c.end());
+      tp.push_back(tb.node_ptr(b));
+    }
+    launch_gather_rows(pos_d, nq, r.shift, stage(cp), r.ncols_top, gat.as<uint32_t>(), (int)stride, (int)r.rows_off,
+                       B_, s_);
+    launch_gather_path(pos_d, nq, r.shift, stage(tp), tb.nleaves, gat.as<uint32_t>(), (int)stride, (int)r.path_off,
+                       B_, s_);
+    size_t off = r.injrows_off;
+    for (size_t k = 0; k < tb.inj.size(); ++k) {
+      std::vector<const uint32_t*> ip;
+      for (int b = 0; b < B_; ++b) ip.push_back(tb.inj_ptr(k, b));
+      int dsh = 0;
+      while ((tb.nleaves >> dsh) > tb.inj_width[k]) ++dsh;
+      launch_gather_digest(pos_d, nq, r.shift, dsh, stage(ip), gat.as<uint32_t>(), (int)stride,
+                           (int)(r.injl_off + 8 * k), B_, s_);
+      std::vector<const uint32_t*> gp;
+      for (int b = 0; b < B_; ++b) {
+        auto c = cols_of_group(ti.groups[tb.order[k + 1]], b);
+        gp.insert(gp.end(), c.begin(), c.end());
+      }
+      launch_gather_rows(pos_d, nq, r.inj_shift[k], stage(gp), r.inj_ncols[k], gat.as<uint32_t>(), (int)stride,
+                         (int)off, B_, s_);
+      off += 2 * r.inj_ncols[k];
+    }
+  }
+  for (int j = first; j < FRI_ROUNDS; ++j) {
+    const Layer& lay = (j == first) ? *lvp[0] : layers[j];
+    const TreeBuf& tb = (j == first) ? levels_tree : ftrees[j];
+    std::vector<const uint32_t*> cp, tp;
+    for (int b = 0; b < B_; ++b) {
+      for (int l = 0; l < 6; ++l) cp.push_back(lay.ptr(b) + (size_t)l * lay.len);
+      tp.push_back(tb.node_ptr(b));
+    }
+    launch_gather_rows(pos_d, nq, j, stage(cp), 6, gat.as<uint32_t>(), (int)stride, (int)frec[j].rows_off, B_, s_);
+    launch_gather_path(pos_d, nq, j, stage(tp), tb.nleaves, gat.as<uint32_t>(), (int)stride, (int)frec[j].path_off,
+                       B_, s_);
+  }
+  for (size_t l = 1; l < L; ++l) {
+    const int jl = FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n;
+    std::vector<const uint32_t*> cp;
+    for (int b = 0; b < B_; ++b)
+      for (int k = 0; k < 6; ++k) cp.push_back(lvp[l]->ptr(b) + (size_t)k * lvp[l]->len);
+    launch_gather_rows(pos_d, nq, jl, stage(cp), 6, gat.as<uint32_t>(), (int)stride, (int)lq_off[l], B_, s_);
+  }
+  std::vector<uint32_t> gh((size_t)B_ * nq * stride);
+  LG_HIP(hipMemcpyAsync(gh.data(), gat.as<void>(), gh.size() * 4, hipMemcpyDeviceToHost, s_));
+  sync();
+  // ---- proof assembly (canonical encoding, see include/loomgpu.h)
+  for (int b = 0; b < B_; ++b) {
+    std::vector<uint32_t>& w = out_[b].proof;
+    w.clear();
+    const std::string hid = "poseidon2";
+    w.push_back((uint32_t)hid.size());
+    for (size_t i = 0; i < hid.size(); i += 4) {
+      uint32_t x = 0;
+      for (size_t k = 0; k < 4 && i + k < hid.size(); ++k) x |= (uint32_t)(uint8_t)hid[i + k] << (8 * k);
+      w.push_back(x);
+    }
+    w.push_back(0);  // exposed values
+    w.push_back((uint32_t)nt);
+    for (size_t t = 0; t < nt; ++t) w.insert(w.end(), roots_[t][b].v, roots_[t][b].v + 8);
+    // claimed values
+    w.push_back((uint32_t)nt);
+    size_t k = 0;
+    for (size_t t = 0; t < nt; ++t) {
+      w.push_back((uint32_t)d_.trees[t].groups.size());
+      for (auto& g : d_.trees[t].groups) {
+        for (int rail = 0; rail < 2; ++rail) {
+          const auto& sh = rail ? g.ext_shifts : g.base_shifts;
+          w.push_back((uint32_t)sh.size());
+          for (auto& ss : sh) {
+            w.push_back((uint32_t)ss.size());
+            for (size_t q = 0; q < ss.size(); ++q, ++k) {
+              const E6& v = cvals[(size_t)b * ncv + k];
+              w.insert(w.end(), v.v, v.v + 6);
+            }
+          }
+        }
+      }
+    }
+    const uint32_t* rec0 = gh.data() + (size_t)b * nq * stride;
+    // FRI proof
+    w.insert(w.end(), levels_root[b].v, levels_root[b].v + 8);
+    w.push_back((uint32_t)(L - 1));
+    for (size_t l = 1; l < L; ++l) {
+      const int jl = FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n;
+      w.push_back((uint32_t)nq);
+      for (int q = 0; q < nq; ++q) {
+        const uint32_t* rec = rec0 + (size_t)q * stride;
+        const uint32_t row = pos[(size_t)b * nq + q] >> jl;
+        w.push_back(1);
+        w.push_back(row);
+        for (int x = 0; x < 6; ++x) w.push_back(rec[lq_off[l] + x]);
+        for (int x = 0; x < 6; ++x) w.push_back(rec[lq_off[l] + 6 + x]);
+        w.push_back(0);
+        w.push_back(0);
+        w.push_back(0);
+      }
+    }
+    w.push_back((uint32_t)fri_roots.size());
+    for (auto& r : fri_roots) w.insert(w.end(), r[b].v, r[b].v + 8);
+    w.push_back(1);
+    w.push_back(flen);
+    for (uint32_t e = 0; e < flen; ++e)
+      for (int l = 0; l < 6; ++l) w.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + e]);
+    w.push_back((uint32_t)nq);
+    for (int q = 0; q < nq; ++q) {
+      const uint32_t* rec = rec0 + (size_t)q * stride;
+      const uint32_t s = pos[(size_t)b * nq + q];
+      w.push_back((uint32_t)(FRI_ROUNDS - first));
+      for (int j = first; j < FRI_ROUNDS; ++j) {
+        const uint32_t row = s >> j;
+        w.push_back(1);
+        w.push_back(row);
+        for (int x = 0; x < 6; ++x) w.push_back(rec[frec[j].rows_off + x]);
+        for (int x = 0; x < 6; ++x) w.push_back(rec[frec[j].rows_off + 6 + x]);
+        w.push_back((row & ~1u) >> 1);
+        w.push_back((uint32_t)frec[j].depth);
+        w.insert(w.end(), rec + frec[j].path_off, rec + frec[j].path_off + 8 * frec[j].depth);
+        w.push_back(0);  // layer r0's injection leaves are dropped (fri.go:719); others have none
+      }
+    }
+    w.push_back(0);  // proofs of work (no grinding)
+    w.push_back((uint32_t)nq);
+    for (int q = 0; q < nq; ++q) {
+      const uint32_t* rec = rec0 + (size_t)q * stride;
+      const uint32_t s = pos[(size_t)b * nq + q];
+      w.push_back((uint32_t)nt);
+      for (size_t t = 0; t < nt; ++t) {
+        const TreeInfo& ti = d_.trees[t];
+        const TreeBuf& tb = trees_[t];
+        const TreeRec& r = trec[t];
+        auto rawrow = [&](const Group& g, const uint32_t* vals, int ncols, int hi) {
+          const uint32_t* v = vals + (hi ? ncols : 0);
+          w.push_back((uint32_t)g.base.size());
+          w.insert(w.end(), v, v + g.base.size());
+          w.push_back((uint32_t)g.ext.size());
+          w.insert(w.end(), v + g.base.size(), v + ncols);
+        };
+        const Group& top = ti.groups[tb.order[0]];
+        rawrow(top, rec + r.rows_off, r.ncols_top, 0);
+        rawrow(top, rec + r.rows_off, r.ncols_top, 1);
+        w.push_back(((s >> r.shift) & ~1u) >> 1);
+        w.push_back((uint32_t)r.depth);
+        w.insert(w.end(), rec + r.path_off, rec + r.path_off + 8 * r.depth);
+        w.push_back((uint32_t)tb.inj.size());
+        w.insert(w.end(), rec + r.injl_off, rec + r.injl_off + 8 * tb.inj.size());
+        w.push_back((uint32_t)tb.inj.size());
+        size_t off = r.injrows_off;
+        for (size_t k2 = 0; k2 < tb.inj.size(); ++k2) {
+          const Group& g = ti.groups[tb.order[k2 + 1]];
+          rawrow(g, rec + off, r.inj_ncols[k2], 0);
+          rawrow(g, rec + off, r.inj_ncols[k2], 1);
+          off += 2 * r.inj_ncols[k2];
+        }
+      }
+    }
+    out_[b].transcript = ts_[b].log_words();
+  }
+  for (auto& x : W) keep_.push_back(std::move(x));
+  for (auto& x : level) keep_.push_back(std::move(x.buf));
+  for (auto& x : layers) keep_.push_back(std::move(x.buf));
+  keep_.push_back(std::move(running.buf));
+  keep_.push_back(std::move(gat));
+  keep_.push_back(std::move(levels_tree.nodes));
+  for (auto& x : levels_tree.inj) keep_.push_back(std::move(x));
+  for (auto& t : ftrees) {
+    keep_.push_back(std::move(t.nodes));
+    for (auto& x : t.inj) keep_.push_back(std::move(x));
+  }
+}
+
+}  // namespace
+
+void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutput>& outputs, const PhaseCallback& cb) {
+  if (inputs.empty()) return;
+  outputs.assign(inputs.size(), ProveOutput());
+  Job job(*this, inputs, outputs, cb);
+  job.run();
+  LG_HIP(hipStreamSynchronize(ctx_->stream));
+}
+
+}  // namespace lg

@@ -1,0 +1,120 @@
+// Prove phase on one HIP device for a batch of instances of one plan.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <functional>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "dagjit.h"
+#include "gpu.h"
+#include "plan.h"
+
+namespace lg {
+
+// Per-device constant tables (twiddles, coset scalings), built on first use.
+class Tables {
+ public:
+  const uint32_t* tw_fwd(int logn);
+  const uint32_t* tw_inv(int logn);
+  const uint32_t* wpow(int logn);                 // w_n^j, j < n
+  const uint32_t* lde_pre(int logn, int b, int rate_log, bool from_values);
+  const uint32_t* quot_pre(int logn, int i, int rho_log);   // (g w_{rho n}^i)^k / n
+  const uint32_t* quot_post(int logn, int i, int rho_log);  // w_{rho n}^{-a i} / (rho n)
+  const uint32_t* ginv_pow(int log_len);          // g^{-m}, m < 2^log_len
+  const uint32_t* fri_xinv2(int log_m);           // (w_m^-1)^{bitrev_{m/2}(i)} / 2
+  const uint32_t* gpow_small(int count);          // g^c, c < count
+
+ private:
+  const uint32_t* get(const std::string& key, const std::function<std::vector<uint32_t>()>& make);
+  std::mutex mu_;
+  std::map<std::string, DevBuf> bufs_;
+};
+
+// Reusable device buffers (by exact size) and pinned staging areas, so that
+// repeated proofs of one shape allocate nothing after the first.
+class Pool {
+ public:
+  DevBuf get(size_t bytes);
+  void put(DevBuf&& b);
+  ~Pool();
+
+ private:
+  std::mutex mu_;
+  std::multimap<size_t, DevBuf> free_;
+};
+
+// Pinned host staging + matching device area: small tables (pointer arrays,
+// challenges, positions) go host -> device with asynchronous copies.
+class Staging {
+ public:
+  explicit Staging(size_t bytes);
+  ~Staging();
+  Staging(const Staging&) = delete;
+  Staging& operator=(const Staging&) = delete;
+  void reset() { off_ = 0; }
+  // Copies n bytes to the device area on stream s; returns the device address.
+  void* put(const void* src, size_t n, hipStream_t s);
+  template <class T>
+  const T* put(const std::vector<T>& v, hipStream_t s) {
+    return static_cast<const T*>(put(v.data(), v.size() * sizeof(T), s));
+  }
+
+ private:
+  uint8_t* host_ = nullptr;
+  DevBuf dev_;
+  size_t cap_ = 0, off_ = 0;
+};
+
+struct Context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Tables tables;
+  Pool pool;
+  std::mutex staging_mu;
+  std::vector<std::unique_ptr<Staging>> staging_free;
+  std::unique_ptr<Staging> take_staging();
+  void give_staging(std::unique_ptr<Staging> s);
+};
+
+// Per-instance caller data.
+struct InstanceInput {
+  std::vector<const uint32_t*> columns;  // plan input order, Montgomery, N_module words each (host)
+  std::vector<const uint32_t*> perms;    // plan permutation order, len words each (host)
+};
+
+struct ProveOutput {
+  std::vector<uint32_t> proof;       // canonical word stream
+  std::vector<uint32_t> transcript;  // challenge digests in computation order
+};
+
+// Prover-side timing of each phase (wall clock, host), for the phase callback.
+using PhaseCallback = std::function<void(const char*, double)>;
+
+class Plan {
+ public:
+  Plan(Context* ctx, const uint32_t* words, size_t n);
+  const PlanData& data() const { return d_; }
+  Context* ctx() const { return ctx_; }
+  // Runs the prove phase for every instance; outputs are resized to batch.
+  void prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutput>& outputs,
+             const PhaseCallback& cb = nullptr);
+  size_t num_inputs() const { return d_.inputs.size(); }
+  size_t num_perms() const { return d_.perms.size(); }
+
+  std::shared_ptr<DagKernel> relation_kernel(size_t m) const { return rel_[m]; }
+  std::shared_ptr<DagKernel> step_kernel(size_t r, size_t s, size_t k) const { return steps_.at({r, s, k}); }
+
+ private:
+  Context* ctx_;
+  PlanData d_;
+  std::vector<std::shared_ptr<DagKernel>> rel_;
+  std::map<std::tuple<size_t, size_t, size_t>, std::shared_ptr<DagKernel>> steps_;
+};
+
+}  // namespace lg

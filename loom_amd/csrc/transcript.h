@@ -1,0 +1,92 @@
+// Fiat-Shamir transcript on the host (internal/fiat-shamir/transcript.go:102-315):
+// challenges are registered in order; a challenge's digest is the Poseidon2
+// sponge of StringToElements(FSID, name) || the previous challenge's digest
+// (except for the first challenge) || its bindings.  Grinding is not used
+// (the reference's default).  All elements are Montgomery-form.
+#pragma once
+#include <cstdint>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "p2.h"
+
+namespace lg {
+
+constexpr uint32_t TAG_FSID = 0x46534944u;
+
+// hash.StringToElements (internal/hash/hash.go:36-53), Montgomery form.
+inline std::vector<uint32_t> string_to_elements(uint32_t tag, const std::string& s) {
+  std::vector<uint32_t> r;
+  r.push_back(kb_to_mont(tag));
+  r.push_back(kb_to_mont((uint32_t)s.size()));
+  for (size_t i = 0; i < s.size(); i += 3) {
+    uint32_t limb = 0;
+    for (size_t j = 0; j < 3 && i + j < s.size(); ++j) limb |= (uint32_t)(uint8_t)s[i + j] << (8 * j);
+    r.push_back(kb_to_mont(limb));
+  }
+  return r;
+}
+
+struct Digest {
+  uint32_t v[8];
+};
+
+class Transcript {
+ public:
+  void new_challenge(const std::string& name) {
+    if (pos_.count(name)) throw std::runtime_error("transcript: challenge already exists: " + name);
+    pos_[name] = names_.size();
+    names_.push_back(name);
+    bindings_.emplace_back();
+    values_.emplace_back();
+    computed_.push_back(false);
+  }
+  void bind(const std::string& name, const uint32_t* e, size_t n) {
+    size_t p = at(name);
+    if (computed_[p]) throw std::runtime_error("transcript: challenge already computed: " + name);
+    bindings_[p].insert(bindings_[p].end(), e, e + n);
+  }
+  void bind(const std::string& name, const std::vector<uint32_t>& e) { bind(name, e.data(), e.size()); }
+  const Digest& compute(const std::string& name) {
+    size_t p = at(name);
+    if (computed_[p]) return values_[p];
+    std::vector<uint32_t> msg = string_to_elements(TAG_FSID, name);
+    if (p != 0) {
+      if (!computed_[p - 1]) throw std::runtime_error("transcript: previous challenge not computed: " + name);
+      msg.insert(msg.end(), values_[p - 1].v, values_[p - 1].v + 8);
+    }
+    msg.insert(msg.end(), bindings_[p].begin(), bindings_[p].end());
+    p2_sponge_host(msg.data(), msg.size(), values_[p].v);
+    computed_[p] = true;
+    log_.push_back(p);
+    return values_[p];
+  }
+  // Digests in computation order.
+  std::vector<uint32_t> log_words() const {
+    std::vector<uint32_t> w;
+    for (size_t p : log_) w.insert(w.end(), values_[p].v, values_[p].v + 8);
+    return w;
+  }
+  std::vector<std::string> log_names() const {
+    std::vector<std::string> r;
+    for (size_t p : log_) r.push_back(names_[p]);
+    return r;
+  }
+
+ private:
+  size_t at(const std::string& name) const {
+    auto it = pos_.find(name);
+    if (it == pos_.end()) throw std::runtime_error("transcript: unknown challenge " + name);
+    return it->second;
+  }
+  std::vector<std::string> names_;
+  std::map<std::string, size_t> pos_;
+  std::vector<std::vector<uint32_t>> bindings_;
+  std::vector<Digest> values_;
+  std::vector<bool> computed_;
+  std::vector<size_t> log_;
+};
+
+}  // namespace lg

@@ -1,0 +1,203 @@
+"""Serialise a compiled program + layout into the plan word stream the C-ABI
+(``lg_plan_create``) consumes.
+
+The plan is everything the prover needs that the reference computes on its
+Go side before the hot path runs: module sizes, the column table (where each
+column's values come from), the rounds with their steps and trace trees, the
+AIR tree, the opening shifts of every committed polynomial
+(protocol.BuildLayout / BuildCanonicalSchedule) and every module's folded
+vanishing relation as a topologically ordered DAG (board.Compile).  A Go
+caller would emit the same words from its compiled ``board.Program``.
+
+Word stream (u32, little endian):
+  MAGIC, VERSION, num_queries
+  n_modules, then per module (sorted names): N, degree, n_chunks
+  n_cols, then per column: field, module, src, a0, a1
+  n_inputs, then per input: column id      (caller passes inputs in this order)
+  n_perms, then per permutation: module, len   (the permutation itself is
+            per-instance data, passed beside the trace columns)
+  n_rounds, then per round: challenge col, trace tree (or NONE), fs_hook,
+            n_steps, per step: kind, module, out col, n_ins, dag...
+  n_trees, then per tree: kind, n_groups, per group: module, N,
+            nb, base cols..., ne, ext cols..., shift lists (base polys then
+            ext polys; each: count, shifts...)
+  per module: relation dag
+  dag := n_nodes, root, then 4 words per node: op, field, a, b
+"""
+import numpy as np
+
+from . import expr as E
+from . import board as B
+from . import protocol as PR
+
+MAGIC = 0x31504C4C
+VERSION = 1
+NONE = 0xFFFFFFFF
+
+SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR = range(7)
+OP_COL, OP_CONST, OP_CHAL, OP_ADD, OP_SUB, OP_MUL, OP_POW = range(7)
+
+
+class Plan:
+    def __init__(self, program, num_queries=PR.NUM_QUERIES_DEFAULT):
+        self.program = program
+        self.layout = PR.build_layout(program)
+        self.num_queries = num_queries
+        self.modules = sorted(program.modules)
+        self.mod_idx = {m: i for i, m in enumerate(self.modules)}
+        self.cols = []          # (name, field, module, src, a0, a1)
+        self.col_idx = {}
+        self.perms = []
+        self.inputs = []
+        self._collect()
+
+    def _add(self, name, field, module, src, a0=0, a1=0):
+        if name in self.col_idx:
+            return self.col_idx[name]
+        self.col_idx[name] = len(self.cols)
+        self.cols.append((name, field, module, src, a0, a1))
+        return self.col_idx[name]
+
+    def _collect(self):
+        prog = self.program
+        gen = {}
+        for mn in self.modules:
+            m = prog.modules[mn]
+            mi = self.mod_idx[mn]
+            for g in m.gencol:
+                if isinstance(g, B.LagrangeGen):
+                    gen[B.lagrange_name(mn, g.i)] = (mi, SRC_LAGRANGE, g.i, 0)
+                elif isinstance(g, B.LagrangeRelativeGen):
+                    gen[B.lagrange_name_relative(mn, g.i)] = (mi, SRC_LAGRANGE, m.N - 1 - g.i, 0)
+                elif isinstance(g, B.PermutationGen):
+                    pi = len(self.perms)
+                    self.perms.append((mi, np.asarray(g.S, dtype=np.int64)))
+                    k = len(g.S) // m.N
+                    for i in range(k):
+                        gen[m_id_name(mn, i, m.N)] = (mi, SRC_ID, i, 0)
+                        gen[g.chunk_name(i)] = (mi, SRC_SIGMA, i, pi)
+                else:
+                    raise NotImplementedError(type(g))
+        step_out = {}
+        for r, rd in enumerate(prog.rounds):
+            self._add(B.challenge_name(r), E.EXT, 0, SRC_CHALLENGE, r)
+            for s, st in enumerate(rd.steps):
+                for o in st.outs:
+                    step_out[o] = (self.mod_idx[st.module], r, s)
+        fields = prog.column_fields
+
+        def reg(name):
+            if name in self.col_idx:
+                return self.col_idx[name]
+            f = fields.get(name, E.BASE)
+            if name in gen:
+                mi, src, a0, a1 = gen[name]
+                return self._add(name, E.BASE, mi, src, a0, a1)
+            if name in step_out:
+                mi, r, s = step_out[name]
+                return self._add(name, f, mi, SRC_STEP, r, s)
+            raise KeyError(name)
+
+        names = set()
+        for t in self.layout.trees:
+            if t.kind == "air":
+                continue
+            for g in t.groups:
+                names.update(g.base)
+                names.update(g.ext)
+        for mn in self.modules:
+            for leaf in E.leaves_full(prog.modules[mn].relation):
+                if leaf.type in (E.COMMITTED, E.VERIFIER):
+                    names.add(leaf.name)
+        for rd in prog.rounds:
+            for st in rd.steps:
+                for e in st.ins:
+                    for leaf in E.leaves_full(e):
+                        if leaf.type in (E.COMMITTED, E.VERIFIER):
+                            names.add(leaf.name)
+        for name in sorted(names):
+            if name in gen or name in step_out:
+                reg(name)
+        owner = {}
+        for mn in self.modules:
+            for leaf in E.leaves_full(prog.modules[mn].relation, E.COMMITTED_ONLY):
+                owner.setdefault(leaf.name, mn)
+        for name in sorted(names):
+            if name not in self.col_idx:
+                if name not in owner:
+                    raise KeyError("column %r has no owning module" % name)
+                ci = self._add(name, fields.get(name, E.BASE), self.mod_idx[owner[name]], SRC_INPUT,
+                               len(self.inputs))
+                self.inputs.append(name)
+        for mn in self.modules:
+            m = prog.modules[mn]
+            for i in range(PR.air_chunk_count(m)):
+                self._add(PR.quotient_chunk_name(mn, i), m.dag.nodes[m.dag.root].field,
+                          self.mod_idx[mn], SRC_AIR, i)
+
+    def input_names(self):
+        return list(self.inputs)
+
+    def perm_arrays(self):
+        """The permutations (plan order) as uint32 arrays."""
+        return [np.ascontiguousarray(S, dtype=np.uint32) for _, S in self.perms]
+
+    def _dag_words(self, e, n, out):
+        dag = E.to_dag(e, self.program.column_fields)
+        out += [len(dag.nodes), dag.root]
+        for nd in dag.nodes:
+            if nd.op == E.OP_LEAF:
+                lf = nd.leaf
+                if lf.type == E.CONSTANT:
+                    out += [OP_CONST, nd.field, lf.value % E.P, 0]
+                elif lf.type == E.CHALLENGE:
+                    out += [OP_CHAL, nd.field, self.col_idx[lf.name], 0]
+                else:
+                    s = ((lf.shift % n) + n) % n
+                    out += [OP_COL, nd.field, self.col_idx[lf.name], s]
+            elif nd.op == E.OP_POW:
+                out += [OP_POW, nd.field, nd.children[0], nd.exp]
+            else:
+                op = {E.OP_ADD: OP_ADD, E.OP_SUB: OP_SUB, E.OP_MUL: OP_MUL}[nd.op]
+                out += [op, nd.field, nd.children[0], nd.children[1]]
+
+    def words(self):
+        prog, lay = self.program, self.layout
+        w = [MAGIC, VERSION, self.num_queries]
+        w.append(len(self.modules))
+        for mn in self.modules:
+            m = prog.modules[mn]
+            w += [m.N, m.degree, PR.air_chunk_count(m)]
+        w.append(len(self.cols))
+        for name, f, mi, src, a0, a1 in self.cols:
+            w += [f, mi, src, a0, a1]
+        w.append(len(self.inputs))
+        w += [self.col_idx[n] for n in self.inputs]
+        w.append(len(self.perms))
+        for mi, S in self.perms:
+            w += [mi, len(S)]
+        w.append(len(prog.rounds))
+        for r, rd in enumerate(prog.rounds):
+            w += [self.col_idx[B.challenge_name(r)], lay.trace_tree_of_round.get(r, NONE), rd.fshook,
+                  len(rd.steps)]
+            for st in rd.steps:
+                n = prog.modules[st.module].N
+                w += [st.kind, self.mod_idx[st.module], self.col_idx[st.outs[0]], len(st.ins)]
+                for e in st.ins:
+                    self._dag_words(e, n, w)
+        w.append(len(lay.trees))
+        for t in lay.trees:
+            w += [0 if t.kind == "trace" else 1, len(t.groups)]
+            for g in t.groups:
+                w += [self.mod_idx[g.module], g.N, len(g.base)] + [self.col_idx[c] for c in g.base]
+                w += [len(g.ext)] + [self.col_idx[c] for c in g.ext]
+                for ss in g.base_shifts + g.ext_shifts:
+                    w += [len(ss)] + list(ss)
+        for mn in self.modules:
+            m = prog.modules[mn]
+            self._dag_words(m.relation, m.N, w)
+        return np.array(w, dtype=np.uint32)
+
+
+def m_id_name(module, i, n):
+    return "%s.ID_%d_%d" % (module, i, n)

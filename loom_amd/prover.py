@@ -1,0 +1,139 @@
+"""Host-side ``Prove`` (mirror of prover.Prove, prover/prover.go:655-719) over
+the HIP library.
+
+    ctx = Context(0)
+    proof_words = prove(ctx, program, trace, num_queries=200)
+    many = prove_batch(ctx, [(program_i, trace_i), ...], num_queries=200)
+
+``trace`` maps column names to canonical values in [0, p) (numpy arrays);
+they are converted to Montgomery u32, the reference's in-memory layout.  The
+result is the canonical proof word stream (include/loomgpu.h); ``proof.decode``
+turns it back into a structure.  Errors from the library raise
+``LoomGpuError`` -- there is no CPU fallback.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .plan import Plan
+from . import protocol as PR
+
+P = 2**31 - 2**24 + 1
+R = (1 << 32) % P
+
+
+def to_mont(a):
+    a = np.asarray(a, dtype=np.uint64) % np.uint64(P)
+    return np.ascontiguousarray((a * np.uint64(R) % np.uint64(P)).astype(np.uint32))
+
+
+class Context:
+    def __init__(self, device=0):
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        _lib.check(L.lg_ctx_create(device, ctypes.byref(h)), "lg_ctx_create")
+        self.h = h
+        self.plans = {}
+
+    def close(self):
+        if self.h:
+            for p in self.plans.values():
+                _lib.lib().lg_plan_destroy(p)
+            self.plans.clear()
+            _lib.lib().lg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def plan_handle(self, words):
+        key = words.tobytes()
+        if key not in self.plans:
+            h = ctypes.c_void_p()
+            w = np.ascontiguousarray(words, dtype=np.uint32)
+            _lib.check(_lib.lib().lg_plan_create(self.h, w.ctypes.data_as(_lib.u32p), w.size, ctypes.byref(h)),
+                       "lg_plan_create")
+            self.plans[key] = h
+        return self.plans[key]
+
+
+def check_plan(words):
+    """Parse + hiprtc-compile a plan without a device; returns code bytes."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    n = ctypes.c_size_t()
+    _lib.check(_lib.lib().lg_plan_check(w.ctypes.data_as(_lib.u32p), w.size, ctypes.byref(n)), "lg_plan_check")
+    return n.value
+
+
+class Prepared:
+    """An instance ready for lg_prove: the plan words and the Montgomery
+    input arrays (kept alive for the duration of the call)."""
+
+    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT):
+        self.plan = Plan(program, num_queries)
+        self.words = self.plan.words()
+        self.columns = [to_mont(trace[name]) for name in self.plan.input_names()]
+        self.perms = self.plan.perm_arrays()
+
+
+def prove_prepared(ctx, items, want_transcript=False, phases=None):
+    """Proves prepared instances sharing one plan shape in one batched call."""
+    if not items:
+        return []
+    words = items[0].words
+    for it in items[1:]:
+        if it.words.shape != words.shape or not np.array_equal(it.words, words):
+            raise ValueError("batched instances must share one plan (same program shape)")
+    ph = ctx.plan_handle(words)
+    B = len(items)
+    nc = len(items[0].columns)
+    npm = len(items[0].perms)
+    col_ptrs = (_lib.u32p * max(1, B * nc))()
+    perm_ptrs = (_lib.u32p * max(1, B * npm))()
+    for b, it in enumerate(items):
+        for k, c in enumerate(it.columns):
+            col_ptrs[b * nc + k] = c.ctypes.data_as(_lib.u32p)
+        for k, s in enumerate(it.perms):
+            perm_ptrs[b * npm + k] = s.ctypes.data_as(_lib.u32p)
+    cap = 1 << 16
+    while True:
+        outs = [np.zeros(cap, dtype=np.uint32) for _ in range(B)]
+        touts = [np.zeros(1 << 16, dtype=np.uint32) for _ in range(B)] if want_transcript else None
+        out_ptrs = (_lib.u32p * B)(*[o.ctypes.data_as(_lib.u32p) for o in outs])
+        caps = (ctypes.c_size_t * B)(*([cap] * B))
+        lens = (ctypes.c_size_t * B)()
+        if want_transcript:
+            t_ptrs = (_lib.u32p * B)(*[o.ctypes.data_as(_lib.u32p) for o in touts])
+            tcaps = (ctypes.c_size_t * B)(*([1 << 16] * B))
+            tlens = (ctypes.c_size_t * B)()
+        else:
+            t_ptrs, tcaps, tlens = None, None, None
+        rc = _lib.lib().lg_prove(ph, B, col_ptrs, perm_ptrs, out_ptrs, caps, lens, t_ptrs, tcaps, tlens)
+        if rc == -2 and max(lens) > cap:
+            cap = max(lens)
+            continue
+        _lib.check(rc, "lg_prove")
+        break
+    if phases is not None:
+        arr = (ctypes.c_double * 3)()
+        _lib.check(_lib.lib().lg_plan_last_phases(ph, arr), "lg_plan_last_phases")
+        phases[:] = list(arr)
+    res = [outs[b][:lens[b]].copy() for b in range(B)]
+    if want_transcript:
+        return res, [touts[b][:tlens[b]].copy() for b in range(B)]
+    return res
+
+
+def prove(ctx, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, want_transcript=False):
+    r = prove_prepared(ctx, [Prepared(program, trace, num_queries)], want_transcript)
+    if want_transcript:
+        return r[0][0], r[1][0]
+    return r[0]
+
+
+def prove_batch(ctx, instances, num_queries=PR.NUM_QUERIES_DEFAULT, want_transcript=False):
+    return prove_prepared(ctx, [Prepared(p, t, num_queries) for p, t in instances], want_transcript)

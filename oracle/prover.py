@@ -122,6 +122,7 @@ class Prover:
         self.committed = [None] * len(self.layout.trees)
         self.commitments = []
         self.log = []   # (challenge name, digest)
+        self.deep_on_h = {}  # DEEP class -> values on the trace domain (natural order)
 
     # -- transcript helpers
     def compute(self, name):
@@ -360,6 +361,7 @@ class Prover:
                 zs = np.array(key, dtype=np.uint64)
                 inv = F.e6_inv(F.e6_sub(np.broadcast_to(zs, (n, 6)), xs))
                 acc = F.e6_add(acc, F.e6_mul(numer, inv))
+            self.deep_on_h[cl] = acc
             deep[cl] = ntt.encode_ext(acc, PR.RATE)
         fri = self.fri_prove([(sizes[cl], deep[cl]) for cl in present])
         samplings = []
