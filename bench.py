@@ -1,0 +1,176 @@
+#!/usr/bin/env python3
+"""Bench: 40 independent plonk-like proofs of N = 2^14 (Poseidon2, 200 FRI
+queries), plain per-instance GPU proving vs all 40 proved together in
+batched launches, bit-exact.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one pass of the prove phase over the 40-instance batch (synthetic
+seeded instances, inputs already in host memory in Montgomery form; the
+device upload is part of the timed prove call, as it is for a caller of
+prover.Prove).  `value` is batched proofs/s; the plain path, p50/p99 prove
+latencies, the kernel roofline and the CPU baseline (the numpy oracle on a
+bounded sample, one 2^14 proof) are reported beside it.  One JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs, dtype=np.float64), q))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--instances", type=int, default=40)
+    ap.add_argument("--logn", type=int, default=14)
+    ap.add_argument("--queries", type=int, default=200)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-oracle-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the proofs are independent, so N GPUs would each run this "
+              "same batch (see DESIGN.md on sharding); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(1)
+    from loom_amd import workloads, prover
+
+    B, logn, nq = args.instances, args.logn, args.queries
+    print("[bench] building %d instances of 2^%d" % (B, logn), file=sys.stderr, flush=True)
+    inst = [workloads.plonk_instance(logn, seed) for seed in range(B)]
+    items = [prover.Prepared(p, t, nq) for p, t in inst]
+    ctx = prover.Context(0)
+
+    # warmup (plan compile, buffer pools) for both paths
+    for _ in range(max(1, args.warmup)):
+        prover.prove_prepared(ctx, items[:1])
+        prover.prove_prepared(ctx, items)
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+
+    # plain: one proof per call, sequentially
+    plain_lat = []
+    plain_words = None
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        out = []
+        for it in items:
+            a = time.perf_counter()
+            out.append(prover.prove_prepared(ctx, [it])[0])
+            plain_lat.append((time.perf_counter() - a) * 1e3)
+        plain_words = out
+    torch.cuda.synchronize()
+    plain_s = time.perf_counter() - t0
+    print("[bench] plain done %.3fs" % plain_s, file=sys.stderr, flush=True)
+
+    # batched: all instances in one call
+    batch_lat = []
+    batch_words = None
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        a = time.perf_counter()
+        batch_words = prover.prove_prepared(ctx, items)
+        batch_lat.append((time.perf_counter() - a) * 1e3)
+    torch.cuda.synchronize()
+    batch_s = time.perf_counter() - t0
+    print("[bench] batched done %.3fs" % batch_s, file=sys.stderr, flush=True)
+
+    identical = all(np.array_equal(a, b) for a, b in zip(plain_words, batch_words))
+
+    # kernel timing pass (untimed): HIP events around every launch of one batched step
+    ctx.set_timing(True)
+    prover.prove_prepared(ctx, items)
+    ctx.set_timing(False)
+    stats = ctx.timing_report()
+    tot_ms = sum(v[1] for v in stats.values())
+    dom = max(stats, key=lambda k: stats[k][1])
+    cnt, ms, byts, perms = stats[dom]
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 4), "share": round(v[1] / tot_ms, 4)}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+    # the HBM-bound kernel family with the largest share gives the roofline line
+    hbm_kernels = [k for k in stats if k.startswith("ntt") or k in ("fri_fold", "deep", "dag_quotient")]
+    hk = max(hbm_kernels, key=lambda k: stats[k][1])
+    hc, hms, hbytes, _ = stats[hk]
+    achieved = (hbytes / hc) / ((hms / hc) * 1e-3) / 1e9
+    roofline = {"kernel": hk, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes_per_launch": hbytes / hc, "avg_launch_us": round(hms / hc * 1e3, 3)}
+    traffic_file = os.path.join(ROOT, "profiles", "traffic_%s.json" % hk)
+    if os.path.exists(traffic_file):
+        with open(traffic_file) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+    p2_rate = sum(v[3] for v in stats.values()) / (sum(v[1] for k, v in stats.items() if v[3] > 0) * 1e-3)
+
+    # CPU baseline: the oracle on a bounded sample (one proof of instance 0)
+    cpu = None
+    oracle_ok = None
+    if not args.no_cpu:
+        from oracle import prover as OP
+        print("[bench] cpu baseline (oracle, 1 proof of 2^%d, %d queries)" % (logn, nq), file=sys.stderr,
+              flush=True)
+        a = time.perf_counter()
+        _, enc, _ = OP.prove(inst[0][0], inst[0][1], nq)
+        dt = time.perf_counter() - a
+        cpu = {"value": round(1.0 / dt, 4), "unit": "proofs/s", "cores": 1, "kind": "port",
+               "sample": "1 proof of instance 0 (2^%d plonk-like, %d queries) by the numpy oracle, %.1f s"
+                         % (logn, nq, dt)}
+        if not args.no_oracle_check:
+            oracle_ok = bool(np.array_equal(enc, batch_words[0]))
+
+    n_proofs = B * args.steps
+    value = n_proofs / batch_s
+    plain_rate = n_proofs / plain_s
+    line = {
+        "metric": "proofs/s + p50/p99 prove ms, 40x2^14 plonk, batched vs plain GPU, bit-exact",
+        "value": round(value, 3),
+        "unit": "proofs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(batch_s / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32 (KoalaBear Montgomery, E6)",
+        "data": "synthetic seeded plonk-like instances (seeds 0..%d)" % (B - 1),
+        "config": {"workload": "%d independent plonk-like proofs, N=2^%d, Poseidon2, %d FRI queries, rate 4"
+                               % (B, logn, nq),
+                   "instances": B, "log_n": logn, "queries": nq, "parallelism": "batched launches, 1 GPU"},
+        "plain_proofs_per_s": round(plain_rate, 3),
+        "batched_over_plain": round(value / plain_rate, 3),
+        "plain_p50_ms": round(pct(plain_lat, 50), 3),
+        "plain_p99_ms": round(pct(plain_lat, 99), 3),
+        "batched_p50_ms": round(pct(batch_lat, 50), 3),
+        "batched_p99_ms": round(pct(batch_lat, 99), 3),
+        "bit_exact_batched_vs_plain": identical,
+        "bit_exact_vs_oracle": oracle_ok,
+        "poseidon2_perms_per_s": round(p2_rate, 1),
+        "dominant_kernel": dom,
+        "kernels": kernels,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

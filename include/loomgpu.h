@@ -87,6 +87,13 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
              uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
              const size_t* transcript_caps, size_t* transcript_lens);
 
+/* Per-kernel device timing with HIP events on the context's stream (off by
+ * default).  lg_ctx_timing_report writes one text line per kernel name,
+ * "name launches device_ms algorithmic_bytes poseidon2_permutations", into
+ * buf (NUL-terminated, *len = bytes needed) and clears the totals. */
+int lg_ctx_set_timing(lg_ctx* ctx, int on);
+int lg_ctx_timing_report(lg_ctx* ctx, char* buf, size_t cap, size_t* len);
+
 /* Phase timings of the last lg_prove on this plan (seconds): execute-steps,
  * compute-air-quotients, pcs-open (the phases prover.WithPhaseCallback
  * reports, prover.go:682-716). */

@@ -22,6 +22,8 @@ EXPORTS = {
     "lg_plan_check": (ctypes.c_int, [u32p, ctypes.c_size_t, sizep]),
     "lg_plan_num_inputs": (ctypes.c_int, [ctypes.c_void_p, sizep, sizep]),
     "lg_prove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, u32pp, sizep, sizep, u32pp, sizep, sizep]),
+    "lg_ctx_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "lg_ctx_timing_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, sizep]),
     "lg_plan_last_phases": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "lg_hash_leaf_pairs": (ctypes.c_int, [ctypes.c_void_p, u32pp, ctypes.c_int, u32pp, ctypes.c_int, ctypes.c_size_t, u32p]),
     "lg_merkle_build": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, u32pp, sizep, ctypes.c_int]),

@@ -3,6 +3,7 @@
 // return code and a thread-local message.
 #include "../../include/loomgpu.h"
 
+#include <algorithm>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -167,6 +168,37 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
     }
     if (rc) g_err = "lg_prove: output buffer too small";
     return rc;
+  });
+}
+
+int lg_ctx_set_timing(lg_ctx* ctx, int on) {
+  return guarded([&] {
+    if (!ctx) throw std::runtime_error("lg_ctx_set_timing: null ctx");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->c.timer.on = on != 0;
+    return 0;
+  });
+}
+
+int lg_ctx_timing_report(lg_ctx* ctx, char* buf, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!ctx) throw std::runtime_error("lg_ctx_timing_report: null ctx");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    std::string out;
+    char line[256];
+    for (auto& kv : ctx->c.timer.stats) {
+      snprintf(line, sizeof line, "%s %llu %.6f %.0f %.0f\n", kv.first.c_str(), (unsigned long long)kv.second.count,
+               kv.second.ms, kv.second.bytes, kv.second.perms);
+      out += line;
+    }
+    if (len) *len = out.size() + 1;
+    if (buf && cap) {
+      size_t k = std::min(cap - 1, out.size());
+      std::memcpy(buf, out.data(), k);
+      buf[k] = 0;
+    }
+    ctx->c.timer.stats.clear();
+    return 0;
   });
 }
 

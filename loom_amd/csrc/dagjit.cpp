@@ -204,7 +204,7 @@ size_t dag_kernel_compile_only(const std::string& source) { return hiprtc_compil
 
 std::shared_ptr<DagKernel> dag_kernel_get(const Dag& dag, const std::vector<ColInfo>& cols) {
   static std::mutex mu;
-  static std::map<std::pair<int, std::string>, std::shared_ptr<DagKernel>> cache;
+  static std::map<std::pair<int, std::string>, std::shared_ptr<DagModule>> cache;
   auto k = std::make_shared<DagKernel>();
   k->source = dag_kernel_source(dag, cols, k->col_ids, k->chal_ids);
   int dev = 0;
@@ -212,11 +212,16 @@ std::shared_ptr<DagKernel> dag_kernel_get(const Dag& dag, const std::vector<ColI
   std::lock_guard<std::mutex> lock(mu);
   auto key = std::make_pair(dev, k->source);
   auto it = cache.find(key);
-  if (it != cache.end()) return it->second;
+  if (it != cache.end()) {
+    k->code = it->second;
+    return k;
+  }
   std::vector<char> code = hiprtc_compile(k->source);
-  LG_HIP(hipModuleLoadData(&k->module, code.data()));
-  LG_HIP(hipModuleGetFunction(&k->fn, k->module, "lg_dag"));
-  cache[key] = k;
+  auto m = std::make_shared<DagModule>();
+  LG_HIP(hipModuleLoadData(&m->module, code.data()));
+  LG_HIP(hipModuleGetFunction(&m->fn, m->module, "lg_dag"));
+  cache[key] = m;
+  k->code = m;
   return k;
 }
 
@@ -233,7 +238,7 @@ void dag_kernel_launch(const DagKernel& k, const DagLaunch& a, hipStream_t s) {
   const uint64_t total = (uint64_t)ncos << logn;
   const unsigned T = 256;
   const unsigned gx = (unsigned)((total + T - 1) / T);
-  LG_HIP(hipModuleLaunchKernel(k.fn, gx, a.batch, 1, T, 1, 1, 0, s, args, nullptr));
+  LG_HIP(hipModuleLaunchKernel(k.code->fn, gx, a.batch, 1, T, 1, 1, 0, s, args, nullptr));
 }
 
 }  // namespace lg

@@ -20,9 +20,15 @@ namespace lg {
 // shift moves along the natural-order evaluations of one coset.  The result
 // (E6, Montgomery) times rowscale[i] (when rowscale is non-null) goes to
 // out[b] + l*out_stride + t for limb l.
-struct DagKernel {
+struct DagModule {
   hipModule_t module = nullptr;
   hipFunction_t fn = nullptr;
+};
+
+// The compiled code depends only on the DAG's shape; which columns feed its
+// slots is per DAG, so two DAGs of one shape share a module but not slots.
+struct DagKernel {
+  std::shared_ptr<DagModule> code;
   std::vector<uint32_t> col_ids;    // leaf columns, in kernel column-slot order
   std::vector<uint32_t> chal_ids;   // challenge columns, in kernel slot order
   std::string source;

@@ -7,6 +7,8 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <set>
 
@@ -126,6 +128,54 @@ const uint32_t* Tables::gpow_small(int count) {
   return get("gs" + std::to_string(count), [&] { return powers(M(MULT_GEN), count); });
 }
 
+// ------------------------------------------------------------------ kernel timer
+hipEvent_t KernelTimer::ev() {
+  if (!free_.empty()) {
+    hipEvent_t e = free_.back();
+    free_.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  LG_HIP(hipEventCreate(&e));
+  return e;
+}
+
+void KernelTimer::begin(const char* name, hipStream_t s, double bytes, double perms) {
+  if (!on) return;
+  Rec r{name, ev(), ev(), bytes, perms};
+  LG_HIP(hipEventRecord(r.a, s));
+  pending_.push_back(r);
+}
+
+void KernelTimer::end(hipStream_t s) {
+  if (!on || pending_.empty()) return;
+  LG_HIP(hipEventRecord(pending_.back().b, s));
+}
+
+void KernelTimer::resolve() {
+  for (auto& r : pending_) {
+    float ms = 0;
+    LG_HIP(hipEventSynchronize(r.b));
+    LG_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    Stat& st = stats[r.name];
+    st.count += 1;
+    st.ms += ms;
+    st.bytes += r.bytes;
+    st.perms += r.perms;
+    free_.push_back(r.a);
+    free_.push_back(r.b);
+  }
+  pending_.clear();
+}
+
+KernelTimer::~KernelTimer() {
+  for (auto& r : pending_) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  for (auto e : free_) (void)hipEventDestroy(e);
+}
+
 // ------------------------------------------------------------------ pool / staging
 DevBuf Pool::get(size_t bytes) {
   std::lock_guard<std::mutex> lock(mu_);
@@ -205,6 +255,16 @@ Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_
 
 // ------------------------------------------------------------------ job
 namespace {
+
+// Brackets one launch with timer events when the context's timer is on.
+struct TimeIt {
+  Context& c;
+  hipStream_t s;
+  TimeIt(Context& ctx, hipStream_t st, const char* name, double bytes, double perms) : c(ctx), s(st) {
+    c.timer.begin(name, s, bytes, perms);
+  }
+  ~TimeIt() { c.timer.end(s); }
+};
 
 struct ColBuf {
   DevBuf buf;
@@ -388,6 +448,8 @@ void Job::lde_columns(const std::vector<uint32_t>& col_ids) {
       cfg.pre[o] = tb_.lde_pre(kv.first, o, RATE_LOG, true);
       cfg.bitrev_out[o] = 1;
     }
+    const double n = double(1u << kv.first);
+    TimeIt t(ctx_, s_, "ntt_lde", kv.second.size() * 5.0 * n * 4, 0);
     launch_ntt(stage(kv.second), (int)kv.second.size(), cfg, tb_.tw_fwd(kv.first), tb_.tw_inv(kv.first), s_);
   }
 }
@@ -425,8 +487,12 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
       tb.inj_width.push_back(width);
       for (int b = 0; b < B_; ++b) outs[b] = tb.inj_ptr(tb.inj.size() - 1, b);
     }
-    launch_leaf_hash(stage(cp), ncols, stage(msg), (int)msg.size(), M(2 * nb), M(2 * ne),
-                     const_cast<uint32_t* const*>(stage(outs)), width, B_, s_);
+    const uint32_t* const* cp_d = stage(cp);
+    const int32_t* msg_d = stage(msg);
+    uint32_t* const* outs_d = const_cast<uint32_t* const*>(stage(outs));
+    TimeIt tm(ctx_, s_, "leaf_hash", (double)B_ * width * (2.0 * ncols + 8) * 4,
+              (double)B_ * width * ((msg.size() + 15) / 16));
+    launch_leaf_hash(cp_d, ncols, msg_d, (int)msg.size(), M(2 * nb), M(2 * ne), outs_d, width, B_, s_);
   }
   std::vector<uint32_t*> tp(B_);
   for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
@@ -436,7 +502,9 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
     while (next < tb.inj.size() && tb.inj_width[next] == width) {
       std::vector<const uint32_t*> ip(B_);
       for (int b = 0; b < B_; ++b) ip[b] = tb.inj_ptr(next, b);
-      launch_inject(tp_d, stage(ip), width, B_, s_);
+      const uint32_t* const* ip_d = stage(ip);
+      TimeIt tm(ctx_, s_, "merkle_inject", (double)B_ * width * 3 * 32, (double)B_ * width);
+      launch_inject(tp_d, ip_d, width, B_, s_);
       ++next;
     }
   };
@@ -451,7 +519,10 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
     const int G = w < 256 ? w : 256;
     int lo = w;
     while (lo > 1 && !has_inj(lo) && (w / (lo / 2)) <= G) lo /= 2;
-    launch_merkle_levels_fused(tp_d, w, lo, B_, s_);
+    {
+      TimeIt tm(ctx_, s_, "merkle", (double)B_ * (2.0 * w - lo) * 3 * 32, (double)B_ * (2.0 * w - lo));
+      launch_merkle_levels_fused(tp_d, w, lo, B_, s_);
+    }
     fold(lo);
     w = lo / 2;
   }
@@ -468,6 +539,26 @@ std::vector<Digest> Job::roots_of(TreeBuf& tb) {
   sync();
   keep_.push_back(std::move(r));
   return out;
+}
+
+// Diagnostic: with LOOMGPU_DEBUG_DIR set, writes instance 0's Lagrange
+// values of every column of a committed tree (and its LDE) to that directory.
+static void debug_dump(const char* tag, const std::vector<uint32_t>& ids, const std::vector<ColBuf>& cols,
+                       hipStream_t s) {
+  const char* dir = getenv("LOOMGPU_DEBUG_DIR");
+  if (!dir) return;
+  LG_HIP(hipStreamSynchronize(s));
+  for (uint32_t c : ids) {
+    const ColBuf& cb = cols[c];
+    std::vector<uint32_t> h(cb.blk);
+    LG_HIP(hipMemcpy(h.data(), cb.lag(0, 0), cb.blk * 4, hipMemcpyDeviceToHost));
+    std::string fn = std::string(dir) + "/" + tag + "_col" + std::to_string(c) + ".bin";
+    FILE* f = fopen(fn.c_str(), "wb");
+    if (f) {
+      fwrite(h.data(), 4, h.size(), f);
+      fclose(f);
+    }
+  }
 }
 
 void Job::commit_tree(size_t t) {
@@ -494,6 +585,7 @@ void Job::commit_tree(size_t t) {
   }
   build_tree(trees_[t], gcols, shape, rows);
   roots_[t] = roots_of(trees_[t]);
+  debug_dump(("tree" + std::to_string(t)).c_str(), ids, cols_, s_);
 }
 
 void Job::run_steps(size_t r) {
@@ -574,7 +666,11 @@ void Job::quotient(size_t mi) {
   cfg.inverse_first = 1;
   cfg.n_out = rho;
   for (int i = 0; i < rho; ++i) cfg.pre[i] = tb_.quot_pre(logn, i, rho_log);
-  launch_ntt(stage(jobs), (int)jobs.size(), cfg, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  {
+    const NttJob* jd = stage(jobs);
+    TimeIt tm(ctx_, s_, "ntt_quot_coset", jobs.size() * (1.0 + rho) * n * 4, 0);
+    launch_ntt(jd, (int)jobs.size(), cfg, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  }
   // 2. numerator / Z_H on the cosets
   DevBuf Q = dev((size_t)B_ * 6 * rn);
   {
@@ -601,6 +697,9 @@ void Job::quotient(size_t mi) {
     a.logn = logn;
     a.ncos = rho;
     a.batch = B_;
+    double leaf_words = 0;
+    for (auto c : K.col_ids) leaf_words += cols_[c].limbs;
+    TimeIt tm(ctx_, s_, "dag_quotient", (double)B_ * rn * (leaf_words + 6) * 4, 0);
     dag_kernel_launch(K, a, s_);
   }
   for (auto& c : cos) keep_.push_back(std::move(c));
@@ -621,7 +720,11 @@ void Job::quotient(size_t mi) {
   c2.logn = logn;
   c2.inverse_first = 1;
   c2.n_out = 0;
-  launch_ntt(stage(jobs), (int)jobs.size(), c2, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  {
+    const NttJob* jd = stage(jobs);
+    TimeIt tm(ctx_, s_, "ntt_quot_back", jobs.size() * 2.0 * n * 4, 0);
+    launch_ntt(jd, (int)jobs.size(), c2, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  }
   // 4. rho-point combination -> chunk coefficients [b][t][l][a]
   DevBuf coef = dev((size_t)B_ * rho * 6 * n);
   {
@@ -658,7 +761,11 @@ void Job::quotient(size_t mi) {
   }
   c3.pre[4] = nullptr;
   c3.bitrev_out[4] = 0;
-  launch_ntt(stage(jobs), (int)jobs.size(), c3, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  {
+    const NttJob* jd = stage(jobs);
+    TimeIt tm(ctx_, s_, "ntt_chunks", jobs.size() * 6.0 * n * 4, 0);
+    launch_ntt(jd, (int)jobs.size(), c3, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+  }
   keep_.push_back(std::move(Q));
   keep_.push_back(std::move(Y));
   keep_.push_back(std::move(coef));
@@ -891,7 +998,10 @@ void Job::open() {
     dd.points = reinterpret_cast<const uint32_t*>(stg_->put(points.data(), points.size() * sizeof(E6), s_));
     dd.wpow = tb_.wpow(m.log_n);
     dd.out = const_cast<uint32_t* const*>(stage(op));
-    launch_deep(dd, B_, s_);
+    {
+      TimeIt tm(ctx_, s_, "deep", (double)B_ * m.N * (bcols.size() + 6) * 4, 0);
+      launch_deep(dd, B_, s_);
+    }
     level[ci].len = 4 * m.N;
     level[ci].buf = dev((size_t)B_ * 6 * 4 * m.N);
     std::vector<NttJob> jobs;
@@ -910,7 +1020,11 @@ void Job::open() {
       cfg.pre[o] = tb_.lde_pre(m.log_n, o, RATE_LOG, true);
       cfg.bitrev_out[o] = 1;
     }
-    launch_ntt(stage(jobs), (int)jobs.size(), cfg, tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
+    {
+      const NttJob* jd = stage(jobs);
+      TimeIt tm(ctx_, s_, "ntt_lde", jobs.size() * 5.0 * m.N * 4, 0);
+      launch_ntt(jd, (int)jobs.size(), cfg, tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
+    }
     keep_.push_back(std::move(dq.buf));
   }
   // ---- FRI (fri.go:606-736)
@@ -1001,9 +1115,14 @@ void Job::open() {
     }
     int log_len = 0;
     while ((1u << log_len) < running.len) ++log_len;
-    launch_fri_fold(stage(ip), const_cast<uint32_t* const*>(stage(op)),
-                    reinterpret_cast<const uint32_t*>(stg_->put(al.data(), al.size() * sizeof(E6), s_)),
-                    tb_.fri_xinv2(log_len), (int)next.len, B_, s_);
+    {
+      const uint32_t* const* ip_d = stage(ip);
+      uint32_t* const* op_d = const_cast<uint32_t* const*>(stage(op));
+      const uint32_t* al_d = reinterpret_cast<const uint32_t*>(stg_->put(al.data(), al.size() * sizeof(E6), s_));
+      const uint32_t* xi = tb_.fri_xinv2(log_len);
+      TimeIt tm(ctx_, s_, "fri_fold", (double)B_ * running.len * 1.5 * 24, 0);
+      launch_fri_fold(ip_d, op_d, al_d, xi, (int)next.len, B_, s_);
+    }
     layers[j] = std::move(running);
     running = std::move(next);
   }
@@ -1275,6 +1394,7 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   Job job(*this, inputs, outputs, cb);
   job.run();
   LG_HIP(hipStreamSynchronize(ctx_->stream));
+  ctx_->timer.resolve();
 }
 
 }  // namespace lg

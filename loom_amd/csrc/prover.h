@@ -71,7 +71,38 @@ class Staging {
   size_t cap_ = 0, off_ = 0;
 };
 
+// Optional per-kernel timing with HIP events on the launching stream: every
+// launch the prover wraps is bracketed by two events; totals per kernel name
+// (launch count, device milliseconds, algorithmic bytes, Poseidon2
+// permutations) are accumulated once the stream has been synchronised.
+class KernelTimer {
+ public:
+  bool on = false;
+  struct Stat {
+    uint64_t count = 0;
+    double ms = 0;
+    double bytes = 0;
+    double perms = 0;
+  };
+  void begin(const char* name, hipStream_t s, double bytes, double perms);
+  void end(hipStream_t s);
+  void resolve();  // call after the stream is synchronised
+  std::map<std::string, Stat> stats;
+  ~KernelTimer();
+
+ private:
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+    double bytes, perms;
+  };
+  hipEvent_t ev();
+  std::vector<Rec> pending_;
+  std::vector<hipEvent_t> free_;
+};
+
 struct Context {
+  KernelTimer timer;
   int device = 0;
   hipStream_t stream = nullptr;
   Tables tables;

@@ -50,6 +50,21 @@ class Context:
         except Exception:
             pass
 
+    def set_timing(self, on):
+        _lib.check(_lib.lib().lg_ctx_set_timing(self.h, 1 if on else 0), "lg_ctx_set_timing")
+
+    def timing_report(self):
+        """{kernel: (launches, device ms, algorithmic bytes, permutations)};
+        clears the library's totals."""
+        n = ctypes.c_size_t()
+        buf = ctypes.create_string_buffer(1 << 16)
+        _lib.check(_lib.lib().lg_ctx_timing_report(self.h, buf, len(buf), ctypes.byref(n)), "lg_ctx_timing_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, ms, by, pm = line.split()
+            out[name] = (int(cnt), float(ms), float(by), float(pm))
+        return out
+
     def plan_handle(self, words):
         key = words.tobytes()
         if key not in self.plans:
@@ -80,6 +95,9 @@ class Prepared:
         self.perms = self.plan.perm_arrays()
 
 
+_proof_cap = {}
+
+
 def prove_prepared(ctx, items, want_transcript=False, phases=None):
     """Proves prepared instances sharing one plan shape in one batched call."""
     if not items:
@@ -99,7 +117,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None):
             col_ptrs[b * nc + k] = c.ctypes.data_as(_lib.u32p)
         for k, s in enumerate(it.perms):
             perm_ptrs[b * npm + k] = s.ctypes.data_as(_lib.u32p)
-    cap = 1 << 16
+    cap = _proof_cap.get(words.tobytes(), 1 << 21)
     while True:
         outs = [np.zeros(cap, dtype=np.uint32) for _ in range(B)]
         touts = [np.zeros(1 << 16, dtype=np.uint32) for _ in range(B)] if want_transcript else None
@@ -117,6 +135,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None):
             cap = max(lens)
             continue
         _lib.check(rc, "lg_prove")
+        _proof_cap[words.tobytes()] = max(lens)
         break
     if phases is not None:
         arr = (ctypes.c_double * 3)()
