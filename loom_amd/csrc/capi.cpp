@@ -4,6 +4,7 @@
 #include "../../include/loomgpu.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <exception>
 #include <mutex>
@@ -11,6 +12,7 @@
 
 #include "kernels.h"
 #include "p2.h"
+#include "pool.h"
 #include "prover.h"
 
 struct lg_ctx {
@@ -152,8 +154,8 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
         if (ph < 3) phases[ph++] = dt;
       });
     }
-    int rc = 0;
-    for (int b = 0; b < batch; ++b) {
+    std::atomic<int> rc{0};
+    lg::parallel_for(batch, [&](int b) {
       proof_lens[b] = out[b].proof.size();
       if (out[b].proof.size() > proof_caps[b]) {
         rc = -2;
@@ -165,9 +167,9 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
         if (!transcript_caps || out[b].transcript.size() > transcript_caps[b]) rc = -2;
         else std::memcpy(transcripts[b], out[b].transcript.data(), out[b].transcript.size() * 4);
       }
-    }
-    if (rc) g_err = "lg_prove: output buffer too small";
-    return rc;
+    });
+    if (rc.load()) g_err = "lg_prove: output buffer too small";
+    return rc.load();
   });
 }
 
@@ -283,9 +285,12 @@ int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint
     const uint32_t* np_ = num;
     const uint32_t* dp_ = den;
     uint32_t* gp_ = gp;
+    lg::DevBuf ratio(6 * n * sizeof(uint32_t));
+    uint32_t* rp_ = ratio.as<uint32_t>();
     lg::DevBuf a = stage_sync(&np_, sizeof(void*)), b = stage_sync(&dp_, sizeof(void*)), c = stage_sync(&gp_, sizeof(void*));
-    lg::launch_grand_product(a.as<const uint32_t*>(), b.as<const uint32_t*>(), c.as<uint32_t*>(), (int)n, 1,
-                             ctx->c.stream);
+    lg::DevBuf r = stage_sync(&rp_, sizeof(void*));
+    lg::launch_grand_product(a.as<const uint32_t*>(), b.as<const uint32_t*>(), r.as<uint32_t*>(), c.as<uint32_t*>(),
+                             (int)n, 1, ctx->c.stream);
     LG_HIP(hipStreamSynchronize(ctx->c.stream));
     return 0;
   });
@@ -313,8 +318,9 @@ int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t
       cfg.bitrev_out[o] = 1;
     }
     lg::DevBuf jd = stage_sync(jobs.data(), jobs.size() * sizeof(lg::NttJob));
-    lg::launch_ntt(jd.as<lg::NttJob>(), n_cols, cfg, ctx->c.tables.tw_fwd(logn), ctx->c.tables.tw_inv(logn),
-                   ctx->c.stream);
+    lg::DevBuf pd = stage_sync(cfg.pre, sizeof(cfg.pre));
+    lg::launch_ntt(jd.as<lg::NttJob>(), n_cols, cfg, pd.as<const uint32_t*>(), ctx->c.tables.tw_fwd(logn),
+                   ctx->c.tables.tw_inv(logn), ctx->c.stream);
     LG_HIP(hipStreamSynchronize(ctx->c.stream));
     return 0;
   });

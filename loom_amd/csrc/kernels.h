@@ -6,6 +6,13 @@
 
 namespace lg {
 
+// Called by every launcher below right before its kernel launch, once the
+// caller's arguments are final: the prover uses it to push the small tables it
+// staged in pinned host memory to the device in one copy per launch.
+using PreLaunchHook = void (*)(void* user);
+void set_prelaunch_hook(PreLaunchHook fn, void* user);
+void prelaunch();
+
 constexpr int NTT_MAX_LOG_LDS = 14;   // one workgroup holds a size-2^14 transform in LDS
 constexpr int NTT_MAX_OUT = 8;
 
@@ -29,8 +36,10 @@ struct NttCfg {
   int bitrev_out[NTT_MAX_OUT];
 };
 
-void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* tw_fwd,
-                const uint32_t* tw_inv, hipStream_t s);
+// pres_dev: device array holding cfg.pre[0 .. n_out) (the host copy in cfg is
+// only used for validation).
+void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
+                const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s);
 
 // Leaf hashing: for instance b (grid y) and pair leaf k, hashes the message
 // [LEAF, hdr1, hdr2, ...] whose element m >= 3 is column msgmap[m] >> 1 at row
@@ -60,9 +69,9 @@ void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_
                   int k, int n, int batch, hipStream_t s);
 
 // Grand product: gp[0] = 1, gp[i] = gp[i-1] * num[i-1] / den[i-1]; num, den,
-// gp are E6 columns as 6 limb arrays (limb stride n) per instance.
-void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* gp, int n,
-                          int batch, hipStream_t s);
+// ratio (scratch) and gp are E6 columns as 6 limb arrays (limb stride n).
+void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
+                          uint32_t* const* gp, int n, int batch, hipStream_t s);
 
 // Quotient coefficients from the per-coset inverse transforms:
 //   coef[b][t][l][a] = ginv[a + t n] * sum_i wr[(t i) mod rho] * Y[b][l][i n + a]

@@ -10,6 +10,18 @@
 
 namespace lg {
 
+static thread_local PreLaunchHook g_hook = nullptr;
+static thread_local void* g_hook_user = nullptr;
+
+void set_prelaunch_hook(PreLaunchHook fn, void* user) {
+  g_hook = fn;
+  g_hook_user = user;
+}
+
+void prelaunch() {
+  if (g_hook) g_hook(g_hook_user);
+}
+
 __device__ __forceinline__ uint32_t brev(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
 
 // ------------------------------------------------------------------ NTT
@@ -37,11 +49,18 @@ __device__ __forceinline__ void lds_dit(uint32_t* sh, int logn, const uint32_t* 
 
 constexpr int NTT_MAX_PER_THREAD = 16;
 
-__global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ jobs, NttCfg cfg,
+struct NttArgs {
+  int logn, inverse_first, n_out, bitrev_mask;
+};
+
+__global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ jobs, NttArgs cfg,
+                                                    const uint32_t* const* __restrict__ pres,
                                                     const uint32_t* __restrict__ tw_fwd,
                                                     const uint32_t* __restrict__ tw_inv) {
   extern __shared__ uint32_t sh[];
-  const NttJob job = jobs[blockIdx.x];
+  const NttJob* __restrict__ jp = jobs + blockIdx.x;
+  const uint32_t* __restrict__ jin = jp->in;
+  const uint32_t* __restrict__ jpost = jp->post;
   const int logn = cfg.logn;
   const int n = 1 << logn;
   const int T = blockDim.x;
@@ -50,7 +69,7 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
   uint32_t reg[NTT_MAX_PER_THREAD];
 #pragma unroll
   for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
-    if (r < per) reg[r] = job.in[tid + r * T];
+    if (r < per) reg[r] = jin[tid + r * T];
   if (cfg.inverse_first) {
 #pragma unroll
     for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
@@ -62,19 +81,19 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
       if (r < per) {
         int k = tid + r * T;
         uint32_t v = sh[k];
-        if (job.post) v = kb_mul(v, job.post[k]);
+        if (jpost) v = kb_mul(v, jpost[k]);
         reg[r] = v;
       }
     __syncthreads();
     if (cfg.n_out == 0) {
 #pragma unroll
       for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
-        if (r < per) job.out[0][tid + r * T] = reg[r];
+        if (r < per) jp->out[0][tid + r * T] = reg[r];
       return;
     }
   }
   for (int o = 0; o < cfg.n_out; ++o) {
-    const uint32_t* pre = cfg.pre[o];
+    const uint32_t* pre = pres[o];
 #pragma unroll
     for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
       if (r < per) {
@@ -85,8 +104,8 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
       }
     __syncthreads();
     lds_dit(sh, logn, tw_fwd);
-    uint32_t* out = job.out[o];
-    const bool br = cfg.bitrev_out[o] != 0;
+    uint32_t* out = jp->out[o];
+    const bool br = (cfg.bitrev_mask >> o) & 1;
 #pragma unroll
     for (int r = 0; r < NTT_MAX_PER_THREAD; ++r)
       if (r < per) {
@@ -97,8 +116,8 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
   }
 }
 
-void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* tw_fwd,
-                const uint32_t* tw_inv, hipStream_t s) {
+void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
+                const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s) {
   if (n_jobs == 0) return;
   if (cfg.logn > NTT_MAX_LOG_LDS || cfg.logn < 1) throw std::runtime_error("launch_ntt: unsupported size");
   const int n = 1 << cfg.logn;
@@ -106,7 +125,11 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   if (T > 1024) T = 1024;
   if (T < 1) T = 1;
   if (n / T > NTT_MAX_PER_THREAD) throw std::runtime_error("launch_ntt: too many values per thread");
-  hipLaunchKernelGGL(k_ntt_multi, dim3(n_jobs), dim3(T), n * sizeof(uint32_t), s, jobs_dev, cfg, tw_fwd, tw_inv);
+  NttArgs a{cfg.logn, cfg.inverse_first, cfg.n_out, 0};
+  for (int o = 0; o < cfg.n_out; ++o) a.bitrev_mask |= (cfg.bitrev_out[o] ? 1 : 0) << o;
+  prelaunch();
+  hipLaunchKernelGGL(k_ntt_multi, dim3(n_jobs), dim3(T), n * sizeof(uint32_t), s, jobs_dev, a, pres_dev, tw_fwd,
+                     tw_inv);
   LG_HIP(hipGetLastError());
 }
 
@@ -149,6 +172,7 @@ void launch_leaf_hash(const uint32_t* const* colptrs, int ncols, const int32_t* 
                       uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch, hipStream_t s) {
   if (nleaves == 0 || batch == 0) return;
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_leaf_hash, dim3((nleaves + T - 1) / T, batch), dim3(T), 0, s, colptrs, ncols, msgmap, msglen,
                      hdr1, hdr2, outs, nleaves);
   LG_HIP(hipGetLastError());
@@ -180,6 +204,7 @@ __global__ __launch_bounds__(256) void k_merkle_level(uint32_t* const* __restric
 
 void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s) {
   const int T = width < 256 ? width : 256;
+  prelaunch();
   hipLaunchKernelGGL(k_merkle_level, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, width);
   LG_HIP(hipGetLastError());
 }
@@ -229,6 +254,7 @@ void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_
   while ((width_hi >> (levels - 1)) > width_lo) ++levels;
   const int G = width_hi < 256 ? width_hi : 256;
   if ((width_hi / width_lo) > G) throw std::runtime_error("merkle fused: span exceeds workgroup");
+  prelaunch();
   hipLaunchKernelGGL(k_merkle_fused, dim3(width_hi / G, batch), dim3(G), 0, s, trees, width_hi, levels);
   LG_HIP(hipGetLastError());
 }
@@ -239,6 +265,7 @@ __global__ void k_copy_roots(const uint32_t* const* trees, uint32_t* roots) {
 }
 
 void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s) {
+  prelaunch();
   hipLaunchKernelGGL(k_copy_roots, dim3(batch), dim3(64), 0, s, trees, roots);
   LG_HIP(hipGetLastError());
 }
@@ -258,6 +285,7 @@ __global__ __launch_bounds__(256) void k_inject(uint32_t* const* __restrict__ tr
 
 void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s) {
   const int T = width < 256 ? width : 256;
+  prelaunch();
   hipLaunchKernelGGL(k_inject, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, inj, width);
   LG_HIP(hipGetLastError());
 }
@@ -271,6 +299,7 @@ __global__ void k_lagrange_col(uint32_t* const* outs, int n, int pos) {
 
 void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s) {
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_lagrange_col, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, n, pos);
   LG_HIP(hipGetLastError());
 }
@@ -283,6 +312,7 @@ __global__ void k_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32
 
 void launch_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n, int batch, hipStream_t s) {
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_id_support, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, wpow, gk, n);
   LG_HIP(hipGetLastError());
 }
@@ -300,6 +330,7 @@ void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_
   int logn = 0;
   while ((1 << logn) < n) ++logn;
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_sigma, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, S, wpow, gpow, k, n, logn);
   LG_HIP(hipGetLastError());
 }
@@ -325,26 +356,32 @@ __device__ __forceinline__ E6 ldc6(const uint32_t* p) {
 }
 
 // ------------------------------------------------------------------ grand product
+// ratio[i] = num[i] / den[i], one element per thread (grid over all rows).
+__global__ __launch_bounds__(256) void k_ratio(const uint32_t* const* num, const uint32_t* const* den,
+                                               uint32_t* const* ratio, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = blockIdx.y;
+  st6(ratio[b], n, i, e6_mul(ld6(num[b], n, i), e6_inv(ld6(den[b], n, i))));
+}
+
 // One workgroup per instance.  Thread t owns the contiguous chunk
-// [t*c, (t+1)*c) of the ratio vector; chunk products are scanned in LDS.
-__global__ __launch_bounds__(1024) void k_grand_product(const uint32_t* const* num, const uint32_t* const* den,
-                                                         uint32_t* const* gp, int n) {
+// [t*c, (t+1)*c) of the ratio vector; chunk products are scanned in LDS;
+// gp[i] = prod_{k < i} ratio[k].
+__global__ __launch_bounds__(1024) void k_prefix_product(const uint32_t* const* ratio, uint32_t* const* gp, int n) {
   __shared__ uint32_t sh[1024 * 6];
   const int b = blockIdx.x;
-  const uint32_t* N_ = num[b];
-  const uint32_t* D_ = den[b];
+  const uint32_t* Rt = ratio[b];
   uint32_t* G = gp[b];
   const int T = blockDim.x;
   const int tid = threadIdx.x;
   const int c = n / T;
   const int lo = tid * c;
-  // chunk product of ratios r_i = num_i / den_i for i in [lo, lo + c)
   E6 acc = e6_one();
-  for (int i = lo; i < lo + c; ++i) acc = e6_mul(acc, e6_mul(ld6(N_, n, i), e6_inv(ld6(D_, n, i))));
+  for (int i = lo; i < lo + c; ++i) acc = e6_mul(acc, ld6(Rt, n, i));
 #pragma unroll
   for (int l = 0; l < 6; ++l) sh[6 * tid + l] = acc.v[l];
   __syncthreads();
-  // inclusive Hillis-Steele scan of chunk products
   for (int off = 1; off < T; off <<= 1) {
     E6 v = ldc6(sh + 6 * tid);
     E6 u = e6_one();
@@ -357,18 +394,22 @@ __global__ __launch_bounds__(1024) void k_grand_product(const uint32_t* const* n
     }
     __syncthreads();
   }
-  // gp[i] = prod_{k < i} r_k : start from the exclusive prefix of this chunk
   E6 run = tid == 0 ? e6_one() : ldc6(sh + 6 * (tid - 1));
   for (int i = lo; i < lo + c; ++i) {
     st6(G, n, i, run);
-    run = e6_mul(run, e6_mul(ld6(N_, n, i), e6_inv(ld6(D_, n, i))));
+    run = e6_mul(run, ld6(Rt, n, i));
   }
 }
 
-void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* gp, int n,
-                          int batch, hipStream_t s) {
-  int T = n < 1024 ? n : 1024;
-  hipLaunchKernelGGL(k_grand_product, dim3(batch), dim3(T), 0, s, num, den, gp, n);
+void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
+                          uint32_t* const* gp, int n, int batch, hipStream_t s) {
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_ratio, dim3((n + T - 1) / T, batch), dim3(T), 0, s, num, den, ratio, n);
+  LG_HIP(hipGetLastError());
+  int T2 = n < 1024 ? n : 1024;
+  prelaunch();
+  hipLaunchKernelGGL(k_prefix_product, dim3(batch), dim3(T2), 0, s, (const uint32_t* const*)ratio, gp, n);
   LG_HIP(hipGetLastError());
 }
 
@@ -398,6 +439,7 @@ void launch_quot_combine(const uint32_t* Y, uint32_t* coef, const uint32_t* ginv
                          int batch, hipStream_t s) {
   if (rho > 8) throw std::runtime_error("quotient: more than 8 cosets");
   const int T = n < 256 ? n : 256;
+  prelaunch();
   hipLaunchKernelGGL(k_quot_combine, dim3((n + T - 1) / T, 6, batch), dim3(T), 0, s, Y, coef, ginv, wr, rho, n);
   LG_HIP(hipGetLastError());
 }
@@ -420,6 +462,7 @@ __global__ void k_lagrange_weights(const uint32_t* zeta, const uint32_t* common,
 void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const uint32_t* wpow, uint32_t* const* W,
                              int n, int batch, hipStream_t s) {
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_lagrange_weights, dim3((n + T - 1) / T, batch), dim3(T), 0, s, zeta, common, wpow, W, n);
   LG_HIP(hipGetLastError());
 }
@@ -450,6 +493,7 @@ __global__ __launch_bounds__(256) void k_claims(const ClaimJob* jobs, int n) {
 
 void launch_claims(const ClaimJob* jobs, int n_jobs, int n, hipStream_t s) {
   if (n_jobs == 0) return;
+  prelaunch();
   hipLaunchKernelGGL(k_claims, dim3(n_jobs), dim3(256), 0, s, jobs, n);
   LG_HIP(hipGetLastError());
 }
@@ -492,6 +536,7 @@ __global__ __launch_bounds__(256) void k_deep(DeepDesc d) {
 void launch_deep(const DeepDesc& d, int batch, hipStream_t s) {
   if (d.n_points > DEEP_MAX_POINTS) throw std::runtime_error("deep: too many distinct opening points");
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_deep, dim3((d.n + T - 1) / T, batch), dim3(T), 0, s, d);
   LG_HIP(hipGetLastError());
 }
@@ -515,6 +560,7 @@ __global__ void k_fri_fold(const uint32_t* const* in, uint32_t* const* out, cons
 void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
                      int half, int batch, hipStream_t s) {
   const int T = half < 256 ? half : 256;
+  prelaunch();
   hipLaunchKernelGGL(k_fri_fold, dim3((half + T - 1) / T, batch), dim3(T), 0, s, in, out, alpha, xinv2, half);
   LG_HIP(hipGetLastError());
 }
@@ -531,6 +577,7 @@ __global__ void k_fri_combine(uint32_t* const* running, const uint32_t* const* l
 void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len,
                         int batch, hipStream_t s) {
   const int T = 256;
+  prelaunch();
   hipLaunchKernelGGL(k_fri_combine, dim3((len + T - 1) / T, batch), dim3(T), 0, s, running, level, gamma, len);
   LG_HIP(hipGetLastError());
 }
@@ -554,6 +601,7 @@ __global__ void k_gather_rows(const uint32_t* pos, int nq, int shift, const uint
 void launch_gather_rows(const uint32_t* pos, int nq, int shift, const uint32_t* const* cols, int ncols, uint32_t* out,
                         int stride, int off, int batch, hipStream_t s) {
   const int T = 64;
+  prelaunch();
   hipLaunchKernelGGL(k_gather_rows, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, cols, ncols, out,
                      stride, off);
   LG_HIP(hipGetLastError());
@@ -578,6 +626,7 @@ __global__ void k_gather_path(const uint32_t* pos, int nq, int shift, const uint
 void launch_gather_path(const uint32_t* pos, int nq, int shift, const uint32_t* const* trees, int nleaves,
                         uint32_t* out, int stride, int off, int batch, hipStream_t s) {
   const int T = 64;
+  prelaunch();
   hipLaunchKernelGGL(k_gather_path, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, trees, nleaves, out,
                      stride, off);
   LG_HIP(hipGetLastError());
@@ -599,6 +648,7 @@ __global__ void k_gather_digest(const uint32_t* pos, int nq, int shift, int d, c
 void launch_gather_digest(const uint32_t* pos, int nq, int shift, int d, const uint32_t* const* inj, uint32_t* out,
                           int stride, int off, int batch, hipStream_t s) {
   const int T = 64;
+  prelaunch();
   hipLaunchKernelGGL(k_gather_digest, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, d, inj, out,
                      stride, off);
   LG_HIP(hipGetLastError());

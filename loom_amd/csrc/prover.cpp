@@ -13,6 +13,7 @@
 #include <set>
 
 #include "kernels.h"
+#include "pool.h"
 #include "transcript.h"
 
 namespace lg {
@@ -206,15 +207,19 @@ Staging::~Staging() {
   if (host_) (void)hipHostFree(host_);
 }
 
-void* Staging::put(const void* src, size_t n, hipStream_t s) {
+void* Staging::put(const void* src, size_t n, hipStream_t) {
   size_t off = (off_ + 255) & ~size_t(255);
   if (off + n > cap_) throw std::runtime_error("staging area exhausted");
-  if (n) {
-    std::memcpy(host_ + off, src, n);
-    LG_HIP(hipMemcpyAsync(dev_.as<uint8_t>() + off, host_ + off, n, hipMemcpyHostToDevice, s));
-  }
+  if (n) std::memcpy(host_ + off, src, n);
   off_ = off + n;
   return dev_.as<uint8_t>() + off;
+}
+
+void Staging::flush(hipStream_t s) {
+  if (off_ > flushed_) {
+    LG_HIP(hipMemcpyAsync(dev_.as<uint8_t>() + flushed_, host_ + flushed_, off_ - flushed_, hipMemcpyHostToDevice, s));
+    flushed_ = off_;
+  }
 }
 
 std::unique_ptr<Staging> Context::take_staging() {
@@ -299,8 +304,11 @@ class Job {
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
         B_((int)in.size()), s_(ctx_.stream) {
     stg_ = ctx_.take_staging();
+    set_prelaunch_hook([](void* u) { static_cast<Job*>(u)->flush_staging(); }, this);
   }
+  void flush_staging() { stg_->flush(s_); }
   ~Job() {
+    set_prelaunch_hook(nullptr, nullptr);
     for (auto& c : cols_) ctx_.pool.put(std::move(c.buf));
     for (auto& t : trees_) {
       ctx_.pool.put(std::move(t.nodes));
@@ -316,6 +324,10 @@ class Job {
   DevBuf dev(size_t words) { return ctx_.pool.get(words * sizeof(uint32_t)); }
   template <class T>
   const T* stage(const std::vector<T>& v) { return stg_->put(v, s_); }
+  const uint32_t* const* pres(const NttCfg& c) {
+    std::vector<const uint32_t*> v(c.pre, c.pre + NTT_MAX_OUT);
+    return stage(v);
+  }
   void sync() { LG_HIP(hipStreamSynchronize(s_)); }
   void phase(const char* name, double& t0) {
     if (cb_) {
@@ -450,7 +462,7 @@ void Job::lde_columns(const std::vector<uint32_t>& col_ids) {
     }
     const double n = double(1u << kv.first);
     TimeIt t(ctx_, s_, "ntt_lde", kv.second.size() * 5.0 * n * 4, 0);
-    launch_ntt(stage(kv.second), (int)kv.second.size(), cfg, tb_.tw_fwd(kv.first), tb_.tw_inv(kv.first), s_);
+    launch_ntt(stage(kv.second), (int)kv.second.size(), cfg, pres(cfg), tb_.tw_fwd(kv.first), tb_.tw_inv(kv.first), s_);
   }
 }
 
@@ -515,6 +527,17 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
   };
   fold(tb.nleaves);
   int w = tb.nleaves / 2;
+  // Wide levels: one launch per level keeps every lane busy.  Once a level
+  // has few nodes per batch, the remaining levels run in one fused launch
+  // (workgroup-local subtrees), trading idle lanes for fewer launches.
+  while (w >= 1 && (size_t)w * B_ >= (size_t)(1 << 15)) {
+    {
+      TimeIt tm(ctx_, s_, "merkle", (double)B_ * w * 3 * 32, (double)B_ * w);
+      launch_merkle_level(tp_d, w, B_, s_);
+    }
+    fold(w);
+    w /= 2;
+  }
   while (w >= 1) {
     const int G = w < 256 ? w : 256;
     int lo = w;
@@ -620,13 +643,19 @@ void Job::run_steps(size_t r) {
       dag_kernel_launch(K, a, s_);
     }
     std::vector<const uint32_t*> np(B_), dp(B_);
-    std::vector<uint32_t*> gp(B_);
+    std::vector<uint32_t*> gp(B_), rp(B_);
+    Layer ratio;
+    ratio.len = n;
+    ratio.buf = dev((size_t)B_ * 6 * n);
     for (int b = 0; b < B_; ++b) {
       np[b] = nd[0].ptr(b);
       dp[b] = nd[1].ptr(b);
       gp[b] = cols_[st.out_col].lag(b, 0);
+      rp[b] = ratio.ptr(b);
     }
-    launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(gp)), n, B_, s_);
+    launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(rp)),
+                         const_cast<uint32_t* const*>(stage(gp)), n, B_, s_);
+    keep_.push_back(std::move(ratio.buf));
     keep_.push_back(std::move(nd[0].buf));
     keep_.push_back(std::move(nd[1].buf));
   }
@@ -669,7 +698,7 @@ void Job::quotient(size_t mi) {
   {
     const NttJob* jd = stage(jobs);
     TimeIt tm(ctx_, s_, "ntt_quot_coset", jobs.size() * (1.0 + rho) * n * 4, 0);
-    launch_ntt(jd, (int)jobs.size(), cfg, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+    launch_ntt(jd, (int)jobs.size(), cfg, pres(cfg), tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
   }
   // 2. numerator / Z_H on the cosets
   DevBuf Q = dev((size_t)B_ * 6 * rn);
@@ -723,7 +752,7 @@ void Job::quotient(size_t mi) {
   {
     const NttJob* jd = stage(jobs);
     TimeIt tm(ctx_, s_, "ntt_quot_back", jobs.size() * 2.0 * n * 4, 0);
-    launch_ntt(jd, (int)jobs.size(), c2, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+    launch_ntt(jd, (int)jobs.size(), c2, pres(c2), tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
   }
   // 4. rho-point combination -> chunk coefficients [b][t][l][a]
   DevBuf coef = dev((size_t)B_ * rho * 6 * n);
@@ -764,7 +793,7 @@ void Job::quotient(size_t mi) {
   {
     const NttJob* jd = stage(jobs);
     TimeIt tm(ctx_, s_, "ntt_chunks", jobs.size() * 6.0 * n * 4, 0);
-    launch_ntt(jd, (int)jobs.size(), c3, tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+    launch_ntt(jd, (int)jobs.size(), c3, pres(c3), tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
   }
   keep_.push_back(std::move(Q));
   keep_.push_back(std::move(Y));
@@ -800,12 +829,12 @@ void Job::run() {
       for (int b = 0; b < B_; ++b) ts_[b].bind(challenge_name(r), roots_[rd.tree][b].v, 8);
     }
     if (rd.fs_hook != 0) throw std::runtime_error("exposed-value rounds are not supported by this build");
-    for (int b = 0; b < B_; ++b) {
+    parallel_for(B_, [&](int b) {
       const Digest& dg = ts_[b].compute(challenge_name(r));
       E6 v;
       for (int l = 0; l < 6; ++l) v.v[l] = dg.v[l];
       chal_[b][rd.challenge_col] = v;
-    }
+    });
   }
   phase("execute-steps", t0);
   // ComputeAIRQuotients (prover.go:461-561)
@@ -817,10 +846,10 @@ void Job::run() {
     for (int b = 0; b < B_; ++b) ts_[b].bind("__zeta", roots_[t][b].v, 8);
   }
   zeta_.resize(B_);
-  for (int b = 0; b < B_; ++b) {
+  parallel_for(B_, [&](int b) {
     const Digest& dg = ts_[b].compute("__zeta");
     for (int l = 0; l < 6; ++l) zeta_[b].v[l] = dg.v[l];
-  }
+  });
   phase("compute-air-quotients", t0);
   open();
   phase("pcs-open", t0);
@@ -894,13 +923,13 @@ void Job::open() {
   sync();
   keep_.push_back(std::move(cvd));
   // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
-  for (int b = 0; b < B_; ++b) {
+  parallel_for(B_, [&](int b) {
     ts_[b].new_challenge("alpha_DEEP");
     for (size_t mi = 0; mi < nm; ++mi)
       for (size_t k = 0; k < ncv; ++k)
         if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi)
           ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
-  }
+  });
   // FRI challenge registration (fri.go:427-437)
   std::vector<size_t> classes;
   for (size_t mi = 0; mi < nm; ++mi)
@@ -916,13 +945,13 @@ void Job::open() {
   });
   const size_t L = classes.size();
   std::vector<E6> alpha(B_);
-  for (int b = 0; b < B_; ++b) {
+  parallel_for(B_, [&](int b) {
     const Digest& dg = ts_[b].compute("alpha_DEEP");
     for (int l = 0; l < 6; ++l) alpha[b].v[l] = dg.v[l];
     for (size_t l = 1; l < L; ++l) ts_[b].new_challenge("fri_level_" + std::to_string(l) + "_gamma");
     for (int j = 0; j < FRI_ROUNDS; ++j) ts_[b].new_challenge("fri_fold_" + std::to_string(j));
     for (int k = 0; k < nq; ++k) ts_[b].new_challenge("fri_query_" + std::to_string(k));
-  }
+  });
   // ---- DEEP quotient per class, RS-encoded (deep.go:71-266)
   std::vector<Layer> level(L);
   for (size_t ci = 0; ci < L; ++ci) {
@@ -1023,7 +1052,7 @@ void Job::open() {
     {
       const NttJob* jd = stage(jobs);
       TimeIt tm(ctx_, s_, "ntt_lde", jobs.size() * 5.0 * m.N * 4, 0);
-      launch_ntt(jd, (int)jobs.size(), cfg, tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
+      launch_ntt(jd, (int)jobs.size(), cfg, pres(cfg), tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
     }
     keep_.push_back(std::move(dq.buf));
   }
@@ -1050,20 +1079,20 @@ void Job::open() {
   ext_tree(levels_tree, lvp);
   std::vector<Digest> levels_root = roots_of(levels_tree);
   std::vector<std::vector<E6>> gamma(L, std::vector<E6>(B_));
-  for (size_t l = 1; l < L; ++l)
-    for (int b = 0; b < B_; ++b) {
+  parallel_for(B_, [&](int b) {
+    for (size_t l = 1; l < L; ++l) {
       const std::string nmg = "fri_level_" + std::to_string(l) + "_gamma";
       if (l == 1) ts_[b].bind(nmg, levels_root[b].v, 8);
       const Digest& dg = ts_[b].compute(nmg);
       for (int k = 0; k < 6; ++k) gamma[l][b].v[k] = dg.v[k];
     }
-  for (int j = 0; j < first; ++j)
-    for (int b = 0; b < B_; ++b) {
+    for (int j = 0; j < first; ++j) {
       const std::string nf = "fri_fold_" + std::to_string(j);
       uint32_t zero[8] = {0};
       ts_[b].bind(nf, zero, 8);
       ts_[b].compute(nf);
     }
+  });
   std::vector<Layer> layers(FRI_ROUNDS + 1);
   std::vector<TreeBuf> ftrees(FRI_ROUNDS);
   std::vector<std::vector<Digest>> fri_roots;
@@ -1098,12 +1127,12 @@ void Job::open() {
       fri_roots.push_back(root);
     }
     std::vector<E6> al(B_);
-    for (int b = 0; b < B_; ++b) {
+    parallel_for(B_, [&](int b) {
       const std::string nf = "fri_fold_" + std::to_string(j);
       ts_[b].bind(nf, root[b].v, 8);
       const Digest& dg = ts_[b].compute(nf);
       for (int k = 0; k < 6; ++k) al[b].v[k] = dg.v[k];
-    }
+    });
     Layer next;
     next.len = running.len / 2;
     next.buf = dev((size_t)B_ * 6 * next.len);
@@ -1132,7 +1161,7 @@ void Job::open() {
   LG_HIP(hipMemcpyAsync(final_w.data(), running.buf.as<void>(), final_w.size() * 4, hipMemcpyDeviceToHost, s_));
   sync();
   std::vector<uint32_t> pos((size_t)B_ * nq);
-  for (int b = 0; b < B_; ++b) {
+  parallel_for(B_, [&](int b) {
     std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
     for (uint32_t k = 0; k < flen; ++k)
       for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
@@ -1144,7 +1173,7 @@ void Job::open() {
       pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
       if (k < nq - 1) ts_[b].bind("fri_query_" + std::to_string(k + 1), dg.v, 8);
     }
-  }
+  });
   // ---- gathers: one record per (instance, query)
   const uint32_t* pos_d = stage(pos);
   size_t stride = 0;
@@ -1262,9 +1291,10 @@ void Job::open() {
   LG_HIP(hipMemcpyAsync(gh.data(), gat.as<void>(), gh.size() * 4, hipMemcpyDeviceToHost, s_));
   sync();
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
-  for (int b = 0; b < B_; ++b) {
+  parallel_for(B_, [&](int b) {
     std::vector<uint32_t>& w = out_[b].proof;
     w.clear();
+    w.reserve(64 + (size_t)nq * (stride + 64 * (nt + FRI_ROUNDS)) + 16 * ncv);
     const std::string hid = "poseidon2";
     w.push_back((uint32_t)hid.size());
     for (size_t i = 0; i < hid.size(); i += 4) {
@@ -1372,7 +1402,7 @@ void Job::open() {
       }
     }
     out_[b].transcript = ts_[b].log_words();
-  }
+  });
   for (auto& x : W) keep_.push_back(std::move(x));
   for (auto& x : level) keep_.push_back(std::move(x.buf));
   for (auto& x : layers) keep_.push_back(std::move(x.buf));

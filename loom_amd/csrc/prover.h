@@ -57,9 +57,12 @@ class Staging {
   ~Staging();
   Staging(const Staging&) = delete;
   Staging& operator=(const Staging&) = delete;
-  void reset() { off_ = 0; }
-  // Copies n bytes to the device area on stream s; returns the device address.
+  void reset() { off_ = flushed_ = 0; }
+  // Appends n bytes to the pinned area and returns the device address they
+  // will have after the next flush.
   void* put(const void* src, size_t n, hipStream_t s);
+  // Copies everything appended since the last flush to the device (one copy).
+  void flush(hipStream_t s);
   template <class T>
   const T* put(const std::vector<T>& v, hipStream_t s) {
     return static_cast<const T*>(put(v.data(), v.size() * sizeof(T), s));
@@ -68,7 +71,7 @@ class Staging {
  private:
   uint8_t* host_ = nullptr;
   DevBuf dev_;
-  size_t cap_ = 0, off_ = 0;
+  size_t cap_ = 0, off_ = 0, flushed_ = 0;
 };
 
 // Optional per-kernel timing with HIP events on the launching stream: every

@@ -119,7 +119,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None):
             perm_ptrs[b * npm + k] = s.ctypes.data_as(_lib.u32p)
     cap = _proof_cap.get(words.tobytes(), 1 << 21)
     while True:
-        outs = [np.zeros(cap, dtype=np.uint32) for _ in range(B)]
+        outs = [np.empty(cap, dtype=np.uint32) for _ in range(B)]
         touts = [np.zeros(1 << 16, dtype=np.uint32) for _ in range(B)] if want_transcript else None
         out_ptrs = (_lib.u32p * B)(*[o.ctypes.data_as(_lib.u32p) for o in outs])
         caps = (ctypes.c_size_t * B)(*([cap] * B))
@@ -141,7 +141,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None):
         arr = (ctypes.c_double * 3)()
         _lib.check(_lib.lib().lg_plan_last_phases(ph, arr), "lg_plan_last_phases")
         phases[:] = list(arr)
-    res = [outs[b][:lens[b]].copy() for b in range(B)]
+    res = [outs[b][:lens[b]] for b in range(B)]
     if want_transcript:
         return res, [touts[b][:tlens[b]].copy() for b in range(B)]
     return res
