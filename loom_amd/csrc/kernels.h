@@ -55,6 +55,10 @@ void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream
 // Levels below `width_hi` down to `width_lo` (inclusive) in one pass over
 // workgroup-local subtrees (width_hi / width_lo <= 256).
 void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s);
+// Same with one octet of lanes per node (cooperative Poseidon2, DPP sums):
+// ~6x lower latency per level, for levels with few nodes per batch
+// (width_hi / width_lo <= 32).
+void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s);
 // roots[8b .. 8b+8) = trees[b][8 .. 16)  (node 1 of every tree)
 void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s);
 // Injection fold: nodes[w + j] = H(nodes[w + j], inj[b][j]).
@@ -123,6 +127,29 @@ void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint
 // running[k] += gamma * level[k]
 void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len,
                         int batch, hipStream_t s);
+
+// All query openings in one launch.  Each op copies, for every (instance b,
+// query q), a fixed number of units into the record of (b, q) at
+// out + (b * nq + q) * stride + out_off; blocks [block0, block0 + nblocks)
+// of the grid belong to the op.
+//   rows:   units = 2 * ncols words: column c at rows lo, lo + 1 (lo first),
+//           row = pos >> shift, lo = row & ~1, columns ptrs[b * ncols + c]
+//   path:   units = ncols (= depth) digests: siblings of leaf lo / 2 of the
+//           tree ptrs[b] with nleaves leaves, leaf level first
+//   digest: units = 1 digest: ptrs[b] + 8 * ((lo / 2) >> d)
+struct GatherOp {
+  int32_t kind;  // 0 rows, 1 path, 2 digest
+  int32_t shift;
+  int32_t ncols;
+  int32_t d;
+  int32_t nleaves;
+  int32_t out_off;
+  int32_t block0;
+  int32_t nblocks;
+  const uint32_t* const* ptrs;
+};
+void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int batch,
+                   uint32_t* out, int stride, hipStream_t s);
 
 // Query gathers.  pos: [b * nq + q] full-domain positions.
 // Rows: row = pos >> shift, lo = row & ~1; writes cols[c][lo], cols[c][lo+1]

@@ -249,6 +249,56 @@ __global__ __launch_bounds__(256) void k_merkle_fused(uint32_t* const* __restric
   }
 }
 
+// Cooperative variant: an octet of lanes per node (see P2Coop), 32 octets per
+// workgroup; a workgroup owns G <= 32 consecutive nodes of level width_hi and
+// reduces them to one node, keeping the running level in LDS.
+__global__ __launch_bounds__(256) void k_merkle_coop(uint32_t* const* __restrict__ trees, int width_hi, int levels) {
+  __shared__ uint4 sh[2][32 * 2];
+  uint32_t* nodes = trees[blockIdx.y];
+  const int q = threadIdx.x & 7;
+  const int oct = threadIdx.x >> 3;
+  const int G = width_hi < 32 ? width_hi : 32;
+  const int base = blockIdx.x * G;
+  P2Coop P;
+  P.init(q);
+  int cnt = G;
+  int cur = 0;
+  for (int lev = 0; lev < levels; ++lev) {
+    const int w = width_hi >> lev;
+    if (oct < cnt) {
+      const size_t node = (size_t)w + (base >> lev) + oct;
+      uint32_t x[4] = {0u, 0u, 0u, 0u};
+      if (q == 0) x[0] = TAG_NODE_MONT;
+      if (q >= 2 && q < 6) {
+        const int child = (q - 2) >> 1, half = (q - 2) & 1;
+        uint4 v;
+        if (lev == 0) v = reinterpret_cast<const uint4*>(nodes + 8 * (2 * node + child))[half];
+        else v = sh[cur][2 * (2 * oct + child) + half];
+        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
+      }
+      P.permute(x);
+      if (q < 2) {
+        const uint4 o = make_uint4(x[0], x[1], x[2], x[3]);
+        reinterpret_cast<uint4*>(nodes + 8 * node)[q] = o;
+        sh[cur ^ 1][2 * oct + q] = o;
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+    cnt >>= 1;
+  }
+}
+
+void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s) {
+  int levels = 1;
+  while ((width_hi >> (levels - 1)) > width_lo) ++levels;
+  const int G = width_hi < 32 ? width_hi : 32;
+  if ((width_hi / width_lo) > G) throw std::runtime_error("merkle coop: span exceeds workgroup");
+  prelaunch();
+  hipLaunchKernelGGL(k_merkle_coop, dim3(width_hi / G, batch), dim3(8 * G), 0, s, trees, width_hi, levels);
+  LG_HIP(hipGetLastError());
+}
+
 void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s) {
   int levels = 1;
   while ((width_hi >> (levels - 1)) > width_lo) ++levels;
@@ -583,6 +633,51 @@ void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, 
 }
 
 // ------------------------------------------------------------------ gathers
+constexpr int GATHER_T = 256;
+
+__global__ __launch_bounds__(GATHER_T) void k_gather(const GatherOp* __restrict__ ops, int n_ops,
+                                                     const uint32_t* __restrict__ pos, int nq, int batch,
+                                                     uint32_t* __restrict__ out, int stride) {
+  int k = 0;
+  while (k + 1 < n_ops && (int)blockIdx.x >= ops[k + 1].block0) ++k;
+  const GatherOp op = ops[k];
+  const long u = (long)((int)blockIdx.x - op.block0) * GATHER_T + threadIdx.x;
+  const int per = op.kind == 0 ? 2 * op.ncols : (op.kind == 1 ? op.ncols : 1);
+  const long total = (long)batch * nq * per;
+  if (u >= total) return;
+  const int bq = (int)(u / per);
+  const int e = (int)(u % per);
+  const int b = bq / nq;
+  const uint32_t row = pos[bq] >> op.shift;
+  const uint32_t lo = row & ~1u;
+  uint32_t* o = out + (size_t)bq * stride + op.out_off;
+  if (op.kind == 0) {
+    const int c = e % op.ncols, hi = e / op.ncols;
+    o[e] = op.ptrs[(size_t)b * op.ncols + c][lo + hi];
+  } else if (op.kind == 1) {
+    const uint32_t node = (((uint32_t)op.nleaves + (lo >> 1)) >> e) ^ 1u;
+    const uint4* src = reinterpret_cast<const uint4*>(op.ptrs[b] + 8 * (size_t)node);
+    uint4* dst = reinterpret_cast<uint4*>(o + 8 * e);
+    uint4 a = src[0], c = src[1];
+    dst[0] = a;
+    dst[1] = c;
+  } else {
+    const uint4* src = reinterpret_cast<const uint4*>(op.ptrs[b] + 8 * (size_t)((lo >> 1) >> op.d));
+    uint4* dst = reinterpret_cast<uint4*>(o);
+    uint4 a = src[0], c = src[1];
+    dst[0] = a;
+    dst[1] = c;
+  }
+}
+
+void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int batch,
+                   uint32_t* out, int stride, hipStream_t s) {
+  if (n_ops == 0 || total_blocks == 0) return;
+  prelaunch();
+  hipLaunchKernelGGL(k_gather, dim3(total_blocks), dim3(GATHER_T), 0, s, ops_dev, n_ops, pos, nq, batch, out, stride);
+  LG_HIP(hipGetLastError());
+}
+
 __global__ void k_gather_rows(const uint32_t* pos, int nq, int shift, const uint32_t* const* cols, int ncols,
                               uint32_t* out, int stride, int off) {
   const int q = blockIdx.x * blockDim.x + threadIdx.x;

@@ -127,3 +127,96 @@ inline void p2_sponge_host(const uint32_t* msg, size_t len, uint32_t* digest) {
 }
 
 }  // namespace lg
+
+#if defined(__HIPCC__)
+namespace lg {
+
+// ---------------------------------------------------------------- cooperative
+// One width-24 state spread over an octet of lanes (8 consecutive lanes of a
+// wavefront): lane q < 6 holds state elements 4q..4q+3, lanes 6 and 7 hold
+// zeros.  Cross-lane sums use DPP (quad_perm and row_half_mirror), so the
+// dependent chain of one permutation is ~6x shorter than with one state per
+// lane; used where few independent permutations are available (tree tops,
+// small FRI layers) and latency, not throughput, bounds the time.
+
+__device__ __forceinline__ uint32_t dpp_quad_xor1(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t dpp_quad_xor2(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);
+}
+__device__ __forceinline__ uint32_t dpp_half_mirror(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);
+}
+
+// Sum over the 8 lanes of the octet, returned to every lane.
+__device__ __forceinline__ uint32_t oct_sum(uint32_t v) {
+  v = kb_add(v, dpp_quad_xor1(v));
+  v = kb_add(v, dpp_quad_xor2(v));
+  return kb_add(v, dpp_half_mirror(v));
+}
+
+struct P2Coop {
+  uint32_t rk[6][4];  // full-round keys of this lane's chunk (zeros on lanes 6, 7)
+  uint32_t d[4];      // internal diagonal of this lane's chunk
+  uint32_t amask;     // all ones on lanes q < 6
+  uint32_t l0mask;    // all ones on lane q == 0
+
+  __device__ __forceinline__ void init(int q) {
+    const bool active = q < 6;
+    amask = active ? 0xffffffffu : 0u;
+    l0mask = q == 0 ? 0xffffffffu : 0u;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = 4 * (q < 6 ? q : 0) + e;
+#pragma unroll
+      for (int r = 0; r < 6; ++r) rk[r][e] = active ? P2_W24_FULL_RK[r][i] : 0u;
+      d[e] = active ? P2_W24_DIAG[i] : 0u;
+    }
+  }
+
+  __device__ __forceinline__ void external(uint32_t* x) const {
+    uint32_t x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
+    uint32_t t01 = kb_add(x0, x1), t23 = kb_add(x2, x3);
+    uint32_t t0123 = kb_add(t01, t23);
+    uint32_t t01123 = kb_add(t0123, x1), t01233 = kb_add(t0123, x3);
+    x[3] = kb_add(kb_dbl(x0), t01233);
+    x[1] = kb_add(kb_dbl(x2), t01123);
+    x[0] = kb_add(t01, t01123);
+    x[2] = kb_add(t23, t01233);
+    // lanes 6, 7 hold zeros, which M4 keeps; adding a masked sum keeps them so
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const uint32_t t = oct_sum(x[e]);
+      x[e] = kb_add(x[e], t & amask);
+    }
+  }
+
+  __device__ __forceinline__ void permute(uint32_t* x) const {
+    external(x);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = kb_cube(kb_add(x[e], rk[r][e]));
+      external(x);
+    }
+#pragma unroll
+    for (int r = 0; r < 21; ++r) {
+      const uint32_t c = kb_cube(kb_add(x[0], P2_W24_PARTIAL_RK[r]));
+      x[0] = (c & l0mask) | (x[0] & ~l0mask);
+      const uint32_t s = oct_sum(kb_add(kb_add(x[0], x[1]), kb_add(x[2], x[3]))) & amask;
+      // inactive lanes: x = 0, d = 0 and s masked to 0, so they stay 0
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = kb_add(s, kb_mul(x[e], d[e]));
+    }
+#pragma unroll
+    for (int r = 3; r < 6; ++r) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) x[e] = kb_cube(kb_add(x[e], rk[r][e]));
+      external(x);
+    }
+  }
+};
+
+}  // namespace lg
+#endif

@@ -222,6 +222,35 @@ void Staging::flush(hipStream_t s) {
   }
 }
 
+uint32_t* Context::pinned(size_t bytes) {
+  if (bytes > pinned_cap_) {
+    if (pinned_) (void)hipHostFree(pinned_);
+    pinned_ = nullptr;
+    pinned_cap_ = 0;
+    size_t cap = std::max<size_t>(bytes, 1u << 20);
+    LG_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), cap, hipHostMallocDefault));
+    pinned_cap_ = cap;
+  }
+  return pinned_;
+}
+
+uint32_t* Context::pinned_in(size_t bytes) {
+  if (bytes > pinned_in_cap_) {
+    if (pinned_in_) (void)hipHostFree(pinned_in_);
+    pinned_in_ = nullptr;
+    pinned_in_cap_ = 0;
+    size_t cap = std::max<size_t>(bytes, 1u << 20);
+    LG_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_in_), cap, hipHostMallocDefault));
+    pinned_in_cap_ = cap;
+  }
+  return pinned_in_;
+}
+
+Context::~Context() {
+  if (pinned_) (void)hipHostFree(pinned_);
+  if (pinned_in_) (void)hipHostFree(pinned_in_);
+}
+
 std::unique_ptr<Staging> Context::take_staging() {
   std::lock_guard<std::mutex> lock(staging_mu);
   if (!staging_free.empty()) {
@@ -315,6 +344,7 @@ class Job {
       for (auto& x : t.inj) ctx_.pool.put(std::move(x));
     }
     for (auto& x : keep_) ctx_.pool.put(std::move(x));
+    for (auto& x : perm_bufs_) ctx_.pool.put(std::move(x));
     if (stg_) ctx_.give_staging(std::move(stg_));
   }
 
@@ -337,6 +367,17 @@ class Job {
       t0 = t;
     }
   }
+
+  // LOOMGPU_TRACE=1: host-side milestones of the run, to stderr.
+  void trace(const char* what) {
+    static const bool on = getenv("LOOMGPU_TRACE") != nullptr;
+    if (!on) return;
+    sync();
+    double t = now_s();
+    fprintf(stderr, "[loomgpu] B=%d %-24s %8.3f ms\n", B_, what, (t - t_trace_) * 1e3);
+    t_trace_ = t;
+  }
+  double t_trace_ = now_s();
 
   void alloc_columns();
   void upload_inputs();
@@ -391,24 +432,48 @@ void Job::alloc_columns() {
   }
 }
 
+// Trace columns and permutations go through one pinned buffer: the copies
+// into it run on the host thread pool, then one DMA per buffer.
 void Job::upload_inputs() {
   for (int b = 0; b < B_; ++b) {
     if (in_[b].columns.size() != d_.inputs.size()) throw std::runtime_error("wrong number of input columns");
     if (in_[b].perms.size() != d_.perms.size()) throw std::runtime_error("wrong number of permutations");
-    for (size_t k = 0; k < d_.inputs.size(); ++k) {
-      const ColBuf& cb = cols_[d_.inputs[k]];
-      if (cb.limbs != 1) throw std::runtime_error("extension-field input columns are not supported");
-      LG_HIP(hipMemcpyAsync(cb.lag(b, 0), in_[b].columns[k], cb.n * sizeof(uint32_t), hipMemcpyHostToDevice, s_));
-    }
   }
-  perm_bufs_.clear();
+  for (size_t k = 0; k < d_.inputs.size(); ++k)
+    if (cols_[d_.inputs[k]].limbs != 1) throw std::runtime_error("extension-field input columns are not supported");
+  // host layout: [b][input k] then [perm p][b]
+  size_t words = 0;
+  std::vector<size_t> col_off((size_t)B_ * d_.inputs.size()), perm_off(d_.perms.size() * B_);
+  for (int b = 0; b < B_; ++b)
+    for (size_t k = 0; k < d_.inputs.size(); ++k) {
+      col_off[(size_t)b * d_.inputs.size() + k] = words;
+      words += cols_[d_.inputs[k]].n;
+    }
   for (size_t p = 0; p < d_.perms.size(); ++p)
     for (int b = 0; b < B_; ++b) {
-      DevBuf x = dev(d_.perms[p].len);
-      LG_HIP(hipMemcpyAsync(x.as<void>(), in_[b].perms[p], d_.perms[p].len * sizeof(uint32_t),
-                            hipMemcpyHostToDevice, s_));
-      perm_bufs_.push_back(std::move(x));
+      perm_off[p * B_ + b] = words;
+      words += d_.perms[p].len;
     }
+  uint32_t* h = ctx_.pinned_in(words * 4);
+  const size_t nin = d_.inputs.size();
+  parallel_for((int)(B_ * (nin + d_.perms.size())), [&](int i) {
+    const int b = i % B_;
+    const size_t k = (size_t)i / B_;
+    if (k < nin) std::memcpy(h + col_off[(size_t)b * nin + k], in_[b].columns[k], cols_[d_.inputs[k]].n * 4);
+    else std::memcpy(h + perm_off[(k - nin) * B_ + b], in_[b].perms[k - nin], d_.perms[k - nin].len * 4);
+  });
+  for (int b = 0; b < B_; ++b)
+    for (size_t k = 0; k < nin; ++k) {
+      const ColBuf& cb = cols_[d_.inputs[k]];
+      LG_HIP(hipMemcpyAsync(cb.lag(b, 0), h + col_off[(size_t)b * nin + k], cb.n * 4, hipMemcpyHostToDevice, s_));
+    }
+  perm_bufs_.clear();
+  for (size_t p = 0; p < d_.perms.size(); ++p) {
+    DevBuf x = dev((size_t)d_.perms[p].len * B_);
+    LG_HIP(hipMemcpyAsync(x.as<void>(), h + perm_off[p * B_], (size_t)d_.perms[p].len * B_ * 4, hipMemcpyHostToDevice,
+                          s_));
+    perm_bufs_.push_back(std::move(x));
+  }
 }
 
 void Job::gencol() {
@@ -428,7 +493,7 @@ void Job::gencol() {
     } else {
       const Perm& pm = d_.perms[ci.a1];
       std::vector<const uint32_t*> S(B_);
-      for (int b = 0; b < B_; ++b) S[b] = perm_bufs_[ci.a1 * B_ + b].as<uint32_t>();
+      for (int b = 0; b < B_; ++b) S[b] = perm_bufs_[ci.a1].as<uint32_t>() + (size_t)b * pm.len;
       launch_sigma(const_cast<uint32_t* const*>(outs_d), stage(S), tb_.wpow(logn), tb_.gpow_small(pm.len / cb.n),
                    ci.a0, cb.n, B_, s_);
     }
@@ -538,13 +603,17 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
     fold(w);
     w /= 2;
   }
+  // Narrow levels: cooperative permutations (an octet of lanes per node),
+  // up to 6 levels per launch on workgroup-local subtrees.
   while (w >= 1) {
-    const int G = w < 256 ? w : 256;
+    const int G = w < 32 ? w : 32;
     int lo = w;
     while (lo > 1 && !has_inj(lo) && (w / (lo / 2)) <= G) lo /= 2;
     {
-      TimeIt tm(ctx_, s_, "merkle", (double)B_ * (2.0 * w - lo) * 3 * 32, (double)B_ * (2.0 * w - lo));
-      launch_merkle_levels_fused(tp_d, w, lo, B_, s_);
+      int span = 0;
+      for (int x = w; x >= lo; x /= 2) span += x;
+      TimeIt tm(ctx_, s_, "merkle_coop", (double)B_ * span * 3 * 32, (double)B_ * span);
+      launch_merkle_coop(tp_d, w, lo, B_, s_);
     }
     fold(lo);
     w = lo / 2;
@@ -805,6 +874,7 @@ void Job::run() {
   LG_HIP(hipSetDevice(ctx_.device));
   alloc_columns();
   upload_inputs();
+  trace("upload");
   trees_.resize(d_.trees.size());
   roots_.resize(d_.trees.size());
   chal_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
@@ -820,6 +890,7 @@ void Job::run() {
     ts_[b].bind(init, sizes);
   }
   gencol();
+  trace("gencol");
   // ExecuteSteps (prover.go:360-412)
   for (size_t r = 0; r < nr; ++r) {
     const Round& rd = d_.rounds[r];
@@ -835,11 +906,13 @@ void Job::run() {
       for (int l = 0; l < 6; ++l) v.v[l] = dg.v[l];
       chal_[b][rd.challenge_col] = v;
     });
+    trace(("round " + std::to_string(r)).c_str());
   }
   phase("execute-steps", t0);
   // ComputeAIRQuotients (prover.go:461-561)
   for (size_t mi = 0; mi < d_.modules.size(); ++mi)
     if (d_.modules[mi].n_chunks) quotient(mi);
+  trace("quotient");
   for (size_t t = 0; t < d_.trees.size(); ++t) {
     if (d_.trees[t].kind != 1) continue;
     commit_tree(t);
@@ -850,6 +923,7 @@ void Job::run() {
     const Digest& dg = ts_[b].compute("__zeta");
     for (int l = 0; l < 6; ++l) zeta_[b].v[l] = dg.v[l];
   });
+  trace("air commit + zeta");
   phase("compute-air-quotients", t0);
   open();
   phase("pcs-open", t0);
@@ -922,6 +996,7 @@ void Job::open() {
   LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
   sync();
   keep_.push_back(std::move(cvd));
+  trace("claimed values");
   // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
   parallel_for(B_, [&](int b) {
     ts_[b].new_challenge("alpha_DEEP");
@@ -1056,6 +1131,7 @@ void Job::open() {
     }
     keep_.push_back(std::move(dq.buf));
   }
+  trace("alpha + deep");
   // ---- FRI (fri.go:606-736)
   auto ext_tree = [&](TreeBuf& tb, const std::vector<const Layer*>& lv) {
     std::vector<std::vector<const uint32_t*>> gcols;
@@ -1155,6 +1231,7 @@ void Job::open() {
     layers[j] = std::move(running);
     running = std::move(next);
   }
+  trace("fri rounds");
   // final polynomial (running after the last round): 4 E6 values
   const uint32_t flen = running.len;
   std::vector<uint32_t> final_w((size_t)B_ * 6 * flen);
@@ -1174,13 +1251,20 @@ void Job::open() {
       if (k < nq - 1) ts_[b].bind("fri_query_" + std::to_string(k + 1), dg.v, 8);
     }
   });
-  // ---- gathers: one record per (instance, query)
+  trace("queries (host)");
+  // ---- gathers: one record per (instance, query), all in one launch
   const uint32_t* pos_d = stage(pos);
   size_t stride = 0;
+  auto seg = [&](size_t words) {  // 16-byte aligned segment of the record
+    size_t o = (stride + 3) & ~size_t(3);
+    stride = o + words;
+    return o;
+  };
   struct TreeRec {
-    size_t rows_off, path_off, injl_off, injrows_off;
+    size_t rows_off, path_off, injl_off;
     int ncols_top, depth, shift;
     std::vector<int> inj_ncols, inj_shift;
+    std::vector<size_t> inj_rows_off;
   };
   std::vector<TreeRec> trec(nt);
   auto cols_of_group = [&](const Group& g, int b) {
@@ -1189,6 +1273,25 @@ void Job::open() {
     for (auto c : g.ext)
       for (int l = 0; l < 6; ++l) cp.push_back(cols_[c].lde(b, l));
     return cp;
+  };
+  std::vector<GatherOp> ops;
+  int blocks = 0;
+  auto add_op = [&](int kind, int shift, int ncols, int d, int nleaves, size_t off,
+                    const std::vector<const uint32_t*>& ptrs) {
+    GatherOp op{};
+    op.kind = kind;
+    op.shift = shift;
+    op.ncols = ncols;
+    op.d = d;
+    op.nleaves = nleaves;
+    op.out_off = (int32_t)off;
+    op.ptrs = stage(ptrs);
+    const long per = kind == 0 ? 2 * ncols : (kind == 1 ? ncols : 1);
+    const long total = (long)B_ * nq * per;
+    op.block0 = blocks;
+    op.nblocks = (int)((total + 255) / 256);
+    blocks += op.nblocks;
+    if (op.nblocks) ops.push_back(op);
   };
   for (size_t t = 0; t < nt; ++t) {
     const TreeInfo& ti = d_.trees[t];
@@ -1199,21 +1302,35 @@ void Job::open() {
     r.depth = 0;
     while ((1 << r.depth) < tb.nleaves) ++r.depth;
     r.shift = FRI_LOG_N - (r.depth + 1);
-    r.rows_off = stride;
-    stride += 2 * r.ncols_top;
-    r.path_off = stride;
-    stride += 8 * r.depth;
-    r.injl_off = stride;
-    stride += 8 * tb.inj.size();
-    r.injrows_off = stride;
-    for (size_t k = 1; k < tb.order.size(); ++k) {
-      const Group& g = ti.groups[tb.order[k]];
-      int nc = (int)(g.base.size() + 6 * g.ext.size());
-      r.inj_ncols.push_back(nc);
+    r.rows_off = seg(2 * r.ncols_top);
+    r.path_off = seg(8 * r.depth);
+    r.injl_off = seg(8 * tb.inj.size());
+    std::vector<const uint32_t*> cp, tp;
+    for (int b = 0; b < B_; ++b) {
+      auto c = cols_of_group(top, b);
+      cp.insert(cp.end(), c.begin(), c.end());
+      tp.push_back(tb.node_ptr(b));
+    }
+    add_op(0, r.shift, r.ncols_top, 0, 0, r.rows_off, cp);
+    add_op(1, r.shift, r.depth, 0, tb.nleaves, r.path_off, tp);
+    for (size_t k = 0; k < tb.inj.size(); ++k) {
+      const Group& g = ti.groups[tb.order[k + 1]];
+      const int nc = (int)(g.base.size() + 6 * g.ext.size());
       int lr = 0;
       while ((1u << lr) < g.N * 4) ++lr;
+      r.inj_ncols.push_back(nc);
       r.inj_shift.push_back(FRI_LOG_N - lr);
-      stride += 2 * nc;
+      r.inj_rows_off.push_back(seg(2 * nc));
+      std::vector<const uint32_t*> ip, gp;
+      for (int b = 0; b < B_; ++b) {
+        ip.push_back(tb.inj_ptr(k, b));
+        auto c = cols_of_group(g, b);
+        gp.insert(gp.end(), c.begin(), c.end());
+      }
+      int dsh = 0;
+      while ((tb.nleaves >> dsh) > tb.inj_width[k]) ++dsh;
+      add_op(2, r.shift, 1, dsh, 0, r.injl_off + 8 * k, ip);
+      add_op(0, r.inj_shift.back(), nc, 0, 0, r.inj_rows_off.back(), gp);
     }
   }
   struct FriRec {
@@ -1223,52 +1340,9 @@ void Job::open() {
   std::vector<FriRec> frec(FRI_ROUNDS);
   for (int j = first; j < FRI_ROUNDS; ++j) {
     FriRec& r = frec[j];
-    const int len_log = FRI_LOG_N - j;
-    r.depth = len_log - 1;
-    r.rows_off = stride;
-    stride += 12;
-    r.path_off = stride;
-    stride += 8 * r.depth;
-  }
-  std::vector<size_t> lq_off(L);
-  for (size_t l = 1; l < L; ++l) {
-    lq_off[l] = stride;
-    stride += 12;
-  }
-  DevBuf gat = dev((size_t)B_ * nq * stride);
-  for (size_t t = 0; t < nt; ++t) {
-    const TreeInfo& ti = d_.trees[t];
-    const TreeBuf& tb = trees_[t];
-    const TreeRec& r = trec[t];
-    std::vector<const uint32_t*> cp, tp;
-    for (int b = 0; b < B_; ++b) {
-      auto c = cols_of_group(ti.groups[tb.order[0]], b);
-      cp.insert(cp.end(), c.begin(), c.end());
-      tp.push_back(tb.node_ptr(b));
-    }
-    launch_gather_rows(pos_d, nq, r.shift, stage(cp), r.ncols_top, gat.as<uint32_t>(), (int)stride, (int)r.rows_off,
-                       B_, s_);
-    launch_gather_path(pos_d, nq, r.shift, stage(tp), tb.nleaves, gat.as<uint32_t>(), (int)stride, (int)r.path_off,
-                       B_, s_);
-    size_t off = r.injrows_off;
-    for (size_t k = 0; k < tb.inj.size(); ++k) {
-      std::vector<const uint32_t*> ip;
-      for (int b = 0; b < B_; ++b) ip.push_back(tb.inj_ptr(k, b));
-      int dsh = 0;
-      while ((tb.nleaves >> dsh) > tb.inj_width[k]) ++dsh;
-      launch_gather_digest(pos_d, nq, r.shift, dsh, stage(ip), gat.as<uint32_t>(), (int)stride,
-                           (int)(r.injl_off + 8 * k), B_, s_);
-      std::vector<const uint32_t*> gp;
-      for (int b = 0; b < B_; ++b) {
-        auto c = cols_of_group(ti.groups[tb.order[k + 1]], b);
-        gp.insert(gp.end(), c.begin(), c.end());
-      }
-      launch_gather_rows(pos_d, nq, r.inj_shift[k], stage(gp), r.inj_ncols[k], gat.as<uint32_t>(), (int)stride,
-                         (int)off, B_, s_);
-      off += 2 * r.inj_ncols[k];
-    }
-  }
-  for (int j = first; j < FRI_ROUNDS; ++j) {
+    r.depth = FRI_LOG_N - j - 1;
+    r.rows_off = seg(12);
+    r.path_off = seg(8 * r.depth);
     const Layer& lay = (j == first) ? *lvp[0] : layers[j];
     const TreeBuf& tb = (j == first) ? levels_tree : ftrees[j];
     std::vector<const uint32_t*> cp, tp;
@@ -1276,20 +1350,31 @@ void Job::open() {
       for (int l = 0; l < 6; ++l) cp.push_back(lay.ptr(b) + (size_t)l * lay.len);
       tp.push_back(tb.node_ptr(b));
     }
-    launch_gather_rows(pos_d, nq, j, stage(cp), 6, gat.as<uint32_t>(), (int)stride, (int)frec[j].rows_off, B_, s_);
-    launch_gather_path(pos_d, nq, j, stage(tp), tb.nleaves, gat.as<uint32_t>(), (int)stride, (int)frec[j].path_off,
-                       B_, s_);
+    add_op(0, j, 6, 0, 0, r.rows_off, cp);
+    add_op(1, j, r.depth, 0, tb.nleaves, r.path_off, tp);
   }
+  std::vector<size_t> lq_off(L);
   for (size_t l = 1; l < L; ++l) {
+    lq_off[l] = seg(12);
     const int jl = FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n;
     std::vector<const uint32_t*> cp;
     for (int b = 0; b < B_; ++b)
       for (int k = 0; k < 6; ++k) cp.push_back(lvp[l]->ptr(b) + (size_t)k * lvp[l]->len);
-    launch_gather_rows(pos_d, nq, jl, stage(cp), 6, gat.as<uint32_t>(), (int)stride, (int)lq_off[l], B_, s_);
+    add_op(0, jl, 6, 0, 0, lq_off[l], cp);
   }
-  std::vector<uint32_t> gh((size_t)B_ * nq * stride);
-  LG_HIP(hipMemcpyAsync(gh.data(), gat.as<void>(), gh.size() * 4, hipMemcpyDeviceToHost, s_));
+  stride = (stride + 3) & ~size_t(3);
+  DevBuf gat = dev((size_t)B_ * nq * stride);
+  {
+    const GatherOp* ops_d = stage(ops);
+    TimeIt tm(ctx_, s_, "gather", (double)B_ * nq * stride * 8, 0);
+    launch_gather(ops_d, (int)ops.size(), blocks, pos_d, nq, B_, gat.as<uint32_t>(), (int)stride, s_);
+  }
+  const size_t gwords = (size_t)B_ * nq * stride;
+  uint32_t* gh = ctx_.pinned(gwords * 4);
+  trace("gather kernels");
+  LG_HIP(hipMemcpyAsync(gh, gat.as<void>(), gwords * 4, hipMemcpyDeviceToHost, s_));
   sync();
+  trace("gather copy-back");
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
   parallel_for(B_, [&](int b) {
     std::vector<uint32_t>& w = out_[b].proof;
@@ -1324,7 +1409,7 @@ void Job::open() {
         }
       }
     }
-    const uint32_t* rec0 = gh.data() + (size_t)b * nq * stride;
+    const uint32_t* rec0 = gh + (size_t)b * nq * stride;
     // FRI proof
     w.insert(w.end(), levels_root[b].v, levels_root[b].v + 8);
     w.push_back((uint32_t)(L - 1));
@@ -1392,17 +1477,16 @@ void Job::open() {
         w.push_back((uint32_t)tb.inj.size());
         w.insert(w.end(), rec + r.injl_off, rec + r.injl_off + 8 * tb.inj.size());
         w.push_back((uint32_t)tb.inj.size());
-        size_t off = r.injrows_off;
         for (size_t k2 = 0; k2 < tb.inj.size(); ++k2) {
           const Group& g = ti.groups[tb.order[k2 + 1]];
-          rawrow(g, rec + off, r.inj_ncols[k2], 0);
-          rawrow(g, rec + off, r.inj_ncols[k2], 1);
-          off += 2 * r.inj_ncols[k2];
+          rawrow(g, rec + r.inj_rows_off[k2], r.inj_ncols[k2], 0);
+          rawrow(g, rec + r.inj_rows_off[k2], r.inj_ncols[k2], 1);
         }
       }
     }
     out_[b].transcript = ts_[b].log_words();
   });
+  trace("proof assembly");
   for (auto& x : W) keep_.push_back(std::move(x));
   for (auto& x : level) keep_.push_back(std::move(x.buf));
   for (auto& x : layers) keep_.push_back(std::move(x.buf));

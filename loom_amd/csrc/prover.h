@@ -114,6 +114,16 @@ struct Context {
   std::vector<std::unique_ptr<Staging>> staging_free;
   std::unique_ptr<Staging> take_staging();
   void give_staging(std::unique_ptr<Staging> s);
+  // Pinned host buffers reused across proves (one prove at a time per context).
+  uint32_t* pinned(size_t bytes);     // device -> host (query openings)
+  uint32_t* pinned_in(size_t bytes);  // host -> device (trace columns)
+  ~Context();
+
+ private:
+  uint32_t* pinned_ = nullptr;
+  size_t pinned_cap_ = 0;
+  uint32_t* pinned_in_ = nullptr;
+  size_t pinned_in_cap_ = 0;
 };
 
 // Per-instance caller data.
