@@ -41,6 +41,19 @@ struct NttCfg {
 void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
                 const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s);
 
+// Two-pass (four-step) transform for 2^15 <= n <= 2^22: out = post * DFT(pre * in)
+// (natural input; natural or bit-reversed output); tmp: n-word scratch per
+// job.  tw: w^k (k < n/2) of the direction, wn: w^k (k < n) of the direction.
+struct BigJob {
+  const uint32_t* in;
+  uint32_t* out;
+  uint32_t* tmp;
+  const uint32_t* pre;
+  const uint32_t* post;
+};
+void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw,
+                    const uint32_t* wn, int bitrev_out, hipStream_t s);
+
 // Leaf hashing: for instance b (grid y) and pair leaf k, hashes the message
 // [LEAF, hdr1, hdr2, ...] whose element m >= 3 is column msgmap[m] >> 1 at row
 // 2k + (msgmap[m] & 1); columns are colptrs[b * ncols + c].  Digest (8 words)
@@ -63,6 +76,9 @@ void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int 
 void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s);
 // Injection fold: nodes[w + j] = H(nodes[w + j], inj[b][j]).
 void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s);
+
+// Batched copy: for job k, dst[k][0 .. n) = src + off[k] (n words each).
+void launch_scatter(const uint32_t* src, const uint64_t* off, uint32_t* const* dst, int n, int n_jobs, hipStream_t s);
 
 // Column fill: out[b][j] = (j == pos) ? one : 0.
 void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s);

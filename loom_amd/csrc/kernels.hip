@@ -133,6 +133,126 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   LG_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ two-pass NTT
+// Size n = n1 * n2 (n1 = 2^a, n2 = 2^b), four-step:
+//   pass A: for each j1 < n1, a size-n2 DFT of x[j1 + n1 j2] (over j2), times
+//           w_n^{j1 k2}; written transposed to tmp[k2 * n1 + j1];
+//   pass B: for each k2 < n2, a size-n1 DFT of tmp[k2 * n1 + .] (over j1),
+//           giving X[k2 + n2 k1].
+// A workgroup handles C = 8 consecutive j1 (pass A) or k2 (pass B), so global
+// accesses come in runs of C words; the C transforms live in LDS.
+constexpr int BIG_C = 8;
+
+
+__device__ __forceinline__ void lds_dit_multi(uint32_t* sh, int logm, int C, const uint32_t* __restrict__ tw,
+                                              int tw_shift) {
+  // C independent transforms of size m = 2^logm stored at sh[c * m + .],
+  // bit-reversed input order, natural output; tw[k << tw_shift] = w_m^k.
+  const int m = 1 << logm;
+  const int T = blockDim.x;
+  const int half_total = C * (m >> 1);
+  for (int s = 0; s < logm; ++s) {
+    const int half = 1 << s;
+    const int tsh = logm - 1 - s + tw_shift;
+    for (int t = threadIdx.x; t < half_total; t += T) {
+      const int c = t >> (logm - 1);
+      const int u = t & ((m >> 1) - 1);
+      const int j = u & (half - 1);
+      const int i0 = c * m + (((u >> s) << (s + 1)) | j);
+      const int i1 = i0 + half;
+      const uint32_t w = tw[j << tsh];
+      const uint32_t x = sh[i0];
+      const uint32_t y = kb_mul(sh[i1], w);
+      sh[i0] = kb_add(x, y);
+      sh[i1] = kb_sub(x, y);
+    }
+    __syncthreads();
+  }
+}
+
+// tw: twiddles of the size-n transform (w_n^k, k < n/2), used with strides
+// for the sub-transforms; wn: w_n^k for k < n (pass A twiddle), direction
+// matching tw.
+__global__ __launch_bounds__(256) void k_ntt_pass_a(const BigJob* __restrict__ jobs, int a, int b,
+                                                    const uint32_t* __restrict__ tw, const uint32_t* __restrict__ wn) {
+  extern __shared__ uint32_t sh[];
+  const BigJob* job = jobs + blockIdx.y;
+  const int n1 = 1 << a, n2 = 1 << b;
+  const int j10 = blockIdx.x * BIG_C;
+  const int tot = BIG_C * n2;
+  const uint32_t* __restrict__ in = job->in;
+  const uint32_t* __restrict__ pre = job->pre;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx & (BIG_C - 1), j2 = idx >> 3;
+    const size_t gi = (size_t)(j10 + c) + (size_t)n1 * j2;
+    uint32_t v = in[gi];
+    if (pre) v = kb_mul(v, pre[gi]);
+    sh[c * n2 + brev(j2, b)] = v;
+  }
+  __syncthreads();
+  // size-n2 sub-transforms: w_{n2} = w_n^{n1}
+  lds_dit_multi(sh, b, BIG_C, tw, a);
+  const uint32_t nmask = (1u << (a + b)) - 1;
+  uint32_t* __restrict__ tmp = job->tmp;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx & (BIG_C - 1), k2 = idx >> 3;
+    const uint32_t j1 = j10 + c;
+    const uint32_t v = kb_mul(sh[c * n2 + k2], wn[(j1 * (uint32_t)k2) & nmask]);
+    tmp[(size_t)k2 * n1 + j1] = v;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_ntt_pass_b(const BigJob* __restrict__ jobs, int a, int b,
+                                                    const uint32_t* __restrict__ tw, int bitrev_out) {
+  extern __shared__ uint32_t sh[];
+  const BigJob* job = jobs + blockIdx.y;
+  const int n1 = 1 << a, n2 = 1 << b;
+  const int k20 = blockIdx.x * BIG_C;
+  const int tot = BIG_C * n1;
+  const uint32_t* __restrict__ tmp = job->tmp;
+  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+    const int c = idx >> a, j1 = idx & (n1 - 1);
+    sh[c * n1 + brev(j1, a)] = tmp[(size_t)(k20 + c) * n1 + j1];
+  }
+  __syncthreads();
+  // size-n1 sub-transforms: w_{n1} = w_n^{n2}
+  lds_dit_multi(sh, a, BIG_C, tw, b);
+  uint32_t* __restrict__ out = job->out;
+  const uint32_t* __restrict__ post = job->post;
+  if (bitrev_out) {
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+      const int c = idx >> a, r1 = idx & (n1 - 1);
+      const uint32_t k2 = k20 + c, k1 = brev(r1, a);
+      uint32_t v = sh[c * n1 + k1];
+      if (post) v = kb_mul(v, post[k2 + (size_t)n2 * k1]);
+      out[(size_t)n1 * brev(k2, b) + r1] = v;
+    }
+  } else {
+    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
+      const int c = idx & (BIG_C - 1), k1 = idx >> 3;
+      const uint32_t k2 = k20 + c;
+      uint32_t v = sh[c * n1 + k1];
+      const size_t o = k2 + (size_t)n2 * k1;
+      if (post) v = kb_mul(v, post[o]);
+      out[o] = v;
+    }
+  }
+}
+
+void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw,
+                    const uint32_t* wn, int bitrev_out, hipStream_t s) {
+  if (n_jobs == 0) return;
+  const int a = (logn + 1) / 2, b = logn - a;  // n1 = 2^a >= n2 = 2^b
+  if (a > 11 || b < 3) throw std::runtime_error("launch_ntt_big: unsupported size");
+  prelaunch();
+  hipLaunchKernelGGL(k_ntt_pass_a, dim3((1 << a) / BIG_C, n_jobs), dim3(256), (size_t)BIG_C * (1 << b) * 4, s, jobs_a,
+                     a, b, tw, wn);
+  LG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_ntt_pass_b, dim3((1 << b) / BIG_C, n_jobs), dim3(256), (size_t)BIG_C * (1 << a) * 4, s, jobs_b,
+                     a, b, tw, bitrev_out);
+  LG_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ hashing
 __global__ __launch_bounds__(256) void k_leaf_hash(const uint32_t* const* __restrict__ colptrs, int ncols,
                                                    const int32_t* __restrict__ msgmap, int msglen, uint32_t hdr1,
@@ -337,6 +457,23 @@ void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width
   const int T = width < 256 ? width : 256;
   prelaunch();
   hipLaunchKernelGGL(k_inject, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, inj, width);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ input scatter
+__global__ void k_scatter(const uint32_t* __restrict__ src, const uint64_t* __restrict__ off,
+                          uint32_t* const* __restrict__ dst, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int k = blockIdx.y;
+  dst[k][j] = src[off[k] + j];
+}
+
+void launch_scatter(const uint32_t* src, const uint64_t* off, uint32_t* const* dst, int n, int n_jobs, hipStream_t s) {
+  if (n_jobs == 0) return;
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_scatter, dim3((n + T - 1) / T, n_jobs), dim3(T), 0, s, src, off, dst, n);
   LG_HIP(hipGetLastError());
 }
 

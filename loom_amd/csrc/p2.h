@@ -11,6 +11,8 @@
 //   sponge: rate 16, overwrite mode, partial last block overwrites only its
 //   lanes, digest = state[0..8] (internal/hash/poseidon2.go:165-255).
 #pragma once
+#include <cstddef>
+
 #include "kb.h"
 #include "poseidon2_constants.h"
 

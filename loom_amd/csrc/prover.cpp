@@ -344,7 +344,7 @@ class Job {
       for (auto& x : t.inj) ctx_.pool.put(std::move(x));
     }
     for (auto& x : keep_) ctx_.pool.put(std::move(x));
-    for (auto& x : perm_bufs_) ctx_.pool.put(std::move(x));
+    ctx_.pool.put(std::move(in_arena_));
     if (stg_) ctx_.give_staging(std::move(stg_));
   }
 
@@ -403,7 +403,8 @@ class Job {
   std::unique_ptr<Staging> stg_;
 
   std::vector<ColBuf> cols_;
-  std::vector<DevBuf> perm_bufs_;   // [perm][b] flattened: perm_bufs_[p*B + b]
+  DevBuf in_arena_;                                 // uploaded trace columns + permutations
+  std::vector<std::vector<const uint32_t*>> perm_ptr_;  // [perm][b] into in_arena_
   std::vector<TreeBuf> trees_;
   std::vector<DevBuf> keep_;
   std::vector<Transcript> ts_;
@@ -462,18 +463,23 @@ void Job::upload_inputs() {
     if (k < nin) std::memcpy(h + col_off[(size_t)b * nin + k], in_[b].columns[k], cols_[d_.inputs[k]].n * 4);
     else std::memcpy(h + perm_off[(k - nin) * B_ + b], in_[b].perms[k - nin], d_.perms[k - nin].len * 4);
   });
+  in_arena_ = dev(words);
+  LG_HIP(hipMemcpyAsync(in_arena_.as<void>(), h, words * 4, hipMemcpyHostToDevice, s_));
+  // columns of one module size share a scatter launch
+  std::map<uint32_t, std::pair<std::vector<uint64_t>, std::vector<uint32_t*>>> by_n;
   for (int b = 0; b < B_; ++b)
     for (size_t k = 0; k < nin; ++k) {
       const ColBuf& cb = cols_[d_.inputs[k]];
-      LG_HIP(hipMemcpyAsync(cb.lag(b, 0), h + col_off[(size_t)b * nin + k], cb.n * 4, hipMemcpyHostToDevice, s_));
+      auto& e = by_n[cb.n];
+      e.first.push_back(col_off[(size_t)b * nin + k]);
+      e.second.push_back(cb.lag(b, 0));
     }
-  perm_bufs_.clear();
-  for (size_t p = 0; p < d_.perms.size(); ++p) {
-    DevBuf x = dev((size_t)d_.perms[p].len * B_);
-    LG_HIP(hipMemcpyAsync(x.as<void>(), h + perm_off[p * B_], (size_t)d_.perms[p].len * B_ * 4, hipMemcpyHostToDevice,
-                          s_));
-    perm_bufs_.push_back(std::move(x));
-  }
+  for (auto& kv : by_n)
+    launch_scatter(in_arena_.as<uint32_t>(), stage(kv.second.first), const_cast<uint32_t* const*>(stage(kv.second.second)),
+                   (int)kv.first, (int)kv.second.first.size(), s_);
+  perm_ptr_.assign(d_.perms.size(), std::vector<const uint32_t*>(B_));
+  for (size_t p = 0; p < d_.perms.size(); ++p)
+    for (int b = 0; b < B_; ++b) perm_ptr_[p][b] = in_arena_.as<uint32_t>() + perm_off[p * B_ + b];
 }
 
 void Job::gencol() {
@@ -493,7 +499,7 @@ void Job::gencol() {
     } else {
       const Perm& pm = d_.perms[ci.a1];
       std::vector<const uint32_t*> S(B_);
-      for (int b = 0; b < B_; ++b) S[b] = perm_bufs_[ci.a1].as<uint32_t>() + (size_t)b * pm.len;
+      for (int b = 0; b < B_; ++b) S[b] = perm_ptr_[ci.a1][b];
       launch_sigma(const_cast<uint32_t* const*>(outs_d), stage(S), tb_.wpow(logn), tb_.gpow_small(pm.len / cb.n),
                    ci.a0, cb.n, B_, s_);
     }
