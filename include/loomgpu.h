@@ -87,6 +87,14 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
              uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
              const size_t* transcript_caps, size_t* transcript_lens);
 
+/* lg_prove with trace inputs already in HBM: columns[] and perms[] are device
+ * pointers on the context's device (checked with hipPointerGetAttributes),
+ * read during the call only.  Same proofs as lg_prove; the caller that keeps
+ * its trace device-resident skips the host->device copy of the inputs. */
+int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                    uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
+                    uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens);
+
 /* Per-kernel device timing with HIP events on the context's stream (off by
  * default).  lg_ctx_timing_report writes one text line per kernel name,
  * "name launches device_ms algorithmic_bytes poseidon2_permutations", into

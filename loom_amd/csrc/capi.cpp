@@ -127,9 +127,9 @@ int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms) 
   });
 }
 
-int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
-             uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
-             const size_t* transcript_caps, size_t* transcript_lens) {
+static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                      bool device, uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
+                      uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens) {
   return guarded([&] {
     if (!plan || batch <= 0 || !proofs || !proof_caps || !proof_lens) throw std::runtime_error("lg_prove: bad arguments");
     const size_t nc = plan->p->num_inputs(), np = plan->p->num_perms();
@@ -144,6 +144,7 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
         if (!perms[b * np + k]) throw std::runtime_error("lg_prove: null permutation pointer");
         in[b].perms.push_back(perms[b * np + k]);
       }
+      in[b].device = device;
     }
     std::vector<lg::ProveOutput> out;
     {
@@ -171,6 +172,20 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
     if (rc.load()) g_err = "lg_prove: output buffer too small";
     return rc.load();
   });
+}
+
+int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+             uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
+             const size_t* transcript_caps, size_t* transcript_lens) {
+  return prove_impl(plan, batch, columns, perms, false, proofs, proof_caps, proof_lens, transcripts, transcript_caps,
+                    transcript_lens);
+}
+
+int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                    uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
+                    uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens) {
+  return prove_impl(plan, batch, columns, perms, true, proofs, proof_caps, proof_lens, transcripts, transcript_caps,
+                    transcript_lens);
 }
 
 int lg_ctx_set_timing(lg_ctx* ctx, int on) {

@@ -28,7 +28,10 @@ struct NttJob {
 // inverse_first is set and x = in otherwise; with n_out == 0 the inverse
 // result itself is written.  Orders: inputs natural; outputs natural or
 // bit-reversed per output.  Scale tables are n-entry Montgomery arrays.
+enum NttKind { NTT_KIND_LDE = 0, NTT_KIND_QUOT_COSET, NTT_KIND_QUOT_BACK, NTT_KIND_CHUNKS, NTT_KIND_DEEP_LDE };
+
 struct NttCfg {
+  int kind;  // NttKind: which phase launches it (kernel symbol only)
   int logn;
   int inverse_first;
   int n_out;
@@ -79,6 +82,8 @@ void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width
 
 // Batched copy: for job k, dst[k][0 .. n) = src + off[k] (n words each).
 void launch_scatter(const uint32_t* src, const uint64_t* off, uint32_t* const* dst, int n, int n_jobs, hipStream_t s);
+// dst[k][j] = src[k][j] for j < n, k < n_jobs (device-resident inputs).
+void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, int n_jobs, hipStream_t s);
 
 // Column fill: out[b][j] = (j == pos) ? one : 0.
 void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s);

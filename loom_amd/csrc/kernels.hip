@@ -53,6 +53,9 @@ struct NttArgs {
   int logn, inverse_first, n_out, bitrev_mask;
 };
 
+// KIND only names the phase (one kernel symbol per phase, so traces separate
+// them); the code is the same.
+template <int KIND>
 __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ jobs, NttArgs cfg,
                                                     const uint32_t* const* __restrict__ pres,
                                                     const uint32_t* __restrict__ tw_fwd,
@@ -128,8 +131,16 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   NttArgs a{cfg.logn, cfg.inverse_first, cfg.n_out, 0};
   for (int o = 0; o < cfg.n_out; ++o) a.bitrev_mask |= (cfg.bitrev_out[o] ? 1 : 0) << o;
   prelaunch();
-  hipLaunchKernelGGL(k_ntt_multi, dim3(n_jobs), dim3(T), n * sizeof(uint32_t), s, jobs_dev, a, pres_dev, tw_fwd,
-                     tw_inv);
+  const dim3 g(n_jobs), b(T);
+  const size_t lds = n * sizeof(uint32_t);
+  switch (cfg.kind) {
+    case NTT_KIND_LDE: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+    case NTT_KIND_QUOT_COSET: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_QUOT_COSET>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+    case NTT_KIND_QUOT_BACK: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_QUOT_BACK>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+    case NTT_KIND_CHUNKS: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_CHUNKS>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+    case NTT_KIND_DEEP_LDE: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_DEEP_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+    default: throw std::runtime_error("launch_ntt: unknown kind");
+  }
   LG_HIP(hipGetLastError());
 }
 
@@ -474,6 +485,21 @@ void launch_scatter(const uint32_t* src, const uint64_t* off, uint32_t* const* d
   const int T = n < 256 ? n : 256;
   prelaunch();
   hipLaunchKernelGGL(k_scatter, dim3((n + T - 1) / T, n_jobs), dim3(T), 0, s, src, off, dst, n);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_copy_cols(const uint32_t* const* __restrict__ src, uint32_t* const* __restrict__ dst, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int k = blockIdx.y;
+  dst[k][j] = src[k][j];
+}
+
+void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, int n_jobs, hipStream_t s) {
+  if (n_jobs == 0) return;
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_copy_cols, dim3((n + T - 1) / T, n_jobs), dim3(T), 0, s, src, dst, n);
   LG_HIP(hipGetLastError());
 }
 

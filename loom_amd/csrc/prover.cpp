@@ -442,6 +442,38 @@ void Job::upload_inputs() {
   }
   for (size_t k = 0; k < d_.inputs.size(); ++k)
     if (cols_[d_.inputs[k]].limbs != 1) throw std::runtime_error("extension-field input columns are not supported");
+  const size_t nin = d_.inputs.size();
+  if (in_[0].device) {
+    // Inputs already in HBM: columns are copied into their buffers by one
+    // launch per module size; permutations are read where they are.
+    for (int b = 0; b < B_; ++b) {
+      if (!in_[b].device) throw std::runtime_error("lg_prove: mixed host and device inputs");
+      for (size_t k = 0; k < nin + d_.perms.size(); ++k) {
+        const void* ptr = k < nin ? (const void*)in_[b].columns[k] : (const void*)in_[b].perms[k - nin];
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, ptr) != hipSuccess || at.type != hipMemoryTypeDevice ||
+            at.device != ctx_.device) {
+          (void)hipGetLastError();
+          throw std::runtime_error("lg_prove_device: input pointer is not device memory of the context's device");
+        }
+      }
+    }
+    std::map<uint32_t, std::pair<std::vector<const uint32_t*>, std::vector<uint32_t*>>> by_n;
+    for (int b = 0; b < B_; ++b)
+      for (size_t k = 0; k < nin; ++k) {
+        const ColBuf& cb = cols_[d_.inputs[k]];
+        auto& e = by_n[cb.n];
+        e.first.push_back(in_[b].columns[k]);
+        e.second.push_back(cb.lag(b, 0));
+      }
+    for (auto& kv : by_n)
+      launch_copy_cols(stage(kv.second.first), const_cast<uint32_t* const*>(stage(kv.second.second)), (int)kv.first,
+                       (int)kv.second.first.size(), s_);
+    perm_ptr_.assign(d_.perms.size(), std::vector<const uint32_t*>(B_));
+    for (size_t p = 0; p < d_.perms.size(); ++p)
+      for (int b = 0; b < B_; ++b) perm_ptr_[p][b] = in_[b].perms[p];
+    return;
+  }
   // host layout: [b][input k] then [perm p][b]
   size_t words = 0;
   std::vector<size_t> col_off((size_t)B_ * d_.inputs.size()), perm_off(d_.perms.size() * B_);
@@ -456,7 +488,6 @@ void Job::upload_inputs() {
       words += d_.perms[p].len;
     }
   uint32_t* h = ctx_.pinned_in(words * 4);
-  const size_t nin = d_.inputs.size();
   parallel_for((int)(B_ * (nin + d_.perms.size())), [&](int i) {
     const int b = i % B_;
     const size_t k = (size_t)i / B_;
@@ -524,6 +555,7 @@ void Job::lde_columns(const std::vector<uint32_t>& col_ids) {
   }
   for (auto& kv : by_log) {
     NttCfg cfg{};
+    cfg.kind = NTT_KIND_LDE;
     cfg.logn = kv.first;
     cfg.inverse_first = 1;
     cfg.n_out = 4;
@@ -766,6 +798,7 @@ void Job::quotient(size_t mi) {
       }
   }
   NttCfg cfg{};
+  cfg.kind = NTT_KIND_QUOT_COSET;
   cfg.logn = logn;
   cfg.inverse_first = 1;
   cfg.n_out = rho;
@@ -821,6 +854,7 @@ void Job::quotient(size_t mi) {
         jobs.push_back(j);
       }
   NttCfg c2{};
+  c2.kind = NTT_KIND_QUOT_BACK;
   c2.logn = logn;
   c2.inverse_first = 1;
   c2.n_out = 0;
@@ -856,6 +890,7 @@ void Job::quotient(size_t mi) {
     cb.lde_ready = true;
   }
   NttCfg c3{};
+  c3.kind = NTT_KIND_CHUNKS;
   c3.logn = logn;
   c3.inverse_first = 0;
   c3.n_out = 5;
@@ -1123,6 +1158,7 @@ void Job::open() {
         jobs.push_back(j);
       }
     NttCfg cfg{};
+    cfg.kind = NTT_KIND_DEEP_LDE;
     cfg.logn = m.log_n;
     cfg.inverse_first = 1;
     cfg.n_out = 4;
@@ -1132,7 +1168,7 @@ void Job::open() {
     }
     {
       const NttJob* jd = stage(jobs);
-      TimeIt tm(ctx_, s_, "ntt_lde", jobs.size() * 5.0 * m.N * 4, 0);
+      TimeIt tm(ctx_, s_, "ntt_deep_lde", jobs.size() * 5.0 * m.N * 4, 0);
       launch_ntt(jd, (int)jobs.size(), cfg, pres(cfg), tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
     }
     keep_.push_back(std::move(dq.buf));

@@ -128,8 +128,9 @@ struct Context {
 
 // Per-instance caller data.
 struct InstanceInput {
-  std::vector<const uint32_t*> columns;  // plan input order, Montgomery, N_module words each (host)
-  std::vector<const uint32_t*> perms;    // plan permutation order, len words each (host)
+  std::vector<const uint32_t*> columns;  // plan input order, Montgomery, N_module words each
+  std::vector<const uint32_t*> perms;    // plan permutation order, len words each
+  bool device = false;                   // pointers are device memory of the context's device (else host)
 };
 
 struct ProveOutput {

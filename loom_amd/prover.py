@@ -98,25 +98,56 @@ class Prepared:
 _proof_cap = {}
 
 
-def prove_prepared(ctx, items, want_transcript=False, phases=None):
-    """Proves prepared instances sharing one plan shape in one batched call."""
-    if not items:
-        return []
-    words = items[0].words
-    for it in items[1:]:
-        if it.words.shape != words.shape or not np.array_equal(it.words, words):
-            raise ValueError("batched instances must share one plan (same program shape)")
-    ph = ctx.plan_handle(words)
+class DeviceInputs:
+    """Prepared instances with their trace columns and permutations copied
+    into device memory (torch tensors on `device`), for lg_prove_device."""
+
+    def __init__(self, items, device=0):
+        import torch
+        dev = torch.device("cuda", device)
+        self.items = items
+        self.columns = [[torch.from_numpy(c.view(np.int32)).to(dev) for c in it.columns] for it in items]
+        self.perms = [[torch.from_numpy(np.ascontiguousarray(s).view(np.int32)).to(dev) for s in it.perms]
+                      for it in items]
+        torch.cuda.synchronize(dev)
+
+
+def _pointer_tables(items, dev_inputs):
     B = len(items)
     nc = len(items[0].columns)
     npm = len(items[0].perms)
     col_ptrs = (_lib.u32p * max(1, B * nc))()
     perm_ptrs = (_lib.u32p * max(1, B * npm))()
     for b, it in enumerate(items):
-        for k, c in enumerate(it.columns):
-            col_ptrs[b * nc + k] = c.ctypes.data_as(_lib.u32p)
-        for k, s in enumerate(it.perms):
-            perm_ptrs[b * npm + k] = s.ctypes.data_as(_lib.u32p)
+        for k in range(nc):
+            if dev_inputs is None:
+                col_ptrs[b * nc + k] = it.columns[k].ctypes.data_as(_lib.u32p)
+            else:
+                col_ptrs[b * nc + k] = ctypes.cast(dev_inputs.columns[b][k].data_ptr(), _lib.u32p)
+        for k in range(npm):
+            if dev_inputs is None:
+                perm_ptrs[b * npm + k] = it.perms[k].ctypes.data_as(_lib.u32p)
+            else:
+                perm_ptrs[b * npm + k] = ctypes.cast(dev_inputs.perms[b][k].data_ptr(), _lib.u32p)
+    return col_ptrs, perm_ptrs
+
+
+def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None):
+    """Proves prepared instances sharing one plan shape in one batched call.
+    With `dev_inputs` (a DeviceInputs of these items) the inputs are read
+    from device memory (lg_prove_device), else from host memory (lg_prove)."""
+    if not items:
+        return []
+    words = items[0].words
+    for it in items[1:]:
+        if it.words.shape != words.shape or not np.array_equal(it.words, words):
+            raise ValueError("batched instances must share one plan (same program shape)")
+    if dev_inputs is not None and len(dev_inputs.items) != len(items):
+        raise ValueError("dev_inputs does not match items")
+    ph = ctx.plan_handle(words)
+    B = len(items)
+    col_ptrs, perm_ptrs = _pointer_tables(items, dev_inputs)
+    entry = _lib.lib().lg_prove if dev_inputs is None else _lib.lib().lg_prove_device
     cap = _proof_cap.get(words.tobytes(), 1 << 21)
     while True:
         outs = [np.empty(cap, dtype=np.uint32) for _ in range(B)]
@@ -130,7 +161,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None):
             tlens = (ctypes.c_size_t * B)()
         else:
             t_ptrs, tcaps, tlens = None, None, None
-        rc = _lib.lib().lg_prove(ph, B, col_ptrs, perm_ptrs, out_ptrs, caps, lens, t_ptrs, tcaps, tlens)
+        rc = entry(ph, B, col_ptrs, perm_ptrs, out_ptrs, caps, lens, t_ptrs, tcaps, tlens)
         if rc == -2 and max(lens) > cap:
             cap = max(lens)
             continue

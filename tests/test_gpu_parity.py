@@ -184,3 +184,36 @@ def test_batched_equals_alone(ctx):
         assert np.array_equal(w, alone)
     enc, _ = _oracle_words(inst[2][0], inst[2][1], 20)
     assert np.array_equal(batch[2], enc)
+
+
+def test_device_inputs_equal_host_inputs(ctx):
+    """lg_prove_device (trace already in HBM) gives the proofs lg_prove gives
+    from host buffers, and a host pointer handed to it is refused."""
+    from loom_amd import workloads, prover, _lib
+    log("device inputs vs host inputs, 3 x 2^9")
+    inst = [workloads.plonk_instance(9, 300 + s) for s in range(3)]
+    items = [prover.Prepared(p, t, 24) for p, t in inst]
+    dv = prover.DeviceInputs(items, 0)
+    a = prover.prove_prepared(ctx, items, dev_inputs=dv)
+    b = prover.prove_prepared(ctx, items)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
+    enc, _ = _oracle_words(inst[1][0], inst[1][1], 24)
+    assert np.array_equal(a[1], enc)
+
+    class HostAsDevice:
+        def __init__(self, it):
+            self.items = [it]
+            self.columns = [[_HostPtr(c) for c in it.columns]]
+            self.perms = [[_HostPtr(s) for s in it.perms]]
+
+    with pytest.raises(_lib.LoomGpuError, match="not device memory"):
+        prover.prove_prepared(ctx, items[:1], dev_inputs=HostAsDevice(items[0]))
+
+
+class _HostPtr:
+    def __init__(self, arr):
+        self.arr = arr
+
+    def data_ptr(self):
+        return self.arr.ctypes.data
