@@ -313,7 +313,8 @@ int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint
 
 int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t* out) {
   return guarded([&] {
-    if (!ctx || !in || !out || n < 2 || (n & (n - 1)) || n_cols <= 0) throw std::runtime_error("lg_rs_encode: bad arguments");
+    if (!ctx || !in || !out || n < 2 || (n & (n - 1)) || n > ((size_t)1 << 22) || n_cols <= 0)
+      throw std::runtime_error("lg_rs_encode: bad arguments");
     int logn = 0;
     while (((size_t)1 << logn) < n) ++logn;
     std::lock_guard<std::mutex> lock(ctx->mu);
@@ -332,10 +333,17 @@ int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t
       cfg.pre[o] = ctx->c.tables.lde_pre(logn, o, 2, true);
       cfg.bitrev_out[o] = 1;
     }
-    lg::DevBuf jd = stage_sync(jobs.data(), jobs.size() * sizeof(lg::NttJob));
-    lg::DevBuf pd = stage_sync(cfg.pre, sizeof(cfg.pre));
-    lg::launch_ntt(jd.as<lg::NttJob>(), n_cols, cfg, pd.as<const uint32_t*>(), ctx->c.tables.tw_fwd(logn),
-                   ctx->c.tables.tw_inv(logn), ctx->c.stream);
+    std::vector<lg::DevBuf> held;
+    lg::run_ntt(
+        ctx->c.tables, cfg, jobs, ctx->c.stream,
+        [&](const void* p, size_t bytes) -> const void* {
+          held.push_back(stage_sync(p, bytes));
+          return held.back().as<void>();
+        },
+        [&](size_t words) {
+          held.emplace_back(words * 4);
+          return held.back().as<uint32_t>();
+        });
     LG_HIP(hipStreamSynchronize(ctx->c.stream));
     return 0;
   });

@@ -87,6 +87,10 @@ const uint32_t* Tables::wpow(int logn) {
   return get("wp" + std::to_string(logn), [&] { return powers(omega(logn), 1u << logn); });
 }
 
+const uint32_t* Tables::wpow_inv(int logn) {
+  return get("wpi" + std::to_string(logn), [&] { return powers(kb_inv(omega(logn)), 1u << logn); });
+}
+
 const uint32_t* Tables::lde_pre(int logn, int b, int rate_log, bool from_values) {
   return get("lde" + std::to_string(logn) + "_" + std::to_string(b) + "_" + std::to_string(from_values), [&] {
     uint32_t s = kb_pow(omega(logn + rate_log), brev_host(b, rate_log));
@@ -270,7 +274,8 @@ void Context::give_staging(std::unique_ptr<Staging> s) {
 // ------------------------------------------------------------------ plan
 Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_plan(words, n)) {
   for (auto& m : d_.modules) {
-    if (m.log_n > NTT_MAX_LOG_LDS) throw std::runtime_error("module size above 2^14 is not supported by this build");
+    if (m.log_n > FRI_LOG_D) throw std::runtime_error("module size above 2^22 (MAX_LOG_MODULE_SIZE)");
+    if (m.log_n < 1) throw std::runtime_error("module size below 2");
     if (m.n_chunks > 8) throw std::runtime_error("constraint degree above 9 is not supported");
   }
   for (auto& pm : d_.perms)
@@ -384,6 +389,9 @@ class Job {
   void gencol();
   void run_steps(size_t r);
   void lde_columns(const std::vector<uint32_t>& col_ids);
+  // Runs the transforms cfg describes on every job: one LDS-resident launch
+  // up to 2^14, the two-pass four-step form above.
+  void ntt(const NttCfg& cfg, const std::vector<NttJob>& jobs);
   void commit_tree(size_t t);
   std::vector<Digest> roots_of(TreeBuf& tb);
   void build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>>& leaf_cols_per_group,
@@ -537,6 +545,62 @@ void Job::gencol() {
   }
 }
 
+}  // namespace
+
+void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hipStream_t s, const StageFn& stage,
+             const ScratchFn& scratch) {
+  const int logn = cfg.logn;
+  if (jobs.empty()) return;
+  if (logn <= NTT_MAX_LOG_LDS) {
+    const NttJob* jd = static_cast<const NttJob*>(stage(jobs.data(), jobs.size() * sizeof(NttJob)));
+    const uint32_t* const* pd = static_cast<const uint32_t* const*>(stage(cfg.pre, sizeof(cfg.pre)));
+    launch_ntt(jd, (int)jobs.size(), cfg, pd, tb.tw_fwd(logn), tb.tw_inv(logn), s);
+    return;
+  }
+  // Four-step form: every transform is two launches over a group of jobs;
+  // per job an n-word transpose buffer, plus an n-word buffer for the
+  // inverse result when forward outputs follow.  Groups bound the scratch.
+  const size_t n = (size_t)1 << logn;
+  const bool need_x = cfg.inverse_first && cfg.n_out > 0;
+  const size_t per_job = n * (need_x ? 2 : 1);
+  const size_t max_words = (size_t)256 << 20;
+  const size_t G = std::min(jobs.size(), std::max<size_t>(1, max_words / per_job));
+  uint32_t* sp = scratch(G * per_job);
+  for (size_t j0 = 0; j0 < jobs.size(); j0 += G) {
+    const size_t g = std::min(G, jobs.size() - j0);
+    std::vector<BigJob> bj(g);
+    auto tmp = [&](size_t k) { return sp + k * per_job; };
+    auto xbuf = [&](size_t k) { return sp + k * per_job + n; };
+    if (cfg.inverse_first) {
+      for (size_t k = 0; k < g; ++k) {
+        const NttJob& J = jobs[j0 + k];
+        bj[k] = BigJob{J.in, need_x ? xbuf(k) : J.out[0], tmp(k), nullptr, J.post};
+      }
+      const BigJob* d = static_cast<const BigJob*>(stage(bj.data(), g * sizeof(BigJob)));
+      launch_ntt_big(d, d, (int)g, logn, tb.tw_inv(logn), tb.wpow_inv(logn), 0, s);
+    }
+    for (int o = 0; o < cfg.n_out; ++o) {
+      for (size_t k = 0; k < g; ++k) {
+        const NttJob& J = jobs[j0 + k];
+        bj[k] = BigJob{cfg.inverse_first ? xbuf(k) : J.in, J.out[o], tmp(k), cfg.pre[o], nullptr};
+      }
+      const BigJob* d = static_cast<const BigJob*>(stage(bj.data(), g * sizeof(BigJob)));
+      launch_ntt_big(d, d, (int)g, logn, tb.tw_fwd(logn), tb.wpow(logn), cfg.bitrev_out[o], s);
+    }
+  }
+}
+
+namespace {
+
+void Job::ntt(const NttCfg& cfg, const std::vector<NttJob>& jobs) {
+  run_ntt(
+      tb_, cfg, jobs, s_, [&](const void* p, size_t bytes) -> const void* { return stg_->put(p, bytes, s_); },
+      [&](size_t words) {
+        keep_.push_back(dev(words));
+        return keep_.back().as<uint32_t>();
+      });
+}
+
 void Job::lde_columns(const std::vector<uint32_t>& col_ids) {
   std::map<int, std::vector<NttJob>> by_log;
   for (uint32_t c : col_ids) {
@@ -565,7 +629,7 @@ void Job::lde_columns(const std::vector<uint32_t>& col_ids) {
     }
     const double n = double(1u << kv.first);
     TimeIt t(ctx_, s_, "ntt_lde", kv.second.size() * 5.0 * n * 4, 0);
-    launch_ntt(stage(kv.second), (int)kv.second.size(), cfg, pres(cfg), tb_.tw_fwd(kv.first), tb_.tw_inv(kv.first), s_);
+    ntt(cfg, kv.second);
   }
 }
 
@@ -804,9 +868,8 @@ void Job::quotient(size_t mi) {
   cfg.n_out = rho;
   for (int i = 0; i < rho; ++i) cfg.pre[i] = tb_.quot_pre(logn, i, rho_log);
   {
-    const NttJob* jd = stage(jobs);
     TimeIt tm(ctx_, s_, "ntt_quot_coset", jobs.size() * (1.0 + rho) * n * 4, 0);
-    launch_ntt(jd, (int)jobs.size(), cfg, pres(cfg), tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+    ntt(cfg, jobs);
   }
   // 2. numerator / Z_H on the cosets
   DevBuf Q = dev((size_t)B_ * 6 * rn);
@@ -859,9 +922,8 @@ void Job::quotient(size_t mi) {
   c2.inverse_first = 1;
   c2.n_out = 0;
   {
-    const NttJob* jd = stage(jobs);
     TimeIt tm(ctx_, s_, "ntt_quot_back", jobs.size() * 2.0 * n * 4, 0);
-    launch_ntt(jd, (int)jobs.size(), c2, pres(c2), tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+    ntt(c2, jobs);
   }
   // 4. rho-point combination -> chunk coefficients [b][t][l][a]
   DevBuf coef = dev((size_t)B_ * rho * 6 * n);
@@ -901,9 +963,8 @@ void Job::quotient(size_t mi) {
   c3.pre[4] = nullptr;
   c3.bitrev_out[4] = 0;
   {
-    const NttJob* jd = stage(jobs);
     TimeIt tm(ctx_, s_, "ntt_chunks", jobs.size() * 6.0 * n * 4, 0);
-    launch_ntt(jd, (int)jobs.size(), c3, pres(c3), tb_.tw_fwd(logn), tb_.tw_inv(logn), s_);
+    ntt(c3, jobs);
   }
   keep_.push_back(std::move(Q));
   keep_.push_back(std::move(Y));
@@ -1167,9 +1228,8 @@ void Job::open() {
       cfg.bitrev_out[o] = 1;
     }
     {
-      const NttJob* jd = stage(jobs);
       TimeIt tm(ctx_, s_, "ntt_deep_lde", jobs.size() * 5.0 * m.N * 4, 0);
-      launch_ntt(jd, (int)jobs.size(), cfg, pres(cfg), tb_.tw_fwd(m.log_n), tb_.tw_inv(m.log_n), s_);
+      ntt(cfg, jobs);
     }
     keep_.push_back(std::move(dq.buf));
   }

@@ -14,6 +14,7 @@
 #include "dagjit.h"
 #include "gpu.h"
 #include "plan.h"
+#include "kernels.h"
 
 namespace lg {
 
@@ -23,6 +24,7 @@ class Tables {
   const uint32_t* tw_fwd(int logn);
   const uint32_t* tw_inv(int logn);
   const uint32_t* wpow(int logn);                 // w_n^j, j < n
+  const uint32_t* wpow_inv(int logn);             // w_n^-j, j < n
   const uint32_t* lde_pre(int logn, int b, int rate_log, bool from_values);
   const uint32_t* quot_pre(int logn, int i, int rho_log);   // (g w_{rho n}^i)^k / n
   const uint32_t* quot_post(int logn, int i, int rho_log);  // w_{rho n}^{-a i} / (rho n)
@@ -35,6 +37,15 @@ class Tables {
   std::mutex mu_;
   std::map<std::string, DevBuf> bufs_;
 };
+
+// Transforms of NttCfg semantics (kernels.h) on every job, at any size up to
+// 2^22: one LDS-resident launch up to 2^14, else the four-step two-pass form
+// with scratch(words) device scratch.  stage(p, bytes) returns a device copy
+// of a small host table valid for the launches that follow on s.
+using StageFn = std::function<const void*(const void*, size_t)>;
+using ScratchFn = std::function<uint32_t*(size_t)>;
+void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hipStream_t s, const StageFn& stage,
+             const ScratchFn& scratch);
 
 // Reusable device buffers (by exact size) and pinned staging areas, so that
 // repeated proofs of one shape allocate nothing after the first.

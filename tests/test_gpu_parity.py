@@ -63,7 +63,7 @@ def canon(a):
     return F.from_mont(np.asarray(a, dtype=np.uint32))
 
 
-@pytest.mark.parametrize("logn", [1, 4, 10, 14])
+@pytest.mark.parametrize("logn", [1, 4, 10, 14, 15, 16, 17])
 def test_rs_encode(torch, ctx, logn):
     from loom_amd import _lib
     from oracle import ntt
@@ -77,6 +77,36 @@ def test_rs_encode(torch, ctx, logn):
     got = canon(host_u32(dout)).reshape(3, 4 * n)
     want = ntt.encode(cols)
     assert np.array_equal(got, want)
+
+
+def test_rs_encode_max_size(torch, ctx):
+    """N = 2^22 (the largest module): the first n rows of the bit-reversed LDE
+    are the Lagrange values in bit-reversed order (w_{4n}^{4 bitrev_n(r)} =
+    w_n^{bitrev_n(r)}), and sampled rows equal the barycentric value
+    P(x) = (x^n - 1)/n * sum_i v_i w^i / (x - w^i) at x = w_{4n}^{bitrev(r)}."""
+    from loom_amd import _lib
+    from oracle import field as F
+    logn = 22
+    n = 1 << logn
+    log("rs_encode logn=22 (properties)")
+    rng = np.random.default_rng(22)
+    col = rng.integers(0, P, size=n, dtype=np.uint64)
+    din = dev_u32(torch, mont(col))
+    dout = torch.zeros(4 * n, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().lg_rs_encode(ctx.h, ptr(din), n, 1, ptr(dout)), "lg_rs_encode")
+    got = canon(host_u32(dout))
+    br = F.bitrev_indices(n)
+    assert np.array_equal(got[:n], col[br])
+    wi = F.powers(F.omega(n), n)
+    w4 = F.omega(4 * n)
+    for r in [n + 5, 2 * n + 12345, 3 * n + 999_999, 4 * n - 1]:
+        rb = int(format(r, "0%db" % (logn + 2))[::-1], 2)
+        x = pow(w4, rb, P)
+        den = (np.uint64(x) + np.uint64(P) - wi) % np.uint64(P)
+        terms = col * wi % np.uint64(P) * F.vinv(den) % np.uint64(P)
+        val = int(np.sum(terms, dtype=np.uint64) % np.uint64(P))  # < 2^22 terms below 2^31: no overflow
+        val = val * (pow(x, n, P) - 1) % P * pow(n, P - 2, P) % P
+        assert int(got[r]) == val, r
 
 
 @pytest.mark.parametrize("nb,ne,logrows", [(9, 0, 3), (5, 1, 6), (0, 4, 8), (1, 0, 12), (0, 1, 15)])
@@ -158,6 +188,18 @@ def test_prove_plonk_small(ctx, logn, nq):
     words, trans = prover.prove(ctx, prog, tr, nq, want_transcript=True)
     assert np.array_equal(trans, tw), "transcript digests differ"
     assert words.shape == enc.shape
+    assert np.array_equal(words, enc)
+
+
+@pytest.mark.parametrize("logn,nq", [(15, 30), (16, 40)])
+def test_prove_plonk_large(ctx, logn, nq):
+    """Modules above 2^14 take the four-step NTT path."""
+    from loom_amd import workloads, prover
+    log("prove plonk logn=%d nq=%d" % (logn, nq))
+    prog, tr = workloads.plonk_instance(logn, 7 + logn)
+    enc, tw = _oracle_words(prog, tr, nq)
+    words, trans = prover.prove(ctx, prog, tr, nq, want_transcript=True)
+    assert np.array_equal(trans, tw), "transcript digests differ"
     assert np.array_equal(words, enc)
 
 
