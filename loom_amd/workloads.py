@@ -30,26 +30,27 @@ def _names(i):
                 QM=s("QM"), QO=s("QO"), QK=s("QK"), perm=s("plonk.S"))
 
 
-def plonk_program(log_n, idx=0):
-    """The compiled program of one plonk-like module (same for every seed)."""
-    n = 1 << log_n
-    nm = _names(idx)
-    b = B.Builder()
-    m = B.Module(nm["module"], n)
+def _plonk_gate(m, nm):
     qll = E.Col(nm["QL"]) * E.Col(nm["L"])
     qrr = E.Col(nm["QR"]) * E.Col(nm["R"])
     qmlr = E.Col(nm["QM"]) * E.Col(nm["L"]) * E.Col(nm["R"])
     qoo = E.Col(nm["QO"]) * E.Col(nm["O"])
     qk = E.Col(nm["QK"])
     m.assert_zero(qll + qrr + qmlr + qoo + qk)
+
+
+def plonk_program(log_n, idx=0):
+    """The builder of one plonk-like module (same for every seed)."""
+    nm = _names(idx)
+    b = B.Builder()
+    m = B.Module(nm["module"], 1 << log_n)
+    _plonk_gate(m, nm)
     b.add_module(m)
     return b, nm
 
 
-def plonk_instance(log_n, seed, idx=0):
-    """Returns (program, trace) with trace: name -> canonical uint64 array."""
-    n = 1 << log_n
-    rng = np.random.Generator(np.random.PCG64(seed))
+def _plonk_values(n, rng):
+    """sigma, wires and selectors of one module, drawn from rng."""
     cells = 3 * n
     sigma = np.arange(cells, dtype=np.int64)
     tied = rng.permutation(cells)[: cells // 10]
@@ -77,11 +78,42 @@ def plonk_instance(log_n, seed, idx=0):
     pp = np.uint64(P)
     acc = (ql * L % pp + qr * R % pp + qm * (L * R % pp) % pp + qo * O % pp) % pp
     qk = (pp - acc) % pp
+    return sigma, dict(L=L.copy(), R=R.copy(), O=O.copy(), QL=ql, QR=qr, QM=qm, QO=qo, QK=qk)
 
-    b, nm = plonk_program(log_n, idx)
+
+def _add_plonk_module(b, log_n, idx, rng):
+    n = 1 << log_n
+    sigma, vals = _plonk_values(n, rng)
+    nm = _names(idx)
+    m = B.Module(nm["module"], n)
+    _plonk_gate(m, nm)
+    b.add_module(m)
     S = B.PermutationGen(sigma, nm["perm"])
     B.copy_constraint(b, nm["module"], [E.Col(nm["L"]), E.Col(nm["R"]), E.Col(nm["O"])], S)
-    prog = B.compile_program(b)
-    trace = {nm["L"]: L.copy(), nm["R"]: R.copy(), nm["O"]: O.copy(), nm["QL"]: ql, nm["QR"]: qr,
-             nm["QM"]: qm, nm["QO"]: qo, nm["QK"]: qk}
-    return prog, trace
+    return {nm[k]: v for k, v in vals.items()}
+
+
+def plonk_instance(log_n, seed, idx=0):
+    """Returns (program, trace) with trace: name -> canonical uint64 array."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = B.Builder()
+    trace = _add_plonk_module(b, log_n, idx, rng)
+    return B.compile_program(b), trace
+
+
+def uneven_instance(log_ns, seed):
+    """One program with len(log_ns) plonk-like modules plonk_0, plonk_1, ...
+    of sizes 2^log_ns[i] (the uneven-batch shape: modules of different sizes
+    proved together); module i draws from PCG64([seed, i])."""
+    b = B.Builder()
+    trace = {}
+    for i, ln in enumerate(log_ns):
+        rng = np.random.Generator(np.random.PCG64([seed, i]))
+        trace.update(_add_plonk_module(b, ln, i, rng))
+    return B.compile_program(b), trace
+
+
+def uneven_sizes(count, seed, lo=10, hi=20):
+    """log2 sizes drawn uniformly from [lo, hi] (PCG64([seed, 1 << 20]))."""
+    rng = np.random.Generator(np.random.PCG64([seed, 1 << 20]))
+    return [int(x) for x in rng.integers(lo, hi + 1, size=count)]

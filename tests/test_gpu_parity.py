@@ -203,6 +203,31 @@ def test_prove_plonk_large(ctx, logn, nq):
     assert np.array_equal(words, enc)
 
 
+@pytest.mark.parametrize("log_ns,nq", [((4, 6, 5, 6, 3), 12), ((10, 12, 8, 14), 30), ((15, 9, 12, 12), 20)])
+def test_prove_uneven_modules(ctx, log_ns, nq):
+    """One program with modules of different sizes: groups become Merkle
+    level injections (equal sizes fold into the leaves), one DEEP class per
+    module, FRI levels entering at different rounds."""
+    from loom_amd import workloads, prover
+    log("prove uneven modules %s nq=%d" % (log_ns, nq))
+    prog, tr = workloads.uneven_instance(list(log_ns), 5)
+    enc, tw = _oracle_words(prog, tr, nq)
+    words, trans = prover.prove(ctx, prog, tr, nq, want_transcript=True)
+    assert np.array_equal(trans, tw), "transcript digests differ"
+    assert np.array_equal(words, enc)
+
+
+def test_prove_uneven_batched(ctx):
+    """Three instances of one uneven shape in one batched call."""
+    from loom_amd import workloads, prover
+    log("prove uneven batched 3 x (7, 9, 6)")
+    inst = [workloads.uneven_instance([7, 9, 6], 40 + s) for s in range(3)]
+    batch = prover.prove_batch(ctx, inst, 16)
+    for (prog, tr), w in zip(inst, batch):
+        enc, _ = _oracle_words(prog, tr, 16)
+        assert np.array_equal(w, enc)
+
+
 def test_prove_plonk_full_size(ctx):
     """Config-0 shape: N = 2^14, 200 queries."""
     from loom_amd import workloads, prover
