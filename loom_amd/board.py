@@ -9,7 +9,10 @@ Mirrors:
 * ``compile_program`` -- board/program.go:45-136 (fold each module's relations
   with Coin(last round), field inference);
 * ``copy_constraint`` -- arguments/permutation.go:37-127 (tuple fold with
-  Coin(0), grand product with Coin(1) at round 2).
+  Coin(0), grand product with Coin(1) at round 2);
+* ``range_check`` / ``lookup`` -- arguments/lookup.go:34-236 (range module,
+  multiplicity step at round 0, cyclic logup columns at round 2 with their
+  exposed totals, logup bus), board/logup.go:30-186.
 
 The compiled ``Program`` is what ``protocol.build_layout`` and the plan
 serialiser consume.
@@ -70,6 +73,32 @@ class RangeColumnGen:
 
 # ------------------------------------------------------------------ steps
 GRAND_PRODUCT = 1
+LOOKUP_MULTIPLICITY = 2
+CYCLIC_LOGUP = 3
+
+# board/logup.go:37, arguments/lookup.go:152
+MAX_LOGUP_DEGREE = 5
+MAX_LOGUP_BATCH = 4
+
+
+def range_module_name(bound):
+    """constants.RangeModuleName (internal/constants/const.go:43-45)."""
+    return "range_%d" % bound
+
+
+def range_col_name(bound):
+    """constants.RangeColName (const.go:47-49)."""
+    return "range.bound_%d" % bound
+
+
+def chunk_name(name, i):
+    """constants.LogupChunkName / MultiplicityChunkName (const.go:51-57)."""
+    return "%s_%d" % (name, i)
+
+
+def logup_total_name(output):
+    """board.LogupTotalName (board/logup.go:64-66)."""
+    return output + "_total"
 
 
 @dataclass
@@ -78,6 +107,31 @@ class ProverStep:
     ins: list
     outs: list
     module: str = ""
+    # LOOKUP_MULTIPLICITY: dict(n_sources, n_targets, width, has_sel_s,
+    # has_sel_t, source_modules); CYCLIC_LOGUP: dict(n_terms)
+    ctx: dict = field(default_factory=dict)
+
+    def exposed_outputs(self):
+        """ProverStep.ExposedOutputs (board/logup.go:189-199)."""
+        if self.kind == CYCLIC_LOGUP:
+            return self.outs[1:2]
+        return []
+
+
+@dataclass
+class LogupTerm:
+    module: str
+    E: object
+    M: object
+
+
+def logup_constraint_degree(terms):
+    """board.LogupConstraintDegree (board/logup.go:46-56)."""
+    sum_e = sum(E.degree(t.E) for t in terms)
+    deg = 1 + sum_e
+    for t in terms:
+        deg = max(deg, E.degree(t.M) + sum_e - E.degree(t.E))
+    return deg
 
 
 @dataclass
@@ -136,6 +190,7 @@ class Builder:
         self.modules = {}
         self.rounds = []
         self.names = 0
+        self.logup_bus = []   # lists of total names that must sum to zero
 
     def fresh_name(self, prefix):
         self.names += 1
@@ -162,6 +217,61 @@ class Builder:
     def add_step_at(self, r, step):
         self._grow(r)
         self.rounds[r].steps.append(step)
+
+    def set_fs_hook(self, r, hook):
+        self._grow(r)
+        self.rounds[r].fshook = hook
+
+    def add_lookup_multiplicity_step(self, r, sel_s, sel_t, S, T, output, source_modules, target_module):
+        """board.AddLookupMultiplicityStep (board/board.go:235-249): one
+        output column per target, named chunk_name(output, k)."""
+        ctx = dict(n_sources=len(S), n_targets=len(T), width=len(S[0]), has_sel_s=sel_s is not None,
+                   has_sel_t=sel_t is not None, source_modules=list(source_modules))
+        ins = list(sel_s or []) + list(sel_t or [])
+        for t in S:
+            ins += list(t)
+        for t in T:
+            ins += list(t)
+        outs = [chunk_name(output, i) for i in range(len(T))]
+        self.add_step_at(r, ProverStep(LOOKUP_MULTIPLICITY, ins, outs, target_module, ctx))
+
+    def add_cyclic_logup_step(self, r, module, terms, output):
+        """board.AddCyclicLogupStep (board/logup.go:68-136): the column L with
+        L_i - L_{i-1} + T/N = sum_j M_j/E_j (row 0 reading L_{N-1}); returns
+        the name of the exposed total T."""
+        if not terms:
+            raise ValueError("AddCyclicLogupStep %s: no terms" % output)
+        for t in terms:
+            if t.module != module:
+                raise ValueError("AddCyclicLogupStep %s: term of module %r in a column of %r" % (output, t.module, module))
+        d = logup_constraint_degree(terms)
+        if d > MAX_LOGUP_DEGREE:
+            raise ValueError("AddCyclicLogupStep %s: constraint degree %d exceeds %d" % (output, d, MAX_LOGUP_DEGREE))
+        m = self.modules[module]
+        total = logup_total_name(output)
+        ins = []
+        for t in terms:
+            ins += [t.E, t.M]
+        self.add_step_at(r, ProverStep(CYCLIC_LOGUP, ins, [output, total], module, dict(n_terms=len(terms))))
+
+        def prod_e(skip):
+            res = None
+            for i, t in enumerate(terms):
+                if i == skip:
+                    continue
+                e = E.clone(t.E)
+                res = e if res is None else res * e
+            return res
+
+        increment = E.Col(output) - E.Col(output, -1) + E.ExposedAverage(total)
+        relation = increment * prod_e(-1)
+        for j, t in enumerate(terms):
+            num = E.clone(t.M)
+            if len(terms) > 1:
+                num = num * prod_e(j)
+            relation = relation - num
+        m.assert_zero(relation)
+        return total
 
     def add_grand_product_step(self, r, module, num, den, output):
         """board.AddGrandProductStep + addGrandProductConstraint (board.go:246-267)."""
@@ -212,6 +322,92 @@ def copy_constraint(b, module, A, S):
     permutation_tuple_within_module(b, module, aid, aperm)
 
 
+@dataclass
+class Column:
+    """board.Column: an expression of one module."""
+    module: str
+    expr: object
+
+
+def range_check(b, S, bound):
+    """arguments.Range (arguments/lookup.go:34-51): S takes values in
+    [0, bound) (bound rounded up to a power of two), through a lookup into the
+    shared range module of that bound."""
+    bound = 1 << max(0, (int(bound) - 1).bit_length())
+    name = range_module_name(bound)
+    if name not in b.modules:
+        rm = Module(name, bound)
+        rm.gencol.append(RangeColumnGen(bound))
+        b.add_module(rm)
+    T = Column(name, E.Col(range_col_name(bound)))
+    lookup(b, S, T)
+
+
+def lookup(b, S, T):
+    """arguments.Lookup -> LookupUnionTuple (lookup.go:60-80) for width-1
+    tables: multiplicities staged at round 0, one logup per module at round 2."""
+    mult = _stage_lookup(b, None, None, [[S]], [[T]])
+    one = E.Const(1)
+    mult_t = [E.Col(chunk_name(mult, 0))]
+    _add_logups(b, [S], [T], [one], mult_t)
+
+
+def _stage_lookup(b, sel_s, sel_t, S, T):
+    """arguments.stageLookup (lookup.go:130-148); S, T: lists of tables, each a
+    list of Columns."""
+    mult = b.fresh_name("Mult_")
+    for tables in (S, T):
+        for t in tables:
+            b.stage_leaves(ROUND_FOLD, *[c.expr for c in t])
+    b.stage_leaves(ROUND_FOLD, *(sel_s or []))
+    b.stage_leaves(ROUND_FOLD, *(sel_t or []))
+    b.add_lookup_multiplicity_step(ROUND_FOLD, sel_s, sel_t, [[c.expr for c in t] for t in S],
+                                   [[c.expr for c in t] for t in T], mult,
+                                   [t[0].module for t in S], T[0][0].module)
+    for i in range(len(T)):
+        b.stage_columns(ROUND_FOLD, chunk_name(mult, i))
+    return mult
+
+
+def _add_logups(b, S, T, num_s, num_t):
+    """arguments.addLogups (lookup.go:157-178): sources contribute
+    num_s/(s - gamma), targets -num_t/(t - gamma); totals go on a bus."""
+    gamma = Coin(ROUND_LOG_DERIVATIVE)
+    minus_one = E.Const(E.P - 1)
+    terms = []
+    for i, s in enumerate(S):
+        terms.append(LogupTerm(s.module, s.expr - gamma, num_s[i]))
+    for i, t in enumerate(T):
+        terms.append(LogupTerm(t.module, t.expr - gamma, minus_one * num_t[i]))
+    totals = add_logup_columns(b, "logup_", terms)
+    b.logup_bus.append(totals)
+
+
+def add_logup_columns(b, prefix, terms, max_degree=None):
+    """arguments.AddLogupColumnsWithMaxDegree (lookup.go:195-230): terms
+    grouped by module in order of first appearance, batched into columns of at
+    most MAX_LOGUP_BATCH terms within the degree limit."""
+    order = []
+    by_module = {}
+    for t in terms:
+        if t.module not in by_module:
+            order.append(t.module)
+            by_module[t.module] = []
+        by_module[t.module].append(t)
+    totals = []
+    for module in order:
+        ts = by_module[module]
+        limit = (max_degree or {}).get(module, MAX_LOGUP_DEGREE)
+        while ts:
+            n = 1
+            while n < min(MAX_LOGUP_BATCH, len(ts)) and logup_constraint_degree(ts[:n + 1]) <= limit:
+                n += 1
+            name = chunk_name(b.fresh_name("%s.%s" % (module, prefix)), 0)
+            totals.append(b.add_cyclic_logup_step(ROUND_RUNNING_SUMS, module, ts[:n], name))
+            ts = ts[n:]
+    return totals
+
+
 # ------------------------------------------------------------------ compile
 @dataclass
 class ColumnRef:
@@ -242,6 +438,7 @@ class Program:
     modules: dict
     rounds: list
     column_fields: dict
+    logup_bus: list = field(default_factory=list)
 
 
 def _column_modules(modules):
@@ -259,7 +456,12 @@ def compile_program(b):
     rounds = []
     staged = {}
     for r, br in enumerate(declared):
-        rd = ProverRound(list(br.steps), [], br.fshook)
+        hook = br.fshook
+        if any(st.exposed_outputs() for st in br.steps):
+            if hook not in (NO_FS_HOOK, BIND_EXPOSED_VALUES):
+                raise ValueError("round %d exposes values but has FS hook %d" % (r, hook))
+            hook = BIND_EXPOSED_VALUES
+        rd = ProverRound(list(br.steps), [], hook)
         for name in br.staged:
             if name in staged:
                 raise ValueError("column %r staged twice" % name)
@@ -289,9 +491,11 @@ def compile_program(b):
     for r, rd in enumerate(rounds):
         setf(challenge_name(r), E.EXT)
         for st in rd.steps:
+            # inferStepOutputFields (board/program.go:181-222)
             f = E.BASE
-            for e in st.ins:
-                f = E.join(f, E.field_of(e, fields))
+            if st.kind != LOOKUP_MULTIPLICITY:
+                for e in st.ins:
+                    f = E.join(f, E.field_of(e, fields))
             for o in st.outs:
                 setf(o, f)
     for rd in rounds:
@@ -304,4 +508,4 @@ def compile_program(b):
         rel = E.fold(fold_ch, m.relations)
         dag = E.to_dag(rel, fields)
         mods[k] = CompiledModule(m.name, m.N, rel, dag, E.degree(rel), list(m.gencol))
-    return Program(mods, rounds, fields)
+    return Program(mods, rounds, fields, [list(x) for x in b.logup_bus])

@@ -98,6 +98,21 @@ void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
                           uint32_t* const* gp, int n, int batch, hipStream_t s);
 
+// Lookups and logup (board/module.go:56-71, poly/lookup_multiplicity.go:37-100,
+// board/logup.go:138-186), batch on grid y, per-instance pointer arrays.
+void launch_range_col(uint32_t* const* outs, int n, uint32_t bound, int batch, hipStream_t s);
+void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s);
+// counts[b][v] += 1 for every canonical value v < bound of src[b][0..n), err[b] += 1 for the others
+void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
+                      int batch, hipStream_t s);
+void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s);
+// acc = (first ? 0 : acc) + m / e (E6 columns, limb stride n)
+void launch_logup_acc(const uint32_t* const* e, const uint32_t* const* m, uint32_t* const* acc, int n, int first,
+                      int batch, hipStream_t s);
+// col = inclusive running sum of acc minus (i+1) * total / n; total to tot[6b..6b+6)
+void launch_cyclic_logup_finish(const uint32_t* const* acc, uint32_t* const* col, uint32_t* tot, int n,
+                                uint32_t ninv, int batch, hipStream_t s);
+
 // Quotient coefficients from the per-coset inverse transforms:
 //   coef[b][t][l][a] = ginv[a + t n] * sum_i wr[(t i) mod rho] * Y[b][l][i n + a]
 // (wr[k] = w_rho^-k, ginv[m] = g^-m).

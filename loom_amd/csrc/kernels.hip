@@ -626,6 +626,151 @@ void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den
   LG_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ lookups and logup
+// Range column (board/module.go:56-71): row i holds i for i < bound.
+__global__ void k_range_col(uint32_t* const* outs, int n, uint32_t bound) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  outs[blockIdx.y][j] = (uint32_t)j < bound ? kb_to_mont((uint32_t)j) : 0u;
+}
+
+void launch_range_col(uint32_t* const* outs, int n, uint32_t bound, int batch, hipStream_t s) {
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_range_col, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, n, bound);
+  LG_HIP(hipGetLastError());
+}
+
+__global__ void k_fill(uint32_t* const* p, int n, uint32_t v) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  p[blockIdx.y][j] = v;
+}
+
+void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s) {
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_fill, dim3((n + T - 1) / T, batch), dim3(T), 0, s, p, n, v);
+  LG_HIP(hipGetLastError());
+}
+
+// Counts the canonical values of src[b][0..n) (Montgomery words) below bound
+// into counts[b]; values at or above bound count into err[b].
+__global__ void k_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound,
+                            uint32_t* err) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const int b = blockIdx.y;
+  const uint32_t v = kb_from_mont(src[b][j]);
+  if (v < bound) atomicAdd(&counts[b][v], 1u);
+  else atomicAdd(&err[b], 1u);
+}
+
+void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
+                      int batch, hipStream_t s) {
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_histogram, dim3((n + T - 1) / T, batch), dim3(T), 0, s, src, n, counts, bound, err);
+  LG_HIP(hipGetLastError());
+}
+
+// In place: counts (below p) to field elements, Montgomery.
+__global__ void k_counts_to_field(uint32_t* const* c, int n) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t* p = c[blockIdx.y];
+  p[j] = kb_to_mont(p[j]);
+}
+
+void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s) {
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_counts_to_field, dim3((n + T - 1) / T, batch), dim3(T), 0, s, c, n);
+  LG_HIP(hipGetLastError());
+}
+
+// acc[i] = (first ? 0 : acc[i]) + m[i] / e[i] over E6 columns (limb stride n).
+__global__ __launch_bounds__(256) void k_logup_acc(const uint32_t* const* e, const uint32_t* const* m,
+                                                   uint32_t* const* acc, int n, int first) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = blockIdx.y;
+  E6 v = e6_mul(ld6(m[b], n, i), e6_inv(ld6(e[b], n, i)));
+  if (!first) v = e6_add(v, ld6(acc[b], n, i));
+  st6(acc[b], n, i, v);
+}
+
+void launch_logup_acc(const uint32_t* const* e, const uint32_t* const* m, uint32_t* const* acc, int n, int first,
+                      int batch, hipStream_t s) {
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_logup_acc, dim3((n + T - 1) / T, batch), dim3(T), 0, s, e, m, acc, n, first);
+  LG_HIP(hipGetLastError());
+}
+
+// Inclusive running sum out[i] = sum_{r <= i} in[r] of an E6 column, one
+// workgroup per instance (chunk per thread, LDS scan of the chunk sums);
+// the total goes to tot[6 b .. 6 b + 6).
+__global__ __launch_bounds__(1024) void k_prefix_sum(const uint32_t* const* in, uint32_t* const* out, uint32_t* tot,
+                                                     int n) {
+  __shared__ uint32_t sh[1024 * 6];
+  const int b = blockIdx.x;
+  const uint32_t* X = in[b];
+  uint32_t* Y = out[b];
+  const int T = blockDim.x;
+  const int tid = threadIdx.x;
+  const int c = n / T;
+  const int lo = tid * c;
+  E6 acc = e6_zero();
+  for (int i = lo; i < lo + c; ++i) acc = e6_add(acc, ld6(X, n, i));
+#pragma unroll
+  for (int l = 0; l < 6; ++l) sh[6 * tid + l] = acc.v[l];
+  __syncthreads();
+  for (int off = 1; off < T; off <<= 1) {
+    E6 v = ldc6(sh + 6 * tid);
+    E6 u = e6_zero();
+    if (tid >= off) u = ldc6(sh + 6 * (tid - off));
+    __syncthreads();
+    if (tid >= off) {
+      E6 w = e6_add(u, v);
+#pragma unroll
+      for (int l = 0; l < 6; ++l) sh[6 * tid + l] = w.v[l];
+    }
+    __syncthreads();
+  }
+  E6 run = tid == 0 ? e6_zero() : ldc6(sh + 6 * (tid - 1));
+  for (int i = lo; i < lo + c; ++i) {
+    run = e6_add(run, ld6(X, n, i));
+    st6(Y, n, i, run);
+  }
+  if (tid == T - 1) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) tot[6 * b + l] = run.v[l];
+  }
+}
+
+// L_i = S_i - (i + 1) * tot / n in place (board/logup.go:160-171).
+__global__ void k_logup_finish(uint32_t* const* col, const uint32_t* tot, int n, uint32_t ninv) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int b = blockIdx.y;
+  const E6 avg = e6_scale(ldc6(tot + 6 * b), ninv);
+  const E6 v = e6_sub(ld6(col[b], n, i), e6_scale(avg, kb_to_mont((uint32_t)i + 1u)));
+  st6(col[b], n, i, v);
+}
+
+void launch_cyclic_logup_finish(const uint32_t* const* acc, uint32_t* const* col, uint32_t* tot, int n,
+                                uint32_t ninv, int batch, hipStream_t s) {
+  int T2 = n < 1024 ? n : 1024;
+  prelaunch();
+  hipLaunchKernelGGL(k_prefix_sum, dim3(batch), dim3(T2), 0, s, acc, col, tot, n);
+  LG_HIP(hipGetLastError());
+  const int T = n < 256 ? n : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_logup_finish, dim3((n + T - 1) / T, batch), dim3(T), 0, s, col, (const uint32_t*)tot, n, ninv);
+  LG_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ quotient
 __global__ void k_quot_combine(const uint32_t* Y, uint32_t* coef, const uint32_t* ginv, const uint32_t* wr, int rho,
                                int n) {

@@ -13,9 +13,12 @@ namespace lg {
 constexpr uint32_t PLAN_MAGIC = 0x31504C4Cu;
 constexpr uint32_t PLAN_NONE = 0xFFFFFFFFu;
 
-enum ColSrc : uint32_t { SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR };
+constexpr uint32_t PLAN_VERSION = 2;
+
+enum ColSrc : uint32_t { SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE };
 enum DagOp : uint32_t { OP_COL = 0, OP_CONST, OP_CHAL, OP_ADD, OP_SUB, OP_MUL, OP_POW };
-enum StepKind : uint32_t { STEP_GRAND_PRODUCT = 1 };
+enum StepKind : uint32_t { STEP_GRAND_PRODUCT = 1, STEP_LOOKUP_MULTIPLICITY = 2, STEP_CYCLIC_LOGUP = 3 };
+enum FsHook : uint32_t { FS_NONE = 0, FS_BIND_EXPOSED_VALUES = 1 };
 
 struct DagNode {
   uint32_t op, field, a, b;
@@ -43,8 +46,15 @@ struct Perm {
 };
 
 struct Step {
-  uint32_t kind, module, out_col;
+  uint32_t kind, module, out_col;  // out_col = outs[0]
+  std::vector<uint32_t> outs;
+  std::vector<uint32_t> params;    // kind-specific, see loom_amd/plan.py
   std::vector<Dag> ins;
+};
+
+struct Exposed {
+  uint32_t col;
+  std::string name;
 };
 
 struct Round {
@@ -70,6 +80,7 @@ struct PlanData {
   std::vector<uint32_t> inputs;
   std::vector<Perm> perms;
   std::vector<Round> rounds;
+  std::vector<Exposed> exposed;  // sorted by name
   std::vector<TreeInfo> trees;
 };
 
@@ -127,7 +138,7 @@ inline int ilog2(uint64_t n) {
 inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
   WordReader r(words, n_words);
   if (r.next() != PLAN_MAGIC) throw std::runtime_error("plan: bad magic");
-  if (r.next() != 1) throw std::runtime_error("plan: unsupported version");
+  if (r.next() != PLAN_VERSION) throw std::runtime_error("plan: unsupported version");
   PlanData p;
   p.num_queries = r.next();
   uint32_t nm = r.next();
@@ -163,10 +174,36 @@ inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
     uint32_t ns = r.next();
     rd.steps.resize(ns);
     for (auto& st : rd.steps) {
-      st.kind = r.next(); st.module = r.next(); st.out_col = r.next();
+      st.kind = r.next(); st.module = r.next();
+      if (st.module >= nm) throw std::runtime_error("plan: step module out of range");
+      uint32_t no = r.next();
+      if (no == 0 || no > 64) throw std::runtime_error("plan: bad step output count");
+      for (uint32_t k = 0; k < no; ++k) {
+        st.outs.push_back(r.next());
+        if (st.outs.back() >= nc) throw std::runtime_error("plan: step output out of range");
+      }
+      st.out_col = st.outs[0];
+      uint32_t npar = r.next();
+      if (npar > 4096) throw std::runtime_error("plan: bad step parameter count");
+      for (uint32_t k = 0; k < npar; ++k) st.params.push_back(r.next());
       uint32_t nin = r.next();
       for (uint32_t k = 0; k < nin; ++k) st.ins.push_back(read_dag(r, nc));
     }
+  }
+  uint32_t nx = r.next();
+  p.exposed.resize(nx);
+  for (auto& x : p.exposed) {
+    x.col = r.next();
+    if (x.col >= nc) throw std::runtime_error("plan: exposed column out of range");
+    uint32_t len = r.next();
+    if (len > 4096) throw std::runtime_error("plan: exposed name too long");
+    std::string s;
+    for (uint32_t k = 0; k < (len + 3) / 4; ++k) {
+      uint32_t wd = r.next();
+      for (int b = 0; b < 4; ++b) s.push_back((char)((wd >> (8 * b)) & 0xff));
+    }
+    s.resize(len);
+    x.name = s;
   }
   uint32_t nt = r.next();
   p.trees.resize(nt);

@@ -284,12 +284,49 @@ Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_
   rel_.resize(d_.modules.size());
   for (size_t m = 0; m < d_.modules.size(); ++m)
     if (d_.modules[m].n_chunks) rel_[m] = dag_kernel_get(d_.modules[m].relation, d_.cols);
+  std::vector<char> logup_total(d_.cols.size(), 0);
   for (size_t r = 0; r < d_.rounds.size(); ++r)
     for (size_t s = 0; s < d_.rounds[r].steps.size(); ++s) {
       const Step& st = d_.rounds[r].steps[s];
-      if (st.kind != STEP_GRAND_PRODUCT || st.ins.size() != 2) throw std::runtime_error("unsupported prover step");
-      for (size_t k = 0; k < st.ins.size(); ++k) steps_[{r, s, k}] = dag_kernel_get(st.ins[k], d_.cols);
+      size_t n_kernels = st.ins.size();
+      switch (st.kind) {
+        case STEP_GRAND_PRODUCT:
+          if (st.ins.size() != 2 || st.outs.size() != 1) throw std::runtime_error("grand product step: bad shape");
+          break;
+        case STEP_LOOKUP_MULTIPLICITY: {
+          // Supported: width-1 lookups without selectors into one generated
+          // range column of the step's module (arguments.Range).
+          if (st.params.size() < 5) throw std::runtime_error("lookup step: bad parameters");
+          const uint32_t ns = st.params[0];
+          if (st.params[1] != 1 || st.params[2] != 1 || st.params[3] || st.params[4] ||
+              st.params.size() != 5 + ns || st.ins.size() != ns + 1 || st.outs.size() != 1)
+            throw std::runtime_error("lookup step: only width-1 lookups without selectors into a range table are supported");
+          for (uint32_t j = 0; j < ns; ++j)
+            if (st.params[5 + j] >= d_.modules.size()) throw std::runtime_error("lookup step: source module out of range");
+          const Dag& t = st.ins[ns];
+          if (t.nodes.size() != 1 || t.nodes[0].op != OP_COL || t.nodes[0].b != 0 ||
+              d_.cols[t.nodes[0].a].src != SRC_RANGE || d_.cols[t.nodes[0].a].module != st.module ||
+              d_.cols[t.nodes[0].a].a0 > d_.modules[st.module].N)
+            throw std::runtime_error("lookup step: the target must be a range column of the step's module");
+          if (d_.cols[st.outs[0]].field != 0 || d_.cols[st.outs[0]].module != st.module)
+            throw std::runtime_error("lookup step: bad multiplicity column");
+          n_kernels = ns;
+          break;
+        }
+        case STEP_CYCLIC_LOGUP:
+          if (st.params.size() != 1 || st.params[0] == 0 || st.ins.size() != 2 * st.params[0] || st.outs.size() != 2)
+            throw std::runtime_error("cyclic logup step: bad shape");
+          if (d_.cols[st.outs[0]].field != 1 || d_.cols[st.outs[1]].field != 1)
+            throw std::runtime_error("cyclic logup step: outputs must be extension columns");
+          logup_total[st.outs[1]] = 1;
+          break;
+        default:
+          throw std::runtime_error("unsupported prover step");
+      }
+      for (size_t k = 0; k < n_kernels; ++k) steps_[{r, s, k}] = dag_kernel_get(st.ins[k], d_.cols);
     }
+  for (auto& x : d_.exposed)
+    if (!logup_total[x.col]) throw std::runtime_error("plan: exposed value is not a logup total");
 }
 
 // ------------------------------------------------------------------ job
@@ -388,6 +425,11 @@ class Job {
   void upload_inputs();
   void gencol();
   void run_steps(size_t r);
+  Layer eval_step_dag(size_t r, size_t s, size_t k, uint32_t mi);
+  void step_grand_product(size_t r, size_t s);
+  void step_multiplicity(size_t r, size_t s);
+  void step_logup(size_t r, size_t s);
+  void finish_round_steps(size_t r);
   void lde_columns(const std::vector<uint32_t>& col_ids);
   // Runs the transforms cfg describes on every job: one LDS-resident launch
   // up to 2^14, the two-pass four-step form above.
@@ -416,13 +458,18 @@ class Job {
   std::vector<TreeBuf> trees_;
   std::vector<DevBuf> keep_;
   std::vector<Transcript> ts_;
-  std::vector<std::vector<E6>> chal_;   // [b][col] challenge values (only challenge columns set)
+  std::vector<std::vector<E6>> chal_;   // [b][col] challenge values (and logup averages T/N)
+  std::vector<std::vector<E6>> totals_;  // [b][col] exposed logup totals
+  std::vector<std::pair<uint32_t, DevBuf>> totals_dev_;  // (total column, B x 6 words) of the current round
+  std::vector<DevBuf> lookup_err_;                       // B words per multiplicity step of the current round
   std::vector<std::vector<Digest>> roots_;  // [tree][b]
   std::vector<E6> zeta_;
 };
 
 void Job::alloc_columns() {
   cols_.resize(d_.cols.size());
+  std::vector<char> is_total(d_.cols.size(), 0);
+  for (auto& x : d_.exposed) is_total[x.col] = 1;
   std::vector<char> committed(d_.cols.size(), 0);
   for (auto& t : d_.trees)
     for (auto& g : t.groups) {
@@ -435,7 +482,7 @@ void Job::alloc_columns() {
     cb.limbs = ci.limbs();
     cb.n = d_.modules[ci.module].N;
     cb.committed = committed[c] != 0;
-    if (ci.src == SRC_CHALLENGE) continue;
+    if (ci.src == SRC_CHALLENGE || is_total[c]) continue;  // per-instance constants, kept in chal_
     cb.blk = (size_t)cb.limbs * cb.n * (cb.committed ? 5 : 1);
     cb.buf = dev(cb.blk * B_);
   }
@@ -525,12 +572,14 @@ void Job::gencol() {
   for (size_t c = 0; c < d_.cols.size(); ++c) {
     const ColInfo& ci = d_.cols[c];
     const ColBuf& cb = cols_[c];
-    if (ci.src != SRC_LAGRANGE && ci.src != SRC_ID && ci.src != SRC_SIGMA) continue;
+    if (ci.src != SRC_LAGRANGE && ci.src != SRC_ID && ci.src != SRC_SIGMA && ci.src != SRC_RANGE) continue;
     const int logn = d_.modules[ci.module].log_n;
     std::vector<uint32_t*> outs(B_);
     for (int b = 0; b < B_; ++b) outs[b] = cb.lag(b, 0);
     const auto* outs_d = stage(outs);
-    if (ci.src == SRC_LAGRANGE) {
+    if (ci.src == SRC_RANGE) {
+      launch_range_col(const_cast<uint32_t* const*>(outs_d), cb.n, ci.a0, B_, s_);
+    } else if (ci.src == SRC_LAGRANGE) {
       if (ci.a0 >= cb.n) throw std::runtime_error("Lagrange column position out of range");
       launch_lagrange_col(const_cast<uint32_t* const*>(outs_d), cb.n, ci.a0, B_, s_);
     } else if (ci.src == SRC_ID) {
@@ -782,53 +831,176 @@ void Job::commit_tree(size_t t) {
   debug_dump(("tree" + std::to_string(t)).c_str(), ids, cols_, s_);
 }
 
+// Evaluates input DAG k of step (r, s) on the rows of module mi (Lagrange
+// values), into an E6 layer.
+Layer Job::eval_step_dag(size_t r, size_t s, size_t k, uint32_t mi) {
+  const DagKernel& K = *plan_.step_kernel(r, s, k);
+  const ModuleInfo& m = d_.modules[mi];
+  Layer out;
+  out.len = m.N;
+  out.buf = dev((size_t)B_ * 6 * m.N);
+  std::vector<const uint32_t*> cp;
+  std::vector<E6> ch;
+  for (int b = 0; b < B_; ++b) {
+    for (auto c : K.col_ids) cp.push_back(cols_[c].lag(b, 0));
+    for (auto c : K.chal_ids) ch.push_back(chal_[b][c]);
+  }
+  std::vector<uint32_t*> op(B_);
+  for (int b = 0; b < B_; ++b) op[b] = out.ptr(b);
+  DagLaunch a{};
+  a.cols = stage(cp);
+  a.chal = reinterpret_cast<const uint32_t*>(stg_->put(ch.data(), ch.size() * sizeof(E6), s_));
+  a.rowscale = nullptr;
+  a.out = const_cast<uint32_t* const*>(stage(op));
+  a.col_stride = m.N;
+  a.out_stride = m.N;
+  a.logn = m.log_n;
+  a.ncos = 1;
+  a.batch = B_;
+  dag_kernel_launch(K, a, s_);
+  return out;
+}
+
 void Job::run_steps(size_t r) {
   const Round& rd = d_.rounds[r];
   for (size_t s = 0; s < rd.steps.size(); ++s) {
-    const Step& st = rd.steps[s];
-    const ModuleInfo& m = d_.modules[st.module];
-    const uint32_t n = m.N;
-    Layer nd[2];
-    for (int k = 0; k < 2; ++k) {
-      const DagKernel& K = *plan_.step_kernel(r, s, k);
-      nd[k].len = n;
-      nd[k].buf = dev((size_t)B_ * 6 * n);
-      std::vector<const uint32_t*> cp;
-      std::vector<E6> ch;
-      for (int b = 0; b < B_; ++b) {
-        for (auto c : K.col_ids) cp.push_back(cols_[c].lag(b, 0));
-        for (auto c : K.chal_ids) ch.push_back(chal_[b][c]);
-      }
-      std::vector<uint32_t*> op(B_);
-      for (int b = 0; b < B_; ++b) op[b] = nd[k].ptr(b);
-      DagLaunch a{};
-      a.cols = stage(cp);
-      a.chal = reinterpret_cast<const uint32_t*>(stg_->put(ch.data(), ch.size() * sizeof(E6), s_));
-      a.rowscale = nullptr;
-      a.out = const_cast<uint32_t* const*>(stage(op));
-      a.col_stride = n;
-      a.out_stride = n;
-      a.logn = m.log_n;
-      a.ncos = 1;
-      a.batch = B_;
-      dag_kernel_launch(K, a, s_);
+    switch (rd.steps[s].kind) {
+      case STEP_GRAND_PRODUCT: step_grand_product(r, s); break;
+      case STEP_LOOKUP_MULTIPLICITY: step_multiplicity(r, s); break;
+      case STEP_CYCLIC_LOGUP: step_logup(r, s); break;
+      default: throw std::runtime_error("unsupported prover step");
     }
-    std::vector<const uint32_t*> np(B_), dp(B_);
-    std::vector<uint32_t*> gp(B_), rp(B_);
-    Layer ratio;
-    ratio.len = n;
-    ratio.buf = dev((size_t)B_ * 6 * n);
+  }
+}
+
+// board.GrandProductStep (board/step.go:301-327).
+void Job::step_grand_product(size_t r, size_t s) {
+  const Step& st = d_.rounds[r].steps[s];
+  const uint32_t n = d_.modules[st.module].N;
+  Layer nd[2] = {eval_step_dag(r, s, 0, st.module), eval_step_dag(r, s, 1, st.module)};
+  std::vector<const uint32_t*> np(B_), dp(B_);
+  std::vector<uint32_t*> gp(B_), rp(B_);
+  Layer ratio;
+  ratio.len = n;
+  ratio.buf = dev((size_t)B_ * 6 * n);
+  for (int b = 0; b < B_; ++b) {
+    np[b] = nd[0].ptr(b);
+    dp[b] = nd[1].ptr(b);
+    gp[b] = cols_[st.out_col].lag(b, 0);
+    rp[b] = ratio.ptr(b);
+  }
+  launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(rp)),
+                       const_cast<uint32_t* const*>(stage(gp)), n, B_, s_);
+  keep_.push_back(std::move(ratio.buf));
+  keep_.push_back(std::move(nd[0].buf));
+  keep_.push_back(std::move(nd[1].buf));
+}
+
+// board.LookupMultiplicityStep (board/step.go:257-295) for a range target:
+// the target row holding v is row v, so the multiplicity column is the
+// histogram of the source values; a value outside the table is an error
+// (poly/lookup_multiplicity.go:90-96), reported after the round's commit.
+void Job::step_multiplicity(size_t r, size_t s) {
+  const Step& st = d_.rounds[r].steps[s];
+  const uint32_t ns = st.params[0];
+  const uint32_t bound = d_.cols[st.ins[ns].nodes[0].a].a0;
+  const ColBuf& mc = cols_[st.out_col];
+  std::vector<uint32_t*> cnt(B_), errp(B_);
+  DevBuf err = dev((size_t)B_);
+  for (int b = 0; b < B_; ++b) {
+    cnt[b] = mc.lag(b, 0);
+    errp[b] = err.as<uint32_t>() + b;
+  }
+  uint32_t* const* cnt_d = const_cast<uint32_t* const*>(stage(cnt));
+  launch_fill(cnt_d, (int)mc.n, 0u, B_, s_);
+  launch_fill(const_cast<uint32_t* const*>(stage(errp)), 1, 0u, B_, s_);
+  for (uint32_t j = 0; j < ns; ++j) {
+    const uint32_t mj = st.params[5 + j];
+    Layer src = eval_step_dag(r, s, j, mj);
+    std::vector<const uint32_t*> sp(B_);
+    for (int b = 0; b < B_; ++b) sp[b] = src.ptr(b);  // limb 0: the base value
+    launch_histogram(stage(sp), (int)d_.modules[mj].N, cnt_d, bound, err.as<uint32_t>(), B_, s_);
+    keep_.push_back(std::move(src.buf));
+  }
+  launch_counts_to_field(cnt_d, (int)mc.n, B_, s_);
+  lookup_err_.push_back(std::move(err));
+}
+
+// board.CyclicLogUpStep (board/logup.go:138-186): L_i = sum_{r<=i} v_r - (i+1) T/N
+// with v = sum_j M_j / E_j; the totals T come back with the round's roots.
+void Job::step_logup(size_t r, size_t s) {
+  const Step& st = d_.rounds[r].steps[s];
+  const ModuleInfo& m = d_.modules[st.module];
+  const uint32_t nt = st.params[0];
+  Layer acc;
+  acc.len = m.N;
+  acc.buf = dev((size_t)B_ * 6 * m.N);
+  std::vector<uint32_t*> ap(B_), lp(B_);
+  for (int b = 0; b < B_; ++b) {
+    ap[b] = acc.ptr(b);
+    lp[b] = cols_[st.outs[0]].lag(b, 0);
+  }
+  uint32_t* const* ap_d = const_cast<uint32_t* const*>(stage(ap));
+  for (uint32_t j = 0; j < nt; ++j) {
+    Layer e = eval_step_dag(r, s, 2 * j, st.module);
+    Layer mm = eval_step_dag(r, s, 2 * j + 1, st.module);
+    std::vector<const uint32_t*> ep(B_), mp(B_);
     for (int b = 0; b < B_; ++b) {
-      np[b] = nd[0].ptr(b);
-      dp[b] = nd[1].ptr(b);
-      gp[b] = cols_[st.out_col].lag(b, 0);
-      rp[b] = ratio.ptr(b);
+      ep[b] = e.ptr(b);
+      mp[b] = mm.ptr(b);
     }
-    launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(rp)),
-                         const_cast<uint32_t* const*>(stage(gp)), n, B_, s_);
-    keep_.push_back(std::move(ratio.buf));
-    keep_.push_back(std::move(nd[0].buf));
-    keep_.push_back(std::move(nd[1].buf));
+    launch_logup_acc(stage(ep), stage(mp), ap_d, (int)m.N, j == 0 ? 1 : 0, B_, s_);
+    keep_.push_back(std::move(e.buf));
+    keep_.push_back(std::move(mm.buf));
+  }
+  DevBuf tot = dev((size_t)B_ * 6);
+  launch_cyclic_logup_finish(ap_d, const_cast<uint32_t* const*>(stage(lp)), tot.as<uint32_t>(), (int)m.N,
+                             kb_inv(M(m.N)), B_, s_);
+  keep_.push_back(std::move(acc.buf));
+  totals_dev_.push_back({st.outs[1], std::move(tot)});
+}
+
+// After round r's commitment: lookup errors, then the logup totals of the
+// round (exposed values, and the constant column T/N the relations read).
+void Job::finish_round_steps(size_t r) {
+  for (auto& e : lookup_err_) {
+    std::vector<uint32_t> h(B_);
+    LG_HIP(hipMemcpyAsync(h.data(), e.as<void>(), (size_t)B_ * 4, hipMemcpyDeviceToHost, s_));
+    sync();
+    for (int b = 0; b < B_; ++b)
+      if (h[b]) throw std::runtime_error("lookup: a source value is not in the range table (instance " +
+                                         std::to_string(b) + ")");
+  }
+  for (auto& e : lookup_err_) keep_.push_back(std::move(e));
+  lookup_err_.clear();
+  for (auto& kv : totals_dev_) {
+    std::vector<uint32_t> h((size_t)B_ * 6);
+    LG_HIP(hipMemcpyAsync(h.data(), kv.second.as<void>(), h.size() * 4, hipMemcpyDeviceToHost, s_));
+    sync();
+    const uint32_t col = kv.first;
+    const uint32_t ninv = kb_inv(M(d_.modules[d_.cols[col].module].N));
+    for (int b = 0; b < B_; ++b) {
+      E6 t;
+      for (int l = 0; l < 6; ++l) t.v[l] = h[6 * b + l];
+      totals_[b][col] = t;
+      chal_[b][col] = e6_scale(t, ninv);
+    }
+    keep_.push_back(std::move(kv.second));
+  }
+  totals_dev_.clear();
+  const Round& rd = d_.rounds[r];
+  if (rd.fs_hook == FS_BIND_EXPOSED_VALUES) {
+    // ExposedValueElements (board/logup.go:213-225): [index 0, 6 limbs] per
+    // total, in step order (prover.go:386-395)
+    for (int b = 0; b < B_; ++b)
+      for (auto& st : rd.steps) {
+        if (st.kind != STEP_CYCLIC_LOGUP) continue;
+        uint32_t w[7] = {0};
+        for (int l = 0; l < 6; ++l) w[1 + l] = totals_[b][st.outs[1]].v[l];
+        ts_[b].bind(challenge_name(r), w, 7);
+      }
+  } else if (rd.fs_hook != FS_NONE) {
+    throw std::runtime_error("unsupported FS hook");
   }
 }
 
@@ -980,6 +1152,7 @@ void Job::run() {
   trees_.resize(d_.trees.size());
   roots_.resize(d_.trees.size());
   chal_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
+  totals_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
   ts_.assign(B_, Transcript());
   const size_t nr = d_.rounds.size();
   for (int b = 0; b < B_; ++b) {
@@ -1001,7 +1174,7 @@ void Job::run() {
       commit_tree(rd.tree);
       for (int b = 0; b < B_; ++b) ts_[b].bind(challenge_name(r), roots_[rd.tree][b].v, 8);
     }
-    if (rd.fs_hook != 0) throw std::runtime_error("exposed-value rounds are not supported by this build");
+    finish_round_steps(r);
     parallel_for(B_, [&](int b) {
       const Digest& dg = ts_[b].compute(challenge_name(r));
       E6 v;
@@ -1489,7 +1662,18 @@ void Job::open() {
       for (size_t k = 0; k < 4 && i + k < hid.size(); ++k) x |= (uint32_t)(uint8_t)hid[i + k] << (8 * k);
       w.push_back(x);
     }
-    w.push_back(0);  // exposed values
+    w.push_back((uint32_t)d_.exposed.size());  // exposed values, sorted by name
+    for (auto& x : d_.exposed) {
+      w.push_back((uint32_t)x.name.size());
+      for (size_t i = 0; i < x.name.size(); i += 4) {
+        uint32_t v = 0;
+        for (size_t k = 0; k < 4 && i + k < x.name.size(); ++k) v |= (uint32_t)(uint8_t)x.name[i + k] << (8 * k);
+        w.push_back(v);
+      }
+      w.push_back(1);  // entries
+      w.push_back(0);  // index
+      w.insert(w.end(), totals_[b][x.col].v, totals_[b][x.col].v + 6);
+    }
     w.push_back((uint32_t)nt);
     for (size_t t = 0; t < nt; ++t) w.insert(w.end(), roots_[t][b].v, roots_[t][b].v + 8);
     // claimed values

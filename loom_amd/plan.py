@@ -17,7 +17,13 @@ Word stream (u32, little endian):
   n_perms, then per permutation: module, len   (the permutation itself is
             per-instance data, passed beside the trace columns)
   n_rounds, then per round: challenge col, trace tree (or NONE), fs_hook,
-            n_steps, per step: kind, module, out col, n_ins, dag...
+            n_steps, per step: kind, module, n_outs, out cols...,
+            n_params, params..., n_ins, dag...
+            (params: lookup multiplicity -> n_sources, n_targets, width,
+            has_sel_s, has_sel_t, source module per source; cyclic logup ->
+            n_terms; grand product -> none)
+  n_exposed, then per exposed value (sorted by name): column, name
+            (byte length, bytes packed 4 per word, little endian)
   n_trees, then per tree: kind, n_groups, per group: module, N,
             nb, base cols..., ne, ext cols..., shift lists (base polys then
             ext polys; each: count, shifts...)
@@ -31,10 +37,10 @@ from . import board as B
 from . import protocol as PR
 
 MAGIC = 0x31504C4C
-VERSION = 1
+VERSION = 2
 NONE = 0xFFFFFFFF
 
-SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR = range(7)
+SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE = range(8)
 OP_COL, OP_CONST, OP_CHAL, OP_ADD, OP_SUB, OP_MUL, OP_POW = range(7)
 
 
@@ -69,6 +75,8 @@ class Plan:
                     gen[B.lagrange_name(mn, g.i)] = (mi, SRC_LAGRANGE, g.i, 0)
                 elif isinstance(g, B.LagrangeRelativeGen):
                     gen[B.lagrange_name_relative(mn, g.i)] = (mi, SRC_LAGRANGE, m.N - 1 - g.i, 0)
+                elif isinstance(g, B.RangeColumnGen):
+                    gen[B.range_col_name(g.bound)] = (mi, SRC_RANGE, g.bound, 0)
                 elif isinstance(g, B.PermutationGen):
                     pi = len(self.perms)
                     self.perms.append((mi, np.asarray(g.S, dtype=np.int64)))
@@ -152,6 +160,9 @@ class Plan:
                     out += [OP_CONST, nd.field, lf.value % E.P, 0]
                 elif lf.type == E.CHALLENGE:
                     out += [OP_CHAL, nd.field, self.col_idx[lf.name], 0]
+                elif lf.type == E.VERIFIER and lf.hook == E.EXPOSED_AVERAGE_HOOK:
+                    # the constant column total/N: a per-instance E6 value, like a challenge
+                    out += [OP_CHAL, nd.field, self.col_idx[lf.name], 0]
                 else:
                     s = ((lf.shift % n) + n) % n
                     out += [OP_COL, nd.field, self.col_idx[lf.name], s]
@@ -182,9 +193,30 @@ class Plan:
                   len(rd.steps)]
             for st in rd.steps:
                 n = prog.modules[st.module].N
-                w += [st.kind, self.mod_idx[st.module], self.col_idx[st.outs[0]], len(st.ins)]
-                for e in st.ins:
-                    self._dag_words(e, n, w)
+                w += [st.kind, self.mod_idx[st.module], len(st.outs)] + [self.col_idx[o] for o in st.outs]
+                sizes = [n] * len(st.ins)
+                if st.kind == B.LOOKUP_MULTIPLICITY:
+                    c = st.ctx
+                    params = [c["n_sources"], c["n_targets"], c["width"], int(c["has_sel_s"]), int(c["has_sel_t"])]
+                    params += [self.mod_idx[x] for x in c["source_modules"]]
+                    ns = [prog.modules[x].N for x in c["source_modules"]]
+                    sizes = (ns if c["has_sel_s"] else []) + ([n] * c["n_targets"] if c["has_sel_t"] else [])
+                    sizes += [x for x in ns for _ in range(c["width"])] + [n] * (c["n_targets"] * c["width"])
+                elif st.kind == B.CYCLIC_LOGUP:
+                    params = [st.ctx["n_terms"]]
+                else:
+                    params = []
+                w += [len(params)] + params + [len(st.ins)]
+                for e, sz in zip(st.ins, sizes):
+                    self._dag_words(e, sz, w)
+        exposed = sorted(o for rd in prog.rounds for st in rd.steps for o in st.exposed_outputs())
+        w.append(len(exposed))
+        for name in exposed:
+            w.append(self.col_idx[name])
+            b = name.encode()
+            w.append(len(b))
+            for i in range(0, len(b), 4):
+                w.append(int.from_bytes(b[i:i + 4].ljust(4, b"\0"), "little"))
         w.append(len(lay.trees))
         for t in lay.trees:
             w += [0 if t.kind == "trace" else 1, len(t.groups)]

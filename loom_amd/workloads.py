@@ -49,8 +49,9 @@ def plonk_program(log_n, idx=0):
     return b, nm
 
 
-def _plonk_values(n, rng):
-    """sigma, wires and selectors of one module, drawn from rng."""
+def _plonk_values(n, rng, wire_bound=P):
+    """sigma, wires (uniform in [0, wire_bound)) and selectors of one module,
+    drawn from rng."""
     cells = 3 * n
     sigma = np.arange(cells, dtype=np.int64)
     tied = rng.permutation(cells)[: cells // 10]
@@ -64,7 +65,7 @@ def _plonk_values(n, rng):
         for k in range(ln):
             sigma[cyc[k]] = cyc[(k + 1) % ln]
         pos += ln
-    w = rng.integers(0, P, size=cells, dtype=np.uint64)
+    w = rng.integers(0, wire_bound, size=cells, dtype=np.uint64)
     for cyc in cycles:
         w[cyc[1:]] = w[cyc[0]]
     L, R, O = w[:n], w[n:2 * n], w[2 * n:]
@@ -81,15 +82,17 @@ def _plonk_values(n, rng):
     return sigma, dict(L=L.copy(), R=R.copy(), O=O.copy(), QL=ql, QR=qr, QM=qm, QO=qo, QK=qk)
 
 
-def _add_plonk_module(b, log_n, idx, rng):
+def _add_plonk_module(b, log_n, idx, rng, range_bound=None):
     n = 1 << log_n
-    sigma, vals = _plonk_values(n, rng)
+    sigma, vals = _plonk_values(n, rng, P if range_bound is None else range_bound)
     nm = _names(idx)
     m = B.Module(nm["module"], n)
     _plonk_gate(m, nm)
     b.add_module(m)
     S = B.PermutationGen(sigma, nm["perm"])
     B.copy_constraint(b, nm["module"], [E.Col(nm["L"]), E.Col(nm["R"]), E.Col(nm["O"])], S)
+    if range_bound is not None:
+        B.range_check(b, B.Column(nm["module"], E.Col(nm["L"])), range_bound)
     return {nm[k]: v for k, v in vals.items()}
 
 
@@ -110,6 +113,18 @@ def uneven_instance(log_ns, seed):
     for i, ln in enumerate(log_ns):
         rng = np.random.Generator(np.random.PCG64([seed, i]))
         trace.update(_add_plonk_module(b, ln, i, rng))
+    return B.compile_program(b), trace
+
+
+def uneven_range_instance(log_ns, seed, bound=256):
+    """The uneven batch with lookups: every module as in uneven_instance, its
+    wires drawn from [0, bound), plus arguments.Range(L, bound) -- a logup
+    lookup of its L column into the shared range_<bound> module."""
+    b = B.Builder()
+    trace = {}
+    for i, ln in enumerate(log_ns):
+        rng = np.random.Generator(np.random.PCG64([seed, i]))
+        trace.update(_add_plonk_module(b, ln, i, rng, bound))
     return B.compile_program(b), trace
 
 

@@ -8,6 +8,14 @@ Follows prover/prover.go:142-719 and the PCS it drives:
   ID_k = g^k * w^j and sigma columns);
 * grand product step -- board/step.go:301-327, poly/iop_utils.go:170-192,
   :315-322 (gp[0] = 1, gp[i] = gp[i-1] * Num[i-1] / Den[i-1]);
+* lookup multiplicities -- board/step.go:257-295,
+  poly/lookup_multiplicity.go:37-100 (weights of the source tuples, each
+  assigned to its first selected matching target row);
+* cyclic logup step -- board/logup.go:138-186, poly/iop_utils.go:108-131,
+  :307-314 (inclusive running sum of sum_j M_j/E_j, L_i = S_i - (i+1) T/N, the
+  total T exposed), bound to the transcript after the round's commitment
+  (prover.go:386-395, board/logup.go:213-225);
+* range columns -- board/module.go:56-71 (row i holds i);
 * commit -- fri/commitment.go:259-386 (RS encode, row-pair leaves, groups by
   decreasing size, smaller groups as level injections);
 * AIR quotient -- poly/compute_quotient.go:223-424 (E/(X^N - 1) on the coset
@@ -123,6 +131,8 @@ class Prover:
         self.commitments = []
         self.log = []   # (challenge name, digest)
         self.deep_on_h = {}  # DEEP class -> values on the trace domain (natural order)
+        self.exposed = {}    # exposed value name -> E6 total
+        self.exposed_avg = {}  # exposed value name -> E6 total / N
 
     # -- transcript helpers
     def compute(self, name):
@@ -143,6 +153,10 @@ class Prover:
                     c = np.zeros(n, dtype=np.uint64)
                     c[n - 1 - g.i] = 1
                     self.base.setdefault(B.lagrange_name_relative(mn, g.i), c)
+                elif isinstance(g, B.RangeColumnGen):
+                    c = np.zeros(n, dtype=np.uint64)
+                    c[:g.bound] = np.arange(g.bound, dtype=np.uint64)
+                    self.base.setdefault(B.range_col_name(g.bound), c)
                 elif isinstance(g, B.PermutationGen):
                     S = np.asarray(g.S, dtype=np.int64)
                     k = len(S) // n
@@ -171,17 +185,91 @@ class Prover:
 
     def run_steps(self, rd):
         for st in rd.steps:
-            if st.kind != B.GRAND_PRODUCT:
+            if st.kind == B.GRAND_PRODUCT:
+                self.grand_product(st)
+            elif st.kind == B.LOOKUP_MULTIPLICITY:
+                self.lookup_multiplicities(st)
+            elif st.kind == B.CYCLIC_LOGUP:
+                self.cyclic_logup(st)
+            else:
                 raise NotImplementedError(st.kind)
-            n = self.prog.modules[st.module].N
-            num = eval_expr(st.ins[0], lambda x: self.leaf_values_lagrange(x, n), n)
-            den = eval_expr(st.ins[1], lambda x: self.leaf_values_lagrange(x, n), n)
-            ratio = F.e6_mul(num, F.e6_inv(den))
-            gp = np.zeros((n, 6), dtype=np.uint64)
-            gp[0, 0] = 1
-            for i in range(1, n):
-                gp[i] = F.e6_mul(gp[i - 1], ratio[i - 1])
-            self.ext[st.outs[0]] = gp
+
+    def grand_product(self, st):
+        n = self.prog.modules[st.module].N
+        num = eval_expr(st.ins[0], lambda x: self.leaf_values_lagrange(x, n), n)
+        den = eval_expr(st.ins[1], lambda x: self.leaf_values_lagrange(x, n), n)
+        ratio = F.e6_mul(num, F.e6_inv(den))
+        gp = np.zeros((n, 6), dtype=np.uint64)
+        gp[0, 0] = 1
+        for i in range(1, n):
+            gp[i] = F.e6_mul(gp[i - 1], ratio[i - 1])
+        self.ext[st.outs[0]] = gp
+
+    def _eval_base(self, e, n):
+        v = eval_expr(e, lambda x: self.leaf_values_lagrange(x, n), n)
+        if np.any(v[:, 1:]):
+            raise ValueError("lookup tuple entry is not a base-field expression")
+        return v[:, 0]
+
+    def lookup_multiplicities(self, st):
+        c = st.ctx
+        ins = list(st.ins)
+        sel_s = sel_t = None
+        if c["has_sel_s"]:
+            sel_s, ins = ins[:c["n_sources"]], ins[c["n_sources"]:]
+        if c["has_sel_t"]:
+            sel_t, ins = ins[:c["n_targets"]], ins[c["n_targets"]:]
+        w = c["width"]
+        S = [ins[j * w:(j + 1) * w] for j in range(c["n_sources"])]
+        ins = ins[c["n_sources"] * w:]
+        T = [ins[k * w:(k + 1) * w] for k in range(c["n_targets"])]
+        weights = {}
+        order = []
+        for j, tup in enumerate(S):
+            n = self.prog.modules[c["source_modules"][j]].N
+            cols = [self._eval_base(e, n) for e in tup]
+            sel = self._eval_base(sel_s[j], n) if sel_s else np.ones(n, dtype=np.uint64)
+            for i in range(n):
+                if sel[i] == 0:
+                    continue
+                key = tuple(int(col[i]) for col in cols)
+                if key not in weights:
+                    order.append(key)
+                    weights[key] = 0
+                weights[key] = (weights[key] + int(sel[i])) % P
+        nt = self.prog.modules[st.module].N
+        assigned = set()
+        for k, tup in enumerate(T):
+            cols = [self._eval_base(e, nt) for e in tup]
+            sel = self._eval_base(sel_t[k], nt) if sel_t else np.ones(nt, dtype=np.uint64)
+            res = np.zeros(nt, dtype=np.uint64)
+            for i in range(nt):
+                if sel[i] == 0:
+                    continue
+                key = tuple(int(col[i]) for col in cols)
+                if key not in weights or key in assigned:
+                    continue
+                res[i] = weights[key] * F.inv(int(sel[i])) % P
+                assigned.add(key)
+            self.base[st.outs[k]] = res
+        for key in order:
+            if key not in assigned and weights[key] != 0:
+                raise ValueError("lookup: source tuple %s not found in any target" % (key,))
+
+    def cyclic_logup(self, st):
+        n = self.prog.modules[st.module].N
+        acc = np.zeros((n, 6), dtype=np.uint64)
+        for j in range(st.ctx["n_terms"]):
+            e = eval_expr(st.ins[2 * j], lambda x: self.leaf_values_lagrange(x, n), n)
+            m = eval_expr(st.ins[2 * j + 1], lambda x: self.leaf_values_lagrange(x, n), n)
+            acc = F.e6_add(acc, F.e6_mul(m, F.e6_inv(e)))
+        sums = np.cumsum(acc, axis=0, dtype=np.uint64) % U64(P)  # n < 2^23 terms below 2^31: no overflow
+        total = sums[n - 1].copy()
+        avg = F.e6_scale(total[None, :], F.inv(n))[0]
+        ramp = F.e6_scale(np.broadcast_to(avg, (n, 6)), np.arange(1, n + 1, dtype=np.uint64) % U64(P))
+        self.ext[st.outs[0]] = F.e6_sub(sums, ramp)
+        self.exposed[st.outs[1]] = total
+        self.exposed_avg[st.outs[1]] = avg
 
     def group_columns(self, g, air=False):
         if air:
@@ -223,6 +311,8 @@ class Prover:
         def leaf(x):
             if x.type == E.CHALLENGE:
                 return np.broadcast_to(self.ext[x.name][0], (big, 6)).copy()
+            if x.type == E.VERIFIER and x.hook == E.EXPOSED_AVERAGE_HOOK:
+                return np.broadcast_to(self.exposed_avg[x.name], (big, 6)).copy()
             v = coset_vals(x.name, x.name in self.ext)
             if x.shift:
                 v = np.roll(v, -rho * x.shift, axis=0)
@@ -277,8 +367,13 @@ class Prover:
             if r in lay.trace_tree_of_round:
                 root = self.commit_tree(lay.trace_tree_of_round[r])
                 self.ts.bind(cn, root)
-            if rd.fshook != B.NO_FS_HOOK:
-                raise NotImplementedError("exposed values")
+            if rd.fshook == B.BIND_EXPOSED_VALUES:
+                # ExposedValueElements: [index, 6 limbs] per entry (one entry, index 0)
+                for st in rd.steps:
+                    for name in st.exposed_outputs():
+                        self.ts.bind(cn, [0] + [int(x) for x in self.exposed[name]])
+            elif rd.fshook != B.NO_FS_HOOK:
+                raise NotImplementedError("FS hook %d" % rd.fshook)
             v = self.compute(cn)
             self.ext[cn] = np.array([v[:6]], dtype=np.uint64)
         # AIR
@@ -367,7 +462,7 @@ class Prover:
         samplings = []
         for s in fri["positions"]:
             samplings.append([self.open_committed(c, s) for c in self.committed])
-        return dict(hash_id="poseidon2", commitments=self.commitments, claimed=claimed,
+        return dict(hash_id="poseidon2", exposed=self.exposed, commitments=self.commitments, claimed=claimed,
                     fri=fri, samplings=samplings)
 
     # -- FRI
@@ -525,7 +620,13 @@ def encode_proof(pf):
             e6v(v)
 
     string(pf["hash_id"])
-    u(0)  # exposed values
+    ex = pf.get("exposed", {})
+    u(len(ex))
+    for name in sorted(ex):   # sorted-name order (the proof's map has no order)
+        string(name)
+        u(1)
+        u(0)
+        e6v(ex[name])
     u(len(pf["commitments"]))
     for d in pf["commitments"]:
         dig(d)

@@ -39,6 +39,15 @@ def test_plan_compiles_without_device():
     assert prover.check_plan(w) > 0
 
 
+def test_plan_with_lookups_compiles_without_device():
+    """Multi-module plan with range lookups: multiplicity and cyclic logup
+    steps, exposed totals, the range module."""
+    from loom_amd import workloads, plan, prover
+    prog, _ = workloads.uneven_range_instance([4, 6], 1)
+    w = plan.Plan(prog, 8).words()
+    assert prover.check_plan(w) > 0
+
+
 def test_plan_rejects_garbage():
     from loom_amd import prover, _lib
     with pytest.raises(_lib.LoomGpuError):

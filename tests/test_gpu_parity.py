@@ -228,6 +228,41 @@ def test_prove_uneven_batched(ctx):
         assert np.array_equal(w, enc)
 
 
+@pytest.mark.parametrize("log_ns,nq", [((4, 6, 3), 12), ((10, 12, 9), 20)])
+def test_prove_range_lookups(ctx, log_ns, nq):
+    """Modules with a logup range lookup into the shared range_256 module:
+    multiplicity histograms, cyclic logup columns, exposed totals bound to
+    the transcript and written to the proof."""
+    from loom_amd import workloads, prover
+    log("prove range lookups %s nq=%d" % (log_ns, nq))
+    prog, tr = workloads.uneven_range_instance(list(log_ns), 8)
+    enc, tw = _oracle_words(prog, tr, nq)
+    words, trans = prover.prove(ctx, prog, tr, nq, want_transcript=True)
+    assert np.array_equal(trans, tw), "transcript digests differ"
+    assert np.array_equal(words, enc)
+
+
+def test_prove_range_lookups_batched(ctx):
+    from loom_amd import workloads, prover
+    log("prove range lookups batched 3 x (6, 8)")
+    inst = [workloads.uneven_range_instance([6, 8], 60 + s) for s in range(3)]
+    batch = prover.prove_batch(ctx, inst, 10)
+    for (prog, tr), w in zip(inst, batch):
+        enc, _ = _oracle_words(prog, tr, 10)
+        assert np.array_equal(w, enc)
+
+
+def test_range_lookup_value_outside_table_is_an_error(ctx):
+    from loom_amd import workloads, prover, _lib
+    log("range lookup with a value outside the table")
+    prog, tr = workloads.uneven_range_instance([5], 3)
+    tr = dict(tr)
+    tr["L_0"] = tr["L_0"].copy()
+    tr["L_0"][7] = 300
+    with pytest.raises(_lib.LoomGpuError, match="not in the range table"):
+        prover.prove(ctx, prog, tr, 4)
+
+
 def test_prove_plonk_full_size(ctx):
     """Config-0 shape: N = 2^14, 200 queries."""
     from loom_amd import workloads, prover

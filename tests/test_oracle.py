@@ -441,3 +441,116 @@ def test_claimed_values_equal_horner(small_proof):
             for c in coeffs[::-1]:
                 r = F.e6_add(F.e6_mul(r, z), c)
             assert np.array_equal(r, np.asarray(v, dtype=np.uint64))
+
+
+# ---------------------------------------------------------------- lookups
+def _claimed_by_name(pv, pf):
+    vals = {}
+    for ti, t in enumerate(pv.layout.trees):
+        for gi, g in enumerate(t.groups):
+            gb, ge = pf["claimed"][ti][gi]
+            for nm, ss, vs in list(zip(g.base, g.base_shifts, gb)) + list(zip(g.ext, g.ext_shifts, ge)):
+                for s, v in zip(ss, vs):
+                    vals[(nm, s)] = np.asarray(v, dtype=np.uint64)
+    return vals
+
+
+def _air_residual(prog, pv, vals, mn):
+    """V(zeta) - (zeta^N - 1) * sum_i zeta^{iN} Q_i(zeta) for module mn, with
+    every verifier column evaluated from its definition at zeta."""
+    from loom_amd import expr as E, protocol as PR
+    from oracle import prover as OP
+    zeta = pv.zeta
+    m = prog.modules[mn]
+    n = m.N
+    w = F.omega(n)
+
+    def leaf(x):
+        if x.type == E.CHALLENGE:
+            return pv.ext[x.name][0][None, :]
+        if x.type == E.VERIFIER and x.hook == E.EXPOSED_AVERAGE_HOOK:
+            return pv.exposed_avg[x.name][None, :]
+        if x.type == E.VERIFIER:
+            k = int(x.name.rsplit("_", 1)[1])
+            i = k % n
+            wi = pow(w, i, P)
+            num = F.e6_scale(F.e6_sub(F.e6_pow(zeta, n), F.e6_one()), wi)
+            den = F.e6_scale(F.e6_sub(zeta, F.e6_from_base(np.array([wi], dtype=np.uint64))[0]), n)
+            return F.e6_mul(num, F.e6_inv(den))[None, :]
+        return vals[(x.name, x.shift % n)][None, :]
+
+    ev = OP.eval_expr(m.relation, leaf, 1)[0]
+    acc = F.e6_zero()
+    for i in range(PR.air_chunk_count(m)):
+        q = vals[(PR.quotient_chunk_name(mn, i), 0)]
+        acc = F.e6_add(acc, F.e6_mul(F.e6_pow(zeta, i * n), q))
+    rhs = F.e6_mul(F.e6_sub(F.e6_pow(zeta, n), F.e6_one()), acc)
+    return F.e6_sub(ev, rhs)
+
+
+@pytest.fixture(scope="module")
+def range_proof():
+    from loom_amd import workloads
+    from oracle import prover as OP
+    prog, tr = workloads.uneven_range_instance([4, 6, 3], 9, 256)
+    pv = OP.Prover(prog, tr, 6)
+    pf = pv.prove()
+    return prog, tr, pv, pf
+
+
+def test_range_lookup_multiplicities_brute_force(range_proof):
+    """Each Mult column of the range module counts its source column's values
+    (the range table holds 0..255 once each)."""
+    prog, tr, pv, pf = range_proof
+    mult_steps = [st for st in prog.rounds[0].steps if st.kind == 2]
+    assert len(mult_steps) == 3
+    for st, i in zip(mult_steps, range(3)):
+        want = np.bincount(np.asarray(tr["L_%d" % i], dtype=np.int64), minlength=256)
+        assert np.array_equal(pv.base[st.outs[0]], want.astype(np.uint64))
+
+
+def test_logup_bus_balances_and_air_holds(range_proof):
+    """Every bus of logup totals sums to zero and every module's relation
+    (plonk gates, copy constraint, logup columns with the exposed average)
+    matches its quotient at zeta."""
+    prog, tr, pv, pf = range_proof
+    for bus in prog.logup_bus:
+        s = F.e6_zero()
+        for nm in bus:
+            s = F.e6_add(s, pf["exposed"][nm])
+        assert not np.any(s)
+    vals = _claimed_by_name(pv, pf)
+    for mn in sorted(prog.modules):
+        assert not np.any(_air_residual(prog, pv, vals, mn)), mn
+
+
+def test_logup_planted_wrong_multiplicity_breaks_bus():
+    """A multiplicity off by one in the oracle unbalances the bus."""
+    from loom_amd import workloads
+    from oracle import prover as OP
+    prog, tr = workloads.uneven_range_instance([4], 2, 256)
+    pv = OP.Prover(prog, tr, 4)
+    orig = pv.lookup_multiplicities
+
+    def bad(st):
+        orig(st)
+        c = pv.base[st.outs[0]]
+        c[0] = (c[0] + 1) % P
+
+    pv.lookup_multiplicities = bad
+    pf = pv.prove()
+    s = F.e6_zero()
+    for nm in prog.logup_bus[0]:
+        s = F.e6_add(s, pf["exposed"][nm])
+    assert np.any(s)
+
+
+def test_lookup_rejects_value_outside_table():
+    from loom_amd import workloads
+    from oracle import prover as OP
+    prog, tr = workloads.uneven_range_instance([3], 4, 256)
+    tr = dict(tr)
+    tr["L_0"] = tr["L_0"].copy()
+    tr["L_0"][3] = 300
+    with pytest.raises(ValueError, match="not found"):
+        OP.prove(prog, tr, 4)
