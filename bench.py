@@ -41,6 +41,10 @@ def main():
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle-check", action="store_true")
+    ap.add_argument("--workload", default="plonk40", choices=["plonk40", "uneven64", "wide"],
+                    help="plonk40: the 40 x 2^14 batch (default); uneven64: one proof of 64 plonk-like "
+                         "modules of 2^10..2^20 rows with range lookups; wide: one proof of a 2^22-row, "
+                         "96-column Fibonacci/cubic trace")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 plain device copies of MIB MiB (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -63,6 +67,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "copies": 5}))
         return
     from loom_amd import workloads, prover
+    if args.workload != "plonk40":
+        return single_proof_bench(args, torch, workloads, prover)
 
     B, logn, nq = args.instances, args.logn, args.queries
     print("[bench] building %d instances of 2^%d" % (B, logn), file=sys.stderr, flush=True)
@@ -192,6 +198,59 @@ def main():
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    ctx.close()
+
+
+def single_proof_bench(args, torch, workloads, prover):
+    """One large proof per step (the uneven-module and wide-trace shapes)."""
+    if args.workload == "uneven64":
+        sizes = workloads.uneven_sizes(64, 0, 10, 20)
+        print("[bench] uneven64: module log sizes %s" % sizes, file=sys.stderr, flush=True)
+        prog, tr = workloads.uneven_range_instance(sizes, 0, 256)
+        desc = "one proof: 64 plonk-like modules, log2 N uniform in 10..20 (total %d rows), each with a " \
+               "logup range lookup into range_256, Poseidon2, %d FRI queries" % (sum(1 << x for x in sizes), args.queries)
+    else:
+        print("[bench] wide: generating 2^22 x 96 trace", file=sys.stderr, flush=True)
+        prog, tr = workloads.wide_instance(22, 0)
+        desc = "one proof: N=2^22, 96 base columns (32 Fibonacci pairs, 32 cubic chains), Poseidon2, " \
+               "%d FRI queries" % args.queries
+    item = prover.Prepared(prog, tr, args.queries)
+    del tr
+    ctx = prover.Context(0)
+    dev = prover.DeviceInputs([item], 0)
+    print("[bench] inputs on device", file=sys.stderr, flush=True)
+    for _ in range(max(1, args.warmup)):
+        first = prover.prove_prepared(ctx, [item], dev_inputs=dev)[0]
+    torch.cuda.synchronize()
+    print("[bench] warmup done", file=sys.stderr, flush=True)
+    lat = []
+    same = True
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        w = prover.prove_prepared(ctx, [item], dev_inputs=dev)[0]
+        lat.append((time.perf_counter() - a) * 1e3)
+        same = same and np.array_equal(w, first)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print("[bench] timed steps done", file=sys.stderr, flush=True)
+    ctx.set_timing(True)
+    prover.prove_prepared(ctx, [item], dev_inputs=dev)
+    ctx.set_timing(False)
+    stats = ctx.timing_report()
+    tot_ms = sum(v[1] for v in stats.values())
+    kernels = {k: {"launches": v[0], "ms": round(v[1], 3), "share": round(v[1] / tot_ms, 4),
+                   "GB_per_s": round(v[2] / (v[1] * 1e-3) / 1e9, 1) if v[2] else None}
+               for k, v in sorted(stats.items(), key=lambda kv: -kv[1][1])}
+    line = {
+        "metric": "proofs/s (single large proof)", "value": round(args.steps / dt, 4), "unit": "proofs/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 2),
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "u32 (KoalaBear Montgomery, E6)", "data": "synthetic seeded",
+        "config": {"workload": desc}, "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+        "deterministic": same, "kernel_ms_total": round(tot_ms, 2), "kernels": kernels,
     }
     print(json.dumps(line))
     ctx.close()

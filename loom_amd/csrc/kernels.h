@@ -44,9 +44,11 @@ struct NttCfg {
 void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
                 const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s);
 
-// Two-pass (four-step) transform for 2^15 <= n <= 2^22: out = post * DFT(pre * in)
-// (natural input; natural or bit-reversed output); tmp: n-word scratch per
-// job.  tw: w^k (k < n/2) of the direction, wn: w^k (k < n) of the direction.
+// Two-pass (four-step) transform for 2^15 <= n <= 2^22, n = n1 n2 with
+// n1 = 2^ceil(logn/2): out = post * DFT(pre * in) (natural input; natural or
+// bit-reversed output); tmp: n-word scratch per job.  tw_a / tw_b: w^k
+// (k < size/2) of the size-n2 / size-n1 sub-transforms; wlo: w_n^k (k < 2048),
+// whi: w_n^{2048 k} (k < n/2048); all in the direction of the transform.
 struct BigJob {
   const uint32_t* in;
   uint32_t* out;
@@ -54,8 +56,8 @@ struct BigJob {
   const uint32_t* pre;
   const uint32_t* post;
 };
-void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw,
-                    const uint32_t* wn, int bitrev_out, hipStream_t s);
+void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw_a,
+                    const uint32_t* tw_b, const uint32_t* wlo, const uint32_t* whi, int bitrev_out, hipStream_t s);
 
 // Leaf hashing: for instance b (grid y) and pair leaf k, hashes the message
 // [LEAF, hdr1, hdr2, ...] whose element m >= 3 is column msgmap[m] >> 1 at row

@@ -181,11 +181,13 @@ __device__ __forceinline__ void lds_dit_multi(uint32_t* sh, int logm, int C, con
   }
 }
 
-// tw: twiddles of the size-n transform (w_n^k, k < n/2), used with strides
-// for the sub-transforms; wn: w_n^k for k < n (pass A twiddle), direction
-// matching tw.
+// tw_a: twiddles of the size-n2 sub-transforms (w_{n2}^k, k < n2/2); the
+// pass-A twiddle w_n^e (e = j1 k2 < n) is wlo[e mod 2048] * whi[e / 2048]
+// (wlo: w_n^k, whi: w_n^{2048 k}), two small tables instead of one of n
+// entries.  All tables in the direction of the transform.
 __global__ __launch_bounds__(256) void k_ntt_pass_a(const BigJob* __restrict__ jobs, int a, int b,
-                                                    const uint32_t* __restrict__ tw, const uint32_t* __restrict__ wn) {
+                                                    const uint32_t* __restrict__ tw_a, const uint32_t* __restrict__ wlo,
+                                                    const uint32_t* __restrict__ whi) {
   extern __shared__ uint32_t sh[];
   const BigJob* job = jobs + blockIdx.y;
   const int n1 = 1 << a, n2 = 1 << b;
@@ -201,20 +203,19 @@ __global__ __launch_bounds__(256) void k_ntt_pass_a(const BigJob* __restrict__ j
     sh[c * n2 + brev(j2, b)] = v;
   }
   __syncthreads();
-  // size-n2 sub-transforms: w_{n2} = w_n^{n1}
-  lds_dit_multi(sh, b, BIG_C, tw, a);
-  const uint32_t nmask = (1u << (a + b)) - 1;
+  lds_dit_multi(sh, b, BIG_C, tw_a, 0);
   uint32_t* __restrict__ tmp = job->tmp;
   for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
     const int c = idx & (BIG_C - 1), k2 = idx >> 3;
     const uint32_t j1 = j10 + c;
-    const uint32_t v = kb_mul(sh[c * n2 + k2], wn[(j1 * (uint32_t)k2) & nmask]);
-    tmp[(size_t)k2 * n1 + j1] = v;
+    const uint32_t e = j1 * (uint32_t)k2;
+    const uint32_t w = kb_mul(wlo[e & 2047u], whi[e >> 11]);
+    tmp[(size_t)k2 * n1 + j1] = kb_mul(sh[c * n2 + k2], w);
   }
 }
 
 __global__ __launch_bounds__(256) void k_ntt_pass_b(const BigJob* __restrict__ jobs, int a, int b,
-                                                    const uint32_t* __restrict__ tw, int bitrev_out) {
+                                                    const uint32_t* __restrict__ tw_b, int bitrev_out) {
   extern __shared__ uint32_t sh[];
   const BigJob* job = jobs + blockIdx.y;
   const int n1 = 1 << a, n2 = 1 << b;
@@ -226,8 +227,7 @@ __global__ __launch_bounds__(256) void k_ntt_pass_b(const BigJob* __restrict__ j
     sh[c * n1 + brev(j1, a)] = tmp[(size_t)(k20 + c) * n1 + j1];
   }
   __syncthreads();
-  // size-n1 sub-transforms: w_{n1} = w_n^{n2}
-  lds_dit_multi(sh, a, BIG_C, tw, b);
+  lds_dit_multi(sh, a, BIG_C, tw_b, 0);
   uint32_t* __restrict__ out = job->out;
   const uint32_t* __restrict__ post = job->post;
   if (bitrev_out) {
@@ -250,17 +250,17 @@ __global__ __launch_bounds__(256) void k_ntt_pass_b(const BigJob* __restrict__ j
   }
 }
 
-void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw,
-                    const uint32_t* wn, int bitrev_out, hipStream_t s) {
+void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw_a,
+                    const uint32_t* tw_b, const uint32_t* wlo, const uint32_t* whi, int bitrev_out, hipStream_t s) {
   if (n_jobs == 0) return;
   const int a = (logn + 1) / 2, b = logn - a;  // n1 = 2^a >= n2 = 2^b
-  if (a > 11 || b < 3) throw std::runtime_error("launch_ntt_big: unsupported size");
+  if (a > 11 || b < 3 || logn < 12) throw std::runtime_error("launch_ntt_big: unsupported size");
   prelaunch();
   hipLaunchKernelGGL(k_ntt_pass_a, dim3((1 << a) / BIG_C, n_jobs), dim3(256), (size_t)BIG_C * (1 << b) * 4, s, jobs_a,
-                     a, b, tw, wn);
+                     a, b, tw_a, wlo, whi);
   LG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_ntt_pass_b, dim3((1 << b) / BIG_C, n_jobs), dim3(256), (size_t)BIG_C * (1 << a) * 4, s, jobs_b,
-                     a, b, tw, bitrev_out);
+                     a, b, tw_b, bitrev_out);
   LG_HIP(hipGetLastError());
 }
 

@@ -610,6 +610,7 @@ void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hip
   // per job an n-word transpose buffer, plus an n-word buffer for the
   // inverse result when forward outputs follow.  Groups bound the scratch.
   const size_t n = (size_t)1 << logn;
+  const int la = (logn + 1) / 2, lb = logn - la;  // sub-transform sizes 2^la (pass B) and 2^lb (pass A)
   const bool need_x = cfg.inverse_first && cfg.n_out > 0;
   const size_t per_job = n * (need_x ? 2 : 1);
   const size_t max_words = (size_t)256 << 20;
@@ -626,7 +627,7 @@ void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hip
         bj[k] = BigJob{J.in, need_x ? xbuf(k) : J.out[0], tmp(k), nullptr, J.post};
       }
       const BigJob* d = static_cast<const BigJob*>(stage(bj.data(), g * sizeof(BigJob)));
-      launch_ntt_big(d, d, (int)g, logn, tb.tw_inv(logn), tb.wpow_inv(logn), 0, s);
+      launch_ntt_big(d, d, (int)g, logn, tb.tw_inv(lb), tb.tw_inv(la), tb.wpow_inv(logn), tb.wpow_inv(logn - 11), 0, s);
     }
     for (int o = 0; o < cfg.n_out; ++o) {
       for (size_t k = 0; k < g; ++k) {
@@ -634,7 +635,8 @@ void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hip
         bj[k] = BigJob{cfg.inverse_first ? xbuf(k) : J.in, J.out[o], tmp(k), cfg.pre[o], nullptr};
       }
       const BigJob* d = static_cast<const BigJob*>(stage(bj.data(), g * sizeof(BigJob)));
-      launch_ntt_big(d, d, (int)g, logn, tb.tw_fwd(logn), tb.wpow(logn), cfg.bitrev_out[o], s);
+      launch_ntt_big(d, d, (int)g, logn, tb.tw_fwd(lb), tb.tw_fwd(la), tb.wpow(logn), tb.wpow(logn - 11),
+                     cfg.bitrev_out[o], s);
     }
   }
 }

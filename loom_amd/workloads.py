@@ -132,3 +132,59 @@ def uneven_sizes(count, seed, lo=10, hi=20):
     """log2 sizes drawn uniformly from [lo, hi] (PCG64([seed, 1 << 20]))."""
     rng = np.random.Generator(np.random.PCG64([seed, 1 << 20]))
     return [int(x) for x in rng.integers(lo, hi + 1, size=count)]
+
+
+def _fibonacci_table(n):
+    """F_0 .. F_n mod p by block doubling: with F_0 .. F_{m+1} known,
+    F_{m+1+j} = F_{m+1} F_{j+1} + F_m F_j for j = 0 .. m."""
+    pp = np.uint64(P)
+    F = np.zeros(2 * (n + 2), dtype=np.uint64)
+    F[1] = F[2] = 1
+    m = 1
+    while m + 1 < n:
+        j = np.arange(m + 1, dtype=np.int64)
+        F[m + 1 + j] = (F[m + 1] * F[j + 1] % pp + F[m] * F[j] % pp) % pp
+        m *= 2
+    return F[:n + 1].copy()
+
+
+def wide_instance(log_n, seed, n_fib=32, n_cubic=32):
+    """The wide single-module trace: N = 2^log_n rows, 2*n_fib + n_cubic base
+    columns.  Pair k: u_{i+1} = v_i, v_{i+1} = u_i + v_i; chain k:
+    w_{i+1} = w_i^3 + c_k (c_k uniform in [0, p)); seeds uniform in [0, p).
+    Every transition is asserted except at the last row
+    (AssertZeroExceptAt(N-1)), so the folded relation has degree 4."""
+    n = 1 << log_n
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pp = np.uint64(P)
+    u0 = rng.integers(0, P, size=n_fib, dtype=np.uint64)
+    v0 = rng.integers(0, P, size=n_fib, dtype=np.uint64)
+    w0 = rng.integers(0, P, size=n_cubic, dtype=np.uint64)
+    cs = rng.integers(0, P, size=n_cubic, dtype=np.uint64)
+    F = _fibonacci_table(n)
+    fprev = np.concatenate(([np.uint64(1)], F[:n - 1]))  # F_{i-1}, with F_{-1} = 1
+    fcur = F[:n]
+    fnext = F[1:n + 1]
+    trace = {}
+    for k in range(n_fib):
+        # (u_i, v_i) = M^i (u0, v0), M^i = [[F_{i-1}, F_i], [F_i, F_{i+1}]]
+        trace["U_%d" % k] = (fprev * u0[k] % pp + fcur * v0[k] % pp) % pp
+        trace["V_%d" % k] = (fcur * u0[k] % pp + fnext * v0[k] % pp) % pp
+    W = np.empty((n, n_cubic), dtype=np.uint64)
+    w = w0.copy()
+    for i in range(n):
+        W[i] = w
+        w = (w * w % pp * w + cs) % pp
+    for k in range(n_cubic):
+        trace["W_%d" % k] = np.ascontiguousarray(W[:, k])
+    b = B.Builder()
+    m = B.Module("wide", n)
+    for k in range(n_fib):
+        u, v = E.Col("U_%d" % k), E.Col("V_%d" % k)
+        m.assert_zero_except_at(E.Col("U_%d" % k, 1) - v, n - 1)
+        m.assert_zero_except_at(E.Col("V_%d" % k, 1) - u - v, n - 1)
+    for k in range(n_cubic):
+        wk = E.Col("W_%d" % k)
+        m.assert_zero_except_at(E.Col("W_%d" % k, 1) - wk * wk * wk - E.Const(int(cs[k])), n - 1)
+    b.add_module(m)
+    return B.compile_program(b), trace

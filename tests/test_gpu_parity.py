@@ -263,6 +263,19 @@ def test_range_lookup_value_outside_table_is_an_error(ctx):
         prover.prove(ctx, prog, tr, 4)
 
 
+@pytest.mark.parametrize("logn,nf,nc,nq", [(10, 4, 4, 16), (15, 3, 2, 12)])
+def test_prove_wide(ctx, logn, nf, nc, nq):
+    """The wide Fibonacci/cubic trace (shifted columns, AssertZeroExceptAt
+    through a Lagrange column, one round, first FRI round at 22 - log N)."""
+    from loom_amd import workloads, prover
+    log("prove wide logn=%d fib=%d cubic=%d nq=%d" % (logn, nf, nc, nq))
+    prog, tr = workloads.wide_instance(logn, 3, nf, nc)
+    enc, tw = _oracle_words(prog, tr, nq)
+    words, trans = prover.prove(ctx, prog, tr, nq, want_transcript=True)
+    assert np.array_equal(trans, tw), "transcript digests differ"
+    assert np.array_equal(words, enc)
+
+
 def test_prove_plonk_full_size(ctx):
     """Config-0 shape: N = 2^14, 200 queries."""
     from loom_amd import workloads, prover
