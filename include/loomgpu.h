@@ -107,6 +107,39 @@ int lg_ctx_timing_report(lg_ctx* ctx, char* buf, size_t cap, size_t* len);
  * reports, prover.go:682-716). */
 int lg_plan_last_phases(const lg_plan* plan, double* out3);
 
+/* ---- serving ---- */
+
+/* A serving loop on one device (no counterpart in the reference, whose
+ * bench proves sequentially; it is the request loop a prover service puts in
+ * front of prover.Prove).  `slots` worker threads (1..16) each own a context
+ * and stream.  A request is admitted, oldest first, when a slot is free and
+ * its HBM footprint (lg_plan_bytes_per_instance x batch) fits within
+ * hbm_budget_bytes beside the running requests (0: 80 % of the free device
+ * memory at creation); a request larger than the budget runs alone.  Queued
+ * requests of the same plan join the prove of the one admitted, up to
+ * max_batch instances (continuous batching); every proof equals the one
+ * lg_prove gives for that instance alone. */
+typedef struct lg_server lg_server;
+int lg_server_create(int device, int slots, uint64_t hbm_budget_bytes, int max_batch, lg_server** out);
+/* Stops the workers after their current prove; queued requests fail. */
+void lg_server_destroy(lg_server* server);
+/* Queues `batch` instances of the plan (inputs as for lg_prove, host memory,
+ * copied before return); *ticket identifies the request. */
+int lg_server_submit(lg_server* server, const uint32_t* plan_words, size_t n_words, int batch,
+                     const uint32_t* const* columns, const uint32_t* const* perms, uint64_t* ticket);
+/* Waits up to timeout_ms (< 0: no limit) for a request.  Returns 0 with the
+ * proofs copied out (the ticket is then released), 1 on timeout, -2 if a
+ * buffer is too small (lengths reported, the result is kept for another
+ * call), -1 if the request failed (message in lg_last_error). */
+int lg_server_wait(lg_server* server, uint64_t ticket, int timeout_ms, uint32_t* const* proofs,
+                   const size_t* proof_caps, size_t* proof_lens);
+/* out[0..n): submitted, completed, failed, queued, running, peak running,
+ * peak admitted bytes, proves launched, requests merged into another's
+ * prove, budget bytes. */
+int lg_server_stats(lg_server* server, uint64_t* out, size_t n);
+/* Admission footprint estimate of one instance of a plan (no device). */
+int lg_plan_bytes_per_instance(const uint32_t* words, size_t n_words, uint64_t* bytes);
+
 /* ---- building blocks (device pointers), the minimum slice of the path ---- */
 
 /* Poseidon2 row-pair leaf hashing of one committed group

@@ -31,6 +31,14 @@ EXPORTS = {
     "lg_merkle_build": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, u32pp, sizep, ctypes.c_int]),
     "lg_grand_product": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_size_t]),
     "lg_rs_encode": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, ctypes.c_int, u32p]),
+    "lg_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "lg_server_destroy": (None, [ctypes.c_void_p]),
+    "lg_server_submit": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, ctypes.c_int, u32pp, u32pp,
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "lg_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int, u32pp, sizep, sizep]),
+    "lg_server_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64), ctypes.c_size_t]),
+    "lg_plan_bytes_per_instance": (ctypes.c_int, [u32p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint64)]),
 }
 
 _lib = None
@@ -46,6 +54,10 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise LoomGpuError("libloomgpu.so is not built (run `python -m loom_amd.build`); "
                                "the prove path has no CPU fallback")
+        # PyTorch ships its own HIP runtime (same soname as /opt/rocm's).  It
+        # must be the one in the process before the library loads, so that
+        # the library binds to it instead of pulling in a second runtime.
+        import torch  # noqa: F401
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in EXPORTS.items():
             f = getattr(L, name)

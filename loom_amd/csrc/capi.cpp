@@ -14,10 +14,15 @@
 #include "p2.h"
 #include "pool.h"
 #include "prover.h"
+#include "server.h"
 
 struct lg_ctx {
   lg::Context c;
   std::mutex mu;
+};
+
+struct lg_server {
+  std::unique_ptr<lg::Server> s;
 };
 
 struct lg_plan {
@@ -345,6 +350,77 @@ int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t
           return held.back().as<uint32_t>();
         });
     LG_HIP(hipStreamSynchronize(ctx->c.stream));
+    return 0;
+  });
+}
+
+int lg_server_create(int device, int slots, uint64_t hbm_budget_bytes, int max_batch, lg_server** out) {
+  return guarded([&] {
+    if (!out) throw std::runtime_error("lg_server_create: null out");
+    auto* s = new lg_server();
+    try {
+      s->s = std::make_unique<lg::Server>(device, slots, (size_t)hbm_budget_bytes, max_batch);
+    } catch (...) {
+      delete s;
+      throw;
+    }
+    *out = s;
+    return 0;
+  });
+}
+
+void lg_server_destroy(lg_server* s) { delete s; }
+
+int lg_server_submit(lg_server* s, const uint32_t* plan_words, size_t n_words, int batch,
+                     const uint32_t* const* columns, const uint32_t* const* perms, uint64_t* ticket) {
+  return guarded([&] {
+    if (!s || !ticket) throw std::runtime_error("lg_server_submit: null argument");
+    *ticket = s->s->submit(plan_words, n_words, batch, columns, perms);
+    return 0;
+  });
+}
+
+int lg_server_wait(lg_server* s, uint64_t ticket, int timeout_ms, uint32_t* const* proofs, const size_t* proof_caps,
+                   size_t* proof_lens) {
+  return guarded([&] {
+    if (!s) throw std::runtime_error("lg_server_wait: null server");
+    std::vector<lg::ProveOutput> out;
+    std::string err;
+    if (!s->s->wait(ticket, timeout_ms, &out, &err, true)) return 1;
+    if (!err.empty()) {
+      s->s->wait(ticket, 0, nullptr, nullptr, false);
+      throw std::runtime_error("request failed: " + err);
+    }
+    int rc = 0;
+    for (size_t b = 0; b < out.size(); ++b) {
+      if (proof_lens) proof_lens[b] = out[b].proof.size();
+      if (!proofs || !proof_caps || out[b].proof.size() > proof_caps[b]) rc = -2;
+    }
+    if (rc) {
+      g_err = "lg_server_wait: output buffer too small";
+      return rc;
+    }
+    for (size_t b = 0; b < out.size(); ++b) std::memcpy(proofs[b], out[b].proof.data(), out[b].proof.size() * 4);
+    s->s->wait(ticket, 0, nullptr, nullptr, false);
+    return 0;
+  });
+}
+
+int lg_server_stats(lg_server* s, uint64_t* out, size_t n) {
+  return guarded([&] {
+    if (!s || !out) throw std::runtime_error("lg_server_stats: null argument");
+    lg::Server::Stats st = s->s->stats();
+    const uint64_t v[10] = {st.submitted, st.completed, st.failed, st.queued, st.running,
+                            st.peak_running, st.peak_bytes, st.proves, st.merged, (uint64_t)s->s->budget()};
+    for (size_t i = 0; i < n && i < 10; ++i) out[i] = v[i];
+    return 0;
+  });
+}
+
+int lg_plan_bytes_per_instance(const uint32_t* words, size_t n_words, uint64_t* bytes) {
+  return guarded([&] {
+    if (!words || !bytes) throw std::runtime_error("lg_plan_bytes_per_instance: null argument");
+    *bytes = lg::plan_bytes_per_instance(lg::parse_plan(words, n_words));
     return 0;
   });
 }

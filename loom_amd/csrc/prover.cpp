@@ -188,15 +188,26 @@ DevBuf Pool::get(size_t bytes) {
   if (it != free_.end()) {
     DevBuf b = std::move(it->second);
     free_.erase(it);
+    cached_ -= bytes;
     return b;
   }
   return DevBuf(bytes);
+}
+
+void Pool::trim(size_t keep_bytes) {
+  std::lock_guard<std::mutex> lock(mu_);
+  while (cached_ > keep_bytes && !free_.empty()) {
+    auto it = std::prev(free_.end());
+    cached_ -= it->first;
+    free_.erase(it);
+  }
 }
 
 void Pool::put(DevBuf&& b) {
   if (!b.bytes()) return;
   std::lock_guard<std::mutex> lock(mu_);
   size_t n = b.bytes();
+  cached_ += n;
   free_.emplace(n, std::move(b));
 }
 

@@ -53,11 +53,14 @@ class Pool {
  public:
   DevBuf get(size_t bytes);
   void put(DevBuf&& b);
+  // Frees cached buffers, largest first, until at most keep_bytes remain.
+  void trim(size_t keep_bytes);
   ~Pool();
 
  private:
   std::mutex mu_;
   std::multimap<size_t, DevBuf> free_;
+  size_t cached_ = 0;
 };
 
 // Pinned host staging + matching device area: small tables (pointer arrays,
