@@ -41,7 +41,10 @@ def main():
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle-check", action="store_true")
-    ap.add_argument("--workload", default="plonk40", choices=["plonk40", "uneven64", "wide"],
+    ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
+    ap.add_argument("--qps", type=float, default=40.0, help="serve: Poisson arrival rate")
+    ap.add_argument("--slots", type=int, default=3, help="serve: concurrent proving slots")
+    ap.add_argument("--workload", default="plonk40", choices=["plonk40", "uneven64", "wide", "serve"],
                     help="plonk40: the 40 x 2^14 batch (default); uneven64: one proof of 64 plonk-like "
                          "modules of 2^10..2^20 rows with range lookups; wide: one proof of a 2^22-row, "
                          "96-column Fibonacci/cubic trace")
@@ -67,6 +70,8 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": n * 4, "copy_probe_bytes_written": n * 4, "copies": 5}))
         return
     from loom_amd import workloads, prover
+    if args.workload == "serve":
+        return serve_bench(args, torch, workloads, prover)
     if args.workload != "plonk40":
         return single_proof_bench(args, torch, workloads, prover)
 
@@ -201,6 +206,91 @@ def main():
     }
     print(json.dumps(line))
     ctx.close()
+
+
+def serve_bench(args, torch, workloads, prover):
+    """Serving: a Poisson stream of requests, each 1-40 plonk-like instances of
+    one size, log2 N geometric on 12..20 (P(12) = 1/2, halving per step),
+    against serial plain proving of the same requests (one instance per call,
+    in arrival order; its latencies are those of a FIFO queue in front of it)."""
+    from loom_amd import serve
+    rng = np.random.Generator(np.random.PCG64(12345))
+    sizes = list(range(12, 21))
+    probs = np.array([2.0 ** -(k + 1) for k in range(len(sizes))])
+    probs /= probs.sum()
+    pool = {}
+    for ln in sizes:  # a few distinct instances per size, drawn by the requests
+        cnt = 6 if ln <= 16 else 2
+        pool[ln] = [prover.Prepared(*workloads.plonk_instance(ln, 1000 * ln + s), args.queries) for s in range(cnt)]
+    print("[bench] serve: instance pool ready", file=sys.stderr, flush=True)
+    reqs = []
+    t = 0.0
+    for _ in range(args.requests):
+        t += rng.exponential(1.0 / args.qps)
+        ln = int(rng.choice(sizes, p=probs))
+        k = int(rng.integers(1, 41))
+        reqs.append((t, ln, [pool[ln][int(rng.integers(0, len(pool[ln])))] for _ in range(k)]))
+    n_inst = sum(len(r[2]) for r in reqs)
+    ctx = prover.Context(0)
+    # warm every plan on the plain context and on the server slots
+    for ln in sizes:
+        prover.prove_prepared(ctx, pool[ln][:1])
+    srv = serve.Server(0, slots=args.slots, max_batch=64)
+    for ln in sizes:
+        srv.wait(srv.submit(pool[ln][:1]))
+    print("[bench] serve: warm, %d requests, %d instances" % (len(reqs), n_inst), file=sys.stderr, flush=True)
+    # serving run
+    pending = {}
+    lat = []
+    results = {}
+    t0 = time.perf_counter()
+    i = 0
+    while i < len(reqs) or pending:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i][0] <= now:
+            pending[srv.submit(reqs[i][2])] = (i, reqs[i][0])
+            i += 1
+        for tk in list(pending):
+            got = srv.wait(tk, 0)
+            if got is not None:
+                ri, ta = pending.pop(tk)
+                lat.append((time.perf_counter() - t0 - ta) * 1e3)
+                results[ri] = got
+        time.sleep(0.0005)
+    serve_s = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    print("[bench] serve: served in %.2fs" % serve_s, file=sys.stderr, flush=True)
+    # serial plain proving of the same requests
+    service = []
+    same = True
+    for ri, (ta, ln, items) in enumerate(reqs):
+        a = time.perf_counter()
+        for j, it in enumerate(items):
+            w = prover.prove_prepared(ctx, [it])[0]
+            same = same and np.array_equal(w, results[ri][j])
+        service.append(time.perf_counter() - a)
+    serial_s = sum(service)
+    fin = 0.0
+    slat = []
+    for (ta, _, _), sv in zip(reqs, service):
+        fin = max(fin, ta) + sv
+        slat.append((fin - ta) * 1e3)
+    ctx.close()
+    line = {
+        "metric": "serving: proofs/s and p50/p99 request latency vs serial plain GPU proving",
+        "value": round(n_inst / serve_s, 2), "unit": "proofs/s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "higher_is_better": True, "scaling": "strong", "vs_baseline": None,
+        "dtype": "u32 (KoalaBear Montgomery, E6)", "data": "synthetic seeded",
+        "config": {"workload": "%d requests, Poisson arrivals at %.1f req/s, 1-40 plonk-like instances each, "
+                               "log2 N geometric on 12..20, %d FRI queries, %d slots"
+                               % (len(reqs), args.qps, args.queries, args.slots)},
+        "instances": n_inst, "served_s": round(serve_s, 3), "p50_ms": round(pct(lat, 50), 2),
+        "p99_ms": round(pct(lat, 99), 2), "serial_plain_proofs_per_s": round(n_inst / serial_s, 2),
+        "serial_plain_p50_ms": round(pct(slat, 50), 2), "serial_plain_p99_ms": round(pct(slat, 99), 2),
+        "bit_exact_served_vs_plain": same, "server_stats": st,
+    }
+    print(json.dumps(line))
 
 
 def single_proof_bench(args, torch, workloads, prover):
