@@ -34,8 +34,8 @@ def pct(xs, q):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--instances", type=int, default=40)
     ap.add_argument("--logn", type=int, default=14)
     ap.add_argument("--queries", type=int, default=200)

@@ -41,7 +41,10 @@ LG_HD uint32_t kb_neg(uint32_t a) { return a ? KB_P - a : 0u; }
 // Montgomery product a*b*2^-32 mod p for a, b < p.
 LG_HD uint32_t kb_mul(uint32_t a, uint32_t b) {
   uint64_t t = (uint64_t)a * b;
-  uint32_t m = (uint32_t)t * KB_PINV;
+  // m = t_lo * p^-1 mod 2^32 with p^-1 = 2^31 + 2^24 + 1: shifts and adds
+  // instead of a 32-bit multiply
+  const uint32_t lo = (uint32_t)t;
+  uint32_t m = lo + (lo << 24) + (lo << 31);
   uint32_t th = (uint32_t)(t >> 32);
   uint32_t mh = (uint32_t)(((uint64_t)m * KB_P) >> 32);
   uint32_t r = th - mh;
@@ -49,6 +52,18 @@ LG_HD uint32_t kb_mul(uint32_t a, uint32_t b) {
 }
 
 LG_HD uint32_t kb_dbl(uint32_t a) { return kb_add(a, a); }
+
+// a * 2^-K mod p (a < p, 1 <= K <= 24), on the Montgomery representation
+// directly (the map is linear): with m = -a mod 2^K, a + m p is divisible by
+// 2^K, and since p = 2^31 - 2^24 + 1 the quotient is
+// (a + m) / 2^K + m 2^(31-K) - m 2^(24-K), below 2p.
+template <int K>
+LG_HD uint32_t kb_div_pow2(uint32_t a) {
+  static_assert(K >= 1 && K <= 24, "K out of range");
+  const uint32_t m = (0u - a) & ((1u << K) - 1u);
+  const uint32_t y = ((a + m) >> K) + (m << (31 - K)) - (m << (24 - K));
+  return y >= KB_P ? y - KB_P : y;
+}
 
 LG_HD uint32_t kb_to_mont(uint32_t x) { return kb_mul(x % KB_P, KB_R2); }
 LG_HD uint32_t kb_from_mont(uint32_t x) { return kb_mul(x, 1u); }

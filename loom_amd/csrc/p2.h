@@ -76,14 +76,80 @@ LG_HD void p2_full_round(uint32_t* s, int r) {
   p2_external<W>(s);
 }
 
+// The internal diagonal entries are small integers and (negated) inverse
+// powers of two; each lane's product d_i * s_i is specialised at compile time
+// (additions, or kb_div_pow2) and falls back to a Montgomery product.
+struct DiagOp {
+  int neg;   // 1: subtract the term from the sum
+  int kind;  // 0: Montgomery product, 1: small integer c, 2: divide by 2^k
+  int c;     // small integer or k
+};
+
+constexpr uint32_t p2_pow_canon(uint32_t b, uint32_t e) {
+  uint64_t r = 1, x = b;
+  while (e) {
+    if (e & 1) r = r * x % KB_P;
+    x = x * x % KB_P;
+    e >>= 1;
+  }
+  return (uint32_t)r;
+}
+
+constexpr DiagOp p2_diag_op(uint32_t d) {
+  for (int neg = 0; neg < 2; ++neg) {
+    const uint32_t v = neg ? (d ? KB_P - d : 0u) : d;
+    for (uint32_t c = 1; c <= 4; ++c)
+      if (v == c) return DiagOp{neg, 1, (int)c};
+    for (int k = 1; k <= 24; ++k)
+      if (v == p2_pow_canon(p2_pow_canon(2, (uint32_t)k), KB_P - 2)) return DiagOp{neg, 2, k};
+  }
+  return DiagOp{0, 0, 0};
+}
+
+template <int W>
+struct P2Diag;
+template <>
+struct P2Diag<24> {
+  static constexpr uint32_t canon(int i) { return P2_W24_DIAG_CANON[i]; }
+};
+template <>
+struct P2Diag<16> {
+  static constexpr uint32_t canon(int i) { return P2_W16_DIAG_CANON[i]; }
+};
+
+template <int W, int I>
+LG_HD uint32_t p2_diag_lane(uint32_t sum, uint32_t x) {
+  constexpr DiagOp op = p2_diag_op(P2Diag<W>::canon(I));
+  uint32_t t;
+  if constexpr (op.kind == 1) {
+    if constexpr (op.c == 1) t = x;
+    else if constexpr (op.c == 2) t = kb_dbl(x);
+    else if constexpr (op.c == 3) t = kb_add(kb_dbl(x), x);
+    else t = kb_dbl(kb_dbl(x));
+  } else if constexpr (op.kind == 2) {
+    t = kb_div_pow2<op.c>(x);
+  } else {
+    return kb_add(sum, kb_mul(x, P2Consts<W>::diag(I)));
+  }
+  if constexpr (op.neg) return kb_sub(sum, t);
+  else return kb_add(sum, t);
+}
+
+template <int W, int I>
+LG_HD void p2_diag_all(uint32_t* s, uint32_t sum) {
+  if constexpr (I < W) {
+    s[I] = p2_diag_lane<W, I>(sum, s[I]);
+    p2_diag_all<W, I + 1>(s, sum);
+  }
+}
+
 template <int W>
 LG_HD void p2_partial_round(uint32_t* s, int r) {
   s[0] = kb_cube(kb_add(s[0], P2Consts<W>::partial(r)));
   uint32_t sum = s[0];
 #pragma unroll
   for (int i = 1; i < W; ++i) sum = kb_add(sum, s[i]);
-#pragma unroll
-  for (int i = 0; i < W; ++i) s[i] = kb_add(sum, kb_mul(s[i], P2Consts<W>::diag(i)));
+  p2_diag_all<W, 0>(s, sum);
 }
 
 template <int W>

@@ -8,6 +8,7 @@
 #include <cstdlib>
 #include <exception>
 #include <functional>
+#include <memory>
 #include <mutex>
 #include <thread>
 #include <vector>
@@ -16,9 +17,26 @@ namespace lg {
 
 class ThreadPool {
  public:
+  static int hardware_threads() {
+    int hw = (int)std::thread::hardware_concurrency();
+    if (const char* e = getenv("LOOMGPU_HOST_THREADS")) hw = atoi(e);
+    return std::max(1, std::min(hw, 16));
+  }
   static ThreadPool& get() {
-    static ThreadPool p;
+    static ThreadPool p(hardware_threads() - 1);
     return p;
+  }
+  // The pool of prover lane `lane` when `lanes` lanes share the host: the
+  // hardware threads are divided between the lanes so that their host work
+  // runs concurrently.
+  static ThreadPool& for_lane(int lane, int lanes) {
+    if (lanes <= 1) return get();
+    static std::mutex mu;
+    static std::unique_ptr<ThreadPool> pools[8][9];
+    std::lock_guard<std::mutex> lk(mu);
+    auto& p = pools[lane & 7][std::min(lanes, 8)];
+    if (!p) p.reset(new ThreadPool(std::max(1, hardware_threads() / lanes) - 1));
+    return *p;
   }
 
   // Runs fn(i) for i in [0, n); rethrows the first exception.
@@ -56,11 +74,8 @@ class ThreadPool {
   }
 
  private:
-  ThreadPool() {
-    int hw = (int)std::thread::hardware_concurrency();
-    if (const char* e = getenv("LOOMGPU_HOST_THREADS")) hw = atoi(e);
-    int n = std::max(0, std::min(hw, 16) - 1);
-    for (int i = 0; i < n; ++i) workers_.emplace_back([this] { loop(); });
+  explicit ThreadPool(int n_workers) {
+    for (int i = 0; i < n_workers; ++i) workers_.emplace_back([this] { loop(); });
   }
 
   void work() {
@@ -106,6 +121,16 @@ class ThreadPool {
   std::exception_ptr err_;
 };
 
-inline void parallel_for(int n, const std::function<void(int)>& fn) { ThreadPool::get().parallel_for(n, fn); }
+// The pool parallel_for uses on this thread (set by a prover lane), or null
+// for the shared one.
+inline ThreadPool*& current_pool() {
+  static thread_local ThreadPool* p = nullptr;
+  return p;
+}
+
+inline void parallel_for(int n, const std::function<void(int)>& fn) {
+  ThreadPool* p = current_pool();
+  (p ? *p : ThreadPool::get()).parallel_for(n, fn);
+}
 
 }  // namespace lg

@@ -5,6 +5,8 @@
 // the final polynomial and the query openings come back from the device.
 #include "prover.h"
 
+#include <thread>
+
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
@@ -237,33 +239,50 @@ void Staging::flush(hipStream_t s) {
   }
 }
 
-uint32_t* Context::pinned(size_t bytes) {
-  if (bytes > pinned_cap_) {
-    if (pinned_) (void)hipHostFree(pinned_);
-    pinned_ = nullptr;
-    pinned_cap_ = 0;
-    size_t cap = std::max<size_t>(bytes, 1u << 20);
-    LG_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_), cap, hipHostMallocDefault));
-    pinned_cap_ = cap;
-  }
-  return pinned_;
+Context::Context() {
+  if (const char* e = getenv("LOOMGPU_LANES")) lanes = std::max(1, std::min(8, atoi(e)));
+  lane_.resize(8);
 }
 
-uint32_t* Context::pinned_in(size_t bytes) {
-  if (bytes > pinned_in_cap_) {
-    if (pinned_in_) (void)hipHostFree(pinned_in_);
-    pinned_in_ = nullptr;
-    pinned_in_cap_ = 0;
-    size_t cap = std::max<size_t>(bytes, 1u << 20);
-    LG_HIP(hipHostMalloc(reinterpret_cast<void**>(&pinned_in_), cap, hipHostMallocDefault));
-    pinned_in_cap_ = cap;
+hipStream_t Context::lane_stream(int lane) {
+  if (lane == 0) return stream;
+  std::lock_guard<std::mutex> lock(lane_mu_);
+  LaneBufs& L = lane_.at(lane);
+  if (!L.stream) LG_HIP(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+  return L.stream;
+}
+
+static uint32_t* grow_pinned(uint32_t*& p, size_t& cap, size_t bytes) {
+  if (bytes > cap) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    cap = 0;
+    size_t c = std::max<size_t>(bytes, 1u << 20);
+    LG_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), c, hipHostMallocDefault));
+    cap = c;
   }
-  return pinned_in_;
+  return p;
+}
+
+uint32_t* Context::pinned(size_t bytes, int lane) {
+  LaneBufs& L = lane_.at(lane);
+  return grow_pinned(L.pinned, L.pinned_cap, bytes);
+}
+
+uint32_t* Context::pinned_in(size_t bytes, int lane) {
+  LaneBufs& L = lane_.at(lane);
+  return grow_pinned(L.pinned_in, L.pinned_in_cap, bytes);
 }
 
 Context::~Context() {
-  if (pinned_) (void)hipHostFree(pinned_);
-  if (pinned_in_) (void)hipHostFree(pinned_in_);
+  for (auto& L : lane_) {
+    if (L.pinned) (void)hipHostFree(L.pinned);
+    if (L.pinned_in) (void)hipHostFree(L.pinned_in);
+    if (L.stream) {
+      (void)hipStreamSynchronize(L.stream);
+      (void)hipStreamDestroy(L.stream);
+    }
+  }
 }
 
 std::unique_ptr<Staging> Context::take_staging() {
@@ -382,9 +401,11 @@ struct Layer {  // an E6 vector of len elements per instance, 6 limb arrays
 
 class Job {
  public:
-  Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb)
+  Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb,
+      int lane)
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
-        B_((int)in.size()), s_(ctx_.stream) {
+        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)) {
+    if (out.size() != in.size()) throw std::runtime_error("prove: output slots do not match the inputs");
     stg_ = ctx_.take_staging();
     set_prelaunch_hook([](void* u) { static_cast<Job*>(u)->flush_staging(); }, this);
   }
@@ -460,6 +481,7 @@ class Job {
   std::vector<ProveOutput>& out_;
   PhaseCallback cb_;
   int B_;
+  int lane_;
   hipStream_t s_;
   std::unique_ptr<Staging> stg_;
 
@@ -553,7 +575,7 @@ void Job::upload_inputs() {
       perm_off[p * B_ + b] = words;
       words += d_.perms[p].len;
     }
-  uint32_t* h = ctx_.pinned_in(words * 4);
+  uint32_t* h = ctx_.pinned_in(words * 4, lane_);
   parallel_for((int)(B_ * (nin + d_.perms.size())), [&](int i) {
     const int b = i % B_;
     const size_t k = (size_t)i / B_;
@@ -1658,7 +1680,7 @@ void Job::open() {
     launch_gather(ops_d, (int)ops.size(), blocks, pos_d, nq, B_, gat.as<uint32_t>(), (int)stride, s_);
   }
   const size_t gwords = (size_t)B_ * nq * stride;
-  uint32_t* gh = ctx_.pinned(gwords * 4);
+  uint32_t* gh = ctx_.pinned(gwords * 4, lane_);
   trace("gather kernels");
   LG_HIP(hipMemcpyAsync(gh, gat.as<void>(), gwords * 4, hipMemcpyDeviceToHost, s_));
   sync();
@@ -1804,10 +1826,48 @@ void Job::open() {
 void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutput>& outputs, const PhaseCallback& cb) {
   if (inputs.empty()) return;
   outputs.assign(inputs.size(), ProveOutput());
-  Job job(*this, inputs, outputs, cb);
-  job.run();
-  LG_HIP(hipStreamSynchronize(ctx_->stream));
-  ctx_->timer.resolve();
+  const int B = (int)inputs.size();
+  constexpr int kMinLaneBatch = 8;
+  int lanes = std::min(ctx_->lanes, std::max(1, B / kMinLaneBatch));
+  if (ctx_->timer.on) lanes = 1;  // per-kernel timing wants one stream
+  if (lanes <= 1) {
+    Job job(*this, inputs, outputs, cb, 0);
+    job.run();
+    LG_HIP(hipStreamSynchronize(ctx_->stream));
+    ctx_->timer.resolve();
+    return;
+  }
+  // Contiguous sub-batches, one per lane; instances are independent, so the
+  // proofs are those of one batch.
+  std::vector<std::vector<InstanceInput>> part(lanes);
+  std::vector<std::vector<ProveOutput>> out(lanes);
+  std::vector<int> first(lanes + 1, 0);
+  for (int l = 0; l < lanes; ++l) first[l + 1] = first[l] + B / lanes + (l < B % lanes ? 1 : 0);
+  for (int l = 0; l < lanes; ++l) {
+    part[l].assign(inputs.begin() + first[l], inputs.begin() + first[l + 1]);
+    out[l].assign(part[l].size(), ProveOutput());  // a Job writes its outputs in place
+  }
+  std::vector<std::exception_ptr> err(lanes);
+  auto run_lane = [&](int l) {
+    try {
+      current_pool() = &ThreadPool::for_lane(l, lanes);
+      Job job(*this, part[l], out[l], l == 0 ? cb : PhaseCallback(), l);  // phases: lane 0's
+      job.run();
+      current_pool() = nullptr;
+      LG_HIP(hipStreamSynchronize(ctx_->lane_stream(l)));
+    } catch (...) {
+      current_pool() = nullptr;
+      err[l] = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int l = 1; l < lanes; ++l) th.emplace_back(run_lane, l);
+  run_lane(0);
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+  for (int l = 0; l < lanes; ++l)
+    for (int b = 0; b < (int)out[l].size(); ++b) outputs[first[l] + b] = std::move(out[l][b]);
 }
 
 }  // namespace lg

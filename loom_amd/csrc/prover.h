@@ -128,16 +128,28 @@ struct Context {
   std::vector<std::unique_ptr<Staging>> staging_free;
   std::unique_ptr<Staging> take_staging();
   void give_staging(std::unique_ptr<Staging> s);
-  // Pinned host buffers reused across proves (one prove at a time per context).
-  uint32_t* pinned(size_t bytes);     // device -> host (query openings)
-  uint32_t* pinned_in(size_t bytes);  // host -> device (trace columns)
+  // A batch is split over `lanes` sub-batches proved by concurrent host
+  // threads, each on its own stream (lane 0 uses `stream`), so that one
+  // lane's host work (transcripts, query derivation, proof assembly)
+  // overlaps another lane's kernels.  LOOMGPU_LANES overrides the default.
+  int lanes = 2;
+  hipStream_t lane_stream(int lane);
+  // Pinned host buffers per lane, reused across proves.
+  uint32_t* pinned(size_t bytes, int lane);     // device -> host (query openings)
+  uint32_t* pinned_in(size_t bytes, int lane);  // host -> device (trace columns)
+  Context();
   ~Context();
 
  private:
-  uint32_t* pinned_ = nullptr;
-  size_t pinned_cap_ = 0;
-  uint32_t* pinned_in_ = nullptr;
-  size_t pinned_in_cap_ = 0;
+  struct LaneBufs {
+    hipStream_t stream = nullptr;  // owned for lanes >= 1
+    uint32_t* pinned = nullptr;
+    size_t pinned_cap = 0;
+    uint32_t* pinned_in = nullptr;
+    size_t pinned_in_cap = 0;
+  };
+  std::mutex lane_mu_;
+  std::vector<LaneBufs> lane_;
 };
 
 // Per-instance caller data.

@@ -301,6 +301,20 @@ def test_batched_equals_alone(ctx):
     assert np.array_equal(batch[2], enc)
 
 
+def test_batch_split_over_lanes_equals_alone(ctx):
+    """A batch of 17 is proved as two concurrent sub-batches (lanes, own
+    streams and host pools); every proof equals the instance's proof alone."""
+    from loom_amd import workloads, prover
+    log("batch of 17 x 2^6 over lanes vs alone")
+    inst = [workloads.plonk_instance(6, 700 + s) for s in range(17)]
+    items = [prover.Prepared(p, t, 10) for p, t in inst]
+    batch = prover.prove_prepared(ctx, items)
+    for it, w in zip(items, batch):
+        assert np.array_equal(w, prover.prove_prepared(ctx, [it])[0])
+    enc, _ = _oracle_words(inst[16][0], inst[16][1], 10)
+    assert np.array_equal(batch[16], enc)
+
+
 def test_device_inputs_equal_host_inputs(ctx):
     """lg_prove_device (trace already in HBM) gives the proofs lg_prove gives
     from host buffers, and a host pointer handed to it is refused."""
