@@ -6,6 +6,7 @@
 #include <stdexcept>
 
 #include "gpu.h"
+#include "ntt16.h"
 #include "p2.h"
 
 namespace lg {
@@ -119,6 +120,130 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
   }
 }
 
+// Radix-16 form of k_ntt_multi for 2^8 <= n <= 2^14 (same job and argument
+// semantics): n/16 threads, 16 values per thread, four radix-2 stages per
+// pass in registers (ntt16.h), one LDS exchange and barrier per pass instead
+// of per stage, twiddles from one table load per pass.  The inverse runs
+// decimation-in-time from a bit-reversed LDS copy and ends with the values of
+// thread t at natural positions t + k n/16, exactly the layout the first
+// forward (decimation-in-frequency) pass reads, so the two transforms chain
+// in registers.  Forward outputs come out in bit-reversed order, 16
+// contiguous values per thread; a natural-order output goes through LDS.
+template <int KIND>
+__global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs, NttArgs cfg,
+                                                const uint32_t* const* __restrict__ pres,
+                                                const uint32_t* __restrict__ tw_fwd,
+                                                const uint32_t* __restrict__ tw_inv) {
+  extern __shared__ uint32_t sh[];
+  const NttJob* __restrict__ jp = jobs + blockIdx.x;
+  const int L = cfg.logn;
+  const uint32_t n = 1u << L;
+  const uint32_t q0 = n >> 4;
+  const uint32_t t = threadIdx.x;
+  const int P = L >> 2, r = L & 3;
+  uint32_t x[16];
+  if (cfg.inverse_first) {
+    const NttRoots Ri = ntt16_roots(tw_inv, L);
+    const uint32_t* __restrict__ in = jp->in;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      const uint32_t i = t + (uint32_t)k * q0;
+      sh[brev(i, L)] = in[i];
+    }
+    __syncthreads();
+    if (r) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = sh[16 * t + k];
+      dit_small_r(x, r, Ri);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) sh[16 * t + k] = x[k];
+      __syncthreads();
+    }
+    for (int p = 0; p < P; ++p) {
+      const int logm = r + 4 * (p + 1);
+      const uint32_t q = 1u << (logm - 4);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = sh[ntt16_pos(t, q, k)];
+      const Ladder Ld = ntt16_ladder(tw_inv[(t % q) << (L - logm)], Ri);
+      dit16(x, Ld);
+      if (p < P - 1) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh[ntt16_pos(t, q, k)] = x[k];
+        __syncthreads();
+      }
+    }
+    const uint32_t* __restrict__ post = jp->post;
+    if (post) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = kb_mul(x[k], post[t + (uint32_t)k * q0]);
+    }
+    if (cfg.n_out == 0) {
+      uint32_t* __restrict__ out = jp->out[0];
+#pragma unroll
+      for (int k = 0; k < 16; ++k) out[t + (uint32_t)k * q0] = x[k];
+      return;
+    }
+    __syncthreads();  // the last inverse pass read LDS; the forward passes write it
+  } else {
+    const uint32_t* __restrict__ in = jp->in;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = in[t + (uint32_t)k * q0];
+  }
+  const NttRoots Rf = ntt16_roots(tw_fwd, L);
+  for (int o = 0; o < cfg.n_out; ++o) {
+    uint32_t y[16];
+    const uint32_t* __restrict__ pre = pres[o];
+    if (pre) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) y[k] = kb_mul(x[k], pre[t + (uint32_t)k * q0]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) y[k] = x[k];
+    }
+    for (int p = 0; p < P; ++p) {
+      const int logm = L - 4 * p;
+      const uint32_t q = 1u << (logm - 4);
+      if (p > 0) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) y[k] = sh[ntt16_pos(t, q, k)];
+      }
+      const Ladder Ld = ntt16_ladder(tw_fwd[(t % q) << (L - logm)], Rf);
+      dif16(y, Ld);
+      if (p < P - 1 || r) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh[ntt16_pos(t, q, k)] = y[k];
+        __syncthreads();
+      }
+    }
+    if (r) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) y[k] = sh[16 * t + k];
+      dif_small_r(y, r, Rf);
+    }
+    // y[k] is now the bit-reversed-order value at position 16 t + k
+    uint32_t* __restrict__ out = jp->out[o];
+    if (((cfg.bitrev_mask >> o) & 1) && !(reinterpret_cast<uintptr_t>(out) & 15)) {
+      uint4* o4 = reinterpret_cast<uint4*>(out + 16 * t);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o4[k] = make_uint4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
+    } else if ((cfg.bitrev_mask >> o) & 1) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) out[16 * t + k] = y[k];
+    } else {
+      __syncthreads();  // other threads may still read LDS of this output's last pass
+#pragma unroll
+      for (int k = 0; k < 16; ++k) sh[16 * t + k] = y[k];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < 16; ++k) {
+        const uint32_t i = t + (uint32_t)k * q0;
+        out[i] = sh[brev(i, L)];
+      }
+    }
+    __syncthreads();  // before the next output writes LDS
+  }
+}
+
 void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
                 const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s) {
   if (n_jobs == 0) return;
@@ -131,8 +256,21 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   NttArgs a{cfg.logn, cfg.inverse_first, cfg.n_out, 0};
   for (int o = 0; o < cfg.n_out; ++o) a.bitrev_mask |= (cfg.bitrev_out[o] ? 1 : 0) << o;
   prelaunch();
-  const dim3 g(n_jobs), b(T);
   const size_t lds = n * sizeof(uint32_t);
+  if (cfg.logn >= 8) {
+    const dim3 g(n_jobs), b(n / 16);
+    switch (cfg.kind) {
+      case NTT_KIND_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+      case NTT_KIND_QUOT_COSET: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_COSET>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+      case NTT_KIND_QUOT_BACK: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_BACK>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+      case NTT_KIND_CHUNKS: hipLaunchKernelGGL(k_ntt16<NTT_KIND_CHUNKS>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+      case NTT_KIND_DEEP_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_DEEP_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+      default: throw std::runtime_error("launch_ntt: unknown kind");
+    }
+    LG_HIP(hipGetLastError());
+    return;
+  }
+  const dim3 g(n_jobs), b(T);
   switch (cfg.kind) {
     case NTT_KIND_LDE: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
     case NTT_KIND_QUOT_COSET: hipLaunchKernelGGL(k_ntt_multi<NTT_KIND_QUOT_COSET>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;

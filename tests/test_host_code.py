@@ -42,3 +42,33 @@ def test_div_pow2_and_permutation_match_oracle(p2_bin):
     perm = p2.Permutation(24)
     want = perm(states.T.copy()).T
     assert np.array_equal(got, want)
+
+
+@pytest.fixture(scope="module")
+def ntt_bin():
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tempfile.mkdtemp()
+    exe = os.path.join(d, "ntt16_host_check")
+    subprocess.run([gxx, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(HERE, "host", "ntt16_host_check.cpp"),
+                    "-o", exe], check=True)
+    yield exe
+    shutil.rmtree(d)
+
+
+@pytest.mark.parametrize("logn", [8, 9, 10, 11, 12, 13, 14])
+def test_radix16_passes_match_dft(ntt_bin, logn):
+    """The radix-16 pass schedule of the LDS NTT: forward output equals the
+    DFT in bit-reversed order, and the inverse passes bring it back (times n)."""
+    from oracle import ntt, field as F
+    P = F.P
+    n = 1 << logn
+    rng = np.random.default_rng(logn)
+    x = rng.integers(0, P, size=n, dtype=np.uint64)
+    r = subprocess.run([ntt_bin], input="%d\n" % logn + " ".join(str(int(v)) for v in x) + "\n",
+                       capture_output=True, text=True, check=True)
+    fwd, inv = [np.array([int(v) for v in ln.split()], dtype=np.uint64) for ln in r.stdout.strip().splitlines()]
+    want = ntt.evaluate(x)
+    assert np.array_equal(fwd, want[F.bitrev_indices(n)])
+    assert np.array_equal(inv, x * np.uint64(n) % np.uint64(P))
