@@ -293,98 +293,99 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
 constexpr int BIG_C = 8;
 
 
-__device__ __forceinline__ void lds_dit_multi(uint32_t* sh, int logm, int C, const uint32_t* __restrict__ tw,
-                                              int tw_shift) {
-  // C independent transforms of size m = 2^logm stored at sh[c * m + .],
-  // bit-reversed input order, natural output; tw[k << tw_shift] = w_m^k.
-  const int m = 1 << logm;
-  const int T = blockDim.x;
-  const int half_total = C * (m >> 1);
-  for (int s = 0; s < logm; ++s) {
-    const int half = 1 << s;
-    const int tsh = logm - 1 - s + tw_shift;
-    for (int t = threadIdx.x; t < half_total; t += T) {
-      const int c = t >> (logm - 1);
-      const int u = t & ((m >> 1) - 1);
-      const int j = u & (half - 1);
-      const int i0 = c * m + (((u >> s) << (s + 1)) | j);
-      const int i1 = i0 + half;
-      const uint32_t w = tw[j << tsh];
-      const uint32_t x = sh[i0];
-      const uint32_t y = kb_mul(sh[i1], w);
-      sh[i0] = kb_add(x, y);
-      sh[i1] = kb_sub(x, y);
+// Forward radix-16 DIF over one column of size m = 2^lg (lg >= 4) held by
+// m/16 threads (index tt); on entry y[k] is the column's natural position
+// tt + k m/16, on exit y[k] is its bit-reversed position 16 tt + k.  Column
+// position `pos` lives at col[pos * cs] (cs: the column interleave stride).
+// Every thread of the block calls it with the same lg (barriers inside).
+__device__ __forceinline__ void col_dif(uint32_t* y, uint32_t* col, int cs, int lg, uint32_t tt,
+                                        const uint32_t* __restrict__ tw, const NttRoots& R) {
+  const int P = lg >> 2, r = lg & 3;
+  for (int p = 0; p < P; ++p) {
+    const int logm = lg - 4 * p;
+    const uint32_t q = 1u << (logm - 4);
+    if (p > 0) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) y[k] = col[ntt16_pos(tt, q, k) * cs];
     }
-    __syncthreads();
+    const Ladder Ld = ntt16_ladder(tw[(tt % q) << (lg - logm)], R);
+    dif16(y, Ld);
+    if (p < P - 1 || r) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) col[ntt16_pos(tt, q, k) * cs] = y[k];
+      __syncthreads();
+    }
+  }
+  if (r) {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) y[k] = col[(16 * tt + k) * cs];
+    dif_small_r(y, r, R);
   }
 }
 
-// tw_a: twiddles of the size-n2 sub-transforms (w_{n2}^k, k < n2/2); the
-// pass-A twiddle w_n^e (e = j1 k2 < n) is wlo[e mod 2048] * whi[e / 2048]
-// (wlo: w_n^k, whi: w_n^{2048 k}), two small tables instead of one of n
-// entries.  All tables in the direction of the transform.
-__global__ __launch_bounds__(256) void k_ntt_pass_a(const BigJob* __restrict__ jobs, int a, int b,
-                                                    const uint32_t* __restrict__ tw_a, const uint32_t* __restrict__ wlo,
-                                                    const uint32_t* __restrict__ whi) {
+// Pass A: the n2-point transforms of 8 adjacent columns j1 per workgroup
+// (radix-16 DIF, columns interleaved in LDS), times w_n^{j1 k2}, transposed
+// into tmp[k2 n1 + j1].  tw_a: w_{n2}^k (k < n2/2); the pass-A twiddle
+// w_n^e (e = j1 k2 < n) is wlo[e mod 2048] * whi[e / 2048].
+__global__ __launch_bounds__(1024) void k_ntt_pass_a(const BigJob* __restrict__ jobs, int a, int b,
+                                                     const uint32_t* __restrict__ tw_a,
+                                                     const uint32_t* __restrict__ wlo,
+                                                     const uint32_t* __restrict__ whi) {
   extern __shared__ uint32_t sh[];
   const BigJob* job = jobs + blockIdx.y;
-  const int n1 = 1 << a, n2 = 1 << b;
-  const int j10 = blockIdx.x * BIG_C;
-  const int tot = BIG_C * n2;
+  const uint32_t n1 = 1u << a, n2 = 1u << b;
+  const uint32_t c = threadIdx.x & (BIG_C - 1), tt = threadIdx.x >> 3;
+  const uint32_t j1 = blockIdx.x * BIG_C + c;
+  const uint32_t qc = n2 >> 4;
   const uint32_t* __restrict__ in = job->in;
   const uint32_t* __restrict__ pre = job->pre;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx & (BIG_C - 1), j2 = idx >> 3;
-    const size_t gi = (size_t)(j10 + c) + (size_t)n1 * j2;
-    uint32_t v = in[gi];
-    if (pre) v = kb_mul(v, pre[gi]);
-    sh[c * n2 + brev(j2, b)] = v;
+  uint32_t y[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const size_t gi = (size_t)j1 + (size_t)n1 * (tt + (uint32_t)k * qc);
+    const uint32_t v = in[gi];
+    y[k] = pre ? kb_mul(v, pre[gi]) : v;
   }
-  __syncthreads();
-  lds_dit_multi(sh, b, BIG_C, tw_a, 0);
+  const NttRoots R = ntt16_roots(tw_a, b);
+  col_dif(y, sh + c, BIG_C, b, tt, tw_a, R);
   uint32_t* __restrict__ tmp = job->tmp;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx & (BIG_C - 1), k2 = idx >> 3;
-    const uint32_t j1 = j10 + c;
-    const uint32_t e = j1 * (uint32_t)k2;
-    const uint32_t w = kb_mul(wlo[e & 2047u], whi[e >> 11]);
-    tmp[(size_t)k2 * n1 + j1] = kb_mul(sh[c * n2 + k2], w);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t k2 = brev(16 * tt + k, b);
+    const uint32_t e = j1 * k2;
+    tmp[(size_t)k2 * n1 + j1] = kb_mul(y[k], kb_mul(wlo[e & 2047u], whi[e >> 11]));
   }
 }
 
-__global__ __launch_bounds__(256) void k_ntt_pass_b(const BigJob* __restrict__ jobs, int a, int b,
-                                                    const uint32_t* __restrict__ tw_b, int bitrev_out) {
+// Pass B: for 8 adjacent k2 per workgroup, the n1-point transform of the row
+// tmp[k2 n1 + .] (radix-16 DIF), giving X[k2 + n2 k1] (times post, natural
+// index), stored at that natural index or at its bit reversal
+// brev_a(k1) + n1 brev_b(k2).
+__global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ jobs, int a, int b,
+                                                     const uint32_t* __restrict__ tw_b, int bitrev_out) {
   extern __shared__ uint32_t sh[];
   const BigJob* job = jobs + blockIdx.y;
-  const int n1 = 1 << a, n2 = 1 << b;
-  const int k20 = blockIdx.x * BIG_C;
-  const int tot = BIG_C * n1;
-  const uint32_t* __restrict__ tmp = job->tmp;
-  for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-    const int c = idx >> a, j1 = idx & (n1 - 1);
-    sh[c * n1 + brev(j1, a)] = tmp[(size_t)(k20 + c) * n1 + j1];
-  }
-  __syncthreads();
-  lds_dit_multi(sh, a, BIG_C, tw_b, 0);
+  const uint32_t n1 = 1u << a, n2 = 1u << b;
+  const uint32_t tpc = n1 >> 4;  // threads per row
+  const uint32_t tt = threadIdx.x % tpc, c = threadIdx.x / tpc;
+  const uint32_t k2 = blockIdx.x * BIG_C + c;
+  const uint32_t* __restrict__ tmp = job->tmp + (size_t)k2 * n1;
+  uint32_t y[16];
+#pragma unroll
+  for (int k = 0; k < 16; ++k) y[k] = tmp[tt + (uint32_t)k * tpc];
+  const NttRoots R = ntt16_roots(tw_b, a);
+  col_dif(y, sh + c * n1, 1, a, tt, tw_b, R);
   uint32_t* __restrict__ out = job->out;
   const uint32_t* __restrict__ post = job->post;
-  if (bitrev_out) {
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-      const int c = idx >> a, r1 = idx & (n1 - 1);
-      const uint32_t k2 = k20 + c, k1 = brev(r1, a);
-      uint32_t v = sh[c * n1 + k1];
-      if (post) v = kb_mul(v, post[k2 + (size_t)n2 * k1]);
-      out[(size_t)n1 * brev(k2, b) + r1] = v;
-    }
-  } else {
-    for (int idx = threadIdx.x; idx < tot; idx += blockDim.x) {
-      const int c = idx & (BIG_C - 1), k1 = idx >> 3;
-      const uint32_t k2 = k20 + c;
-      uint32_t v = sh[c * n1 + k1];
-      const size_t o = k2 + (size_t)n2 * k1;
-      if (post) v = kb_mul(v, post[o]);
-      out[o] = v;
-    }
+  const uint32_t k2r = brev(k2, b);
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    const uint32_t pbr = 16 * tt + k;  // = brev_a(k1)
+    const uint32_t k1 = brev(pbr, a);
+    uint32_t v = y[k];
+    if (post) v = kb_mul(v, post[k2 + (size_t)n2 * k1]);
+    if (bitrev_out) out[pbr + (size_t)n1 * k2r] = v;
+    else out[k2 + (size_t)n2 * k1] = v;
   }
 }
 
@@ -392,13 +393,13 @@ void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int 
                     const uint32_t* tw_b, const uint32_t* wlo, const uint32_t* whi, int bitrev_out, hipStream_t s) {
   if (n_jobs == 0) return;
   const int a = (logn + 1) / 2, b = logn - a;  // n1 = 2^a >= n2 = 2^b
-  if (a > 11 || b < 3 || logn < 12) throw std::runtime_error("launch_ntt_big: unsupported size");
+  if (a > 11 || b < 7 || logn < 15) throw std::runtime_error("launch_ntt_big: unsupported size");
   prelaunch();
-  hipLaunchKernelGGL(k_ntt_pass_a, dim3((1 << a) / BIG_C, n_jobs), dim3(256), (size_t)BIG_C * (1 << b) * 4, s, jobs_a,
-                     a, b, tw_a, wlo, whi);
+  hipLaunchKernelGGL(k_ntt_pass_a, dim3((1 << a) / BIG_C, n_jobs), dim3(BIG_C * ((1 << b) >> 4)),
+                     (size_t)BIG_C * (1 << b) * 4, s, jobs_a, a, b, tw_a, wlo, whi);
   LG_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_ntt_pass_b, dim3((1 << b) / BIG_C, n_jobs), dim3(256), (size_t)BIG_C * (1 << a) * 4, s, jobs_b,
-                     a, b, tw_b, bitrev_out);
+  hipLaunchKernelGGL(k_ntt_pass_b, dim3((1 << b) / BIG_C, n_jobs), dim3(BIG_C * ((1 << a) >> 4)),
+                     (size_t)BIG_C * (1 << a) * 4, s, jobs_b, a, b, tw_b, bitrev_out);
   LG_HIP(hipGetLastError());
 }
 
