@@ -150,6 +150,8 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
         in[b].perms.push_back(perms[b * np + k]);
       }
       in[b].device = device;
+      in[b].proof_dst = proofs[b];
+      in[b].proof_cap = proof_caps[b];
     }
     std::vector<lg::ProveOutput> out;
     {
@@ -162,10 +164,10 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
     }
     std::atomic<int> rc{0};
     lg::parallel_for(batch, [&](int b) {
-      proof_lens[b] = out[b].proof.size();
-      if (out[b].proof.size() > proof_caps[b]) {
+      proof_lens[b] = out[b].proof_len;
+      if (out[b].proof_len > proof_caps[b]) {
         rc = -2;
-      } else {
+      } else if (!out[b].direct) {
         std::memcpy(proofs[b], out[b].proof.data(), out[b].proof.size() * 4);
       }
       if (transcripts && transcripts[b]) {

@@ -372,6 +372,27 @@ struct TimeIt {
   ~TimeIt() { c.timer.end(s); }
 };
 
+// push_back / insert(end(), first, last) into a caller buffer of cap words;
+// n counts every word, also those past the capacity (which are dropped).
+struct SpanWriter {
+  uint32_t* p;
+  size_t cap;
+  size_t n = 0;
+  struct End {};
+  End end() const { return {}; }
+  void push_back(uint32_t x) {
+    if (n < cap) p[n] = x;
+    ++n;
+  }
+  void insert(End, const uint32_t* a, const uint32_t* b) {
+    const size_t k = (size_t)(b - a);
+    if (n + k <= cap) std::memcpy(p + n, a, k * 4);
+    else
+      for (size_t i = 0; i < k && n + i < cap; ++i) p[n + i] = a[i];
+    n += k;
+  }
+};
+
 struct ColBuf {
   DevBuf buf;
   bool committed = false;
@@ -1687,9 +1708,10 @@ void Job::open() {
   trace("gather copy-back");
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
   parallel_for(B_, [&](int b) {
-    std::vector<uint32_t>& w = out_[b].proof;
-    w.clear();
-    w.reserve(64 + (size_t)nq * (stride + 64 * (nt + FRI_ROUNDS)) + 16 * ncv);
+   // The words go straight to the caller's buffer when it gave one
+   // (SpanWriter counts past its capacity so the caller learns the length),
+   // else to out_[b].proof.
+   auto emit = [&](auto& w) {
     const std::string hid = "poseidon2";
     w.push_back((uint32_t)hid.size());
     for (size_t i = 0; i < hid.size(); i += 4) {
@@ -1804,6 +1826,19 @@ void Job::open() {
           rawrow(g, rec + r.inj_rows_off[k2], r.inj_ncols[k2], 1);
         }
       }
+    }
+   };
+    if (in_[b].proof_dst) {
+      SpanWriter w{in_[b].proof_dst, in_[b].proof_cap};
+      emit(w);
+      out_[b].proof_len = w.n;
+      out_[b].direct = true;
+    } else {
+      std::vector<uint32_t>& w = out_[b].proof;
+      w.clear();
+      w.reserve(64 + (size_t)nq * (stride + 64 * (nt + FRI_ROUNDS)) + 16 * ncv);
+      emit(w);
+      out_[b].proof_len = w.size();
     }
     out_[b].transcript = ts_[b].log_words();
   });

@@ -157,10 +157,14 @@ struct InstanceInput {
   std::vector<const uint32_t*> columns;  // plan input order, Montgomery, N_module words each
   std::vector<const uint32_t*> perms;    // plan permutation order, len words each
   bool device = false;                   // pointers are device memory of the context's device (else host)
+  uint32_t* proof_dst = nullptr;         // optional caller buffer for the proof words
+  size_t proof_cap = 0;                  // its capacity in words
 };
 
 struct ProveOutput {
-  std::vector<uint32_t> proof;       // canonical word stream
+  std::vector<uint32_t> proof;       // canonical word stream (when no caller buffer was given)
+  size_t proof_len = 0;              // proof length in words
+  bool direct = false;               // written to the caller's buffer (complete if proof_len <= cap)
   std::vector<uint32_t> transcript;  // challenge digests in computation order
 };
 
