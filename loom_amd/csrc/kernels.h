@@ -134,7 +134,11 @@ struct ClaimJob {
   int32_t shift;
   int32_t ext;
 };
-void launch_claims(const ClaimJob* jobs, int n_jobs, int n, hipStream_t s);
+// Rows are split over claims_split(n) blocks per job for large n; part then
+// holds n_jobs x claims_split(n) x 6 words of partial sums.
+constexpr int CLAIMS_MAX_SPLIT = 64;
+int claims_split(int n);
+void launch_claims(const ClaimJob* jobs, int n_jobs, int n, uint32_t* part, hipStream_t s);
 
 // DEEP quotient on the trace domain for one class: per row x (natural order)
 //   DQ[x] = sum_p (c_p - sum_{bundles i at p} a_i * P_i(x)) / (z_p - w^x)

@@ -964,11 +964,18 @@ void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const
   LG_HIP(hipGetLastError());
 }
 
-__global__ __launch_bounds__(256) void k_claims(const ClaimJob* jobs, int n) {
+// Block (job, chunk) sums W[(i - shift) mod n] * P[i] over rows
+// [chunk n/S, (chunk+1) n/S); with one chunk the sum is the claimed value,
+// else it is a partial sum in part[(job S + chunk) 6 ..] that k_claims_sum
+// adds up.
+__global__ __launch_bounds__(256) void k_claims(const ClaimJob* jobs, int n, uint32_t* part) {
   __shared__ uint32_t sh[256 * 6];
   const ClaimJob job = jobs[blockIdx.x];
+  const int S = gridDim.y;
+  const int len = n / S;
+  const int lo = blockIdx.y * len;
   E6 acc = e6_zero();
-  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+  for (int i = lo + (int)threadIdx.x; i < lo + len; i += blockDim.x) {
     const int wi = (i - job.shift) & (n - 1);
     E6 w = ld6(job.W, n, wi);
     if (job.ext) acc = e6_add(acc, e6_mul(w, ld6(job.poly, n, i)));
@@ -985,14 +992,34 @@ __global__ __launch_bounds__(256) void k_claims(const ClaimJob* jobs, int n) {
     }
     __syncthreads();
   }
-  if (threadIdx.x < 6) job.out[threadIdx.x] = sh[threadIdx.x];
+  if (threadIdx.x < 6) {
+    if (S == 1) job.out[threadIdx.x] = sh[threadIdx.x];
+    else part[((size_t)blockIdx.x * S + blockIdx.y) * 6 + threadIdx.x] = sh[threadIdx.x];
+  }
 }
 
-void launch_claims(const ClaimJob* jobs, int n_jobs, int n, hipStream_t s) {
+__global__ void k_claims_sum(const ClaimJob* jobs, int n_jobs, int S, const uint32_t* part) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs) return;
+  E6 acc = e6_zero();
+  for (int c = 0; c < S; ++c) acc = e6_add(acc, ldc6(part + ((size_t)j * S + c) * 6));
+#pragma unroll
+  for (int l = 0; l < 6; ++l) jobs[j].out[l] = acc.v[l];
+}
+
+int claims_split(int n) { return n >= (1 << 16) ? std::min(CLAIMS_MAX_SPLIT, n >> 15) : 1; }
+
+void launch_claims(const ClaimJob* jobs, int n_jobs, int n, uint32_t* part, hipStream_t s) {
   if (n_jobs == 0) return;
+  const int S = claims_split(n);
+  if (S > 1 && !part) throw std::runtime_error("launch_claims: partial-sum buffer missing");
   prelaunch();
-  hipLaunchKernelGGL(k_claims, dim3(n_jobs), dim3(256), 0, s, jobs, n);
+  hipLaunchKernelGGL(k_claims, dim3(n_jobs, S), dim3(256), 0, s, jobs, n, part);
   LG_HIP(hipGetLastError());
+  if (S > 1) {
+    hipLaunchKernelGGL(k_claims_sum, dim3((n_jobs + 63) / 64), dim3(64), 0, s, jobs, n_jobs, S, (const uint32_t*)part);
+    LG_HIP(hipGetLastError());
+  }
 }
 
 // ------------------------------------------------------------------ DEEP

@@ -1322,7 +1322,16 @@ void Job::open() {
       j.ext = v.ext;
       cj[n].push_back(j);
     }
-  for (auto& kv : cj) launch_claims(stage(kv.second), (int)kv.second.size(), kv.first, s_);
+  for (auto& kv : cj) {
+    const int S = claims_split((int)kv.first);
+    uint32_t* part = nullptr;
+    if (S > 1) {
+      keep_.push_back(dev(kv.second.size() * (size_t)S * 6));
+      part = keep_.back().as<uint32_t>();
+    }
+    TimeIt tm(ctx_, s_, "claims", (double)kv.second.size() * kv.first * 7 * 4, 0);
+    launch_claims(stage(kv.second), (int)kv.second.size(), kv.first, part, s_);
+  }
   std::vector<E6> cvals((size_t)B_ * ncv);
   LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
   sync();
