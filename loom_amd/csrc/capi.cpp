@@ -311,8 +311,10 @@ int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint
     uint32_t* rp_ = ratio.as<uint32_t>();
     lg::DevBuf a = stage_sync(&np_, sizeof(void*)), b = stage_sync(&dp_, sizeof(void*)), c = stage_sync(&gp_, sizeof(void*));
     lg::DevBuf r = stage_sync(&rp_, sizeof(void*));
+    const size_t tmp_words = lg::scan_tmp_words((int)n, 1);
+    lg::DevBuf tmp(tmp_words ? tmp_words * sizeof(uint32_t) : 0);
     lg::launch_grand_product(a.as<const uint32_t*>(), b.as<const uint32_t*>(), r.as<uint32_t*>(), c.as<uint32_t*>(),
-                             (int)n, 1, ctx->c.stream);
+                             (int)n, 1, tmp_words ? tmp.as<uint32_t>() : nullptr, ctx->c.stream);
     LG_HIP(hipStreamSynchronize(ctx->c.stream));
     return 0;
   });

@@ -97,8 +97,12 @@ void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_
 
 // Grand product: gp[0] = 1, gp[i] = gp[i-1] * num[i-1] / den[i-1]; num, den,
 // ratio (scratch) and gp are E6 columns as 6 limb arrays (limb stride n).
+// Scans of n >= SCAN_MIN_TILED rows run over many workgroups and need
+// scan_tmp_words(n, batch) words of scratch (scan_tmp); smaller ones need none.
+constexpr int SCAN_MIN_TILED = 1 << 16;
+size_t scan_tmp_words(int n, int batch);
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
-                          uint32_t* const* gp, int n, int batch, hipStream_t s);
+                          uint32_t* const* gp, int n, int batch, uint32_t* scan_tmp, hipStream_t s);
 
 // Lookups and logup (board/module.go:56-71, poly/lookup_multiplicity.go:37-100,
 // board/logup.go:138-186), batch on grid y, per-instance pointer arrays.
@@ -113,7 +117,7 @@ void launch_logup_acc(const uint32_t* const* e, const uint32_t* const* m, uint32
                       int batch, hipStream_t s);
 // col = inclusive running sum of acc minus (i+1) * total / n; total to tot[6b..6b+6)
 void launch_cyclic_logup_finish(const uint32_t* const* acc, uint32_t* const* col, uint32_t* tot, int n,
-                                uint32_t ninv, int batch, hipStream_t s);
+                                uint32_t ninv, int batch, uint32_t* scan_tmp, hipStream_t s);
 
 // Quotient coefficients from the per-coset inverse transforms:
 //   coef[b][t][l][a] = ginv[a + t n] * sum_i wr[(t i) mod rho] * Y[b][l][i n + a]

@@ -753,12 +753,165 @@ __global__ __launch_bounds__(1024) void k_prefix_product(const uint32_t* const* 
   }
 }
 
+// Three-pass scan of E6 columns for n >= 2^16 (tiles of 2048 rows): tile
+// aggregates, an exclusive scan of the aggregates per instance, then each tile
+// rescanned from its prefix.  MUL: products (grand product), else sums
+// (logup); the field operations commute, so a tile's aggregate can be formed
+// in any order, while the rescan keeps the row order.
+constexpr int SCAN_T = 256, SCAN_PER = 8, SCAN_TILE = SCAN_T * SCAN_PER;
+
+template <bool MUL>
+__device__ __forceinline__ E6 scan_op(const E6& a, const E6& b) {
+  if constexpr (MUL) return e6_mul(a, b);
+  else return e6_add(a, b);
+}
+template <bool MUL>
+__device__ __forceinline__ E6 scan_id() {
+  if constexpr (MUL) return e6_one();
+  else return e6_zero();
+}
+
+template <bool MUL>
+__device__ __forceinline__ E6 block_reduce(E6 v, uint32_t* sh) {
+#pragma unroll
+  for (int l = 0; l < 6; ++l) sh[6 * threadIdx.x + l] = v.v[l];
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      const E6 w = scan_op<MUL>(ldc6(sh + 6 * threadIdx.x), ldc6(sh + 6 * (threadIdx.x + off)));
+#pragma unroll
+      for (int l = 0; l < 6; ++l) sh[6 * threadIdx.x + l] = w.v[l];
+    }
+    __syncthreads();
+  }
+  return ldc6(sh);
+}
+
+// Inclusive scan of one E6 per thread over the block (Hillis-Steele in LDS);
+// returns the exclusive prefix of this thread.
+template <bool MUL>
+__device__ __forceinline__ E6 block_exclusive(E6 v, uint32_t* sh) {
+#pragma unroll
+  for (int l = 0; l < 6; ++l) sh[6 * threadIdx.x + l] = v.v[l];
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    E6 me = ldc6(sh + 6 * threadIdx.x);
+    const bool has = (int)threadIdx.x >= off;
+    E6 other = has ? ldc6(sh + 6 * (threadIdx.x - off)) : scan_id<MUL>();
+    __syncthreads();
+    if (has) {
+      me = scan_op<MUL>(other, me);
+#pragma unroll
+      for (int l = 0; l < 6; ++l) sh[6 * threadIdx.x + l] = me.v[l];
+    }
+    __syncthreads();
+  }
+  E6 r = threadIdx.x == 0 ? scan_id<MUL>() : ldc6(sh + 6 * (threadIdx.x - 1));
+  __syncthreads();
+  return r;
+}
+
+template <bool MUL>
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* const* in, uint32_t* agg, int n) {
+  __shared__ uint32_t sh[SCAN_T * 6];
+  const int b = blockIdx.y, tile = blockIdx.x, nt = gridDim.x;
+  const uint32_t* X = in[b];
+  E6 acc = scan_id<MUL>();
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) acc = scan_op<MUL>(acc, ld6(X, n, (size_t)tile * SCAN_TILE + k * SCAN_T + threadIdx.x));
+  const E6 r = block_reduce<MUL>(acc, sh);
+  if (threadIdx.x < 6) agg[((size_t)b * nt + tile) * 6 + threadIdx.x] = r.v[threadIdx.x];
+}
+
+// agg[b][tile] -> exclusive prefix of the tile aggregates; tot[b] (optional)
+// = the product/sum of all of them.
+template <bool MUL>
+__global__ __launch_bounds__(1024) void k_scan_tiles(uint32_t* agg, uint32_t* tot, int nt) {
+  __shared__ uint32_t sh[1024 * 6];
+  const int b = blockIdx.x;
+  uint32_t* A = agg + (size_t)b * nt * 6;
+  const int c = nt / blockDim.x;
+  const int lo = threadIdx.x * c;
+  E6 acc = scan_id<MUL>();
+  for (int i = lo; i < lo + c; ++i) acc = scan_op<MUL>(acc, ldc6(A + 6 * i));
+  E6 run = block_exclusive<MUL>(acc, sh);
+  for (int i = lo; i < lo + c; ++i) {
+    const E6 a = ldc6(A + 6 * i);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) A[6 * i + l] = run.v[l];
+    run = scan_op<MUL>(run, a);
+  }
+  if (tot && threadIdx.x == blockDim.x - 1) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) tot[6 * b + l] = run.v[l];
+  }
+}
+
+template <bool MUL, bool INCL>
+__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t* const* in, uint32_t* const* out, const uint32_t* pre,
+                                                    int n) {
+  __shared__ uint32_t tl[6 * SCAN_TILE];
+  __shared__ uint32_t sh[SCAN_T * 6];
+  const int b = blockIdx.y, tile = blockIdx.x, nt = gridDim.x;
+  const uint32_t* X = in[b];
+  const size_t base = (size_t)tile * SCAN_TILE;
+#pragma unroll
+  for (int l = 0; l < 6; ++l)
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k)
+      tl[l * SCAN_TILE + k * SCAN_T + threadIdx.x] = X[(size_t)l * n + base + k * SCAN_T + threadIdx.x];
+  __syncthreads();
+  E6 v[SCAN_PER];
+  E6 acc = scan_id<MUL>();
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+#pragma unroll
+    for (int l = 0; l < 6; ++l) v[k].v[l] = tl[l * SCAN_TILE + SCAN_PER * threadIdx.x + k];
+    acc = scan_op<MUL>(acc, v[k]);
+  }
+  E6 run = scan_op<MUL>(ldc6(pre + ((size_t)b * nt + tile) * 6), block_exclusive<MUL>(acc, sh));
+#pragma unroll
+  for (int k = 0; k < SCAN_PER; ++k) {
+    if constexpr (INCL) run = scan_op<MUL>(run, v[k]);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) tl[l * SCAN_TILE + SCAN_PER * threadIdx.x + k] = run.v[l];
+    if constexpr (!INCL) run = scan_op<MUL>(run, v[k]);
+  }
+  __syncthreads();
+  uint32_t* Y = out[b];
+#pragma unroll
+  for (int l = 0; l < 6; ++l)
+#pragma unroll
+    for (int k = 0; k < SCAN_PER; ++k)
+      Y[(size_t)l * n + base + k * SCAN_T + threadIdx.x] = tl[l * SCAN_TILE + k * SCAN_T + threadIdx.x];
+}
+
+size_t scan_tmp_words(int n, int batch) { return n >= SCAN_MIN_TILED ? (size_t)batch * (n / SCAN_TILE) * 6 : 0; }
+
+template <bool MUL, bool INCL>
+static void launch_scan_tiled(const uint32_t* const* in, uint32_t* const* out, uint32_t* tmp, uint32_t* tot, int n,
+                              int batch, hipStream_t s) {
+  if (!tmp) throw std::runtime_error("scan: scratch missing");
+  const int nt = n / SCAN_TILE;
+  prelaunch();
+  hipLaunchKernelGGL(k_scan_reduce<MUL>, dim3(nt, batch), dim3(SCAN_T), 0, s, in, tmp, n);
+  LG_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_scan_tiles<MUL>, dim3(batch), dim3(nt < 1024 ? nt : 1024), 0, s, tmp, tot, nt);
+  LG_HIP(hipGetLastError());
+  hipLaunchKernelGGL((k_scan_apply<MUL, INCL>), dim3(nt, batch), dim3(SCAN_T), 0, s, in, out, (const uint32_t*)tmp, n);
+  LG_HIP(hipGetLastError());
+}
+
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
-                          uint32_t* const* gp, int n, int batch, hipStream_t s) {
+                          uint32_t* const* gp, int n, int batch, uint32_t* scan_tmp, hipStream_t s) {
   const int T = n < 256 ? n : 256;
   prelaunch();
   hipLaunchKernelGGL(k_ratio, dim3((n + T - 1) / T, batch), dim3(T), 0, s, num, den, ratio, n);
   LG_HIP(hipGetLastError());
+  if (n >= SCAN_MIN_TILED) {
+    launch_scan_tiled<true, false>((const uint32_t* const*)ratio, gp, scan_tmp, nullptr, n, batch, s);
+    return;
+  }
   int T2 = n < 1024 ? n : 1024;
   prelaunch();
   hipLaunchKernelGGL(k_prefix_product, dim3(batch), dim3(T2), 0, s, (const uint32_t* const*)ratio, gp, n);
@@ -899,11 +1052,15 @@ __global__ void k_logup_finish(uint32_t* const* col, const uint32_t* tot, int n,
 }
 
 void launch_cyclic_logup_finish(const uint32_t* const* acc, uint32_t* const* col, uint32_t* tot, int n,
-                                uint32_t ninv, int batch, hipStream_t s) {
-  int T2 = n < 1024 ? n : 1024;
-  prelaunch();
-  hipLaunchKernelGGL(k_prefix_sum, dim3(batch), dim3(T2), 0, s, acc, col, tot, n);
-  LG_HIP(hipGetLastError());
+                                uint32_t ninv, int batch, uint32_t* scan_tmp, hipStream_t s) {
+  if (n >= SCAN_MIN_TILED) {
+    launch_scan_tiled<false, true>(acc, col, scan_tmp, tot, n, batch, s);
+  } else {
+    int T2 = n < 1024 ? n : 1024;
+    prelaunch();
+    hipLaunchKernelGGL(k_prefix_sum, dim3(batch), dim3(T2), 0, s, acc, col, tot, n);
+    LG_HIP(hipGetLastError());
+  }
   const int T = n < 256 ? n : 256;
   prelaunch();
   hipLaunchKernelGGL(k_logup_finish, dim3((n + T - 1) / T, batch), dim3(T), 0, s, col, (const uint32_t*)tot, n, ninv);

@@ -945,8 +945,13 @@ void Job::step_grand_product(size_t r, size_t s) {
     gp[b] = cols_[st.out_col].lag(b, 0);
     rp[b] = ratio.ptr(b);
   }
+  uint32_t* scan_tmp = nullptr;
+  if (scan_tmp_words((int)n, B_)) {
+    keep_.push_back(dev(scan_tmp_words((int)n, B_)));
+    scan_tmp = keep_.back().as<uint32_t>();
+  }
   launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(rp)),
-                       const_cast<uint32_t* const*>(stage(gp)), n, B_, s_);
+                       const_cast<uint32_t* const*>(stage(gp)), n, B_, scan_tmp, s_);
   keep_.push_back(std::move(ratio.buf));
   keep_.push_back(std::move(nd[0].buf));
   keep_.push_back(std::move(nd[1].buf));
@@ -1010,8 +1015,13 @@ void Job::step_logup(size_t r, size_t s) {
     keep_.push_back(std::move(mm.buf));
   }
   DevBuf tot = dev((size_t)B_ * 6);
+  uint32_t* scan_tmp = nullptr;
+  if (scan_tmp_words((int)m.N, B_)) {
+    keep_.push_back(dev(scan_tmp_words((int)m.N, B_)));
+    scan_tmp = keep_.back().as<uint32_t>();
+  }
   launch_cyclic_logup_finish(ap_d, const_cast<uint32_t* const*>(stage(lp)), tot.as<uint32_t>(), (int)m.N,
-                             kb_inv(M(m.N)), B_, s_);
+                             kb_inv(M(m.N)), B_, scan_tmp, s_);
   keep_.push_back(std::move(acc.buf));
   totals_dev_.push_back({st.outs[1], std::move(tot)});
 }
@@ -1226,8 +1236,10 @@ void Job::run() {
   for (size_t r = 0; r < nr; ++r) {
     const Round& rd = d_.rounds[r];
     run_steps(r);
+    trace(("steps " + std::to_string(r)).c_str());
     if (rd.tree != PLAN_NONE) {
       commit_tree(rd.tree);
+      trace(("commit " + std::to_string(r)).c_str());
       for (int b = 0; b < B_; ++b) ts_[b].bind(challenge_name(r), roots_[rd.tree][b].v, 8);
     }
     finish_round_steps(r);

@@ -150,7 +150,7 @@ def test_merkle(torch, ctx, logn, inj):
     assert np.array_equal(got[1:], t.nodes.T[1:])
 
 
-@pytest.mark.parametrize("logn", [0, 3, 6, 10, 14])
+@pytest.mark.parametrize("logn", [0, 3, 6, 10, 14, 16, 17])
 def test_grand_product(torch, ctx, logn):
     from loom_amd import _lib
     from oracle import field as F
@@ -170,6 +170,26 @@ def test_grand_product(torch, ctx, logn):
     for i in range(1, n):
         want[i] = F.e6_mul(want[i - 1], ratio[i - 1])
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("logn", [20, 22])
+def test_grand_product_large(torch, ctx, logn):
+    """Multi-workgroup scans (tiles of 2048 rows, tile prefixes scanned per
+    instance): gp[0] = 1 and gp[i+1] den[i] = gp[i] num[i] for every row."""
+    from loom_amd import _lib
+    from oracle import field as F
+    log("grand product n=2^%d" % logn)
+    n = 1 << logn
+    rng = np.random.default_rng(logn + 5)
+    num = rng.integers(1, P, size=(n, 6), dtype=np.uint64)
+    den = rng.integers(1, P, size=(n, 6), dtype=np.uint64)
+    dn = dev_u32(torch, mont(num.T.copy()).reshape(-1))
+    dd = dev_u32(torch, mont(den.T.copy()).reshape(-1))
+    dg = torch.zeros(6 * n, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().lg_grand_product(ctx.h, ptr(dn), ptr(dd), ptr(dg), n), "lg_grand_product")
+    got = canon(host_u32(dg)).reshape(6, n).T
+    assert list(got[0]) == [1, 0, 0, 0, 0, 0]
+    assert np.array_equal(F.e6_mul(got[1:], den[:-1]), F.e6_mul(got[:-1], num[:-1]))
 
 
 def _oracle_words(prog, trace, nq):
@@ -228,7 +248,7 @@ def test_prove_uneven_batched(ctx):
         assert np.array_equal(w, enc)
 
 
-@pytest.mark.parametrize("log_ns,nq", [((4, 6, 3), 12), ((10, 12, 9), 20)])
+@pytest.mark.parametrize("log_ns,nq", [((4, 6, 3), 12), ((10, 12, 9), 20), ((16, 9), 16)])
 def test_prove_range_lookups(ctx, log_ns, nq):
     """Modules with a logup range lookup into the shared range_256 module:
     multiplicity histograms, cyclic logup columns, exposed totals bound to
