@@ -38,17 +38,20 @@ LG_HD uint32_t kb_sub(uint32_t a, uint32_t b) {
 
 LG_HD uint32_t kb_neg(uint32_t a) { return a ? KB_P - a : 0u; }
 
-// Montgomery product a*b*2^-32 mod p for a, b < p.
+// Montgomery product a*b*2^-32 mod p for a, b < p.  With m = -t p^-1 mod 2^32
+// (-p^-1 = 0x7effffff), t + m p is divisible by 2^32 and below
+// 2^62 + 2^63, and (t + m p) / 2^32 < p^2/2^32 + p < 2p: one 64-bit
+// multiply-add and one conditional subtraction (on gfx950 a v_mad_u64_u32 with
+// the accumulator, then v_add + v_min).
 LG_HD uint32_t kb_mul(uint32_t a, uint32_t b) {
-  uint64_t t = (uint64_t)a * b;
-  // m = t_lo * p^-1 mod 2^32 with p^-1 = 2^31 + 2^24 + 1: shifts and adds
-  // instead of a 32-bit multiply
-  const uint32_t lo = (uint32_t)t;
-  uint32_t m = lo + (lo << 24) + (lo << 31);
-  uint32_t th = (uint32_t)(t >> 32);
-  uint32_t mh = (uint32_t)(((uint64_t)m * KB_P) >> 32);
-  uint32_t r = th - mh;
-  return th < mh ? r + KB_P : r;
+  const uint64_t t = (uint64_t)a * b;
+  const uint32_t m = (uint32_t)t * 0x7effffffu;
+  const uint32_t r = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_elementwise_min(r, r - KB_P);
+#else
+  return r >= KB_P ? r - KB_P : r;
+#endif
 }
 
 LG_HD uint32_t kb_dbl(uint32_t a) { return kb_add(a, a); }
@@ -61,8 +64,24 @@ template <int K>
 LG_HD uint32_t kb_div_pow2(uint32_t a) {
   static_assert(K >= 1 && K <= 24, "K out of range");
   const uint32_t m = (0u - a) & ((1u << K) - 1u);
+#if defined(__HIP_DEVICE_COMPILE__)
+  // m (2^(31-K) - 2^(24-K)) = (127 m) 2^(24-K) with 127 m < 2^24 for K <= 17:
+  // a full-rate 24-bit multiply and a shift-add.  The empty asm keeps the
+  // compiler from folding them back into one 64-bit multiply-add (a
+  // quarter-rate instruction) when the constant exceeds 24 bits (K < 7).
+  uint32_t y;
+  if constexpr (K < 7) {
+    uint32_t q = __umul24(m, 127u);
+    asm("" : "+v"(q));
+    y = (q << (24 - K)) + ((a + m) >> K);
+  } else {
+    y = ((a + m) >> K) + (m << (31 - K)) - (m << (24 - K));
+  }
+  return __builtin_elementwise_min(y, y - KB_P);
+#else
   const uint32_t y = ((a + m) >> K) + (m << (31 - K)) - (m << (24 - K));
   return y >= KB_P ? y - KB_P : y;
+#endif
 }
 
 LG_HD uint32_t kb_to_mont(uint32_t x) { return kb_mul(x % KB_P, KB_R2); }

@@ -72,3 +72,21 @@ def test_radix16_passes_match_dft(ntt_bin, logn):
     want = ntt.evaluate(x)
     assert np.array_equal(fwd, want[F.bitrev_indices(n)])
     assert np.array_equal(inv, x * np.uint64(n) % np.uint64(P))
+
+
+def test_montgomery_product_matches_closed_form():
+    """kb_mul (one 64-bit multiply-add Montgomery reduction) equals
+    a*b*2^-32 mod p computed in 128-bit integers."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tempfile.mkdtemp()
+    try:
+        exe = os.path.join(d, "kb_mul_check")
+        subprocess.run([gxx, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(HERE, "host", "kb_mul_check.cpp"),
+                        "-o", exe], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True)
+        assert r.stdout.strip() == "kb_mul_mismatches 0"
+        assert r.returncode == 0
+    finally:
+        shutil.rmtree(d)
