@@ -168,6 +168,11 @@ void launch_deep(const DeepDesc& d, int batch, hipStream_t s);
 
 // FRI fold (fri.go:1177-1198): next[i] = (p+q)/2 + alpha*(p-q)/2*xinv[i],
 // p = layer[2i], q = layer[2i+1]; E6 layers as 6 limb arrays.
+// FRI fold challenge per instance on the device: digest of the sponge over
+// prefix (host memory, n_prefix <= 32 words) || prev[8 b ..] || root of
+// trees[b]; writes dig[8 b ..] and alpha[6 b ..] (first 6 limbs).
+void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* prev, uint32_t* const* trees,
+                          uint32_t* dig, uint32_t* alpha, int batch, hipStream_t s);
 void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
                      int half, int batch, hipStream_t s);
 // running[k] += gamma * level[k]

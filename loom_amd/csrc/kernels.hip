@@ -1246,6 +1246,66 @@ void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint
   LG_HIP(hipGetLastError());
 }
 
+// Fiat-Shamir challenge of one FRI fold round on the device, so the commit
+// phase of FRI runs without a host round trip per round.  Per instance the
+// message is prefix (StringToElements(FSID, name), shared) || prev[b] (the
+// previous challenge's digest) || the root of trees[b] (node 1), absorbed by
+// the rate-16 overwrite sponge (internal/fiat-shamir/transcript.go:285-315;
+// internal/hash/poseidon2.go:165-255).  One octet of lanes per instance
+// (P2Coop); the digest goes to dig[b] (8 words) and the E6 challenge, its
+// first 6 limbs, to alpha[b].
+constexpr int CHAL_MAX_PREFIX = 32;
+struct ChalPrefix {
+  uint32_t w[CHAL_MAX_PREFIX];
+};
+
+__global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre, const uint32_t* __restrict__ prev,
+                                                      uint32_t* const* __restrict__ trees, uint32_t* __restrict__ dig,
+                                                      uint32_t* __restrict__ alpha, int batch) {
+  const int q = threadIdx.x & 7;
+  const int b_raw = blockIdx.x * 8 + (threadIdx.x >> 3);
+  const bool live = b_raw < batch;
+  const int b = live ? b_raw : batch - 1;
+  const uint32_t* root = trees[b] + 8;
+  P2Coop P;
+  P.init(q);
+  uint32_t x[4] = {0u, 0u, 0u, 0u};
+  const int len = n_pre + 16;
+  for (int off = 0; off < len; off += 16) {
+    const int k = len - off < 16 ? len - off : 16;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int idx = 4 * q + e;
+      if (q < 6 && idx < k) {
+        const int i = off + idx;
+        x[e] = i < n_pre ? pre.w[i] : (i < n_pre + 8 ? prev[8 * b + (i - n_pre)] : root[i - n_pre - 8]);
+      }
+    }
+    P.permute(x);
+  }
+  if (live && q < 2) {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) dig[8 * b + 4 * q + e] = x[e];
+    alpha[6 * b + 4 * q] = x[0];
+    alpha[6 * b + 4 * q + 1] = x[1];
+    if (q == 0) {
+      alpha[6 * b + 2] = x[2];
+      alpha[6 * b + 3] = x[3];
+    }
+  }
+}
+
+void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* prev, uint32_t* const* trees,
+                          uint32_t* dig, uint32_t* alpha, int batch, hipStream_t s) {
+  if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("fri challenge: bad prefix length");
+  ChalPrefix pre{};
+  for (int i = 0; i < n_prefix; ++i) pre.w[i] = prefix[i];
+  prelaunch();
+  hipLaunchKernelGGL(k_fri_challenge, dim3((batch + 7) / 8), dim3(64), 0, s, pre, n_prefix, prev, trees, dig, alpha,
+                     batch);
+  LG_HIP(hipGetLastError());
+}
+
 __global__ void k_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= len) return;

@@ -1529,7 +1529,16 @@ void Job::open() {
   running.buf = dev((size_t)B_ * 6 * running.len);
   LG_HIP(hipMemcpyAsync(running.buf.as<void>(), lvp[0]->buf.as<void>(), (size_t)B_ * 6 * running.len * 4,
                         hipMemcpyDeviceToDevice, s_));
-  std::vector<const uint32_t*> layer_of(FRI_ROUNDS, nullptr);  // device base of layer j (instance 0)
+  // Fold challenges: round `first` binds the levels root (on the host, which
+  // has it); later rounds bind the root of the running layer's tree, and
+  // their challenges are computed on the device (launch_fri_challenge), so
+  // the rounds are enqueued without waiting for the host.  The host replays
+  // the roots and digests into the transcripts after the last round.
+  const size_t fb = (size_t)B_ * 8;
+  DevBuf chal_dig = dev((size_t)FRI_ROUNDS * fb);
+  DevBuf chal_alpha = dev((size_t)FRI_ROUNDS * B_ * 6);
+  DevBuf root_dev = dev((size_t)FRI_ROUNDS * fb);
+  const uint32_t* prev_dig = nullptr;
   for (int j = first; j < FRI_ROUNDS; ++j) {
     for (size_t l = 1; l < L; ++l) {
       const int jl = FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n;
@@ -1546,21 +1555,31 @@ void Job::open() {
                          reinterpret_cast<const uint32_t*>(stg_->put(gm.data(), gm.size() * sizeof(E6), s_)),
                          (int)running.len, B_, s_);
     }
-    std::vector<Digest> root;
+    uint32_t* dig_j = chal_dig.as<uint32_t>() + (size_t)j * fb;
+    const uint32_t* alpha_j = chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6;
     if (j == first) {
-      root = levels_root;
+      std::vector<uint32_t> dg((size_t)B_ * 8), al((size_t)B_ * 6);
+      parallel_for(B_, [&](int b) {
+        const std::string nf = "fri_fold_" + std::to_string(j);
+        ts_[b].bind(nf, levels_root[b].v, 8);
+        const Digest& d = ts_[b].compute(nf);
+        for (int k = 0; k < 8; ++k) dg[(size_t)8 * b + k] = d.v[k];
+        for (int k = 0; k < 6; ++k) al[(size_t)6 * b + k] = d.v[k];
+      });
+      // staged words reach the device with the next launch's staging copy
+      prev_dig = stage(dg);
+      alpha_j = stage(al);
     } else {
       ext_tree(ftrees[j], {&running});
-      root = roots_of(ftrees[j]);
-      fri_roots.push_back(root);
+      std::vector<uint32_t*> tp(B_);
+      for (int b = 0; b < B_; ++b) tp[b] = ftrees[j].node_ptr(b);
+      uint32_t* const* tp_d = const_cast<uint32_t* const*>(stage(tp));
+      launch_copy_roots(tp_d, root_dev.as<uint32_t>() + (size_t)j * fb, B_, s_);
+      const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, "fri_fold_" + std::to_string(j));
+      launch_fri_challenge(pre.data(), (int)pre.size(), prev_dig, tp_d, dig_j,
+                           chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6, B_, s_);
+      prev_dig = dig_j;
     }
-    std::vector<E6> al(B_);
-    parallel_for(B_, [&](int b) {
-      const std::string nf = "fri_fold_" + std::to_string(j);
-      ts_[b].bind(nf, root[b].v, 8);
-      const Digest& dg = ts_[b].compute(nf);
-      for (int k = 0; k < 6; ++k) al[b].v[k] = dg.v[k];
-    });
     Layer next;
     next.len = running.len / 2;
     next.buf = dev((size_t)B_ * 6 * next.len);
@@ -1575,14 +1594,35 @@ void Job::open() {
     {
       const uint32_t* const* ip_d = stage(ip);
       uint32_t* const* op_d = const_cast<uint32_t* const*>(stage(op));
-      const uint32_t* al_d = reinterpret_cast<const uint32_t*>(stg_->put(al.data(), al.size() * sizeof(E6), s_));
       const uint32_t* xi = tb_.fri_xinv2(log_len);
       TimeIt tm(ctx_, s_, "fri_fold", (double)B_ * running.len * 1.5 * 24, 0);
-      launch_fri_fold(ip_d, op_d, al_d, xi, (int)next.len, B_, s_);
+      launch_fri_fold(ip_d, op_d, alpha_j, xi, (int)next.len, B_, s_);
     }
     layers[j] = std::move(running);
     running = std::move(next);
   }
+  {
+    std::vector<uint32_t> dg((size_t)FRI_ROUNDS * fb), rt((size_t)FRI_ROUNDS * fb);
+    LG_HIP(hipMemcpyAsync(dg.data(), chal_dig.as<void>(), dg.size() * 4, hipMemcpyDeviceToHost, s_));
+    LG_HIP(hipMemcpyAsync(rt.data(), root_dev.as<void>(), rt.size() * 4, hipMemcpyDeviceToHost, s_));
+    sync();
+    for (int j = first + 1; j < FRI_ROUNDS; ++j) {
+      std::vector<Digest> root(B_);
+      for (int b = 0; b < B_; ++b)
+        for (int k = 0; k < 8; ++k) root[b].v[k] = rt[(size_t)j * fb + (size_t)8 * b + k];
+      fri_roots.push_back(root);
+    }
+    parallel_for(B_, [&](int b) {
+      for (int j = first + 1; j < FRI_ROUNDS; ++j) {
+        const std::string nf = "fri_fold_" + std::to_string(j);
+        ts_[b].bind(nf, fri_roots[j - first - 1][b].v, 8);
+        ts_[b].accept(nf, &dg[(size_t)j * fb + (size_t)8 * b]);
+      }
+    });
+  }
+  keep_.push_back(std::move(chal_dig));
+  keep_.push_back(std::move(chal_alpha));
+  keep_.push_back(std::move(root_dev));
   trace("fri rounds");
   // final polynomial (running after the last round): 4 E6 values
   const uint32_t flen = running.len;
@@ -1590,17 +1630,21 @@ void Job::open() {
   LG_HIP(hipMemcpyAsync(final_w.data(), running.buf.as<void>(), final_w.size() * 4, hipMemcpyDeviceToHost, s_));
   sync();
   std::vector<uint32_t> pos((size_t)B_ * nq);
+  // the query challenges are registered consecutively; their name prefixes
+  // are the same for every instance
+  std::vector<std::vector<uint32_t>> qpre(nq);
+  for (int k = 0; k < nq; ++k) qpre[k] = string_to_elements(TAG_FSID, "fri_query_" + std::to_string(k));
   parallel_for(B_, [&](int b) {
     std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
     for (uint32_t k = 0; k < flen; ++k)
       for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
     ts_[b].bind("fri_query_0", fp);
+    const size_t p0 = ts_[b].index("fri_query_0");
     for (int k = 0; k < nq; ++k) {
-      const std::string qn = "fri_query_" + std::to_string(k);
-      Digest dg = ts_[b].compute(qn);
+      const Digest& dg = ts_[b].compute_at(p0 + k, qpre[k]);
       uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
       pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
-      if (k < nq - 1) ts_[b].bind("fri_query_" + std::to_string(k + 1), dg.v, 8);
+      if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);
     }
   });
   trace("queries (host)");

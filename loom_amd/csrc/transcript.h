@@ -63,6 +63,46 @@ class Transcript {
     log_.push_back(p);
     return values_[p];
   }
+  // Position-based forms for long runs of consecutive challenges (the FRI
+  // queries): `prefix` must be string_to_elements(TAG_FSID, name at p).
+  size_t index(const std::string& name) const { return at(name); }
+  void bind_at(size_t p, const uint32_t* e, size_t n) {
+    if (p >= names_.size() || computed_[p]) throw std::runtime_error("transcript: bad bind position");
+    bindings_[p].insert(bindings_[p].end(), e, e + n);
+  }
+  const Digest& compute_at(size_t p, const std::vector<uint32_t>& prefix) {
+    if (p >= names_.size()) throw std::runtime_error("transcript: bad challenge position");
+    if (computed_[p]) return values_[p];
+    if (p != 0 && !computed_[p - 1]) throw std::runtime_error("transcript: previous challenge not computed: " + names_[p]);
+    uint32_t small[64];
+    const size_t len = prefix.size() + (p != 0 ? 8 : 0) + bindings_[p].size();
+    std::vector<uint32_t> big;
+    uint32_t* msg = small;
+    if (len > 64) {
+      big.resize(len);
+      msg = big.data();
+    }
+    size_t o = 0;
+    for (uint32_t x : prefix) msg[o++] = x;
+    if (p != 0)
+      for (int i = 0; i < 8; ++i) msg[o++] = values_[p - 1].v[i];
+    for (uint32_t x : bindings_[p]) msg[o++] = x;
+    p2_sponge_host(msg, len, values_[p].v);
+    computed_[p] = true;
+    log_.push_back(p);
+    return values_[p];
+  }
+  // Records the digest of `name` as computed elsewhere (the device) from this
+  // transcript's message for it (string_to_elements(FSID, name) || previous
+  // digest || bindings); the same order checks as compute().
+  void accept(const std::string& name, const uint32_t* digest) {
+    size_t p = at(name);
+    if (computed_[p]) throw std::runtime_error("transcript: challenge already computed: " + name);
+    if (p != 0 && !computed_[p - 1]) throw std::runtime_error("transcript: previous challenge not computed: " + name);
+    for (int i = 0; i < 8; ++i) values_[p].v[i] = digest[i];
+    computed_[p] = true;
+    log_.push_back(p);
+  }
   // Digests in computation order.
   std::vector<uint32_t> log_words() const {
     std::vector<uint32_t> w;

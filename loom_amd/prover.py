@@ -98,6 +98,43 @@ class Prepared:
 _proof_cap = {}
 
 
+class _OutputPool:
+    """Reusable host buffers for proof words.  A proof is returned as a view
+    of one buffer; the buffer is handed out again only once no view of it is
+    referenced any more, so results stay valid for as long as the caller keeps
+    them.  Reuse avoids the page faults of fresh multi-megabyte allocations
+    on every call (first-touch costs ~0.4 ms per MB on the host)."""
+
+    MAX_PER_CAP = 256
+
+    def __init__(self):
+        import threading
+        self._lock = threading.Lock()
+        self._bufs = {}
+
+    def take(self, cap, n):
+        import sys
+        with self._lock:
+            pool = self._bufs.setdefault(cap, [])
+            out = []
+            for a in pool:
+                if len(out) == n:
+                    break
+                # referenced by the pool list, the loop variable and the call
+                # argument only: no view of it is alive
+                if sys.getrefcount(a) <= 3:
+                    out.append(a)
+            while len(out) < n:
+                a = np.empty(cap, dtype=np.uint32)
+                if len(pool) < self.MAX_PER_CAP:
+                    pool.append(a)
+                out.append(a)
+            return out
+
+
+_out_pool = _OutputPool()
+
+
 class DeviceInputs:
     """Prepared instances with their trace columns and permutations copied
     into device memory (torch tensors on `device`), for lg_prove_device."""
@@ -150,7 +187,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
     entry = _lib.lib().lg_prove if dev_inputs is None else _lib.lib().lg_prove_device
     cap = _proof_cap.get(words.tobytes(), 1 << 21)
     while True:
-        outs = [np.empty(cap, dtype=np.uint32) for _ in range(B)]
+        outs = _out_pool.take(cap, B)
         touts = [np.zeros(1 << 16, dtype=np.uint32) for _ in range(B)] if want_transcript else None
         out_ptrs = (_lib.u32p * B)(*[o.ctypes.data_as(_lib.u32p) for o in outs])
         caps = (ctypes.c_size_t * B)(*([cap] * B))
