@@ -26,14 +26,27 @@ constexpr uint32_t KB_PINV = 0x81000001u;   // p^-1 mod 2^32
 constexpr uint32_t KB_R = 0x01fffffeu;      // 2^32 mod p  (Montgomery one)
 constexpr uint32_t KB_R2 = 0x17f7efe4u;     // 2^64 mod p
 
+// On the device the conditional corrections are unsigned minima: a + b < 2p,
+// and min(s, s - p) is s - p exactly when s >= p (else s - p wraps above s);
+// for a - b, min(d, d + p) is d + p exactly when the difference wrapped.  This
+// keeps them to three VALU instructions without a compare into VCC (and the
+// hazard wait a compare-then-select pair needs).
 LG_HD uint32_t kb_add(uint32_t a, uint32_t b) {
-  uint32_t r = a + b;
+  const uint32_t r = a + b;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_elementwise_min(r, r - KB_P);
+#else
   return r >= KB_P ? r - KB_P : r;
+#endif
 }
 
 LG_HD uint32_t kb_sub(uint32_t a, uint32_t b) {
-  uint32_t r = a - b;
+  const uint32_t r = a - b;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_elementwise_min(r, r + KB_P);
+#else
   return a < b ? r + KB_P : r;
+#endif
 }
 
 LG_HD uint32_t kb_neg(uint32_t a) { return a ? KB_P - a : 0u; }

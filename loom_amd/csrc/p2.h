@@ -143,13 +143,31 @@ LG_HD void p2_diag_all(uint32_t* s, uint32_t sum) {
   }
 }
 
+// Sum of the W state elements mod p.  On the device: pair sums of two
+// elements below p fit 32 bits, the W/2 pair sums accumulate in 64 bits
+// (v_lshl_add_u64), and one reduction with 2^31 = 2^24 - 1 (mod p) follows:
+// acc = hi 2^31 + lo with hi < W, so lo + hi 2^24 - hi < 2^31 + 2^29 < 2p.
+template <int W>
+LG_HD uint32_t p2_state_sum(const uint32_t* s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  static_assert(W % 2 == 0 && W <= 32, "pairwise sum needs an even width up to 32");
+  uint64_t acc = 0;
+#pragma unroll
+  for (int i = 0; i < W; i += 2) acc += (uint64_t)(s[i] + s[i + 1]);
+  const uint32_t lo = (uint32_t)acc & 0x7fffffffu, hi = (uint32_t)(acc >> 31);
+  const uint32_t r = lo + (hi << 24) - hi;
+  return __builtin_elementwise_min(r, r - KB_P);
+#else
+  uint32_t sum = s[0];
+  for (int i = 1; i < W; ++i) sum = kb_add(sum, s[i]);
+  return sum;
+#endif
+}
+
 template <int W>
 LG_HD void p2_partial_round(uint32_t* s, int r) {
   s[0] = kb_cube(kb_add(s[0], P2Consts<W>::partial(r)));
-  uint32_t sum = s[0];
-#pragma unroll
-  for (int i = 1; i < W; ++i) sum = kb_add(sum, s[i]);
-  p2_diag_all<W, 0>(s, sum);
+  p2_diag_all<W, 0>(s, p2_state_sum<W>(s));
 }
 
 template <int W>
