@@ -25,6 +25,40 @@ constexpr uint32_t TAG_NODE_MONT = 0x6425c2feu;  // Montgomery form of TAG_NODE
 
 template <int W>
 LG_HD void p2_external(uint32_t* s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Device form with one reduction per output: the M4 outputs (coefficient
+  // sum 7) and the chunk-wise sums are formed in 64 bits from 32-bit pair
+  // sums (x0 + x1 < 2p fits), so every output is below 49p < 49 * 2^31;
+  // with v = hi 2^31 + lo and 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is
+  // below 2^31 + 49 * 2^24 < 2p, and one unsigned minimum finishes.
+  uint64_t y[W];
+#pragma unroll
+  for (int c = 0; c < W / 4; ++c) {
+    const uint32_t x0 = s[4 * c], x1 = s[4 * c + 1], x2 = s[4 * c + 2], x3 = s[4 * c + 3];
+    const uint32_t a = x0 + x1, b = x2 + x3;
+    const uint64_t t0123 = (uint64_t)a + b;
+    const uint64_t t01123 = t0123 + x1, t01233 = t0123 + x3;
+    y[4 * c + 0] = t01123 + a;
+    y[4 * c + 1] = t01123 + 2 * (uint64_t)x2;
+    y[4 * c + 2] = t01233 + b;
+    y[4 * c + 3] = t01233 + 2 * (uint64_t)x0;
+  }
+  uint64_t sum[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    uint64_t acc = y[j];
+#pragma unroll
+    for (int c = 1; c < W / 4; ++c) acc += y[4 * c + j];
+    sum[j] = acc;
+  }
+#pragma unroll
+  for (int i = 0; i < W; ++i) {
+    const uint64_t v = y[i] + sum[i & 3];
+    const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
+    const uint32_t r = lo + (hi << 24) - hi;
+    s[i] = __builtin_elementwise_min(r, r - KB_P);
+  }
+#else
 #pragma unroll
   for (int c = 0; c < W / 4; ++c) {
     uint32_t x0 = s[4 * c], x1 = s[4 * c + 1], x2 = s[4 * c + 2], x3 = s[4 * c + 3];
@@ -48,6 +82,7 @@ LG_HD void p2_external(uint32_t* s) {
   }
 #pragma unroll
   for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], sum[i & 3]);
+#endif
 }
 
 template <int W>
@@ -67,7 +102,15 @@ struct P2Consts<16> {
   static LG_HD uint32_t diag(int i) { return P2_W16_DIAG[i]; }
 };
 
-LG_HD uint32_t kb_cube(uint32_t x) { return kb_mul(kb_mul(x, x), x); }
+// x^3 for x < p.  The square skips its final correction: it is below 2p, and
+// a Montgomery product of a < 2p and b < p stays exact (t = ab < 2p^2 < 2^63,
+// t + m p < 2^64) and below 2p before kb_mul's own correction.
+LG_HD uint32_t kb_cube(uint32_t x) {
+  const uint64_t t = (uint64_t)x * x;
+  const uint32_t m = (uint32_t)t * 0x7effffffu;
+  const uint32_t sq = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
+  return kb_mul(sq, x);
+}
 
 template <int W>
 LG_HD void p2_full_round(uint32_t* s, int r) {
