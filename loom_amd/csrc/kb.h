@@ -26,6 +26,15 @@ constexpr uint32_t KB_PINV = 0x81000001u;   // p^-1 mod 2^32
 constexpr uint32_t KB_R = 0x01fffffeu;      // 2^32 mod p  (Montgomery one)
 constexpr uint32_t KB_R2 = 0x17f7efe4u;     // 2^64 mod p
 
+// Unsigned minimum (v_min_u32 on the device).
+LG_HD uint32_t kb_umin(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_elementwise_min(a, b);
+#else
+  return a < b ? a : b;
+#endif
+}
+
 // On the device the conditional corrections are unsigned minima: a + b < 2p,
 // and min(s, s - p) is s - p exactly when s >= p (else s - p wraps above s);
 // for a - b, min(d, d + p) is d + p exactly when the difference wrapped.  This

@@ -25,12 +25,11 @@ constexpr uint32_t TAG_NODE_MONT = 0x6425c2feu;  // Montgomery form of TAG_NODE
 
 template <int W>
 LG_HD void p2_external(uint32_t* s) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  // Device form with one reduction per output: the M4 outputs (coefficient
-  // sum 7) and the chunk-wise sums are formed in 64 bits from 32-bit pair
-  // sums (x0 + x1 < 2p fits), so every output is below 49p < 49 * 2^31;
-  // with v = hi 2^31 + lo and 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is
-  // below 2^31 + 49 * 2^24 < 2p, and one unsigned minimum finishes.
+  // One reduction per output: the M4 outputs (coefficient sum 7) and the
+  // chunk-wise sums are formed in 64 bits from 32-bit pair sums (x0 + x1 < 2p
+  // fits), so every output is below 49p < 49 * 2^31; with v = hi 2^31 + lo
+  // and 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is below
+  // 2^31 + 49 * 2^24 < 2p, and one unsigned minimum finishes.
   uint64_t y[W];
 #pragma unroll
   for (int c = 0; c < W / 4; ++c) {
@@ -56,33 +55,8 @@ LG_HD void p2_external(uint32_t* s) {
     const uint64_t v = y[i] + sum[i & 3];
     const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
     const uint32_t r = lo + (hi << 24) - hi;
-    s[i] = __builtin_elementwise_min(r, r - KB_P);
+    s[i] = kb_umin(r, r - KB_P);
   }
-#else
-#pragma unroll
-  for (int c = 0; c < W / 4; ++c) {
-    uint32_t x0 = s[4 * c], x1 = s[4 * c + 1], x2 = s[4 * c + 2], x3 = s[4 * c + 3];
-    uint32_t t01 = kb_add(x0, x1);
-    uint32_t t23 = kb_add(x2, x3);
-    uint32_t t0123 = kb_add(t01, t23);
-    uint32_t t01123 = kb_add(t0123, x1);
-    uint32_t t01233 = kb_add(t0123, x3);
-    s[4 * c + 3] = kb_add(kb_dbl(x0), t01233);
-    s[4 * c + 1] = kb_add(kb_dbl(x2), t01123);
-    s[4 * c + 0] = kb_add(t01, t01123);
-    s[4 * c + 2] = kb_add(t23, t01233);
-  }
-  uint32_t sum[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    uint32_t a = s[j];
-#pragma unroll
-    for (int c = 1; c < W / 4; ++c) a = kb_add(a, s[4 * c + j]);
-    sum[j] = a;
-  }
-#pragma unroll
-  for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], sum[i & 3]);
-#endif
 }
 
 template <int W>
@@ -186,25 +160,19 @@ LG_HD void p2_diag_all(uint32_t* s, uint32_t sum) {
   }
 }
 
-// Sum of the W state elements mod p.  On the device: pair sums of two
-// elements below p fit 32 bits, the W/2 pair sums accumulate in 64 bits
-// (v_lshl_add_u64), and one reduction with 2^31 = 2^24 - 1 (mod p) follows:
-// acc = hi 2^31 + lo with hi < W, so lo + hi 2^24 - hi < 2^31 + 2^29 < 2p.
+// Sum of the W state elements mod p: pair sums of two elements below p fit
+// 32 bits, the W/2 pair sums accumulate in 64 bits (v_lshl_add_u64 on the
+// device), and one reduction with 2^31 = 2^24 - 1 (mod p) follows: acc =
+// hi 2^31 + lo with hi < W, so lo + hi 2^24 - hi < 2^31 + 2^29 < 2p.
 template <int W>
 LG_HD uint32_t p2_state_sum(const uint32_t* s) {
-#if defined(__HIP_DEVICE_COMPILE__)
   static_assert(W % 2 == 0 && W <= 32, "pairwise sum needs an even width up to 32");
   uint64_t acc = 0;
 #pragma unroll
   for (int i = 0; i < W; i += 2) acc += (uint64_t)(s[i] + s[i + 1]);
   const uint32_t lo = (uint32_t)acc & 0x7fffffffu, hi = (uint32_t)(acc >> 31);
   const uint32_t r = lo + (hi << 24) - hi;
-  return __builtin_elementwise_min(r, r - KB_P);
-#else
-  uint32_t sum = s[0];
-  for (int i = 1; i < W; ++i) sum = kb_add(sum, s[i]);
-  return sum;
-#endif
+  return kb_umin(r, r - KB_P);
 }
 
 template <int W>
