@@ -133,10 +133,12 @@ struct E6 {
 LG_HD E2 e2_add(E2 x, E2 y) { return {kb_add(x.a0, y.a0), kb_add(x.a1, y.a1)}; }
 LG_HD E2 e2_sub(E2 x, E2 y) { return {kb_sub(x.a0, y.a0), kb_sub(x.a1, y.a1)}; }
 LG_HD uint32_t kb_mul3(uint32_t a) { return kb_add(kb_dbl(a), a); }
-// (a0 + a1 u)(b0 + b1 u) = (a0 b0 + 3 a1 b1) + (a0 b1 + a1 b0) u
+// (a0 + a1 u)(b0 + b1 u) = (a0 b0 + 3 a1 b1) + (a0 b1 + a1 b0) u, with the
+// u coefficient as (a0 + a1)(b0 + b1) - a0 b0 - a1 b1 (three products).
 LG_HD E2 e2_mul(E2 x, E2 y) {
-  uint32_t t = kb_mul(x.a1, y.a1);
-  return {kb_add(kb_mul(x.a0, y.a0), kb_mul3(t)), kb_add(kb_mul(x.a0, y.a1), kb_mul(x.a1, y.a0))};
+  const uint32_t t0 = kb_mul(x.a0, y.a0), t1 = kb_mul(x.a1, y.a1);
+  const uint32_t s = kb_mul(kb_add(x.a0, x.a1), kb_add(y.a0, y.a1));
+  return {kb_add(t0, kb_mul3(t1)), kb_sub(s, kb_add(t0, t1))};
 }
 // times xi = 1 + u: (p + q u)(1 + u) = (p + 3q) + (p + q) u
 LG_HD E2 e2_mul_xi(E2 x) { return {kb_add(x.a0, kb_mul3(x.a1)), kb_add(x.a0, x.a1)}; }
@@ -198,16 +200,18 @@ LG_HD E6 e6_add_base(const E6& x, uint32_t b) {
   return r;
 }
 
+// (x0 + x1 v + x2 v^2)(y0 + y1 v + y2 v^2) with v^3 = xi: the schoolbook
+// coefficients z0 = x0y0 + xi (x1y2 + x2y1), z1 = x0y1 + x1y0 + xi x2y2,
+// z2 = x0y2 + x1y1 + x2y0, with the cross sums from (xi + xj)(yi + yj) - xiyi
+// - xjyj (six E2 products instead of nine).
 LG_HD E6 e6_mul(const E6& x, const E6& y) {
-  E2 x0 = e6_c(x, 0), x1 = e6_c(x, 1), x2 = e6_c(x, 2);
-  E2 y0 = e6_c(y, 0), y1 = e6_c(y, 1), y2 = e6_c(y, 2);
-  E2 p00 = e2_mul(x0, y0), p01 = e2_mul(x0, y1), p02 = e2_mul(x0, y2);
-  E2 p10 = e2_mul(x1, y0), p11 = e2_mul(x1, y1), p12 = e2_mul(x1, y2);
-  E2 p20 = e2_mul(x2, y0), p21 = e2_mul(x2, y1), p22 = e2_mul(x2, y2);
-  E2 z0 = e2_add(p00, e2_mul_xi(e2_add(p12, p21)));
-  E2 z1 = e2_add(e2_add(p01, p10), e2_mul_xi(p22));
-  E2 z2 = e2_add(e2_add(p02, p11), p20);
-  return e6_make(z0, z1, z2);
+  const E2 x0 = e6_c(x, 0), x1 = e6_c(x, 1), x2 = e6_c(x, 2);
+  const E2 y0 = e6_c(y, 0), y1 = e6_c(y, 1), y2 = e6_c(y, 2);
+  const E2 v0 = e2_mul(x0, y0), v1 = e2_mul(x1, y1), v2 = e2_mul(x2, y2);
+  const E2 s12 = e2_sub(e2_mul(e2_add(x1, x2), e2_add(y1, y2)), e2_add(v1, v2));
+  const E2 s01 = e2_sub(e2_mul(e2_add(x0, x1), e2_add(y0, y1)), e2_add(v0, v1));
+  const E2 s02 = e2_sub(e2_mul(e2_add(x0, x2), e2_add(y0, y2)), e2_add(v0, v2));
+  return e6_make(e2_add(v0, e2_mul_xi(s12)), e2_add(s01, e2_mul_xi(v2)), e2_add(s02, v1));
 }
 
 LG_HD E2 e2_inv(E2 x) {

@@ -90,3 +90,32 @@ def test_montgomery_product_matches_closed_form():
         assert r.returncode == 0
     finally:
         shutil.rmtree(d)
+
+
+def test_e6_product_matches_oracle():
+    """e6_mul / e2_mul of kb.h (Karatsuba-style cross terms) equal the
+    oracle's schoolbook E6 product on random operands and edge values."""
+    from oracle import field as F
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tempfile.mkdtemp()
+    try:
+        exe = os.path.join(d, "e6_mul_check")
+        subprocess.run([gxx, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(HERE, "host", "e6_mul_check.cpp"),
+                        "-o", exe], check=True)
+        rng = np.random.default_rng(11)
+        P = F.P
+        x = rng.integers(0, P, size=(400, 6), dtype=np.uint64)
+        y = rng.integers(0, P, size=(400, 6), dtype=np.uint64)
+        x[0] = P - 1
+        y[0] = P - 1
+        x[1] = 0
+        y[2, :] = [1, 0, 0, 0, 0, 0]
+        inp = "%d\n" % len(x) + "\n".join(" ".join(str(int(v)) for v in np.concatenate([a, b]))
+                                          for a, b in zip(x, y)) + "\n"
+        r = subprocess.run([exe], input=inp, capture_output=True, text=True, check=True)
+        got = np.array([[int(v) for v in ln.split()] for ln in r.stdout.strip().splitlines()], dtype=np.uint64)
+        assert np.array_equal(got, F.e6_mul(x, y))
+    finally:
+        shutil.rmtree(d)
