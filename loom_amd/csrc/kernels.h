@@ -170,9 +170,15 @@ void launch_deep(const DeepDesc& d, int batch, hipStream_t s);
 // p = layer[2i], q = layer[2i+1]; E6 layers as 6 limb arrays.
 // FRI fold challenge per instance on the device: digest of the sponge over
 // prefix (host memory, n_prefix <= 32 words) || prev[8 b ..] || root of
-// trees[b]; writes dig[8 b ..] and alpha[6 b ..] (first 6 limbs).
+// trees[b]; writes dig[8 b ..], alpha[6 b ..] (first 6 limbs) and the root
+// to roots[8 b ..].
 void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* prev, uint32_t* const* trees,
-                          uint32_t* dig, uint32_t* alpha, int batch, hipStream_t s);
+                          uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch, hipStream_t s);
+// FRI fold into out (length half) fused with the leaf hashes of out's tree
+// (row pairs, one E6 column) written to leaves[b] (half/2 digests).
+void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha,
+                          const uint32_t* xinv2, int half, uint32_t hdr2, uint32_t* const* leaves, int batch,
+                          hipStream_t s);
 void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
                      int half, int batch, hipStream_t s);
 // running[k] += gamma * level[k]

@@ -1238,6 +1238,53 @@ __global__ void k_fri_fold(const uint32_t* const* in, uint32_t* const* out, cons
   st6(out[b], half, i, e6_add(s, dlt));
 }
 
+// One FRI fold fused with the leaf hashing of the folded layer's tree: thread
+// k forms next[2k] and next[2k+1] (from in[4k .. 4k+3]), stores them, and
+// hashes the row pair as the leaf sponge [LEAF, 0, hdr2, next[2k] limbs,
+// next[2k+1] limbs] (nb = 0, ne = 1: 15 elements, one permutation;
+// internal/fri/hash.go:39-56) into leaves[b] + 8k.
+__global__ __launch_bounds__(256) void k_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out,
+                                                       const uint32_t* alpha, const uint32_t* xinv2, int half,
+                                                       uint32_t hdr2, uint32_t* const* leaves) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (2 * k >= half) return;
+  const int b = blockIdx.y;
+  const uint32_t* L = in[b];
+  const size_t m = 2 * (size_t)half;
+  constexpr uint32_t INV2_MONT = 0x00ffffffu;  // 1/2 in Montgomery form (2^31 mod p)
+  const E6 a = ldc6(alpha + 6 * b);
+  uint32_t s[24];
+#pragma unroll
+  for (int i = 0; i < 24; ++i) s[i] = 0;
+  s[0] = TAG_LEAF_MONT;
+  s[2] = hdr2;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int i = 2 * k + t;
+    const E6 p = ld6(L, m, 2 * i), q = ld6(L, m, 2 * i + 1);
+    const E6 r = e6_add(e6_scale(e6_add(p, q), INV2_MONT), e6_mul(e6_scale(e6_sub(p, q), xinv2[i]), a));
+    st6(out[b], half, i, r);
+#pragma unroll
+    for (int l = 0; l < 6; ++l) s[3 + 6 * t + l] = r.v[l];
+  }
+  p2_permute<24>(s);
+  uint32_t* o = leaves[b] + (size_t)8 * k;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) o[i] = s[i];
+}
+
+void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha,
+                          const uint32_t* xinv2, int half, uint32_t hdr2, uint32_t* const* leaves, int batch,
+                          hipStream_t s) {
+  if (half < 2 || (half & 1)) throw std::runtime_error("fri fold+leaf: layer length must be even");
+  const int pairs = half / 2;
+  const int T = pairs < 256 ? pairs : 256;
+  prelaunch();
+  hipLaunchKernelGGL(k_fri_fold_leaf, dim3((pairs + T - 1) / T, batch), dim3(T), 0, s, in, out, alpha, xinv2, half,
+                     hdr2, leaves);
+  LG_HIP(hipGetLastError());
+}
+
 void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
                      int half, int batch, hipStream_t s) {
   const int T = half < 256 ? half : 256;
@@ -1252,8 +1299,8 @@ void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint
 // previous challenge's digest) || the root of trees[b] (node 1), absorbed by
 // the rate-16 overwrite sponge (internal/fiat-shamir/transcript.go:285-315;
 // internal/hash/poseidon2.go:165-255).  One octet of lanes per instance
-// (P2Coop); the digest goes to dig[b] (8 words) and the E6 challenge, its
-// first 6 limbs, to alpha[b].
+// (P2Coop); the digest goes to dig[b] (8 words), the E6 challenge, its first
+// 6 limbs, to alpha[b], and the root itself to roots[b] (8 words).
 constexpr int CHAL_MAX_PREFIX = 32;
 struct ChalPrefix {
   uint32_t w[CHAL_MAX_PREFIX];
@@ -1261,7 +1308,8 @@ struct ChalPrefix {
 
 __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre, const uint32_t* __restrict__ prev,
                                                       uint32_t* const* __restrict__ trees, uint32_t* __restrict__ dig,
-                                                      uint32_t* __restrict__ alpha, int batch) {
+                                                      uint32_t* __restrict__ alpha, uint32_t* __restrict__ roots,
+                                                      int batch) {
   const int q = threadIdx.x & 7;
   const int b_raw = blockIdx.x * 8 + (threadIdx.x >> 3);
   const bool live = b_raw < batch;
@@ -1285,7 +1333,10 @@ __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre,
   }
   if (live && q < 2) {
 #pragma unroll
-    for (int e = 0; e < 4; ++e) dig[8 * b + 4 * q + e] = x[e];
+    for (int e = 0; e < 4; ++e) {
+      dig[8 * b + 4 * q + e] = x[e];
+      roots[8 * b + 4 * q + e] = root[4 * q + e];
+    }
     alpha[6 * b + 4 * q] = x[0];
     alpha[6 * b + 4 * q + 1] = x[1];
     if (q == 0) {
@@ -1296,13 +1347,13 @@ __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre,
 }
 
 void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* prev, uint32_t* const* trees,
-                          uint32_t* dig, uint32_t* alpha, int batch, hipStream_t s) {
+                          uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch, hipStream_t s) {
   if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("fri challenge: bad prefix length");
   ChalPrefix pre{};
   for (int i = 0; i < n_prefix; ++i) pre.w[i] = prefix[i];
   prelaunch();
   hipLaunchKernelGGL(k_fri_challenge, dim3((batch + 7) / 8), dim3(64), 0, s, pre, n_prefix, prev, trees, dig, alpha,
-                     batch);
+                     roots, batch);
   LG_HIP(hipGetLastError());
 }
 

@@ -491,6 +491,7 @@ class Job {
   std::vector<Digest> roots_of(TreeBuf& tb);
   void build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>>& leaf_cols_per_group,
                   const std::vector<std::pair<int, int>>& group_shape, const std::vector<int>& group_rows);
+  void tree_levels(TreeBuf& tb, uint32_t* const* tp_d);
   void quotient(size_t m);
   void open();
 
@@ -780,7 +781,12 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
   }
   std::vector<uint32_t*> tp(B_);
   for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
-  uint32_t* const* tp_d = const_cast<uint32_t* const*>(stage(tp));
+  tree_levels(tb, const_cast<uint32_t* const*>(stage(tp)));
+}
+
+// The inner levels of a tree whose leaves (and injection digests) are in
+// place; tp_d: the staged node-array pointers of the batch.
+void Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d) {
   size_t next = 0;
   auto fold = [&](int width) {
     while (next < tb.inj.size() && tb.inj_width[next] == width) {
@@ -1524,83 +1530,117 @@ void Job::open() {
   std::vector<Layer> layers(FRI_ROUNDS + 1);
   std::vector<TreeBuf> ftrees(FRI_ROUNDS);
   std::vector<std::vector<Digest>> fri_roots;
-  Layer running;
-  running.len = lvp[0]->len;
-  running.buf = dev((size_t)B_ * 6 * running.len);
-  LG_HIP(hipMemcpyAsync(running.buf.as<void>(), lvp[0]->buf.as<void>(), (size_t)B_ * 6 * running.len * 4,
+  // Every layer, tree and pointer table of the commit phase is allocated and
+  // staged before its first launch (one staging copy for all rounds).  Round
+  // j (> first) builds the tree of layer j and derives its fold challenge on
+  // the device (k_fri_challenge: no host round trip); the fold of round j
+  // also hashes the leaves of layer j+1's tree (k_fri_fold_leaf) unless a
+  // level joins layer j+1 first.  The host replays roots and digests into the
+  // transcripts after the last round.
+  std::vector<char> joins(FRI_ROUNDS + 1, 0);
+  for (size_t l = 1; l < L; ++l) joins[FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n] = 1;
+  layers[first].len = lvp[0]->len;
+  layers[first].buf = dev((size_t)B_ * 6 * layers[first].len);
+  LG_HIP(hipMemcpyAsync(layers[first].buf.as<void>(), lvp[0]->buf.as<void>(), (size_t)B_ * 6 * layers[first].len * 4,
                         hipMemcpyDeviceToDevice, s_));
-  // Fold challenges: round `first` binds the levels root (on the host, which
-  // has it); later rounds bind the root of the running layer's tree, and
-  // their challenges are computed on the device (launch_fri_challenge), so
-  // the rounds are enqueued without waiting for the host.  The host replays
-  // the roots and digests into the transcripts after the last round.
+  for (int j = first + 1; j <= FRI_ROUNDS; ++j) {
+    layers[j].len = layers[j - 1].len / 2;
+    layers[j].buf = dev((size_t)B_ * 6 * layers[j].len);
+  }
+  auto ptrs = [&](const Layer& x) {
+    std::vector<uint32_t*> v(B_);
+    for (int b = 0; b < B_; ++b) v[b] = const_cast<uint32_t*>(x.ptr(b));
+    return const_cast<uint32_t* const*>(stage(v));
+  };
+  std::vector<uint32_t* const*> lay_d(FRI_ROUNDS + 1), tree_d(FRI_ROUNDS), leaf_d(FRI_ROUNDS);
+  std::vector<const uint32_t* const*> limb_d(FRI_ROUNDS);  // 6 limb columns per instance, for leaf_hash
+  for (int j = first; j <= FRI_ROUNDS; ++j) lay_d[j] = ptrs(layers[j]);
+  for (int j = first + 1; j < FRI_ROUNDS; ++j) {
+    TreeBuf& tb = ftrees[j];
+    tb.order = {0};
+    tb.nleaves = (int)(layers[j].len / 2);
+    tb.nodes = dev((size_t)B_ * 2 * tb.nleaves * 8);
+    std::vector<uint32_t*> tp(B_), lp(B_);
+    std::vector<const uint32_t*> cp;
+    for (int b = 0; b < B_; ++b) {
+      tp[b] = tb.node_ptr(b);
+      lp[b] = tb.node_ptr(b) + (size_t)8 * tb.nleaves;
+      for (int l = 0; l < 6; ++l) cp.push_back(layers[j].ptr(b) + (size_t)l * layers[j].len);
+    }
+    tree_d[j] = const_cast<uint32_t* const*>(stage(tp));
+    leaf_d[j] = const_cast<uint32_t* const*>(stage(lp));
+    limb_d[j] = stage(cp);
+  }
+  std::vector<int32_t> ext_msg{0, 0, 0};
+  for (int i = 0; i < 6; ++i) ext_msg.push_back(2 * i);
+  for (int i = 0; i < 6; ++i) ext_msg.push_back(2 * i + 1);
+  const int32_t* ext_msg_d = stage(ext_msg);
+  std::vector<const uint32_t*> level_lp(L);
+  std::vector<const uint32_t* const*> level_d(L);
+  std::vector<const uint32_t*> gamma_d(L);
+  for (size_t l = 1; l < L; ++l) {
+    std::vector<const uint32_t*> lp(B_);
+    for (int b = 0; b < B_; ++b) lp[b] = lvp[l]->ptr(b);
+    level_d[l] = stage(lp);
+    gamma_d[l] = reinterpret_cast<const uint32_t*>(stg_->put(gamma[l].data(), gamma[l].size() * sizeof(E6), s_));
+  }
   const size_t fb = (size_t)B_ * 8;
   DevBuf chal_dig = dev((size_t)FRI_ROUNDS * fb);
   DevBuf chal_alpha = dev((size_t)FRI_ROUNDS * B_ * 6);
   DevBuf root_dev = dev((size_t)FRI_ROUNDS * fb);
+  // round `first` binds the levels root, which the host has: its challenge is
+  // computed here
   const uint32_t* prev_dig = nullptr;
+  const uint32_t* alpha_first = nullptr;
+  {
+    std::vector<uint32_t> dg((size_t)B_ * 8), al((size_t)B_ * 6);
+    parallel_for(B_, [&](int b) {
+      const std::string nf = "fri_fold_" + std::to_string(first);
+      ts_[b].bind(nf, levels_root[b].v, 8);
+      const Digest& d = ts_[b].compute(nf);
+      for (int k = 0; k < 8; ++k) dg[(size_t)8 * b + k] = d.v[k];
+      for (int k = 0; k < 6; ++k) al[(size_t)6 * b + k] = d.v[k];
+    });
+    prev_dig = stage(dg);
+    alpha_first = stage(al);
+  }
+  const uint32_t hdr_ext = M(2);  // 2 ne for one E6 column
+  std::vector<char> leaves_done(FRI_ROUNDS, 0);
   for (int j = first; j < FRI_ROUNDS; ++j) {
     for (size_t l = 1; l < L; ++l) {
       const int jl = FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n;
-      if (jl != j) continue;
-      std::vector<uint32_t*> rp(B_);
-      std::vector<const uint32_t*> lp(B_);
-      std::vector<E6> gm(B_);
-      for (int b = 0; b < B_; ++b) {
-        rp[b] = running.ptr(b);
-        lp[b] = lvp[l]->ptr(b);
-        gm[b] = gamma[l][b];
+      if (jl == j) launch_fri_combine(lay_d[j], level_d[l], gamma_d[l], (int)layers[j].len, B_, s_);
+    }
+    const uint32_t* alpha_j = alpha_first;
+    if (j > first) {
+      TreeBuf& tb = ftrees[j];
+      if (!leaves_done[j]) {
+        TimeIt tm(ctx_, s_, "leaf_hash", (double)B_ * tb.nleaves * (2.0 * 6 + 8) * 4, (double)B_ * tb.nleaves);
+        launch_leaf_hash(limb_d[j], 6, ext_msg_d, (int)ext_msg.size(), 0u, hdr_ext, leaf_d[j], tb.nleaves, B_, s_);
       }
-      launch_fri_combine(const_cast<uint32_t* const*>(stage(rp)), stage(lp),
-                         reinterpret_cast<const uint32_t*>(stg_->put(gm.data(), gm.size() * sizeof(E6), s_)),
-                         (int)running.len, B_, s_);
-    }
-    uint32_t* dig_j = chal_dig.as<uint32_t>() + (size_t)j * fb;
-    const uint32_t* alpha_j = chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6;
-    if (j == first) {
-      std::vector<uint32_t> dg((size_t)B_ * 8), al((size_t)B_ * 6);
-      parallel_for(B_, [&](int b) {
-        const std::string nf = "fri_fold_" + std::to_string(j);
-        ts_[b].bind(nf, levels_root[b].v, 8);
-        const Digest& d = ts_[b].compute(nf);
-        for (int k = 0; k < 8; ++k) dg[(size_t)8 * b + k] = d.v[k];
-        for (int k = 0; k < 6; ++k) al[(size_t)6 * b + k] = d.v[k];
-      });
-      // staged words reach the device with the next launch's staging copy
-      prev_dig = stage(dg);
-      alpha_j = stage(al);
-    } else {
-      ext_tree(ftrees[j], {&running});
-      std::vector<uint32_t*> tp(B_);
-      for (int b = 0; b < B_; ++b) tp[b] = ftrees[j].node_ptr(b);
-      uint32_t* const* tp_d = const_cast<uint32_t* const*>(stage(tp));
-      launch_copy_roots(tp_d, root_dev.as<uint32_t>() + (size_t)j * fb, B_, s_);
+      tree_levels(tb, tree_d[j]);
+      uint32_t* dig_j = chal_dig.as<uint32_t>() + (size_t)j * fb;
+      uint32_t* al_j = chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6;
       const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, "fri_fold_" + std::to_string(j));
-      launch_fri_challenge(pre.data(), (int)pre.size(), prev_dig, tp_d, dig_j,
-                           chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6, B_, s_);
+      launch_fri_challenge(pre.data(), (int)pre.size(), prev_dig, tree_d[j], dig_j, al_j,
+                           root_dev.as<uint32_t>() + (size_t)j * fb, B_, s_);
       prev_dig = dig_j;
-    }
-    Layer next;
-    next.len = running.len / 2;
-    next.buf = dev((size_t)B_ * 6 * next.len);
-    std::vector<const uint32_t*> ip(B_);
-    std::vector<uint32_t*> op(B_);
-    for (int b = 0; b < B_; ++b) {
-      ip[b] = running.ptr(b);
-      op[b] = next.ptr(b);
+      alpha_j = al_j;
     }
     int log_len = 0;
-    while ((1u << log_len) < running.len) ++log_len;
-    {
-      const uint32_t* const* ip_d = stage(ip);
-      uint32_t* const* op_d = const_cast<uint32_t* const*>(stage(op));
-      const uint32_t* xi = tb_.fri_xinv2(log_len);
-      TimeIt tm(ctx_, s_, "fri_fold", (double)B_ * running.len * 1.5 * 24, 0);
-      launch_fri_fold(ip_d, op_d, alpha_j, xi, (int)next.len, B_, s_);
+    while ((1u << log_len) < layers[j].len) ++log_len;
+    const uint32_t* xi = tb_.fri_xinv2(log_len);
+    const int half = (int)layers[j + 1].len;
+    if (j + 1 < FRI_ROUNDS && !joins[j + 1]) {
+      TimeIt tm(ctx_, s_, "fri_fold_leaf", (double)B_ * layers[j].len * 1.5 * 24, (double)B_ * half / 2);
+      launch_fri_fold_leaf(lay_d[j], lay_d[j + 1], alpha_j, xi, half, hdr_ext, leaf_d[j + 1], B_, s_);
+      leaves_done[j + 1] = 1;
+    } else {
+      TimeIt tm(ctx_, s_, "fri_fold", (double)B_ * layers[j].len * 1.5 * 24, 0);
+      launch_fri_fold(lay_d[j], lay_d[j + 1], alpha_j, xi, half, B_, s_);
     }
-    layers[j] = std::move(running);
-    running = std::move(next);
   }
+  Layer running = std::move(layers[FRI_ROUNDS]);
   {
     std::vector<uint32_t> dg((size_t)FRI_ROUNDS * fb), rt((size_t)FRI_ROUNDS * fb);
     LG_HIP(hipMemcpyAsync(dg.data(), chal_dig.as<void>(), dg.size() * 4, hipMemcpyDeviceToHost, s_));
