@@ -519,38 +519,38 @@ __global__ __launch_bounds__(256) void k_merkle_fused(uint32_t* const* __restric
   }
 }
 
-// Cooperative variant: an octet of lanes per node (see P2Coop), 32 octets per
-// workgroup; a workgroup owns G <= 32 consecutive nodes of level width_hi and
-// reduces them to one node, keeping the running level in LDS.
-__global__ __launch_bounds__(256) void k_merkle_coop(uint32_t* const* __restrict__ trees, int width_hi, int levels) {
-  __shared__ uint4 sh[2][32 * 2];
+// Cooperative variant: a row of 16 lanes per node (P2Coop16), up to 32 rows
+// per workgroup; a workgroup owns G <= 32 consecutive nodes of level width_hi
+// and reduces them to one node, keeping the running level in LDS.  Node state:
+// element 0 = NODE, 1..7 = 0, 8..15 = left child, 16..23 = right child; lane l
+// holds elements l and l + 12.
+__global__ __launch_bounds__(512) void k_merkle_coop(uint32_t* const* __restrict__ trees, int width_hi, int levels) {
+  __shared__ uint32_t sh[2][32 * 8];
   uint32_t* nodes = trees[blockIdx.y];
-  const int q = threadIdx.x & 7;
-  const int oct = threadIdx.x >> 3;
+  const int l = threadIdx.x & 15;
+  const int grp = threadIdx.x >> 4;
   const int G = width_hi < 32 ? width_hi : 32;
   const int base = blockIdx.x * G;
-  P2Coop P;
-  P.init(q);
+  P2Coop16 P;
+  P.init(l);
   int cnt = G;
   int cur = 0;
   for (int lev = 0; lev < levels; ++lev) {
     const int w = width_hi >> lev;
-    if (oct < cnt) {
-      const size_t node = (size_t)w + (base >> lev) + oct;
-      uint32_t x[4] = {0u, 0u, 0u, 0u};
-      if (q == 0) x[0] = TAG_NODE_MONT;
-      if (q >= 2 && q < 6) {
-        const int child = (q - 2) >> 1, half = (q - 2) & 1;
-        uint4 v;
-        if (lev == 0) v = reinterpret_cast<const uint4*>(nodes + 8 * (2 * node + child))[half];
-        else v = sh[cur][2 * (2 * oct + child) + half];
-        x[0] = v.x; x[1] = v.y; x[2] = v.z; x[3] = v.w;
-      }
-      P.permute(x);
-      if (q < 2) {
-        const uint4 o = make_uint4(x[0], x[1], x[2], x[3]);
-        reinterpret_cast<uint4*>(nodes + 8 * node)[q] = o;
-        sh[cur ^ 1][2 * oct + q] = o;
+    if (grp < cnt) {
+      const size_t node = (size_t)w + (base >> lev) + grp;
+      // word cw (0..15) of the two children: child cw >> 3, word cw & 7
+      auto child = [&](int cw) -> uint32_t {
+        return lev == 0 ? nodes[8 * (2 * node + (cw >> 3)) + (cw & 7)] : sh[cur][8 * (2 * grp + (cw >> 3)) + (cw & 7)];
+      };
+      uint32_t a = 0u, b = 0u;
+      if (l == 0) a = TAG_NODE_MONT;
+      else if (l >= 8 && l < 12) a = child(l - 8);
+      if (l < 12) b = child(l + 4);
+      P.permute(a, b);
+      if (l < 8) {
+        nodes[8 * node + l] = a;
+        sh[cur ^ 1][8 * grp + l] = a;
       }
     }
     __syncthreads();
@@ -565,7 +565,7 @@ void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int 
   const int G = width_hi < 32 ? width_hi : 32;
   if ((width_hi / width_lo) > G) throw std::runtime_error("merkle coop: span exceeds workgroup");
   prelaunch();
-  hipLaunchKernelGGL(k_merkle_coop, dim3(width_hi / G, batch), dim3(8 * G), 0, s, trees, width_hi, levels);
+  hipLaunchKernelGGL(k_merkle_coop, dim3(width_hi / G, batch), dim3(16 * G), 0, s, trees, width_hi, levels);
   LG_HIP(hipGetLastError());
 }
 
@@ -1298,8 +1298,8 @@ void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint
 // message is prefix (StringToElements(FSID, name), shared) || prev[b] (the
 // previous challenge's digest) || the root of trees[b] (node 1), absorbed by
 // the rate-16 overwrite sponge (internal/fiat-shamir/transcript.go:285-315;
-// internal/hash/poseidon2.go:165-255).  One octet of lanes per instance
-// (P2Coop); the digest goes to dig[b] (8 words), the E6 challenge, its first
+// internal/hash/poseidon2.go:165-255).  One row of 16 lanes per instance
+// (P2Coop16); the digest goes to dig[b] (8 words), the E6 challenge, its first
 // 6 limbs, to alpha[b], and the root itself to roots[b] (8 words).
 constexpr int CHAL_MAX_PREFIX = 32;
 struct ChalPrefix {
@@ -1310,39 +1310,28 @@ __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre,
                                                       uint32_t* const* __restrict__ trees, uint32_t* __restrict__ dig,
                                                       uint32_t* __restrict__ alpha, uint32_t* __restrict__ roots,
                                                       int batch) {
-  const int q = threadIdx.x & 7;
-  const int b_raw = blockIdx.x * 8 + (threadIdx.x >> 3);
+  const int l = threadIdx.x & 15;
+  const int b_raw = blockIdx.x * 4 + (threadIdx.x >> 4);
   const bool live = b_raw < batch;
   const int b = live ? b_raw : batch - 1;
   const uint32_t* root = trees[b] + 8;
-  P2Coop P;
-  P.init(q);
-  uint32_t x[4] = {0u, 0u, 0u, 0u};
+  P2Coop16 P;
+  P.init(l);
+  auto msg = [&](int i) -> uint32_t {
+    return i < n_pre ? pre.w[i] : (i < n_pre + 8 ? prev[8 * b + (i - n_pre)] : root[i - n_pre - 8]);
+  };
+  uint32_t a = 0u, c = 0u;  // elements l and l + 12
   const int len = n_pre + 16;
   for (int off = 0; off < len; off += 16) {
     const int k = len - off < 16 ? len - off : 16;
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int idx = 4 * q + e;
-      if (q < 6 && idx < k) {
-        const int i = off + idx;
-        x[e] = i < n_pre ? pre.w[i] : (i < n_pre + 8 ? prev[8 * b + (i - n_pre)] : root[i - n_pre - 8]);
-      }
-    }
-    P.permute(x);
+    if (l < 12 && l < k) a = msg(off + l);
+    if (l < 12 && l + 12 < k) c = msg(off + l + 12);
+    P.permute(a, c);
   }
-  if (live && q < 2) {
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      dig[8 * b + 4 * q + e] = x[e];
-      roots[8 * b + 4 * q + e] = root[4 * q + e];
-    }
-    alpha[6 * b + 4 * q] = x[0];
-    alpha[6 * b + 4 * q + 1] = x[1];
-    if (q == 0) {
-      alpha[6 * b + 2] = x[2];
-      alpha[6 * b + 3] = x[3];
-    }
+  if (live && l < 8) {
+    dig[8 * b + l] = a;
+    roots[8 * b + l] = root[l];
+    if (l < 6) alpha[6 * b + l] = a;
   }
 }
 
@@ -1352,7 +1341,7 @@ void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* 
   ChalPrefix pre{};
   for (int i = 0; i < n_prefix; ++i) pre.w[i] = prefix[i];
   prelaunch();
-  hipLaunchKernelGGL(k_fri_challenge, dim3((batch + 7) / 8), dim3(64), 0, s, pre, n_prefix, prev, trees, dig, alpha,
+  hipLaunchKernelGGL(k_fri_challenge, dim3((batch + 3) / 4), dim3(64), 0, s, pre, n_prefix, prev, trees, dig, alpha,
                      roots, batch);
   LG_HIP(hipGetLastError());
 }

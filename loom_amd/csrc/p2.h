@@ -229,88 +229,98 @@ inline void p2_sponge_host(const uint32_t* msg, size_t len, uint32_t* digest) {
 namespace lg {
 
 // ---------------------------------------------------------------- cooperative
-// One width-24 state spread over an octet of lanes (8 consecutive lanes of a
-// wavefront): lane q < 6 holds state elements 4q..4q+3, lanes 6 and 7 hold
-// zeros.  Cross-lane sums use DPP (quad_perm and row_half_mirror), so the
-// dependent chain of one permutation is ~6x shorter than with one state per
-// lane; used where few independent permutations are available (tree tops,
-// small FRI layers) and latency, not throughput, bounds the time.
+// One width-24 state spread over a DPP row (16 consecutive lanes of a
+// wavefront): lane l < 12 holds state elements l (register a) and l + 12
+// (register b), lanes 12..15 hold zeros.  Chunks 0-2 of the external layer sit
+// in a and chunks 3-5 in b, on lanes 4c..4c+3 in both, so M4 is a quad
+// permutation and the chunk-wise and full sums are row rotations (DPP, no
+// LDS).  The dependent chain of one permutation is a few times shorter than
+// with one state per lane; used where few independent permutations are
+// available (tree tops, FRI challenges) and latency bounds the time.
 
-__device__ __forceinline__ uint32_t dpp_quad_xor1(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);
+template <int CTRL>
+__device__ __forceinline__ uint32_t dpp(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, CTRL, 0xF, 0xF, true);
 }
-__device__ __forceinline__ uint32_t dpp_quad_xor2(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);
-}
-__device__ __forceinline__ uint32_t dpp_half_mirror(uint32_t v) {
-  return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);
+// quad_perm [1,2,3,0], [2,3,0,1], [3,0,1,2], [1,0,3,2]; row_ror:4, row_ror:8
+constexpr int DPP_QUAD_ROT1 = 0x39, DPP_QUAD_ROT2 = 0x4E, DPP_QUAD_ROT3 = 0x93, DPP_QUAD_XOR1 = 0xB1;
+constexpr int DPP_ROW_ROR4 = 0x124, DPP_ROW_ROR8 = 0x128;
+
+// Sum over the lanes of a row with the same index mod 4, to every lane.
+__device__ __forceinline__ uint32_t row_col_sum(uint32_t v) {
+  v = kb_add(v, dpp<DPP_ROW_ROR4>(v));
+  return kb_add(v, dpp<DPP_ROW_ROR8>(v));
 }
 
-// Sum over the 8 lanes of the octet, returned to every lane.
-__device__ __forceinline__ uint32_t oct_sum(uint32_t v) {
-  v = kb_add(v, dpp_quad_xor1(v));
-  v = kb_add(v, dpp_quad_xor2(v));
-  return kb_add(v, dpp_half_mirror(v));
+// Sum over all lanes of a row, to every lane.
+__device__ __forceinline__ uint32_t row_sum(uint32_t v) {
+  v = kb_add(v, dpp<DPP_QUAD_XOR1>(v));
+  v = kb_add(v, dpp<DPP_QUAD_ROT2>(v));
+  return row_col_sum(v);
 }
 
-struct P2Coop {
-  uint32_t rk[6][4];  // full-round keys of this lane's chunk (zeros on lanes 6, 7)
-  uint32_t d[4];      // internal diagonal of this lane's chunk
-  uint32_t amask;     // all ones on lanes q < 6
-  uint32_t l0mask;    // all ones on lane q == 0
+// M4 of the lane's quad, row j of the circulant [[2,3,1,1],[1,2,3,1],...]:
+// 2 x_j + 3 x_{j+1} + x_{j+2} + x_{j+3}, formed in 64 bits (below 7p) and
+// reduced once with 2^31 = 2^24 - 1 (mod p).
+__device__ __forceinline__ uint32_t quad_m4(uint32_t x) {
+  const uint32_t x1 = dpp<DPP_QUAD_ROT1>(x), x2 = dpp<DPP_QUAD_ROT2>(x), x3 = dpp<DPP_QUAD_ROT3>(x);
+  const uint64_t v = 2 * (uint64_t)(x + x1) + x1 + (uint64_t)(x2 + x3);
+  const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
+  const uint32_t r = lo + (hi << 24) - hi;
+  return kb_umin(r, r - KB_P);
+}
 
-  __device__ __forceinline__ void init(int q) {
-    const bool active = q < 6;
+struct P2Coop16 {
+  uint32_t rka[6], rkb[6];  // full-round keys of elements l and l + 12 (zeros on lanes 12..15)
+  uint32_t da, db;          // internal diagonal of elements l and l + 12
+  uint32_t amask;           // all ones on lanes l < 12
+  uint32_t l0mask;          // all ones on lane 0
+
+  __device__ __forceinline__ void init(int l) {
+    const bool active = l < 12;
+    const int i = active ? l : 0;
     amask = active ? 0xffffffffu : 0u;
-    l0mask = q == 0 ? 0xffffffffu : 0u;
+    l0mask = l == 0 ? 0xffffffffu : 0u;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int i = 4 * (q < 6 ? q : 0) + e;
-#pragma unroll
-      for (int r = 0; r < 6; ++r) rk[r][e] = active ? P2_W24_FULL_RK[r][i] : 0u;
-      d[e] = active ? P2_W24_DIAG[i] : 0u;
+    for (int r = 0; r < 6; ++r) {
+      rka[r] = active ? P2_W24_FULL_RK[r][i] : 0u;
+      rkb[r] = active ? P2_W24_FULL_RK[r][i + 12] : 0u;
     }
+    da = active ? P2_W24_DIAG[i] : 0u;
+    db = active ? P2_W24_DIAG[i + 12] : 0u;
   }
 
-  __device__ __forceinline__ void external(uint32_t* x) const {
-    uint32_t x0 = x[0], x1 = x[1], x2 = x[2], x3 = x[3];
-    uint32_t t01 = kb_add(x0, x1), t23 = kb_add(x2, x3);
-    uint32_t t0123 = kb_add(t01, t23);
-    uint32_t t01123 = kb_add(t0123, x1), t01233 = kb_add(t0123, x3);
-    x[3] = kb_add(kb_dbl(x0), t01233);
-    x[1] = kb_add(kb_dbl(x2), t01123);
-    x[0] = kb_add(t01, t01123);
-    x[2] = kb_add(t23, t01233);
-    // lanes 6, 7 hold zeros, which M4 keeps; adding a masked sum keeps them so
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const uint32_t t = oct_sum(x[e]);
-      x[e] = kb_add(x[e], t & amask);
-    }
+  // lanes 12..15 hold zeros, which M4 keeps; the chunk-wise sums reach them
+  // too, so they are masked off there
+  __device__ __forceinline__ void external(uint32_t& a, uint32_t& b) const {
+    const uint32_t ya = quad_m4(a), yb = quad_m4(b);
+    const uint32_t s = row_col_sum(kb_add(ya, yb)) & amask;
+    a = kb_add(ya, s);
+    b = kb_add(yb, s);
   }
 
-  __device__ __forceinline__ void permute(uint32_t* x) const {
-    external(x);
+  __device__ __forceinline__ void permute(uint32_t& a, uint32_t& b) const {
+    external(a, b);
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) x[e] = kb_cube(kb_add(x[e], rk[r][e]));
-      external(x);
+      a = kb_cube(kb_add(a, rka[r]));
+      b = kb_cube(kb_add(b, rkb[r]));
+      external(a, b);
     }
 #pragma unroll
     for (int r = 0; r < 21; ++r) {
-      const uint32_t c = kb_cube(kb_add(x[0], P2_W24_PARTIAL_RK[r]));
-      x[0] = (c & l0mask) | (x[0] & ~l0mask);
-      const uint32_t s = oct_sum(kb_add(kb_add(x[0], x[1]), kb_add(x[2], x[3]))) & amask;
-      // inactive lanes: x = 0, d = 0 and s masked to 0, so they stay 0
-#pragma unroll
-      for (int e = 0; e < 4; ++e) x[e] = kb_add(s, kb_mul(x[e], d[e]));
+      const uint32_t c = kb_cube(kb_add(a, P2_W24_PARTIAL_RK[r]));
+      a = (c & l0mask) | (a & ~l0mask);
+      // inactive lanes: a = b = 0, da = db = 0 and the sum masked to 0
+      const uint32_t s = row_sum(kb_add(a, b)) & amask;
+      a = kb_add(s, kb_mul(a, da));
+      b = kb_add(s, kb_mul(b, db));
     }
 #pragma unroll
     for (int r = 3; r < 6; ++r) {
-#pragma unroll
-      for (int e = 0; e < 4; ++e) x[e] = kb_cube(kb_add(x[e], rk[r][e]));
-      external(x);
+      a = kb_cube(kb_add(a, rka[r]));
+      b = kb_cube(kb_add(b, rkb[r]));
+      external(a, b);
     }
   }
 };

@@ -1808,11 +1808,30 @@ void Job::open() {
   const size_t gwords = (size_t)B_ * nq * stride;
   uint32_t* gh = ctx_.pinned(gwords * 4, lane_);
   trace("gather kernels");
-  LG_HIP(hipMemcpyAsync(gh, gat.as<void>(), gwords * 4, hipMemcpyDeviceToHost, s_));
-  sync();
+  // The records come back in up to four chunks of instances, each followed by
+  // an event, so that the assembly of the first instances overlaps the copy
+  // of the later ones.
+  struct Events {
+    std::vector<hipEvent_t> e;
+    ~Events() {
+      for (auto x : e) (void)hipEventDestroy(x);
+    }
+  } ev;
+  const int nchunk = std::min(B_, 4);
+  const int per_chunk = (B_ + nchunk - 1) / nchunk;
+  for (int c = 0; c * per_chunk < B_; ++c) {
+    const size_t b0 = (size_t)c * per_chunk, b1 = std::min<size_t>(B_, b0 + per_chunk);
+    const size_t w0 = b0 * nq * stride, nw = (b1 - b0) * nq * stride;
+    LG_HIP(hipMemcpyAsync(gh + w0, gat.as<uint32_t>() + w0, nw * 4, hipMemcpyDeviceToHost, s_));
+    hipEvent_t e;
+    LG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    ev.e.push_back(e);
+    LG_HIP(hipEventRecord(e, s_));
+  }
   trace("gather copy-back");
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
   parallel_for(B_, [&](int b) {
+   LG_HIP(hipEventSynchronize(ev.e[b / per_chunk]));
    // The words go straight to the caller's buffer when it gave one
    // (SpanWriter counts past its capacity so the caller learns the length),
    // else to out_[b].proof.
