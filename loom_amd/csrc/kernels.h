@@ -185,10 +185,10 @@ void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint
 void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len,
                         int batch, hipStream_t s);
 
-// All query openings in one launch.  Each op copies, for every (instance b,
-// query q), a fixed number of units into the record of (b, q) at
-// out + (b * nq + q) * stride + out_off; blocks [block0, block0 + nblocks)
-// of the grid belong to the op.
+// Query openings of a range of instances in one launch.  Each op copies, for
+// every (instance b, query q), a fixed number of units into the record of
+// (b, q) at out + (b * nq + q) * stride + out_off; blocks [block0, block0 +
+// nblocks) of the grid belong to the op.
 //   rows:   units = 2 * ncols words: column c at rows lo, lo + 1 (lo first),
 //           row = pos >> shift, lo = row & ~1, columns ptrs[b * ncols + c]
 //   path:   units = ncols (= depth) digests: siblings of leaf lo / 2 of the
@@ -205,7 +205,9 @@ struct GatherOp {
   int32_t nblocks;
   const uint32_t* const* ptrs;
 };
-void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int batch,
+// Instances b0 .. b0 + nb - 1 of the batch; each op's blocks cover at least
+// nb * nq records.
+void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int b0, int nb,
                    uint32_t* out, int stride, hipStream_t s);
 
 // Query gathers.  pos: [b * nq + q] full-domain positions.

@@ -1367,16 +1367,16 @@ void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, 
 constexpr int GATHER_T = 256;
 
 __global__ __launch_bounds__(GATHER_T) void k_gather(const GatherOp* __restrict__ ops, int n_ops,
-                                                     const uint32_t* __restrict__ pos, int nq, int batch,
+                                                     const uint32_t* __restrict__ pos, int nq, int b0, int nb,
                                                      uint32_t* __restrict__ out, int stride) {
   int k = 0;
   while (k + 1 < n_ops && (int)blockIdx.x >= ops[k + 1].block0) ++k;
   const GatherOp op = ops[k];
   const long u = (long)((int)blockIdx.x - op.block0) * GATHER_T + threadIdx.x;
   const int per = op.kind == 0 ? 2 * op.ncols : (op.kind == 1 ? op.ncols : 1);
-  const long total = (long)batch * nq * per;
+  const long total = (long)nb * nq * per;
   if (u >= total) return;
-  const int bq = (int)(u / per);
+  const int bq = b0 * nq + (int)(u / per);
   const int e = (int)(u % per);
   const int b = bq / nq;
   const uint32_t row = pos[bq] >> op.shift;
@@ -1401,11 +1401,11 @@ __global__ __launch_bounds__(GATHER_T) void k_gather(const GatherOp* __restrict_
   }
 }
 
-void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int batch,
+void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int b0, int nb,
                    uint32_t* out, int stride, hipStream_t s) {
-  if (n_ops == 0 || total_blocks == 0) return;
+  if (n_ops == 0 || total_blocks == 0 || nb == 0) return;
   prelaunch();
-  hipLaunchKernelGGL(k_gather, dim3(total_blocks), dim3(GATHER_T), 0, s, ops_dev, n_ops, pos, nq, batch, out, stride);
+  hipLaunchKernelGGL(k_gather, dim3(total_blocks), dim3(GATHER_T), 0, s, ops_dev, n_ops, pos, nq, b0, nb, out, stride);
   LG_HIP(hipGetLastError());
 }
 

@@ -8,6 +8,8 @@
 #include <thread>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -1669,27 +1671,12 @@ void Job::open() {
   std::vector<uint32_t> final_w((size_t)B_ * 6 * flen);
   LG_HIP(hipMemcpyAsync(final_w.data(), running.buf.as<void>(), final_w.size() * 4, hipMemcpyDeviceToHost, s_));
   sync();
-  std::vector<uint32_t> pos((size_t)B_ * nq);
-  // the query challenges are registered consecutively; their name prefixes
-  // are the same for every instance
-  std::vector<std::vector<uint32_t>> qpre(nq);
-  for (int k = 0; k < nq; ++k) qpre[k] = string_to_elements(TAG_FSID, "fri_query_" + std::to_string(k));
-  parallel_for(B_, [&](int b) {
-    std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
-    for (uint32_t k = 0; k < flen; ++k)
-      for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
-    ts_[b].bind("fri_query_0", fp);
-    const size_t p0 = ts_[b].index("fri_query_0");
-    for (int k = 0; k < nq; ++k) {
-      const Digest& dg = ts_[b].compute_at(p0 + k, qpre[k]);
-      uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
-      pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
-      if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);
-    }
-  });
-  trace("queries (host)");
-  // ---- gathers: one record per (instance, query), all in one launch
-  const uint32_t* pos_d = stage(pos);
+  // ---- query openings.  The gather ops (one record per (instance, query))
+  // are built and staged first; the query chains then run on the host, and
+  // as soon as every position of a chunk of instances is known, that chunk's
+  // positions go to the device, its records are gathered and copied back,
+  // and an event marks them ready for the proof assembly.
+  const int chunk = std::min(B_, 8);
   size_t stride = 0;
   auto seg = [&](size_t words) {  // 16-byte aligned segment of the record
     size_t o = (stride + 3) & ~size_t(3);
@@ -1723,7 +1710,7 @@ void Job::open() {
     op.out_off = (int32_t)off;
     op.ptrs = stage(ptrs);
     const long per = kind == 0 ? 2 * ncols : (kind == 1 ? ncols : 1);
-    const long total = (long)B_ * nq * per;
+    const long total = (long)chunk * nq * per;  // one launch per chunk of instances
     op.block0 = blocks;
     op.nblocks = (int)((total + 255) / 256);
     blocks += op.nblocks;
@@ -1799,35 +1786,62 @@ void Job::open() {
     add_op(0, jl, 6, 0, 0, lq_off[l], cp);
   }
   stride = (stride + 3) & ~size_t(3);
-  DevBuf gat = dev((size_t)B_ * nq * stride);
-  {
-    const GatherOp* ops_d = stage(ops);
-    TimeIt tm(ctx_, s_, "gather", (double)B_ * nq * stride * 8, 0);
-    launch_gather(ops_d, (int)ops.size(), blocks, pos_d, nq, B_, gat.as<uint32_t>(), (int)stride, s_);
-  }
   const size_t gwords = (size_t)B_ * nq * stride;
+  DevBuf gat = dev(gwords);
+  DevBuf pos_dev = dev((size_t)B_ * nq);
+  const GatherOp* ops_d = stage(ops);
+  stg_->flush(s_);  // the chunk launches below stage nothing
   uint32_t* gh = ctx_.pinned(gwords * 4, lane_);
-  trace("gather kernels");
-  // The records come back in up to four chunks of instances, each followed by
-  // an event, so that the assembly of the first instances overlaps the copy
-  // of the later ones.
+  uint32_t* pos = ctx_.pinned_in((size_t)B_ * nq * 4, lane_);  // the inputs' upload finished long ago
   struct Events {
     std::vector<hipEvent_t> e;
     ~Events() {
       for (auto x : e) (void)hipEventDestroy(x);
     }
   } ev;
-  const int nchunk = std::min(B_, 4);
-  const int per_chunk = (B_ + nchunk - 1) / nchunk;
-  for (int c = 0; c * per_chunk < B_; ++c) {
-    const size_t b0 = (size_t)c * per_chunk, b1 = std::min<size_t>(B_, b0 + per_chunk);
-    const size_t w0 = b0 * nq * stride, nw = (b1 - b0) * nq * stride;
-    LG_HIP(hipMemcpyAsync(gh + w0, gat.as<uint32_t>() + w0, nw * 4, hipMemcpyDeviceToHost, s_));
+  const int nchunks = (B_ + chunk - 1) / chunk;
+  std::vector<std::atomic<int>> left(nchunks);
+  for (int c = 0; c < nchunks; ++c) {
+    left[c] = std::min(chunk, B_ - c * chunk);
     hipEvent_t e;
     LG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ev.e.push_back(e);
-    LG_HIP(hipEventRecord(e, s_));
   }
+  std::mutex launch_mu;
+  auto launch_chunk = [&](int c) {
+    const int b0 = c * chunk, nb = std::min(chunk, B_ - b0);
+    const size_t q0 = (size_t)b0 * nq, w0 = q0 * stride;
+    std::lock_guard<std::mutex> lk(launch_mu);
+    LG_HIP(hipSetDevice(ctx_.device));
+    LG_HIP(hipMemcpyAsync(pos_dev.as<uint32_t>() + q0, pos + q0, (size_t)nb * nq * 4, hipMemcpyHostToDevice, s_));
+    {
+      TimeIt tm(ctx_, s_, "gather", (double)nb * nq * stride * 8, 0);
+      launch_gather(ops_d, (int)ops.size(), blocks, pos_dev.as<uint32_t>(), nq, b0, nb, gat.as<uint32_t>(),
+                    (int)stride, s_);
+    }
+    LG_HIP(hipMemcpyAsync(gh + w0, gat.as<uint32_t>() + w0, (size_t)nb * nq * stride * 4, hipMemcpyDeviceToHost, s_));
+    LG_HIP(hipEventRecord(ev.e[c], s_));
+  };
+  // the query challenges are registered consecutively; their name prefixes
+  // are the same for every instance
+  std::vector<std::vector<uint32_t>> qpre(nq);
+  for (int k = 0; k < nq; ++k) qpre[k] = string_to_elements(TAG_FSID, "fri_query_" + std::to_string(k));
+  parallel_for(B_, [&](int b) {
+    std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
+    for (uint32_t k = 0; k < flen; ++k)
+      for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
+    ts_[b].bind("fri_query_0", fp);
+    const size_t p0 = ts_[b].index("fri_query_0");
+    for (int k = 0; k < nq; ++k) {
+      const Digest& dg = ts_[b].compute_at(p0 + k, qpre[k]);
+      uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
+      pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
+      if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);
+    }
+    if (--left[b / chunk] == 0) launch_chunk(b / chunk);
+  });
+  trace("queries + gathers");
+  const int per_chunk = chunk;
   trace("gather copy-back");
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
   parallel_for(B_, [&](int b) {
