@@ -1826,26 +1826,49 @@ void Job::open() {
   // are the same for every instance
   std::vector<std::vector<uint32_t>> qpre(nq);
   for (int k = 0; k < nq; ++k) qpre[k] = string_to_elements(TAG_FSID, "fri_query_" + std::to_string(k));
-  parallel_for(B_, [&](int b) {
-    std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
-    for (uint32_t k = 0; k < flen; ++k)
-      for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
-    ts_[b].bind("fri_query_0", fp);
-    const size_t p0 = ts_[b].index("fri_query_0");
-    for (int k = 0; k < nq; ++k) {
-      const Digest& dg = ts_[b].compute_at(p0 + k, qpre[k]);
-      uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
-      pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
-      if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);
+  // A chunk's records may be awaited only once its event has been recorded.
+  std::mutex ready_mu;
+  std::condition_variable ready_cv;
+  std::vector<char> ready(nchunks, 0);
+  bool failed = false;
+  auto mark = [&](int c, bool ok) {
+    {
+      std::lock_guard<std::mutex> lk(ready_mu);
+      if (ok) ready[c] = 1;
+      else failed = true;
     }
-    if (--left[b / chunk] == 0) launch_chunk(b / chunk);
-  });
-  trace("queries + gathers");
-  const int per_chunk = chunk;
-  trace("gather copy-back");
+    ready_cv.notify_all();
+  };
+  auto query_task = [&](int b) {
+    try {
+      std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
+      for (uint32_t k = 0; k < flen; ++k)
+        for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);
+      ts_[b].bind("fri_query_0", fp);
+      const size_t p0 = ts_[b].index("fri_query_0");
+      for (int k = 0; k < nq; ++k) {
+        const Digest& dg = ts_[b].compute_at(p0 + k, qpre[k]);
+        uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
+        pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
+        if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);
+      }
+      if (--left[b / chunk] == 0) {
+        launch_chunk(b / chunk);
+        mark(b / chunk, true);
+      }
+    } catch (...) {
+      mark(0, false);
+      throw;
+    }
+  };
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
-  parallel_for(B_, [&](int b) {
-   LG_HIP(hipEventSynchronize(ev.e[b / per_chunk]));
+  auto assemble_task = [&](int b) {
+   {
+     std::unique_lock<std::mutex> lk(ready_mu);
+     ready_cv.wait(lk, [&] { return failed || ready[b / chunk]; });
+     if (!ready[b / chunk]) return;  // a query task failed; its error is rethrown
+   }
+   LG_HIP(hipEventSynchronize(ev.e[b / chunk]));
    // The words go straight to the caller's buffer when it gave one
    // (SpanWriter counts past its capacity so the caller learns the length),
    // else to out_[b].proof.
@@ -1979,8 +2002,15 @@ void Job::open() {
       out_[b].proof_len = w.size();
     }
     out_[b].transcript = ts_[b].log_words();
+  };
+  // Query chains first (tasks are handed out in index order), then the
+  // assemblies, which start as soon as threads free up and their chunk is
+  // back, overlapping the last query chains.
+  parallel_for(2 * B_, [&](int t) {
+    if (t < B_) query_task(t);
+    else assemble_task(t - B_);
   });
-  trace("proof assembly");
+  trace("queries, gathers, proof assembly");
   for (auto& x : W) keep_.push_back(std::move(x));
   for (auto& x : level) keep_.push_back(std::move(x.buf));
   for (auto& x : layers) keep_.push_back(std::move(x.buf));
