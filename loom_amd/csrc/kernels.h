@@ -77,6 +77,12 @@ void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_
 // ~6x lower latency per level, for levels with few nodes per batch
 // (width_hi / width_lo <= 32).
 void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s);
+// The top of a tree (width_hi <= 32 nodes down to the root, one workgroup per
+// instance) followed by the FRI fold challenge of the root, as
+// launch_fri_challenge computes it.
+void launch_merkle_coop_challenge(uint32_t* const* trees, int width_hi, const uint32_t* prefix, int n_prefix,
+                                  const uint32_t* prev, uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch,
+                                  hipStream_t s);
 // roots[8b .. 8b+8) = trees[b][8 .. 16)  (node 1 of every tree)
 void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s);
 // Injection fold: nodes[w + j] = H(nodes[w + j], inj[b][j]).

@@ -519,14 +519,53 @@ __global__ __launch_bounds__(256) void k_merkle_fused(uint32_t* const* __restric
   }
 }
 
+// Prefix words of a Fiat-Shamir challenge message, passed by value.
+constexpr int CHAL_MAX_PREFIX = 32;
+struct ChalPrefix {
+  uint32_t w[CHAL_MAX_PREFIX];
+};
+
+// Sponge of one FRI fold challenge on a 16-lane row (see k_fri_challenge):
+// prefix || prev (8 words) || root (8 words); returns the digest word of
+// lane l < 8 in a (the state elements l and l + 12 in a, c).
+__device__ __forceinline__ uint32_t fri_challenge_row(const P2Coop16& P, int l, const ChalPrefix& pre, int n_pre,
+                                                      const uint32_t* prev, const uint32_t* root) {
+  auto msg = [&](int i) -> uint32_t {
+    return i < n_pre ? pre.w[i] : (i < n_pre + 8 ? prev[i - n_pre] : root[i - n_pre - 8]);
+  };
+  uint32_t a = 0u, c = 0u;
+  const int len = n_pre + 16;
+  for (int off = 0; off < len; off += 16) {
+    const int k = len - off < 16 ? len - off : 16;
+    if (l < 12 && l < k) a = msg(off + l);
+    if (l < 12 && l + 12 < k) c = msg(off + l + 12);
+    P.permute(a, c);
+  }
+  return a;
+}
+
 // Cooperative variant: a row of 16 lanes per node (P2Coop16), up to 32 rows
 // per workgroup; a workgroup owns G <= 32 consecutive nodes of level width_hi
 // and reduces them to one node, keeping the running level in LDS.  Node state:
 // element 0 = NODE, 1..7 = 0, 8..15 = left child, 16..23 = right child; lane l
-// holds elements l and l + 12.
-__global__ __launch_bounds__(512) void k_merkle_coop(uint32_t* const* __restrict__ trees, int width_hi, int levels) {
+// holds elements l and l + 12.  With CHAL (one workgroup per instance, last
+// level = the root) the first row then derives the FRI fold challenge of the
+// root, as k_fri_challenge does.
+struct FriChalArgs {
+  ChalPrefix pre;
+  int n_pre;
+  const uint32_t* prev;
+  uint32_t* dig;
+  uint32_t* alpha;
+  uint32_t* roots;
+};
+
+template <bool CHAL>
+__global__ __launch_bounds__(512) void k_merkle_coop(uint32_t* const* __restrict__ trees, int width_hi, int levels,
+                                                     FriChalArgs ch) {
   __shared__ uint32_t sh[2][32 * 8];
-  uint32_t* nodes = trees[blockIdx.y];
+  const int bi = blockIdx.y;
+  uint32_t* nodes = trees[bi];
   const int l = threadIdx.x & 15;
   const int grp = threadIdx.x >> 4;
   const int G = width_hi < 32 ? width_hi : 32;
@@ -557,6 +596,17 @@ __global__ __launch_bounds__(512) void k_merkle_coop(uint32_t* const* __restrict
     cur ^= 1;
     cnt >>= 1;
   }
+  if constexpr (CHAL) {
+    if (grp == 0) {
+      const uint32_t* root = sh[cur];
+      const uint32_t d = fri_challenge_row(P, l, ch.pre, ch.n_pre, ch.prev + 8 * bi, root);
+      if (l < 8) {
+        ch.dig[8 * bi + l] = d;
+        ch.roots[8 * bi + l] = root[l];
+        if (l < 6) ch.alpha[6 * bi + l] = d;
+      }
+    }
+  }
 }
 
 void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s) {
@@ -565,7 +615,28 @@ void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int 
   const int G = width_hi < 32 ? width_hi : 32;
   if ((width_hi / width_lo) > G) throw std::runtime_error("merkle coop: span exceeds workgroup");
   prelaunch();
-  hipLaunchKernelGGL(k_merkle_coop, dim3(width_hi / G, batch), dim3(16 * G), 0, s, trees, width_hi, levels);
+  hipLaunchKernelGGL(k_merkle_coop<false>, dim3(width_hi / G, batch), dim3(16 * G), 0, s, trees, width_hi, levels,
+                     FriChalArgs{});
+  LG_HIP(hipGetLastError());
+}
+
+void launch_merkle_coop_challenge(uint32_t* const* trees, int width_hi, const uint32_t* prefix, int n_prefix,
+                                  const uint32_t* prev, uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch,
+                                  hipStream_t s) {
+  if (width_hi < 1 || width_hi > 32 || (width_hi & (width_hi - 1)))
+    throw std::runtime_error("merkle coop challenge: the top launch must hold the whole remaining tree");
+  if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("fri challenge: bad prefix length");
+  int levels = 1;
+  while ((width_hi >> (levels - 1)) > 1) ++levels;
+  FriChalArgs ch{};
+  for (int i = 0; i < n_prefix; ++i) ch.pre.w[i] = prefix[i];
+  ch.n_pre = n_prefix;
+  ch.prev = prev;
+  ch.dig = dig;
+  ch.alpha = alpha;
+  ch.roots = roots;
+  prelaunch();
+  hipLaunchKernelGGL(k_merkle_coop<true>, dim3(1, batch), dim3(16 * width_hi), 0, s, trees, width_hi, levels, ch);
   LG_HIP(hipGetLastError());
 }
 
@@ -1301,10 +1372,6 @@ void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint
 // internal/hash/poseidon2.go:165-255).  One row of 16 lanes per instance
 // (P2Coop16); the digest goes to dig[b] (8 words), the E6 challenge, its first
 // 6 limbs, to alpha[b], and the root itself to roots[b] (8 words).
-constexpr int CHAL_MAX_PREFIX = 32;
-struct ChalPrefix {
-  uint32_t w[CHAL_MAX_PREFIX];
-};
 
 __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre, const uint32_t* __restrict__ prev,
                                                       uint32_t* const* __restrict__ trees, uint32_t* __restrict__ dig,
@@ -1317,21 +1384,11 @@ __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre,
   const uint32_t* root = trees[b] + 8;
   P2Coop16 P;
   P.init(l);
-  auto msg = [&](int i) -> uint32_t {
-    return i < n_pre ? pre.w[i] : (i < n_pre + 8 ? prev[8 * b + (i - n_pre)] : root[i - n_pre - 8]);
-  };
-  uint32_t a = 0u, c = 0u;  // elements l and l + 12
-  const int len = n_pre + 16;
-  for (int off = 0; off < len; off += 16) {
-    const int k = len - off < 16 ? len - off : 16;
-    if (l < 12 && l < k) a = msg(off + l);
-    if (l < 12 && l + 12 < k) c = msg(off + l + 12);
-    P.permute(a, c);
-  }
+  const uint32_t d = fri_challenge_row(P, l, pre, n_pre, prev + 8 * b, root);
   if (live && l < 8) {
-    dig[8 * b + l] = a;
+    dig[8 * b + l] = d;
     roots[8 * b + l] = root[l];
-    if (l < 6) alpha[6 * b + l] = a;
+    if (l < 6) alpha[6 * b + l] = d;
   }
 }
 

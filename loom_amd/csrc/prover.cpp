@@ -493,7 +493,14 @@ class Job {
   std::vector<Digest> roots_of(TreeBuf& tb);
   void build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>>& leaf_cols_per_group,
                   const std::vector<std::pair<int, int>>& group_shape, const std::vector<int>& group_rows);
-  void tree_levels(TreeBuf& tb, uint32_t* const* tp_d);
+  // A FRI fold challenge to derive from a tree's root in the launch that
+  // finishes the tree (see launch_merkle_coop_challenge).
+  struct RootChallenge {
+    std::vector<uint32_t> prefix;
+    const uint32_t* prev;
+    uint32_t *dig, *alpha, *roots;
+  };
+  bool tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* chal = nullptr);
   void quotient(size_t m);
   void open();
 
@@ -787,8 +794,11 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
 }
 
 // The inner levels of a tree whose leaves (and injection digests) are in
-// place; tp_d: the staged node-array pointers of the batch.
-void Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d) {
+// place; tp_d: the staged node-array pointers of the batch.  With chal, the
+// launch that reaches the root also derives the challenge; returns whether it
+// did (a tree of one leaf pair has no such launch).
+bool Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* chal) {
+  bool chal_done = false;
   size_t next = 0;
   auto fold = [&](int width) {
     while (next < tb.inj.size() && tb.inj_width[next] == width) {
@@ -828,12 +838,19 @@ void Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d) {
       int span = 0;
       for (int x = w; x >= lo; x /= 2) span += x;
       TimeIt tm(ctx_, s_, "merkle_coop", (double)B_ * span * 3 * 32, (double)B_ * span);
-      launch_merkle_coop(tp_d, w, lo, B_, s_);
+      if (chal && lo == 1 && next == tb.inj.size()) {
+        launch_merkle_coop_challenge(tp_d, w, chal->prefix.data(), (int)chal->prefix.size(), chal->prev, chal->dig,
+                                     chal->alpha, chal->roots, B_, s_);
+        chal_done = true;
+      } else {
+        launch_merkle_coop(tp_d, w, lo, B_, s_);
+      }
     }
     fold(lo);
     w = lo / 2;
   }
   if (next != tb.inj.size()) throw std::runtime_error("merkle: injection width does not match a level");
+  return chal_done;
 }
 
 std::vector<Digest> Job::roots_of(TreeBuf& tb) {
@@ -1513,7 +1530,12 @@ void Job::open() {
   const int first = FRI_LOG_D - d_.modules[classes[lv_order[0]]].log_n;
   TreeBuf levels_tree;
   ext_tree(levels_tree, lvp);
-  std::vector<Digest> levels_root = roots_of(levels_tree);
+  // With one FRI level nothing on the host needs the levels root before the
+  // fold rounds: round `first` then derives its challenge on the device like
+  // the later rounds, and the root comes back with theirs.
+  const bool first_on_host = L > 1;
+  std::vector<Digest> levels_root(B_);
+  if (first_on_host) levels_root = roots_of(levels_tree);
   std::vector<std::vector<E6>> gamma(L, std::vector<E6>(B_));
   parallel_for(B_, [&](int b) {
     for (size_t l = 1; l < L; ++l) {
@@ -1590,11 +1612,27 @@ void Job::open() {
   DevBuf chal_dig = dev((size_t)FRI_ROUNDS * fb);
   DevBuf chal_alpha = dev((size_t)FRI_ROUNDS * B_ * 6);
   DevBuf root_dev = dev((size_t)FRI_ROUNDS * fb);
-  // round `first` binds the levels root, which the host has: its challenge is
-  // computed here
+  // round `first` binds the levels root: with several levels the host has it
+  // (the gammas bound it) and computes the challenge here
   const uint32_t* prev_dig = nullptr;
   const uint32_t* alpha_first = nullptr;
-  {
+  const std::string first_name = "fri_fold_" + std::to_string(first);
+  if (!first_on_host) {
+    std::vector<uint32_t> pv((size_t)B_ * 8);
+    for (int b = 0; b < B_; ++b) {
+      const Digest& d = ts_[b].previous_value(first_name);
+      for (int k = 0; k < 8; ++k) pv[(size_t)8 * b + k] = d.v[k];
+    }
+    std::vector<uint32_t*> tp(B_);
+    for (int b = 0; b < B_; ++b) tp[b] = levels_tree.node_ptr(b);
+    uint32_t* dig_f = chal_dig.as<uint32_t>() + (size_t)first * fb;
+    uint32_t* al_f = chal_alpha.as<uint32_t>() + (size_t)first * B_ * 6;
+    const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, first_name);
+    launch_fri_challenge(pre.data(), (int)pre.size(), stage(pv), const_cast<uint32_t* const*>(stage(tp)), dig_f, al_f,
+                         root_dev.as<uint32_t>() + (size_t)first * fb, B_, s_);
+    prev_dig = dig_f;
+    alpha_first = al_f;
+  } else {
     std::vector<uint32_t> dg((size_t)B_ * 8), al((size_t)B_ * 6);
     parallel_for(B_, [&](int b) {
       const std::string nf = "fri_fold_" + std::to_string(first);
@@ -1620,12 +1658,13 @@ void Job::open() {
         TimeIt tm(ctx_, s_, "leaf_hash", (double)B_ * tb.nleaves * (2.0 * 6 + 8) * 4, (double)B_ * tb.nleaves);
         launch_leaf_hash(limb_d[j], 6, ext_msg_d, (int)ext_msg.size(), 0u, hdr_ext, leaf_d[j], tb.nleaves, B_, s_);
       }
-      tree_levels(tb, tree_d[j]);
       uint32_t* dig_j = chal_dig.as<uint32_t>() + (size_t)j * fb;
       uint32_t* al_j = chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6;
-      const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, "fri_fold_" + std::to_string(j));
-      launch_fri_challenge(pre.data(), (int)pre.size(), prev_dig, tree_d[j], dig_j, al_j,
-                           root_dev.as<uint32_t>() + (size_t)j * fb, B_, s_);
+      const RootChallenge rc{string_to_elements(TAG_FSID, "fri_fold_" + std::to_string(j)), prev_dig, dig_j, al_j,
+                             root_dev.as<uint32_t>() + (size_t)j * fb};
+      if (!tree_levels(tb, tree_d[j], &rc))
+        launch_fri_challenge(rc.prefix.data(), (int)rc.prefix.size(), prev_dig, tree_d[j], dig_j, al_j, rc.roots, B_,
+                             s_);
       prev_dig = dig_j;
       alpha_j = al_j;
     }
@@ -1654,7 +1693,14 @@ void Job::open() {
         for (int k = 0; k < 8; ++k) root[b].v[k] = rt[(size_t)j * fb + (size_t)8 * b + k];
       fri_roots.push_back(root);
     }
+    if (!first_on_host)
+      for (int b = 0; b < B_; ++b)
+        for (int k = 0; k < 8; ++k) levels_root[b].v[k] = rt[(size_t)first * fb + (size_t)8 * b + k];
     parallel_for(B_, [&](int b) {
+      if (!first_on_host) {
+        ts_[b].bind(first_name, levels_root[b].v, 8);
+        ts_[b].accept(first_name, &dg[(size_t)first * fb + (size_t)8 * b]);
+      }
       for (int j = first + 1; j < FRI_ROUNDS; ++j) {
         const std::string nf = "fri_fold_" + std::to_string(j);
         ts_[b].bind(nf, fri_roots[j - first - 1][b].v, 8);

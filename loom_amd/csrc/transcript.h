@@ -92,6 +92,13 @@ class Transcript {
     log_.push_back(p);
     return values_[p];
   }
+  // The digest of the challenge registered just before `name` (the "previous
+  // value" its message starts with); it must have been computed.
+  const Digest& previous_value(const std::string& name) const {
+    const size_t p = at(name);
+    if (p == 0 || !computed_[p - 1]) throw std::runtime_error("transcript: no computed predecessor of " + name);
+    return values_[p - 1];
+  }
   // Records the digest of `name` as computed elsewhere (the device) from this
   // transcript's message for it (string_to_elements(FSID, name) || previous
   // digest || bindings); the same order checks as compute().
