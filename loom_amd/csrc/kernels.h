@@ -41,8 +41,10 @@ struct NttCfg {
 
 // pres_dev: device array holding cfg.pre[0 .. n_out) (the host copy in cfg is
 // only used for validation).
+// lad_fwd / lad_inv: the radix-16 pass twiddle tables (Tables::ntt_ladders).
 void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
-                const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s);
+                const uint32_t* tw_fwd, const uint32_t* tw_inv, const uint32_t* lad_fwd, const uint32_t* lad_inv,
+                hipStream_t s);
 
 // Two-pass (four-step) transform for 2^15 <= n <= 2^22, n = n1 n2 with
 // n1 = 2^ceil(logn/2): out = post * DFT(pre * in) (natural input; natural or

@@ -129,11 +129,28 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
 // forward (decimation-in-frequency) pass reads, so the two transforms chain
 // in registers.  Forward outputs come out in bit-reversed order, 16
 // contiguous values per thread; a natural-order output goes through LDS.
+// The pass twiddles of sub-size 2^logm at b (= t mod m/16), from the table
+// the host builds with ntt16_ladder: four 16-byte loads instead of 14 field
+// products per pass.
+__device__ __forceinline__ Ladder load_ladder(const uint32_t* __restrict__ lad, int logm, uint32_t b) {
+  const uint4* p = reinterpret_cast<const uint4*>(lad + 16 * ((1u << (logm - 4)) - 1u + b));
+  const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+  Ladder L;
+  L.a[0] = v0.x; L.a[1] = v0.y; L.a[2] = v0.z; L.a[3] = v0.w;
+  L.a[4] = v1.x; L.a[5] = v1.y; L.a[6] = v1.z; L.a[7] = v1.w;
+  L.b[0] = v2.x; L.b[1] = v2.y; L.b[2] = v2.z; L.b[3] = v2.w;
+  L.c[0] = v3.x; L.c[1] = v3.y;
+  L.d = v3.z;
+  return L;
+}
+
 template <int KIND>
 __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs, NttArgs cfg,
                                                 const uint32_t* const* __restrict__ pres,
                                                 const uint32_t* __restrict__ tw_fwd,
-                                                const uint32_t* __restrict__ tw_inv) {
+                                                const uint32_t* __restrict__ tw_inv,
+                                                const uint32_t* __restrict__ lad_fwd,
+                                                const uint32_t* __restrict__ lad_inv) {
   extern __shared__ uint32_t sh[];
   const NttJob* __restrict__ jp = jobs + blockIdx.x;
   const int L = cfg.logn;
@@ -164,7 +181,7 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
       const uint32_t q = 1u << (logm - 4);
 #pragma unroll
       for (int k = 0; k < 16; ++k) x[k] = sh[ntt16_pos(t, q, k)];
-      const Ladder Ld = ntt16_ladder(tw_inv[(t % q) << (L - logm)], Ri);
+      const Ladder Ld = load_ladder(lad_inv, logm, t % q);
       dit16(x, Ld);
       if (p < P - 1) {
 #pragma unroll
@@ -207,7 +224,7 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
 #pragma unroll
         for (int k = 0; k < 16; ++k) y[k] = sh[ntt16_pos(t, q, k)];
       }
-      const Ladder Ld = ntt16_ladder(tw_fwd[(t % q) << (L - logm)], Rf);
+      const Ladder Ld = load_ladder(lad_fwd, logm, t % q);
       dif16(y, Ld);
       if (p < P - 1 || r) {
 #pragma unroll
@@ -245,7 +262,8 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
 }
 
 void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uint32_t* const* pres_dev,
-                const uint32_t* tw_fwd, const uint32_t* tw_inv, hipStream_t s) {
+                const uint32_t* tw_fwd, const uint32_t* tw_inv, const uint32_t* lad_fwd, const uint32_t* lad_inv,
+                hipStream_t s) {
   if (n_jobs == 0) return;
   if (cfg.logn > NTT_MAX_LOG_LDS || cfg.logn < 1) throw std::runtime_error("launch_ntt: unsupported size");
   const int n = 1 << cfg.logn;
@@ -260,11 +278,11 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   if (cfg.logn >= 8) {
     const dim3 g(n_jobs), b(n / 16);
     switch (cfg.kind) {
-      case NTT_KIND_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
-      case NTT_KIND_QUOT_COSET: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_COSET>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
-      case NTT_KIND_QUOT_BACK: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_BACK>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
-      case NTT_KIND_CHUNKS: hipLaunchKernelGGL(k_ntt16<NTT_KIND_CHUNKS>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
-      case NTT_KIND_DEEP_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_DEEP_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv); break;
+      case NTT_KIND_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_QUOT_COSET: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_COSET>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_QUOT_BACK: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_BACK>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_CHUNKS: hipLaunchKernelGGL(k_ntt16<NTT_KIND_CHUNKS>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_DEEP_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_DEEP_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
       default: throw std::runtime_error("launch_ntt: unknown kind");
     }
     LG_HIP(hipGetLastError());

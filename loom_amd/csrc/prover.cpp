@@ -17,6 +17,7 @@
 #include <set>
 
 #include "kernels.h"
+#include "ntt16.h"
 #include "pool.h"
 #include "transcript.h"
 
@@ -85,6 +86,31 @@ const uint32_t* Tables::tw_fwd(int logn) {
 const uint32_t* Tables::tw_inv(int logn) {
   return get("twi" + std::to_string(logn),
              [&] { return powers(kb_inv(omega(logn)), std::max<size_t>(1, (1u << logn) / 2)); });
+}
+
+const uint32_t* Tables::ntt_ladders(bool inverse) {
+  return get(inverse ? "ladi" : "ladf", [&] {
+    auto root = [&](int logn) { return inverse ? kb_inv(omega(logn)) : omega(logn); };
+    NttRoots R;
+    for (int k = 0; k < 8; ++k) R.r16[k] = kb_pow(root(4), k);
+    for (int k = 0; k < 4; ++k) R.r8[k] = kb_pow(root(3), k);
+    R.r4 = root(2);
+    std::vector<uint32_t> t((size_t)16 * ((1u << (NTT_MAX_LOG_LDS - 3)) - 1), 0);
+    for (int logm = 4; logm <= NTT_MAX_LOG_LDS; ++logm) {
+      const uint32_t q = 1u << (logm - 4), w = root(logm);
+      uint32_t W = KB_R;
+      for (uint32_t b = 0; b < q; ++b, W = kb_mul(W, w)) {
+        const Ladder L = ntt16_ladder(W, R);
+        uint32_t* e = &t[(size_t)16 * (q - 1 + b)];
+        for (int k = 0; k < 8; ++k) e[k] = L.a[k];
+        for (int k = 0; k < 4; ++k) e[8 + k] = L.b[k];
+        e[12] = L.c[0];
+        e[13] = L.c[1];
+        e[14] = L.d;
+      }
+    }
+    return t;
+  });
 }
 
 const uint32_t* Tables::wpow(int logn) {
@@ -667,7 +693,8 @@ void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hip
   if (logn <= NTT_MAX_LOG_LDS) {
     const NttJob* jd = static_cast<const NttJob*>(stage(jobs.data(), jobs.size() * sizeof(NttJob)));
     const uint32_t* const* pd = static_cast<const uint32_t* const*>(stage(cfg.pre, sizeof(cfg.pre)));
-    launch_ntt(jd, (int)jobs.size(), cfg, pd, tb.tw_fwd(logn), tb.tw_inv(logn), s);
+    launch_ntt(jd, (int)jobs.size(), cfg, pd, tb.tw_fwd(logn), tb.tw_inv(logn), tb.ntt_ladders(false),
+               tb.ntt_ladders(true), s);
     return;
   }
   // Four-step form: every transform is two launches over a group of jobs;

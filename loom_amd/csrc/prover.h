@@ -31,6 +31,10 @@ class Tables {
   const uint32_t* ginv_pow(int log_len);          // g^{-m}, m < 2^log_len
   const uint32_t* fri_xinv2(int log_m);           // (w_m^-1)^{bitrev_{m/2}(i)} / 2
   const uint32_t* gpow_small(int count);          // g^c, c < count
+  // Radix-16 pass twiddles of the LDS NTT: for each sub-size m = 2^logm
+  // (4 <= logm <= 14) and b < m/16, the 15 words of ntt16_ladder(w_m^b)
+  // (w_m^-b when inverse) at 16 * (m/16 - 1 + b), padded to 16.
+  const uint32_t* ntt_ladders(bool inverse);
 
  private:
   const uint32_t* get(const std::string& key, const std::function<std::vector<uint32_t>()>& make);
