@@ -24,12 +24,17 @@ constexpr uint32_t TAG_LEAF_MONT = 0x6d420179u;  // Montgomery form of TAG_LEAF
 constexpr uint32_t TAG_NODE_MONT = 0x6425c2feu;  // Montgomery form of TAG_NODE
 
 template <int W>
+struct P2Consts;
+
+// External layer; with RK >= 0 the round keys of full round RK (which the
+// next S-boxes take) are added before the reduction.  One reduction per
+// output: the M4 outputs (coefficient sum 7) and the chunk-wise sums are
+// formed in 64 bits from 32-bit pair sums (x0 + x1 < 2p fits), so every output
+// (plus a key) is below 50p < 50 * 2^31; with v = hi 2^31 + lo and
+// 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is below 2^31 + 50 * 2^24 < 2p,
+// and one unsigned minimum finishes.
+template <int W, int RK = -1>
 LG_HD void p2_external(uint32_t* s) {
-  // One reduction per output: the M4 outputs (coefficient sum 7) and the
-  // chunk-wise sums are formed in 64 bits from 32-bit pair sums (x0 + x1 < 2p
-  // fits), so every output is below 49p < 49 * 2^31; with v = hi 2^31 + lo
-  // and 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is below
-  // 2^31 + 49 * 2^24 < 2p, and one unsigned minimum finishes.
   uint64_t y[W];
 #pragma unroll
   for (int c = 0; c < W / 4; ++c) {
@@ -52,15 +57,13 @@ LG_HD void p2_external(uint32_t* s) {
   }
 #pragma unroll
   for (int i = 0; i < W; ++i) {
-    const uint64_t v = y[i] + sum[i & 3];
+    uint64_t v = y[i] + sum[i & 3];
+    if constexpr (RK >= 0) v += P2Consts<W>::full(RK, i);
     const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
     const uint32_t r = lo + (hi << 24) - hi;
     s[i] = kb_umin(r, r - KB_P);
   }
 }
-
-template <int W>
-struct P2Consts;
 
 template <>
 struct P2Consts<24> {
@@ -86,11 +89,11 @@ LG_HD uint32_t kb_cube(uint32_t x) {
   return kb_mul(sq, x);
 }
 
+// S-boxes of a full round whose keys are already in the state.
 template <int W>
-LG_HD void p2_full_round(uint32_t* s, int r) {
+LG_HD void p2_cube_all(uint32_t* s) {
 #pragma unroll
-  for (int i = 0; i < W; ++i) s[i] = kb_cube(kb_add(s[i], P2Consts<W>::full(r, i)));
-  p2_external<W>(s);
+  for (int i = 0; i < W; ++i) s[i] = kb_cube(s[i]);
 }
 
 // The internal diagonal entries are small integers and (negated) inverse
@@ -183,13 +186,25 @@ LG_HD void p2_partial_round(uint32_t* s, int r) {
 
 template <int W>
 LG_HD void p2_permute(uint32_t* s) {
+  // full round r = keys, S-boxes, external layer; the keys of a full round
+  // that follows an external layer are folded into that layer's reduction
+  p2_external<W, 0>(s);
+  p2_cube_all<W>(s);
+  p2_external<W, 1>(s);
+  p2_cube_all<W>(s);
+  p2_external<W, 2>(s);
+  p2_cube_all<W>(s);
   p2_external<W>(s);
-#pragma unroll
-  for (int r = 0; r < 3; ++r) p2_full_round<W>(s, r);
 #pragma unroll
   for (int r = 0; r < 21; ++r) p2_partial_round<W>(s, r);
 #pragma unroll
-  for (int r = 3; r < 6; ++r) p2_full_round<W>(s, r);
+  for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], P2Consts<W>::full(3, i));
+  p2_cube_all<W>(s);
+  p2_external<W, 4>(s);
+  p2_cube_all<W>(s);
+  p2_external<W, 5>(s);
+  p2_cube_all<W>(s);
+  p2_external<W>(s);
 }
 
 // node = P([NODE, 0 x 7, L, R])[0..8]; l, r, out: 8 Montgomery elements each.
