@@ -1327,6 +1327,9 @@ __global__ void k_fri_fold(const uint32_t* const* in, uint32_t* const* out, cons
   st6(out[b], half, i, e6_add(s, dlt));
 }
 
+// Layers of at most this many leaf pairs take the cooperative form below.
+constexpr int FRI_COOP_MAX_PAIRS = 512;
+
 // One FRI fold fused with the leaf hashing of the folded layer's tree: thread
 // k forms next[2k] and next[2k+1] (from in[4k .. 4k+3]), stores them, and
 // hashes the row pair as the leaf sponge [LEAF, 0, hdr2, next[2k] limbs,
@@ -1362,15 +1365,66 @@ __global__ __launch_bounds__(256) void k_fri_fold_leaf(const uint32_t* const* in
   for (int i = 0; i < 8; ++i) o[i] = s[i];
 }
 
+// The same for small layers, latency-bound: one 16-lane row per leaf
+// (P2Coop16).  Every lane of the row forms both fold outputs; lanes 0..5
+// store the limbs of next[2k], lanes 6..11 those of next[2k+1], and the row
+// hashes the leaf with one cooperative permutation.
+__global__ __launch_bounds__(256) void k_fri_fold_leaf_coop(const uint32_t* const* in, uint32_t* const* out,
+                                                            const uint32_t* alpha, const uint32_t* xinv2, int half,
+                                                            uint32_t hdr2, uint32_t* const* leaves) {
+  const int l = threadIdx.x & 15;
+  const int k = blockIdx.x * 16 + (threadIdx.x >> 4);
+  if (2 * k >= half) return;  // whole rows only: 16 divides the block
+  const int b = blockIdx.y;
+  const uint32_t* L = in[b];
+  const size_t m = 2 * (size_t)half;
+  constexpr uint32_t INV2_MONT = 0x00ffffffu;
+  const E6 a6 = ldc6(alpha + 6 * b);
+  E6 r[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int i = 2 * k + t;
+    const E6 p = ld6(L, m, 2 * i), q = ld6(L, m, 2 * i + 1);
+    r[t] = e6_add(e6_scale(e6_add(p, q), INV2_MONT), e6_mul(e6_scale(e6_sub(p, q), xinv2[i]), a6));
+  }
+  if (l < 12) {
+    const int t = l < 6 ? 0 : 1, limb = l < 6 ? l : l - 6;
+    out[b][(size_t)limb * half + 2 * k + t] = r[t].v[limb];
+  }
+  // state elements: 0 LEAF, 1 zero, 2 hdr2, 3..8 next[2k], 9..14 next[2k+1]
+  uint32_t sa = 0u, sb = 0u;
+  if (l == 0) sa = TAG_LEAF_MONT;
+  else if (l == 2) sa = hdr2;
+#pragma unroll
+  for (int i = 3; i < 9; ++i)
+    if (l == i) sa = r[0].v[i - 3];
+#pragma unroll
+  for (int i = 9; i < 12; ++i)
+    if (l == i) sa = r[1].v[i - 9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    if (l == i) sb = r[1].v[3 + i];
+  P2Coop16 P;
+  P.init(l);
+  P.permute(sa, sb);
+  if (l < 8) leaves[b][(size_t)8 * k + l] = sa;
+}
+
 void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha,
                           const uint32_t* xinv2, int half, uint32_t hdr2, uint32_t* const* leaves, int batch,
                           hipStream_t s) {
   if (half < 2 || (half & 1)) throw std::runtime_error("fri fold+leaf: layer length must be even");
   const int pairs = half / 2;
-  const int T = pairs < 256 ? pairs : 256;
   prelaunch();
-  hipLaunchKernelGGL(k_fri_fold_leaf, dim3((pairs + T - 1) / T, batch), dim3(T), 0, s, in, out, alpha, xinv2, half,
-                     hdr2, leaves);
+  if (pairs <= FRI_COOP_MAX_PAIRS) {
+    // rows of 16 lanes, 16 rows per block; rows past the layer return whole
+    hipLaunchKernelGGL(k_fri_fold_leaf_coop, dim3((pairs + 15) / 16, batch), dim3(256), 0, s, in, out, alpha, xinv2,
+                       half, hdr2, leaves);
+  } else {
+    const int T = pairs < 256 ? pairs : 256;
+    hipLaunchKernelGGL(k_fri_fold_leaf, dim3((pairs + T - 1) / T, batch), dim3(T), 0, s, in, out, alpha, xinv2,
+                       half, hdr2, leaves);
+  }
   LG_HIP(hipGetLastError());
 }
 
