@@ -107,7 +107,7 @@ void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_
 // ratio (scratch) and gp are E6 columns as 6 limb arrays (limb stride n).
 // Scans of n >= SCAN_MIN_TILED rows run over many workgroups and need
 // scan_tmp_words(n, batch) words of scratch (scan_tmp); smaller ones need none.
-constexpr int SCAN_MIN_TILED = 1 << 16;
+constexpr int SCAN_MIN_TILED = 1 << 13;
 size_t scan_tmp_words(int n, int batch);
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
                           uint32_t* const* gp, int n, int batch, uint32_t* scan_tmp, hipStream_t s);

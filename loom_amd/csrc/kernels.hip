@@ -842,7 +842,7 @@ __global__ __launch_bounds__(1024) void k_prefix_product(const uint32_t* const* 
   }
 }
 
-// Three-pass scan of E6 columns for n >= 2^16 (tiles of 2048 rows): tile
+// Three-pass scan of E6 columns for n >= 2^13 (tiles of 2048 rows): tile
 // aggregates, an exclusive scan of the aggregates per instance, then each tile
 // rescanned from its prefix.  MUL: products (grand product), else sums
 // (logup); the field operations commute, so a tile's aggregate can be formed

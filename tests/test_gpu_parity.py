@@ -150,7 +150,7 @@ def test_merkle(torch, ctx, logn, inj):
     assert np.array_equal(got[1:], t.nodes.T[1:])
 
 
-@pytest.mark.parametrize("logn", [0, 3, 6, 10, 14, 16, 17])
+@pytest.mark.parametrize("logn", [0, 3, 6, 10, 12, 13, 14, 16, 17])
 def test_grand_product(torch, ctx, logn):
     from loom_amd import _lib
     from oracle import field as F
