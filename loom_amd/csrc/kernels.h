@@ -173,6 +173,23 @@ struct DeepDesc {
   uint32_t* const* out;       // [b] 6 limb arrays, stride n
 };
 void launch_deep(const DeepDesc& d, int batch, hipStream_t s);
+// alpha_DEEP derived on the device with the DEEP constants of a single class
+// (see k_deep_alpha): prev: the previous challenge's digest per instance;
+// cvals: [b][ncv][6] claimed values, order: binding order (ncv indexes);
+// terms: (bundle, claimed value, point) triples of the class.
+struct DeepAlphaArgs {
+  int n_pre;
+  const uint32_t* prev;
+  const uint32_t* cvals;
+  const int32_t* order;
+  int ncv;
+  const int3* terms;
+  int n_terms, n_bundles, n_points;
+  uint32_t* dig;     // [b][8] the alpha_DEEP digest
+  uint32_t* scale;   // [b][bundle][6]
+  uint32_t* consts;  // [b][point][6]
+};
+void launch_deep_alpha(const DeepAlphaArgs& a, const uint32_t* prefix, int n_prefix, int batch, hipStream_t s);
 
 // FRI fold (fri.go:1177-1198): next[i] = (p+q)/2 + alpha*(p-q)/2*xinv[i],
 // p = layer[2i], q = layer[2i+1]; E6 layers as 6 limb arrays.

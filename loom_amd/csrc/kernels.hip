@@ -1464,6 +1464,72 @@ __global__ __launch_bounds__(64) void k_fri_challenge(ChalPrefix pre, int n_pre,
   }
 }
 
+// alpha_DEEP on the device, with the DEEP constants of one class
+// (deep.go:71-266): per instance (one 16-lane row) the sponge of
+// prefix || prev[b] || the claimed values in binding order (value order[j] of
+// cvals[b]), then, on lane 0, scale[b][i] = alpha^i for the bundles and
+// consts[b][p] = sum over the terms (i, k, p) of alpha^i cvals[b][k].
+__global__ __launch_bounds__(64) void k_deep_alpha(DeepAlphaArgs a, ChalPrefix pre, int batch) {
+  __shared__ uint32_t alpha_sh[4][8];
+  const int l = threadIdx.x & 15, row = threadIdx.x >> 4;
+  const int b_raw = blockIdx.x * 4 + row;
+  const bool live = b_raw < batch;
+  const int b = live ? b_raw : batch - 1;
+  const uint32_t* cv = a.cvals + (size_t)b * a.ncv * 6;
+  auto msg = [&](int i) -> uint32_t {
+    if (i < a.n_pre) return pre.w[i];
+    if (i < a.n_pre + 8) return a.prev[8 * b + (i - a.n_pre)];
+    const int j = i - a.n_pre - 8;
+    return cv[6 * a.order[j / 6] + j % 6];
+  };
+  P2Coop16 P;
+  P.init(l);
+  uint32_t x = 0u, y = 0u;
+  const int len = a.n_pre + 8 + 6 * a.ncv;
+  for (int off = 0; off < len; off += 16) {
+    const int k = len - off < 16 ? len - off : 16;
+    if (l < 12 && l < k) x = msg(off + l);
+    if (l < 12 && l + 12 < k) y = msg(off + l + 12);
+    P.permute(x, y);
+  }
+  if (l < 8) {
+    alpha_sh[row][l] = x;
+    if (live) a.dig[8 * b + l] = x;
+  }
+  __syncthreads();
+  if (!live || l != 0) return;
+  E6 al;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) al.v[i] = alpha_sh[row][i];
+  E6 pw = e6_one();
+  for (int i = 0; i < a.n_bundles; ++i) {
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.scale[((size_t)b * a.n_bundles + i) * 6 + q] = pw.v[q];
+    pw = e6_mul(pw, al);
+  }
+  for (int p = 0; p < a.n_points; ++p) {
+    E6 c = e6_zero();
+    for (int t = 0; t < a.n_terms; ++t) {
+      const int3 tm = a.terms[t];  // (bundle, claimed value, point)
+      if (tm.z != p) continue;
+      c = e6_add(c, e6_mul(ldc6(a.scale + ((size_t)b * a.n_bundles + tm.x) * 6), ldc6(cv + 6 * tm.y)));
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) a.consts[((size_t)b * a.n_points + p) * 6 + q] = c.v[q];
+  }
+}
+
+void launch_deep_alpha(const DeepAlphaArgs& a, const uint32_t* prefix, int n_prefix, int batch, hipStream_t s) {
+  if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("deep alpha: bad prefix length");
+  ChalPrefix pre{};
+  for (int i = 0; i < n_prefix; ++i) pre.w[i] = prefix[i];
+  DeepAlphaArgs args = a;
+  args.n_pre = n_prefix;
+  prelaunch();
+  hipLaunchKernelGGL(k_deep_alpha, dim3((batch + 3) / 4), dim3(64), 0, s, args, pre, batch);
+  LG_HIP(hipGetLastError());
+}
+
 void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* prev, uint32_t* const* trees,
                           uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch, hipStream_t s) {
   if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("fri challenge: bad prefix length");

@@ -302,10 +302,16 @@ uint32_t* Context::pinned_in(size_t bytes, int lane) {
   return grow_pinned(L.pinned_in, L.pinned_in_cap, bytes);
 }
 
+uint32_t* Context::pinned_aux(size_t bytes, int lane) {
+  LaneBufs& L = lane_.at(lane);
+  return grow_pinned(L.pinned_aux, L.pinned_aux_cap, bytes);
+}
+
 Context::~Context() {
   for (auto& L : lane_) {
     if (L.pinned) (void)hipHostFree(L.pinned);
     if (L.pinned_in) (void)hipHostFree(L.pinned_in);
+    if (L.pinned_aux) (void)hipHostFree(L.pinned_aux);
     if (L.stream) {
       (void)hipStreamSynchronize(L.stream);
       (void)hipStreamDestroy(L.stream);
@@ -438,6 +444,7 @@ struct TreeBuf {
   std::vector<size_t> order;          // group indexes, decreasing N (stable)
   std::vector<DevBuf> inj;            // per order position >= 1: B x width x 8
   std::vector<int> inj_width;
+  uint32_t* const* nodes_d = nullptr;  // staged node-array pointers of the batch (set by build_tree)
   uint32_t* node_ptr(int b) const { return nodes.as<uint32_t>() + (size_t)b * 2 * nleaves * 8; }
   uint32_t* inj_ptr(size_t k, int b) const { return inj[k].as<uint32_t>() + (size_t)b * inj_width[k] * 8; }
 };
@@ -817,7 +824,8 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
   }
   std::vector<uint32_t*> tp(B_);
   for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
-  tree_levels(tb, const_cast<uint32_t* const*>(stage(tp)));
+  tb.nodes_d = const_cast<uint32_t* const*>(stage(tp));
+  tree_levels(tb, tb.nodes_d);
 }
 
 // The inner levels of a tree whose leaves (and injection digests) are in
@@ -881,10 +889,14 @@ bool Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* c
 }
 
 std::vector<Digest> Job::roots_of(TreeBuf& tb) {
-  std::vector<const uint32_t*> tp(B_);
-  for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
+  const uint32_t* const* tp_d = tb.nodes_d;
+  if (!tp_d) {
+    std::vector<const uint32_t*> tp(B_);
+    for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
+    tp_d = stage(tp);
+  }
   DevBuf r = dev((size_t)B_ * 8);
-  launch_copy_roots(stage(tp), r.as<uint32_t>(), B_, s_);
+  launch_copy_roots(tp_d, r.as<uint32_t>(), B_, s_);
   std::vector<Digest> out(B_);
   LG_HIP(hipMemcpyAsync(out.data(), r.as<void>(), (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
   sync();
@@ -1396,20 +1408,11 @@ void Job::open() {
     TimeIt tm(ctx_, s_, "claims", (double)kv.second.size() * kv.first * 7 * 4, 0);
     launch_claims(stage(kv.second), (int)kv.second.size(), kv.first, part, s_);
   }
-  std::vector<E6> cvals((size_t)B_ * ncv);
-  LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
-  sync();
-  keep_.push_back(std::move(cvd));
-  trace("claimed values");
   // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
-  parallel_for(B_, [&](int b) {
-    ts_[b].new_challenge("alpha_DEEP");
-    for (size_t mi = 0; mi < nm; ++mi)
-      for (size_t k = 0; k < ncv; ++k)
-        if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi)
-          ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
-  });
-  // FRI challenge registration (fri.go:427-437)
+  std::vector<int32_t> bind_order;  // claimed-value indexes in binding order
+  for (size_t mi = 0; mi < nm; ++mi)
+    for (size_t k = 0; k < ncv; ++k)
+      if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi) bind_order.push_back((int32_t)k);
   std::vector<size_t> classes;
   for (size_t mi = 0; mi < nm; ++mi)
     for (size_t k = 0; k < ncv; ++k)
@@ -1423,14 +1426,37 @@ void Job::open() {
     return d_.modules[classes[a]].N > d_.modules[classes[b]].N;
   });
   const size_t L = classes.size();
+  // With one DEEP class (every module the same size), alpha_DEEP and the DEEP
+  // constants are derived on the device (k_deep_alpha) and the host replays
+  // the claimed values and the digest into its transcripts while the DEEP
+  // codeword and the levels tree are computed; otherwise the host derives
+  // alpha_DEEP from the claimed values it copies back.
+  const bool deep_on_device = L == 1;
+  std::vector<E6> cvals((size_t)B_ * ncv);
   std::vector<E6> alpha(B_);
   parallel_for(B_, [&](int b) {
-    const Digest& dg = ts_[b].compute("alpha_DEEP");
-    for (int l = 0; l < 6; ++l) alpha[b].v[l] = dg.v[l];
+    ts_[b].new_challenge("alpha_DEEP");
     for (size_t l = 1; l < L; ++l) ts_[b].new_challenge("fri_level_" + std::to_string(l) + "_gamma");
     for (int j = 0; j < FRI_ROUNDS; ++j) ts_[b].new_challenge("fri_fold_" + std::to_string(j));
     for (int k = 0; k < nq; ++k) ts_[b].new_challenge("fri_query_" + std::to_string(k));
   });
+  if (!deep_on_device) {
+    LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
+    sync();
+    parallel_for(B_, [&](int b) {
+      for (int32_t k : bind_order) ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
+      const Digest& dg = ts_[b].compute("alpha_DEEP");
+      for (int l = 0; l < 6; ++l) alpha[b].v[l] = dg.v[l];
+    });
+  }
+  trace("claimed values + alpha");
+  struct EventHolder {
+    hipEvent_t e = nullptr;
+    ~EventHolder() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } alpha_ready;
+  uint32_t* alpha_h = nullptr;  // pinned: [b][8] digests, then the claimed values
   // ---- DEEP quotient per class, RS-encoded (deep.go:71-266)
   std::vector<Layer> level(L);
   for (size_t ci = 0; ci < L; ++ci) {
@@ -1472,21 +1498,64 @@ void Job::open() {
     const int np = (int)pt_shift.size();
     std::vector<E6> scale((size_t)B_ * bundles.size()), consts((size_t)B_ * np), points((size_t)B_ * np);
     std::vector<const uint32_t*> cp;
+    const uint32_t* scale_d = nullptr;
+    const uint32_t* consts_d = nullptr;
     for (int b = 0; b < B_; ++b) {
-      E6 a = e6_one();
-      for (int p = 0; p < np; ++p) {
-        consts[(size_t)b * np + p] = e6_zero();
-        points[(size_t)b * np + p] = e6_scale(zeta_[b], kb_pow(omega(m.log_n), pt_shift[p]));
-      }
-      for (size_t i = 0; i < bundles.size(); ++i) {
-        scale[(size_t)b * bundles.size() + i] = a;
-        for (auto& kv : bundle_vals[i]) {
-          E6& c = consts[(size_t)b * np + kv.second];
-          c = e6_add(c, e6_mul(a, cvals[(size_t)b * ncv + kv.first]));
-        }
-        a = e6_mul(a, alpha[b]);
-      }
+      for (int p = 0; p < np; ++p) points[(size_t)b * np + p] = e6_scale(zeta_[b], kb_pow(omega(m.log_n), pt_shift[p]));
       for (auto c : bcols) cp.push_back(cols_[c].lag(b, 0));
+    }
+    if (deep_on_device) {
+      std::vector<int3> terms;
+      for (size_t i = 0; i < bundles.size(); ++i)
+        for (auto& kv : bundle_vals[i]) terms.push_back(make_int3((int)i, (int)kv.first, kv.second));
+      std::vector<uint32_t> prev((size_t)B_ * 8);
+      for (int b = 0; b < B_; ++b) {
+        const Digest& d = ts_[b].previous_value("alpha_DEEP");
+        for (int k = 0; k < 8; ++k) prev[(size_t)8 * b + k] = d.v[k];
+      }
+      DevBuf dig = dev((size_t)B_ * 8);
+      DevBuf sc = dev((size_t)B_ * bundles.size() * 6 + 6);
+      DevBuf cs = dev((size_t)B_ * np * 6 + 6);
+      DeepAlphaArgs da{};
+      da.prev = stage(prev);
+      da.cvals = cvd.as<uint32_t>();
+      da.order = stage(bind_order);
+      da.ncv = (int)ncv;
+      da.terms = stage(terms);
+      da.n_terms = (int)terms.size();
+      da.n_bundles = (int)bundles.size();
+      da.n_points = np;
+      da.dig = dig.as<uint32_t>();
+      da.scale = sc.as<uint32_t>();
+      da.consts = cs.as<uint32_t>();
+      const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, "alpha_DEEP");
+      launch_deep_alpha(da, pre.data(), (int)pre.size(), B_, s_);
+      alpha_h = ctx_.pinned_aux(((size_t)B_ * 8 + (size_t)B_ * ncv * 6) * 4, lane_);
+      LG_HIP(hipMemcpyAsync(alpha_h, dig.as<void>(), (size_t)B_ * 8 * 4, hipMemcpyDeviceToHost, s_));
+      LG_HIP(hipMemcpyAsync(alpha_h + (size_t)B_ * 8, cvd.as<void>(), (size_t)B_ * ncv * 6 * 4, hipMemcpyDeviceToHost,
+                            s_));
+      LG_HIP(hipEventCreateWithFlags(&alpha_ready.e, hipEventDisableTiming));
+      LG_HIP(hipEventRecord(alpha_ready.e, s_));
+      scale_d = sc.as<uint32_t>();
+      consts_d = cs.as<uint32_t>();
+      keep_.push_back(std::move(dig));
+      keep_.push_back(std::move(sc));
+      keep_.push_back(std::move(cs));
+    } else {
+      for (int b = 0; b < B_; ++b) {
+        E6 a = e6_one();
+        for (int p = 0; p < np; ++p) consts[(size_t)b * np + p] = e6_zero();
+        for (size_t i = 0; i < bundles.size(); ++i) {
+          scale[(size_t)b * bundles.size() + i] = a;
+          for (auto& kv : bundle_vals[i]) {
+            E6& c = consts[(size_t)b * np + kv.second];
+            c = e6_add(c, e6_mul(a, cvals[(size_t)b * ncv + kv.first]));
+          }
+          a = e6_mul(a, alpha[b]);
+        }
+      }
+      scale_d = reinterpret_cast<const uint32_t*>(stg_->put(scale.data(), scale.size() * sizeof(E6), s_));
+      consts_d = reinterpret_cast<const uint32_t*>(stg_->put(consts.data(), consts.size() * sizeof(E6), s_));
     }
     Layer dq;
     dq.len = m.N;
@@ -1501,8 +1570,8 @@ void Job::open() {
     dd.pt_list = stage(ptl);
     dd.cols = stage(cp);
     dd.n_cols = (int)bcols.size();
-    dd.scale = reinterpret_cast<const uint32_t*>(stg_->put(scale.data(), scale.size() * sizeof(E6), s_));
-    dd.consts = reinterpret_cast<const uint32_t*>(stg_->put(consts.data(), consts.size() * sizeof(E6), s_));
+    dd.scale = scale_d;
+    dd.consts = consts_d;
     dd.points = reinterpret_cast<const uint32_t*>(stg_->put(points.data(), points.size() * sizeof(E6), s_));
     dd.wpow = tb_.wpow(m.log_n);
     dd.out = const_cast<uint32_t* const*>(stage(op));
@@ -1557,6 +1626,18 @@ void Job::open() {
   const int first = FRI_LOG_D - d_.modules[classes[lv_order[0]]].log_n;
   TreeBuf levels_tree;
   ext_tree(levels_tree, lvp);
+  if (deep_on_device) {
+    // the claimed values and alpha_DEEP come back while the DEEP codeword and
+    // the levels tree are computed; replay them into the transcripts
+    LG_HIP(hipEventSynchronize(alpha_ready.e));
+    const uint32_t* cv_h = alpha_h + (size_t)B_ * 8;
+    std::memcpy(cvals.data(), cv_h, cvals.size() * sizeof(E6));
+    parallel_for(B_, [&](int b) {
+      for (int32_t k : bind_order) ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
+      ts_[b].accept("alpha_DEEP", alpha_h + (size_t)8 * b);
+    });
+  }
+  keep_.push_back(std::move(cvd));
   // With one FRI level nothing on the host needs the levels root before the
   // fold rounds: round `first` then derives its challenge on the device like
   // the later rounds, and the root comes back with theirs.

@@ -141,6 +141,7 @@ struct Context {
   // Pinned host buffers per lane, reused across proves.
   uint32_t* pinned(size_t bytes, int lane);     // device -> host (query openings)
   uint32_t* pinned_in(size_t bytes, int lane);  // host -> device (trace columns)
+  uint32_t* pinned_aux(size_t bytes, int lane);  // small device -> host results (claimed values, digests)
   Context();
   ~Context();
 
@@ -151,6 +152,8 @@ struct Context {
     size_t pinned_cap = 0;
     uint32_t* pinned_in = nullptr;
     size_t pinned_in_cap = 0;
+    uint32_t* pinned_aux = nullptr;
+    size_t pinned_aux_cap = 0;
   };
   std::mutex lane_mu_;
   std::vector<LaneBufs> lane_;
