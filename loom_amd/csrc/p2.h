@@ -184,6 +184,70 @@ LG_HD void p2_partial_round(uint32_t* s, int r) {
   p2_diag_all<W, 0>(s, p2_state_sum<W>(s));
 }
 
+#if !defined(__HIP_DEVICE_COMPILE__)
+// Host form of the partial rounds.  The internal layer is linear with small
+// diagonal entries, so lanes 1..W-1 are carried as signed 64-bit integers
+// congruent to the state and reduced only every fourth round: the sum is a
+// plain integer sum, c*x a plain product, and x*2^-k is (x + m p) >> k with
+// m = -x mod 2^k (p = 1 mod 2^24 makes x + m p divisible by 2^k; >> of a
+// negative value is the arithmetic shift in GCC and Clang).  Lane 0 is
+// reduced every round for its S-box.  Bound: a lane starts a run below
+// M = 2^31, one round gives |S| <= p + (W-1) M and |lane| <= |S| + 4 M + p
+// <= 27 M + 2p (W = 24), so four rounds stay below 2^50.2, the round sums
+// below 2^55, well inside the 2^62 that kb_reduce_i64 accepts.
+inline uint32_t kb_reduce_i64(int64_t v) {
+  constexpr uint64_t OFF = ((1ull << 62) / KB_P + 1) * KB_P;  // multiple of p, >= 2^62
+  return (uint32_t)(((uint64_t)v + OFF) % KB_P);
+}
+
+template <int W, int I>
+inline int64_t p2_diag_i64(int64_t sum, int64_t x) {
+  constexpr DiagOp op = p2_diag_op(P2Diag<W>::canon(I));
+  int64_t t;
+  if constexpr (op.kind == 1) {
+    t = (int64_t)op.c * x;
+  } else if constexpr (op.kind == 2) {
+    const uint64_t m = (0u - (uint64_t)x) & ((1ull << op.c) - 1u);
+    t = (x + (int64_t)(m * KB_P)) >> op.c;
+  } else {
+    t = kb_mul(kb_reduce_i64(x), P2Consts<W>::diag(I));
+  }
+  return op.neg ? sum - t : sum + t;
+}
+
+template <int W, int I>
+inline void p2_diag_rest_i64(int64_t* v, int64_t sum) {
+  if constexpr (I < W) {
+    v[I] = p2_diag_i64<W, I>(sum, v[I]);
+    p2_diag_rest_i64<W, I + 1>(v, sum);
+  }
+}
+
+template <int W>
+inline void p2_partial_rounds_host(uint32_t* s) {
+  static_assert(W <= 24 && W % 2 == 0, "lazy bound derived for even widths up to 24");
+  int64_t v[W];  // v[0] holds lane 0 between rounds
+  for (int i = 0; i < W; ++i) v[i] = s[i];
+  // not unrolled: the unrolled rounds overflow the instruction cache
+#pragma nounroll
+  for (int r = 0; r < 21; ++r) {
+    const uint32_t x0 = kb_cube(kb_add(kb_reduce_i64(v[0]), P2Consts<W>::partial(r)));
+    // the sum of lanes 1..W-1 does not wait for the S-box (pairwise: short chain)
+    int64_t pr[W / 2];
+    pr[0] = v[1];
+    for (int i = 1; i < W / 2; ++i) pr[i] = v[2 * i] + v[2 * i + 1];
+    for (int w = 1; w < W / 2; w *= 2)
+      for (int i = 0; i + w < W / 2; i += 2 * w) pr[i] += pr[i + w];
+    const int64_t sum = pr[0] + x0;
+    v[0] = p2_diag_i64<W, 0>(sum, x0);
+    p2_diag_rest_i64<W, 1>(v, sum);
+    if ((r & 3) == 3)
+      for (int i = 1; i < W; ++i) v[i] = kb_reduce_i64(v[i]);
+  }
+  for (int i = 0; i < W; ++i) s[i] = kb_reduce_i64(v[i]);
+}
+#endif
+
 template <int W>
 LG_HD void p2_permute(uint32_t* s) {
   // full round r = keys, S-boxes, external layer; the keys of a full round
@@ -195,8 +259,12 @@ LG_HD void p2_permute(uint32_t* s) {
   p2_external<W, 2>(s);
   p2_cube_all<W>(s);
   p2_external<W>(s);
+#if defined(__HIP_DEVICE_COMPILE__)
 #pragma unroll
   for (int r = 0; r < 21; ++r) p2_partial_round<W>(s, r);
+#else
+  p2_partial_rounds_host<W>(s);
+#endif
 #pragma unroll
   for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], P2Consts<W>::full(3, i));
   p2_cube_all<W>(s);

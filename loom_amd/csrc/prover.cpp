@@ -1830,7 +1830,25 @@ void Job::open() {
   // as soon as every position of a chunk of instances is known, that chunk's
   // positions go to the device, its records are gathered and copied back,
   // and an event marks them ready for the proof assembly.
+  // The top PATH_TOP levels of every tree deeper than that are shared by the
+  // queries' authentication paths: they are copied back once per tree
+  // (nodes 0 .. 2^(PATH_TOP+1) of the heap layout) while the query chains
+  // run, and the records carry only the lower siblings of each path.
   const int chunk = std::min(B_, 8);
+  constexpr int PATH_TOP = 7;
+  constexpr size_t TAB_WORDS = (size_t)8 << (PATH_TOP + 1);
+  int ntab = 0;
+  std::vector<const uint32_t*> tab_src;  // [table][b] node arrays
+  auto path_split = [&](int depth, int& low, int& tab, const TreeBuf& tb) {
+    if (depth > PATH_TOP) {
+      low = depth - PATH_TOP;
+      tab = ntab++;
+      for (int b = 0; b < B_; ++b) tab_src.push_back(tb.node_ptr(b));
+    } else {
+      low = depth;
+      tab = -1;
+    }
+  };
   size_t stride = 0;
   auto seg = [&](size_t words) {  // 16-byte aligned segment of the record
     size_t o = (stride + 3) & ~size_t(3);
@@ -1839,7 +1857,7 @@ void Job::open() {
   };
   struct TreeRec {
     size_t rows_off, path_off, injl_off;
-    int ncols_top, depth, shift;
+    int ncols_top, depth, shift, low, tab;
     std::vector<int> inj_ncols, inj_shift;
     std::vector<size_t> inj_rows_off;
   };
@@ -1879,8 +1897,9 @@ void Job::open() {
     r.depth = 0;
     while ((1 << r.depth) < tb.nleaves) ++r.depth;
     r.shift = FRI_LOG_N - (r.depth + 1);
+    path_split(r.depth, r.low, r.tab, tb);
     r.rows_off = seg(2 * r.ncols_top);
-    r.path_off = seg(8 * r.depth);
+    r.path_off = seg(8 * r.low);
     r.injl_off = seg(8 * tb.inj.size());
     std::vector<const uint32_t*> cp, tp;
     for (int b = 0; b < B_; ++b) {
@@ -1889,7 +1908,7 @@ void Job::open() {
       tp.push_back(tb.node_ptr(b));
     }
     add_op(0, r.shift, r.ncols_top, 0, 0, r.rows_off, cp);
-    add_op(1, r.shift, r.depth, 0, tb.nleaves, r.path_off, tp);
+    add_op(1, r.shift, r.low, 0, tb.nleaves, r.path_off, tp);
     for (size_t k = 0; k < tb.inj.size(); ++k) {
       const Group& g = ti.groups[tb.order[k + 1]];
       const int nc = (int)(g.base.size() + 6 * g.ext.size());
@@ -1912,23 +1931,25 @@ void Job::open() {
   }
   struct FriRec {
     size_t rows_off, path_off;
-    int depth;
+    int depth, low, tab, nleaves;
   };
   std::vector<FriRec> frec(FRI_ROUNDS);
   for (int j = first; j < FRI_ROUNDS; ++j) {
     FriRec& r = frec[j];
-    r.depth = FRI_LOG_N - j - 1;
-    r.rows_off = seg(12);
-    r.path_off = seg(8 * r.depth);
     const Layer& lay = (j == first) ? *lvp[0] : layers[j];
     const TreeBuf& tb = (j == first) ? levels_tree : ftrees[j];
+    r.depth = FRI_LOG_N - j - 1;
+    r.nleaves = tb.nleaves;
+    path_split(r.depth, r.low, r.tab, tb);
+    r.rows_off = seg(12);
+    r.path_off = seg(8 * r.low);
     std::vector<const uint32_t*> cp, tp;
     for (int b = 0; b < B_; ++b) {
       for (int l = 0; l < 6; ++l) cp.push_back(lay.ptr(b) + (size_t)l * lay.len);
       tp.push_back(tb.node_ptr(b));
     }
     add_op(0, j, 6, 0, 0, r.rows_off, cp);
-    add_op(1, j, r.depth, 0, tb.nleaves, r.path_off, tp);
+    add_op(1, j, r.low, 0, tb.nleaves, r.path_off, tp);
   }
   std::vector<size_t> lq_off(L);
   for (size_t l = 1; l < L; ++l) {
@@ -1941,11 +1962,21 @@ void Job::open() {
   }
   stride = (stride + 3) & ~size_t(3);
   const size_t gwords = (size_t)B_ * nq * stride;
-  DevBuf gat = dev(gwords);
+  const size_t twords = (size_t)ntab * B_ * TAB_WORDS;
+  DevBuf gat = dev(gwords + twords);
   DevBuf pos_dev = dev((size_t)B_ * nq);
   const GatherOp* ops_d = stage(ops);
+  std::vector<uint32_t*> tab_dst(tab_src.size());
+  for (size_t i = 0; i < tab_src.size(); ++i) tab_dst[i] = gat.as<uint32_t>() + gwords + i * TAB_WORDS;
+  const uint32_t* const* tab_src_d = tab_src.empty() ? nullptr : stage(tab_src);
+  uint32_t* const* tab_dst_d = tab_src.empty() ? nullptr : const_cast<uint32_t* const*>(stage(tab_dst));
   stg_->flush(s_);  // the chunk launches below stage nothing
-  uint32_t* gh = ctx_.pinned(gwords * 4, lane_);
+  uint32_t* gh = ctx_.pinned((gwords + twords) * 4, lane_);
+  const uint32_t* tabs_h = gh + gwords;  // [table][b][TAB_WORDS]
+  if (twords) {
+    launch_copy_cols(tab_src_d, tab_dst_d, (int)TAB_WORDS, (int)tab_src.size(), s_);
+    LG_HIP(hipMemcpyAsync(gh + gwords, gat.as<uint32_t>() + gwords, twords * 4, hipMemcpyDeviceToHost, s_));
+  }
   uint32_t* pos = ctx_.pinned_in((size_t)B_ * nq * 4, lane_);  // the inputs' upload finished long ago
   struct Events {
     std::vector<hipEvent_t> e;
@@ -1993,6 +2024,9 @@ void Job::open() {
     }
     ready_cv.notify_all();
   };
+  // host timestamps of the phase (printed with LOOMGPU_TRACE)
+  const double q_t0 = now_s();
+  std::vector<double> q_chain(B_), q_back(B_), q_done(B_);
   auto query_task = [&](int b) {
     try {
       std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
@@ -2006,6 +2040,7 @@ void Job::open() {
         pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
         if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);
       }
+      q_chain[b] = now_s();
       if (--left[b / chunk] == 0) {
         launch_chunk(b / chunk);
         mark(b / chunk, true);
@@ -2023,9 +2058,22 @@ void Job::open() {
      if (!ready[b / chunk]) return;  // a query task failed; its error is rethrown
    }
    LG_HIP(hipEventSynchronize(ev.e[b / chunk]));
+   q_back[b] = now_s();
    // The words go straight to the caller's buffer when it gave one
    // (SpanWriter counts past its capacity so the caller learns the length),
    // else to out_[b].proof.
+   // an authentication path: the gathered lower siblings, then the top
+   // levels from the tree's table (heap index (nleaves + leaf) >> e, ^ 1)
+   auto put_path = [&](auto& w, const uint32_t* low_rec, int depth, int low, int tab, uint32_t leaf,
+                       uint32_t nleaves) {
+    w.insert(w.end(), low_rec, low_rec + 8 * low);
+    if (tab < 0) return;
+    const uint32_t* tt = tabs_h + ((size_t)tab * B_ + b) * TAB_WORDS;
+    for (int e = low; e < depth; ++e) {
+      const uint32_t node = ((nleaves + leaf) >> e) ^ 1u;
+      w.insert(w.end(), tt + 8 * (size_t)node, tt + 8 * (size_t)node + 8);
+    }
+   };
    auto emit = [&](auto& w) {
     const std::string hid = "poseidon2";
     w.push_back((uint32_t)hid.size());
@@ -2105,7 +2153,8 @@ void Job::open() {
         for (int x = 0; x < 6; ++x) w.push_back(rec[frec[j].rows_off + 6 + x]);
         w.push_back((row & ~1u) >> 1);
         w.push_back((uint32_t)frec[j].depth);
-        w.insert(w.end(), rec + frec[j].path_off, rec + frec[j].path_off + 8 * frec[j].depth);
+        put_path(w, rec + frec[j].path_off, frec[j].depth, frec[j].low, frec[j].tab, row >> 1,
+                 (uint32_t)frec[j].nleaves);
         w.push_back(0);  // layer r0's injection leaves are dropped (fri.go:719); others have none
       }
     }
@@ -2131,7 +2180,7 @@ void Job::open() {
         rawrow(top, rec + r.rows_off, r.ncols_top, 1);
         w.push_back(((s >> r.shift) & ~1u) >> 1);
         w.push_back((uint32_t)r.depth);
-        w.insert(w.end(), rec + r.path_off, rec + r.path_off + 8 * r.depth);
+        put_path(w, rec + r.path_off, r.depth, r.low, r.tab, (s >> r.shift) >> 1, (uint32_t)tb.nleaves);
         w.push_back((uint32_t)tb.inj.size());
         w.insert(w.end(), rec + r.injl_off, rec + r.injl_off + 8 * tb.inj.size());
         w.push_back((uint32_t)tb.inj.size());
@@ -2156,6 +2205,7 @@ void Job::open() {
       out_[b].proof_len = w.size();
     }
     out_[b].transcript = ts_[b].log_words();
+    q_done[b] = now_s();
   };
   // Query chains first (tasks are handed out in index order), then the
   // assemblies, which start as soon as threads free up and their chunk is
@@ -2164,6 +2214,11 @@ void Job::open() {
     if (t < B_) query_task(t);
     else assemble_task(t - B_);
   });
+  if (getenv("LOOMGPU_TRACE")) {
+    auto last = [&](const std::vector<double>& v) { return (*std::max_element(v.begin(), v.end()) - q_t0) * 1e3; };
+    fprintf(stderr, "[loomgpu] B=%d queries: chains %.3f ms, records back %.3f ms, assembled %.3f ms, %zu words/proof, %zu record words\n",
+            B_, last(q_chain), last(q_back), last(q_done), out_[0].proof_len, gwords);
+  }
   trace("queries, gathers, proof assembly");
   for (auto& x : W) keep_.push_back(std::move(x));
   for (auto& x : level) keep_.push_back(std::move(x.buf));

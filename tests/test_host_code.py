@@ -1,6 +1,7 @@
 """The device field/Poseidon2 headers built for the host with g++ (CPU):
 specialised divisions by 2^k equal Montgomery products, and the width-24
-permutation equals the oracle's on random states."""
+permutation (its host form: lazy 64-bit partial rounds) equals the oracle's
+on random and extreme states."""
 import os
 import shutil
 import subprocess
@@ -30,9 +31,16 @@ def test_div_pow2_and_permutation_match_oracle(p2_bin):
     from oracle import poseidon2 as p2
     rng = np.random.default_rng(5)
     P = 2**31 - 2**24 + 1
-    states = rng.integers(0, P, size=(40, 24), dtype=np.uint64)
+    # the host permutation carries the partial rounds in lazy 64-bit form:
+    # extreme and alternating states as well as random ones
+    states = rng.integers(0, P, size=(200, 24), dtype=np.uint64)
     states[0] = 0
     states[1] = P - 1
+    states[2, 0::2] = P - 1
+    states[2, 1::2] = 0
+    states[3, 0::2] = 0
+    states[3, 1::2] = P - 1
+    states[4] = np.arange(24) * (P // 24)
     inp = "%d\n" % len(states) + "\n".join(" ".join(str(int(v)) for v in st) for st in states) + "\n"
     r = subprocess.run([p2_bin], input=inp, capture_output=True, text=True)
     lines = r.stdout.strip().splitlines()
