@@ -336,6 +336,39 @@ void Context::give_staging(std::unique_ptr<Staging> s) {
 }
 
 // ------------------------------------------------------------------ plan
+// The modules whose claimed values form a DEEP class (fri.go:427-437): those
+// with at least one claimed value in some tree group, in module order.  Each
+// class is one FRI level.
+static std::vector<size_t> deep_classes(const PlanData& d) {
+  std::vector<char> has(d.modules.size(), 0);
+  for (auto& t : d.trees)
+    for (auto& g : t.groups) {
+      bool any = false;
+      for (auto& sh : g.base_shifts) any = any || !sh.empty();
+      for (auto& sh : g.ext_shifts) any = any || !sh.empty();
+      if (any) has.at(g.module) = 1;
+    }
+  std::vector<size_t> r;
+  for (size_t m = 0; m < has.size(); ++m)
+    if (has[m]) r.push_back(m);
+  return r;
+}
+
+// Registration order of the challenges: the round challenges and __zeta
+// (prover/prover.go:208-212), alpha_DEEP (internal/fri/open.go:151), the
+// level gammas, the fold rounds and the queries (internal/fri/fri.go:425-436).
+static std::shared_ptr<const ChallengeSchedule> challenge_schedule(const PlanData& d) {
+  auto s = std::make_shared<ChallengeSchedule>();
+  for (size_t r = 0; r < d.rounds.size(); ++r) s->add(challenge_name(r));
+  s->add("__zeta");
+  s->add("alpha_DEEP");
+  const size_t L = deep_classes(d).size();
+  for (size_t l = 1; l < L; ++l) s->add("fri_level_" + std::to_string(l) + "_gamma");
+  for (int j = 0; j < FRI_ROUNDS; ++j) s->add("fri_fold_" + std::to_string(j));
+  for (uint32_t k = 0; k < d.num_queries; ++k) s->add("fri_query_" + std::to_string(k));
+  return s;
+}
+
 Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_plan(words, n)) {
   for (auto& m : d_.modules) {
     if (m.log_n > FRI_LOG_D) throw std::runtime_error("module size above 2^22 (MAX_LOG_MODULE_SIZE)");
@@ -344,6 +377,7 @@ Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_
   }
   for (auto& pm : d_.perms)
     if (pm.len / d_.modules[pm.module].N > 64) throw std::runtime_error("permutation with more than 64 chunks");
+  sched_ = challenge_schedule(d_);
   LG_HIP(hipSetDevice(ctx->device));
   rel_.resize(d_.modules.size());
   for (size_t m = 0; m < d_.modules.size(); ++m)
@@ -1283,11 +1317,9 @@ void Job::run() {
   roots_.resize(d_.trees.size());
   chal_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
   totals_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
-  ts_.assign(B_, Transcript());
+  ts_.assign(B_, Transcript(plan_.schedule()));
   const size_t nr = d_.rounds.size();
   for (int b = 0; b < B_; ++b) {
-    for (size_t r = 0; r < nr; ++r) ts_[b].new_challenge(challenge_name(r));
-    ts_[b].new_challenge("__zeta");
     const std::string init = nr ? challenge_name(0) : "__zeta";
     ts_[b].bind(init, string_to_elements(TAG_HASH, "poseidon2"));
     std::vector<uint32_t> sizes{M(TAG_SIZE), M((uint32_t)d_.modules.size())};
@@ -1413,13 +1445,7 @@ void Job::open() {
   for (size_t mi = 0; mi < nm; ++mi)
     for (size_t k = 0; k < ncv; ++k)
       if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi) bind_order.push_back((int32_t)k);
-  std::vector<size_t> classes;
-  for (size_t mi = 0; mi < nm; ++mi)
-    for (size_t k = 0; k < ncv; ++k)
-      if (d_.trees[cvs[k].t].groups[cvs[k].g].module == mi) {
-        classes.push_back(mi);
-        break;
-      }
+  const std::vector<size_t> classes = deep_classes(d_);
   std::vector<size_t> lv_order(classes.size());
   for (size_t l = 0; l < lv_order.size(); ++l) lv_order[l] = l;
   std::stable_sort(lv_order.begin(), lv_order.end(), [&](size_t a, size_t b) {
@@ -1434,12 +1460,6 @@ void Job::open() {
   const bool deep_on_device = L == 1;
   std::vector<E6> cvals((size_t)B_ * ncv);
   std::vector<E6> alpha(B_);
-  parallel_for(B_, [&](int b) {
-    ts_[b].new_challenge("alpha_DEEP");
-    for (size_t l = 1; l < L; ++l) ts_[b].new_challenge("fri_level_" + std::to_string(l) + "_gamma");
-    for (int j = 0; j < FRI_ROUNDS; ++j) ts_[b].new_challenge("fri_fold_" + std::to_string(j));
-    for (int k = 0; k < nq; ++k) ts_[b].new_challenge("fri_query_" + std::to_string(k));
-  });
   if (!deep_on_device) {
     LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
     sync();
@@ -2007,10 +2027,6 @@ void Job::open() {
     LG_HIP(hipMemcpyAsync(gh + w0, gat.as<uint32_t>() + w0, (size_t)nb * nq * stride * 4, hipMemcpyDeviceToHost, s_));
     LG_HIP(hipEventRecord(ev.e[c], s_));
   };
-  // the query challenges are registered consecutively; their name prefixes
-  // are the same for every instance
-  std::vector<std::vector<uint32_t>> qpre(nq);
-  for (int k = 0; k < nq; ++k) qpre[k] = string_to_elements(TAG_FSID, "fri_query_" + std::to_string(k));
   // A chunk's records may be awaited only once its event has been recorded.
   std::mutex ready_mu;
   std::condition_variable ready_cv;
@@ -2035,7 +2051,7 @@ void Job::open() {
       ts_[b].bind("fri_query_0", fp);
       const size_t p0 = ts_[b].index("fri_query_0");
       for (int k = 0; k < nq; ++k) {
-        const Digest& dg = ts_[b].compute_at(p0 + k, qpre[k]);
+        const Digest& dg = ts_[b].compute_at(p0 + k);
         uint64_t c0 = kb_from_mont(dg.v[0]), c1 = kb_from_mont(dg.v[1]);
         pos[(size_t)b * nq + k] = (uint32_t)(((c0 << 31) ^ c1) % (1ull << FRI_LOG_N));
         if (k < nq - 1) ts_[b].bind_at(p0 + k + 1, dg.v, 8);

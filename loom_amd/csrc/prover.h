@@ -178,6 +178,8 @@ struct ProveOutput {
 // Prover-side timing of each phase (wall clock, host), for the phase callback.
 using PhaseCallback = std::function<void(const char*, double)>;
 
+struct ChallengeSchedule;
+
 class Plan {
  public:
   Plan(Context* ctx, const uint32_t* words, size_t n);
@@ -191,12 +193,15 @@ class Plan {
 
   std::shared_ptr<DagKernel> relation_kernel(size_t m) const { return rel_[m]; }
   std::shared_ptr<DagKernel> step_kernel(size_t r, size_t s, size_t k) const { return steps_.at({r, s, k}); }
+  // the Fiat-Shamir challenges of one proof of this plan, in order
+  std::shared_ptr<const ChallengeSchedule> schedule() const { return sched_; }
 
  private:
   Context* ctx_;
   PlanData d_;
   std::vector<std::shared_ptr<DagKernel>> rel_;
   std::map<std::tuple<size_t, size_t, size_t>, std::shared_ptr<DagKernel>> steps_;
+  std::shared_ptr<const ChallengeSchedule> sched_;
 };
 
 }  // namespace lg

@@ -6,6 +6,7 @@
 #pragma once
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -33,12 +34,34 @@ struct Digest {
   uint32_t v[8];
 };
 
+// Challenge names in registration order with their message prefixes
+// (string_to_elements(FSID, name)).  A prover builds the schedule of a proof
+// shape once and shares it between the transcripts of all its proofs.
+struct ChallengeSchedule {
+  std::vector<std::string> names;
+  std::map<std::string, size_t> pos;
+  std::vector<std::vector<uint32_t>> prefix;
+  void add(const std::string& name) {
+    if (pos.count(name)) throw std::runtime_error("transcript: challenge already exists: " + name);
+    pos[name] = names.size();
+    names.push_back(name);
+    prefix.push_back(string_to_elements(TAG_FSID, name));
+  }
+};
+
 class Transcript {
  public:
+  Transcript() = default;
+  // A transcript with the schedule's challenges registered, in its order.
+  explicit Transcript(std::shared_ptr<const ChallengeSchedule> s) : fixed_(std::move(s)) {
+    const size_t n = fixed_->names.size();
+    bindings_.resize(n);
+    values_.resize(n);
+    computed_.assign(n, false);
+  }
   void new_challenge(const std::string& name) {
-    if (pos_.count(name)) throw std::runtime_error("transcript: challenge already exists: " + name);
-    pos_[name] = names_.size();
-    names_.push_back(name);
+    if (fixed_) throw std::runtime_error("transcript: the challenge schedule is fixed: " + name);
+    own_.add(name);
     bindings_.emplace_back();
     values_.emplace_back();
     computed_.push_back(false);
@@ -52,7 +75,7 @@ class Transcript {
   const Digest& compute(const std::string& name) {
     size_t p = at(name);
     if (computed_[p]) return values_[p];
-    std::vector<uint32_t> msg = string_to_elements(TAG_FSID, name);
+    std::vector<uint32_t> msg = sch().prefix[p];
     if (p != 0) {
       if (!computed_[p - 1]) throw std::runtime_error("transcript: previous challenge not computed: " + name);
       msg.insert(msg.end(), values_[p - 1].v, values_[p - 1].v + 8);
@@ -64,16 +87,18 @@ class Transcript {
     return values_[p];
   }
   // Position-based forms for long runs of consecutive challenges (the FRI
-  // queries): `prefix` must be string_to_elements(TAG_FSID, name at p).
+  // queries).
   size_t index(const std::string& name) const { return at(name); }
   void bind_at(size_t p, const uint32_t* e, size_t n) {
-    if (p >= names_.size() || computed_[p]) throw std::runtime_error("transcript: bad bind position");
+    if (p >= computed_.size() || computed_[p]) throw std::runtime_error("transcript: bad bind position");
     bindings_[p].insert(bindings_[p].end(), e, e + n);
   }
-  const Digest& compute_at(size_t p, const std::vector<uint32_t>& prefix) {
-    if (p >= names_.size()) throw std::runtime_error("transcript: bad challenge position");
+  const Digest& compute_at(size_t p) {
+    if (p >= computed_.size()) throw std::runtime_error("transcript: bad challenge position");
     if (computed_[p]) return values_[p];
-    if (p != 0 && !computed_[p - 1]) throw std::runtime_error("transcript: previous challenge not computed: " + names_[p]);
+    const std::vector<uint32_t>& prefix = sch().prefix[p];
+    if (p != 0 && !computed_[p - 1])
+      throw std::runtime_error("transcript: previous challenge not computed: " + sch().names[p]);
     uint32_t small[64];
     const size_t len = prefix.size() + (p != 0 ? 8 : 0) + bindings_[p].size();
     std::vector<uint32_t> big;
@@ -118,18 +143,20 @@ class Transcript {
   }
   std::vector<std::string> log_names() const {
     std::vector<std::string> r;
-    for (size_t p : log_) r.push_back(names_[p]);
+    for (size_t p : log_) r.push_back(sch().names[p]);
     return r;
   }
 
  private:
+  const ChallengeSchedule& sch() const { return fixed_ ? *fixed_ : own_; }
   size_t at(const std::string& name) const {
-    auto it = pos_.find(name);
-    if (it == pos_.end()) throw std::runtime_error("transcript: unknown challenge " + name);
+    const auto& pos = sch().pos;
+    auto it = pos.find(name);
+    if (it == pos.end()) throw std::runtime_error("transcript: unknown challenge " + name);
     return it->second;
   }
-  std::vector<std::string> names_;
-  std::map<std::string, size_t> pos_;
+  ChallengeSchedule own_;                           // challenges registered one by one
+  std::shared_ptr<const ChallengeSchedule> fixed_;  // or a shared schedule
   std::vector<std::vector<uint32_t>> bindings_;
   std::vector<Digest> values_;
   std::vector<bool> computed_;
