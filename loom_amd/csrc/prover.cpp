@@ -523,6 +523,13 @@ class Job {
     return stage(v);
   }
   void sync() { LG_HIP(hipStreamSynchronize(s_)); }
+  // host timestamps of named points, printed at the end with LOOMGPU_MARKS
+  void mark(const char* what) {
+    static const bool on = getenv("LOOMGPU_MARKS") != nullptr;
+    if (on) marks_.push_back({what, now_s()});
+  }
+  std::vector<std::pair<std::string, double>> marks_;
+  double t0_run_ = 0;
   void phase(const char* name, double& t0) {
     if (cb_) {
       sync();
@@ -933,7 +940,9 @@ std::vector<Digest> Job::roots_of(TreeBuf& tb) {
   launch_copy_roots(tp_d, r.as<uint32_t>(), B_, s_);
   std::vector<Digest> out(B_);
   LG_HIP(hipMemcpyAsync(out.data(), r.as<void>(), (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
+  mark("roots enqueued");
   sync();
+  mark("roots synced");
   keep_.push_back(std::move(r));
   return out;
 }
@@ -1309,6 +1318,7 @@ void Job::quotient(size_t mi) {
 
 void Job::run() {
   double t0 = now_s();
+  t0_run_ = t0;
   LG_HIP(hipSetDevice(ctx_.device));
   alloc_columns();
   upload_inputs();
@@ -1332,6 +1342,7 @@ void Job::run() {
   for (size_t r = 0; r < nr; ++r) {
     const Round& rd = d_.rounds[r];
     run_steps(r);
+    mark("steps enqueued");
     trace(("steps " + std::to_string(r)).c_str());
     if (rd.tree != PLAN_NONE) {
       commit_tree(rd.tree);
@@ -1339,12 +1350,14 @@ void Job::run() {
       for (int b = 0; b < B_; ++b) ts_[b].bind(challenge_name(r), roots_[rd.tree][b].v, 8);
     }
     finish_round_steps(r);
+    mark("round finished");
     parallel_for(B_, [&](int b) {
       const Digest& dg = ts_[b].compute(challenge_name(r));
       E6 v;
       for (int l = 0; l < 6; ++l) v.v[l] = dg.v[l];
       chal_[b][rd.challenge_col] = v;
     });
+    mark("challenge");
     trace(("round " + std::to_string(r)).c_str());
   }
   phase("execute-steps", t0);
@@ -1362,10 +1375,20 @@ void Job::run() {
     const Digest& dg = ts_[b].compute("__zeta");
     for (int l = 0; l < 6; ++l) zeta_[b].v[l] = dg.v[l];
   });
+  mark("zeta");
   trace("air commit + zeta");
   phase("compute-air-quotients", t0);
   open();
   phase("pcs-open", t0);
+  if (!marks_.empty() && lane_ == 0) {
+    std::string line = "[loomgpu] marks (ms):";
+    char buf[96];
+    for (auto& m : marks_) {
+      snprintf(buf, sizeof buf, " %s %.3f;", m.first.c_str(), (m.second - t0_run_) * 1e3);
+      line += buf;
+    }
+    fprintf(stderr, "%s\n", line.c_str());
+  }
 }
 
 // PCS.Open (fri/open.go:98-215) + multi-degree FRI (fri/fri.go:352-736) +
