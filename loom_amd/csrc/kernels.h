@@ -191,6 +191,32 @@ struct DeepAlphaArgs {
 };
 void launch_deep_alpha(const DeepAlphaArgs& a, const uint32_t* prefix, int n_prefix, int batch, hipStream_t s);
 
+// Fiat-Shamir challenges on the device (transcript.go:285-315): the sponge
+// of a message given as segments; word i of segment k for instance b is
+// seg[k].p[b * seg[k].stride + i] (stride 0: the same words for every
+// instance).  Writes dig[8 b ..] and, if e6, the first six words to
+// e6[b * e6_stride ..].
+struct FsSeg {
+  const uint32_t* p;
+  uint32_t len;
+  uint32_t stride;
+};
+// seg_dev: nseg segments in device memory; len: their total length.
+void launch_fs_challenge(const FsSeg* seg_dev, int nseg, int len, uint32_t* dig, uint32_t* e6, int e6_stride,
+                         int batch, hipStream_t s);
+// out[b][k] = chal[b][ids[k]] for the nch E6 challenge operands of a DAG.
+void launch_chal_compact(const uint32_t* chal, int ncols, const uint32_t* ids, int nch, uint32_t* out, int batch,
+                         hipStream_t s);
+// chal[b][col] = tot[b] * s_mont (E6 times a base-field element).
+void launch_scale_into(const uint32_t* tot, uint32_t s_mont, uint32_t* chal, int ncols, int col, int batch,
+                       hipStream_t s);
+// z[b] = zeta_b and cm[j][b] = (zeta_b^sizes[j] - 1) * ninv[j], zeta_b the
+// first six words of zdig[8 b ..].
+void launch_zeta_consts(const uint32_t* zdig, const uint32_t* sizes, const uint32_t* ninv, int nmod, uint32_t* z,
+                        uint32_t* cm, int batch, hipStream_t s);
+// out[b][p] = zeta_b * w[p].
+void launch_zeta_points(const uint32_t* zdig, const uint32_t* w, int np, uint32_t* out, int batch, hipStream_t s);
+
 // FRI fold (fri.go:1177-1198): next[i] = (p+q)/2 + alpha*(p-q)/2*xinv[i],
 // p = layer[2i], q = layer[2i+1]; E6 layers as 6 limb arrays.
 // FRI fold challenge per instance on the device: digest of the sponge over

@@ -1530,6 +1530,130 @@ void launch_deep_alpha(const DeepAlphaArgs& a, const uint32_t* prefix, int n_pre
   LG_HIP(hipGetLastError());
 }
 
+// Fiat-Shamir challenge per instance from a message of segments (transcript.go:
+// 285-315): the sponge runs on a 16-lane row (four instances per 64-thread
+// block), word i of segment k of instance b being seg[k].p[b stride + i].
+// A lane's word indexes only grow, so each of its two word streams keeps a
+// segment cursor.
+struct FsCursor {
+  int k = 0, base = 0;  // current segment and its first word index
+  __device__ uint32_t word(const FsSeg* __restrict__ seg, int nseg, int b, int i) {
+    while (k < nseg && i >= base + (int)seg[k].len) {
+      base += (int)seg[k].len;
+      ++k;
+    }
+    if (k >= nseg) return 0u;
+    return seg[k].p[(size_t)b * seg[k].stride + (i - base)];
+  }
+};
+
+__global__ __launch_bounds__(64) void k_fs_challenge(const FsSeg* __restrict__ seg, int nseg, int len, uint32_t* dig,
+                                                     uint32_t* e6, int e6_stride, int batch) {
+  const int l = threadIdx.x & 15, row = threadIdx.x >> 4;
+  const int b_raw = blockIdx.x * 4 + row;
+  const bool live = b_raw < batch;
+  const int b = live ? b_raw : batch - 1;
+  FsCursor ca, cb;
+  P2Coop16 P;
+  P.init(l);
+  uint32_t x = 0u, y = 0u;
+  for (int off = 0; off < len; off += 16) {
+    const int k = len - off < 16 ? len - off : 16;
+    if (l < 12 && l < k) x = ca.word(seg, nseg, b, off + l);
+    if (l < 12 && l + 12 < k) y = cb.word(seg, nseg, b, off + l + 12);
+    P.permute(x, y);
+  }
+  if (!live) return;
+  if (l < 8) dig[8 * b + l] = x;
+  if (e6 && l < 6) e6[(size_t)b * e6_stride + l] = x;
+}
+
+void launch_fs_challenge(const FsSeg* seg_dev, int nseg, int len, uint32_t* dig, uint32_t* e6, int e6_stride,
+                         int batch, hipStream_t s) {
+  if (nseg < 1 || len < 1) throw std::runtime_error("fs challenge: empty message");
+  prelaunch();
+  hipLaunchKernelGGL(k_fs_challenge, dim3((batch + 3) / 4), dim3(64), 0, s, seg_dev, nseg, len, dig, e6, e6_stride,
+                     batch);
+  LG_HIP(hipGetLastError());
+}
+
+// out[b][k] = chal[b][ids[k]] (E6 slots): the challenge operands of one DAG.
+__global__ void k_chal_compact(const uint32_t* __restrict__ chal, int ncols, const uint32_t* __restrict__ ids, int nch,
+                               uint32_t* __restrict__ out) {
+  const int b = blockIdx.x;
+  for (int i = threadIdx.x; i < nch * 6; i += blockDim.x)
+    out[((size_t)b * nch) * 6 + i] = chal[((size_t)b * ncols + ids[i / 6]) * 6 + i % 6];
+}
+
+void launch_chal_compact(const uint32_t* chal, int ncols, const uint32_t* ids, int nch, uint32_t* out, int batch,
+                         hipStream_t s) {
+  if (nch == 0) return;
+  prelaunch();
+  hipLaunchKernelGGL(k_chal_compact, dim3(batch), dim3(64), 0, s, chal, ncols, ids, nch, out);
+  LG_HIP(hipGetLastError());
+}
+
+// chal[b][col] = tot[b] * s (an E6 times a base-field scalar).
+__global__ void k_scale_into(const uint32_t* __restrict__ tot, uint32_t s, uint32_t* __restrict__ chal, int ncols,
+                             int col, int batch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch * 6) return;
+  const int b = i / 6, l = i % 6;
+  chal[((size_t)b * ncols + col) * 6 + l] = kb_mul(tot[i], s);
+}
+
+void launch_scale_into(const uint32_t* tot, uint32_t s_mont, uint32_t* chal, int ncols, int col, int batch,
+                       hipStream_t s) {
+  prelaunch();
+  hipLaunchKernelGGL(k_scale_into, dim3((batch * 6 + 255) / 256), dim3(256), 0, s, tot, s_mont, chal, ncols, col,
+                     batch);
+  LG_HIP(hipGetLastError());
+}
+
+// From zeta (first six words of zdig[b]): z[b] = zeta and, per module j,
+// cm[j][b] = (zeta^N_j - 1) / N_j (the Lagrange-weight constant).
+__global__ void k_zeta_consts(const uint32_t* __restrict__ zdig, const uint32_t* __restrict__ sizes,
+                              const uint32_t* __restrict__ ninv, int nmod, uint32_t* __restrict__ z,
+                              uint32_t* __restrict__ cm, int batch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch * nmod) return;
+  const int b = i % batch, j = i / batch;
+  E6 zeta;
+  for (int l = 0; l < 6; ++l) zeta.v[l] = zdig[8 * b + l];
+  if (j == 0)
+    for (int l = 0; l < 6; ++l) z[6 * b + l] = zeta.v[l];
+  const E6 c = e6_scale(e6_sub(e6_pow(zeta, sizes[j]), e6_one()), ninv[j]);
+  for (int l = 0; l < 6; ++l) cm[((size_t)j * batch + b) * 6 + l] = c.v[l];
+}
+
+void launch_zeta_consts(const uint32_t* zdig, const uint32_t* sizes, const uint32_t* ninv, int nmod, uint32_t* z,
+                        uint32_t* cm, int batch, hipStream_t s) {
+  if (nmod == 0) return;
+  prelaunch();
+  hipLaunchKernelGGL(k_zeta_consts, dim3((batch * nmod + 63) / 64), dim3(64), 0, s, zdig, sizes, ninv, nmod, z, cm,
+                     batch);
+  LG_HIP(hipGetLastError());
+}
+
+// out[b][p] = zeta_b * w[p] (the DEEP points zeta * omega^shift).
+__global__ void k_zeta_points(const uint32_t* __restrict__ zdig, const uint32_t* __restrict__ w, int np,
+                              uint32_t* __restrict__ out, int batch) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= batch * np) return;
+  const int b = i / np, p = i % np;
+  E6 zeta;
+  for (int l = 0; l < 6; ++l) zeta.v[l] = zdig[8 * b + l];
+  const E6 r = e6_scale(zeta, w[p]);
+  for (int l = 0; l < 6; ++l) out[(size_t)i * 6 + l] = r.v[l];
+}
+
+void launch_zeta_points(const uint32_t* zdig, const uint32_t* w, int np, uint32_t* out, int batch, hipStream_t s) {
+  if (np == 0) return;
+  prelaunch();
+  hipLaunchKernelGGL(k_zeta_points, dim3((batch * np + 63) / 64), dim3(64), 0, s, zdig, w, np, out, batch);
+  LG_HIP(hipGetLastError());
+}
+
 void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* prev, uint32_t* const* trees,
                           uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch, hipStream_t s) {
   if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("fri challenge: bad prefix length");

@@ -307,11 +307,17 @@ uint32_t* Context::pinned_aux(size_t bytes, int lane) {
   return grow_pinned(L.pinned_aux, L.pinned_aux_cap, bytes);
 }
 
+uint32_t* Context::pinned_fs(size_t bytes, int lane) {
+  LaneBufs& L = lane_.at(lane);
+  return grow_pinned(L.pinned_fs, L.pinned_fs_cap, bytes);
+}
+
 Context::~Context() {
   for (auto& L : lane_) {
     if (L.pinned) (void)hipHostFree(L.pinned);
     if (L.pinned_in) (void)hipHostFree(L.pinned_in);
     if (L.pinned_aux) (void)hipHostFree(L.pinned_aux);
+    if (L.pinned_fs) (void)hipHostFree(L.pinned_fs);
     if (L.stream) {
       (void)hipStreamSynchronize(L.stream);
       (void)hipStreamDestroy(L.stream);
@@ -502,6 +508,11 @@ class Job {
   void flush_staging() { stg_->flush(s_); }
   ~Job() {
     set_prelaunch_hook(nullptr, nullptr);
+    // after an error work may still be queued on the stream: drain it before
+    // the buffers go back to the shared pool
+    (void)hipStreamSynchronize(s_);
+    ctx_.pool.put(std::move(chal_dev_));
+    ctx_.pool.put(std::move(fs_dig_));
     for (auto& c : cols_) ctx_.pool.put(std::move(c.buf));
     for (auto& t : trees_) {
       ctx_.pool.put(std::move(t.nodes));
@@ -559,6 +570,13 @@ class Job {
   void step_multiplicity(size_t r, size_t s);
   void step_logup(size_t r, size_t s);
   void finish_round_steps(size_t r);
+  // Round and __zeta challenges on the device (k_fs_challenge); the host
+  // replays them into its transcripts at its first synchronisation after
+  // the claimed values (replay_device_fs).
+  void round_challenge(size_t r);
+  void zeta_challenge();
+  void replay_device_fs();
+  const uint32_t* chal_operands(const std::vector<uint32_t>& chal_ids);
   void lde_columns(const std::vector<uint32_t>& col_ids);
   // Runs the transforms cfg describes on every job: one LDS-resident launch
   // up to 2^14, the two-pass four-step form above.
@@ -596,10 +614,21 @@ class Job {
   std::vector<TreeBuf> trees_;
   std::vector<DevBuf> keep_;
   std::vector<Transcript> ts_;
-  std::vector<std::vector<E6>> chal_;   // [b][col] challenge values (and logup averages T/N)
-  std::vector<std::vector<E6>> totals_;  // [b][col] exposed logup totals
+  std::vector<std::vector<E6>> totals_;  // [b][col] exposed logup totals (host, after the replay)
   std::vector<std::pair<uint32_t, DevBuf>> totals_dev_;  // (total column, B x 6 words) of the current round
   std::vector<DevBuf> lookup_err_;                       // B words per multiplicity step of the current round
+  // Device transcript data.  chal_dev_: [b][col][6] challenge values and
+  // logup averages T/N (the DAG kernels' per-instance constants); fs_dig_:
+  // [pos][b][8] digests of the round challenges (pos = round) and __zeta
+  // (pos = rounds); fs_h_: pinned copies of the digests, the tree roots, the
+  // logup totals and the lookup flags, read by replay_device_fs.
+  DevBuf chal_dev_, fs_dig_;
+  uint32_t* fs_h_ = nullptr;
+  size_t fs_roots_off_ = 0, fs_tot_off_ = 0, fs_err_off_ = 0;
+  int n_err_ = 0;
+  std::vector<uint32_t> fs_init_;                    // bindings of the first challenge (hash id, sizes)
+  std::map<uint32_t, const uint32_t*> total_ptr_;    // logup total column -> device B x 6 words
+  std::vector<uint32_t> total_cols_;
   std::vector<std::vector<Digest>> roots_;  // [tree][b]
   std::vector<E6> zeta_;
 };
@@ -620,7 +649,7 @@ void Job::alloc_columns() {
     cb.limbs = ci.limbs();
     cb.n = d_.modules[ci.module].N;
     cb.committed = committed[c] != 0;
-    if (ci.src == SRC_CHALLENGE || is_total[c]) continue;  // per-instance constants, kept in chal_
+    if (ci.src == SRC_CHALLENGE || is_total[c]) continue;  // per-instance constants, kept in chal_dev_
     cb.blk = (size_t)cb.limbs * cb.n * (cb.committed ? 5 : 1);
     cb.buf = dev(cb.blk * B_);
   }
@@ -990,7 +1019,14 @@ void Job::commit_tree(size_t t) {
     rows.push_back((int)g.N * 4);
   }
   build_tree(trees_[t], gcols, shape, rows);
-  roots_[t] = roots_of(trees_[t]);
+  {
+    // the roots come back with the transcript data (replay_device_fs)
+    DevBuf r = dev((size_t)B_ * 8);
+    launch_copy_roots(trees_[t].nodes_d, r.as<uint32_t>(), B_, s_);
+    LG_HIP(hipMemcpyAsync(fs_h_ + fs_roots_off_ + t * (size_t)B_ * 8, r.as<void>(), (size_t)B_ * 32,
+                          hipMemcpyDeviceToHost, s_));
+    keep_.push_back(std::move(r));
+  }
   debug_dump(("tree" + std::to_string(t)).c_str(), ids, cols_, s_);
 }
 
@@ -1003,16 +1039,13 @@ Layer Job::eval_step_dag(size_t r, size_t s, size_t k, uint32_t mi) {
   out.len = m.N;
   out.buf = dev((size_t)B_ * 6 * m.N);
   std::vector<const uint32_t*> cp;
-  std::vector<E6> ch;
-  for (int b = 0; b < B_; ++b) {
+  for (int b = 0; b < B_; ++b)
     for (auto c : K.col_ids) cp.push_back(cols_[c].lag(b, 0));
-    for (auto c : K.chal_ids) ch.push_back(chal_[b][c]);
-  }
   std::vector<uint32_t*> op(B_);
   for (int b = 0; b < B_; ++b) op[b] = out.ptr(b);
   DagLaunch a{};
   a.cols = stage(cp);
-  a.chal = reinterpret_cast<const uint32_t*>(stg_->put(ch.data(), ch.size() * sizeof(E6), s_));
+  a.chal = chal_operands(K.chal_ids);
   a.rowscale = nullptr;
   a.out = const_cast<uint32_t* const*>(stage(op));
   a.col_stride = m.N;
@@ -1137,44 +1170,133 @@ void Job::step_logup(size_t r, size_t s) {
 // round (exposed values, and the constant column T/N the relations read).
 void Job::finish_round_steps(size_t r) {
   for (auto& e : lookup_err_) {
-    std::vector<uint32_t> h(B_);
-    LG_HIP(hipMemcpyAsync(h.data(), e.as<void>(), (size_t)B_ * 4, hipMemcpyDeviceToHost, s_));
-    sync();
-    for (int b = 0; b < B_; ++b)
-      if (h[b]) throw std::runtime_error("lookup: a source value is not in the range table (instance " +
-                                         std::to_string(b) + ")");
+    LG_HIP(hipMemcpyAsync(fs_h_ + fs_err_off_ + (size_t)n_err_ * B_, e.as<void>(), (size_t)B_ * 4,
+                          hipMemcpyDeviceToHost, s_));
+    ++n_err_;
+    keep_.push_back(std::move(e));
   }
-  for (auto& e : lookup_err_) keep_.push_back(std::move(e));
   lookup_err_.clear();
   for (auto& kv : totals_dev_) {
-    std::vector<uint32_t> h((size_t)B_ * 6);
-    LG_HIP(hipMemcpyAsync(h.data(), kv.second.as<void>(), h.size() * 4, hipMemcpyDeviceToHost, s_));
-    sync();
     const uint32_t col = kv.first;
     const uint32_t ninv = kb_inv(M(d_.modules[d_.cols[col].module].N));
-    for (int b = 0; b < B_; ++b) {
-      E6 t;
-      for (int l = 0; l < 6; ++l) t.v[l] = h[6 * b + l];
-      totals_[b][col] = t;
-      chal_[b][col] = e6_scale(t, ninv);
-    }
+    launch_scale_into(kv.second.as<uint32_t>(), ninv, chal_dev_.as<uint32_t>(), (int)d_.cols.size(), (int)col, B_,
+                      s_);
+    LG_HIP(hipMemcpyAsync(fs_h_ + fs_tot_off_ + (size_t)col * B_ * 6, kv.second.as<void>(), (size_t)B_ * 24,
+                          hipMemcpyDeviceToHost, s_));
+    total_ptr_[col] = kv.second.as<uint32_t>();
+    total_cols_.push_back(col);
     keep_.push_back(std::move(kv.second));
   }
   totals_dev_.clear();
   const Round& rd = d_.rounds[r];
-  if (rd.fs_hook == FS_BIND_EXPOSED_VALUES) {
-    // ExposedValueElements (board/logup.go:213-225): [index 0, 6 limbs] per
-    // total, in step order (prover.go:386-395)
-    for (int b = 0; b < B_; ++b)
-      for (auto& st : rd.steps) {
-        if (st.kind != STEP_CYCLIC_LOGUP) continue;
-        uint32_t w[7] = {0};
-        for (int l = 0; l < 6; ++l) w[1 + l] = totals_[b][st.outs[1]].v[l];
-        ts_[b].bind(challenge_name(r), w, 7);
-      }
-  } else if (rd.fs_hook != FS_NONE) {
-    throw std::runtime_error("unsupported FS hook");
+  if (rd.fs_hook != FS_NONE && rd.fs_hook != FS_BIND_EXPOSED_VALUES) throw std::runtime_error("unsupported FS hook");
+}
+
+// The DAG's challenge operands, gathered from chal_dev_ into [b][k][6].
+const uint32_t* Job::chal_operands(const std::vector<uint32_t>& chal_ids) {
+  if (chal_ids.empty()) return nullptr;
+  keep_.push_back(dev((size_t)B_ * chal_ids.size() * 6));
+  launch_chal_compact(chal_dev_.as<uint32_t>(), (int)d_.cols.size(), stage(chal_ids), (int)chal_ids.size(),
+                      keep_.back().as<uint32_t>(), B_, s_);
+  return keep_.back().as<uint32_t>();
+}
+
+// Message segments of a device challenge.
+struct FsMsgBuilder {
+  std::vector<FsSeg> seg;
+  int len = 0;
+  void add(const uint32_t* p, uint32_t len_words, uint32_t stride) {
+    if (len_words == 0) return;
+    seg.push_back(FsSeg{p, len_words, stride});
+    len += (int)len_words;
   }
+};
+
+// Challenge of round r: StringToElements(FSID, name) || previous digest ||
+// bindings: the first challenge's init words, the round tree's root
+// (prover.go:360-412), then with FS_BIND_EXPOSED_VALUES [0, 6 limbs] per
+// logup total in step order (prover.go:386-395, board/logup.go:213-225).
+void Job::round_challenge(size_t r) {
+  const Round& rd = d_.rounds[r];
+  const ChallengeSchedule& sc = *plan_.schedule();
+  FsMsgBuilder mb;
+  mb.add(stage(sc.prefix[r]), (uint32_t)sc.prefix[r].size(), 0);
+  if (r > 0) mb.add(fs_dig_.as<uint32_t>() + (r - 1) * (size_t)B_ * 8, 8, 8);
+  else mb.add(stage(fs_init_), (uint32_t)fs_init_.size(), 0);
+  if (rd.tree != PLAN_NONE) {
+    const TreeBuf& tb = trees_[rd.tree];
+    mb.add(tb.node_ptr(0) + 8, 8, (uint32_t)(16 * tb.nleaves));
+  }
+  if (rd.fs_hook == FS_BIND_EXPOSED_VALUES) {
+    static const std::vector<uint32_t> zero{0u};
+    const uint32_t* z = stage(zero);
+    for (auto& st : rd.steps) {
+      if (st.kind != STEP_CYCLIC_LOGUP) continue;
+      mb.add(z, 1, 0);
+      mb.add(total_ptr_.at(st.outs[1]), 6, 6);
+    }
+  }
+  uint32_t* dig = fs_dig_.as<uint32_t>() + r * (size_t)B_ * 8;
+  launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, dig,
+                      chal_dev_.as<uint32_t>() + (size_t)rd.challenge_col * 6, (int)d_.cols.size() * 6, B_, s_);
+  LG_HIP(hipMemcpyAsync(fs_h_ + r * (size_t)B_ * 8, dig, (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
+}
+
+// __zeta (prover/prover.go:545-554): previous digest (or the init words when there
+// are no rounds) || the roots of the AIR trees in tree order.
+void Job::zeta_challenge() {
+  const size_t nr = d_.rounds.size();
+  const ChallengeSchedule& sc = *plan_.schedule();
+  FsMsgBuilder mb;
+  mb.add(stage(sc.prefix[nr]), (uint32_t)sc.prefix[nr].size(), 0);
+  if (nr > 0) mb.add(fs_dig_.as<uint32_t>() + (nr - 1) * (size_t)B_ * 8, 8, 8);
+  else mb.add(stage(fs_init_), (uint32_t)fs_init_.size(), 0);
+  for (size_t t = 0; t < d_.trees.size(); ++t)
+    if (d_.trees[t].kind == 1) mb.add(trees_[t].node_ptr(0) + 8, 8, (uint32_t)(16 * trees_[t].nleaves));
+  uint32_t* dig = fs_dig_.as<uint32_t>() + nr * (size_t)B_ * 8;
+  launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, dig, nullptr, 0, B_, s_);
+  LG_HIP(hipMemcpyAsync(fs_h_ + nr * (size_t)B_ * 8, dig, (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
+}
+
+// Host side of the device challenges, once the stream has passed their
+// copies: lookup flags, logup totals and roots, then every instance's
+// transcript gets each round's and __zeta's bindings and digest.
+void Job::replay_device_fs() {
+  for (int k = 0; k < n_err_; ++k)
+    for (int b = 0; b < B_; ++b)
+      if (fs_h_[fs_err_off_ + (size_t)k * B_ + b])
+        throw std::runtime_error("lookup: a source value is not in the range table (instance " + std::to_string(b) +
+                                 ")");
+  for (uint32_t col : total_cols_)
+    for (int b = 0; b < B_; ++b)
+      for (int l = 0; l < 6; ++l) totals_[b][col].v[l] = fs_h_[fs_tot_off_ + ((size_t)col * B_ + b) * 6 + l];
+  for (size_t t = 0; t < d_.trees.size(); ++t) {
+    roots_[t].resize(B_);
+    for (int b = 0; b < B_; ++b)
+      for (int l = 0; l < 8; ++l) roots_[t][b].v[l] = fs_h_[fs_roots_off_ + (t * B_ + b) * 8 + l];
+  }
+  const size_t nr = d_.rounds.size();
+  zeta_.resize(B_);
+  parallel_for(B_, [&](int b) {
+    for (size_t r = 0; r < nr; ++r) {
+      const Round& rd = d_.rounds[r];
+      const std::string name = challenge_name(r);
+      if (rd.tree != PLAN_NONE) ts_[b].bind(name, roots_[rd.tree][b].v, 8);
+      if (rd.fs_hook == FS_BIND_EXPOSED_VALUES)
+        for (auto& st : rd.steps) {
+          if (st.kind != STEP_CYCLIC_LOGUP) continue;
+          uint32_t w[7] = {0};
+          for (int l = 0; l < 6; ++l) w[1 + l] = totals_[b][st.outs[1]].v[l];
+          ts_[b].bind(name, w, 7);
+        }
+      ts_[b].accept(name, fs_h_ + (r * B_ + b) * 8);
+    }
+    for (size_t t = 0; t < d_.trees.size(); ++t)
+      if (d_.trees[t].kind == 1) ts_[b].bind("__zeta", roots_[t][b].v, 8);
+    const uint32_t* zd = fs_h_ + (nr * B_ + b) * 8;
+    ts_[b].accept("__zeta", zd);
+    for (int l = 0; l < 6; ++l) zeta_[b].v[l] = zd[l];
+  });
 }
 
 // AIR quotient of one module (compute_quotient.go:223-424 and :484-499,
@@ -1220,12 +1342,9 @@ void Job::quotient(size_t mi) {
   DevBuf Q = dev((size_t)B_ * 6 * rn);
   {
     std::vector<const uint32_t*> cp;
-    std::vector<E6> ch;
-    for (int b = 0; b < B_; ++b) {
+    for (int b = 0; b < B_; ++b)
       for (size_t k = 0; k < K.col_ids.size(); ++k)
         cp.push_back(cos[k].as<uint32_t>() + (size_t)b * cols_[K.col_ids[k]].limbs * rn);
-      for (auto c : K.chal_ids) ch.push_back(chal_[b][c]);
-    }
     // (g w_{rho n}^i)^n - 1 = g^n w_rho^i - 1
     std::vector<uint32_t> zh(rho);
     const uint32_t gn = kb_pow(M(MULT_GEN), n);
@@ -1234,7 +1353,7 @@ void Job::quotient(size_t mi) {
     for (int b = 0; b < B_; ++b) op[b] = Q.as<uint32_t>() + (size_t)b * 6 * rn;
     DagLaunch a{};
     a.cols = stage(cp);
-    a.chal = reinterpret_cast<const uint32_t*>(stg_->put(ch.data(), ch.size() * sizeof(E6), s_));
+    a.chal = chal_operands(K.chal_ids);
     a.rowscale = stage(zh);
     a.out = const_cast<uint32_t* const*>(stage(op));
     a.col_stride = rn;
@@ -1325,16 +1444,30 @@ void Job::run() {
   trace("upload");
   trees_.resize(d_.trees.size());
   roots_.resize(d_.trees.size());
-  chal_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
   totals_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
   ts_.assign(B_, Transcript(plan_.schedule()));
   const size_t nr = d_.rounds.size();
-  for (int b = 0; b < B_; ++b) {
+  // the first challenge binds the hash backend id and the module sizes
+  // (prover/prover.go:214-220)
+  fs_init_ = string_to_elements(TAG_HASH, "poseidon2");
+  fs_init_.push_back(M(TAG_SIZE));
+  fs_init_.push_back(M((uint32_t)d_.modules.size()));
+  for (auto& m : d_.modules) fs_init_.push_back(M(m.N));
+  {
     const std::string init = nr ? challenge_name(0) : "__zeta";
-    ts_[b].bind(init, string_to_elements(TAG_HASH, "poseidon2"));
-    std::vector<uint32_t> sizes{M(TAG_SIZE), M((uint32_t)d_.modules.size())};
-    for (auto& m : d_.modules) sizes.push_back(M(m.N));
-    ts_[b].bind(init, sizes);
+    for (int b = 0; b < B_; ++b) ts_[b].bind(init, fs_init_);
+  }
+  chal_dev_ = dev((size_t)B_ * d_.cols.size() * 6);
+  LG_HIP(hipMemsetAsync(chal_dev_.as<void>(), 0, (size_t)B_ * d_.cols.size() * 24, s_));
+  fs_dig_ = dev((nr + 1) * (size_t)B_ * 8);
+  {
+    size_t n_mult = 0;
+    for (auto& rd : d_.rounds)
+      for (auto& st : rd.steps) n_mult += st.kind == STEP_LOOKUP_MULTIPLICITY;
+    fs_roots_off_ = (nr + 1) * (size_t)B_ * 8;
+    fs_tot_off_ = fs_roots_off_ + d_.trees.size() * (size_t)B_ * 8;
+    fs_err_off_ = fs_tot_off_ + d_.cols.size() * (size_t)B_ * 6;
+    fs_h_ = ctx_.pinned_fs((fs_err_off_ + n_mult * (size_t)B_) * 4, lane_);
   }
   gencol();
   trace("gencol");
@@ -1347,16 +1480,10 @@ void Job::run() {
     if (rd.tree != PLAN_NONE) {
       commit_tree(rd.tree);
       trace(("commit " + std::to_string(r)).c_str());
-      for (int b = 0; b < B_; ++b) ts_[b].bind(challenge_name(r), roots_[rd.tree][b].v, 8);
     }
     finish_round_steps(r);
     mark("round finished");
-    parallel_for(B_, [&](int b) {
-      const Digest& dg = ts_[b].compute(challenge_name(r));
-      E6 v;
-      for (int l = 0; l < 6; ++l) v.v[l] = dg.v[l];
-      chal_[b][rd.challenge_col] = v;
-    });
+    round_challenge(r);
     mark("challenge");
     trace(("round " + std::to_string(r)).c_str());
   }
@@ -1368,13 +1495,8 @@ void Job::run() {
   for (size_t t = 0; t < d_.trees.size(); ++t) {
     if (d_.trees[t].kind != 1) continue;
     commit_tree(t);
-    for (int b = 0; b < B_; ++b) ts_[b].bind("__zeta", roots_[t][b].v, 8);
   }
-  zeta_.resize(B_);
-  parallel_for(B_, [&](int b) {
-    const Digest& dg = ts_[b].compute("__zeta");
-    for (int l = 0; l < 6; ++l) zeta_[b].v[l] = dg.v[l];
-  });
+  zeta_challenge();
   mark("zeta");
   trace("air commit + zeta");
   phase("compute-air-quotients", t0);
@@ -1397,29 +1519,33 @@ void Job::open() {
   const size_t nt = d_.trees.size();
   const size_t nm = d_.modules.size();
   const int nq = (int)d_.num_queries;
-  // ---- claimed values
+  // ---- claimed values; zeta is the device digest of __zeta
+  const uint32_t* zdig = fs_dig_.as<uint32_t>() + d_.rounds.size() * (size_t)B_ * 8;
   std::vector<DevBuf> W(nm);
+  std::vector<uint32_t> wmods, wsizes, wninv;
   for (size_t mi = 0; mi < nm; ++mi) {
     bool used = false;
     for (auto& t : d_.trees)
       for (auto& g : t.groups) used |= g.module == mi;
     if (!used) continue;
-    const ModuleInfo& m = d_.modules[mi];
-    W[mi] = dev((size_t)B_ * 6 * m.N);
-    std::vector<uint32_t> z(6 * B_), cm(6 * B_);
-    const uint32_t ninv = kb_inv(M(m.N));
-    for (int b = 0; b < B_; ++b) {
-      E6 zn = e6_pow(zeta_[b], m.N);
-      E6 c = e6_scale(e6_sub(zn, e6_one()), ninv);
-      for (int l = 0; l < 6; ++l) {
-        z[6 * b + l] = zeta_[b].v[l];
-        cm[6 * b + l] = c.v[l];
-      }
-    }
-    std::vector<uint32_t*> wp(B_);
-    for (int b = 0; b < B_; ++b) wp[b] = W[mi].as<uint32_t>() + (size_t)b * 6 * m.N;
-    launch_lagrange_weights(stage(z), stage(cm), tb_.wpow(m.log_n), const_cast<uint32_t* const*>(stage(wp)), m.N, B_, s_);
+    wmods.push_back((uint32_t)mi);
+    wsizes.push_back(d_.modules[mi].N);
+    wninv.push_back(kb_inv(M(d_.modules[mi].N)));
   }
+  DevBuf zb = dev((size_t)B_ * 6), cmb = dev(wmods.size() * (size_t)B_ * 6 + 6);
+  launch_zeta_consts(zdig, stage(wsizes), stage(wninv), (int)wmods.size(), zb.as<uint32_t>(), cmb.as<uint32_t>(), B_,
+                     s_);
+  for (size_t j = 0; j < wmods.size(); ++j) {
+    const ModuleInfo& m = d_.modules[wmods[j]];
+    DevBuf& w = W[wmods[j]];
+    w = dev((size_t)B_ * 6 * m.N);
+    std::vector<uint32_t*> wp(B_);
+    for (int b = 0; b < B_; ++b) wp[b] = w.as<uint32_t>() + (size_t)b * 6 * m.N;
+    launch_lagrange_weights(zb.as<uint32_t>(), cmb.as<uint32_t>() + j * (size_t)B_ * 6, tb_.wpow(m.log_n),
+                            const_cast<uint32_t* const*>(stage(wp)), m.N, B_, s_);
+  }
+  keep_.push_back(std::move(zb));
+  keep_.push_back(std::move(cmb));
   // value index per (tree, group, rail, poly, shift), in canonical order
   struct CV {
     size_t t, g;
@@ -1486,6 +1612,7 @@ void Job::open() {
   if (!deep_on_device) {
     LG_HIP(hipMemcpyAsync(cvals.data(), cvd.as<void>(), cvals.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
     sync();
+    replay_device_fs();
     parallel_for(B_, [&](int b) {
       for (int32_t k : bind_order) ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
       const Digest& dg = ts_[b].compute("alpha_DEEP");
@@ -1539,28 +1666,26 @@ void Job::open() {
         }
       }
     const int np = (int)pt_shift.size();
-    std::vector<E6> scale((size_t)B_ * bundles.size()), consts((size_t)B_ * np), points((size_t)B_ * np);
+    std::vector<E6> scale((size_t)B_ * bundles.size()), consts((size_t)B_ * np);
     std::vector<const uint32_t*> cp;
     const uint32_t* scale_d = nullptr;
     const uint32_t* consts_d = nullptr;
-    for (int b = 0; b < B_; ++b) {
-      for (int p = 0; p < np; ++p) points[(size_t)b * np + p] = e6_scale(zeta_[b], kb_pow(omega(m.log_n), pt_shift[p]));
+    for (int b = 0; b < B_; ++b)
       for (auto c : bcols) cp.push_back(cols_[c].lag(b, 0));
-    }
+    // points zeta * omega^shift, on the device
+    std::vector<uint32_t> wsh(np);
+    for (int p = 0; p < np; ++p) wsh[p] = kb_pow(omega(m.log_n), pt_shift[p]);
+    DevBuf pts = dev((size_t)B_ * np * 6 + 6);
+    launch_zeta_points(zdig, stage(wsh), np, pts.as<uint32_t>(), B_, s_);
     if (deep_on_device) {
       std::vector<int3> terms;
       for (size_t i = 0; i < bundles.size(); ++i)
         for (auto& kv : bundle_vals[i]) terms.push_back(make_int3((int)i, (int)kv.first, kv.second));
-      std::vector<uint32_t> prev((size_t)B_ * 8);
-      for (int b = 0; b < B_; ++b) {
-        const Digest& d = ts_[b].previous_value("alpha_DEEP");
-        for (int k = 0; k < 8; ++k) prev[(size_t)8 * b + k] = d.v[k];
-      }
       DevBuf dig = dev((size_t)B_ * 8);
       DevBuf sc = dev((size_t)B_ * bundles.size() * 6 + 6);
       DevBuf cs = dev((size_t)B_ * np * 6 + 6);
       DeepAlphaArgs da{};
-      da.prev = stage(prev);
+      da.prev = zdig;  // alpha_DEEP follows __zeta
       da.cvals = cvd.as<uint32_t>();
       da.order = stage(bind_order);
       da.ncv = (int)ncv;
@@ -1615,7 +1740,7 @@ void Job::open() {
     dd.n_cols = (int)bcols.size();
     dd.scale = scale_d;
     dd.consts = consts_d;
-    dd.points = reinterpret_cast<const uint32_t*>(stg_->put(points.data(), points.size() * sizeof(E6), s_));
+    dd.points = pts.as<uint32_t>();
     dd.wpow = tb_.wpow(m.log_n);
     dd.out = const_cast<uint32_t* const*>(stage(op));
     {
@@ -1646,6 +1771,7 @@ void Job::open() {
       ntt(cfg, jobs);
     }
     keep_.push_back(std::move(dq.buf));
+    keep_.push_back(std::move(pts));
   }
   trace("alpha + deep");
   // ---- FRI (fri.go:606-736)
@@ -1673,6 +1799,7 @@ void Job::open() {
     // the claimed values and alpha_DEEP come back while the DEEP codeword and
     // the levels tree are computed; replay them into the transcripts
     LG_HIP(hipEventSynchronize(alpha_ready.e));
+    replay_device_fs();
     const uint32_t* cv_h = alpha_h + (size_t)B_ * 8;
     std::memcpy(cvals.data(), cv_h, cvals.size() * sizeof(E6));
     parallel_for(B_, [&](int b) {

@@ -142,6 +142,7 @@ struct Context {
   uint32_t* pinned(size_t bytes, int lane);     // device -> host (query openings)
   uint32_t* pinned_in(size_t bytes, int lane);  // host -> device (trace columns)
   uint32_t* pinned_aux(size_t bytes, int lane);  // small device -> host results (claimed values, digests)
+  uint32_t* pinned_fs(size_t bytes, int lane);   // transcript data (round digests, roots, logup totals)
   Context();
   ~Context();
 
@@ -154,6 +155,8 @@ struct Context {
     size_t pinned_in_cap = 0;
     uint32_t* pinned_aux = nullptr;
     size_t pinned_aux_cap = 0;
+    uint32_t* pinned_fs = nullptr;
+    size_t pinned_fs_cap = 0;
   };
   std::mutex lane_mu_;
   std::vector<LaneBufs> lane_;
