@@ -83,10 +83,25 @@ struct P2Consts<16> {
 // a Montgomery product of a < 2p and b < p stays exact (t = ab < 2p^2 < 2^63,
 // t + m p < 2^64) and below 2p before kb_mul's own correction.
 LG_HD uint32_t kb_cube(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
   const uint64_t t = (uint64_t)x * x;
   const uint32_t m = (uint32_t)t * 0x7effffffu;
   const uint32_t sq = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
   return kb_mul(sq, x);
+#else
+  // On the host one reduction by 2^64 of the 93-bit cube (x^3 R^3 -> x^3 R):
+  // m = t (-p^-1) mod 2^64, t + m p < 2^95 is divisible by 2^64, and the
+  // quotient is below p^3/2^64 + p < 1.25 p.  Four multiplies instead of six.
+  constexpr uint64_t NEG_PINV64 = [] {
+    uint64_t inv = KB_P;  // Newton: correct to 3, 6, 12, 24, 48, 96 bits
+    for (int i = 0; i < 6; ++i) inv *= 2 - (uint64_t)KB_P * inv;
+    return 0 - inv;
+  }();
+  const unsigned __int128 t = (unsigned __int128)((uint64_t)x * x) * x;
+  const uint64_t m = (uint64_t)t * NEG_PINV64;
+  const uint32_t r = (uint32_t)((t + (unsigned __int128)m * KB_P) >> 64);
+  return r >= KB_P ? r - KB_P : r;
+#endif
 }
 
 // S-boxes of a full round whose keys are already in the state.

@@ -84,7 +84,8 @@ def test_radix16_passes_match_dft(ntt_bin, logn):
 
 def test_montgomery_product_matches_closed_form():
     """kb_mul (one 64-bit multiply-add Montgomery reduction) equals
-    a*b*2^-32 mod p computed in 128-bit integers."""
+    a*b*2^-32 mod p computed in 128-bit integers, and the host S-box cube
+    equals a^3*2^-64 mod p."""
     gxx = shutil.which("g++")
     if gxx is None:
         pytest.skip("g++ not available")
@@ -94,7 +95,7 @@ def test_montgomery_product_matches_closed_form():
         subprocess.run([gxx, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(HERE, "host", "kb_mul_check.cpp"),
                         "-o", exe], check=True)
         r = subprocess.run([exe], capture_output=True, text=True)
-        assert r.stdout.strip() == "kb_mul_mismatches 0"
+        assert r.stdout.split() == ["kb_mul_mismatches", "0", "kb_cube_mismatches", "0"], r.stdout
         assert r.returncode == 0
     finally:
         shutil.rmtree(d)
