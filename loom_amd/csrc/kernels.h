@@ -95,14 +95,6 @@ void launch_scatter(const uint32_t* src, const uint64_t* off, uint32_t* const* d
 // dst[k][j] = src[k][j] for j < n, k < n_jobs (device-resident inputs).
 void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, int n_jobs, hipStream_t s);
 
-// Column fill: out[b][j] = (j == pos) ? one : 0.
-void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s);
-// Permutation support ID_k[j] = gk * w^j  (wpow: w^j table).
-void launch_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n, int batch, hipStream_t s);
-// sigma_k[j] = g^(t / n) * w^(t % n), t = S[b][k*n + j]   (gpow: g^c for c < chunks).
-void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_t* wpow, const uint32_t* gpow,
-                  int k, int n, int batch, hipStream_t s);
-
 // Grand product: gp[0] = 1, gp[i] = gp[i-1] * num[i-1] / den[i-1]; num, den,
 // ratio (scratch) and gp are E6 columns as 6 limb arrays (limb stride n).
 // Scans of n >= SCAN_MIN_TILED rows run over many workgroups and need
@@ -114,7 +106,7 @@ void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den
 
 // Lookups and logup (board/module.go:56-71, poly/lookup_multiplicity.go:37-100,
 // board/logup.go:138-186), batch on grid y, per-instance pointer arrays.
-void launch_range_col(uint32_t* const* outs, int n, uint32_t bound, int batch, hipStream_t s);
+// (The range table column itself is a generated column: launch_gencol.)
 void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s);
 // counts[b][v] += 1 for every canonical value v < bound of src[b][0..n), err[b] += 1 for the others
 void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
@@ -190,6 +182,24 @@ struct DeepAlphaArgs {
   uint32_t* consts;  // [b][point][6]
 };
 void launch_deep_alpha(const DeepAlphaArgs& a, const uint32_t* prefix, int n_prefix, int batch, hipStream_t s);
+
+// Generated columns (module.go:77-101, 229-293) in one launch, one job per
+// column: GENCOL_RANGE 0..bound-1 then zeros, GENCOL_LAGRANGE the unit vector
+// at a0, GENCOL_ID g^a0 w^j, GENCOL_SIGMA g^(t >> logn) w^(t mod n) for
+// t = S[b][a0 n + j].  Job k covers blocks [block0, block0 + ceil(n/256)) in
+// x; y is the instance.
+enum GenColKind : int { GENCOL_RANGE = 0, GENCOL_LAGRANGE = 1, GENCOL_ID = 2, GENCOL_SIGMA = 3 };
+struct GenColJob {
+  int kind, n, logn;
+  uint32_t a0;               // range bound, Lagrange position or sigma chunk
+  uint32_t gk;               // identity support: g^a0
+  const uint32_t* wpow;      // identity support, sigma: powers of w_n
+  const uint32_t* gpow;      // sigma: powers of g
+  uint32_t* const* outs;     // B column blocks
+  const uint32_t* const* S;  // sigma: B permutations
+  int block0;
+};
+void launch_gencol(const GenColJob* jobs_dev, int njobs, int total_blocks, int batch, hipStream_t s);
 
 // Fiat-Shamir challenges on the device (transcript.go:285-315): the sponge
 // of a message given as segments; word i of segment k for instance b is

@@ -732,47 +732,39 @@ void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, i
 }
 
 // ------------------------------------------------------------------ GenCol
-__global__ void k_lagrange_col(uint32_t* const* outs, int n, int pos) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  outs[blockIdx.y][j] = (j == pos) ? KB_R : 0u;
+// All generated columns of a prove in one launch: block x of job k (the last
+// job with block0 <= x) covers rows [(x - block0) 256, +256) of every instance
+// (blockIdx.y); the per-kind values are those of k_range_col, k_lagrange_col,
+// k_id_support and k_sigma.
+__global__ __launch_bounds__(256) void k_gencol(const GenColJob* __restrict__ jobs, int njobs) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].block0 <= (int)blockIdx.x) lo = mid;
+    else hi = mid - 1;
+  }
+  const GenColJob& J = jobs[lo];
+  const int j = ((int)blockIdx.x - J.block0) * 256 + (int)threadIdx.x;
+  if (j >= J.n) return;
+  const int b = blockIdx.y;
+  uint32_t v;
+  if (J.kind == GENCOL_RANGE) {
+    v = (uint32_t)j < J.a0 ? kb_to_mont((uint32_t)j) : 0u;
+  } else if (J.kind == GENCOL_LAGRANGE) {
+    v = (uint32_t)j == J.a0 ? KB_R : 0u;
+  } else if (J.kind == GENCOL_ID) {
+    v = kb_mul(J.gk, J.wpow[j]);
+  } else {
+    const uint32_t t = J.S[b][(size_t)J.a0 * J.n + j];
+    v = kb_mul(J.gpow[t >> J.logn], J.wpow[t & (uint32_t)(J.n - 1)]);
+  }
+  J.outs[b][j] = v;
 }
 
-void launch_lagrange_col(uint32_t* const* outs, int n, int pos, int batch, hipStream_t s) {
-  const int T = 256;
+void launch_gencol(const GenColJob* jobs_dev, int njobs, int total_blocks, int batch, hipStream_t s) {
+  if (njobs == 0 || total_blocks == 0) return;
   prelaunch();
-  hipLaunchKernelGGL(k_lagrange_col, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, n, pos);
-  LG_HIP(hipGetLastError());
-}
-
-__global__ void k_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  outs[blockIdx.y][j] = kb_mul(gk, wpow[j]);
-}
-
-void launch_id_support(uint32_t* const* outs, const uint32_t* wpow, uint32_t gk, int n, int batch, hipStream_t s) {
-  const int T = 256;
-  prelaunch();
-  hipLaunchKernelGGL(k_id_support, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, wpow, gk, n);
-  LG_HIP(hipGetLastError());
-}
-
-__global__ void k_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_t* wpow, const uint32_t* gpow,
-                        int k, int n, int logn) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  const uint32_t t = S[blockIdx.y][(size_t)k * n + j];
-  outs[blockIdx.y][j] = kb_mul(gpow[t >> logn], wpow[t & (n - 1)]);
-}
-
-void launch_sigma(uint32_t* const* outs, const uint32_t* const* S, const uint32_t* wpow, const uint32_t* gpow, int k,
-                  int n, int batch, hipStream_t s) {
-  int logn = 0;
-  while ((1 << logn) < n) ++logn;
-  const int T = 256;
-  prelaunch();
-  hipLaunchKernelGGL(k_sigma, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, S, wpow, gpow, k, n, logn);
+  hipLaunchKernelGGL(k_gencol, dim3(total_blocks, batch), dim3(256), 0, s, jobs_dev, njobs);
   LG_HIP(hipGetLastError());
 }
 
@@ -1009,19 +1001,6 @@ void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den
 
 // ------------------------------------------------------------------ lookups and logup
 // Range column (board/module.go:56-71): row i holds i for i < bound.
-__global__ void k_range_col(uint32_t* const* outs, int n, uint32_t bound) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n) return;
-  outs[blockIdx.y][j] = (uint32_t)j < bound ? kb_to_mont((uint32_t)j) : 0u;
-}
-
-void launch_range_col(uint32_t* const* outs, int n, uint32_t bound, int batch, hipStream_t s) {
-  const int T = n < 256 ? n : 256;
-  prelaunch();
-  hipLaunchKernelGGL(k_range_col, dim3((n + T - 1) / T, batch), dim3(T), 0, s, outs, n, bound);
-  LG_HIP(hipGetLastError());
-}
-
 __global__ void k_fill(uint32_t* const* p, int n, uint32_t v) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;

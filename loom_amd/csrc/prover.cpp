@@ -177,13 +177,21 @@ hipEvent_t KernelTimer::ev() {
 
 void KernelTimer::begin(const char* name, hipStream_t s, double bytes, double perms) {
   if (!on) return;
+  (void)s;
   Rec r{name, ev(), ev(), bytes, perms};
-  LG_HIP(hipEventRecord(r.a, s));
   pending_.push_back(r);
+  start_pending_ = true;
+}
+
+void KernelTimer::launch_point(hipStream_t s) {
+  if (!on || !start_pending_) return;
+  LG_HIP(hipEventRecord(pending_.back().a, s));
+  start_pending_ = false;
 }
 
 void KernelTimer::end(hipStream_t s) {
   if (!on || pending_.empty()) return;
+  launch_point(s);  // a record without a launch times nothing
   LG_HIP(hipEventRecord(pending_.back().b, s));
 }
 
@@ -505,7 +513,10 @@ class Job {
     stg_ = ctx_.take_staging();
     set_prelaunch_hook([](void* u) { static_cast<Job*>(u)->flush_staging(); }, this);
   }
-  void flush_staging() { stg_->flush(s_); }
+  void flush_staging() {
+    stg_->flush(s_);
+    ctx_.timer.launch_point(s_);
+  }
   ~Job() {
     set_prelaunch_hook(nullptr, nullptr);
     // after an error work may still be queued on the stream: drain it before
@@ -736,6 +747,8 @@ void Job::upload_inputs() {
 }
 
 void Job::gencol() {
+  std::vector<GenColJob> jobs;
+  int blocks = 0;
   for (size_t c = 0; c < d_.cols.size(); ++c) {
     const ColInfo& ci = d_.cols[c];
     const ColBuf& cb = cols_[c];
@@ -743,22 +756,36 @@ void Job::gencol() {
     const int logn = d_.modules[ci.module].log_n;
     std::vector<uint32_t*> outs(B_);
     for (int b = 0; b < B_; ++b) outs[b] = cb.lag(b, 0);
-    const auto* outs_d = stage(outs);
+    GenColJob j{};
+    j.n = (int)cb.n;
+    j.logn = logn;
+    j.outs = const_cast<uint32_t* const*>(stage(outs));
     if (ci.src == SRC_RANGE) {
-      launch_range_col(const_cast<uint32_t* const*>(outs_d), cb.n, ci.a0, B_, s_);
+      j.kind = GENCOL_RANGE;
+      j.a0 = ci.a0;
     } else if (ci.src == SRC_LAGRANGE) {
       if (ci.a0 >= cb.n) throw std::runtime_error("Lagrange column position out of range");
-      launch_lagrange_col(const_cast<uint32_t* const*>(outs_d), cb.n, ci.a0, B_, s_);
+      j.kind = GENCOL_LAGRANGE;
+      j.a0 = ci.a0;
     } else if (ci.src == SRC_ID) {
-      launch_id_support(const_cast<uint32_t* const*>(outs_d), tb_.wpow(logn), kb_pow(M(MULT_GEN), ci.a0), cb.n, B_, s_);
+      j.kind = GENCOL_ID;
+      j.wpow = tb_.wpow(logn);
+      j.gk = kb_pow(M(MULT_GEN), ci.a0);
     } else {
       const Perm& pm = d_.perms[ci.a1];
       std::vector<const uint32_t*> S(B_);
       for (int b = 0; b < B_; ++b) S[b] = perm_ptr_[ci.a1][b];
-      launch_sigma(const_cast<uint32_t* const*>(outs_d), stage(S), tb_.wpow(logn), tb_.gpow_small(pm.len / cb.n),
-                   ci.a0, cb.n, B_, s_);
+      j.kind = GENCOL_SIGMA;
+      j.a0 = ci.a0;
+      j.wpow = tb_.wpow(logn);
+      j.gpow = tb_.gpow_small(pm.len / cb.n);
+      j.S = stage(S);
     }
+    j.block0 = blocks;
+    blocks += (j.n + 255) / 256;
+    jobs.push_back(j);
   }
+  if (!jobs.empty()) launch_gencol(stage(jobs), (int)jobs.size(), blocks, B_, s_);
 }
 
 }  // namespace

@@ -105,7 +105,10 @@ class KernelTimer {
     double bytes = 0;
     double perms = 0;
   };
+  // begin() opens a record whose start event is recorded at the next launch
+  // point (after that launch's staging copy), so a record times kernels only.
   void begin(const char* name, hipStream_t s, double bytes, double perms);
+  void launch_point(hipStream_t s);
   void end(hipStream_t s);
   void resolve();  // call after the stream is synchronised
   std::map<std::string, Stat> stats;
@@ -120,6 +123,7 @@ class KernelTimer {
   hipEvent_t ev();
   std::vector<Rec> pending_;
   std::vector<hipEvent_t> free_;
+  bool start_pending_ = false;
 };
 
 struct Context {
