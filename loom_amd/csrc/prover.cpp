@@ -2219,8 +2219,9 @@ void Job::open() {
   };
   // host timestamps of the phase (printed with LOOMGPU_TRACE)
   const double q_t0 = now_s();
-  std::vector<double> q_chain(B_), q_back(B_), q_done(B_);
+  std::vector<double> q_chain(B_), q_back(B_), q_done(B_), q_cstart(B_), q_astart(B_);
   auto query_task = [&](int b) {
+    q_cstart[b] = now_s();
     try {
       std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
       for (uint32_t k = 0; k < flen; ++k)
@@ -2245,6 +2246,7 @@ void Job::open() {
   };
   // ---- proof assembly (canonical encoding, see include/loomgpu.h)
   auto assemble_task = [&](int b) {
+   q_astart[b] = now_s();
    {
      std::unique_lock<std::mutex> lk(ready_mu);
      ready_cv.wait(lk, [&] { return failed || ready[b / chunk]; });
@@ -2409,8 +2411,17 @@ void Job::open() {
   });
   if (getenv("LOOMGPU_TRACE")) {
     auto last = [&](const std::vector<double>& v) { return (*std::max_element(v.begin(), v.end()) - q_t0) * 1e3; };
-    fprintf(stderr, "[loomgpu] B=%d queries: chains %.3f ms, records back %.3f ms, assembled %.3f ms, %zu words/proof, %zu record words\n",
-            B_, last(q_chain), last(q_back), last(q_done), out_[0].proof_len, gwords);
+    double chain = 0, wait = 0, assemble = 0;
+    for (int b = 0; b < B_; ++b) {
+      chain += q_chain[b] - q_cstart[b];
+      wait += q_back[b] - q_astart[b];
+      assemble += q_done[b] - q_back[b];
+    }
+    fprintf(stderr,
+            "[loomgpu] B=%d queries: chains %.3f ms, records back %.3f ms, assembled %.3f ms, %zu words/proof, "
+            "%zu record words; per instance: chain %.3f ms, wait %.3f ms, assembly %.3f ms\n",
+            B_, last(q_chain), last(q_back), last(q_done), out_[0].proof_len, gwords, chain / B_ * 1e3,
+            wait / B_ * 1e3, assemble / B_ * 1e3);
   }
   trace("queries, gathers, proof assembly");
   for (auto& x : W) keep_.push_back(std::move(x));
