@@ -1,7 +1,7 @@
 """The device field/Poseidon2 headers built for the host with g++ (CPU):
 specialised divisions by 2^k equal Montgomery products, and the width-24
 permutation (its host form: lazy 64-bit partial rounds) equals the oracle's
-on random and extreme states."""
+on random and extreme states; the host transcript equals the oracle's."""
 import os
 import shutil
 import subprocess
@@ -128,3 +128,40 @@ def test_e6_product_matches_oracle():
         assert np.array_equal(got, F.e6_mul(x, y))
     finally:
         shutil.rmtree(d)
+
+
+def test_transcript_forms_match_oracle():
+    """transcript.h over a shared challenge schedule (by name and by
+    position), with names registered one by one, and with accepted digests
+    gives the oracle transcript's digests for the prove path's challenge
+    names and random bindings."""
+    from oracle.transcript import Transcript as OT
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    rng = np.random.default_rng(17)
+    P = 2**31 - 2**24 + 1
+    names = ["challenge@loom_0", "challenge@loom_1", "__zeta", "alpha_DEEP", "fri_level_1_gamma"]
+    names += ["fri_fold_%d" % j for j in range(3)] + ["fri_query_%d" % k for k in range(12)]
+    binds = [[int(v) for v in rng.integers(0, P, size=int(rng.integers(0, 40)))] for _ in names]
+    binds[0] = [P - 1] * 17
+    binds[2] = []
+    d = tempfile.mkdtemp()
+    try:
+        exe = os.path.join(d, "transcript_check")
+        subprocess.run([gxx, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(HERE, "host", "transcript_check.cpp"),
+                        "-o", exe], check=True)
+        inp = "%d\n" % len(names) + "".join("%s %d %s\n" % (n, len(b), " ".join(map(str, b)))
+                                            for n, b in zip(names, binds))
+        r = subprocess.run([exe], input=inp, capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        got = [[int(x) for x in ln.split()] for ln in r.stdout.strip().splitlines()]
+    finally:
+        shutil.rmtree(d)
+    t = OT(names)
+    want = []
+    for n, b in zip(names, binds):
+        t.bind(n, b)
+        want.append([int(v) for v in t.compute(n)])
+    assert got == want
+
