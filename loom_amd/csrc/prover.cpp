@@ -1316,12 +1316,12 @@ void Job::replay_device_fs() {
           for (int l = 0; l < 6; ++l) w[1 + l] = totals_[b][st.outs[1]].v[l];
           ts_[b].bind(name, w, 7);
         }
-      ts_[b].accept(name, fs_h_ + (r * B_ + b) * 8);
+      ts_[b].check(name, fs_h_ + (r * B_ + b) * 8);
     }
     for (size_t t = 0; t < d_.trees.size(); ++t)
       if (d_.trees[t].kind == 1) ts_[b].bind("__zeta", roots_[t][b].v, 8);
     const uint32_t* zd = fs_h_ + (nr * B_ + b) * 8;
-    ts_[b].accept("__zeta", zd);
+    ts_[b].check("__zeta", zd);
     for (int l = 0; l < 6; ++l) zeta_[b].v[l] = zd[l];
   });
 }
@@ -1831,7 +1831,7 @@ void Job::open() {
     std::memcpy(cvals.data(), cv_h, cvals.size() * sizeof(E6));
     parallel_for(B_, [&](int b) {
       for (int32_t k : bind_order) ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
-      ts_[b].accept("alpha_DEEP", alpha_h + (size_t)8 * b);
+      ts_[b].check("alpha_DEEP", alpha_h + (size_t)8 * b);
     });
   }
   keep_.push_back(std::move(cvd));
@@ -1859,6 +1859,7 @@ void Job::open() {
   std::vector<Layer> layers(FRI_ROUNDS + 1);
   std::vector<TreeBuf> ftrees(FRI_ROUNDS);
   std::vector<std::vector<Digest>> fri_roots;
+  std::vector<uint32_t> fri_dig;  // device fold-challenge digests, [round][b][8]
   // Every layer, tree and pointer table of the commit phase is allocated and
   // staged before its first launch (one staging copy for all rounds).  Round
   // j (> first) builds the tree of layer j and derives its fold challenge on
@@ -2001,17 +2002,7 @@ void Job::open() {
     if (!first_on_host)
       for (int b = 0; b < B_; ++b)
         for (int k = 0; k < 8; ++k) levels_root[b].v[k] = rt[(size_t)first * fb + (size_t)8 * b + k];
-    parallel_for(B_, [&](int b) {
-      if (!first_on_host) {
-        ts_[b].bind(first_name, levels_root[b].v, 8);
-        ts_[b].accept(first_name, &dg[(size_t)first * fb + (size_t)8 * b]);
-      }
-      for (int j = first + 1; j < FRI_ROUNDS; ++j) {
-        const std::string nf = "fri_fold_" + std::to_string(j);
-        ts_[b].bind(nf, fri_roots[j - first - 1][b].v, 8);
-        ts_[b].accept(nf, &dg[(size_t)j * fb + (size_t)8 * b]);
-      }
-    });
+    fri_dig = std::move(dg);
   }
   keep_.push_back(std::move(chal_dig));
   keep_.push_back(std::move(chal_alpha));
@@ -2220,9 +2211,25 @@ void Job::open() {
   // host timestamps of the phase (printed with LOOMGPU_TRACE)
   const double q_t0 = now_s();
   std::vector<double> q_chain(B_), q_back(B_), q_done(B_), q_cstart(B_), q_astart(B_);
+  // The host transcript binds the fold roots and computes every fold
+  // challenge itself, checking the device's digests (each query task does
+  // its instance before the query chain).
+  const size_t fold0 = ts_.empty() ? 0 : ts_[0].index(first_name);  // fri_fold_j at fold0 + j - first
+  auto replay_fri = [&](int b) {
+    if (!first_on_host) {
+      ts_[b].bind_at(fold0, levels_root[b].v, 8);
+      ts_[b].check_at(fold0, &fri_dig[(size_t)first * fb + (size_t)8 * b]);
+    }
+    for (int j = first + 1; j < FRI_ROUNDS; ++j) {
+      const size_t p = fold0 + (size_t)(j - first);
+      ts_[b].bind_at(p, fri_roots[j - first - 1][b].v, 8);
+      ts_[b].check_at(p, &fri_dig[(size_t)j * fb + (size_t)8 * b]);
+    }
+  };
   auto query_task = [&](int b) {
     q_cstart[b] = now_s();
     try {
+      replay_fri(b);
       std::vector<uint32_t> fp{M(TAG_EXTP), M(flen)};
       for (uint32_t k = 0; k < flen; ++k)
         for (int l = 0; l < 6; ++l) fp.push_back(final_w[(size_t)b * 6 * flen + (size_t)l * flen + k]);

@@ -124,6 +124,20 @@ class Transcript {
     if (p == 0 || !computed_[p - 1]) throw std::runtime_error("transcript: no computed predecessor of " + name);
     return values_[p - 1];
   }
+  // Computes `name` and checks it against a digest computed elsewhere (the
+  // device) from the same message; a mismatch is an error.
+  const Digest& check(const std::string& name, const uint32_t* digest) {
+    const Digest& d = compute(name);
+    for (int i = 0; i < 8; ++i)
+      if (d.v[i] != digest[i]) throw std::runtime_error("transcript: device digest differs for " + name);
+    return d;
+  }
+  const Digest& check_at(size_t p, const uint32_t* digest) {
+    const Digest& d = compute_at(p);
+    for (int i = 0; i < 8; ++i)
+      if (d.v[i] != digest[i]) throw std::runtime_error("transcript: device digest differs for " + sch().names[p]);
+    return d;
+  }
   // Records the digest of `name` as computed elsewhere (the device) from this
   // transcript's message for it (string_to_elements(FSID, name) || previous
   // digest || bindings); the same order checks as compute().
