@@ -120,6 +120,18 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
   }
 }
 
+// LDS word index of element a in k_ntt16: a ^ f(a >> 6) with
+// f(h) = ((h << 2) ^ h ^ (h >> 5)) mod 64, a bijection on [0, n) for n >= 64
+// (only the bank bits change, as a function of the others).  The kernel's
+// access patterns (bit-reversed scatter, 16 consecutive words per thread,
+// pass strides q = n/16 .. 1) then meet at most 2 words per bank where the
+// plain index meets up to 64 (found by exhaustive search over XOR forms for
+// 2^8..2^14 with 32 and 64 banks).
+__device__ __forceinline__ uint32_t lds_swz(uint32_t a) {
+  const uint32_t h = a >> 6;
+  return a ^ (((h << 2) ^ h ^ (h >> 5)) & 63u);
+}
+
 // Radix-16 form of k_ntt_multi for 2^8 <= n <= 2^14 (same job and argument
 // semantics): n/16 threads, 16 values per thread, four radix-2 stages per
 // pass in registers (ntt16.h), one LDS exchange and barrier per pass instead
@@ -165,27 +177,27 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const uint32_t i = t + (uint32_t)k * q0;
-      sh[brev(i, L)] = in[i];
+      sh[lds_swz(brev(i, L))] = in[i];
     }
     __syncthreads();
     if (r) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = sh[16 * t + k];
+      for (int k = 0; k < 16; ++k) x[k] = sh[lds_swz(16 * t + k)];
       dit_small_r(x, r, Ri);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sh[16 * t + k] = x[k];
+      for (int k = 0; k < 16; ++k) sh[lds_swz(16 * t + k)] = x[k];
       __syncthreads();
     }
     for (int p = 0; p < P; ++p) {
       const int logm = r + 4 * (p + 1);
       const uint32_t q = 1u << (logm - 4);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = sh[ntt16_pos(t, q, k)];
+      for (int k = 0; k < 16; ++k) x[k] = sh[lds_swz(ntt16_pos(t, q, k))];
       const Ladder Ld = load_ladder(lad_inv, logm, t % q);
       dit16(x, Ld);
       if (p < P - 1) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) sh[ntt16_pos(t, q, k)] = x[k];
+        for (int k = 0; k < 16; ++k) sh[lds_swz(ntt16_pos(t, q, k))] = x[k];
         __syncthreads();
       }
     }
@@ -222,19 +234,19 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
       const uint32_t q = 1u << (logm - 4);
       if (p > 0) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) y[k] = sh[ntt16_pos(t, q, k)];
+        for (int k = 0; k < 16; ++k) y[k] = sh[lds_swz(ntt16_pos(t, q, k))];
       }
       const Ladder Ld = load_ladder(lad_fwd, logm, t % q);
       dif16(y, Ld);
       if (p < P - 1 || r) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) sh[ntt16_pos(t, q, k)] = y[k];
+        for (int k = 0; k < 16; ++k) sh[lds_swz(ntt16_pos(t, q, k))] = y[k];
         __syncthreads();
       }
     }
     if (r) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) y[k] = sh[16 * t + k];
+      for (int k = 0; k < 16; ++k) y[k] = sh[lds_swz(16 * t + k)];
       dif_small_r(y, r, Rf);
     }
     // y[k] is now the bit-reversed-order value at position 16 t + k
@@ -249,12 +261,12 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
     } else {
       __syncthreads();  // other threads may still read LDS of this output's last pass
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sh[16 * t + k] = y[k];
+      for (int k = 0; k < 16; ++k) sh[lds_swz(16 * t + k)] = y[k];
       __syncthreads();
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         const uint32_t i = t + (uint32_t)k * q0;
-        out[i] = sh[brev(i, L)];
+        out[i] = sh[lds_swz(brev(i, L))];
       }
     }
     __syncthreads();  // before the next output writes LDS
