@@ -331,24 +331,27 @@ constexpr int BIG_C = 8;
 __device__ __forceinline__ void col_dif(uint32_t* y, uint32_t* col, int cs, int lg, uint32_t tt,
                                         const uint32_t* __restrict__ tw, const NttRoots& R) {
   const int P = lg >> 2, r = lg & 3;
+  // a contiguous column (cs == 1, 2^lg >= 256 words) takes the bank swizzle of
+  // k_ntt16; its access patterns are the same
+  auto at = [&](uint32_t i) -> uint32_t& { return col[(cs == 1 ? lds_swz(i) : i) * cs]; };
   for (int p = 0; p < P; ++p) {
     const int logm = lg - 4 * p;
     const uint32_t q = 1u << (logm - 4);
     if (p > 0) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) y[k] = col[ntt16_pos(tt, q, k) * cs];
+      for (int k = 0; k < 16; ++k) y[k] = at(ntt16_pos(tt, q, k));
     }
     const Ladder Ld = ntt16_ladder(tw[(tt % q) << (lg - logm)], R);
     dif16(y, Ld);
     if (p < P - 1 || r) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) col[ntt16_pos(tt, q, k) * cs] = y[k];
+      for (int k = 0; k < 16; ++k) at(ntt16_pos(tt, q, k)) = y[k];
       __syncthreads();
     }
   }
   if (r) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) y[k] = col[(16 * tt + k) * cs];
+    for (int k = 0; k < 16; ++k) y[k] = at(16 * tt + k);
     dif_small_r(y, r, R);
   }
 }
