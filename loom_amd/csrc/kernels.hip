@@ -411,14 +411,26 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ 
   uint32_t* __restrict__ out = job->out;
   const uint32_t* __restrict__ post = job->post;
   const uint32_t k2r = brev(k2, b);
+  if (post) {
 #pragma unroll
-  for (int k = 0; k < 16; ++k) {
-    const uint32_t pbr = 16 * tt + k;  // = brev_a(k1)
-    const uint32_t k1 = brev(pbr, a);
-    uint32_t v = y[k];
-    if (post) v = kb_mul(v, post[k2 + (size_t)n2 * k1]);
-    if (bitrev_out) out[pbr + (size_t)n1 * k2r] = v;
-    else out[k2 + (size_t)n2 * k1] = v;
+    for (int k = 0; k < 16; ++k) y[k] = kb_mul(y[k], post[k2 + (size_t)n2 * brev(16 * tt + k, a)]);
+  }
+  if (bitrev_out) {
+    // the 16 values of a thread are contiguous there (positions 16 tt + k =
+    // brev_a(k1)): four 16-byte stores per thread instead of sixteen strided
+    // dword stores
+    uint32_t* o = out + 16 * tt + (size_t)n1 * k2r;
+    if (!(reinterpret_cast<uintptr_t>(o) & 15)) {
+      uint4* o4 = reinterpret_cast<uint4*>(o);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o4[k] = make_uint4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) o[k] = y[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 16; ++k) out[k2 + (size_t)n2 * brev(16 * tt + k, a)] = y[k];
   }
 }
 
