@@ -356,6 +356,16 @@ __device__ __forceinline__ void col_dif(uint32_t* y, uint32_t* col, int cs, int 
   }
 }
 
+// Column group of a four-step workgroup.  Workgroups go to the 8 XCDs in
+// turn (x mod 8), and each XCD has its own L2; a group's 8 adjacent columns
+// are 32 bytes of every row, so adjacent groups share cache lines.  Giving
+// the workgroups of one XCD a contiguous range of groups keeps those
+// neighbours in one L2.  A bijection on [0, G) for G a multiple of 8 (the
+// group counts here are 2^(a-3) and 2^(b-3) >= 16).
+__device__ __forceinline__ uint32_t xcd_group(uint32_t x, uint32_t G) {
+  return (G & 7u) ? x : (x & 7u) * (G >> 3) + (x >> 3);
+}
+
 // Pass A: the n2-point transforms of 8 adjacent columns j1 per workgroup
 // (radix-16 DIF, columns interleaved in LDS), times w_n^{j1 k2}, transposed
 // into tmp[k2 n1 + j1].  tw_a: w_{n2}^k (k < n2/2); the pass-A twiddle
@@ -368,7 +378,7 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_a(const BigJob* __restrict__ 
   const BigJob* job = jobs + blockIdx.y;
   const uint32_t n1 = 1u << a, n2 = 1u << b;
   const uint32_t c = threadIdx.x & (BIG_C - 1), tt = threadIdx.x >> 3;
-  const uint32_t j1 = blockIdx.x * BIG_C + c;
+  const uint32_t j1 = xcd_group(blockIdx.x, gridDim.x) * BIG_C + c;
   const uint32_t qc = n2 >> 4;
   const uint32_t* __restrict__ in = job->in;
   const uint32_t* __restrict__ pre = job->pre;
@@ -401,7 +411,7 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ 
   const uint32_t n1 = 1u << a, n2 = 1u << b;
   const uint32_t tpc = n1 >> 4;  // threads per row
   const uint32_t tt = threadIdx.x % tpc, c = threadIdx.x / tpc;
-  const uint32_t k2 = blockIdx.x * BIG_C + c;
+  const uint32_t k2 = xcd_group(blockIdx.x, gridDim.x) * BIG_C + c;
   const uint32_t* __restrict__ tmp = job->tmp + (size_t)k2 * n1;
   uint32_t y[16];
 #pragma unroll
