@@ -1063,11 +1063,46 @@ __global__ void k_histogram(const uint32_t* const* src, int n, uint32_t* const* 
   else atomicAdd(&err[b], 1u);
 }
 
+// Tables up to HIST_LDS_BINS entries: each workgroup counts HIST_ROWS rows into
+// an LDS copy of the table and adds its non-zero bins once (the global
+// table sees one atomic per bin and workgroup instead of one per row).
+constexpr int HIST_LDS_BINS = 4096;
+constexpr int HIST_ROWS = 4096;
+
+__global__ __launch_bounds__(256) void k_histogram_lds(const uint32_t* const* src, int n, uint32_t* const* counts,
+                                                       uint32_t bound, uint32_t* err) {
+  __shared__ uint32_t h[HIST_LDS_BINS];
+  __shared__ uint32_t e;
+  const int b = blockIdx.y;
+  for (uint32_t i = threadIdx.x; i < bound; i += blockDim.x) h[i] = 0u;
+  if (threadIdx.x == 0) e = 0u;
+  __syncthreads();
+  const int j0 = blockIdx.x * HIST_ROWS;
+  const int j1 = min(n, j0 + HIST_ROWS);
+  const uint32_t* in = src[b];
+  for (int j = j0 + (int)threadIdx.x; j < j1; j += blockDim.x) {
+    const uint32_t v = kb_from_mont(in[j]);
+    if (v < bound) atomicAdd(&h[v], 1u);
+    else atomicAdd(&e, 1u);
+  }
+  __syncthreads();
+  uint32_t* c = counts[b];
+  for (uint32_t i = threadIdx.x; i < bound; i += blockDim.x)
+    if (h[i]) atomicAdd(&c[i], h[i]);
+  if (threadIdx.x == 0 && e) atomicAdd(&err[b], e);
+}
+
 void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
                       int batch, hipStream_t s) {
-  const int T = n < 256 ? n : 256;
+  if (n <= 0 || batch <= 0) return;
   prelaunch();
-  hipLaunchKernelGGL(k_histogram, dim3((n + T - 1) / T, batch), dim3(T), 0, s, src, n, counts, bound, err);
+  if (bound <= (uint32_t)HIST_LDS_BINS) {
+    hipLaunchKernelGGL(k_histogram_lds, dim3((n + HIST_ROWS - 1) / HIST_ROWS, batch), dim3(256), 0, s, src, n, counts,
+                       bound, err);
+  } else {
+    const int T = n < 256 ? n : 256;
+    hipLaunchKernelGGL(k_histogram, dim3((n + T - 1) / T, batch), dim3(T), 0, s, src, n, counts, bound, err);
+  }
   LG_HIP(hipGetLastError());
 }
 
