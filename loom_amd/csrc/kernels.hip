@@ -332,8 +332,12 @@ __device__ __forceinline__ void col_dif(uint32_t* y, uint32_t* col, int cs, int 
                                         const uint32_t* __restrict__ tw, const NttRoots& R) {
   const int P = lg >> 2, r = lg & 3;
   // a contiguous column (cs == 1, 2^lg >= 256 words) takes the bank swizzle of
-  // k_ntt16; its access patterns are the same
-  auto at = [&](uint32_t i) -> uint32_t& { return col[(cs == 1 ? lds_swz(i) : i) * cs]; };
+  // k_ntt16, whose access patterns it shares; 8 interleaved columns (cs == 8,
+  // 2^lg >= 128) flip the low 3 index bits by bits 4-6, which leaves every
+  // pattern of the passes conflict-free (checked exhaustively for 2^7..2^11)
+  auto at = [&](uint32_t i) -> uint32_t& {
+    return col[(cs == 1 ? lds_swz(i) : (cs == 8 ? i ^ ((i >> 4) & 7u) : i)) * cs];
+  };
   for (int p = 0; p < P; ++p) {
     const int logm = lg - 4 * p;
     const uint32_t q = 1u << (logm - 4);
