@@ -1308,7 +1308,9 @@ __global__ void k_claims_sum(const ClaimJob* jobs, int n_jobs, int S, const uint
   for (int l = 0; l < 6; ++l) jobs[j].out[l] = acc.v[l];
 }
 
-int claims_split(int n) { return n >= (1 << 16) ? std::min(CLAIMS_MAX_SPLIT, n >> 15) : 1; }
+// 4096 rows per block from 2^12 rows up: a batch's claims then fill the
+// device (400 claims of 2^14 rows: 1600 blocks instead of 400).
+int claims_split(int n) { return n >= (1 << 12) ? std::min(CLAIMS_MAX_SPLIT, n >> 12) : 1; }
 
 void launch_claims(const ClaimJob* jobs, int n_jobs, int n, uint32_t* part, hipStream_t s) {
   if (n_jobs == 0) return;
