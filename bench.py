@@ -53,7 +53,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the proofs are independent, so N GPUs would each run this "
-              "same batch (see DESIGN.md on sharding); run with --gpus 1", file=sys.stderr)
+              "same batch in a process of their own (one context per GPU, no exchange); run with --gpus 1",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
