@@ -20,22 +20,52 @@
  * one HIP stream on one device; calls on one context are serialised by the
  * library, several contexts may run concurrently.
  *
+ * What of prover.Prove (prover/prover.go:55-108, 655-719) this boundary
+ * carries, and what it refuses:
+ *   - trace.Trace: the input columns (base field) and the copy-constraint
+ *     permutations; the prover steps' columns are computed on the device.
+ *   - board.Program + layout: the plan word stream.
+ *   - WithNumFriQueries: the plan's num_queries.
+ *   - WithHashBackend: Poseidon2 only (hash id "poseidon2").
+ *   - WithPhaseCallback: lg_plan_last_phases after the call.
+ *   - Refused (a plan or call that needs them is rejected with an error):
+ *     a non-empty setup.ProvingKey (precommitted setup trees), non-empty
+ *     public.Inputs, WithTranscript and WithNewTranscriptHasher (the
+ *     transcript always starts fresh with the Poseidon2 sponge), SkipFRI,
+ *     WithFriOption (no proof-of-work grinding: the PoW list is empty), other
+ *     hash backends, lookups other than width-1 selector-free range lookups.
+ *
  * Plan word stream: see loom_amd/plan.py (module sizes, column table, rounds
  * and prover steps, commitment trees with opening shifts, folded constraint
- * DAGs).
+ * DAGs).  lg_plan_create and lg_plan_check check every index in it.
  *
  * Proof word stream (all u32; field elements Montgomery; E6 = 6 words in the
- * order B0.A0 B0.A1 B1.A0 B1.A1 B2.A0 B2.A1):
+ * order B0.A0 B0.A1 B1.A0 B1.A1 B2.A0 B2.A1).  It encodes proof.Proof
+ * (proof/proof.go:31-51) with every map in sorted-key order:
  *   hash id (byte length, then bytes packed 4 per word, little endian)
- *   #exposed (= 0)
+ *   #exposed, then per exposed value in ascending byte order of its name
+ *     (proof.ExposedValues, proof/public_inputs.go:23-49):
+ *       name (byte length, then bytes packed 4 per word, little endian)
+ *       #entries, then per entry: Idx, ExtValue() as E6 (6 words; a base
+ *       entry lifted to E6)
+ *     (the logup totals of cyclic logup columns: one entry, Idx 0, each)
  *   #commitments, roots (8 words each; trace rounds then AIR)
- *   claimed values: #trees { #groups { #base { #shifts E6... } #ext { #shifts E6... } } }
- *   FRI: levels root; #levels-1 { #queries { 1, row, P, Q, 0, 0, 0 } };
- *        #roots, roots; 1 (ext), #final, final E6...;
- *        #queries { #layers { 1, row, P, Q, path } }; 0 (proofs of work)
- *   samplings: #queries { #trees { lo row, hi row, path, #inj, lo row, hi row ... } }
- *   path := leaf index, #siblings, siblings (8 each), #injection leaves, leaves (8 each)
- *   row  := #base, base values, #ext, ext values (6 each)
+ *   claimed values (fri.OpeningProof.ClaimedValues):
+ *     #trees { #groups { #base { #shifts, E6 per shift } #ext { #shifts, E6 per shift } } }
+ *   FRI proof (fri.Proof):
+ *     levels root (8 words)
+ *     #levels - 1, then per level l >= 1: #queries { opening }   (no path)
+ *     #roots, roots (8 words each; one per folded layer)
+ *     1 (the final polynomial is extension-field), #final, E6 per value
+ *     #queries, then per query: #layers { opening }
+ *     #proofs of work (0: no grinding)
+ *   samplings (PointSamplings): #queries { #trees { row(lo), row(hi), path,
+ *     #injected groups { row(lo), row(hi) } } }
+ *   opening := 1 (extension-field layer), row index, P (E6), Q (E6), path:
+ *     the layer's values at rows 2k and 2k+1 (P at the even row)
+ *   path := leaf index, #siblings, siblings (8 words each, leaf to root),
+ *           #injection leaves, leaves (8 words each)
+ *   row  := #base, base values, #ext, ext values (6 words each)
  */
 #ifndef LOOMGPU_H
 #define LOOMGPU_H
@@ -73,6 +103,10 @@ int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms);
 /* Proves `batch` independent instances of one plan (same shapes, different
  * witnesses), in lockstep: one launch sequence per phase for all instances.
  * With batch = 1 this is prover.Prove for one statement.
+ * An instance with an invalid witness (a lookup value outside its table, a
+ * permutation entry outside its cells) fails alone: its proof_lens[b] is 0,
+ * the other instances' proofs are written, and the call returns -3 with
+ * lg_last_error() naming each failed instance ("instance b: ...").
  *   columns[b * n_columns + k]: host pointer, N words (Montgomery), column k of
  *                               instance b in plan input order
  *   perms[b * n_perms + p]:     host pointer, permutation p of instance b
@@ -82,7 +116,8 @@ int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms);
  *   transcripts[b]:             optional (may be NULL): receives the challenge
  *                               digests (8 words each) in computation order,
  *                               capacity transcript_caps[b] words
- * Returns -2 when a buffer is too small (lengths are still reported). */
+ * Returns -2 when a buffer is too small (lengths are still reported), -3 when
+ * some instances failed (see above), -1 on any other failure. */
 int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
              uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens, uint32_t* const* transcripts,
              const size_t* transcript_caps, size_t* transcript_lens);
@@ -118,7 +153,9 @@ int lg_plan_last_phases(const lg_plan* plan, double* out3);
  * memory at creation); a request larger than the budget runs alone.  Queued
  * requests of the same plan join the prove of the one admitted, up to
  * max_batch instances (continuous batching); every proof equals the one
- * lg_prove gives for that instance alone. */
+ * lg_prove gives for that instance alone.  Failures are per request: a request
+ * with an invalid instance fails (lg_server_wait returns -1 naming it) while
+ * the requests merged into the same prove complete normally. */
 typedef struct lg_server lg_server;
 int lg_server_create(int device, int slots, uint64_t hbm_budget_bytes, int max_batch, lg_server** out);
 /* Stops the workers after their current prove; queued requests fail. */

@@ -163,9 +163,17 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
       });
     }
     std::atomic<int> rc{0};
+    std::string failed;
+    for (int b = 0; b < batch; ++b)
+      if (!out[b].error.empty()) failed += (failed.empty() ? "" : "; ") + std::string("instance ") + std::to_string(b) +
+                                           ": " + out[b].error;
     lg::parallel_for(batch, [&](int b) {
       proof_lens[b] = out[b].proof_len;
-      if (out[b].proof_len > proof_caps[b]) {
+      if (!out[b].error.empty()) {
+        proof_lens[b] = 0;
+        if (transcripts && transcripts[b] && transcript_lens) transcript_lens[b] = 0;
+        return;
+      } else if (out[b].proof_len > proof_caps[b]) {
         rc = -2;
       } else if (!out[b].direct) {
         std::memcpy(proofs[b], out[b].proof.data(), out[b].proof.size() * 4);
@@ -176,6 +184,11 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
         else std::memcpy(transcripts[b], out[b].transcript.data(), out[b].transcript.size() * 4);
       }
     });
+    if (!failed.empty()) {
+      // the other instances' proofs are complete (unless a buffer was short)
+      g_err = "lg_prove: " + failed;
+      return rc.load() ? rc.load() : -3;
+    }
     if (rc.load()) g_err = "lg_prove: output buffer too small";
     return rc.load();
   });

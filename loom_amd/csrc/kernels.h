@@ -131,18 +131,27 @@ void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const
                              uint32_t* const* W, int n, int batch, hipStream_t s);
 
 // Claimed value jobs: value = sum_i W[(i - shift) mod n] * P[i].
-struct ClaimJob {
-  const uint32_t* W;     // 6 limb arrays, stride n
+// Claimed values (internal/fri/values.go:46-110): value = sum_j W[j] P[(j + s) mod n]
+// with W the Lagrange weights L_j(zeta) of the polynomial's module.  The
+// claims of one (instance, module) share W: a ClaimSet lists them, so W is
+// read once per row for up to CLAIMS_GROUP claims at a time.
+struct ClaimItem {
   const uint32_t* poly;  // 1 (base) or 6 (ext) limb arrays, stride n
   uint32_t* out;         // 6 words
   int32_t shift;
   int32_t ext;
 };
-// Rows are split over claims_split(n) blocks per job for large n; part then
-// holds n_jobs x claims_split(n) x 6 words of partial sums.
-constexpr int CLAIMS_MAX_SPLIT = 64;
+struct ClaimSet {
+  const uint32_t* W;     // 6 limb arrays, stride n
+  int32_t first, count;  // items [first, first + count)
+};
+constexpr int CLAIMS_GROUP = 8;
+// Rows of a set are split over claims_split(n) blocks; part then holds
+// n_items x claims_split(n) x 6 words of partial sums (item-major).
+constexpr int CLAIMS_MAX_SPLIT = 256;
 int claims_split(int n);
-void launch_claims(const ClaimJob* jobs, int n_jobs, int n, uint32_t* part, hipStream_t s);
+void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int n_items, int n, uint32_t* part,
+                   hipStream_t s);
 
 // DEEP quotient on the trace domain for one class: per row x (natural order)
 //   DQ[x] = sum_p (c_p - sum_{bundles i at p} a_i * P_i(x)) / (z_p - w^x)
@@ -197,6 +206,8 @@ struct GenColJob {
   const uint32_t* gpow;      // sigma: powers of g
   uint32_t* const* outs;     // B column blocks
   const uint32_t* const* S;  // sigma: B permutations
+  uint32_t len;              // sigma: permutation length (entries must be below it)
+  uint32_t* err;             // sigma: B flags, set to 1 for an instance with an entry >= len
   int block0;
 };
 void launch_gencol(const GenColJob* jobs_dev, int njobs, int total_blocks, int batch, hipStream_t s);

@@ -796,8 +796,15 @@ __global__ __launch_bounds__(256) void k_gencol(const GenColJob* __restrict__ jo
   } else if (J.kind == GENCOL_ID) {
     v = kb_mul(J.gk, J.wpow[j]);
   } else {
+    // an entry outside the permutation's cells raises the instance's error
+    // flag and reads no table (generatePermutation fails on it)
     const uint32_t t = J.S[b][(size_t)J.a0 * J.n + j];
-    v = kb_mul(J.gpow[t >> J.logn], J.wpow[t & (uint32_t)(J.n - 1)]);
+    if (t < J.len) {
+      v = kb_mul(J.gpow[t >> J.logn], J.wpow[t & (uint32_t)(J.n - 1)]);
+    } else {
+      J.err[b] = 1u;
+      v = 0u;
+    }
   }
   J.outs[b][j] = v;
 }
@@ -1265,62 +1272,87 @@ void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const
   LG_HIP(hipGetLastError());
 }
 
-// Block (job, chunk) sums W[(i - shift) mod n] * P[i] over rows
-// [chunk n/S, (chunk+1) n/S); with one chunk the sum is the claimed value,
-// else it is a partial sum in part[(job S + chunk) 6 ..] that k_claims_sum
-// adds up.
-__global__ __launch_bounds__(256) void k_claims(const ClaimJob* jobs, int n, uint32_t* part) {
-  __shared__ uint32_t sh[256 * 6];
-  const ClaimJob job = jobs[blockIdx.x];
+// Sum of a field word over the 256 threads of a block: wave reduction by
+// lane exchanges, then the 4 wave sums through LDS (sh: 4 words per value).
+__device__ __forceinline__ uint32_t block_sum4(uint32_t v, uint32_t* sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = kb_add(v, (uint32_t)__shfl_xor((int)v, o, 64));
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) sh[wv] = v;
+  __syncthreads();
+  return kb_add(kb_add(sh[0], sh[1]), kb_add(sh[2], sh[3]));
+}
+
+// Block (set, chunk): rows j in [chunk n/S, (chunk+1) n/S) of one claim set.
+// For each group of up to CLAIMS_GROUP claims, every thread loads W[j] once
+// per row and accumulates W[j] P_c[(j + s_c) mod n] for every claim c of the
+// group; the block sums are the claimed values (S == 1) or partial sums.
+__global__ __launch_bounds__(256) void k_claims(const ClaimSet* __restrict__ sets, const ClaimItem* __restrict__ items,
+                                                int n, uint32_t* part) {
+  __shared__ uint32_t sh[CLAIMS_GROUP * 6][4];
+  const ClaimSet set = sets[blockIdx.x];
   const int S = gridDim.y;
   const int len = n / S;
   const int lo = blockIdx.y * len;
-  E6 acc = e6_zero();
-  for (int i = lo + (int)threadIdx.x; i < lo + len; i += blockDim.x) {
-    const int wi = (i - job.shift) & (n - 1);
-    E6 w = ld6(job.W, n, wi);
-    if (job.ext) acc = e6_add(acc, e6_mul(w, ld6(job.poly, n, i)));
-    else acc = e6_add(acc, e6_scale(w, job.poly[i]));
-  }
+  const uint32_t mask = (uint32_t)(n - 1);
+  for (int g0 = 0; g0 < set.count; g0 += CLAIMS_GROUP) {
+    const int cnt = min(CLAIMS_GROUP, set.count - g0);
+    E6 acc[CLAIMS_GROUP];
 #pragma unroll
-  for (int l = 0; l < 6; ++l) sh[6 * threadIdx.x + l] = acc.v[l];
-  __syncthreads();
-  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
-    if ((int)threadIdx.x < off) {
+    for (int c = 0; c < CLAIMS_GROUP; ++c) acc[c] = e6_zero();
+    for (int j = lo + (int)threadIdx.x; j < lo + len; j += blockDim.x) {
+      const E6 w = ld6(set.W, n, j);
 #pragma unroll
-      for (int l = 0; l < 6; ++l)
-        sh[6 * threadIdx.x + l] = kb_add(sh[6 * threadIdx.x + l], sh[6 * (threadIdx.x + off) + l]);
+      for (int c = 0; c < CLAIMS_GROUP; ++c) {
+        if (c < cnt) {
+          const ClaimItem it = items[set.first + g0 + c];
+          const uint32_t i = ((uint32_t)j + (uint32_t)it.shift) & mask;
+          if (it.ext) acc[c] = e6_add(acc[c], e6_mul(w, ld6(it.poly, n, i)));
+          else acc[c] = e6_add(acc[c], e6_scale(w, it.poly[i]));
+        }
+      }
     }
-    __syncthreads();
-  }
-  if (threadIdx.x < 6) {
-    if (S == 1) job.out[threadIdx.x] = sh[threadIdx.x];
-    else part[((size_t)blockIdx.x * S + blockIdx.y) * 6 + threadIdx.x] = sh[threadIdx.x];
+#pragma unroll
+    for (int c = 0; c < CLAIMS_GROUP; ++c) {
+      if (c < cnt) {
+        const int item = set.first + g0 + c;
+#pragma unroll
+        for (int l = 0; l < 6; ++l) {
+          const uint32_t v = block_sum4(acc[c].v[l], sh[c * 6 + l]);
+          if (threadIdx.x == 0) {
+            if (S == 1) items[item].out[l] = v;
+            else part[((size_t)item * S + blockIdx.y) * 6 + l] = v;
+          }
+        }
+      }
+    }
+    __syncthreads();  // sh is reused by the next group
   }
 }
 
-__global__ void k_claims_sum(const ClaimJob* jobs, int n_jobs, int S, const uint32_t* part) {
+__global__ void k_claims_sum(const ClaimItem* items, int n_items, int S, const uint32_t* part) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_jobs) return;
+  if (j >= n_items) return;
   E6 acc = e6_zero();
   for (int c = 0; c < S; ++c) acc = e6_add(acc, ldc6(part + ((size_t)j * S + c) * 6));
 #pragma unroll
-  for (int l = 0; l < 6; ++l) jobs[j].out[l] = acc.v[l];
+  for (int l = 0; l < 6; ++l) items[j].out[l] = acc.v[l];
 }
 
-// 4096 rows per block from 2^12 rows up: a batch's claims then fill the
-// device (400 claims of 2^14 rows: 1600 blocks instead of 400).
-int claims_split(int n) { return n >= (1 << 12) ? std::min(CLAIMS_MAX_SPLIT, n >> 12) : 1; }
+// 1024 rows per block (a 40-instance batch of 2^14-row modules: 640 blocks)
+int claims_split(int n) { return std::max(1, std::min(CLAIMS_MAX_SPLIT, n >> 10)); }
 
-void launch_claims(const ClaimJob* jobs, int n_jobs, int n, uint32_t* part, hipStream_t s) {
-  if (n_jobs == 0) return;
+void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int n_items, int n, uint32_t* part,
+                   hipStream_t s) {
+  if (n_sets == 0 || n_items == 0) return;
   const int S = claims_split(n);
   if (S > 1 && !part) throw std::runtime_error("launch_claims: partial-sum buffer missing");
   prelaunch();
-  hipLaunchKernelGGL(k_claims, dim3(n_jobs, S), dim3(256), 0, s, jobs, n, part);
+  hipLaunchKernelGGL(k_claims, dim3(n_sets, S), dim3(256), 0, s, sets, items, n, part);
   LG_HIP(hipGetLastError());
   if (S > 1) {
-    hipLaunchKernelGGL(k_claims_sum, dim3((n_jobs + 63) / 64), dim3(64), 0, s, jobs, n_jobs, S, (const uint32_t*)part);
+    hipLaunchKernelGGL(k_claims_sum, dim3((n_items + 63) / 64), dim3(64), 0, s, items, n_items, S,
+                       (const uint32_t*)part);
     LG_HIP(hipGetLastError());
   }
 }
@@ -1328,43 +1360,74 @@ void launch_claims(const ClaimJob* jobs, int n_jobs, int n, uint32_t* part, hipS
 // ------------------------------------------------------------------ DEEP
 constexpr int DEEP_MAX_POINTS = 8;
 
+// acc += sum_{p in [C0, C0 + C)} num[p] / (z_p - wx) with one E6 inversion
+// (prefix products, one inverse, back substitution); C <= 4 keeps the
+// products in registers.
+template <int NP, int C0, int C>
+__device__ __forceinline__ void deep_points(const E6* num, const DeepDesc& d, int b, uint32_t wx, E6& acc) {
+  E6 den[C], pre[C];
+#pragma unroll
+  for (int k = 0; k < C; ++k) {
+    den[k] = ldc6(d.points + ((size_t)b * NP + C0 + k) * 6);
+    den[k].v[0] = kb_sub(den[k].v[0], wx);
+    pre[k] = k ? e6_mul(pre[k - 1], den[k]) : den[k];
+  }
+  E6 inv = e6_inv(pre[C - 1]);
+#pragma unroll
+  for (int k = C - 1; k >= 0; --k) {
+    const E6 ik = k ? e6_mul(inv, pre[k - 1]) : inv;  // 1 / den_k
+    if (k) inv = e6_mul(inv, den[k]);
+    acc = e6_add(acc, e6_mul(num[C0 + k], ik));
+  }
+}
+
+// One row x per thread.  NP (the class's distinct opening points) is a
+// compile-time constant, so num[] and the denominators stay in registers
+// (every index below is static after unrolling), and the NP inverses
+// 1/(z_p - w^x) come from one E6 inversion (prefix products, one inverse,
+// back substitution).
+template <int NP>
 __global__ __launch_bounds__(256) void k_deep(DeepDesc d) {
   const int x = blockIdx.x * blockDim.x + threadIdx.x;
   const int b = blockIdx.y;
   if (x >= d.n) return;
-  E6 num[DEEP_MAX_POINTS];
+  E6 num[NP];
 #pragma unroll
-  for (int p = 0; p < DEEP_MAX_POINTS; ++p)
-    if (p < d.n_points) num[p] = ldc6(d.consts + ((size_t)b * d.n_points + p) * 6);
+  for (int p = 0; p < NP; ++p) num[p] = ldc6(d.consts + ((size_t)b * NP + p) * 6);
   const uint32_t* const* cols = d.cols + (size_t)b * d.n_cols;
   for (int i = 0; i < d.n_bundles; ++i) {
     const DeepBundle bd = d.bundles[i];
-    E6 a = ldc6(d.scale + ((size_t)b * d.n_bundles + i) * 6);
-    E6 t = bd.ext ? e6_mul(a, ld6(cols[bd.col], d.n, x)) : e6_scale(a, cols[bd.col][x]);
+    const E6 a = ldc6(d.scale + ((size_t)b * d.n_bundles + i) * 6);
+    const E6 t = bd.ext ? e6_mul(a, ld6(cols[bd.col], d.n, x)) : e6_scale(a, cols[bd.col][x]);
     for (int k = 0; k < bd.n_pts; ++k) {
       const int p = d.pt_list[bd.first_pt + k];
 #pragma unroll
-      for (int q = 0; q < DEEP_MAX_POINTS; ++q)
+      for (int q = 0; q < NP; ++q)
         if (q == p) num[q] = e6_sub(num[q], t);
     }
   }
   const uint32_t wx = d.wpow[x];
   E6 acc = e6_zero();
-#pragma unroll
-  for (int p = 0; p < DEEP_MAX_POINTS; ++p)
-    if (p < d.n_points) {
-      E6 z = ldc6(d.points + ((size_t)b * d.n_points + p) * 6);
-      z.v[0] = kb_sub(z.v[0], wx);
-      acc = e6_add(acc, e6_mul(num[p], e6_inv(z)));
-    }
+  deep_points<NP, 0, (NP < 4 ? NP : 4)>(num, d, b, wx, acc);
+  if constexpr (NP > 4) deep_points<NP, 4, NP - 4>(num, d, b, wx, acc);
   st6(d.out[b], d.n, x, acc);
 }
 
 void launch_deep(const DeepDesc& d, int batch, hipStream_t s) {
-  if (d.n_points > DEEP_MAX_POINTS) throw std::runtime_error("deep: too many distinct opening points");
+  if (d.n_points < 1 || d.n_points > DEEP_MAX_POINTS) throw std::runtime_error("deep: 1..8 distinct opening points");
   const int T = 256;
+  const dim3 g((d.n + T - 1) / T, batch);
   prelaunch();
-  hipLaunchKernelGGL(k_deep, dim3((d.n + T - 1) / T, batch), dim3(T), 0, s, d);
+  switch (d.n_points) {
+    case 1: hipLaunchKernelGGL(k_deep<1>, g, dim3(T), 0, s, d); break;
+    case 2: hipLaunchKernelGGL(k_deep<2>, g, dim3(T), 0, s, d); break;
+    case 3: hipLaunchKernelGGL(k_deep<3>, g, dim3(T), 0, s, d); break;
+    case 4: hipLaunchKernelGGL(k_deep<4>, g, dim3(T), 0, s, d); break;
+    case 5: hipLaunchKernelGGL(k_deep<5>, g, dim3(T), 0, s, d); break;
+    case 6: hipLaunchKernelGGL(k_deep<6>, g, dim3(T), 0, s, d); break;
+    case 7: hipLaunchKernelGGL(k_deep<7>, g, dim3(T), 0, s, d); break;
+    default: hipLaunchKernelGGL(k_deep<8>, g, dim3(T), 0, s, d); break;
+  }
   LG_HIP(hipGetLastError());
 }
 

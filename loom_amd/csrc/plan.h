@@ -135,6 +135,94 @@ inline int ilog2(uint64_t n) {
   return k;
 }
 
+// Every cross reference of a parsed plan, checked before anything indexes
+// with it: column sources, permutations, rounds, trees and groups, opening
+// shifts and DAG leaves.  A plan that passes addresses only what it declares.
+inline void validate_plan(const PlanData& p) {
+  auto bad = [](const char* what) { throw std::runtime_error(std::string("plan: ") + what); };
+  const size_t nm = p.modules.size(), nc = p.cols.size(), np = p.perms.size();
+  const size_t nr = p.rounds.size(), nt = p.trees.size();
+  for (auto& m : p.modules) {
+    if (m.n_chunks > 8 || (m.n_chunks & (m.n_chunks - 1))) bad("chunk count must be 0 or a power of two <= 8");
+  }
+  for (auto& pm : p.perms)
+    if (pm.len == 0 || pm.len / p.modules[pm.module].N > 64) bad("permutation must hold 1..64 chunks");
+  for (auto& c : p.cols) {
+    if (c.field > 1) bad("column field must be 0 (base) or 1 (extension)");
+    const uint32_t N = p.modules[c.module].N;
+    switch (c.src) {
+      case SRC_INPUT:
+        if (c.field != 0) bad("input columns must be base-field");
+        break;
+      case SRC_LAGRANGE:
+        if (c.field != 0 || c.a0 >= N) bad("Lagrange column position out of range");
+        break;
+      case SRC_ID:
+        if (c.field != 0 || c.a0 >= 64) bad("identity support index out of range");
+        break;
+      case SRC_SIGMA:
+        if (c.field != 0 || c.a1 >= np || p.perms[c.a1].module != c.module || c.a0 >= p.perms[c.a1].len / N)
+          bad("sigma column refers to a missing permutation chunk");
+        break;
+      case SRC_STEP:
+        if (c.a0 >= nr || c.a1 >= p.rounds[c.a0].steps.size()) bad("step column refers to a missing step");
+        break;
+      case SRC_CHALLENGE:
+        if (c.field != 1) bad("challenge columns must be extension-field");
+        break;
+      case SRC_AIR:
+        if (c.field != 1 || c.a0 >= p.modules[c.module].n_chunks) bad("AIR chunk column out of range");
+        break;
+      case SRC_RANGE:
+        if (c.field != 0 || c.a0 > N) bad("range table larger than its module");
+        break;
+      default:
+        bad("unknown column source");
+    }
+  }
+  for (auto x : p.inputs)
+    if (p.cols[x].src != SRC_INPUT) bad("input list names a generated column");
+  auto check_dag = [&](const Dag& d) {
+    for (auto& x : d.nodes) {
+      if (x.field > 1) bad("dag node field must be 0 or 1");
+      if (x.op == OP_COL && x.b >= p.modules[p.cols[x.a].module].N) bad("dag leaf shift out of range");
+    }
+  };
+  for (auto& rd : p.rounds) {
+    if (rd.challenge_col >= nc || p.cols[rd.challenge_col].src != SRC_CHALLENGE) bad("round challenge column");
+    if (rd.tree != PLAN_NONE && (rd.tree >= nt || p.trees[rd.tree].kind != 0)) bad("round tree out of range");
+    if (rd.fs_hook > FS_BIND_EXPOSED_VALUES) bad("unknown Fiat-Shamir hook");
+    for (auto& st : rd.steps) {
+      for (auto o : st.outs)
+        if (p.cols[o].module != st.module && st.kind != STEP_LOOKUP_MULTIPLICITY) bad("step output of another module");
+      for (auto& d : st.ins) check_dag(d);
+    }
+  }
+  for (auto& m : p.modules) check_dag(m.relation);
+  for (auto& t : p.trees) {
+    if (t.kind > 1) bad("tree kind must be 0 (trace) or 1 (AIR)");
+    if (t.groups.empty()) bad("tree without groups");
+    for (auto& g : t.groups) {
+      if (g.module >= nm || g.N != p.modules[g.module].N) bad("group module or size mismatch");
+      if (g.base.empty() && g.ext.empty()) bad("empty group");
+      if (g.base_shifts.size() != g.base.size() || g.ext_shifts.size() != g.ext.size()) bad("shift lists");
+      for (int rail = 0; rail < 2; ++rail) {
+        const auto& ids = rail ? g.ext : g.base;
+        const auto& sh = rail ? g.ext_shifts : g.base_shifts;
+        for (size_t i = 0; i < ids.size(); ++i) {
+          if (ids[i] >= nc) bad("group column out of range");
+          const ColInfo& c = p.cols[ids[i]];
+          if (c.module != g.module || c.field != (uint32_t)rail) bad("group column of another module or rail");
+          if (c.src == SRC_CHALLENGE) bad("a challenge column cannot be committed");
+          if (sh[i].size() > 8) bad("more than 8 opening shifts per polynomial");
+          for (auto s : sh[i])
+            if (s >= g.N) bad("opening shift out of range");
+        }
+      }
+    }
+  }
+}
+
 inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
   WordReader r(words, n_words);
   if (r.next() != PLAN_MAGIC) throw std::runtime_error("plan: bad magic");
@@ -230,6 +318,7 @@ inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
   }
   for (auto& m : p.modules) m.relation = read_dag(r, nc);
   if (r.pos() != r.size()) throw std::runtime_error("plan: trailing words");
+  validate_plan(p);
   return p;
 }
 

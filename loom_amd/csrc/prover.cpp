@@ -581,6 +581,7 @@ class Job {
   void step_multiplicity(size_t r, size_t s);
   void step_logup(size_t r, size_t s);
   void finish_round_steps(size_t r);
+  void flush_err_flags();
   // Round and __zeta challenges on the device (k_fs_challenge); the host
   // replays them into its transcripts at its first synchronisation after
   // the claimed values (replay_device_fs).
@@ -627,7 +628,12 @@ class Job {
   std::vector<Transcript> ts_;
   std::vector<std::vector<E6>> totals_;  // [b][col] exposed logup totals (host, after the replay)
   std::vector<std::pair<uint32_t, DevBuf>> totals_dev_;  // (total column, B x 6 words) of the current round
-  std::vector<DevBuf> lookup_err_;                       // B words per multiplicity step of the current round
+  // Per-instance error flags (B words each) raised by kernels of the current
+  // round, with the message an instance with a set flag fails with; copied
+  // back after the round's commit (finish_round_steps).
+  std::vector<std::pair<std::string, DevBuf>> pending_err_;
+  std::vector<std::string> err_msg_;  // message of flag k in the pinned flag area
+  std::vector<std::string> inst_err_;  // [b]: the first failure of instance b ("" if none)
   // Device transcript data.  chal_dev_: [b][col][6] challenge values and
   // logup averages T/N (the DAG kernels' per-instance constants); fs_dig_:
   // [pos][b][8] digests of the round challenges (pos = round) and __zeta
@@ -636,7 +642,6 @@ class Job {
   DevBuf chal_dev_, fs_dig_;
   uint32_t* fs_h_ = nullptr;
   size_t fs_roots_off_ = 0, fs_tot_off_ = 0, fs_err_off_ = 0;
-  int n_err_ = 0;
   std::vector<uint32_t> fs_init_;                    // bindings of the first challenge (hash id, sizes)
   std::map<uint32_t, const uint32_t*> total_ptr_;    // logup total column -> device B x 6 words
   std::vector<uint32_t> total_cols_;
@@ -749,6 +754,7 @@ void Job::upload_inputs() {
 void Job::gencol() {
   std::vector<GenColJob> jobs;
   int blocks = 0;
+  uint32_t* sig_err = nullptr;
   for (size_t c = 0; c < d_.cols.size(); ++c) {
     const ColInfo& ci = d_.cols[c];
     const ColBuf& cb = cols_[c];
@@ -775,11 +781,21 @@ void Job::gencol() {
       const Perm& pm = d_.perms[ci.a1];
       std::vector<const uint32_t*> S(B_);
       for (int b = 0; b < B_; ++b) S[b] = perm_ptr_[ci.a1][b];
+      if (!sig_err) {
+        DevBuf e = dev((size_t)B_);
+        sig_err = e.as<uint32_t>();
+        std::vector<uint32_t*> ep(B_);
+        for (int b = 0; b < B_; ++b) ep[b] = sig_err + b;
+        launch_fill(const_cast<uint32_t* const*>(stage(ep)), 1, 0u, B_, s_);
+        pending_err_.push_back({"copy constraint: a permutation entry is outside the permutation's cells", std::move(e)});
+      }
       j.kind = GENCOL_SIGMA;
       j.a0 = ci.a0;
       j.wpow = tb_.wpow(logn);
       j.gpow = tb_.gpow_small(pm.len / cb.n);
       j.S = stage(S);
+      j.len = pm.len;
+      j.err = sig_err;
     }
     j.block0 = blocks;
     blocks += (j.n + 255) / 256;
@@ -1151,7 +1167,7 @@ void Job::step_multiplicity(size_t r, size_t s) {
     keep_.push_back(std::move(src.buf));
   }
   launch_counts_to_field(cnt_d, (int)mc.n, B_, s_);
-  lookup_err_.push_back(std::move(err));
+  pending_err_.push_back({"lookup: a source value is not in the range table", std::move(err)});
 }
 
 // board.CyclicLogUpStep (board/logup.go:138-186): L_i = sum_{r<=i} v_r - (i+1) T/N
@@ -1195,14 +1211,18 @@ void Job::step_logup(size_t r, size_t s) {
 
 // After round r's commitment: lookup errors, then the logup totals of the
 // round (exposed values, and the constant column T/N the relations read).
-void Job::finish_round_steps(size_t r) {
-  for (auto& e : lookup_err_) {
-    LG_HIP(hipMemcpyAsync(fs_h_ + fs_err_off_ + (size_t)n_err_ * B_, e.as<void>(), (size_t)B_ * 4,
+void Job::flush_err_flags() {
+  for (auto& e : pending_err_) {
+    LG_HIP(hipMemcpyAsync(fs_h_ + fs_err_off_ + err_msg_.size() * (size_t)B_, e.second.as<void>(), (size_t)B_ * 4,
                           hipMemcpyDeviceToHost, s_));
-    ++n_err_;
-    keep_.push_back(std::move(e));
+    err_msg_.push_back(e.first);
+    keep_.push_back(std::move(e.second));
   }
-  lookup_err_.clear();
+  pending_err_.clear();
+}
+
+void Job::finish_round_steps(size_t r) {
+  flush_err_flags();
   for (auto& kv : totals_dev_) {
     const uint32_t col = kv.first;
     const uint32_t ninv = kb_inv(M(d_.modules[d_.cols[col].module].N));
@@ -1289,11 +1309,11 @@ void Job::zeta_challenge() {
 // copies: lookup flags, logup totals and roots, then every instance's
 // transcript gets each round's and __zeta's bindings and digest.
 void Job::replay_device_fs() {
-  for (int k = 0; k < n_err_; ++k)
+  // A raised flag fails its instance only: the batch runs on in lockstep (the
+  // kernels read nothing out of bounds for it) and its proof is not written.
+  for (size_t k = 0; k < err_msg_.size(); ++k)
     for (int b = 0; b < B_; ++b)
-      if (fs_h_[fs_err_off_ + (size_t)k * B_ + b])
-        throw std::runtime_error("lookup: a source value is not in the range table (instance " + std::to_string(b) +
-                                 ")");
+      if (fs_h_[fs_err_off_ + k * B_ + b] && inst_err_[b].empty()) inst_err_[b] = err_msg_[k];
   for (uint32_t col : total_cols_)
     for (int b = 0; b < B_; ++b)
       for (int l = 0; l < 6; ++l) totals_[b][col].v[l] = fs_h_[fs_tot_off_ + ((size_t)col * B_ + b) * 6 + l];
@@ -1473,6 +1493,7 @@ void Job::run() {
   roots_.resize(d_.trees.size());
   totals_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
   ts_.assign(B_, Transcript(plan_.schedule()));
+  inst_err_.assign(B_, std::string());
   const size_t nr = d_.rounds.size();
   // the first challenge binds the hash backend id and the module sizes
   // (prover/prover.go:214-220)
@@ -1494,7 +1515,7 @@ void Job::run() {
     fs_roots_off_ = (nr + 1) * (size_t)B_ * 8;
     fs_tot_off_ = fs_roots_off_ + d_.trees.size() * (size_t)B_ * 8;
     fs_err_off_ = fs_tot_off_ + d_.cols.size() * (size_t)B_ * 6;
-    fs_h_ = ctx_.pinned_fs((fs_err_off_ + n_mult * (size_t)B_) * 4, lane_);
+    fs_h_ = ctx_.pinned_fs((fs_err_off_ + (n_mult + 1) * (size_t)B_) * 4, lane_);
   }
   gencol();
   trace("gencol");
@@ -1514,6 +1535,7 @@ void Job::run() {
     mark("challenge");
     trace(("round " + std::to_string(r)).c_str());
   }
+  flush_err_flags();  // the generated columns' flag when there are no rounds
   phase("execute-steps", t0);
   // ComputeAIRQuotients (prover.go:461-561)
   for (size_t mi = 0; mi < d_.modules.size(); ++mi)
@@ -1591,30 +1613,47 @@ void Job::open() {
     }
   const size_t ncv = cvs.size();
   DevBuf cvd = dev((size_t)B_ * ncv * 6);
-  // claims are grouped by module size in one launch per size
-  std::map<uint32_t, std::vector<ClaimJob>> cj;
+  // One claim set per (instance, module): its claims share the module's
+  // Lagrange weights.  One launch per module size.
+  struct SizeClaims {
+    std::vector<ClaimSet> sets;
+    std::vector<ClaimItem> items;
+    double bytes = 0;  // algorithmic: W once per set, each distinct polynomial once
+  };
+  std::map<uint32_t, SizeClaims> cl;
   for (int b = 0; b < B_; ++b)
-    for (size_t k = 0; k < ncv; ++k) {
-      const CV& v = cvs[k];
-      const uint32_t mi = d_.cols[v.col].module;
+    for (size_t mi = 0; mi < nm; ++mi) {
+      if (!W[mi].bytes()) continue;
       const uint32_t n = d_.modules[mi].N;
-      ClaimJob j;
-      j.W = W[mi].as<uint32_t>() + (size_t)b * 6 * n;
-      j.poly = cols_[v.col].lag(b, 0);
-      j.out = cvd.as<uint32_t>() + ((size_t)b * ncv + k) * 6;
-      j.shift = (int32_t)v.shift;
-      j.ext = v.ext;
-      cj[n].push_back(j);
+      SizeClaims& sc = cl[n];
+      ClaimSet set{W[mi].as<uint32_t>() + (size_t)b * 6 * n, (int32_t)sc.items.size(), 0};
+      std::set<uint32_t> polys;
+      for (size_t k = 0; k < ncv; ++k) {
+        const CV& v = cvs[k];
+        if (d_.cols[v.col].module != mi) continue;
+        ClaimItem it;
+        it.poly = cols_[v.col].lag(b, 0);
+        it.out = cvd.as<uint32_t>() + ((size_t)b * ncv + k) * 6;
+        it.shift = (int32_t)v.shift;
+        it.ext = v.ext;
+        sc.items.push_back(it);
+        if (polys.insert(v.col).second) sc.bytes += (double)n * 4 * (v.ext ? 6 : 1);
+      }
+      set.count = (int32_t)(sc.items.size() - set.first);
+      if (!set.count) continue;
+      sc.sets.push_back(set);
+      sc.bytes += (double)n * 24 + set.count * 24.0;
     }
-  for (auto& kv : cj) {
+  for (auto& kv : cl) {
     const int S = claims_split((int)kv.first);
     uint32_t* part = nullptr;
     if (S > 1) {
-      keep_.push_back(dev(kv.second.size() * (size_t)S * 6));
+      keep_.push_back(dev(kv.second.items.size() * (size_t)S * 6));
       part = keep_.back().as<uint32_t>();
     }
-    TimeIt tm(ctx_, s_, "claims", (double)kv.second.size() * kv.first * 7 * 4, 0);
-    launch_claims(stage(kv.second), (int)kv.second.size(), kv.first, part, s_);
+    TimeIt tm(ctx_, s_, "claims", kv.second.bytes, 0);
+    launch_claims(stage(kv.second.sets), (int)kv.second.sets.size(), stage(kv.second.items),
+                  (int)kv.second.items.size(), kv.first, part, s_);
   }
   // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
   std::vector<int32_t> bind_order;  // claimed-value indexes in binding order
@@ -2261,6 +2300,12 @@ void Job::open() {
    }
    LG_HIP(hipEventSynchronize(ev.e[b / chunk]));
    q_back[b] = now_s();
+   if (!inst_err_[b].empty()) {  // a failed instance gets no proof
+     out_[b].error = inst_err_[b];
+     out_[b].proof_len = 0;
+     q_done[b] = now_s();
+     return;
+   }
    // The words go straight to the caller's buffer when it gave one
    // (SpanWriter counts past its capacity so the caller learns the length),
    // else to out_[b].proof.

@@ -180,6 +180,10 @@ struct ProveOutput {
   size_t proof_len = 0;              // proof length in words
   bool direct = false;               // written to the caller's buffer (complete if proof_len <= cap)
   std::vector<uint32_t> transcript;  // challenge digests in computation order
+  // Non-empty when this instance failed (a lookup value outside its table, a
+  // permutation entry outside its cells): no proof was written.  The other
+  // instances of the batch are unaffected.
+  std::string error;
 };
 
 // Prover-side timing of each phase (wall clock, host), for the phase callback.
@@ -193,6 +197,8 @@ class Plan {
   const PlanData& data() const { return d_; }
   Context* ctx() const { return ctx_; }
   // Runs the prove phase for every instance; outputs are resized to batch.
+  // An instance whose witness is invalid fails alone (ProveOutput::error);
+  // a failure of the plan or the device throws for the whole batch.
   void prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutput>& outputs,
              const PhaseCallback& cb = nullptr);
   size_t num_inputs() const { return d_.inputs.size(); }

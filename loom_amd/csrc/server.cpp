@@ -213,12 +213,22 @@ void Server::worker(int si) {
     {
       std::lock_guard<std::mutex> lk(mu_);
       size_t off = 0;
+      // Errors are per request: an instance that failed (invalid witness)
+      // fails its own request only; its merge-mates get their proofs.  A
+      // failure of the whole prove (err) fails every request in it.
       for (auto& r : take) {
-        if (err.empty()) {
+        std::string rerr = err;
+        if (rerr.empty())
+          for (int b = 0; b < r->batch; ++b)
+            if (!out[off + b].error.empty()) {
+              rerr = "instance " + std::to_string(b) + ": " + out[off + b].error;
+              break;
+            }
+        if (rerr.empty()) {
           r->out.assign(std::make_move_iterator(out.begin() + off), std::make_move_iterator(out.begin() + off + r->batch));
           ++st_.completed;
         } else {
-          r->error = err;
+          r->error = rerr;
           ++st_.failed;
         }
         off += r->batch;

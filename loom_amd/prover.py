@@ -169,10 +169,15 @@ def _pointer_tables(items, dev_inputs):
     return col_ptrs, perm_ptrs
 
 
-def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None):
+def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None, allow_failures=False):
     """Proves prepared instances sharing one plan shape in one batched call.
     With `dev_inputs` (a DeviceInputs of these items) the inputs are read
-    from device memory (lg_prove_device), else from host memory (lg_prove)."""
+    from device memory (lg_prove_device), else from host memory (lg_prove).
+
+    An instance with an invalid witness (a lookup value outside its table, a
+    permutation entry outside its cells) fails alone: by default the call
+    raises LoomGpuError naming it; with `allow_failures` its result is None
+    and the other instances' proofs are returned."""
     if not items:
         return []
     words = items[0].words
@@ -202,16 +207,18 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
         if rc == -2 and max(lens) > cap:
             cap = max(lens)
             continue
+        if rc == -3 and allow_failures:
+            rc = 0
         _lib.check(rc, "lg_prove")
-        _proof_cap[words.tobytes()] = max(lens)
+        _proof_cap[words.tobytes()] = max(cap if max(lens) == 0 else max(lens), 1)
         break
     if phases is not None:
         arr = (ctypes.c_double * 3)()
         _lib.check(_lib.lib().lg_plan_last_phases(ph, arr), "lg_plan_last_phases")
         phases[:] = list(arr)
-    res = [outs[b][:lens[b]] for b in range(B)]
+    res = [outs[b][:lens[b]] if lens[b] else None for b in range(B)]
     if want_transcript:
-        return res, [touts[b][:tlens[b]].copy() for b in range(B)]
+        return res, [touts[b][:tlens[b]].copy() if lens[b] else None for b in range(B)]
     return res
 
 

@@ -49,9 +49,11 @@ def plonk_program(log_n, idx=0):
     return b, nm
 
 
-def _plonk_values(n, rng, wire_bound=P):
-    """sigma, wires (uniform in [0, wire_bound)) and selectors of one module,
-    drawn from rng."""
+def _plonk_values(n, rng, l_bound=P):
+    """sigma, wires and selectors of one module, drawn from rng: L uniform in
+    [0, l_bound), R and O uniform in [0, p).  A copy cycle that holds an L
+    cell carries the value of its first L cell (so a range-checked L stays in
+    its table), any other cycle the value of its first cell."""
     cells = 3 * n
     sigma = np.arange(cells, dtype=np.int64)
     tied = rng.permutation(cells)[: cells // 10]
@@ -65,9 +67,16 @@ def _plonk_values(n, rng, wire_bound=P):
         for k in range(ln):
             sigma[cyc[k]] = cyc[(k + 1) % ln]
         pos += ln
-    w = rng.integers(0, wire_bound, size=cells, dtype=np.uint64)
+    w = rng.integers(0, P, size=cells, dtype=np.uint64)
+    if l_bound != P:
+        w[:n] = rng.integers(0, l_bound, size=n, dtype=np.uint64)
     for cyc in cycles:
-        w[cyc[1:]] = w[cyc[0]]
+        src = cyc[0]
+        if l_bound != P:
+            in_l = cyc[cyc < n]
+            if len(in_l):
+                src = in_l[0]
+        w[cyc] = w[src]
     L, R, O = w[:n], w[n:2 * n], w[2 * n:]
     kind = rng.choice(3, size=n, p=[0.4, 0.4, 0.2])
     one, mone = np.uint64(1), np.uint64(P - 1)
@@ -118,8 +127,9 @@ def uneven_instance(log_ns, seed):
 
 def uneven_range_instance(log_ns, seed, bound=256):
     """The uneven batch with lookups: every module as in uneven_instance, its
-    wires drawn from [0, bound), plus arguments.Range(L, bound) -- a logup
-    lookup of its L column into the shared range_<bound> module."""
+    L wire drawn from [0, bound) (R and O uniform in [0, p)), plus
+    arguments.Range(L, bound) -- a logup lookup of its L column into the
+    shared range_<bound> module."""
     b = B.Builder()
     trace = {}
     for i, ln in enumerate(log_ns):

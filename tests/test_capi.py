@@ -57,3 +57,115 @@ def test_plan_rejects_garbage():
     w = plan.Plan(prog, 8).words()
     with pytest.raises(_lib.LoomGpuError):
         prover.check_plan(w[:-3])
+
+
+def _plan_offsets(w):
+    """Word offsets of the fields of a plan stream (the layout loom_amd/plan.py
+    writes), for planting bad indexes."""
+    w = [int(x) for x in w]
+    i = 3
+    off = {"cols": [], "perms": [], "rounds": [], "group_cols": [], "group_hdr": [], "shifts": []}
+    nm = w[i]; i += 1
+    off["modules"] = i
+    i += 3 * nm
+    nc = w[i]; i += 1
+    for _ in range(nc):
+        off["cols"].append(i)
+        i += 5
+    ni = w[i]; i += 1 + ni
+    npm = w[i]; i += 1
+    for _ in range(npm):
+        off["perms"].append(i)
+        i += 2
+
+    def dag(i):
+        n = w[i]
+        return i + 2 + 4 * n
+
+    nr = w[i]; i += 1
+    for _ in range(nr):
+        off["rounds"].append(i)
+        ns = w[i + 3]
+        i += 4
+        for _ in range(ns):
+            no = w[i + 2]
+            i += 3 + no
+            npar = w[i]
+            i += 1 + npar
+            nin = w[i]
+            i += 1
+            for _ in range(nin):
+                i = dag(i)
+    nx = w[i]; i += 1
+    for _ in range(nx):
+        ln = w[i + 1]
+        i += 2 + (ln + 3) // 4
+    nt = w[i]; i += 1
+    for _ in range(nt):
+        ng = w[i + 1]
+        i += 2
+        for _ in range(ng):
+            off["group_hdr"].append(i)
+            nb = w[i + 2]
+            off["group_cols"] += list(range(i + 3, i + 3 + nb))
+            i += 3 + nb
+            ne = w[i]
+            off["group_cols"] += list(range(i + 1, i + 1 + ne))
+            i += 1 + ne
+            for _ in range(nb + ne):
+                k = w[i]
+                off["shifts"] += list(range(i + 1, i + 1 + k))
+                i += 1 + k
+    for _ in range(nm):
+        i = dag(i)
+    assert i == len(w)
+    return off
+
+
+def _mutations():
+    from loom_amd import workloads, plan
+    prog, _ = workloads.plonk_instance(4, 0)
+    w = plan.Plan(prog, 8).words()
+    o = _plan_offsets(w)
+    SRC_SIGMA, SRC_LAGRANGE = 3, 1
+    sig = [c for c in o["cols"] if w[c + 2] == SRC_SIGMA]
+    lag = [c for c in o["cols"] if w[c + 2] == SRC_LAGRANGE]
+    assert sig and lag and o["group_cols"] and o["shifts"] and o["rounds"]
+    cases = {
+        "group column index": (o["group_cols"][0], 100000),
+        "group module": (o["group_hdr"][0], 7),
+        "group size": (o["group_hdr"][0] + 1, 1 << 10),
+        "opening shift": (o["shifts"][0], 1 << 20),
+        "round tree": (o["rounds"][0] + 1, 9),
+        "round challenge column": (o["rounds"][0], 100000),
+        "sigma permutation": (sig[0] + 4, 5),
+        "sigma chunk": (sig[0] + 3, 60),
+        "Lagrange position": (lag[0] + 3, 1 << 20),
+        "column field": (o["cols"][0], 2),
+    }
+    out = {}
+    for name, (at, val) in cases.items():
+        bad = w.copy()
+        bad[at] = val
+        out[name] = bad
+    return w, out
+
+
+def test_malformed_plans_are_rejected():
+    """Every index of the plan is checked by the parser: lg_plan_check and
+    lg_plan_bytes_per_instance (the admission estimate, which writes per-column
+    flags) return an error for each planted bad index instead of touching
+    memory out of bounds."""
+    import ctypes
+    from loom_amd import prover, serve, _lib
+    good, bad = _mutations()
+    assert prover.check_plan(good) > 0
+    assert serve.plan_bytes_per_instance(good) > 0
+    for name, w in bad.items():
+        with pytest.raises(_lib.LoomGpuError, match="plan"):
+            prover.check_plan(w)
+        v = ctypes.c_uint64()
+        ww = np.ascontiguousarray(w, dtype=np.uint32)
+        rc = _lib.lib().lg_plan_bytes_per_instance(ww.ctypes.data_as(_lib.u32p), ww.size, ctypes.byref(v))
+        assert rc == -1, name
+        assert "plan" in _lib.lib().lg_last_error().decode(), name
