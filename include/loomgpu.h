@@ -201,6 +201,19 @@ int lg_merkle_build(lg_ctx* ctx, uint32_t* nodes, size_t n_leaves, const uint32_
  * 6 limb arrays of n words each (limb l at +l*n), n a power of two. */
 int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n);
 
+/* lg_grand_product without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same inputs and result as lg_grand_product (device pointers);
+ * the ratio column and the scan partials live in a file-scope device array of
+ * about 6 MiB per device, held until the process ends, so n is limited to
+ * LG_NOALLOC_MAX_ROWS rows (a larger n is refused with an error).  The array
+ * is shared by every call on one device: calls that overlap on one device
+ * (from different contexts or threads) race; calls on one context are
+ * serialised by the library. */
+#define LG_NOALLOC_MAX_ROWS (1u << 18)
+int lg_grand_product_noalloc(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n);
+
 /* RS encoding (reedsolomon.Encoder.Encode, internal/reedsolomon/rs.go:52-70) of
  * n_cols columns of n words each (Lagrange form on the size-n subgroup, one
  * column after the other) to 4n bit-reversed evaluations each. */

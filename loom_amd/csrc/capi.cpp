@@ -66,6 +66,7 @@ int lg_ctx_create(int device, lg_ctx** out) {
     LG_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) throw std::runtime_error("lg_ctx_create: no such HIP device");
     LG_HIP(hipSetDevice(device));
+    lg::noalloc_init(device);  // this device's scratch address for lg_grand_product_noalloc
     auto* c = new lg_ctx();
     c->c.device = device;
     if (hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking) != hipSuccess) {
@@ -328,6 +329,17 @@ int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint
     lg::DevBuf tmp(tmp_words ? tmp_words * sizeof(uint32_t) : 0);
     lg::launch_grand_product(a.as<const uint32_t*>(), b.as<const uint32_t*>(), r.as<uint32_t*>(), c.as<uint32_t*>(),
                              (int)n, 1, tmp_words ? tmp.as<uint32_t>() : nullptr, ctx->c.stream);
+    LG_HIP(hipStreamSynchronize(ctx->c.stream));
+    return 0;
+  });
+}
+
+int lg_grand_product_noalloc(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n) {
+  return guarded([&] {
+    if (!ctx || !num || !den || !gp) throw std::runtime_error("lg_grand_product_noalloc: null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    LG_HIP(hipSetDevice(ctx->c.device));
+    lg::grand_product_noalloc(ctx->c.device, ctx->c.stream, num, den, gp, n);
     LG_HIP(hipStreamSynchronize(ctx->c.stream));
     return 0;
   });

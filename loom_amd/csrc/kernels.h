@@ -101,6 +101,12 @@ void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, i
 // scan_tmp_words(n, batch) words of scratch (scan_tmp); smaller ones need none.
 constexpr int SCAN_MIN_TILED = 1 << 13;
 size_t scan_tmp_words(int n, int batch);
+// The allocation-free grand product (noalloc.hip): noalloc_init looks up the
+// device's file-scope scratch array once (device current); the variant runs
+// lg_grand_product's kernels with all scratch inside that array.
+uint32_t* noalloc_init(int device);
+void grand_product_noalloc(int device, hipStream_t s, const uint32_t* num, const uint32_t* den, uint32_t* gp,
+                           size_t n);
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
                           uint32_t* const* gp, int n, int batch, uint32_t* scan_tmp, hipStream_t s);
 
