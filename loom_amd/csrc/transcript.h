@@ -81,10 +81,18 @@ class Transcript {
       msg.insert(msg.end(), values_[p - 1].v, values_[p - 1].v + 8);
     }
     msg.insert(msg.end(), bindings_[p].begin(), bindings_[p].end());
-    p2_sponge_host(msg.data(), msg.size(), values_[p].v);
+    hash(msg.data(), msg.size(), values_[p].v);
     computed_[p] = true;
     log_.push_back(p);
     return values_[p];
+  }
+  // The function a challenge's message is hashed with (the transcript
+  // hasher, fiatshamir.NewTranscriptHasher, transcript.go:74): the Poseidon2
+  // sponge unless one is set (a test records the messages with it).
+  using Hasher = void (*)(const uint32_t* msg, size_t len, uint32_t out[8], void* user);
+  void set_hasher(Hasher h, void* user) {
+    hasher_ = h;
+    hasher_user_ = user;
   }
   // Position-based forms for long runs of consecutive challenges (the FRI
   // queries).
@@ -112,7 +120,7 @@ class Transcript {
     if (p != 0)
       for (int i = 0; i < 8; ++i) msg[o++] = values_[p - 1].v[i];
     for (uint32_t x : bindings_[p]) msg[o++] = x;
-    p2_sponge_host(msg, len, values_[p].v);
+    hash(msg, len, values_[p].v);
     computed_[p] = true;
     log_.push_back(p);
     return values_[p];
@@ -163,6 +171,12 @@ class Transcript {
 
  private:
   const ChallengeSchedule& sch() const { return fixed_ ? *fixed_ : own_; }
+  void hash(const uint32_t* msg, size_t len, uint32_t out[8]) {
+    if (hasher_) hasher_(msg, len, out, hasher_user_);
+    else p2_sponge_host(msg, len, out);
+  }
+  Hasher hasher_ = nullptr;
+  void* hasher_user_ = nullptr;
   size_t at(const std::string& name) const {
     const auto& pos = sch().pos;
     auto it = pos.find(name);
