@@ -130,6 +130,15 @@ int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, co
                     uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
                     uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens);
 
+/* HIP graphs per shape bucket (on by default; LOOMGPU_GRAPHS=0 turns the
+ * default off).  For a plan whose modules share one size (one DEEP class), the
+ * device phase of a prove (generated columns through the FRI commit phase)
+ * needs no host round trip: the second prove of a shape (batch size) records
+ * it into a HIP graph over buffers kept for that shape, later proves of the
+ * shape replay the graph.  Proofs are the same either way.  At most 4 shapes
+ * per plan are kept (least recently used dropped). */
+int lg_ctx_set_graphs(lg_ctx* ctx, int on);
+
 /* Per-kernel device timing with HIP events on the context's stream (off by
  * default).  lg_ctx_timing_report writes one text line per kernel name,
  * "name launches device_ms algorithmic_bytes poseidon2_permutations", into

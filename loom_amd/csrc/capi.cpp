@@ -209,6 +209,15 @@ int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, co
                     transcript_lens);
 }
 
+int lg_ctx_set_graphs(lg_ctx* ctx, int on) {
+  return guarded([&] {
+    if (!ctx) throw std::runtime_error("lg_ctx_set_graphs: null ctx");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->c.graphs = on != 0;
+    return 0;
+  });
+}
+
 int lg_ctx_set_timing(lg_ctx* ctx, int on) {
   return guarded([&] {
     if (!ctx) throw std::runtime_error("lg_ctx_set_timing: null ctx");

@@ -239,7 +239,7 @@ void dag_kernel_launch(const DagKernel& k, const DagLaunch& a, hipStream_t s) {
   const uint64_t total = (uint64_t)ncos << logn;
   const unsigned T = 256;
   const unsigned gx = (unsigned)((total + T - 1) / T);
-  prelaunch();
+  if (!prelaunch()) return;
   LG_HIP(hipModuleLaunchKernel(k.code->fn, gx, a.batch, 1, T, 1, 1, 0, s, args, nullptr));
 }
 

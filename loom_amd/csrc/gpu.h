@@ -25,32 +25,44 @@ class DevBuf {
   ~DevBuf() { release(); }
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+  DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), owned_(o.owned_) { o.p_ = nullptr; o.n_ = 0; }
   DevBuf& operator=(DevBuf&& o) noexcept {
     if (this != &o) {
       release();
-      p_ = o.p_; n_ = o.n_;
+      p_ = o.p_; n_ = o.n_; owned_ = o.owned_;
       o.p_ = nullptr; o.n_ = 0;
     }
     return *this;
+  }
+  // A non-owning handle of memory owned elsewhere (an arena lends its buffers
+  // this way); releasing it frees nothing.
+  static DevBuf view(void* p, size_t bytes) {
+    DevBuf b;
+    b.p_ = p;
+    b.n_ = bytes;
+    b.owned_ = false;
+    return b;
   }
   void alloc(size_t bytes) {
     release();
     if (bytes) LG_HIP(hipMalloc(&p_, bytes));
     n_ = bytes;
+    owned_ = true;
   }
   void release() {
-    if (p_) (void)hipFree(p_);
+    if (p_ && owned_) (void)hipFree(p_);
     p_ = nullptr;
     n_ = 0;
   }
   template <class T = uint32_t>
   T* as() const { return reinterpret_cast<T*>(p_); }
   size_t bytes() const { return n_; }
+  bool owned() const { return owned_; }
 
  private:
   void* p_ = nullptr;
   size_t n_ = 0;
+  bool owned_ = true;
 };
 
 // Device copy of a small host vector (kernel argument tables).  The copy is

@@ -11,7 +11,13 @@ namespace lg {
 // staged in pinned host memory to the device in one copy per launch.
 using PreLaunchHook = void (*)(void* user);
 void set_prelaunch_hook(PreLaunchHook fn, void* user);
-void prelaunch();
+// Called by every launch wrapper before it launches: runs the calling
+// thread's hook (staging flush) and returns true, or returns false without
+// launching anything while launches are suppressed on this thread (a prove
+// that replays a recorded HIP graph re-runs its host logic only).
+bool prelaunch();
+void set_launch_suppression(bool on);
+bool launch_suppressed();
 
 constexpr int NTT_MAX_LOG_LDS = 14;   // one workgroup holds a size-2^14 transform in LDS
 constexpr int NTT_MAX_OUT = 8;

@@ -19,8 +19,15 @@ void set_prelaunch_hook(PreLaunchHook fn, void* user) {
   g_hook_user = user;
 }
 
-void prelaunch() {
+static thread_local bool g_suppress = false;
+
+void set_launch_suppression(bool on) { g_suppress = on; }
+bool launch_suppressed() { return g_suppress; }
+
+bool prelaunch() {
+  if (g_suppress) return false;
   if (g_hook) g_hook(g_hook_user);
+  return true;
 }
 
 __device__ __forceinline__ uint32_t brev(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
@@ -285,7 +292,7 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   if (n / T > NTT_MAX_PER_THREAD) throw std::runtime_error("launch_ntt: too many values per thread");
   NttArgs a{cfg.logn, cfg.inverse_first, cfg.n_out, 0};
   for (int o = 0; o < cfg.n_out; ++o) a.bitrev_mask |= (cfg.bitrev_out[o] ? 1 : 0) << o;
-  prelaunch();
+  if (!prelaunch()) return;
   const size_t lds = n * sizeof(uint32_t);
   if (cfg.logn >= 8) {
     const dim3 g(n_jobs), b(n / 16);
@@ -453,7 +460,7 @@ void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int 
   if (n_jobs == 0) return;
   const int a = (logn + 1) / 2, b = logn - a;  // n1 = 2^a >= n2 = 2^b
   if (a > 11 || b < 7 || logn < 15) throw std::runtime_error("launch_ntt_big: unsupported size");
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_ntt_pass_a, dim3((1 << a) / BIG_C, n_jobs), dim3(BIG_C * ((1 << b) >> 4)),
                      (size_t)BIG_C * (1 << b) * 4, s, jobs_a, a, b, tw_a, wlo, whi);
   LG_HIP(hipGetLastError());
@@ -501,7 +508,7 @@ void launch_leaf_hash(const uint32_t* const* colptrs, int ncols, const int32_t* 
                       uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch, hipStream_t s) {
   if (nleaves == 0 || batch == 0) return;
   const int T = 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_leaf_hash, dim3((nleaves + T - 1) / T, batch), dim3(T), 0, s, colptrs, ncols, msgmap, msglen,
                      hdr1, hdr2, outs, nleaves);
   LG_HIP(hipGetLastError());
@@ -533,7 +540,7 @@ __global__ __launch_bounds__(256) void k_merkle_level(uint32_t* const* __restric
 
 void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s) {
   const int T = width < 256 ? width : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_merkle_level, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, width);
   LG_HIP(hipGetLastError());
 }
@@ -673,7 +680,7 @@ void launch_merkle_coop(uint32_t* const* trees, int width_hi, int width_lo, int 
   while ((width_hi >> (levels - 1)) > width_lo) ++levels;
   const int G = width_hi < 32 ? width_hi : 32;
   if ((width_hi / width_lo) > G) throw std::runtime_error("merkle coop: span exceeds workgroup");
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_merkle_coop<false>, dim3(width_hi / G, batch), dim3(16 * G), 0, s, trees, width_hi, levels,
                      FriChalArgs{});
   LG_HIP(hipGetLastError());
@@ -694,7 +701,7 @@ void launch_merkle_coop_challenge(uint32_t* const* trees, int width_hi, const ui
   ch.dig = dig;
   ch.alpha = alpha;
   ch.roots = roots;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_merkle_coop<true>, dim3(1, batch), dim3(16 * width_hi), 0, s, trees, width_hi, levels, ch);
   LG_HIP(hipGetLastError());
 }
@@ -704,7 +711,7 @@ void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_
   while ((width_hi >> (levels - 1)) > width_lo) ++levels;
   const int G = width_hi < 256 ? width_hi : 256;
   if ((width_hi / width_lo) > G) throw std::runtime_error("merkle fused: span exceeds workgroup");
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_merkle_fused, dim3(width_hi / G, batch), dim3(G), 0, s, trees, width_hi, levels);
   LG_HIP(hipGetLastError());
 }
@@ -715,7 +722,7 @@ __global__ void k_copy_roots(const uint32_t* const* trees, uint32_t* roots) {
 }
 
 void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s) {
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_copy_roots, dim3(batch), dim3(64), 0, s, trees, roots);
   LG_HIP(hipGetLastError());
 }
@@ -735,7 +742,7 @@ __global__ __launch_bounds__(256) void k_inject(uint32_t* const* __restrict__ tr
 
 void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s) {
   const int T = width < 256 ? width : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_inject, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, inj, width);
   LG_HIP(hipGetLastError());
 }
@@ -752,7 +759,7 @@ __global__ void k_scatter(const uint32_t* __restrict__ src, const uint64_t* __re
 void launch_scatter(const uint32_t* src, const uint64_t* off, uint32_t* const* dst, int n, int n_jobs, hipStream_t s) {
   if (n_jobs == 0) return;
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_scatter, dim3((n + T - 1) / T, n_jobs), dim3(T), 0, s, src, off, dst, n);
   LG_HIP(hipGetLastError());
 }
@@ -767,7 +774,7 @@ __global__ void k_copy_cols(const uint32_t* const* __restrict__ src, uint32_t* c
 void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, int n_jobs, hipStream_t s) {
   if (n_jobs == 0) return;
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_copy_cols, dim3((n + T - 1) / T, n_jobs), dim3(T), 0, s, src, dst, n);
   LG_HIP(hipGetLastError());
 }
@@ -811,7 +818,7 @@ __global__ __launch_bounds__(256) void k_gencol(const GenColJob* __restrict__ jo
 
 void launch_gencol(const GenColJob* jobs_dev, int njobs, int total_blocks, int batch, hipStream_t s) {
   if (njobs == 0 || total_blocks == 0) return;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_gencol, dim3(total_blocks, batch), dim3(256), 0, s, jobs_dev, njobs);
   LG_HIP(hipGetLastError());
 }
@@ -1022,7 +1029,7 @@ static void launch_scan_tiled(const uint32_t* const* in, uint32_t* const* out, u
                               int batch, hipStream_t s) {
   if (!tmp) throw std::runtime_error("scan: scratch missing");
   const int nt = n / SCAN_TILE;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_scan_reduce<MUL>, dim3(nt, batch), dim3(SCAN_T), 0, s, in, tmp, n);
   LG_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_scan_tiles<MUL>, dim3(batch), dim3(nt < 1024 ? nt : 1024), 0, s, tmp, tot, nt);
@@ -1034,7 +1041,7 @@ static void launch_scan_tiled(const uint32_t* const* in, uint32_t* const* out, u
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
                           uint32_t* const* gp, int n, int batch, uint32_t* scan_tmp, hipStream_t s) {
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_ratio, dim3((n + T - 1) / T, batch), dim3(T), 0, s, num, den, ratio, n);
   LG_HIP(hipGetLastError());
   if (n >= SCAN_MIN_TILED) {
@@ -1042,7 +1049,7 @@ void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den
     return;
   }
   int T2 = n < 1024 ? n : 1024;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_prefix_product, dim3(batch), dim3(T2), 0, s, (const uint32_t* const*)ratio, gp, n);
   LG_HIP(hipGetLastError());
 }
@@ -1057,7 +1064,7 @@ __global__ void k_fill(uint32_t* const* p, int n, uint32_t v) {
 
 void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s) {
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_fill, dim3((n + T - 1) / T, batch), dim3(T), 0, s, p, n, v);
   LG_HIP(hipGetLastError());
 }
@@ -1106,7 +1113,7 @@ __global__ __launch_bounds__(256) void k_histogram_lds(const uint32_t* const* sr
 void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
                       int batch, hipStream_t s) {
   if (n <= 0 || batch <= 0) return;
-  prelaunch();
+  if (!prelaunch()) return;
   if (bound <= (uint32_t)HIST_LDS_BINS) {
     hipLaunchKernelGGL(k_histogram_lds, dim3((n + HIST_ROWS - 1) / HIST_ROWS, batch), dim3(256), 0, s, src, n, counts,
                        bound, err);
@@ -1127,7 +1134,7 @@ __global__ void k_counts_to_field(uint32_t* const* c, int n) {
 
 void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s) {
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_counts_to_field, dim3((n + T - 1) / T, batch), dim3(T), 0, s, c, n);
   LG_HIP(hipGetLastError());
 }
@@ -1146,7 +1153,7 @@ __global__ __launch_bounds__(256) void k_logup_acc(const uint32_t* const* e, con
 void launch_logup_acc(const uint32_t* const* e, const uint32_t* const* m, uint32_t* const* acc, int n, int first,
                       int batch, hipStream_t s) {
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_logup_acc, dim3((n + T - 1) / T, batch), dim3(T), 0, s, e, m, acc, n, first);
   LG_HIP(hipGetLastError());
 }
@@ -1208,12 +1215,12 @@ void launch_cyclic_logup_finish(const uint32_t* const* acc, uint32_t* const* col
     launch_scan_tiled<false, true>(acc, col, scan_tmp, tot, n, batch, s);
   } else {
     int T2 = n < 1024 ? n : 1024;
-    prelaunch();
+    if (!prelaunch()) return;
     hipLaunchKernelGGL(k_prefix_sum, dim3(batch), dim3(T2), 0, s, acc, col, tot, n);
     LG_HIP(hipGetLastError());
   }
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_logup_finish, dim3((n + T - 1) / T, batch), dim3(T), 0, s, col, (const uint32_t*)tot, n, ninv);
   LG_HIP(hipGetLastError());
 }
@@ -1244,7 +1251,7 @@ void launch_quot_combine(const uint32_t* Y, uint32_t* coef, const uint32_t* ginv
                          int batch, hipStream_t s) {
   if (rho > 8) throw std::runtime_error("quotient: more than 8 cosets");
   const int T = n < 256 ? n : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_quot_combine, dim3((n + T - 1) / T, 6, batch), dim3(T), 0, s, Y, coef, ginv, wr, rho, n);
   LG_HIP(hipGetLastError());
 }
@@ -1267,7 +1274,7 @@ __global__ void k_lagrange_weights(const uint32_t* zeta, const uint32_t* common,
 void launch_lagrange_weights(const uint32_t* zeta, const uint32_t* common, const uint32_t* wpow, uint32_t* const* W,
                              int n, int batch, hipStream_t s) {
   const int T = 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_lagrange_weights, dim3((n + T - 1) / T, batch), dim3(T), 0, s, zeta, common, wpow, W, n);
   LG_HIP(hipGetLastError());
 }
@@ -1347,7 +1354,7 @@ void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int
   if (n_sets == 0 || n_items == 0) return;
   const int S = claims_split(n);
   if (S > 1 && !part) throw std::runtime_error("launch_claims: partial-sum buffer missing");
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_claims, dim3(n_sets, S), dim3(256), 0, s, sets, items, n, part);
   LG_HIP(hipGetLastError());
   if (S > 1) {
@@ -1417,7 +1424,7 @@ void launch_deep(const DeepDesc& d, int batch, hipStream_t s) {
   if (d.n_points < 1 || d.n_points > DEEP_MAX_POINTS) throw std::runtime_error("deep: 1..8 distinct opening points");
   const int T = 256;
   const dim3 g((d.n + T - 1) / T, batch);
-  prelaunch();
+  if (!prelaunch()) return;
   switch (d.n_points) {
     case 1: hipLaunchKernelGGL(k_deep<1>, g, dim3(T), 0, s, d); break;
     case 2: hipLaunchKernelGGL(k_deep<2>, g, dim3(T), 0, s, d); break;
@@ -1535,7 +1542,7 @@ void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const
                           hipStream_t s) {
   if (half < 2 || (half & 1)) throw std::runtime_error("fri fold+leaf: layer length must be even");
   const int pairs = half / 2;
-  prelaunch();
+  if (!prelaunch()) return;
   if (pairs <= FRI_COOP_MAX_PAIRS) {
     // rows of 16 lanes, 16 rows per block; rows past the layer return whole
     hipLaunchKernelGGL(k_fri_fold_leaf_coop, dim3((pairs + 15) / 16, batch), dim3(256), 0, s, in, out, alpha, xinv2,
@@ -1551,7 +1558,7 @@ void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const
 void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
                      int half, int batch, hipStream_t s) {
   const int T = half < 256 ? half : 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_fri_fold, dim3((half + T - 1) / T, batch), dim3(T), 0, s, in, out, alpha, xinv2, half);
   LG_HIP(hipGetLastError());
 }
@@ -1645,7 +1652,7 @@ void launch_deep_alpha(const DeepAlphaArgs& a, const uint32_t* prefix, int n_pre
   for (int i = 0; i < n_prefix; ++i) pre.w[i] = prefix[i];
   DeepAlphaArgs args = a;
   args.n_pre = n_prefix;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_deep_alpha, dim3((batch + 3) / 4), dim3(64), 0, s, args, pre, batch);
   LG_HIP(hipGetLastError());
 }
@@ -1691,7 +1698,7 @@ __global__ __launch_bounds__(64) void k_fs_challenge(const FsSeg* __restrict__ s
 void launch_fs_challenge(const FsSeg* seg_dev, int nseg, int len, uint32_t* dig, uint32_t* e6, int e6_stride,
                          int batch, hipStream_t s) {
   if (nseg < 1 || len < 1) throw std::runtime_error("fs challenge: empty message");
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_fs_challenge, dim3((batch + 3) / 4), dim3(64), 0, s, seg_dev, nseg, len, dig, e6, e6_stride,
                      batch);
   LG_HIP(hipGetLastError());
@@ -1708,7 +1715,7 @@ __global__ void k_chal_compact(const uint32_t* __restrict__ chal, int ncols, con
 void launch_chal_compact(const uint32_t* chal, int ncols, const uint32_t* ids, int nch, uint32_t* out, int batch,
                          hipStream_t s) {
   if (nch == 0) return;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_chal_compact, dim3(batch), dim3(64), 0, s, chal, ncols, ids, nch, out);
   LG_HIP(hipGetLastError());
 }
@@ -1724,7 +1731,7 @@ __global__ void k_scale_into(const uint32_t* __restrict__ tot, uint32_t s, uint3
 
 void launch_scale_into(const uint32_t* tot, uint32_t s_mont, uint32_t* chal, int ncols, int col, int batch,
                        hipStream_t s) {
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_scale_into, dim3((batch * 6 + 255) / 256), dim3(256), 0, s, tot, s_mont, chal, ncols, col,
                      batch);
   LG_HIP(hipGetLastError());
@@ -1749,7 +1756,7 @@ __global__ void k_zeta_consts(const uint32_t* __restrict__ zdig, const uint32_t*
 void launch_zeta_consts(const uint32_t* zdig, const uint32_t* sizes, const uint32_t* ninv, int nmod, uint32_t* z,
                         uint32_t* cm, int batch, hipStream_t s) {
   if (nmod == 0) return;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_zeta_consts, dim3((batch * nmod + 63) / 64), dim3(64), 0, s, zdig, sizes, ninv, nmod, z, cm,
                      batch);
   LG_HIP(hipGetLastError());
@@ -1769,7 +1776,7 @@ __global__ void k_zeta_points(const uint32_t* __restrict__ zdig, const uint32_t*
 
 void launch_zeta_points(const uint32_t* zdig, const uint32_t* w, int np, uint32_t* out, int batch, hipStream_t s) {
   if (np == 0) return;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_zeta_points, dim3((batch * np + 63) / 64), dim3(64), 0, s, zdig, w, np, out, batch);
   LG_HIP(hipGetLastError());
 }
@@ -1779,7 +1786,7 @@ void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* 
   if (n_prefix < 1 || n_prefix > CHAL_MAX_PREFIX) throw std::runtime_error("fri challenge: bad prefix length");
   ChalPrefix pre{};
   for (int i = 0; i < n_prefix; ++i) pre.w[i] = prefix[i];
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_fri_challenge, dim3((batch + 3) / 4), dim3(64), 0, s, pre, n_prefix, prev, trees, dig, alpha,
                      roots, batch);
   LG_HIP(hipGetLastError());
@@ -1797,7 +1804,7 @@ __global__ void k_fri_combine(uint32_t* const* running, const uint32_t* const* l
 void launch_fri_combine(uint32_t* const* running, const uint32_t* const* level, const uint32_t* gamma, int len,
                         int batch, hipStream_t s) {
   const int T = 256;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_fri_combine, dim3((len + T - 1) / T, batch), dim3(T), 0, s, running, level, gamma, len);
   LG_HIP(hipGetLastError());
 }
@@ -1843,7 +1850,7 @@ __global__ __launch_bounds__(GATHER_T) void k_gather(const GatherOp* __restrict_
 void launch_gather(const GatherOp* ops_dev, int n_ops, int total_blocks, const uint32_t* pos, int nq, int b0, int nb,
                    uint32_t* out, int stride, hipStream_t s) {
   if (n_ops == 0 || total_blocks == 0 || nb == 0) return;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_gather, dim3(total_blocks), dim3(GATHER_T), 0, s, ops_dev, n_ops, pos, nq, b0, nb, out, stride);
   LG_HIP(hipGetLastError());
 }
@@ -1866,7 +1873,7 @@ __global__ void k_gather_rows(const uint32_t* pos, int nq, int shift, const uint
 void launch_gather_rows(const uint32_t* pos, int nq, int shift, const uint32_t* const* cols, int ncols, uint32_t* out,
                         int stride, int off, int batch, hipStream_t s) {
   const int T = 64;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_gather_rows, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, cols, ncols, out,
                      stride, off);
   LG_HIP(hipGetLastError());
@@ -1891,7 +1898,7 @@ __global__ void k_gather_path(const uint32_t* pos, int nq, int shift, const uint
 void launch_gather_path(const uint32_t* pos, int nq, int shift, const uint32_t* const* trees, int nleaves,
                         uint32_t* out, int stride, int off, int batch, hipStream_t s) {
   const int T = 64;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_gather_path, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, trees, nleaves, out,
                      stride, off);
   LG_HIP(hipGetLastError());
@@ -1913,7 +1920,7 @@ __global__ void k_gather_digest(const uint32_t* pos, int nq, int shift, int d, c
 void launch_gather_digest(const uint32_t* pos, int nq, int shift, int d, const uint32_t* const* inj, uint32_t* out,
                           int stride, int off, int batch, hipStream_t s) {
   const int T = 64;
-  prelaunch();
+  if (!prelaunch()) return;
   hipLaunchKernelGGL(k_gather_digest, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, d, inj, out,
                      stride, off);
   LG_HIP(hipGetLastError());

@@ -242,7 +242,7 @@ void Pool::trim(size_t keep_bytes) {
 }
 
 void Pool::put(DevBuf&& b) {
-  if (!b.bytes()) return;
+  if (!b.bytes() || !b.owned()) return;  // views of arena memory go back to their arena
   std::lock_guard<std::mutex> lock(mu_);
   size_t n = b.bytes();
   cached_ += n;
@@ -268,6 +268,8 @@ void* Staging::put(const void* src, size_t n, hipStream_t) {
   return dev_.as<uint8_t>() + off;
 }
 
+void Staging::assume_flushed() { flushed_ = off_; }
+
 void Staging::flush(hipStream_t s) {
   if (off_ > flushed_) {
     LG_HIP(hipMemcpyAsync(dev_.as<uint8_t>() + flushed_, host_ + flushed_, off_ - flushed_, hipMemcpyHostToDevice, s));
@@ -277,6 +279,7 @@ void Staging::flush(hipStream_t s) {
 
 Context::Context() {
   if (const char* e = getenv("LOOMGPU_LANES")) lanes = std::max(1, std::min(8, atoi(e)));
+  if (const char* e = getenv("LOOMGPU_GRAPHS")) graphs = atoi(e) != 0;
   lane_.resize(8);
 }
 
@@ -320,12 +323,20 @@ uint32_t* Context::pinned_fs(size_t bytes, int lane) {
   return grow_pinned(L.pinned_fs, L.pinned_fs_cap, bytes);
 }
 
+uint32_t* Context::pinned_ext(size_t bytes, int lane, size_t idx) {
+  LaneBufs& L = lane_.at(lane);
+  if (L.ext.size() <= idx) L.ext.resize(idx + 1, {nullptr, 0});
+  return grow_pinned(L.ext[idx].first, L.ext[idx].second, bytes);
+}
+
 Context::~Context() {
   for (auto& L : lane_) {
     if (L.pinned) (void)hipHostFree(L.pinned);
     if (L.pinned_in) (void)hipHostFree(L.pinned_in);
     if (L.pinned_aux) (void)hipHostFree(L.pinned_aux);
     if (L.pinned_fs) (void)hipHostFree(L.pinned_fs);
+    for (auto& e : L.ext)
+      if (e.first) (void)hipHostFree(e.first);
     if (L.stream) {
       (void)hipStreamSynchronize(L.stream);
       (void)hipStreamDestroy(L.stream);
@@ -439,6 +450,72 @@ Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_
     }
   for (auto& x : d_.exposed)
     if (!logup_total[x.col]) throw std::runtime_error("plan: exposed value is not a logup total");
+  graph_eligible_ = deep_classes(d_).size() == 1;
+}
+
+GraphBucket::~GraphBucket() {
+  if (exec) (void)hipGraphExecDestroy(exec);
+  for (auto& p : pins) (void)hipHostFree(p.first);
+}
+
+// The bucket a lane's Job of `batch` instances uses, and its mode (Job::Mode):
+// the first prove of a shape runs eagerly (and creates every lazily built
+// table), the second records the graph, later ones replay it.  Without a
+// bucket (ineligible plan, graphs off, per-kernel timing or diagnostics on,
+// a failed recording) the prove is eager.
+GraphBucket* Plan::take_bucket(int batch, int lane, bool device_inputs, int* mode) {
+  *mode = 0;
+  static const bool diag = getenv("LOOMGPU_TRACE") || getenv("LOOMGPU_DEBUG_DIR");
+  if (!graph_eligible_ || !ctx_->graphs || ctx_->timer.on || diag) return nullptr;
+  std::lock_guard<std::mutex> lk(gmu_);
+  auto key = std::make_tuple(batch, lane, device_inputs);
+  auto it = buckets_.find(key);
+  if (it == buckets_.end()) {
+    while ((int)buckets_.size() >= MAX_GRAPH_BUCKETS) {
+      auto lru = buckets_.end();
+      for (auto i = buckets_.begin(); i != buckets_.end(); ++i)
+        if (!i->second->busy && (lru == buckets_.end() || i->second->last < lru->second->last)) lru = i;
+      if (lru == buckets_.end()) return nullptr;
+      buckets_.erase(lru);
+    }
+    auto b = std::make_unique<GraphBucket>();
+    b->batch = batch;
+    b->lane = lane;
+    it = buckets_.emplace(key, std::move(b)).first;
+  }
+  GraphBucket* b = it->second.get();
+  b->last = ++tick_;
+  if (b->broken || b->busy) return nullptr;
+  if (++b->proves == 1) return nullptr;
+  b->busy = true;
+  *mode = b->exec ? 2 : 1;
+  return b;
+}
+
+void Plan::give_bucket(GraphBucket* b) {
+  if (!b) return;
+  std::lock_guard<std::mutex> lk(gmu_);
+  b->busy = false;
+  if (b->broken) {  // free what a failed recording or replay held
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->bufs.clear();
+    b->bytes = 0;
+  }
+}
+
+size_t Plan::graph_bytes() {
+  std::lock_guard<std::mutex> lk(gmu_);
+  size_t n = 0;
+  for (auto& kv : buckets_) n += kv.second->bytes;
+  return n;
+}
+
+void Plan::drop_graphs() {
+  std::lock_guard<std::mutex> lk(gmu_);
+  for (auto it = buckets_.begin(); it != buckets_.end();)
+    if (it->second->busy) ++it;
+    else it = buckets_.erase(it);
 }
 
 // ------------------------------------------------------------------ job
@@ -505,19 +582,41 @@ struct Layer {  // an E6 vector of len elements per instance, 6 limb arrays
 
 class Job {
  public:
+  // With a graph bucket the Job's device buffers, staging area and pinned
+  // buffers are the bucket's (the same addresses on every prove of that
+  // shape), and its device phase is recorded into the bucket's HIP graph
+  // (mode CAPTURE) or replayed from it (mode REPLAY): in REPLAY the host
+  // logic runs unchanged but launches nothing until region_end launches the
+  // graph.
+  enum Mode { EAGER, CAPTURE, REPLAY };
   Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb,
-      int lane)
+      int lane, GraphBucket* gb = nullptr, Mode mode = EAGER)
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
-        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)) {
+        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER) {
     if (out.size() != in.size()) throw std::runtime_error("prove: output slots do not match the inputs");
-    stg_ = ctx_.take_staging();
+    if (gb_) {
+      if (!gb_->stg) gb_->stg = std::make_unique<Staging>(64u << 20);
+      gb_->stg->reset();
+    } else {
+      stg_ = ctx_.take_staging();
+    }
     set_prelaunch_hook([](void* u) { static_cast<Job*>(u)->flush_staging(); }, this);
   }
+  Staging& stg() { return gb_ ? *gb_->stg : *stg_; }
   void flush_staging() {
-    stg_->flush(s_);
+    stg().flush(s_);
     ctx_.timer.launch_point(s_);
   }
   ~Job() {
+    if (in_region_) {  // an error inside the device phase: leave capture / suppression
+      set_launch_suppression(false);
+      if (mode_ == CAPTURE) {
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(s_, &g);
+        if (g) (void)hipGraphDestroy(g);
+      }
+      if (gb_) gb_->broken = true;
+    }
     set_prelaunch_hook(nullptr, nullptr);
     // after an error work may still be queued on the stream: drain it before
     // the buffers go back to the shared pool
@@ -531,15 +630,81 @@ class Job {
     }
     for (auto& x : keep_) ctx_.pool.put(std::move(x));
     ctx_.pool.put(std::move(in_arena_));
+    ctx_.pool.put(std::move(pre_fold_.dev));
     if (stg_) ctx_.give_staging(std::move(stg_));
   }
 
   void run();
 
  private:
-  DevBuf dev(size_t words) { return ctx_.pool.get(words * sizeof(uint32_t)); }
+  DevBuf dev(size_t words) {
+    const size_t bytes = words * sizeof(uint32_t);
+    if (!gb_) return ctx_.pool.get(bytes);
+    if (next_buf_ == gb_->bufs.size()) {
+      if (mode_ == REPLAY) throw std::runtime_error("graph bucket: allocation sequence changed");
+      gb_->bufs.emplace_back(bytes);
+      gb_->bytes += bytes;
+    }
+    DevBuf& b = gb_->bufs[next_buf_++];
+    if (b.bytes() != bytes) throw std::runtime_error("graph bucket: allocation sequence changed");
+    return DevBuf::view(b.as<void>(), bytes);
+  }
+  // Pinned host memory for device -> host copies of the device phase (the
+  // bucket's, in graph mode: a recorded copy writes to a fixed address).
+  uint32_t* pin(size_t bytes) {
+    if (!gb_) {
+      uint32_t* p = ctx_.pinned_ext(bytes, lane_, n_pins_);
+      ++n_pins_;
+      return p;
+    }
+    if (n_pins_ == gb_->pins.size()) {
+      if (mode_ == REPLAY) throw std::runtime_error("graph bucket: pinned sequence changed");
+      void* p = nullptr;
+      LG_HIP(hipHostMalloc(&p, bytes, hipHostMallocDefault));
+      gb_->pins.push_back({static_cast<uint32_t*>(p), bytes});
+    }
+    auto& e = gb_->pins[n_pins_++];
+    if (e.second != bytes) throw std::runtime_error("graph bucket: pinned sequence changed");
+    return e.first;
+  }
+  // hipMemcpyAsync on the Job's stream, not issued while launches are
+  // suppressed (the recorded graph holds it).
+  void copy_async(void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (launch_suppressed() || !bytes) return;
+    LG_HIP(hipMemcpyAsync(dst, src, bytes, kind, s_));
+  }
+  // REPLAY launches the graph at once: the host logic of the region then only
+  // re-derives what the prove needs afterwards and rewrites the bucket's
+  // staging area with the bytes it already holds (the bucket is this Job's
+  // alone, and every staged word of the device phase is determined by the
+  // plan, the batch size and the bucket's addresses).
+  void region_begin() {
+    stg().flush(s_);
+    in_region_ = true;
+    if (mode_ == CAPTURE) LG_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
+    if (mode_ == REPLAY) {
+      LG_HIP(hipGraphLaunch(gb_->exec, s_));
+      set_launch_suppression(true);
+    }
+  }
+  void region_end() {
+    if (!in_region_) return;
+    if (mode_ == CAPTURE) {
+      hipGraph_t g = nullptr;
+      LG_HIP(hipStreamEndCapture(s_, &g));
+      const hipError_t e = hipGraphInstantiate(&gb_->exec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      LG_HIP(e);
+    }
+    if (mode_ == REPLAY) {
+      set_launch_suppression(false);
+      stg().assume_flushed();  // the graph copies what the host logic staged
+    }
+    in_region_ = false;
+    if (mode_ == CAPTURE) LG_HIP(hipGraphLaunch(gb_->exec, s_));
+  }
   template <class T>
-  const T* stage(const std::vector<T>& v) { return stg_->put(v, s_); }
+  const T* stage(const std::vector<T>& v) { return stg().put(v, s_); }
   const uint32_t* const* pres(const NttCfg& c) {
     std::vector<const uint32_t*> v(c.pre, c.pre + NTT_MAX_OUT);
     return stage(v);
@@ -553,6 +718,10 @@ class Job {
   std::vector<std::pair<std::string, double>> marks_;
   double t0_run_ = 0;
   void phase(const char* name, double& t0) {
+    if (cb_ && in_region_) {  // a recorded device phase is timed as a whole (pcs-open)
+      cb_(name, 0.0);
+      return;
+    }
     if (cb_) {
       sync();
       double t = now_s();
@@ -619,6 +788,17 @@ class Job {
   int lane_;
   hipStream_t s_;
   std::unique_ptr<Staging> stg_;
+  GraphBucket* gb_;
+  Mode mode_;
+  bool in_region_ = false;
+  size_t next_buf_ = 0, n_pins_ = 0;
+  // the fold rounds before the level enters, chained on the device
+  struct {
+    DevBuf dev;
+    const uint32_t* prev = nullptr;
+    int n = 0;
+  } pre_fold_;
+  size_t fold_pos_ = 0;  // schedule position of fri_fold_0
 
   std::vector<ColBuf> cols_;
   DevBuf in_arena_;                                 // uploaded trace columns + permutations
@@ -856,7 +1036,7 @@ namespace {
 
 void Job::ntt(const NttCfg& cfg, const std::vector<NttJob>& jobs) {
   run_ntt(
-      tb_, cfg, jobs, s_, [&](const void* p, size_t bytes) -> const void* { return stg_->put(p, bytes, s_); },
+      tb_, cfg, jobs, s_, [&](const void* p, size_t bytes) -> const void* { return stg().put(p, bytes, s_); },
       [&](size_t words) {
         keep_.push_back(dev(words));
         return keep_.back().as<uint32_t>();
@@ -1066,8 +1246,7 @@ void Job::commit_tree(size_t t) {
     // the roots come back with the transcript data (replay_device_fs)
     DevBuf r = dev((size_t)B_ * 8);
     launch_copy_roots(trees_[t].nodes_d, r.as<uint32_t>(), B_, s_);
-    LG_HIP(hipMemcpyAsync(fs_h_ + fs_roots_off_ + t * (size_t)B_ * 8, r.as<void>(), (size_t)B_ * 32,
-                          hipMemcpyDeviceToHost, s_));
+    copy_async(fs_h_ + fs_roots_off_ + t * (size_t)B_ * 8, r.as<void>(), (size_t)B_ * 32, hipMemcpyDeviceToHost);
     keep_.push_back(std::move(r));
   }
   debug_dump(("tree" + std::to_string(t)).c_str(), ids, cols_, s_);
@@ -1213,8 +1392,8 @@ void Job::step_logup(size_t r, size_t s) {
 // round (exposed values, and the constant column T/N the relations read).
 void Job::flush_err_flags() {
   for (auto& e : pending_err_) {
-    LG_HIP(hipMemcpyAsync(fs_h_ + fs_err_off_ + err_msg_.size() * (size_t)B_, e.second.as<void>(), (size_t)B_ * 4,
-                          hipMemcpyDeviceToHost, s_));
+    copy_async(fs_h_ + fs_err_off_ + err_msg_.size() * (size_t)B_, e.second.as<void>(), (size_t)B_ * 4,
+               hipMemcpyDeviceToHost);
     err_msg_.push_back(e.first);
     keep_.push_back(std::move(e.second));
   }
@@ -1228,8 +1407,8 @@ void Job::finish_round_steps(size_t r) {
     const uint32_t ninv = kb_inv(M(d_.modules[d_.cols[col].module].N));
     launch_scale_into(kv.second.as<uint32_t>(), ninv, chal_dev_.as<uint32_t>(), (int)d_.cols.size(), (int)col, B_,
                       s_);
-    LG_HIP(hipMemcpyAsync(fs_h_ + fs_tot_off_ + (size_t)col * B_ * 6, kv.second.as<void>(), (size_t)B_ * 24,
-                          hipMemcpyDeviceToHost, s_));
+    copy_async(fs_h_ + fs_tot_off_ + (size_t)col * B_ * 6, kv.second.as<void>(), (size_t)B_ * 24,
+               hipMemcpyDeviceToHost);
     total_ptr_[col] = kv.second.as<uint32_t>();
     total_cols_.push_back(col);
     keep_.push_back(std::move(kv.second));
@@ -1286,7 +1465,7 @@ void Job::round_challenge(size_t r) {
   uint32_t* dig = fs_dig_.as<uint32_t>() + r * (size_t)B_ * 8;
   launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, dig,
                       chal_dev_.as<uint32_t>() + (size_t)rd.challenge_col * 6, (int)d_.cols.size() * 6, B_, s_);
-  LG_HIP(hipMemcpyAsync(fs_h_ + r * (size_t)B_ * 8, dig, (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
+  copy_async(fs_h_ + r * (size_t)B_ * 8, dig, (size_t)B_ * 32, hipMemcpyDeviceToHost);
 }
 
 // __zeta (prover/prover.go:545-554): previous digest (or the init words when there
@@ -1302,7 +1481,7 @@ void Job::zeta_challenge() {
     if (d_.trees[t].kind == 1) mb.add(trees_[t].node_ptr(0) + 8, 8, (uint32_t)(16 * trees_[t].nleaves));
   uint32_t* dig = fs_dig_.as<uint32_t>() + nr * (size_t)B_ * 8;
   launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, dig, nullptr, 0, B_, s_);
-  LG_HIP(hipMemcpyAsync(fs_h_ + nr * (size_t)B_ * 8, dig, (size_t)B_ * 32, hipMemcpyDeviceToHost, s_));
+  copy_async(fs_h_ + nr * (size_t)B_ * 8, dig, (size_t)B_ * 32, hipMemcpyDeviceToHost);
 }
 
 // Host side of the device challenges, once the stream has passed their
@@ -1506,8 +1685,14 @@ void Job::run() {
     for (int b = 0; b < B_; ++b) ts_[b].bind(init, fs_init_);
   }
   chal_dev_ = dev((size_t)B_ * d_.cols.size() * 6);
-  LG_HIP(hipMemsetAsync(chal_dev_.as<void>(), 0, (size_t)B_ * d_.cols.size() * 24, s_));
   fs_dig_ = dev((nr + 1) * (size_t)B_ * 8);
+  fold_pos_ = plan_.schedule()->pos.at("fri_fold_0");
+  region_begin();  // the device phase: from the generated columns to the FRI commit phase
+  {
+    // zeroed with a fill kernel (a recorded graph replays it like any launch)
+    std::vector<uint32_t*> cp{chal_dev_.as<uint32_t>()};
+    launch_fill(const_cast<uint32_t* const*>(stage(cp)), (int)((size_t)B_ * d_.cols.size() * 6), 0u, 1, s_);
+  }
   {
     size_t n_mult = 0;
     for (auto& rd : d_.rounds)
@@ -1515,7 +1700,7 @@ void Job::run() {
     fs_roots_off_ = (nr + 1) * (size_t)B_ * 8;
     fs_tot_off_ = fs_roots_off_ + d_.trees.size() * (size_t)B_ * 8;
     fs_err_off_ = fs_tot_off_ + d_.cols.size() * (size_t)B_ * 6;
-    fs_h_ = ctx_.pinned_fs((fs_err_off_ + (n_mult + 1) * (size_t)B_) * 4, lane_);
+    fs_h_ = pin((fs_err_off_ + (n_mult + 1) * (size_t)B_) * 4);
   }
   gencol();
   trace("gencol");
@@ -1686,12 +1871,6 @@ void Job::open() {
     });
   }
   trace("claimed values + alpha");
-  struct EventHolder {
-    hipEvent_t e = nullptr;
-    ~EventHolder() {
-      if (e) (void)hipEventDestroy(e);
-    }
-  } alpha_ready;
   uint32_t* alpha_h = nullptr;  // pinned: [b][8] digests, then the claimed values
   // ---- DEEP quotient per class, RS-encoded (deep.go:71-266)
   std::vector<Layer> level(L);
@@ -1764,12 +1943,33 @@ void Job::open() {
       da.consts = cs.as<uint32_t>();
       const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, "alpha_DEEP");
       launch_deep_alpha(da, pre.data(), (int)pre.size(), B_, s_);
-      alpha_h = ctx_.pinned_aux(((size_t)B_ * 8 + (size_t)B_ * ncv * 6) * 4, lane_);
-      LG_HIP(hipMemcpyAsync(alpha_h, dig.as<void>(), (size_t)B_ * 8 * 4, hipMemcpyDeviceToHost, s_));
-      LG_HIP(hipMemcpyAsync(alpha_h + (size_t)B_ * 8, cvd.as<void>(), (size_t)B_ * ncv * 6 * 4, hipMemcpyDeviceToHost,
-                            s_));
-      LG_HIP(hipEventCreateWithFlags(&alpha_ready.e, hipEventDisableTiming));
-      LG_HIP(hipEventRecord(alpha_ready.e, s_));
+      // The fold rounds before the level enters bind the zero digest
+      // (fri.go:630-640): their digests are chained on the device too, so
+      // nothing before the query phase waits for the host.
+      const int first_j = FRI_LOG_D - m.log_n;
+      pre_fold_.dev = dev((size_t)std::max(1, first_j) * B_ * 8);
+      {
+        static const std::vector<uint32_t> zero8(8, 0u);
+        const uint32_t* z = stage(zero8);
+        const uint32_t* prev = dig.as<uint32_t>();
+        for (int j = 0; j < first_j; ++j) {
+          const std::vector<uint32_t>& pf = plan_.schedule()->prefix[fold_pos_ + j];
+          FsMsgBuilder mb;
+          mb.add(stage(pf), (uint32_t)pf.size(), 0);
+          mb.add(prev, 8, 8);
+          mb.add(z, 8, 0);
+          uint32_t* out = pre_fold_.dev.as<uint32_t>() + (size_t)j * B_ * 8;
+          launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, out, nullptr, 0, B_, s_);
+          prev = out;
+        }
+        pre_fold_.prev = prev;
+        pre_fold_.n = first_j;
+      }
+      alpha_h = pin(((size_t)B_ * 8 + (size_t)B_ * ncv * 6 + (size_t)std::max(1, first_j) * B_ * 8) * 4);
+      copy_async(alpha_h, dig.as<void>(), (size_t)B_ * 8 * 4, hipMemcpyDeviceToHost);
+      copy_async(alpha_h + (size_t)B_ * 8, cvd.as<void>(), (size_t)B_ * ncv * 6 * 4, hipMemcpyDeviceToHost);
+      copy_async(alpha_h + (size_t)B_ * 8 + (size_t)B_ * ncv * 6, pre_fold_.dev.as<void>(),
+                 (size_t)first_j * B_ * 8 * 4, hipMemcpyDeviceToHost);
       scale_d = sc.as<uint32_t>();
       consts_d = cs.as<uint32_t>();
       keep_.push_back(std::move(dig));
@@ -1788,8 +1988,8 @@ void Job::open() {
           a = e6_mul(a, alpha[b]);
         }
       }
-      scale_d = reinterpret_cast<const uint32_t*>(stg_->put(scale.data(), scale.size() * sizeof(E6), s_));
-      consts_d = reinterpret_cast<const uint32_t*>(stg_->put(consts.data(), consts.size() * sizeof(E6), s_));
+      scale_d = reinterpret_cast<const uint32_t*>(stg().put(scale.data(), scale.size() * sizeof(E6), s_));
+      consts_d = reinterpret_cast<const uint32_t*>(stg().put(consts.data(), consts.size() * sizeof(E6), s_));
     }
     Layer dq;
     dq.len = m.N;
@@ -1861,18 +2061,24 @@ void Job::open() {
   const int first = FRI_LOG_D - d_.modules[classes[lv_order[0]]].log_n;
   TreeBuf levels_tree;
   ext_tree(levels_tree, lvp);
-  if (deep_on_device) {
-    // the claimed values and alpha_DEEP come back while the DEEP codeword and
-    // the levels tree are computed; replay them into the transcripts
-    LG_HIP(hipEventSynchronize(alpha_ready.e));
+  // With one DEEP class the host replays the claimed values, alpha_DEEP and
+  // the fold rounds before the level after the FRI commit phase (below):
+  // until then nothing the device does waits for the host.
+  auto replay_alpha = [&] {
     replay_device_fs();
     const uint32_t* cv_h = alpha_h + (size_t)B_ * 8;
+    const uint32_t* pre_h = cv_h + (size_t)B_ * ncv * 6;
     std::memcpy(cvals.data(), cv_h, cvals.size() * sizeof(E6));
     parallel_for(B_, [&](int b) {
       for (int32_t k : bind_order) ts_[b].bind("alpha_DEEP", cvals[(size_t)b * ncv + k].v, 6);
       ts_[b].check("alpha_DEEP", alpha_h + (size_t)8 * b);
+      static const uint32_t zero[8] = {0};
+      for (int j = 0; j < pre_fold_.n; ++j) {
+        ts_[b].bind_at(fold_pos_ + j, zero, 8);
+        ts_[b].check_at(fold_pos_ + j, pre_h + ((size_t)j * B_ + b) * 8);
+      }
     });
-  }
+  };
   keep_.push_back(std::move(cvd));
   // With one FRI level nothing on the host needs the levels root before the
   // fold rounds: round `first` then derives its challenge on the device like
@@ -1881,20 +2087,21 @@ void Job::open() {
   std::vector<Digest> levels_root(B_);
   if (first_on_host) levels_root = roots_of(levels_tree);
   std::vector<std::vector<E6>> gamma(L, std::vector<E6>(B_));
-  parallel_for(B_, [&](int b) {
-    for (size_t l = 1; l < L; ++l) {
-      const std::string nmg = "fri_level_" + std::to_string(l) + "_gamma";
-      if (l == 1) ts_[b].bind(nmg, levels_root[b].v, 8);
-      const Digest& dg = ts_[b].compute(nmg);
-      for (int k = 0; k < 6; ++k) gamma[l][b].v[k] = dg.v[k];
-    }
-    for (int j = 0; j < first; ++j) {
-      const std::string nf = "fri_fold_" + std::to_string(j);
-      uint32_t zero[8] = {0};
-      ts_[b].bind(nf, zero, 8);
-      ts_[b].compute(nf);
-    }
-  });
+  if (first_on_host)
+    parallel_for(B_, [&](int b) {
+      for (size_t l = 1; l < L; ++l) {
+        const std::string nmg = "fri_level_" + std::to_string(l) + "_gamma";
+        if (l == 1) ts_[b].bind(nmg, levels_root[b].v, 8);
+        const Digest& dg = ts_[b].compute(nmg);
+        for (int k = 0; k < 6; ++k) gamma[l][b].v[k] = dg.v[k];
+      }
+      for (int j = 0; j < first; ++j) {
+        const std::string nf = "fri_fold_" + std::to_string(j);
+        uint32_t zero[8] = {0};
+        ts_[b].bind(nf, zero, 8);
+        ts_[b].compute(nf);
+      }
+    });
   std::vector<Layer> layers(FRI_ROUNDS + 1);
   std::vector<TreeBuf> ftrees(FRI_ROUNDS);
   std::vector<std::vector<Digest>> fri_roots;
@@ -1910,8 +2117,8 @@ void Job::open() {
   for (size_t l = 1; l < L; ++l) joins[FRI_LOG_D - d_.modules[classes[lv_order[l]]].log_n] = 1;
   layers[first].len = lvp[0]->len;
   layers[first].buf = dev((size_t)B_ * 6 * layers[first].len);
-  LG_HIP(hipMemcpyAsync(layers[first].buf.as<void>(), lvp[0]->buf.as<void>(), (size_t)B_ * 6 * layers[first].len * 4,
-                        hipMemcpyDeviceToDevice, s_));
+  copy_async(layers[first].buf.as<void>(), lvp[0]->buf.as<void>(), (size_t)B_ * 6 * layers[first].len * 4,
+             hipMemcpyDeviceToDevice);
   for (int j = first + 1; j <= FRI_ROUNDS; ++j) {
     layers[j].len = layers[j - 1].len / 2;
     layers[j].buf = dev((size_t)B_ * 6 * layers[j].len);
@@ -1951,7 +2158,7 @@ void Job::open() {
     std::vector<const uint32_t*> lp(B_);
     for (int b = 0; b < B_; ++b) lp[b] = lvp[l]->ptr(b);
     level_d[l] = stage(lp);
-    gamma_d[l] = reinterpret_cast<const uint32_t*>(stg_->put(gamma[l].data(), gamma[l].size() * sizeof(E6), s_));
+    gamma_d[l] = reinterpret_cast<const uint32_t*>(stg().put(gamma[l].data(), gamma[l].size() * sizeof(E6), s_));
   }
   const size_t fb = (size_t)B_ * 8;
   DevBuf chal_dig = dev((size_t)FRI_ROUNDS * fb);
@@ -1963,18 +2170,15 @@ void Job::open() {
   const uint32_t* alpha_first = nullptr;
   const std::string first_name = "fri_fold_" + std::to_string(first);
   if (!first_on_host) {
-    std::vector<uint32_t> pv((size_t)B_ * 8);
-    for (int b = 0; b < B_; ++b) {
-      const Digest& d = ts_[b].previous_value(first_name);
-      for (int k = 0; k < 8; ++k) pv[(size_t)8 * b + k] = d.v[k];
-    }
+    // the previous digest is the device's (alpha_DEEP or the last zero-bound
+    // fold round, chained above)
     std::vector<uint32_t*> tp(B_);
     for (int b = 0; b < B_; ++b) tp[b] = levels_tree.node_ptr(b);
     uint32_t* dig_f = chal_dig.as<uint32_t>() + (size_t)first * fb;
     uint32_t* al_f = chal_alpha.as<uint32_t>() + (size_t)first * B_ * 6;
     const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, first_name);
-    launch_fri_challenge(pre.data(), (int)pre.size(), stage(pv), const_cast<uint32_t* const*>(stage(tp)), dig_f, al_f,
-                         root_dev.as<uint32_t>() + (size_t)first * fb, B_, s_);
+    launch_fri_challenge(pre.data(), (int)pre.size(), pre_fold_.prev, const_cast<uint32_t* const*>(stage(tp)), dig_f,
+                         al_f, root_dev.as<uint32_t>() + (size_t)first * fb, B_, s_);
     prev_dig = dig_f;
     alpha_first = al_f;
   } else {
@@ -2027,11 +2231,19 @@ void Job::open() {
     }
   }
   Layer running = std::move(layers[FRI_ROUNDS]);
+  // final polynomial (running after the last round): 4 E6 values
+  const uint32_t flen = running.len;
+  const size_t fw_words = (size_t)B_ * 6 * flen;
+  uint32_t* fri_h = pin((2 * (size_t)FRI_ROUNDS * fb + fw_words) * 4);  // digests, roots, final polynomial
+  copy_async(fri_h, chal_dig.as<void>(), (size_t)FRI_ROUNDS * fb * 4, hipMemcpyDeviceToHost);
+  copy_async(fri_h + FRI_ROUNDS * fb, root_dev.as<void>(), (size_t)FRI_ROUNDS * fb * 4, hipMemcpyDeviceToHost);
+  copy_async(fri_h + 2 * FRI_ROUNDS * fb, running.buf.as<void>(), fw_words * 4, hipMemcpyDeviceToHost);
+  region_end();  // the device phase of the prove ends here
+  sync();
+  if (deep_on_device) replay_alpha();
+  std::vector<uint32_t> final_w(fri_h + 2 * FRI_ROUNDS * fb, fri_h + 2 * FRI_ROUNDS * fb + fw_words);
   {
-    std::vector<uint32_t> dg((size_t)FRI_ROUNDS * fb), rt((size_t)FRI_ROUNDS * fb);
-    LG_HIP(hipMemcpyAsync(dg.data(), chal_dig.as<void>(), dg.size() * 4, hipMemcpyDeviceToHost, s_));
-    LG_HIP(hipMemcpyAsync(rt.data(), root_dev.as<void>(), rt.size() * 4, hipMemcpyDeviceToHost, s_));
-    sync();
+    std::vector<uint32_t> dg(fri_h, fri_h + FRI_ROUNDS * fb), rt(fri_h + FRI_ROUNDS * fb, fri_h + 2 * FRI_ROUNDS * fb);
     for (int j = first + 1; j < FRI_ROUNDS; ++j) {
       std::vector<Digest> root(B_);
       for (int b = 0; b < B_; ++b)
@@ -2047,11 +2259,6 @@ void Job::open() {
   keep_.push_back(std::move(chal_alpha));
   keep_.push_back(std::move(root_dev));
   trace("fri rounds");
-  // final polynomial (running after the last round): 4 E6 values
-  const uint32_t flen = running.len;
-  std::vector<uint32_t> final_w((size_t)B_ * 6 * flen);
-  LG_HIP(hipMemcpyAsync(final_w.data(), running.buf.as<void>(), final_w.size() * 4, hipMemcpyDeviceToHost, s_));
-  sync();
   // ---- query openings.  The gather ops (one record per (instance, query))
   // are built and staged first; the query chains then run on the host, and
   // as soon as every position of a chunk of instances is known, that chunk's
@@ -2197,7 +2404,7 @@ void Job::open() {
   for (size_t i = 0; i < tab_src.size(); ++i) tab_dst[i] = gat.as<uint32_t>() + gwords + i * TAB_WORDS;
   const uint32_t* const* tab_src_d = tab_src.empty() ? nullptr : stage(tab_src);
   uint32_t* const* tab_dst_d = tab_src.empty() ? nullptr : const_cast<uint32_t* const*>(stage(tab_dst));
-  stg_->flush(s_);  // the chunk launches below stage nothing
+  stg().flush(s_);  // the chunk launches below stage nothing
   uint32_t* gh = ctx_.pinned((gwords + twords) * 4, lane_);
   const uint32_t* tabs_h = gh + gwords;  // [table][b][TAB_WORDS]
   if (twords) {
@@ -2498,9 +2705,18 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   constexpr int kMinLaneBatch = 8;
   int lanes = std::min(ctx_->lanes, std::max(1, B / kMinLaneBatch));
   if (ctx_->timer.on) lanes = 1;  // per-kernel timing wants one stream
+  struct BucketHold {  // returns a lane's bucket however the lane ends
+    Plan* p;
+    GraphBucket* b;
+    ~BucketHold() { p->give_bucket(b); }
+  };
   if (lanes <= 1) {
-    Job job(*this, inputs, outputs, cb, 0);
-    job.run();
+    int mode = 0;
+    BucketHold hold{this, take_bucket(B, 0, inputs[0].device, &mode)};
+    {
+      Job job(*this, inputs, outputs, cb, 0, hold.b, (Job::Mode)mode);
+      job.run();
+    }
     LG_HIP(hipStreamSynchronize(ctx_->stream));
     ctx_->timer.resolve();
     return;
@@ -2519,8 +2735,12 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   auto run_lane = [&](int l) {
     try {
       current_pool() = &ThreadPool::for_lane(l, lanes);
-      Job job(*this, part[l], out[l], l == 0 ? cb : PhaseCallback(), l);  // phases: lane 0's
-      job.run();
+      int mode = 0;
+      BucketHold hold{this, take_bucket((int)part[l].size(), l, inputs[0].device, &mode)};
+      {
+        Job job(*this, part[l], out[l], l == 0 ? cb : PhaseCallback(), l, hold.b, (Job::Mode)mode);  // phases: lane 0's
+        job.run();
+      }
       current_pool() = nullptr;
       LG_HIP(hipStreamSynchronize(ctx_->lane_stream(l)));
     } catch (...) {

@@ -81,6 +81,8 @@ class Staging {
   void* put(const void* src, size_t n, hipStream_t s);
   // Copies everything appended since the last flush to the device (one copy).
   void flush(hipStream_t s);
+  // Marks everything appended as copied (a replayed graph holds the copies).
+  void assume_flushed();
   template <class T>
   const T* put(const std::vector<T>& v, hipStream_t s) {
     return static_cast<const T*>(put(v.data(), v.size() * sizeof(T), s));
@@ -147,6 +149,9 @@ struct Context {
   uint32_t* pinned_in(size_t bytes, int lane);  // host -> device (trace columns)
   uint32_t* pinned_aux(size_t bytes, int lane);  // small device -> host results (claimed values, digests)
   uint32_t* pinned_fs(size_t bytes, int lane);   // transcript data (round digests, roots, logup totals)
+  uint32_t* pinned_ext(size_t bytes, int lane, size_t idx);  // device-phase results, idx-th buffer of the lane
+  // Record proves into HIP graphs per shape bucket (LOOMGPU_GRAPHS=0: off).
+  bool graphs = true;
   Context();
   ~Context();
 
@@ -161,6 +166,7 @@ struct Context {
     size_t pinned_aux_cap = 0;
     uint32_t* pinned_fs = nullptr;
     size_t pinned_fs_cap = 0;
+    std::vector<std::pair<uint32_t*, size_t>> ext;
   };
   std::mutex lane_mu_;
   std::vector<LaneBufs> lane_;
@@ -191,6 +197,26 @@ using PhaseCallback = std::function<void(const char*, double)>;
 
 struct ChallengeSchedule;
 
+// One shape bucket of a plan (batch size, lane and whether the inputs are
+// host or device memory): the device buffers,
+// staging area and pinned buffers of its prove in allocation order, so that
+// every prove of the bucket uses the same addresses, and the instantiated
+// HIP graph of the prove's device phase (generated columns through the FRI
+// commit phase).
+struct GraphBucket {
+  int batch = 0, lane = 0;
+  std::vector<DevBuf> bufs;
+  std::vector<std::pair<uint32_t*, size_t>> pins;
+  std::unique_ptr<Staging> stg;
+  hipGraphExec_t exec = nullptr;
+  int proves = 0;       // proves of this shape seen (the graph is recorded on the second)
+  uint64_t last = 0;    // use stamp (least recently used buckets are dropped)
+  size_t bytes = 0;     // device bytes held
+  bool broken = false;  // a recording or replay failed: prove this shape eagerly
+  bool busy = false;
+  ~GraphBucket();
+};
+
 class Plan {
  public:
   Plan(Context* ctx, const uint32_t* words, size_t n);
@@ -215,6 +241,21 @@ class Plan {
   std::vector<std::shared_ptr<DagKernel>> rel_;
   std::map<std::tuple<size_t, size_t, size_t>, std::shared_ptr<DagKernel>> steps_;
   std::shared_ptr<const ChallengeSchedule> sched_;
+  // Graph buckets by (batch, lane); a plan whose modules form one DEEP class
+  // (its device phase needs no host round trip) is eligible.
+  bool graph_eligible_ = false;
+  std::mutex gmu_;
+  std::map<std::tuple<int, int, bool>, std::unique_ptr<GraphBucket>> buckets_;  // (batch, lane, device inputs)
+  uint64_t tick_ = 0;
+  GraphBucket* take_bucket(int batch, int lane, bool device_inputs, int* mode);
+  void give_bucket(GraphBucket* b);
+
+ public:
+  static constexpr int MAX_GRAPH_BUCKETS = 4;
+  // Device bytes held by this plan's graph buckets.
+  size_t graph_bytes();
+  // Drops every graph bucket (frees its buffers and graph).
+  void drop_graphs();
 };
 
 }  // namespace lg

@@ -205,8 +205,14 @@ void Server::worker(int si) {
       auto& pl = sl.plans[take[0]->plan.get()];
       if (!pl) pl = std::make_unique<Plan>(sl.ctx.get(), take[0]->plan->words.data(), take[0]->plan->words.size());
       pl->prove(in, out, nullptr);
-      // keep this slot's cached device buffers within its share of the budget
-      sl.ctx->pool.trim(budget_ / slots_.size());
+      // keep this slot's cached device buffers within its share of the budget:
+      // graph buckets (buffers kept per recorded shape) get half of it
+      const size_t share = budget_ / slots_.size();
+      size_t held = 0;
+      for (auto& kv : sl.plans) held += kv.second->graph_bytes();
+      if (held > share / 2)
+        for (auto& kv : sl.plans) kv.second->drop_graphs();
+      sl.ctx->pool.trim(share);
     } catch (const std::exception& e) {
       err = e.what();
     }

@@ -50,6 +50,9 @@ class Context:
         except Exception:
             pass
 
+    def set_graphs(self, on):
+        _lib.check(_lib.lib().lg_ctx_set_graphs(self.h, 1 if on else 0), "lg_ctx_set_graphs")
+
     def set_timing(self, on):
         _lib.check(_lib.lib().lg_ctx_set_timing(self.h, 1 if on else 0), "lg_ctx_set_timing")
 

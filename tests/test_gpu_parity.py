@@ -366,3 +366,51 @@ class _HostPtr:
 
     def data_ptr(self):
         return self.arr.ctypes.data
+
+
+@pytest.mark.gpu
+def test_graph_replay_equals_eager():
+    """HIP graphs per shape bucket: the same shape proved repeatedly with new
+    witnesses (eager, then recorded, then replayed) gives the proofs and
+    transcripts of a context without graphs -- one lane (batch 3), two lanes
+    (batch 16), and a plan with range lookups whose replayed batch holds an
+    invalid instance that fails alone."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from loom_amd import prover, workloads
+    g = prover.Context(0)
+    e = prover.Context(0)
+    e.set_graphs(False)
+    try:
+        for B, logn in ((3, 9), (16, 8)):
+            for rep in range(4):
+                log("graphs: batch %d, 2^%d rows, call %d" % (B, logn, rep))
+                items = [prover.Prepared(*workloads.plonk_instance(logn, 1000 * rep + 10 * B + k), 24)
+                         for k in range(B)]
+                a, ta = prover.prove_prepared(g, items, want_transcript=True)
+                b, tb = prover.prove_prepared(e, items, want_transcript=True)
+                for k in range(B):
+                    assert np.array_equal(a[k], b[k]), (B, rep, k)
+                    assert np.array_equal(ta[k], tb[k]), (B, rep, k)
+                # the same shape from device inputs is a bucket of its own
+                d = prover.prove_prepared(g, items, dev_inputs=prover.DeviceInputs(items))
+                for k in range(B):
+                    assert np.array_equal(d[k], b[k]), (B, rep, k)
+        for rep in range(4):
+            log("graphs: range lookups, call %d" % rep)
+            items = [prover.Prepared(*workloads.uneven_range_instance([8], 70 + 5 * rep + k), 16) for k in range(4)]
+            if rep == 3:
+                bad = items[1]
+                bad.columns = [c.copy() for c in bad.columns]
+                bad.columns[bad.plan.input_names().index("L_0")][4] = prover.to_mont([4000])[0]
+            a = prover.prove_prepared(g, items, allow_failures=True)
+            b = prover.prove_prepared(e, items, allow_failures=True)
+            for k in range(4):
+                if rep == 3 and k == 1:
+                    assert a[k] is None and b[k] is None
+                else:
+                    assert np.array_equal(a[k], b[k]), (rep, k)
+    finally:
+        g.close()
+        e.close()
