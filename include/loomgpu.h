@@ -28,9 +28,13 @@
  *   - WithNumFriQueries: the plan's num_queries.
  *   - WithHashBackend: Poseidon2 only (hash id "poseidon2").
  *   - WithPhaseCallback: lg_plan_last_phases after the call.
+ *   - setup.ProvingKey: lg_plan_set_setup commits the setup columns once per
+ *     plan and keeps them resident (base-field setup columns).
+ *   - public.Inputs: base-field entries at rows the plan lists; each
+ *     instance passes the values after its input columns.
  *   - Refused (a plan or call that needs them is rejected with an error):
- *     a non-empty setup.ProvingKey (precommitted setup trees), non-empty
- *     public.Inputs, WithTranscript and WithNewTranscriptHasher (the
+ *     extension-field setup columns or public-input entries,
+ *     WithTranscript and WithNewTranscriptHasher (the
  *     transcript always starts fresh with the Poseidon2 sponge), SkipFRI,
  *     WithFriOption (no proof-of-work grinding: the PoW list is empty), other
  *     hash backends, lookups other than width-1 selector-free range lookups.
@@ -97,8 +101,23 @@ void lg_plan_destroy(lg_plan* plan);
  * board.Compile time.  Returns the total code-object bytes in *code_bytes. */
 int lg_plan_check(const uint32_t* words, size_t n_words, size_t* code_bytes);
 
-/* Number of trace input columns / permutations an instance must provide. */
+/* Number of input arrays / permutations an instance must provide: the trace
+ * input columns (N words each, plan input order), then for each public-input
+ * column of the plan (public.Inputs, name order) its entry values (one word
+ * per entry row the plan lists, ascending rows). */
 int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms);
+
+/* setup.Setup (setup/setup.go:68-124) for a plan with setup columns
+ * (expr.Setup leaves, e.g. PLONK selectors, plonk_gnark_utils.go:217-219):
+ * setup_columns[k] is setup column k in plan setup order (host memory, N words
+ * of its module, Montgomery).  The columns are RS-encoded and committed into
+ * the setup tree (one group per module) once; columns, LDE and tree stay in
+ * device memory for every later prove of the plan (the proving key), which
+ * binds the root to its first challenge and opens the tree at the queries.
+ * root: 8 words, the verification key (setup.VerificationKey.Roots[0]).
+ * Calling it again replaces the key.  A plan with setup columns refuses to
+ * prove before its key is set. */
+int lg_plan_set_setup(lg_plan* plan, const uint32_t* const* setup_columns, size_t n_columns, uint32_t* root);
 
 /* Proves `batch` independent instances of one plan (same shapes, different
  * witnesses), in lockstep: one launch sequence per phase for all instances.

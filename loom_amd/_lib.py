@@ -20,6 +20,7 @@ EXPORTS = {
     "lg_plan_create": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_void_p)]),
     "lg_plan_destroy": (None, [ctypes.c_void_p]),
     "lg_plan_check": (ctypes.c_int, [u32p, ctypes.c_size_t, sizep]),
+    "lg_plan_set_setup": (ctypes.c_int, [ctypes.c_void_p, u32pp, ctypes.c_size_t, u32p]),
     "lg_plan_num_inputs": (ctypes.c_int, [ctypes.c_void_p, sizep, sizep]),
     "lg_prove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, u32pp, sizep, sizep, u32pp, sizep, sizep]),
     "lg_prove_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, u32pp, sizep, sizep, u32pp, sizep,

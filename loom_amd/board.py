@@ -439,6 +439,11 @@ class Program:
     rounds: list
     column_fields: dict
     logup_bus: list = field(default_factory=list)
+    # precommitted setup columns (board.Program.SetupColumns, program.go:19):
+    # sorted by (module, name) as collectSetupColumns (program.go:138-165)
+    setup_columns: list = field(default_factory=list)
+    # public-input columns referenced by the relations: (name, module), by name
+    public_columns: list = field(default_factory=list)
 
 
 def _column_modules(modules):
@@ -502,10 +507,30 @@ def compile_program(b):
         for c in rd.staged:
             c.field = E.join(c.field, fields.get(c.name, E.BASE))
 
+    # setup columns: every setup leaf of a module's relations
+    # (collectSetupColumns, board/program.go:138-165); public-input leaves
+    setup = {}
+    public = {}
+    for mn, m in b.modules.items():
+        for rel in m.relations:
+            for leaf in E.leaves_full(rel):
+                if leaf.type == E.SETUP:
+                    setup[mn + "\x00" + leaf.name] = ColumnRef(leaf.name, mn, E.join(
+                        setup.get(mn + "\x00" + leaf.name, ColumnRef("", "", E.BASE)).field, leaf.field_kind()))
+                elif leaf.type == E.VERIFIER and leaf.hook == E.PUBLIC_INPUT_HOOK:
+                    if public.get(leaf.name, mn) != mn:
+                        raise ValueError("public input %r used by two modules" % leaf.name)
+                    public[leaf.name] = mn
+    setup_columns = [setup[k] for k in sorted(setup)]
+    for c in setup_columns:
+        setf(c.name, c.field)
+        c.field = E.join(c.field, fields.get(c.name, E.BASE))
+
     fold_ch = Coin(len(rounds) - 1)
     mods = {}
     for k, m in b.modules.items():
         rel = E.fold(fold_ch, m.relations)
         dag = E.to_dag(rel, fields)
         mods[k] = CompiledModule(m.name, m.N, rel, dag, E.degree(rel), list(m.gencol))
-    return Program(mods, rounds, fields, [list(x) for x in b.logup_bus])
+    return Program(mods, rounds, fields, [list(x) for x in b.logup_bus], setup_columns,
+                   sorted(public.items()))

@@ -124,6 +124,17 @@ int lg_plan_check(const uint32_t* words, size_t n_words, size_t* code_bytes) {
   });
 }
 
+int lg_plan_set_setup(lg_plan* plan, const uint32_t* const* setup_columns, size_t n_columns, uint32_t* root) {
+  return guarded([&] {
+    if (!plan || !setup_columns || !root) throw std::runtime_error("lg_plan_set_setup: null argument");
+    for (size_t k = 0; k < n_columns; ++k)
+      if (!setup_columns[k]) throw std::runtime_error("lg_plan_set_setup: null column pointer");
+    std::lock_guard<std::mutex> lock(plan->ctx->mu);
+    plan->p->set_setup(setup_columns, n_columns, root);
+    return 0;
+  });
+}
+
 int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms) {
   return guarded([&] {
     if (!plan) throw std::runtime_error("lg_plan_num_inputs: null plan");

@@ -223,6 +223,10 @@ struct GenColJob {
   int block0;
 };
 void launch_gencol(const GenColJob* jobs_dev, int njobs, int total_blocks, int batch, hipStream_t s);
+// dst[b][rows[i]] = src[b][i], i < k (rows checked against the column size by
+// the plan parser).
+void launch_scatter_rows(const uint32_t* const* src, const uint32_t* rows, int k, uint32_t* const* dst, int batch,
+                         hipStream_t s);
 
 // Fiat-Shamir challenges on the device (transcript.go:285-315): the sponge
 // of a message given as segments; word i of segment k for instance b is

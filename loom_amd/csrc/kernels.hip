@@ -779,6 +779,24 @@ void launch_copy_cols(const uint32_t* const* src, uint32_t* const* dst, int n, i
   LG_HIP(hipGetLastError());
 }
 
+// dst[b][rows[i]] = src[b][i] for i < k (public-input entries into their
+// zero-filled columns).
+__global__ void k_scatter_rows(const uint32_t* const* __restrict__ src, const uint32_t* __restrict__ rows, int k,
+                               uint32_t* const* __restrict__ dst) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= k) return;
+  dst[blockIdx.y][rows[i]] = src[blockIdx.y][i];
+}
+
+void launch_scatter_rows(const uint32_t* const* src, const uint32_t* rows, int k, uint32_t* const* dst, int batch,
+                         hipStream_t s) {
+  if (k <= 0) return;
+  const int T = 256;
+  if (!prelaunch()) return;
+  hipLaunchKernelGGL(k_scatter_rows, dim3((k + T - 1) / T, batch), dim3(T), 0, s, src, rows, k, dst);
+  LG_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------ GenCol
 // All generated columns of a prove in one launch: block x of job k (the last
 // job with block0 <= x) covers rows [(x - block0) 256, +256) of every instance

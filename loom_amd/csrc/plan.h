@@ -13,9 +13,14 @@ namespace lg {
 constexpr uint32_t PLAN_MAGIC = 0x31504C4Cu;
 constexpr uint32_t PLAN_NONE = 0xFFFFFFFFu;
 
-constexpr uint32_t PLAN_VERSION = 2;
+constexpr uint32_t PLAN_VERSION = 3;
 
-enum ColSrc : uint32_t { SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE };
+enum ColSrc : uint32_t {
+  SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE,
+  SRC_SETUP,   // a precommitted setup column (a0: its index in the setup order)
+  SRC_PUBLIC   // a public-input column (a0: its index in the public list)
+};
+enum TreeKind : uint32_t { TREE_TRACE = 0, TREE_AIR = 1, TREE_SETUP = 2 };
 enum DagOp : uint32_t { OP_COL = 0, OP_CONST, OP_CHAL, OP_ADD, OP_SUB, OP_MUL, OP_POW };
 enum StepKind : uint32_t { STEP_GRAND_PRODUCT = 1, STEP_LOOKUP_MULTIPLICITY = 2, STEP_CYCLIC_LOGUP = 3 };
 enum FsHook : uint32_t { FS_NONE = 0, FS_BIND_EXPOSED_VALUES = 1 };
@@ -69,8 +74,16 @@ struct Group {
 };
 
 struct TreeInfo {
-  uint32_t kind;  // 0 trace, 1 AIR
+  uint32_t kind;  // TreeKind
   std::vector<Group> groups;
+};
+
+// A public-input column (public.Input): its entry rows are fixed by the plan,
+// each instance passes the values.
+struct PublicCol {
+  uint32_t col, module;
+  std::string name, module_name;
+  std::vector<uint32_t> rows;  // ascending
 };
 
 struct PlanData {
@@ -81,7 +94,9 @@ struct PlanData {
   std::vector<Perm> perms;
   std::vector<Round> rounds;
   std::vector<Exposed> exposed;  // sorted by name
+  std::vector<PublicCol> publics;  // sorted by name
   std::vector<TreeInfo> trees;
+  uint32_t n_setup = 0;            // setup columns (SRC_SETUP, a0 = 0 .. n_setup-1)
 };
 
 class WordReader {
@@ -176,6 +191,13 @@ inline void validate_plan(const PlanData& p) {
       case SRC_RANGE:
         if (c.field != 0 || c.a0 > N) bad("range table larger than its module");
         break;
+      case SRC_SETUP:
+        if (c.field != 0 || c.a0 >= p.n_setup) bad("setup column index out of range (base-field setup columns only)");
+        break;
+      case SRC_PUBLIC:
+        if (c.field != 0 || c.a0 >= p.publics.size() || p.publics[c.a0].col != (uint32_t)(&c - &p.cols[0]))
+          bad("public-input column does not match its record");
+        break;
       default:
         bad("unknown column source");
     }
@@ -199,9 +221,35 @@ inline void validate_plan(const PlanData& p) {
     }
   }
   for (auto& m : p.modules) check_dag(m.relation);
-  for (auto& t : p.trees) {
-    if (t.kind > 1) bad("tree kind must be 0 (trace) or 1 (AIR)");
+  for (size_t i = 0; i < p.publics.size(); ++i) {
+    const PublicCol& pc = p.publics[i];
+    if (pc.col >= nc || p.cols[pc.col].src != SRC_PUBLIC || p.cols[pc.col].a0 != i) bad("public-input record column");
+    if (pc.module >= nm || p.cols[pc.col].module != pc.module) bad("public-input module");
+    if (i && !(p.publics[i - 1].name < pc.name)) bad("public inputs not sorted by name");
+    for (size_t k = 0; k < pc.rows.size(); ++k)
+      if (pc.rows[k] >= p.modules[pc.module].N || (k && pc.rows[k] <= pc.rows[k - 1]))
+        bad("public-input rows must ascend within the module");
+  }
+  std::vector<uint32_t> setup_seen(p.n_setup, 0);
+  for (size_t ti = 0; ti < p.trees.size(); ++ti) {
+    const TreeInfo& t = p.trees[ti];
+    if (t.kind > TREE_SETUP) bad("tree kind must be 0 (trace), 1 (AIR) or 2 (setup)");
+    if (t.kind == TREE_SETUP && ti != 0) bad("the setup tree must come first");
     if (t.groups.empty()) bad("tree without groups");
+    for (auto& g : t.groups)
+      for (auto c : g.base)
+        if (c < nc && ((p.cols[c].src == SRC_SETUP) != (t.kind == TREE_SETUP)))
+          bad("setup columns belong to the setup tree only");
+    if (t.kind == TREE_SETUP)
+      for (auto& g : t.groups) {
+        if (!g.ext.empty()) bad("extension-field setup columns are not supported");
+        for (auto c : g.base)
+          if (c < nc && p.cols[c].a0 < p.n_setup) ++setup_seen[p.cols[c].a0];
+      }
+  }
+  for (auto v : setup_seen)
+    if (v != 1) bad("every setup column must be committed once in the setup tree");
+  for (auto& t : p.trees) {
     for (auto& g : t.groups) {
       if (g.module >= nm || g.N != p.modules[g.module].N) bad("group module or size mismatch");
       if (g.base.empty() && g.ext.empty()) bad("empty group");
@@ -293,6 +341,30 @@ inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
     s.resize(len);
     x.name = s;
   }
+  auto rd_string = [&](size_t max_len) {
+    uint32_t len = r.next();
+    if (len > max_len) throw std::runtime_error("plan: string too long");
+    std::string s;
+    for (uint32_t k = 0; k < (len + 3) / 4; ++k) {
+      uint32_t wd = r.next();
+      for (int b = 0; b < 4; ++b) s.push_back((char)((wd >> (8 * b)) & 0xff));
+    }
+    s.resize(len);
+    return s;
+  };
+  uint32_t npub = r.next();
+  if (npub > 4096) throw std::runtime_error("plan: too many public inputs");
+  p.publics.resize(npub);
+  for (auto& pc : p.publics) {
+    pc.col = r.next();
+    pc.module = r.next();
+    pc.name = rd_string(4096);
+    pc.module_name = rd_string(4096);
+    uint32_t k = r.next();
+    if (k > (1u << 22)) throw std::runtime_error("plan: too many public-input entries");
+    for (uint32_t j = 0; j < k; ++j) pc.rows.push_back(r.next());
+  }
+  for (auto& c : p.cols) p.n_setup += c.src == SRC_SETUP;
   uint32_t nt = r.next();
   p.trees.resize(nt);
   for (auto& t : p.trees) {

@@ -570,15 +570,32 @@ struct TreeBuf {
   std::vector<DevBuf> inj;            // per order position >= 1: B x width x 8
   std::vector<int> inj_width;
   uint32_t* const* nodes_d = nullptr;  // staged node-array pointers of the batch (set by build_tree)
-  uint32_t* node_ptr(int b) const { return nodes.as<uint32_t>() + (size_t)b * 2 * nleaves * 8; }
-  uint32_t* inj_ptr(size_t k, int b) const { return inj[k].as<uint32_t>() + (size_t)b * inj_width[k] * 8; }
+  bool shared = false;                 // one tree for every instance (the setup tree)
+  uint32_t* node_ptr(int b) const { return nodes.as<uint32_t>() + (shared ? 0 : (size_t)b * 2 * nleaves * 8); }
+  uint32_t* inj_ptr(size_t k, int b) const {
+    return inj[k].as<uint32_t>() + (shared ? 0 : (size_t)b * inj_width[k] * 8);
+  }
 };
+
+}  // namespace
+
+// The resident proving key of a plan: every setup column (Lagrange values and
+// rate-4 LDE, 5N words, in setup order) and the setup tree.
+struct SetupData {
+  std::vector<DevBuf> cols;
+  TreeBuf tree;
+  uint32_t root[8] = {0};  // Montgomery
+};
+
+namespace {
 
 struct Layer {  // an E6 vector of len elements per instance, 6 limb arrays
   DevBuf buf;
   uint32_t len = 0;
   uint32_t* ptr(int b) const { return buf.as<uint32_t>() + (size_t)b * 6 * len; }
 };
+
+struct FsMsgBuilder;
 
 class Job {
  public:
@@ -635,6 +652,9 @@ class Job {
   }
 
   void run();
+  // setup.Setup on the device: uploads, RS-encodes and commits the setup
+  // columns (this Job has one instance) into `out`, which then owns them.
+  void commit_setup(const uint32_t* const* host_cols, SetupData& out);
 
  private:
   DevBuf dev(size_t words) {
@@ -743,6 +763,7 @@ class Job {
 
   void alloc_columns();
   void upload_inputs();
+  void public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_dev);
   void gencol();
   void run_steps(size_t r);
   Layer eval_step_dag(size_t r, size_t s, size_t k, uint32_t mi);
@@ -755,6 +776,7 @@ class Job {
   // replays them into its transcripts at its first synchronisation after
   // the claimed values (replay_device_fs).
   void round_challenge(size_t r);
+  void init_segments(FsMsgBuilder& mb);
   void zeta_challenge();
   void replay_device_fs();
   const uint32_t* chal_operands(const std::vector<uint32_t>& chal_ids);
@@ -822,7 +844,9 @@ class Job {
   DevBuf chal_dev_, fs_dig_;
   uint32_t* fs_h_ = nullptr;
   size_t fs_roots_off_ = 0, fs_tot_off_ = 0, fs_err_off_ = 0;
-  std::vector<uint32_t> fs_init_;                    // bindings of the first challenge (hash id, sizes)
+  std::vector<uint32_t> fs_init_;                    // bindings of the first challenge (hash id, sizes, setup root)
+  std::vector<std::vector<uint32_t>> pi_words_;      // [b] public-input transcript elements (Montgomery)
+  const uint32_t* pi_dev_ = nullptr;                 // the same, [b][len] staged before the device phase
   std::map<uint32_t, const uint32_t*> total_ptr_;    // logup total column -> device B x 6 words
   std::vector<uint32_t> total_cols_;
   std::vector<std::vector<Digest>> roots_;  // [tree][b]
@@ -846,16 +870,79 @@ void Job::alloc_columns() {
     cb.n = d_.modules[ci.module].N;
     cb.committed = committed[c] != 0;
     if (ci.src == SRC_CHALLENGE || is_total[c]) continue;  // per-instance constants, kept in chal_dev_
+    if (ci.src == SRC_SETUP) {  // the plan's resident setup column, shared by every instance
+      const DevBuf& sb = plan_.setup()->cols.at(ci.a0);
+      cb.buf = DevBuf::view(sb.as<void>(), sb.bytes());
+      cb.blk = 0;
+      cb.lde_ready = true;
+      continue;
+    }
     cb.blk = (size_t)cb.limbs * cb.n * (cb.committed ? 5 : 1);
     cb.buf = dev(cb.blk * B_);
   }
+  trees_.resize(d_.trees.size());
+  for (size_t t = 0; t < d_.trees.size(); ++t) {
+    if (d_.trees[t].kind != TREE_SETUP) continue;
+    const TreeBuf& st = plan_.setup()->tree;
+    TreeBuf& tb = trees_[t];
+    tb.shared = true;
+    tb.nleaves = st.nleaves;
+    tb.order = st.order;
+    tb.inj_width = st.inj_width;
+    tb.nodes = DevBuf::view(st.nodes.as<void>(), st.nodes.bytes());
+    for (auto& x : st.inj) tb.inj.push_back(DevBuf::view(x.as<void>(), x.bytes()));
+  }
+}
+
+void Job::commit_setup(const uint32_t* const* host_cols, SetupData& out) {
+  const TreeInfo& ti = d_.trees.at(0);
+  if (ti.kind != TREE_SETUP) throw std::runtime_error("set_setup: no setup tree");
+  cols_.resize(d_.cols.size());
+  out.cols.resize(d_.n_setup);
+  std::vector<uint32_t> ids;
+  for (size_t c = 0; c < d_.cols.size(); ++c) {
+    const ColInfo& ci = d_.cols[c];
+    if (ci.src != SRC_SETUP) continue;
+    ColBuf& cb = cols_[c];
+    cb.limbs = 1;
+    cb.n = d_.modules[ci.module].N;
+    cb.committed = true;
+    cb.blk = (size_t)5 * cb.n;
+    out.cols[ci.a0].alloc(cb.blk * 4);
+    cb.buf = DevBuf::view(out.cols[ci.a0].as<void>(), cb.blk * 4);
+    LG_HIP(hipMemcpyAsync(cb.lag(0, 0), host_cols[ci.a0], (size_t)cb.n * 4, hipMemcpyHostToDevice, s_));
+  }
+  for (auto& g : ti.groups) ids.insert(ids.end(), g.base.begin(), g.base.end());
+  lde_columns(ids);
+  std::vector<std::vector<const uint32_t*>> gcols;
+  std::vector<std::pair<int, int>> shape;
+  std::vector<int> rows;
+  for (auto& g : ti.groups) {
+    std::vector<const uint32_t*> cp;
+    for (auto c : g.base) cp.push_back(cols_[c].lde(0, 0));
+    gcols.push_back(cp);
+    shape.push_back({(int)g.base.size(), 0});
+    rows.push_back((int)g.N * 4);
+  }
+  trees_.resize(d_.trees.size());
+  build_tree(trees_[0], gcols, shape, rows);
+  sync();
+  LG_HIP(hipMemcpy(out.root, trees_[0].node_ptr(0) + 8, 32, hipMemcpyDeviceToHost));
+  // the tree's buffers come from the context pool: the key takes them over
+  out.tree.nleaves = trees_[0].nleaves;
+  out.tree.order = trees_[0].order;
+  out.tree.inj_width = trees_[0].inj_width;
+  out.tree.nodes = std::move(trees_[0].nodes);
+  out.tree.inj = std::move(trees_[0].inj);
+  trees_[0].inj.clear();
 }
 
 // Trace columns and permutations go through one pinned buffer: the copies
 // into it run on the host thread pool, then one DMA per buffer.
 void Job::upload_inputs() {
+  const size_t npub = d_.publics.size();
   for (int b = 0; b < B_; ++b) {
-    if (in_[b].columns.size() != d_.inputs.size()) throw std::runtime_error("wrong number of input columns");
+    if (in_[b].columns.size() != d_.inputs.size() + npub) throw std::runtime_error("wrong number of input columns");
     if (in_[b].perms.size() != d_.perms.size()) throw std::runtime_error("wrong number of permutations");
   }
   for (size_t k = 0; k < d_.inputs.size(); ++k)
@@ -866,8 +953,9 @@ void Job::upload_inputs() {
     // launch per module size; permutations are read where they are.
     for (int b = 0; b < B_; ++b) {
       if (!in_[b].device) throw std::runtime_error("lg_prove: mixed host and device inputs");
-      for (size_t k = 0; k < nin + d_.perms.size(); ++k) {
-        const void* ptr = k < nin ? (const void*)in_[b].columns[k] : (const void*)in_[b].perms[k - nin];
+      for (size_t k = 0; k < nin + npub + d_.perms.size(); ++k) {
+        const void* ptr = k < nin + npub ? (const void*)in_[b].columns[k] : (const void*)in_[b].perms[k - nin - npub];
+        if (k >= nin && k < nin + npub && d_.publics[k - nin].rows.empty()) continue;  // no entries: not read
         hipPointerAttribute_t at;
         if (hipPointerGetAttributes(&at, ptr) != hipSuccess || at.type != hipMemoryTypeDevice ||
             at.device != ctx_.device) {
@@ -890,9 +978,13 @@ void Job::upload_inputs() {
     perm_ptr_.assign(d_.perms.size(), std::vector<const uint32_t*>(B_));
     for (size_t p = 0; p < d_.perms.size(); ++p)
       for (int b = 0; b < B_; ++b) perm_ptr_[p][b] = in_[b].perms[p];
+    std::vector<std::vector<const uint32_t*>> pv(npub, std::vector<const uint32_t*>(B_));
+    for (size_t i = 0; i < npub; ++i)
+      for (int b = 0; b < B_; ++b) pv[i][b] = in_[b].columns[nin + i];
+    public_inputs(pv);
     return;
   }
-  // host layout: [b][input k] then [perm p][b]
+  // host layout: [b][input k] then [perm p][b], then [public i][b]
   size_t words = 0;
   std::vector<size_t> col_off((size_t)B_ * d_.inputs.size()), perm_off(d_.perms.size() * B_);
   for (int b = 0; b < B_; ++b)
@@ -905,6 +997,12 @@ void Job::upload_inputs() {
       perm_off[p * B_ + b] = words;
       words += d_.perms[p].len;
     }
+  std::vector<size_t> pub_off(npub * B_);
+  for (size_t i = 0; i < npub; ++i)
+    for (int b = 0; b < B_; ++b) {
+      pub_off[i * B_ + b] = words;
+      words += d_.publics[i].rows.size();
+    }
   uint32_t* h = ctx_.pinned_in(words * 4, lane_);
   parallel_for((int)(B_ * (nin + d_.perms.size())), [&](int i) {
     const int b = i % B_;
@@ -912,6 +1010,9 @@ void Job::upload_inputs() {
     if (k < nin) std::memcpy(h + col_off[(size_t)b * nin + k], in_[b].columns[k], cols_[d_.inputs[k]].n * 4);
     else std::memcpy(h + perm_off[(k - nin) * B_ + b], in_[b].perms[k - nin], d_.perms[k - nin].len * 4);
   });
+  for (size_t i = 0; i < npub; ++i)
+    for (int b = 0; b < B_; ++b)
+      std::memcpy(h + pub_off[i * B_ + b], in_[b].columns[nin + i], d_.publics[i].rows.size() * 4);
   in_arena_ = dev(words);
   LG_HIP(hipMemcpyAsync(in_arena_.as<void>(), h, words * 4, hipMemcpyHostToDevice, s_));
   // columns of one module size share a scatter launch
@@ -929,6 +1030,66 @@ void Job::upload_inputs() {
   perm_ptr_.assign(d_.perms.size(), std::vector<const uint32_t*>(B_));
   for (size_t p = 0; p < d_.perms.size(); ++p)
     for (int b = 0; b < B_; ++b) perm_ptr_[p][b] = in_arena_.as<uint32_t>() + perm_off[p * B_ + b];
+  std::vector<std::vector<const uint32_t*>> pv(npub, std::vector<const uint32_t*>(B_));
+  for (size_t i = 0; i < npub; ++i)
+    for (int b = 0; b < B_; ++b) pv[i][b] = in_arena_.as<uint32_t>() + pub_off[i * B_ + b];
+  public_inputs(pv);
+}
+
+// Public inputs (prover.go:236-273): each column is zero except at its entry
+// rows, which take the instance's values; their transcript elements
+// (public.Inputs.TranscriptElements, public/public_inputs.go:50-74: PUBI,
+// #inputs, then per input its tagged name and module, #entries and
+// (row, field 0, value) per entry) are bound to the first challenge.  The
+// per-instance words are staged before the device phase, so a recorded
+// graph reads this prove's values.
+void Job::public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_dev) {
+  const size_t npub = d_.publics.size();
+  pi_words_.assign(B_, std::vector<uint32_t>());
+  if (!npub) return;
+  const size_t nin = d_.inputs.size();
+  constexpr uint32_t TAG_PUBI = 0x50554249u;
+  for (size_t i = 0; i < npub; ++i) {
+    const PublicCol& pc = d_.publics[i];
+    const ColBuf& cb = cols_[pc.col];
+    std::vector<uint32_t*> dst(B_);
+    for (int b = 0; b < B_; ++b) dst[b] = cb.lag(b, 0);
+    uint32_t* const* dst_d = const_cast<uint32_t* const*>(stage(dst));
+    launch_fill(dst_d, (int)cb.n, 0u, B_, s_);
+    if (!pc.rows.empty())
+      launch_scatter_rows(stage(vals_dev[i]), stage(pc.rows), (int)pc.rows.size(), dst_d, B_, s_);
+  }
+  // the values on the host: the caller's arrays, or a copy of the device ones
+  std::vector<std::vector<uint32_t>> hv(npub * B_);
+  for (size_t i = 0; i < npub; ++i)
+    for (int b = 0; b < B_; ++b) {
+      const size_t k = d_.publics[i].rows.size();
+      hv[i * B_ + b].resize(k);
+      if (!k) continue;
+      if (in_[b].device) LG_HIP(hipMemcpy(hv[i * B_ + b].data(), in_[b].columns[nin + i], k * 4, hipMemcpyDeviceToHost));
+      else std::memcpy(hv[i * B_ + b].data(), in_[b].columns[nin + i], k * 4);
+    }
+  std::vector<uint32_t> all;
+  for (int b = 0; b < B_; ++b) {
+    std::vector<uint32_t>& w = pi_words_[b];
+    w.push_back(M(TAG_PUBI));
+    w.push_back(M((uint32_t)npub));
+    for (size_t i = 0; i < npub; ++i) {
+      const PublicCol& pc = d_.publics[i];
+      auto a = string_to_elements(TAG_PUBI, pc.name), m = string_to_elements(TAG_PUBI, pc.module_name);
+      w.insert(w.end(), a.begin(), a.end());
+      w.insert(w.end(), m.begin(), m.end());
+      w.push_back(M((uint32_t)pc.rows.size()));
+      for (size_t e = 0; e < pc.rows.size(); ++e) {
+        w.push_back(M(pc.rows[e]));
+        w.push_back(0u);  // field.Base
+        w.push_back(hv[i * B_ + b][e]);
+      }
+    }
+    if (b && w.size() != pi_words_[0].size()) throw std::runtime_error("public inputs: instance shapes differ");
+    all.insert(all.end(), w.begin(), w.end());
+  }
+  pi_dev_ = stage(all);
 }
 
 void Job::gencol() {
@@ -1438,6 +1599,14 @@ struct FsMsgBuilder {
   }
 };
 
+// The first challenge's bindings: the common words, then each instance's
+// public-input words.
+void Job::init_segments(FsMsgBuilder& mb) {
+  mb.add(stage(fs_init_), (uint32_t)fs_init_.size(), 0);
+  if (!pi_words_.empty() && !pi_words_[0].empty())
+    mb.add(pi_dev_, (uint32_t)pi_words_[0].size(), (uint32_t)pi_words_[0].size());
+}
+
 // Challenge of round r: StringToElements(FSID, name) || previous digest ||
 // bindings: the first challenge's init words, the round tree's root
 // (prover.go:360-412), then with FS_BIND_EXPOSED_VALUES [0, 6 limbs] per
@@ -1448,7 +1617,7 @@ void Job::round_challenge(size_t r) {
   FsMsgBuilder mb;
   mb.add(stage(sc.prefix[r]), (uint32_t)sc.prefix[r].size(), 0);
   if (r > 0) mb.add(fs_dig_.as<uint32_t>() + (r - 1) * (size_t)B_ * 8, 8, 8);
-  else mb.add(stage(fs_init_), (uint32_t)fs_init_.size(), 0);
+  else init_segments(mb);
   if (rd.tree != PLAN_NONE) {
     const TreeBuf& tb = trees_[rd.tree];
     mb.add(tb.node_ptr(0) + 8, 8, (uint32_t)(16 * tb.nleaves));
@@ -1476,7 +1645,7 @@ void Job::zeta_challenge() {
   FsMsgBuilder mb;
   mb.add(stage(sc.prefix[nr]), (uint32_t)sc.prefix[nr].size(), 0);
   if (nr > 0) mb.add(fs_dig_.as<uint32_t>() + (nr - 1) * (size_t)B_ * 8, 8, 8);
-  else mb.add(stage(fs_init_), (uint32_t)fs_init_.size(), 0);
+  else init_segments(mb);
   for (size_t t = 0; t < d_.trees.size(); ++t)
     if (d_.trees[t].kind == 1) mb.add(trees_[t].node_ptr(0) + 8, 8, (uint32_t)(16 * trees_[t].nleaves));
   uint32_t* dig = fs_dig_.as<uint32_t>() + nr * (size_t)B_ * 8;
@@ -1680,9 +1849,14 @@ void Job::run() {
   fs_init_.push_back(M(TAG_SIZE));
   fs_init_.push_back(M((uint32_t)d_.modules.size()));
   for (auto& m : d_.modules) fs_init_.push_back(M(m.N));
+  // the setup tree's root, then the public inputs (prover.go:224-234)
+  if (d_.n_setup) fs_init_.insert(fs_init_.end(), plan_.setup()->root, plan_.setup()->root + 8);
   {
     const std::string init = nr ? challenge_name(0) : "__zeta";
-    for (int b = 0; b < B_; ++b) ts_[b].bind(init, fs_init_);
+    for (int b = 0; b < B_; ++b) {
+      ts_[b].bind(init, fs_init_);
+      if (!pi_words_[b].empty()) ts_[b].bind(init, pi_words_[b]);
+    }
   }
   chal_dev_ = dev((size_t)B_ * d_.cols.size() * 6);
   fs_dig_ = dev((nr + 1) * (size_t)B_ * 8);
@@ -2548,8 +2722,10 @@ void Job::open() {
       w.push_back(0);  // index
       w.insert(w.end(), totals_[b][x.col].v, totals_[b][x.col].v + 6);
     }
-    w.push_back((uint32_t)nt);
-    for (size_t t = 0; t < nt; ++t) w.insert(w.end(), roots_[t][b].v, roots_[t][b].v + 8);
+    // commitments: the trace and AIR roots; setup roots are the verification key's
+    w.push_back((uint32_t)(nt - (d_.n_setup ? 1 : 0)));
+    for (size_t t = 0; t < nt; ++t)
+      if (d_.trees[t].kind != TREE_SETUP) w.insert(w.end(), roots_[t][b].v, roots_[t][b].v + 8);
     // claimed values
     w.push_back((uint32_t)nt);
     size_t k = 0;
@@ -2698,8 +2874,29 @@ void Job::open() {
 
 }  // namespace
 
+Plan::~Plan() = default;
+
+void Plan::set_setup(const uint32_t* const* cols, size_t n_cols, uint32_t* root) {
+  if (n_cols != d_.n_setup) throw std::runtime_error("set_setup: the plan has " + std::to_string(d_.n_setup) +
+                                                     " setup columns");
+  if (!d_.n_setup) throw std::runtime_error("set_setup: the plan has no setup columns");
+  drop_graphs();  // recorded graphs hold the previous key's root and buffers
+  auto sd = std::make_unique<SetupData>();
+  LG_HIP(hipSetDevice(ctx_->device));
+  std::vector<InstanceInput> in(1);
+  std::vector<ProveOutput> out(1);
+  {
+    Job job(*this, in, out, nullptr, 0);
+    job.commit_setup(cols, *sd);
+  }
+  LG_HIP(hipStreamSynchronize(ctx_->stream));
+  std::memcpy(root, sd->root, 32);
+  setup_ = std::move(sd);
+}
+
 void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutput>& outputs, const PhaseCallback& cb) {
   if (inputs.empty()) return;
+  if (d_.n_setup && !setup_) throw std::runtime_error("prove: the plan's setup columns are not committed (lg_plan_set_setup)");
   outputs.assign(inputs.size(), ProveOutput());
   const int B = (int)inputs.size();
   constexpr int kMinLaneBatch = 8;

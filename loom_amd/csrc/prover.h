@@ -196,6 +196,7 @@ struct ProveOutput {
 using PhaseCallback = std::function<void(const char*, double)>;
 
 struct ChallengeSchedule;
+struct SetupData;
 
 // One shape bucket of a plan (batch size, lane and whether the inputs are
 // host or device memory): the device buffers,
@@ -220,6 +221,13 @@ struct GraphBucket {
 class Plan {
  public:
   Plan(Context* ctx, const uint32_t* words, size_t n);
+  ~Plan();
+  // setup.Setup (setup/setup.go:68-124) on the device: commits the setup
+  // columns (host, N words each, in setup order) into the setup tree once and
+  // keeps columns, LDE and tree resident for every later prove of the plan.
+  // root: 8 words (Montgomery), the verification key.
+  void set_setup(const uint32_t* const* cols, size_t n_cols, uint32_t* root);
+  const SetupData* setup() const { return setup_.get(); }
   const PlanData& data() const { return d_; }
   Context* ctx() const { return ctx_; }
   // Runs the prove phase for every instance; outputs are resized to batch.
@@ -227,7 +235,9 @@ class Plan {
   // a failure of the plan or the device throws for the whole batch.
   void prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutput>& outputs,
              const PhaseCallback& cb = nullptr);
-  size_t num_inputs() const { return d_.inputs.size(); }
+  // input arrays per instance: the input columns, then each public input's
+  // entry values
+  size_t num_inputs() const { return d_.inputs.size() + d_.publics.size(); }
   size_t num_perms() const { return d_.perms.size(); }
 
   std::shared_ptr<DagKernel> relation_kernel(size_t m) const { return rel_[m]; }
@@ -241,6 +251,7 @@ class Plan {
   std::vector<std::shared_ptr<DagKernel>> rel_;
   std::map<std::tuple<size_t, size_t, size_t>, std::shared_ptr<DagKernel>> steps_;
   std::shared_ptr<const ChallengeSchedule> sched_;
+  std::unique_ptr<SetupData> setup_;
   // Graph buckets by (batch, lane); a plan whose modules form one DEEP class
   // (its device phase needs no host round trip) is eligible.
   bool graph_eligible_ = false;

@@ -22,7 +22,7 @@ size_t plan_bytes_per_instance(const PlanData& d) {
   size_t words = 0;
   for (size_t c = 0; c < d.cols.size(); ++c) {
     const ColInfo& ci = d.cols[c];
-    if (ci.src == SRC_CHALLENGE || is_total[c]) continue;
+    if (ci.src == SRC_CHALLENGE || ci.src == SRC_SETUP || is_total[c]) continue;  // setup: shared by the plan
     words += (size_t)ci.limbs() * d.modules[ci.module].N * (committed[c] ? 5 : 1);
   }
   size_t nmax = 0;

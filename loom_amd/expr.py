@@ -115,6 +115,17 @@ def ExtCol(name, shift=0):
     return Leaf(COMMITTED, name, shift, EXT)
 
 
+def Setup(name, shift=0):
+    """A precommitted setup column (expr.Setup, ast.go:211-213)."""
+    return Leaf(SETUP, name, shift, BASE)
+
+
+def PublicInput(name):
+    """A public-input column (expr.PublicInput, ast.go:230-232): a verifier
+    column the prover materialises from the statement's entries."""
+    return Leaf(VERIFIER, name, 0, BASE, PUBLIC_INPUT_HOOK)
+
+
 def Challenge(name):
     return Leaf(CHALLENGE, name, 0, EXT)
 

@@ -24,7 +24,11 @@ Word stream (u32, little endian):
             n_terms; grand product -> none)
   n_exposed, then per exposed value (sorted by name): column, name
             (byte length, bytes packed 4 per word, little endian)
-  n_trees, then per tree: kind, n_groups, per group: module, N,
+  n_public, then per public-input column (sorted by name): column, module,
+            name, module name (strings as above), n_entries, entry rows...
+            (the caller passes each instance's entry values, n_entries words,
+            after its input columns)
+  n_trees, then per tree: kind (0 trace, 1 AIR, 2 setup), n_groups, per group: module, N,
             nb, base cols..., ne, ext cols..., shift lists (base polys then
             ext polys; each: count, shifts...)
   per module: relation dag
@@ -37,16 +41,20 @@ from . import board as B
 from . import protocol as PR
 
 MAGIC = 0x31504C4C
-VERSION = 2
+VERSION = 3
 NONE = 0xFFFFFFFF
 
-SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE = range(8)
+SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE, SRC_SETUP, SRC_PUBLIC = range(10)
 OP_COL, OP_CONST, OP_CHAL, OP_ADD, OP_SUB, OP_MUL, OP_POW = range(7)
 
 
 class Plan:
-    def __init__(self, program, num_queries=PR.NUM_QUERIES_DEFAULT):
+    def __init__(self, program, num_queries=PR.NUM_QUERIES_DEFAULT, public_rows=None):
+        """public_rows: public-input column name -> its entry rows (the
+        statement's public.Inputs fix which rows are public; their values are
+        per-instance inputs)."""
         self.program = program
+        self.public_rows = {k: [int(i) for i in v] for k, v in (public_rows or {}).items()}
         self.layout = PR.build_layout(program)
         self.num_queries = num_queries
         self.modules = sorted(program.modules)
@@ -55,6 +63,8 @@ class Plan:
         self.col_idx = {}
         self.perms = []
         self.inputs = []
+        self.setup = []   # setup column names, in lg_plan_set_setup order (the setup tree's)
+        self.public = []  # public-input column names, by name
         self._collect()
 
     def _add(self, name, field, module, src, a0=0, a1=0):
@@ -66,6 +76,21 @@ class Plan:
 
     def _collect(self):
         prog = self.program
+        for t in self.layout.trees:
+            if t.kind != "setup":
+                continue
+            for g in t.groups:
+                for name in g.base + g.ext:
+                    f = prog.column_fields.get(name, E.BASE)
+                    if f != E.BASE:
+                        raise NotImplementedError("extension-field setup column %r" % name)
+                    self._add(name, f, self.mod_idx[g.module], SRC_SETUP, len(self.setup))
+                    self.setup.append(name)
+        for name, mn in getattr(prog, "public_columns", []):
+            if name not in self.public_rows:
+                raise KeyError("public input %r has no entry rows" % name)
+            self._add(name, E.BASE, self.mod_idx[mn], SRC_PUBLIC, len(self.public))
+            self.public.append(name)
         gen = {}
         for mn in self.modules:
             m = prog.modules[mn]
@@ -210,16 +235,29 @@ class Plan:
                 for e, sz in zip(st.ins, sizes):
                     self._dag_words(e, sz, w)
         exposed = sorted(o for rd in prog.rounds for st in rd.steps for o in st.exposed_outputs())
-        w.append(len(exposed))
-        for name in exposed:
-            w.append(self.col_idx[name])
-            b = name.encode()
+
+        def string(x):
+            b = x.encode()
             w.append(len(b))
             for i in range(0, len(b), 4):
                 w.append(int.from_bytes(b[i:i + 4].ljust(4, b"\0"), "little"))
+
+        w.append(len(exposed))
+        for name in exposed:
+            w.append(self.col_idx[name])
+            string(name)
+        w.append(len(self.public))
+        for name in self.public:
+            mn = dict(prog.public_columns)[name]
+            w += [self.col_idx[name], self.mod_idx[mn]]
+            string(name)
+            string(mn)
+            rows = sorted(self.public_rows[name])
+            w.append(len(rows))
+            w += rows
         w.append(len(lay.trees))
         for t in lay.trees:
-            w += [0 if t.kind == "trace" else 1, len(t.groups)]
+            w += [{"trace": 0, "air": 1, "setup": 2}[t.kind], len(t.groups)]
             for g in t.groups:
                 w += [self.mod_idx[g.module], g.N, len(g.base)] + [self.col_idx[c] for c in g.base]
                 w += [len(g.ext)] + [self.col_idx[c] for c in g.ext]

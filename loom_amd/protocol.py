@@ -52,7 +52,7 @@ class Group:
 
 @dataclass
 class Tree:
-    kind: str          # "trace" or "air"
+    kind: str          # "setup", "trace" or "air"
     round: int         # trace round index (-1 for AIR)
     groups: list
 
@@ -84,6 +84,15 @@ def build_layout(program):
             return len(trees) - 1
         return None
 
+    # setup section: one tree, one group per module, columns sorted by name
+    # (protocol.SetupGroups, layout.go:143-166)
+    by = {}
+    for c in getattr(program, "setup_columns", []):
+        if c.module in program.modules:
+            by.setdefault(c.module, []).append((c.name, c.field))
+    for m in by:
+        by[m].sort()
+    add_tree("setup", -1, by)
     for r, rd in enumerate(program.rounds):
         by = {}
         for c in rd.staged:
@@ -120,6 +129,34 @@ def _fill_schedule(program, lay):
             else:
                 g.base_shifts = [sorted(shifts.get(n, ())) for n in g.base]
                 g.ext_shifts = [sorted(shifts.get(n, ())) for n in g.ext]
+
+
+def setup_tree_count(lay):
+    return sum(1 for t in lay.trees if t.kind == "setup")
+
+
+PUBI_TAG = 0x50554249
+
+
+def _string_elements(tag, s):
+    b = s.encode()
+    return [tag, len(b)] + [int.from_bytes(b[i:i + 3], "little") for i in range(0, len(b), 3)]
+
+
+def public_input_elements(public_inputs):
+    """public.Inputs.TranscriptElements (public/public_inputs.go:50-74) for
+    base-field entries: [PUBI, #inputs], then per input in name order the
+    tagged name and module, #entries and (index, field 0, value) per entry in
+    index order.  public_inputs: name -> (module, [(index, value), ...])."""
+    out = [PUBI_TAG, len(public_inputs)]
+    for name in sorted(public_inputs):
+        module, entries = public_inputs[name]
+        out += _string_elements(PUBI_TAG, name) + _string_elements(PUBI_TAG, module)
+        ents = sorted((int(i), int(v) % E.P) for i, v in entries)
+        out.append(len(ents))
+        for i, v in ents:
+            out += [i, 0, v]
+    return out
 
 
 def module_sizes_transcript(program):

@@ -35,6 +35,7 @@ class Context:
         _lib.check(L.lg_ctx_create(device, ctypes.byref(h)), "lg_ctx_create")
         self.h = h
         self.plans = {}
+        self.setups = {}
 
     def close(self):
         if self.h:
@@ -68,7 +69,10 @@ class Context:
             out[name] = (int(cnt), float(ms), float(by), float(pm))
         return out
 
-    def plan_handle(self, words):
+    def plan_handle(self, words, setup=None):
+        """The plan for these words; with setup columns (Montgomery arrays in
+        plan setup order) they are committed once (lg_plan_set_setup) and kept
+        on the device until other setup columns are given."""
         key = words.tobytes()
         if key not in self.plans:
             h = ctypes.c_void_p()
@@ -76,7 +80,29 @@ class Context:
             _lib.check(_lib.lib().lg_plan_create(self.h, w.ctypes.data_as(_lib.u32p), w.size, ctypes.byref(h)),
                        "lg_plan_create")
             self.plans[key] = h
+        if setup:
+            import hashlib
+            digest = hashlib.sha256(b"".join(np.ascontiguousarray(c).tobytes() for c in setup)).digest()
+            if self.setups.get(key) != digest:
+                set_setup(self.plans[key], setup)
+                self.setups[key] = digest
         return self.plans[key]
+
+
+def set_setup(plan_h, setup):
+    """Commits a plan's setup columns (setup.Setup): returns the setup root
+    (the verification key), canonical values."""
+    cols = [np.ascontiguousarray(c, dtype=np.uint32) for c in setup]
+    ptrs = (_lib.u32p * len(cols))(*[c.ctypes.data_as(_lib.u32p) for c in cols])
+    root = np.zeros(8, dtype=np.uint32)
+    _lib.check(_lib.lib().lg_plan_set_setup(plan_h, ptrs, len(cols), root.ctypes.data_as(_lib.u32p)),
+               "lg_plan_set_setup")
+    return from_mont(root)
+
+
+def from_mont(a):
+    rinv = pow(R, P - 2, P)
+    return (np.asarray(a, dtype=np.uint64) * np.uint64(rinv) % np.uint64(P)).astype(np.uint64)
 
 
 def check_plan(words):
@@ -89,13 +115,21 @@ def check_plan(words):
 
 class Prepared:
     """An instance ready for lg_prove: the plan words and the Montgomery
-    input arrays (kept alive for the duration of the call)."""
+    input arrays (kept alive for the duration of the call).  The trace holds
+    the witness columns and, for a program with setup columns, the proving
+    key's setup columns; public_inputs maps a public-input column to
+    (module, [(row, value), ...]) (base-field entries)."""
 
-    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT):
-        self.plan = Plan(program, num_queries)
+    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None):
+        public_inputs = dict(public_inputs or {})
+        self.public_inputs = public_inputs
+        self.plan = Plan(program, num_queries, {k: [i for i, _ in v[1]] for k, v in public_inputs.items()})
         self.words = self.plan.words()
         self.columns = [to_mont(trace[name]) for name in self.plan.input_names()]
+        for name in self.plan.public:
+            self.columns.append(to_mont([v for _, v in sorted((int(i), int(v)) for i, v in public_inputs[name][1])]))
         self.perms = self.plan.perm_arrays()
+        self.setup = [to_mont(trace[name]) for name in self.plan.setup]
 
 
 _proof_cap = {}
@@ -189,7 +223,12 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
             raise ValueError("batched instances must share one plan (same program shape)")
     if dev_inputs is not None and len(dev_inputs.items) != len(items):
         raise ValueError("dev_inputs does not match items")
-    ph = ctx.plan_handle(words)
+    if items[0].setup:
+        for it in items[1:]:
+            if len(it.setup) != len(items[0].setup) or any(not np.array_equal(a, b)
+                                                          for a, b in zip(it.setup, items[0].setup)):
+                raise ValueError("batched instances must share one proving key (setup columns)")
+    ph = ctx.plan_handle(words, items[0].setup)
     B = len(items)
     col_ptrs, perm_ptrs = _pointer_tables(items, dev_inputs)
     entry = _lib.lib().lg_prove if dev_inputs is None else _lib.lib().lg_prove_device

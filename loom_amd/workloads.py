@@ -198,3 +198,69 @@ def wide_instance(log_n, seed, n_fib=32, n_cubic=32):
         m.assert_zero_except_at(E.Col("W_%d" % k, 1) - wk * wk * wk - E.Const(int(cs[k])), n - 1)
     b.add_module(m)
     return B.compile_program(b), trace
+
+
+def plonk_setup_instance(log_n, setup_seed, witness_seed, n_public=0):
+    """A plonk-like module whose selectors are setup columns
+    (PrepareIthPlonkWithSetup, integration_test/gnark_plonk/
+    plonk_gnark_utils.go:217-232): qL*L + qR*R + qM*L*R + qO*O + qK (+ PI)
+    with QL..QK precommitted, and a copy constraint over (L, R, O).
+
+    The circuit (selectors, copy cycles) comes from setup_seed, so instances
+    of one setup_seed share one proving key; the witness from witness_seed.
+    Gates as in plonk_instance (40% add, 40% mul, 20% constant), qK uniform in
+    [0, p); copy cycles tie 10% of the free cells (L and R of add and mul rows,
+    R of constant rows); each cycle carries one random value, R and the free L
+    are random, then O (add, mul rows) or L (constant rows) is solved from the
+    gate.  With n_public > 0 the relation adds the public-input column PI
+    (entries at rows 0 .. n_public-1, random values), and the gates absorb it.
+    Returns (program, trace, public_inputs) -- trace holds the witness and the
+    setup columns; public_inputs maps PI -> (module, [(row, value)])."""
+    n = 1 << log_n
+    pp = np.uint64(P)
+    rs = np.random.Generator(np.random.PCG64(setup_seed))
+    kind = rs.choice(3, size=n, p=[0.4, 0.4, 0.2])
+    one, mone, zero = np.uint64(1), np.uint64(P - 1), np.uint64(0)
+    ql = np.where(kind == 1, zero, one).astype(np.uint64)
+    qr = np.where(kind == 0, one, zero).astype(np.uint64)
+    qm = np.where(kind == 1, one, zero).astype(np.uint64)
+    qo = np.where(kind == 2, zero, mone).astype(np.uint64)
+    qk = rs.integers(0, P, size=n, dtype=np.uint64)
+    free = np.concatenate([np.nonzero(kind != 2)[0], n + np.arange(n)])  # L of add/mul rows, every R
+    tied = rs.permutation(free)[: 3 * n // 10]
+    sigma = np.arange(3 * n, dtype=np.int64)
+    cycles = []
+    pos = 0
+    while pos + 2 <= len(tied):
+        ln = min(int(rs.integers(2, 5)), len(tied) - pos)
+        cyc = tied[pos:pos + ln]
+        cycles.append(cyc)
+        for k in range(ln):
+            sigma[cyc[k]] = cyc[(k + 1) % ln]
+        pos += ln
+    rw = np.random.Generator(np.random.PCG64(witness_seed))
+    w = rw.integers(0, P, size=3 * n, dtype=np.uint64)
+    for cyc in cycles:
+        w[cyc] = w[cyc[0]]
+    pi = np.zeros(n, dtype=np.uint64)
+    pi_entries = []
+    if n_public:
+        vals = rw.integers(0, P, size=n_public, dtype=np.uint64)
+        pi[:n_public] = vals
+        pi_entries = [(i, int(vals[i])) for i in range(n_public)]
+    L, R = w[:n].copy(), w[n:2 * n].copy()
+    c = (qk + pi) % pp
+    O = np.where(kind == 0, (L + R + c) % pp, np.where(kind == 1, (L * R % pp + c) % pp, zero))
+    L = np.where(kind == 2, (pp - c) % pp, L)
+    trace = dict(L=L, R=R, O=O.astype(np.uint64), QL=ql, QR=qr, QM=qm, QO=qo, QK=qk)
+    b = B.Builder()
+    m = B.Module("plonk", n)
+    gate = (E.Setup("QL") * E.Col("L") + E.Setup("QR") * E.Col("R") + E.Setup("QM") * E.Col("L") * E.Col("R")
+            + E.Setup("QO") * E.Col("O") + E.Setup("QK"))
+    if n_public:
+        gate = gate + E.PublicInput("PI")
+    m.assert_zero(gate)
+    b.add_module(m)
+    B.copy_constraint(b, "plonk", [E.Col("L"), E.Col("R"), E.Col("O")], B.PermutationGen(sigma, "plonk.S"))
+    public = {"PI": ("plonk", pi_entries)} if n_public else {}
+    return B.compile_program(b), trace, public

@@ -3,7 +3,12 @@ against, bit for bit.
 
 Follows prover/prover.go:142-719 and the PCS it drives:
 * transcript set-up and bindings -- prover.go:199-234 (HASH string, module
-  sizes), :321-358 (trace-round roots), :528-558 (AIR root, zeta);
+  sizes, setup roots, public inputs), :321-358 (trace-round roots), :528-558
+  (AIR root, zeta);
+* setup tree -- setup/setup.go:68-124 (one tree, one group per module, its
+  root bound to the first challenge and left out of the proof's commitments);
+* public inputs -- prover.go:236-273 (columns materialised from the entries),
+  public/public_inputs.go:50-74 (transcript elements);
 * GenCol -- board/module.go:77-101 (Lagrange), :229-293 (permutation support
   ID_k = g^k * w^j and sigma columns);
 * grand product step -- board/step.go:301-327, poly/iop_utils.go:170-192,
@@ -120,11 +125,21 @@ def as_e6_rows(v, m):
 
 # ------------------------------------------------------------------ prover
 class Prover:
-    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT):
+    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None):
+        """trace: column name -> canonical values (the witness, and the setup
+        columns of the proving key); public_inputs: name -> (module,
+        [(index, value), ...]) base-field entries."""
         self.prog = program
         self.layout = PR.build_layout(program)
         self.nq = num_queries
         self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items()}
+        self.pi = dict(public_inputs or {})
+        for name, (mod, entries) in self.pi.items():
+            col = np.zeros(program.modules[mod].N, dtype=np.uint64)
+            for i, v in entries:
+                col[int(i)] = int(v) % P
+            self.base[name] = col
+        self.setup_roots = []
         self.ext = {}
         self.air_ext = {}
         self.committed = [None] * len(self.layout.trees)
@@ -285,7 +300,10 @@ class Prover:
         c = Committed([self.group_columns(g, air) for g in t.groups])
         self.committed[t_idx] = c
         root = c.root()
-        self.commitments.append(root)
+        if t.kind == "setup":
+            self.setup_roots.append(root)  # the verification key, not a proof commitment
+        else:
+            self.commitments.append(root)
         return root
 
     # -- quotient
@@ -360,6 +378,11 @@ class Prover:
         init = names[0]
         self.ts.bind(init, string_to_elements(HASH_TAG, "poseidon2"))
         self.ts.bind(init, PR.module_sizes_transcript(prog))
+        for t, tree in enumerate(lay.trees):
+            if tree.kind == "setup":
+                self.ts.bind(init, self.commit_tree(t))
+        if self.pi:
+            self.ts.bind(init, PR.public_input_elements(self.pi))
         self.gencol()
         for r, rd in enumerate(prog.rounds):
             self.run_steps(rd)
@@ -684,8 +707,15 @@ def encode_proof(pf):
     return np.array(out, dtype=np.uint32)
 
 
-def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT):
+def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None):
     """Returns (proof dict, canonical u32 encoding, transcript log)."""
-    pv = Prover(program, trace, num_queries)
+    pv = Prover(program, trace, num_queries, public_inputs)
     pf = pv.prove()
     return pf, encode_proof(pf), pv.log
+
+
+def setup_roots(program, trace):
+    """setup.Setup's verification key: the root of the setup tree (none
+    without setup columns), in canonical form."""
+    pv = Prover(program, trace, 1)
+    return [pv.commit_tree(t) for t, tree in enumerate(pv.layout.trees) if tree.kind == "setup"]

@@ -209,16 +209,23 @@ def _e(v):
 
 
 # ------------------------------------------------------------------ verify
-def verify(program, num_queries, words, check_queries=None):
+def verify(program, num_queries, words, check_queries=None, setup_roots=(), public_inputs=None):
     """Checks proof words against the program; returns the decoded proof.
     check_queries bounds how many queries have their openings checked (all by
-    default); the transcript replay always covers every query."""
+    default); the transcript replay always covers every query.  setup_roots:
+    the verification key (setup.VerificationKey.Roots); public_inputs: name ->
+    (module, [(index, value), ...])."""
     pf = decode_proof(words)
     lay = PR.build_layout(program)
     prog = program
+    pi = dict(public_inputs or {})
     check(pf["hash_id"] == "poseidon2", "hash id")
     nt = len(lay.trees)
-    check(len(pf["commitments"]) == nt, "commitment count")
+    ns = PR.setup_tree_count(lay)
+    check(len(setup_roots) == ns, "verification key: setup root count")
+    check(len(pf["commitments"]) == nt - ns, "commitment count")
+    # roots by canonical tree index: setup roots from the key, then the proof's
+    roots_by_tree = [list(map(int, r)) for r in setup_roots] + list(pf["commitments"])
     check(len(pf["claimed"]) == nt, "claimed values: tree count")
     exposed_names = sorted(o for rd in prog.rounds for st in rd.steps for o in st.exposed_outputs())
     check(sorted(pf["exposed"]) == exposed_names, "exposed value names")
@@ -229,11 +236,15 @@ def verify(program, num_queries, words, check_queries=None):
     ts = Transcript(names)
     ts.bind(names[0], string_to_elements(HASH_TAG, "poseidon2"))
     ts.bind(names[0], PR.module_sizes_transcript(prog))
+    for r in setup_roots:
+        ts.bind(names[0], list(map(int, r)))
+    if pi:
+        ts.bind(names[0], PR.public_input_elements(pi))
     chal = {}
     for r, rd in enumerate(prog.rounds):
         cn = B.challenge_name(r)
         if r in lay.trace_tree_of_round:
-            ts.bind(cn, pf["commitments"][lay.trace_tree_of_round[r]])
+            ts.bind(cn, roots_by_tree[lay.trace_tree_of_round[r]])
         if rd.fshook == B.BIND_EXPOSED_VALUES:
             for st in rd.steps:
                 for name in st.exposed_outputs():
@@ -241,7 +252,7 @@ def verify(program, num_queries, words, check_queries=None):
         chal[cn] = _e(ts.compute(cn)[:6])
     air_trees = [i for i, t in enumerate(lay.trees) if t.kind == "air"]
     for t in air_trees:
-        ts.bind("__zeta", pf["commitments"][t])
+        ts.bind("__zeta", roots_by_tree[t])
     zeta = _e(ts.compute("__zeta")[:6])
 
     # claimed values by (tree, group) and by (name, shift)
@@ -283,7 +294,18 @@ def verify(program, num_queries, words, check_queries=None):
                 num = F.e6_scale(zn1, wi * F.inv(n) % P)
                 den = F.e6_sub(zeta, F.e6_from_base(np.array([wi], dtype=np.uint64))[0])
                 return F.e6_mul(num, F.e6_inv(den))[None, :]
-            check(x.type == E.COMMITTED, "unsupported leaf in a relation")
+            if x.type == E.VERIFIER and x.hook == E.PUBLIC_INPUT_HOOK:
+                # sum_i v_i L_i(zeta) over the entries (verifier/hookFns.go:51-70)
+                mod, entries = pi[x.name]
+                check(mod == mn, "public input %s used from module %s" % (x.name, mn))
+                acc = F.e6_zero()
+                for i, v in entries:
+                    wi = pow(w, int(i) % n, P)
+                    num = F.e6_scale(zn1, wi * F.inv(n) % P * (int(v) % P) % P)
+                    den = F.e6_sub(zeta, F.e6_from_base(np.array([wi], dtype=np.uint64))[0])
+                    acc = F.e6_add(acc, F.e6_mul(num, F.e6_inv(den)))
+                return acc[None, :]
+            check(x.type in (E.COMMITTED, E.SETUP), "unsupported leaf in a relation")
             return vals[(x.name, x.shift % n)][None, :]
 
         ev = eval_expr(m.relation, leaf, 1)[0]
@@ -384,7 +406,7 @@ def verify(program, num_queries, words, check_queries=None):
             got = np.array([wp["path"][2][j - 1] for wp in wps], dtype=np.uint64).reshape(Q, 8).T
             check(np.array_equal(got, leaves[j]), "injection leaves do not hash the injected rows of tree %d" % ti)
         roots = _paths_root(leaves[0], [wp["path"] for wp in wps], n_top // 2, inj_w)
-        check(np.all(roots == np.array(pf["commitments"][ti], dtype=np.uint64)[:, None]),
+        check(np.all(roots == np.array(roots_by_tree[ti], dtype=np.uint64)[:, None]),
               "opening of tree %d does not reach its commitment" % ti)
     # DEEP quotient at the two rows of each pair, per class
     deep_at = {}

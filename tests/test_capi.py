@@ -100,6 +100,12 @@ def _plan_offsets(w):
     for _ in range(nx):
         ln = w[i + 1]
         i += 2 + (ln + 3) // 4
+    npub = w[i]; i += 1
+    for _ in range(npub):
+        i += 2
+        for _ in range(2):
+            i += 1 + (w[i] + 3) // 4
+        i += 1 + w[i]
     nt = w[i]; i += 1
     for _ in range(nt):
         ng = w[i + 1]

@@ -9,8 +9,8 @@ tests hold it to the reference's own literal expectations instead:
 * internal/protocol/layout_test.go -- rail-relative PolyIdx (:171-211),
   mixed-size trace round in one tree (:213-249), canonical schedule shifts of
   mixed groups (:251-309), groups follow module order (:385-413), the module
-  sizes transcript (:415-436).  The tests there that use setup columns
-  (:31-169) are not restated: setup columns are refused by this build.
+  sizes transcript (:415-436), base-only slot stability with setup columns
+  (:31-125) and the mixed-size setup tree (:127-169).
 * the config-0 commitment layout of the plonk-like module (T0, T2, AIR).
 * internal/fiat-shamir/transcript_test.go:13-78 -- the exact hasher inputs
   of a two-challenge transcript, for the oracle transcript and the host C++
@@ -43,10 +43,11 @@ FSID, HASH, SIZE, EXTP = 0x46534944, 0x48415348, 0x53495A45, 0x45585450
 
 
 # ------------------------------------------------------------------ helpers
-def program(modules, rounds, fields=None):
+def program(modules, rounds, fields=None, setup=()):
     """A compiled program given directly (as the Go tests build
     board.Program): modules {name: (N, relation or None)}, rounds as lists of
-    (column, module, field) staged in that order."""
+    (column, module, field) staged in that order, setup columns as
+    (column, module, field)."""
     fields = dict(fields or {})
     mods = {}
     for name, (n, rel) in modules.items():
@@ -56,7 +57,7 @@ def program(modules, rounds, fields=None):
     for rd in rds:
         for c in rd.staged:
             fields.setdefault(c.name, c.field)
-    return B.Program(mods, rds, fields)
+    return B.Program(mods, rds, fields, [], [B.ColumnRef(c, m, f) for c, m, f in setup])
 
 
 def col_slots(lay):
@@ -176,6 +177,37 @@ def test_groups_follow_module_order(sizes):
     assert tree_groups(lay) == [[("a", sizes[0]), ("m", sizes[1]), ("z", sizes[2])]]
     s = col_slots(lay)
     assert [s[n][1] for n in ("a_0", "m_0", "z_0")] == [0, 1, 2]
+
+
+def test_base_only_slot_stability():
+    """layout_test.go:31-125: a module with setup columns and a lookup
+    multiplicity step compiled by the builder: setup columns sorted by name in
+    tree 0, the unstaged trace columns swept in name order into tree 1, the
+    AIR chunk in tree 2."""
+    b = B.Builder()
+    m = B.Module("m", 8)
+    mul = B.chunk_name("mul", 0)
+    m.assert_zero(E.Col("z") + E.Col(mul) + E.Col("s") + E.Col("t") + E.Setup("pz") + E.Setup("pa") + E.Setup("pm"))
+    b.add_module(m)
+    b.add_lookup_multiplicity_step(0, None, None, [[E.Col("s")]], [[E.Col("t")]], "mul", ["m"], "m")
+    lay = PR.build_layout(B.compile_program(b))
+    want = {"pa": (0, 0, 0, E.BASE), "pm": (0, 0, 1, E.BASE), "pz": (0, 0, 2, E.BASE),
+            mul: (1, 0, 0, E.BASE), "s": (1, 0, 1, E.BASE), "t": (1, 0, 2, E.BASE), "z": (1, 0, 3, E.BASE),
+            "m.0": (2, 0, 0, E.BASE)}
+    assert col_slots(lay) == want
+    assert len(lay.trees) == 3 and [t.kind for t in lay.trees] == ["setup", "trace", "air"]
+    assert tree_groups(lay) == [[("m", 8)], [("m", 8)], [("m", 8)]]
+
+
+def test_mixed_size_setup_uses_single_tree():
+    """layout_test.go:127-169."""
+    prog = program({"big": (8, None), "small": (4, None)}, [],
+                   setup=[("pb", "big", E.BASE), ("ps", "small", E.BASE)])
+    lay = PR.build_layout(prog)
+    assert len(lay.trees) == 1 and lay.trees[0].kind == "setup"
+    assert tree_groups(lay) == [[("big", 8), ("small", 4)]]
+    s = col_slots(lay)
+    assert s["pb"] == (0, 0, 0, E.BASE) and s["ps"] == (0, 1, 0, E.BASE)
 
 
 def test_module_sizes_transcript():
