@@ -242,6 +242,14 @@ int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint
 #define LG_NOALLOC_MAX_ROWS (1u << 18)
 int lg_grand_product_noalloc(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n);
 
+/* Diagnostic (no counterpart in the reference): the device's issue rate of
+ * the integer instructions the field arithmetic uses, measured with eight
+ * independent chains per lane.  out5[0..3]: lane operations per ns of
+ * v_add_u32, v_mul_lo_u32, v_mad_u64_u32 and the Montgomery product;
+ * out5[4]: the device clock in MHz.  The VALU roofline of the Poseidon2
+ * kernels is priced against these rates. */
+int lg_probe_issue_rate(lg_ctx* ctx, double* out5);
+
 /* RS encoding (reedsolomon.Encoder.Encode, internal/reedsolomon/rs.go:52-70) of
  * n_cols columns of n words each (Lagrange form on the size-n subgroup, one
  * column after the other) to 4n bit-reversed evaluations each. */

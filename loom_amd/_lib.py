@@ -32,6 +32,7 @@ EXPORTS = {
     "lg_hash_leaf_pairs": (ctypes.c_int, [ctypes.c_void_p, u32pp, ctypes.c_int, u32pp, ctypes.c_int, ctypes.c_size_t, u32p]),
     "lg_merkle_build": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, u32pp, sizep, ctypes.c_int]),
     "lg_grand_product": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_size_t]),
+    "lg_probe_issue_rate": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),
     "lg_grand_product_noalloc": (ctypes.c_int, [ctypes.c_void_p, u32p, u32p, u32p, ctypes.c_size_t]),
     "lg_rs_encode": (ctypes.c_int, [ctypes.c_void_p, u32p, ctypes.c_size_t, ctypes.c_int, u32p]),
     "lg_server_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_uint64, ctypes.c_int,

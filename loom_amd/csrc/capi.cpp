@@ -354,6 +354,15 @@ int lg_grand_product(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint
   });
 }
 
+int lg_probe_issue_rate(lg_ctx* ctx, double* out5) {
+  return guarded([&] {
+    if (!ctx || !out5) throw std::runtime_error("lg_probe_issue_rate: null argument");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    lg::issue_probe(ctx->c.device, out5, out5 + 4);
+    return 0;
+  });
+}
+
 int lg_grand_product_noalloc(lg_ctx* ctx, const uint32_t* num, const uint32_t* den, uint32_t* gp, size_t n) {
   return guarded([&] {
     if (!ctx || !num || !den || !gp) throw std::runtime_error("lg_grand_product_noalloc: null argument");

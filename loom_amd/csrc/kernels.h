@@ -113,6 +113,11 @@ size_t scan_tmp_words(int n, int batch);
 uint32_t* noalloc_init(int device);
 void grand_product_noalloc(int device, hipStream_t s, const uint32_t* num, const uint32_t* den, uint32_t* gp,
                            size_t n);
+// Issue-rate probe on `device`: lane operations per ns of v_add_u32,
+// v_mul_lo_u32, v_mad_u64_u32 and the Montgomery product kb_mul (gops[0..3]),
+// each in eight independent chains per lane over 16 workgroups per CU, and
+// the device's clock (MHz).
+void issue_probe(int device, double* gops, double* clock_mhz);
 void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den, uint32_t* const* ratio,
                           uint32_t* const* gp, int n, int batch, uint32_t* scan_tmp, hipStream_t s);
 
@@ -160,7 +165,7 @@ struct ClaimSet {
 constexpr int CLAIMS_GROUP = 8;
 // Rows of a set are split over claims_split(n) blocks; part then holds
 // n_items x claims_split(n) x 6 words of partial sums (item-major).
-constexpr int CLAIMS_MAX_SPLIT = 256;
+constexpr int CLAIMS_MAX_SPLIT = 4096;
 int claims_split(int n);
 void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int n_items, int n, uint32_t* part,
                    hipStream_t s);

@@ -1355,16 +1355,21 @@ __global__ __launch_bounds__(256) void k_claims(const ClaimSet* __restrict__ set
   }
 }
 
-__global__ void k_claims_sum(const ClaimItem* items, int n_items, int S, const uint32_t* part) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_items) return;
+// One block per claim: the sum of its S partial sums.
+__global__ __launch_bounds__(256) void k_claims_sum(const ClaimItem* items, int S, const uint32_t* part) {
+  __shared__ uint32_t sh[6][4];
+  const int j = blockIdx.x;
   E6 acc = e6_zero();
-  for (int c = 0; c < S; ++c) acc = e6_add(acc, ldc6(part + ((size_t)j * S + c) * 6));
+  for (int c = threadIdx.x; c < S; c += blockDim.x) acc = e6_add(acc, ldc6(part + ((size_t)j * S + c) * 6));
 #pragma unroll
-  for (int l = 0; l < 6; ++l) items[j].out[l] = acc.v[l];
+  for (int l = 0; l < 6; ++l) {
+    const uint32_t v = block_sum4(acc.v[l], sh[l]);
+    if (threadIdx.x == 0) items[j].out[l] = v;
+  }
 }
 
-// 1024 rows per block (a 40-instance batch of 2^14-row modules: 640 blocks)
+// 1024 rows per block (a 40-instance batch of 2^14-row modules: 640 blocks;
+// one 2^22-row module: 4096 blocks)
 int claims_split(int n) { return std::max(1, std::min(CLAIMS_MAX_SPLIT, n >> 10)); }
 
 void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int n_items, int n, uint32_t* part,
@@ -1376,8 +1381,7 @@ void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int
   hipLaunchKernelGGL(k_claims, dim3(n_sets, S), dim3(256), 0, s, sets, items, n, part);
   LG_HIP(hipGetLastError());
   if (S > 1) {
-    hipLaunchKernelGGL(k_claims_sum, dim3((n_items + 63) / 64), dim3(64), 0, s, items, n_items, S,
-                       (const uint32_t*)part);
+    hipLaunchKernelGGL(k_claims_sum, dim3(n_items), dim3(256), 0, s, items, S, (const uint32_t*)part);
     LG_HIP(hipGetLastError());
   }
 }
@@ -1942,6 +1946,94 @@ void launch_gather_digest(const uint32_t* pos, int nq, int shift, int d, const u
   hipLaunchKernelGGL(k_gather_digest, dim3((nq + T - 1) / T, batch), dim3(T), 0, s, pos, nq, shift, d, inj, out,
                      stride, off);
   LG_HIP(hipGetLastError());
+}
+
+}  // namespace lg
+
+namespace lg {
+
+// ------------------------------------------------------------------ issue-rate probe
+// Eight independent dependency chains per lane of one instruction class, so
+// that issue (not latency) bounds the loop: KIND 0 v_add_u32, 1 v_mul_lo_u32,
+// 2 v_mad_u64_u32 (the high word of x*y + x feeds the next step), 3 the full
+// Montgomery product kb_mul.  Each lane stores its chains' xor so that no
+// step is dead.
+template <int KIND>
+__global__ __launch_bounds__(256) void k_issue_probe(const uint32_t* __restrict__ seed, uint32_t* __restrict__ out,
+                                                     int iters) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t x[8];
+  uint64_t acc[8];
+  const uint32_t y = seed[8] | 1u;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    x[i] = seed[i] ^ (uint32_t)t;
+    acc[i] = x[i];
+  }
+  for (int k = 0; k < iters; ++k) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      if constexpr (KIND == 0) {
+        x[i] = x[i] + y;
+      } else if constexpr (KIND == 1) {
+        x[i] = x[i] * y;
+      } else if constexpr (KIND == 2) {
+        // acc = x * y + acc, one v_mad_u64_u32 per step; each chain's carry-out
+        // goes to an SGPR pair of its own (no write-after-write waits)
+#define LG_PROBE_MAD(I, S0, S1)                                                                                 \
+  if (i == I)                                                                                                   \
+    asm volatile("v_mad_u64_u32 %0, s[" #S0 ":" #S1 "], %1, %2, %0" : "+v"(acc[I]) : "v"(x[I]), "v"(y)     \
+                 : "s" #S0, "s" #S1);
+        LG_PROBE_MAD(0, 64, 65) LG_PROBE_MAD(1, 66, 67) LG_PROBE_MAD(2, 68, 69) LG_PROBE_MAD(3, 70, 71)
+        LG_PROBE_MAD(4, 72, 73) LG_PROBE_MAD(5, 74, 75) LG_PROBE_MAD(6, 76, 77) LG_PROBE_MAD(7, 78, 79)
+#undef LG_PROBE_MAD
+      } else {
+        x[i] = kb_mul(x[i], y);
+      }
+    }
+  }
+  uint32_t r = 0;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) r ^= x[i] ^ (uint32_t)acc[i] ^ (uint32_t)(acc[i] >> 32);
+  out[t] = r;
+}
+
+void issue_probe(int device, double* gops, double* clock_mhz) {
+  int clk_khz = 0, cus = 0;
+  LG_HIP(hipSetDevice(device));
+  LG_HIP(hipDeviceGetAttribute(&clk_khz, hipDeviceAttributeClockRate, device));
+  LG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+  *clock_mhz = clk_khz / 1000.0;
+  const int T = 256, blocks = cus * 16, iters = 4096;
+  const size_t lanes = (size_t)T * blocks;
+  DevBuf seed(64), out(lanes * 4);
+  const uint32_t sv[16] = {0x1234567u, 0x2345678u, 0x3456789u, 0x456789au, 0x56789abu, 0x6789abcu,
+                           0x789abcdu, 0x89abcdeu, 0x7654321u};
+  LG_HIP(hipMemcpy(seed.as<void>(), sv, 64, hipMemcpyHostToDevice));
+  hipEvent_t a, b;
+  LG_HIP(hipEventCreate(&a));
+  LG_HIP(hipEventCreate(&b));
+  for (int kind = 0; kind < 4; ++kind) {
+    auto run = [&] {
+      switch (kind) {
+        case 0: hipLaunchKernelGGL(k_issue_probe<0>, dim3(blocks), dim3(T), 0, 0, seed.as<uint32_t>(), out.as<uint32_t>(), iters); break;
+        case 1: hipLaunchKernelGGL(k_issue_probe<1>, dim3(blocks), dim3(T), 0, 0, seed.as<uint32_t>(), out.as<uint32_t>(), iters); break;
+        case 2: hipLaunchKernelGGL(k_issue_probe<2>, dim3(blocks), dim3(T), 0, 0, seed.as<uint32_t>(), out.as<uint32_t>(), iters); break;
+        default: hipLaunchKernelGGL(k_issue_probe<3>, dim3(blocks), dim3(T), 0, 0, seed.as<uint32_t>(), out.as<uint32_t>(), iters); break;
+      }
+      LG_HIP(hipGetLastError());
+    };
+    run();  // warm-up
+    LG_HIP(hipEventRecord(a, 0));
+    for (int r = 0; r < 5; ++r) run();
+    LG_HIP(hipEventRecord(b, 0));
+    LG_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    LG_HIP(hipEventElapsedTime(&ms, a, b));
+    gops[kind] = 5.0 * lanes * iters * 8 / (ms * 1e-3) / 1e9;  // lane operations per ns
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
 }
 
 }  // namespace lg

@@ -73,10 +73,19 @@ const uint32_t* Tables::get(const std::string& key, const std::function<std::vec
   auto it = bufs_.find(key);
   if (it != bufs_.end()) return it->second.as<uint32_t>();
   std::vector<uint32_t> v = make();
-  DevBuf b = upload(v);
+  // copied on a stream of its own, not the legacy stream: another thread may
+  // be recording a graph (a legacy-stream copy would invalidate it)
+  if (!stream_) LG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  DevBuf b(v.size() * 4);
+  LG_HIP(hipMemcpyAsync(b.as<void>(), v.data(), v.size() * 4, hipMemcpyHostToDevice, stream_));
+  LG_HIP(hipStreamSynchronize(stream_));
   const uint32_t* p = b.as<uint32_t>();
   bufs_.emplace(key, std::move(b));
   return p;
+}
+
+Tables::~Tables() {
+  if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 const uint32_t* Tables::tw_fwd(int logn) {
@@ -270,6 +279,15 @@ void* Staging::put(const void* src, size_t n, hipStream_t) {
 
 void Staging::assume_flushed() { flushed_ = off_; }
 
+void Staging::flush_sync(hipStream_t side) {
+  if (off_ > flushed_) {
+    LG_HIP(hipMemcpyAsync(dev_.as<uint8_t>() + flushed_, host_ + flushed_, off_ - flushed_, hipMemcpyHostToDevice,
+                          side));
+    LG_HIP(hipStreamSynchronize(side));
+    flushed_ = off_;
+  }
+}
+
 void Staging::flush(hipStream_t s) {
   if (off_ > flushed_) {
     LG_HIP(hipMemcpyAsync(dev_.as<uint8_t>() + flushed_, host_ + flushed_, off_ - flushed_, hipMemcpyHostToDevice, s));
@@ -281,6 +299,12 @@ Context::Context() {
   if (const char* e = getenv("LOOMGPU_LANES")) lanes = std::max(1, std::min(8, atoi(e)));
   if (const char* e = getenv("LOOMGPU_GRAPHS")) graphs = atoi(e) != 0;
   lane_.resize(8);
+}
+
+hipStream_t Context::side_stream() {
+  std::lock_guard<std::mutex> lock(lane_mu_);
+  if (!side_) LG_HIP(hipStreamCreateWithFlags(&side_, hipStreamNonBlocking));
+  return side_;
 }
 
 hipStream_t Context::lane_stream(int lane) {
@@ -330,6 +354,10 @@ uint32_t* Context::pinned_ext(size_t bytes, int lane, size_t idx) {
 }
 
 Context::~Context() {
+  if (side_) {
+    (void)hipStreamSynchronize(side_);
+    (void)hipStreamDestroy(side_);
+  }
   for (auto& L : lane_) {
     if (L.pinned) (void)hipHostFree(L.pinned);
     if (L.pinned_in) (void)hipHostFree(L.pinned_in);
@@ -620,8 +648,12 @@ class Job {
     set_prelaunch_hook([](void* u) { static_cast<Job*>(u)->flush_staging(); }, this);
   }
   Staging& stg() { return gb_ ? *gb_->stg : *stg_; }
+  // While a graph is recorded, staged tables go to the device at once (a
+  // synchronous copy outside the graph): they are the same on every prove of
+  // the bucket and stay in its staging area, so a replay copies nothing.
   void flush_staging() {
-    stg().flush(s_);
+    if (mode_ == CAPTURE && in_region_) stg().flush_sync(ctx_.side_stream());
+    else stg().flush(s_);
     ctx_.timer.launch_point(s_);
   }
   ~Job() {
@@ -710,6 +742,7 @@ class Job {
   void region_end() {
     if (!in_region_) return;
     if (mode_ == CAPTURE) {
+      stg().flush_sync(ctx_.side_stream());
       hipGraph_t g = nullptr;
       LG_HIP(hipStreamEndCapture(s_, &g));
       const hipError_t e = hipGraphInstantiate(&gb_->exec, g, nullptr, nullptr, 0);
@@ -927,7 +960,8 @@ void Job::commit_setup(const uint32_t* const* host_cols, SetupData& out) {
   trees_.resize(d_.trees.size());
   build_tree(trees_[0], gcols, shape, rows);
   sync();
-  LG_HIP(hipMemcpy(out.root, trees_[0].node_ptr(0) + 8, 32, hipMemcpyDeviceToHost));
+  LG_HIP(hipMemcpyAsync(out.root, trees_[0].node_ptr(0) + 8, 32, hipMemcpyDeviceToHost, s_));
+  sync();
   // the tree's buffers come from the context pool: the key takes them over
   out.tree.nleaves = trees_[0].nleaves;
   out.tree.order = trees_[0].order;
@@ -1066,7 +1100,10 @@ void Job::public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_de
       const size_t k = d_.publics[i].rows.size();
       hv[i * B_ + b].resize(k);
       if (!k) continue;
-      if (in_[b].device) LG_HIP(hipMemcpy(hv[i * B_ + b].data(), in_[b].columns[nin + i], k * 4, hipMemcpyDeviceToHost));
+      if (in_[b].device) {
+        LG_HIP(hipMemcpyAsync(hv[i * B_ + b].data(), in_[b].columns[nin + i], k * 4, hipMemcpyDeviceToHost, s_));
+        sync();
+      }
       else std::memcpy(hv[i * B_ + b].data(), in_[b].columns[nin + i], k * 4);
     }
   std::vector<uint32_t> all;
@@ -2907,13 +2944,31 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
     GraphBucket* b;
     ~BucketHold() { p->give_bucket(b); }
   };
+  // A recording can fail for reasons outside the prove (another thread's
+  // legacy-stream work invalidates a capture): the bucket is then marked
+  // broken and the same prove runs again eagerly.
+  auto run_job = [&](const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& pcb,
+                     int lane, GraphBucket* gb, int mode) {
+    if (mode == Job::CAPTURE) {
+      try {
+        Job job(*this, in, out, pcb, lane, gb, Job::CAPTURE);
+        job.run();
+        return;
+      } catch (const std::exception&) {
+        gb->broken = true;
+        out.assign(in.size(), ProveOutput());
+      }
+      Job job(*this, in, out, pcb, lane);
+      job.run();
+      return;
+    }
+    Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode);
+    job.run();
+  };
   if (lanes <= 1) {
     int mode = 0;
     BucketHold hold{this, take_bucket(B, 0, inputs[0].device, &mode)};
-    {
-      Job job(*this, inputs, outputs, cb, 0, hold.b, (Job::Mode)mode);
-      job.run();
-    }
+    run_job(inputs, outputs, cb, 0, hold.b, mode);
     LG_HIP(hipStreamSynchronize(ctx_->stream));
     ctx_->timer.resolve();
     return;
@@ -2934,10 +2989,7 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
       current_pool() = &ThreadPool::for_lane(l, lanes);
       int mode = 0;
       BucketHold hold{this, take_bucket((int)part[l].size(), l, inputs[0].device, &mode)};
-      {
-        Job job(*this, part[l], out[l], l == 0 ? cb : PhaseCallback(), l, hold.b, (Job::Mode)mode);  // phases: lane 0's
-        job.run();
-      }
+      run_job(part[l], out[l], l == 0 ? cb : PhaseCallback(), l, hold.b, mode);  // phases: lane 0's
       current_pool() = nullptr;
       LG_HIP(hipStreamSynchronize(ctx_->lane_stream(l)));
     } catch (...) {

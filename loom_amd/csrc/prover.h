@@ -21,6 +21,7 @@ namespace lg {
 // Per-device constant tables (twiddles, coset scalings), built on first use.
 class Tables {
  public:
+  ~Tables();
   const uint32_t* tw_fwd(int logn);
   const uint32_t* tw_inv(int logn);
   const uint32_t* wpow(int logn);                 // w_n^j, j < n
@@ -40,6 +41,7 @@ class Tables {
   const uint32_t* get(const std::string& key, const std::function<std::vector<uint32_t>()>& make);
   std::mutex mu_;
   std::map<std::string, DevBuf> bufs_;
+  hipStream_t stream_ = nullptr;
 };
 
 // Transforms of NttCfg semantics (kernels.h) on every job, at any size up to
@@ -83,6 +85,8 @@ class Staging {
   void flush(hipStream_t s);
   // Marks everything appended as copied (a replayed graph holds the copies).
   void assume_flushed();
+  // flush() on `side` (a stream no graph records), waited for.
+  void flush_sync(hipStream_t side);
   template <class T>
   const T* put(const std::vector<T>& v, hipStream_t s) {
     return static_cast<const T*>(put(v.data(), v.size() * sizeof(T), s));
@@ -144,6 +148,8 @@ struct Context {
   // overlaps another lane's kernels.  LOOMGPU_LANES overrides the default.
   int lanes = 2;
   hipStream_t lane_stream(int lane);
+  // a non-blocking stream for one-off copies beside the lanes' work
+  hipStream_t side_stream();
   // Pinned host buffers per lane, reused across proves.
   uint32_t* pinned(size_t bytes, int lane);     // device -> host (query openings)
   uint32_t* pinned_in(size_t bytes, int lane);  // host -> device (trace columns)
@@ -170,6 +176,7 @@ struct Context {
   };
   std::mutex lane_mu_;
   std::vector<LaneBufs> lane_;
+  hipStream_t side_ = nullptr;
 };
 
 // Per-instance caller data.

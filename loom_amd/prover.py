@@ -51,6 +51,14 @@ class Context:
         except Exception:
             pass
 
+    def issue_probe(self):
+        """Measured issue rates (lg_probe_issue_rate): lane operations per ns
+        of v_add_u32, v_mul_lo_u32, v_mad_u64_u32, Montgomery products, and
+        the clock in MHz."""
+        v = (ctypes.c_double * 5)()
+        _lib.check(_lib.lib().lg_probe_issue_rate(self.h, v), "lg_probe_issue_rate")
+        return dict(add=v[0], mul_lo=v[1], mad_u64=v[2], montgomery=v[3], clock_mhz=v[4])
+
     def set_graphs(self, on):
         _lib.check(_lib.lib().lg_ctx_set_graphs(self.h, 1 if on else 0), "lg_ctx_set_graphs")
 
