@@ -41,6 +41,7 @@ def main():
     ap.add_argument("--queries", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-oracle-check", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="prove with the default launch choices (no autotuning)")
     ap.add_argument("--requests", type=int, default=100, help="serve: number of requests")
     ap.add_argument("--qps", type=float, default=40.0, help="serve: Poisson arrival rate")
     ap.add_argument("--slots", type=int, default=3, help="serve: concurrent proving slots")
@@ -85,7 +86,15 @@ def main():
     dev_all = prover.DeviceInputs(items, 0)
     dev_one = [prover.DeviceInputs([it], 0) for it in items]
 
-    # warmup (plan compile, buffer pools) for both paths
+    # autotuning (untimed): launch choices for the batched and the plain path,
+    # picked by timing whole proves of these inputs (lg_plan_autotune)
+    tuning = None
+    if not args.no_tune:
+        print("[bench] autotuning", file=sys.stderr, flush=True)
+        tb = prover.autotune(ctx, items, dev_inputs=dev_all, reps=3)
+        tp = prover.autotune(ctx, items[:1], dev_inputs=dev_one[0], reps=3)
+        tuning = {"batched": tb, "plain": tp}
+    # warmup (plan compile, buffer pools, graph recording) for both paths
     for _ in range(max(1, args.warmup)):
         prover.prove_prepared(ctx, items[:1], dev_inputs=dev_one[0])
         prover.prove_prepared(ctx, items, dev_inputs=dev_all)
@@ -214,6 +223,7 @@ def main():
         "poseidon2_perms_per_s": round(p2_rate, 1),
         "dominant_kernel": dom,
         "kernels": kernels,
+        "tuning": tuning,
         "roofline": roofline,
         "valu_roofline": valu_roofline,
         "issue_probe_gops": probe,
@@ -326,6 +336,7 @@ def single_proof_bench(args, torch, workloads, prover):
     ctx = prover.Context(0)
     dev = prover.DeviceInputs([item], 0)
     print("[bench] inputs on device", file=sys.stderr, flush=True)
+    tuning = None if args.no_tune else prover.autotune(ctx, [item], dev_inputs=dev, reps=1)
     for _ in range(max(1, args.warmup)):
         first = prover.prove_prepared(ctx, [item], dev_inputs=dev)[0]
     torch.cuda.synchronize()
@@ -363,6 +374,7 @@ def single_proof_bench(args, torch, workloads, prover):
         "dtype": "u32 (KoalaBear Montgomery, E6)", "data": "synthetic seeded",
         "config": {"workload": desc}, "p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
         "deterministic": same, "kernel_ms_total": round(tot_ms, 2), "kernels": kernels, "roofline": roofline,
+        "tuning": tuning,
     }
     print(json.dumps(line))
     ctx.close()

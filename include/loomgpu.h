@@ -149,6 +149,24 @@ int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, co
                     uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
                     uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens);
 
+/* Autotuner (no counterpart in the reference).  Launch choices that do not
+ * change the proofs -- proving lanes, the Merkle width from which levels run
+ * one launch per level, threads per block of leaf and level hashing, the FRI
+ * layer size up to which fold and leaf hashing run cooperatively, rows per
+ * block of the claimed values -- are picked per (plan, batch) by timing whole
+ * proves of the given instances (inputs as for lg_prove, or device pointers
+ * with device_inputs = 1) over a coordinate descent, `reps` timed proves per
+ * trial (median).  The choice is kept by the plan for proves of that batch
+ * size; with LOOMGPU_TUNE_FILE set it is appended to that table, which
+ * lg_plan_create reads back for the same plan and device.
+ *   tuning_out: 6 ints: lanes, merkle_wide_min, hash_threads, fri_coop_pairs,
+ *               claims_rows, trials;  ms_out: 2 doubles: the starting and the
+ *               chosen configuration's median prove time. */
+int lg_plan_autotune(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                     int device_inputs, int reps, int32_t* tuning_out, double* ms_out);
+/* Sets the launch choices for proves of `batch` instances (5 ints, as above). */
+int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning);
+
 /* HIP graphs per shape bucket (on by default; LOOMGPU_GRAPHS=0 turns the
  * default off).  For a plan whose modules share one size (one DEEP class), the
  * device phase of a prove (generated columns through the FRI commit phase)

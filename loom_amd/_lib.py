@@ -25,6 +25,9 @@ EXPORTS = {
     "lg_prove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, u32pp, sizep, sizep, u32pp, sizep, sizep]),
     "lg_prove_device": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, u32pp, sizep, sizep, u32pp, sizep,
                                        sizep]),
+    "lg_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
+    "lg_plan_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "lg_ctx_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_ctx_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_ctx_timing_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, sizep]),

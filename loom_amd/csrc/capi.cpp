@@ -220,6 +220,64 @@ int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, co
                     transcript_lens);
 }
 
+static std::vector<lg::InstanceInput> instance_inputs(lg_plan* plan, int batch, const uint32_t* const* columns,
+                                                     const uint32_t* const* perms, bool device) {
+  const size_t nc = plan->p->num_inputs(), np = plan->p->num_perms();
+  if ((nc && !columns) || (np && !perms)) throw std::runtime_error("missing inputs");
+  std::vector<lg::InstanceInput> in(batch);
+  for (int b = 0; b < batch; ++b) {
+    for (size_t k = 0; k < nc; ++k) {
+      if (!columns[b * nc + k]) throw std::runtime_error("null column pointer");
+      in[b].columns.push_back(columns[b * nc + k]);
+    }
+    for (size_t k = 0; k < np; ++k) {
+      if (!perms[b * np + k]) throw std::runtime_error("null permutation pointer");
+      in[b].perms.push_back(perms[b * np + k]);
+    }
+    in[b].device = device;
+  }
+  return in;
+}
+
+int lg_plan_autotune(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                     int device_inputs, int reps, int32_t* tuning_out, double* ms_out) {
+  return guarded([&] {
+    if (!plan || batch <= 0) throw std::runtime_error("lg_plan_autotune: bad arguments");
+    std::vector<lg::InstanceInput> in = instance_inputs(plan, batch, columns, perms, device_inputs != 0);
+    std::lock_guard<std::mutex> lock(plan->ctx->mu);
+    double ms[2] = {0, 0};
+    int trials = 0;
+    const lg::Tuning t = plan->p->autotune(in, reps, ms, &trials);
+    if (tuning_out) {
+      const int32_t v[6] = {t.lanes, t.merkle_wide_min, t.hash_threads, t.fri_coop_pairs, t.claims_rows, trials};
+      for (int i = 0; i < 6; ++i) tuning_out[i] = v[i];
+    }
+    if (ms_out) {
+      ms_out[0] = ms[0];
+      ms_out[1] = ms[1];
+    }
+    return 0;
+  });
+}
+
+int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning) {
+  return guarded([&] {
+    if (!plan || batch <= 0 || !tuning) throw std::runtime_error("lg_plan_set_tuning: bad arguments");
+    lg::Tuning t;
+    t.lanes = tuning[0];
+    t.merkle_wide_min = tuning[1];
+    t.hash_threads = tuning[2];
+    t.fri_coop_pairs = tuning[3];
+    t.claims_rows = tuning[4];
+    if (t.lanes < 1 || t.lanes > 8 || t.merkle_wide_min < 1 || t.hash_threads < 64 || t.hash_threads > 256 ||
+        t.hash_threads % 64 || t.fri_coop_pairs < 1 || t.claims_rows < 256)
+      throw std::runtime_error("lg_plan_set_tuning: value out of range");
+    std::lock_guard<std::mutex> lock(plan->ctx->mu);
+    plan->p->set_tuning(batch, t);
+    return 0;
+  });
+}
+
 int lg_ctx_set_graphs(lg_ctx* ctx, int on) {
   return guarded([&] {
     if (!ctx) throw std::runtime_error("lg_ctx_set_graphs: null ctx");

@@ -73,11 +73,11 @@ void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int 
 // to outs[b] + 8k.
 void launch_leaf_hash(const uint32_t* const* colptrs, int ncols, const int32_t* msgmap, int msglen,
                       uint32_t hdr1, uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch,
-                      hipStream_t s);
+                      hipStream_t s, int threads = 256);
 
 // Merkle level: nodes[w + i] = H(nodes[2(w+i)], nodes[2(w+i)+1]) for i < w,
 // for every tree trees[b] (flat 1-indexed node array, 8 words per node).
-void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s);
+void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s, int threads = 256);
 // Levels below `width_hi` down to `width_lo` (inclusive) in one pass over
 // workgroup-local subtrees (width_hi / width_lo <= 256).
 void launch_merkle_levels_fused(uint32_t* const* trees, int width_hi, int width_lo, int batch, hipStream_t s);
@@ -166,9 +166,23 @@ constexpr int CLAIMS_GROUP = 8;
 // Rows of a set are split over claims_split(n) blocks; part then holds
 // n_items x claims_split(n) x 6 words of partial sums (item-major).
 constexpr int CLAIMS_MAX_SPLIT = 4096;
-int claims_split(int n);
+int claims_split(int n, int rows_per_block = 1024);
 void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int n_items, int n, uint32_t* part,
-                   hipStream_t s);
+                   hipStream_t s, int rows_per_block = 1024);
+
+// Launch choices of a prove that do not change its result, picked per
+// (plan, batch) by Plan::autotune: proving lanes, the width x batch from
+// which Merkle levels run one launch per level (below it, cooperative fused
+// launches), threads per block of the leaf and level hashing kernels, the FRI
+// layer size (leaf pairs) up to which the fold and leaf hashing run
+// cooperatively, and rows per block of the claimed values.
+struct Tuning {
+  int lanes = 2;
+  int merkle_wide_min = 1 << 15;
+  int hash_threads = 256;
+  int fri_coop_pairs = 512;
+  int claims_rows = 1024;
+};
 
 // DEEP quotient on the trace domain for one class: per row x (natural order)
 //   DQ[x] = sum_p (c_p - sum_{bundles i at p} a_i * P_i(x)) / (z_p - w^x)
@@ -269,9 +283,10 @@ void launch_fri_challenge(const uint32_t* prefix, int n_prefix, const uint32_t* 
                           uint32_t* dig, uint32_t* alpha, uint32_t* roots, int batch, hipStream_t s);
 // FRI fold into out (length half) fused with the leaf hashes of out's tree
 // (row pairs, one E6 column) written to leaves[b] (half/2 digests).
+// Layers of at most coop_pairs leaf pairs use the cooperative form.
 void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha,
                           const uint32_t* xinv2, int half, uint32_t hdr2, uint32_t* const* leaves, int batch,
-                          hipStream_t s);
+                          hipStream_t s, int coop_pairs = 512);
 void launch_fri_fold(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha, const uint32_t* xinv2,
                      int half, int batch, hipStream_t s);
 // running[k] += gamma * level[k]

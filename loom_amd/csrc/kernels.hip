@@ -505,9 +505,10 @@ __global__ __launch_bounds__(256) void k_leaf_hash(const uint32_t* const* __rest
 }
 
 void launch_leaf_hash(const uint32_t* const* colptrs, int ncols, const int32_t* msgmap, int msglen, uint32_t hdr1,
-                      uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch, hipStream_t s) {
+                      uint32_t hdr2, uint32_t* const* outs, int nleaves, int batch, hipStream_t s, int threads) {
   if (nleaves == 0 || batch == 0) return;
-  const int T = 256;
+  if (threads < 64 || threads > 256 || (threads & 63)) throw std::runtime_error("leaf hash: threads must be 64..256");
+  const int T = threads;
   if (!prelaunch()) return;
   hipLaunchKernelGGL(k_leaf_hash, dim3((nleaves + T - 1) / T, batch), dim3(T), 0, s, colptrs, ncols, msgmap, msglen,
                      hdr1, hdr2, outs, nleaves);
@@ -538,8 +539,9 @@ __global__ __launch_bounds__(256) void k_merkle_level(uint32_t* const* __restric
   store8(nodes + 8 * node, d);
 }
 
-void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s) {
-  const int T = width < 256 ? width : 256;
+void launch_merkle_level(uint32_t* const* trees, int width, int batch, hipStream_t s, int threads) {
+  if (threads < 64 || threads > 256 || (threads & 63)) throw std::runtime_error("merkle level: threads must be 64..256");
+  const int T = width < threads ? width : threads;
   if (!prelaunch()) return;
   hipLaunchKernelGGL(k_merkle_level, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, width);
   LG_HIP(hipGetLastError());
@@ -1370,12 +1372,14 @@ __global__ __launch_bounds__(256) void k_claims_sum(const ClaimItem* items, int 
 
 // 1024 rows per block (a 40-instance batch of 2^14-row modules: 640 blocks;
 // one 2^22-row module: 4096 blocks)
-int claims_split(int n) { return std::max(1, std::min(CLAIMS_MAX_SPLIT, n >> 10)); }
+int claims_split(int n, int rows_per_block) {
+  return std::max(1, std::min(CLAIMS_MAX_SPLIT, n / std::max(256, rows_per_block)));
+}
 
 void launch_claims(const ClaimSet* sets, int n_sets, const ClaimItem* items, int n_items, int n, uint32_t* part,
-                   hipStream_t s) {
+                   hipStream_t s, int rows_per_block) {
   if (n_sets == 0 || n_items == 0) return;
-  const int S = claims_split(n);
+  const int S = claims_split(n, rows_per_block);
   if (S > 1 && !part) throw std::runtime_error("launch_claims: partial-sum buffer missing");
   if (!prelaunch()) return;
   hipLaunchKernelGGL(k_claims, dim3(n_sets, S), dim3(256), 0, s, sets, items, n, part);
@@ -1477,7 +1481,6 @@ __global__ void k_fri_fold(const uint32_t* const* in, uint32_t* const* out, cons
 }
 
 // Layers of at most this many leaf pairs take the cooperative form below.
-constexpr int FRI_COOP_MAX_PAIRS = 512;
 
 // One FRI fold fused with the leaf hashing of the folded layer's tree: thread
 // k forms next[2k] and next[2k+1] (from in[4k .. 4k+3]), stores them, and
@@ -1561,11 +1564,11 @@ __global__ __launch_bounds__(256) void k_fri_fold_leaf_coop(const uint32_t* cons
 
 void launch_fri_fold_leaf(const uint32_t* const* in, uint32_t* const* out, const uint32_t* alpha,
                           const uint32_t* xinv2, int half, uint32_t hdr2, uint32_t* const* leaves, int batch,
-                          hipStream_t s) {
+                          hipStream_t s, int coop_pairs) {
   if (half < 2 || (half & 1)) throw std::runtime_error("fri fold+leaf: layer length must be even");
   const int pairs = half / 2;
   if (!prelaunch()) return;
-  if (pairs <= FRI_COOP_MAX_PAIRS) {
+  if (pairs <= coop_pairs) {
     // rows of 16 lanes, 16 rows per block; rows past the layer return whole
     hipLaunchKernelGGL(k_fri_fold_leaf_coop, dim3((pairs + 15) / 16, batch), dim3(256), 0, s, in, out, alpha, xinv2,
                        half, hdr2, leaves);

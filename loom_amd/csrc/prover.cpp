@@ -296,7 +296,10 @@ void Staging::flush(hipStream_t s) {
 }
 
 Context::Context() {
-  if (const char* e = getenv("LOOMGPU_LANES")) lanes = std::max(1, std::min(8, atoi(e)));
+  if (const char* e = getenv("LOOMGPU_LANES")) {
+    lanes = std::max(1, std::min(8, atoi(e)));
+    lanes_forced = true;
+  }
   if (const char* e = getenv("LOOMGPU_GRAPHS")) graphs = atoi(e) != 0;
   lane_.resize(8);
 }
@@ -422,7 +425,145 @@ static std::shared_ptr<const ChallengeSchedule> challenge_schedule(const PlanDat
   return s;
 }
 
+// FNV-1a over the plan words: the key of a plan in the tuning table.
+static uint64_t plan_hash(const uint32_t* w, size_t n) {
+  uint64_t h = 1469598103934665603ull;
+  for (size_t i = 0; i < n; ++i)
+    for (int b = 0; b < 4; ++b) {
+      h ^= (w[i] >> (8 * b)) & 0xffu;
+      h *= 1099511628211ull;
+    }
+  return h;
+}
+
+static std::string device_arch(int dev) {
+  hipDeviceProp_t p;
+  if (hipGetDeviceProperties(&p, dev) != hipSuccess) return "unknown";
+  std::string a = p.gcnArchName;
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
+Tuning Plan::tuning(int batch) const {
+  auto it = tuned_.find(batch);
+  return it == tuned_.end() ? Tuning() : it->second;
+}
+
+void Plan::set_tuning(int batch, const Tuning& t) {
+  tuned_[batch] = t;
+  drop_graphs();  // recorded graphs hold the previous launch choices
+}
+
+Tuning Plan::autotune(const std::vector<InstanceInput>& inputs, int reps, double* ms, int* trials) {
+  if (inputs.empty()) throw std::runtime_error("autotune: no instances");
+  const int B = (int)inputs.size();
+  reps = std::max(1, std::min(reps, 9));
+  std::vector<InstanceInput> in = inputs;
+  for (auto& x : in) {  // the trial proofs are discarded
+    x.proof_dst = nullptr;
+    x.proof_cap = 0;
+  }
+  // Trials run as production proves do: with graphs on, a trial's first
+  // prove runs eagerly and its second records the graph, the timed ones
+  // replay it.
+  drop_graphs();
+  int n_trials = 0;
+  auto time_it = [&](const Tuning& t) {
+    tuned_[B] = t;
+    drop_graphs();
+    std::vector<ProveOutput> out;
+    prove(in, out);  // warm-up: tables, buffers
+    if (ctx_->graphs) prove(in, out);  // records the graph
+    std::vector<double> v;
+    for (int r = 0; r < reps; ++r) {
+      const double a = now_s();
+      prove(in, out);
+      v.push_back((now_s() - a) * 1e3);
+    }
+    std::sort(v.begin(), v.end());
+    ++n_trials;
+    return v[v.size() / 2];
+  };
+  Tuning best = tuning(B);
+  double best_ms = 0;
+  try {
+    best_ms = time_it(best);
+    if (ms) ms[0] = best_ms;
+    auto consider = [&](Tuning t) {
+      const double m = time_it(t);
+      if (m < 0.99 * best_ms) {
+        best = t;
+        best_ms = m;
+      }
+    };
+    for (int v : {1, 2, 3})
+      if (v != best.lanes && (v == 1 || B >= 8 * v)) {
+        Tuning t = best;
+        t.lanes = v;
+        consider(t);
+      }
+    for (int v : {1 << 13, 1 << 14, 1 << 15, 1 << 16, 1 << 17})
+      if (v != best.merkle_wide_min) {
+        Tuning t = best;
+        t.merkle_wide_min = v;
+        consider(t);
+      }
+    for (int v : {64, 128, 256})
+      if (v != best.hash_threads) {
+        Tuning t = best;
+        t.hash_threads = v;
+        consider(t);
+      }
+    for (int v : {128, 512, 2048})
+      if (v != best.fri_coop_pairs) {
+        Tuning t = best;
+        t.fri_coop_pairs = v;
+        consider(t);
+      }
+    for (int v : {512, 1024, 4096})
+      if (v != best.claims_rows) {
+        Tuning t = best;
+        t.claims_rows = v;
+        consider(t);
+      }
+  } catch (...) {
+    tuned_[B] = best;
+    drop_graphs();
+    throw;
+  }
+  tuned_[B] = best;
+  drop_graphs();
+  if (ms) ms[1] = best_ms;
+  if (trials) *trials = n_trials;
+  if (const char* f = getenv("LOOMGPU_TUNE_FILE")) {
+    if (FILE* fp = fopen(f, "a")) {
+      fprintf(fp, "LGTUNE1 %016llx %d %s %d %d %d %d %d %.4f\n", (unsigned long long)hash_, B,
+              device_arch(ctx_->device).c_str(), best.lanes, best.merkle_wide_min, best.hash_threads,
+              best.fri_coop_pairs, best.claims_rows, best_ms);
+      fclose(fp);
+    }
+  }
+  return best;
+}
+
 Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_plan(words, n)) {
+  hash_ = plan_hash(words, n);
+  if (const char* f = getenv("LOOMGPU_TUNE_FILE")) {  // earlier autotune results for this plan and device
+    if (FILE* fp = fopen(f, "r")) {
+      const std::string arch = device_arch(ctx->device);
+      char tag[16], hs[32], ar[64];
+      int batch;
+      Tuning t;
+      double m;
+      while (fscanf(fp, "%15s %31s %d %63s %d %d %d %d %d %lf", tag, hs, &batch, ar, &t.lanes, &t.merkle_wide_min,
+                    &t.hash_threads, &t.fri_coop_pairs, &t.claims_rows, &m) == 10) {
+        if (std::string(tag) == "LGTUNE1" && strtoull(hs, nullptr, 16) == hash_ && arch == ar && t.lanes >= 1 &&
+            t.lanes <= 8 && t.hash_threads >= 64 && t.hash_threads <= 256 && t.hash_threads % 64 == 0)
+          tuned_[batch] = t;  // the last line of a key wins
+      }
+      fclose(fp);
+    }
+  }
   for (auto& m : d_.modules) {
     if (m.log_n > FRI_LOG_D) throw std::runtime_error("module size above 2^22 (MAX_LOG_MODULE_SIZE)");
     if (m.log_n < 1) throw std::runtime_error("module size below 2");
@@ -637,7 +778,8 @@ class Job {
   Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb,
       int lane, GraphBucket* gb = nullptr, Mode mode = EAGER)
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
-        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER) {
+        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER),
+        tune_(plan.tuning(plan.tuning_batch())) {
     if (out.size() != in.size()) throw std::runtime_error("prove: output slots do not match the inputs");
     if (gb_) {
       if (!gb_->stg) gb_->stg = std::make_unique<Staging>(64u << 20);
@@ -845,6 +987,7 @@ class Job {
   std::unique_ptr<Staging> stg_;
   GraphBucket* gb_;
   Mode mode_;
+  Tuning tune_;  // the plan's launch choices for this prove's batch
   bool in_region_ = false;
   size_t next_buf_ = 0, n_pins_ = 0;
   // the fold rounds before the level enters, chained on the device
@@ -1311,7 +1454,7 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
     uint32_t* const* outs_d = const_cast<uint32_t* const*>(stage(outs));
     TimeIt tm(ctx_, s_, "leaf_hash", (double)B_ * width * (2.0 * ncols + 8) * 4,
               (double)B_ * width * ((msg.size() + 15) / 16));
-    launch_leaf_hash(cp_d, ncols, msg_d, (int)msg.size(), M(2 * nb), M(2 * ne), outs_d, width, B_, s_);
+    launch_leaf_hash(cp_d, ncols, msg_d, (int)msg.size(), M(2 * nb), M(2 * ne), outs_d, width, B_, s_, tune_.hash_threads);
   }
   std::vector<uint32_t*> tp(B_);
   for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
@@ -1346,10 +1489,10 @@ bool Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* c
   // Wide levels: one launch per level keeps every lane busy.  Once a level
   // has few nodes per batch, the remaining levels run in one fused launch
   // (workgroup-local subtrees), trading idle lanes for fewer launches.
-  while (w >= 1 && (size_t)w * B_ >= (size_t)(1 << 15)) {
+  while (w >= 1 && (size_t)w * B_ >= (size_t)tune_.merkle_wide_min) {
     {
       TimeIt tm(ctx_, s_, "merkle", (double)B_ * w * 3 * 32, (double)B_ * w);
-      launch_merkle_level(tp_d, w, B_, s_);
+      launch_merkle_level(tp_d, w, B_, s_, tune_.hash_threads);
     }
     fold(w);
     w /= 2;
@@ -2041,7 +2184,7 @@ void Job::open() {
       sc.bytes += (double)n * 24 + set.count * 24.0;
     }
   for (auto& kv : cl) {
-    const int S = claims_split((int)kv.first);
+    const int S = claims_split((int)kv.first, tune_.claims_rows);
     uint32_t* part = nullptr;
     if (S > 1) {
       keep_.push_back(dev(kv.second.items.size() * (size_t)S * 6));
@@ -2049,7 +2192,7 @@ void Job::open() {
     }
     TimeIt tm(ctx_, s_, "claims", kv.second.bytes, 0);
     launch_claims(stage(kv.second.sets), (int)kv.second.sets.size(), stage(kv.second.items),
-                  (int)kv.second.items.size(), kv.first, part, s_);
+                  (int)kv.second.items.size(), kv.first, part, s_, tune_.claims_rows);
   }
   // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
   std::vector<int32_t> bind_order;  // claimed-value indexes in binding order
@@ -2416,7 +2559,8 @@ void Job::open() {
       TreeBuf& tb = ftrees[j];
       if (!leaves_done[j]) {
         TimeIt tm(ctx_, s_, "leaf_hash", (double)B_ * tb.nleaves * (2.0 * 6 + 8) * 4, (double)B_ * tb.nleaves);
-        launch_leaf_hash(limb_d[j], 6, ext_msg_d, (int)ext_msg.size(), 0u, hdr_ext, leaf_d[j], tb.nleaves, B_, s_);
+        launch_leaf_hash(limb_d[j], 6, ext_msg_d, (int)ext_msg.size(), 0u, hdr_ext, leaf_d[j], tb.nleaves, B_, s_,
+                         tune_.hash_threads);
       }
       uint32_t* dig_j = chal_dig.as<uint32_t>() + (size_t)j * fb;
       uint32_t* al_j = chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6;
@@ -2434,7 +2578,8 @@ void Job::open() {
     const int half = (int)layers[j + 1].len;
     if (j + 1 < FRI_ROUNDS && !joins[j + 1]) {
       TimeIt tm(ctx_, s_, "fri_fold_leaf", (double)B_ * layers[j].len * 1.5 * 24, (double)B_ * half / 2);
-      launch_fri_fold_leaf(lay_d[j], lay_d[j + 1], alpha_j, xi, half, hdr_ext, leaf_d[j + 1], B_, s_);
+      launch_fri_fold_leaf(lay_d[j], lay_d[j + 1], alpha_j, xi, half, hdr_ext, leaf_d[j + 1], B_, s_,
+                           tune_.fri_coop_pairs);
       leaves_done[j + 1] = 1;
     } else {
       TimeIt tm(ctx_, s_, "fri_fold", (double)B_ * layers[j].len * 1.5 * 24, 0);
@@ -2936,8 +3081,10 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   if (d_.n_setup && !setup_) throw std::runtime_error("prove: the plan's setup columns are not committed (lg_plan_set_setup)");
   outputs.assign(inputs.size(), ProveOutput());
   const int B = (int)inputs.size();
+  cur_batch_ = B;
   constexpr int kMinLaneBatch = 8;
-  int lanes = std::min(ctx_->lanes, std::max(1, B / kMinLaneBatch));
+  const int want_lanes = ctx_->lanes_forced ? ctx_->lanes : tuning(B).lanes;
+  int lanes = std::min(want_lanes, std::max(1, B / kMinLaneBatch));
   if (ctx_->timer.on) lanes = 1;  // per-kernel timing wants one stream
   struct BucketHold {  // returns a lane's bucket however the lane ends
     Plan* p;

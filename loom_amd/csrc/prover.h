@@ -147,6 +147,7 @@ struct Context {
   // lane's host work (transcripts, query derivation, proof assembly)
   // overlaps another lane's kernels.  LOOMGPU_LANES overrides the default.
   int lanes = 2;
+  bool lanes_forced = false;  // LOOMGPU_LANES set: overrides the tuned lane count
   hipStream_t lane_stream(int lane);
   // a non-blocking stream for one-off copies beside the lanes' work
   hipStream_t side_stream();
@@ -252,6 +253,20 @@ class Plan {
   // the Fiat-Shamir challenges of one proof of this plan, in order
   std::shared_ptr<const ChallengeSchedule> schedule() const { return sched_; }
 
+  // Launch choices for proves of `batch` instances (defaults until tuned) and
+  // the batch of the prove in progress.
+  Tuning tuning(int batch) const;
+  int tuning_batch() const { return cur_batch_; }
+  void set_tuning(int batch, const Tuning& t);
+  // The autotuner: times whole proves of these instances (as they run in
+  // production: graph replays when graphs are on) over a
+  // coordinate descent of the Tuning fields, starting from the current
+  // choices, and keeps the fastest for this batch size.  With
+  // LOOMGPU_TUNE_FILE set the result is appended to that table, which plan
+  // creation reads back.  Returns the choice; ms[0] = the starting
+  // configuration's median prove time, ms[1] = the chosen one's.
+  Tuning autotune(const std::vector<InstanceInput>& inputs, int reps, double* ms, int* trials);
+
  private:
   Context* ctx_;
   PlanData d_;
@@ -259,6 +274,9 @@ class Plan {
   std::map<std::tuple<size_t, size_t, size_t>, std::shared_ptr<DagKernel>> steps_;
   std::shared_ptr<const ChallengeSchedule> sched_;
   std::unique_ptr<SetupData> setup_;
+  std::map<int, Tuning> tuned_;
+  int cur_batch_ = 0;
+  uint64_t hash_ = 0;  // of the plan words (tuning table key)
   // Graph buckets by (batch, lane); a plan whose modules form one DEEP class
   // (its device phase needs no host round trip) is eligible.
   bool graph_eligible_ = false;

@@ -272,6 +272,29 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
     return res
 
 
+TUNING_FIELDS = ["lanes", "merkle_wide_min", "hash_threads", "fri_coop_pairs", "claims_rows"]
+
+
+def autotune(ctx, items, dev_inputs=None, reps=3):
+    """Runs the library's autotuner on these prepared instances (one plan);
+    returns {"tuning": {field: value}, "trials": n, "start_ms": ..., "best_ms": ...}.
+    The choice applies to later proves of this batch size on this context."""
+    words = items[0].words
+    ph = ctx.plan_handle(words, items[0].setup)
+    col_ptrs, perm_ptrs = _pointer_tables(items, dev_inputs)
+    t = (ctypes.c_int32 * 6)()
+    ms = (ctypes.c_double * 2)()
+    _lib.check(_lib.lib().lg_plan_autotune(ph, len(items), col_ptrs, perm_ptrs, 0 if dev_inputs is None else 1, reps,
+                                           t, ms), "lg_plan_autotune")
+    return {"tuning": dict(zip(TUNING_FIELDS, list(t)[:5])), "trials": t[5], "start_ms": ms[0], "best_ms": ms[1]}
+
+
+def set_tuning(ctx, items, tuning):
+    ph = ctx.plan_handle(items[0].words, items[0].setup)
+    v = (ctypes.c_int32 * 5)(*[int(tuning[f]) for f in TUNING_FIELDS])
+    _lib.check(_lib.lib().lg_plan_set_tuning(ph, len(items), v), "lg_plan_set_tuning")
+
+
 def prove(ctx, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, want_transcript=False):
     r = prove_prepared(ctx, [Prepared(program, trace, num_queries)], want_transcript)
     if want_transcript:

@@ -414,3 +414,41 @@ def test_graph_replay_equals_eager():
     finally:
         g.close()
         e.close()
+
+
+@pytest.mark.gpu
+def test_tuning_choices_keep_the_proofs(tmp_path, monkeypatch):
+    """Launch choices change how a prove runs, not what it proves: the proofs
+    under extreme settings of every knob equal those under the defaults; the
+    autotuner returns a valid choice, keeps it for the batch size and appends
+    it to LOOMGPU_TUNE_FILE."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+    from loom_amd import prover, workloads
+    table = tmp_path / "tune.txt"
+    monkeypatch.setenv("LOOMGPU_TUNE_FILE", str(table))
+    items = [prover.Prepared(*workloads.plonk_instance(11, 300 + k), 16) for k in range(16)]
+    ctx = prover.Context(0)
+    try:
+        want = prover.prove_prepared(ctx, items)
+        for tun in (dict(lanes=1, merkle_wide_min=1 << 13, hash_threads=64, fri_coop_pairs=2048, claims_rows=4096),
+                    dict(lanes=2, merkle_wide_min=1 << 17, hash_threads=128, fri_coop_pairs=128, claims_rows=512)):
+            log("tuning %s" % tun)
+            prover.set_tuning(ctx, items, tun)
+            for _ in range(3):  # eager, recorded, replayed
+                got = prover.prove_prepared(ctx, items)
+                for a, b in zip(got, want):
+                    assert np.array_equal(a, b), tun
+        r = prover.autotune(ctx, items, reps=1)
+        log("autotune: %s" % r)
+        t = r["tuning"]
+        assert t["lanes"] in (1, 2) and t["hash_threads"] in (64, 128, 256) and r["trials"] >= 10
+        assert r["best_ms"] <= r["start_ms"] * 1.05
+        got = prover.prove_prepared(ctx, items)
+        for a, b in zip(got, want):
+            assert np.array_equal(a, b)
+        lines = table.read_text().split("\n")
+        assert any(ln.startswith("LGTUNE1 ") and " 16 " in ln for ln in lines)
+    finally:
+        ctx.close()
