@@ -94,6 +94,9 @@ void launch_merkle_coop_challenge(uint32_t* const* trees, int width_hi, const ui
 // roots[8b .. 8b+8) = trees[b][8 .. 16)  (node 1 of every tree)
 void launch_copy_roots(const uint32_t* const* trees, uint32_t* roots, int batch, hipStream_t s);
 // Injection fold: nodes[w + j] = H(nodes[w + j], inj[b][j]).
+// count injections at one level width, applied in order (inj[k * batch + b]).
+void launch_inject_seq(uint32_t* const* trees, const uint32_t* const* inj, int count, int width, int batch,
+                       hipStream_t s);
 void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s);
 
 // Batched copy: for job k, dst[k][0 .. n) = src + off[k] (n words each).
@@ -200,7 +203,8 @@ struct DeepDesc {
   int n_cols;
   const uint32_t* scale;      // [b][bundle][6]  alpha^i
   const uint32_t* consts;     // [b][point][6]   c_p
-  const uint32_t* points;     // [b][point][6]   z_p
+  const uint32_t* points;     // [b % points_mod][point][6]   z_p
+  int points_mod;             // 0: points per launch instance b
   const uint32_t* wpow;       // w^x
   uint32_t* const* out;       // [b] 6 limb arrays, stride n
 };

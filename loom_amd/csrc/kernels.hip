@@ -742,6 +742,33 @@ __global__ __launch_bounds__(256) void k_inject(uint32_t* const* __restrict__ tr
   store8(nodes + 8 * ((size_t)width + j), d);
 }
 
+// count injections at one level width: node j of tree b becomes
+// H(...H(H(node, inj[0][b][j]), inj[1][b][j])..., inj[count-1][b][j]) (the
+// schedule order of the level's injections); inj is [k * batch + b].
+__global__ __launch_bounds__(256) void k_inject_seq(uint32_t* const* __restrict__ trees,
+                                                    const uint32_t* const* __restrict__ inj, int count, int width,
+                                                    int batch) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= width) return;
+  uint32_t* nodes = trees[blockIdx.y];
+  uint32_t l[8], r[8];
+  load8(nodes + 8 * ((size_t)width + j), l);
+  for (int k = 0; k < count; ++k) {
+    load8(inj[(size_t)k * batch + blockIdx.y] + 8 * (size_t)j, r);
+    p2_node(l, r, l);
+  }
+  store8(nodes + 8 * ((size_t)width + j), l);
+}
+
+void launch_inject_seq(uint32_t* const* trees, const uint32_t* const* inj, int count, int width, int batch,
+                       hipStream_t s) {
+  if (count <= 0) return;
+  const int T = width < 256 ? width : 256;
+  if (!prelaunch()) return;
+  hipLaunchKernelGGL(k_inject_seq, dim3((width + T - 1) / T, batch), dim3(T), 0, s, trees, inj, count, width, batch);
+  LG_HIP(hipGetLastError());
+}
+
 void launch_inject(uint32_t* const* trees, const uint32_t* const* inj, int width, int batch, hipStream_t s) {
   const int T = width < 256 ? width : 256;
   if (!prelaunch()) return;
@@ -1399,9 +1426,10 @@ constexpr int DEEP_MAX_POINTS = 8;
 template <int NP, int C0, int C>
 __device__ __forceinline__ void deep_points(const E6* num, const DeepDesc& d, int b, uint32_t wx, E6& acc) {
   E6 den[C], pre[C];
+  const int pb = d.points_mod ? b % d.points_mod : b;
 #pragma unroll
   for (int k = 0; k < C; ++k) {
-    den[k] = ldc6(d.points + ((size_t)b * NP + C0 + k) * 6);
+    den[k] = ldc6(d.points + ((size_t)pb * NP + C0 + k) * 6);
     den[k].v[0] = kb_sub(den[k].v[0], wx);
     pre[k] = k ? e6_mul(pre[k - 1], den[k]) : den[k];
   }

@@ -971,7 +971,9 @@ class Job {
     uint32_t *dig, *alpha, *roots;
   };
   bool tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* chal = nullptr);
-  void quotient(size_t m);
+  // AIR quotients of modules of one class (same size, chunk count and
+  // relation kernel code), proved together as (module, instance) pairs.
+  void quotient(const std::vector<size_t>& mods);
   void open();
 
   Plan& plan_;
@@ -1428,33 +1430,49 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
   tb.nodes = dev((size_t)B_ * 2 * tb.nleaves * 8);
   tb.inj.clear();
   tb.inj_width.clear();
+  // Groups of one shape (base and extension columns) and width hash in one
+  // launch: the (group, instance) pairs form the launch's batch.
+  struct LeafClass {
+    int nb, ne, width;
+    std::vector<const uint32_t*> cp;  // [member][b][column]
+    std::vector<uint32_t*> outs;      // [member][b]
+  };
+  std::vector<LeafClass> classes;
   for (size_t k = 0; k < ng; ++k) {
     const size_t g = tb.order[k];
     const int nb = gshape[g].first, ne = gshape[g].second;
-    const int ncols = nb + 6 * ne;
     const int width = grows[g] / 2;
-    std::vector<const uint32_t*> cp;
-    cp.reserve((size_t)B_ * ncols);
-    for (int b = 0; b < B_; ++b) cp.insert(cp.end(), gcols[g * B_ + b].begin(), gcols[g * B_ + b].end());
+    if (k > 0) {
+      tb.inj.push_back(dev((size_t)B_ * width * 8));
+      tb.inj_width.push_back(width);
+    }
+    LeafClass* lc = nullptr;
+    for (auto& c : classes)
+      if (c.nb == nb && c.ne == ne && c.width == width) lc = &c;
+    if (!lc) {
+      classes.push_back(LeafClass{nb, ne, width, {}, {}});
+      lc = &classes.back();
+    }
+    for (int b = 0; b < B_; ++b) {
+      lc->cp.insert(lc->cp.end(), gcols[g * B_ + b].begin(), gcols[g * B_ + b].end());
+      lc->outs.push_back(k == 0 ? tb.node_ptr(b) + (size_t)8 * tb.nleaves : tb.inj_ptr(tb.inj.size() - 1, b));
+    }
+  }
+  for (auto& lc : classes) {
+    const int nb = lc.nb, ne = lc.ne, ncols = nb + 6 * ne, width = lc.width;
+    const int nbatch = (int)lc.outs.size();
     std::vector<int32_t> msg(3, 0);
     for (int i = 0; i < nb; ++i) msg.push_back(2 * i);
     for (int i = 0; i < nb; ++i) msg.push_back(2 * i + 1);
     for (int i = 0; i < 6 * ne; ++i) msg.push_back(2 * (nb + i));
     for (int i = 0; i < 6 * ne; ++i) msg.push_back(2 * (nb + i) + 1);
-    std::vector<uint32_t*> outs(B_);
-    if (k == 0) {
-      for (int b = 0; b < B_; ++b) outs[b] = tb.node_ptr(b) + (size_t)8 * tb.nleaves;
-    } else {
-      tb.inj.push_back(dev((size_t)B_ * width * 8));
-      tb.inj_width.push_back(width);
-      for (int b = 0; b < B_; ++b) outs[b] = tb.inj_ptr(tb.inj.size() - 1, b);
-    }
-    const uint32_t* const* cp_d = stage(cp);
+    const uint32_t* const* cp_d = stage(lc.cp);
     const int32_t* msg_d = stage(msg);
-    uint32_t* const* outs_d = const_cast<uint32_t* const*>(stage(outs));
-    TimeIt tm(ctx_, s_, "leaf_hash", (double)B_ * width * (2.0 * ncols + 8) * 4,
-              (double)B_ * width * ((msg.size() + 15) / 16));
-    launch_leaf_hash(cp_d, ncols, msg_d, (int)msg.size(), M(2 * nb), M(2 * ne), outs_d, width, B_, s_, tune_.hash_threads);
+    uint32_t* const* outs_d = const_cast<uint32_t* const*>(stage(lc.outs));
+    TimeIt tm(ctx_, s_, "leaf_hash", (double)nbatch * width * (2.0 * ncols + 8) * 4,
+              (double)nbatch * width * ((msg.size() + 15) / 16));
+    launch_leaf_hash(cp_d, ncols, msg_d, (int)msg.size(), M(2 * nb), M(2 * ne), outs_d, width, nbatch, s_,
+                     tune_.hash_threads);
   }
   std::vector<uint32_t*> tp(B_);
   for (int b = 0; b < B_; ++b) tp[b] = tb.node_ptr(b);
@@ -1469,15 +1487,20 @@ void Job::build_tree(TreeBuf& tb, const std::vector<std::vector<const uint32_t*>
 bool Job::tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* chal) {
   bool chal_done = false;
   size_t next = 0;
+  // every injection at one level width in one launch, applied per node in
+  // schedule order
   auto fold = [&](int width) {
+    std::vector<const uint32_t*> ip;  // [injection][b]
+    int count = 0;
     while (next < tb.inj.size() && tb.inj_width[next] == width) {
-      std::vector<const uint32_t*> ip(B_);
-      for (int b = 0; b < B_; ++b) ip[b] = tb.inj_ptr(next, b);
-      const uint32_t* const* ip_d = stage(ip);
-      TimeIt tm(ctx_, s_, "merkle_inject", (double)B_ * width * 3 * 32, (double)B_ * width);
-      launch_inject(tp_d, ip_d, width, B_, s_);
+      for (int b = 0; b < B_; ++b) ip.push_back(tb.inj_ptr(next, b));
+      ++count;
       ++next;
     }
+    if (!count) return;
+    const uint32_t* const* ip_d = stage(ip);
+    TimeIt tm(ctx_, s_, "merkle_inject", (double)B_ * width * (2 + count) * 32, (double)B_ * width * count);
+    launch_inject_seq(tp_d, ip_d, count, width, B_, s_);
   };
   auto has_inj = [&](int width) {
     for (size_t k = next; k < tb.inj.size(); ++k)
@@ -1879,9 +1902,10 @@ void Job::replay_device_fs() {
 // generated DAG kernel divides by X^n - 1, per-coset inverse transforms and a
 // rho-point combination give the coefficients of Q, each n-chunk is then
 // evaluated on H (Lagrange form) and on the rate-4 LDE domain.
-void Job::quotient(size_t mi) {
-  const ModuleInfo& m = d_.modules[mi];
-  const DagKernel& K = *plan_.relation_kernel(mi);
+void Job::quotient(const std::vector<size_t>& mods) {
+  const size_t NM = mods.size();
+  const ModuleInfo& m = d_.modules[mods[0]];
+  const DagKernel& K0 = *plan_.relation_kernel(mods[0]);
   const uint32_t n = m.N;
   const int logn = m.log_n;
   const int rho = (int)m.n_chunks;
@@ -1889,19 +1913,26 @@ void Job::quotient(size_t mi) {
   while ((1 << rho_log) < rho) ++rho_log;
   if ((1 << rho_log) != rho) throw std::runtime_error("quotient: chunk count is not a power of two");
   const size_t rn = (size_t)rho * n;
-  // 1. coset evaluations of every leaf column
-  std::vector<DevBuf> cos(K.col_ids.size());
+  const int MB = (int)NM * B_;  // the launch batch: (member module, instance) pairs, mb = member * B + b
+  const size_t nslot = K0.col_ids.size();
+  // 1. coset evaluations of every leaf column of every member
+  std::vector<std::vector<DevBuf>> cos(NM);
   std::vector<NttJob> jobs;
-  for (size_t k = 0; k < K.col_ids.size(); ++k) {
-    const ColBuf& cb = cols_[K.col_ids[k]];
-    cos[k] = dev((size_t)B_ * cb.limbs * rn);
-    for (int b = 0; b < B_; ++b)
-      for (int l = 0; l < cb.limbs; ++l) {
-        NttJob j{};
-        j.in = cb.lag(b, l);
-        for (int i = 0; i < rho; ++i) j.out[i] = cos[k].as<uint32_t>() + ((size_t)b * cb.limbs + l) * rn + (size_t)i * n;
-        jobs.push_back(j);
-      }
+  for (size_t mm = 0; mm < NM; ++mm) {
+    const DagKernel& K = *plan_.relation_kernel(mods[mm]);
+    cos[mm].resize(nslot);
+    for (size_t k = 0; k < nslot; ++k) {
+      const ColBuf& cb = cols_[K.col_ids[k]];
+      cos[mm][k] = dev((size_t)B_ * cb.limbs * rn);
+      for (int b = 0; b < B_; ++b)
+        for (int l = 0; l < cb.limbs; ++l) {
+          NttJob j{};
+          j.in = cb.lag(b, l);
+          for (int i = 0; i < rho; ++i)
+            j.out[i] = cos[mm][k].as<uint32_t>() + ((size_t)b * cb.limbs + l) * rn + (size_t)i * n;
+          jobs.push_back(j);
+        }
+    }
   }
   NttCfg cfg{};
   cfg.kind = NTT_KIND_QUOT_COSET;
@@ -1914,42 +1945,58 @@ void Job::quotient(size_t mi) {
     ntt(cfg, jobs);
   }
   // 2. numerator / Z_H on the cosets
-  DevBuf Q = dev((size_t)B_ * 6 * rn);
+  DevBuf Q = dev((size_t)MB * 6 * rn);
   {
     std::vector<const uint32_t*> cp;
-    for (int b = 0; b < B_; ++b)
-      for (size_t k = 0; k < K.col_ids.size(); ++k)
-        cp.push_back(cos[k].as<uint32_t>() + (size_t)b * cols_[K.col_ids[k]].limbs * rn);
+    std::vector<uint32_t*> op;
+    for (size_t mm = 0; mm < NM; ++mm) {
+      const DagKernel& K = *plan_.relation_kernel(mods[mm]);
+      for (int b = 0; b < B_; ++b) {
+        for (size_t k = 0; k < nslot; ++k)
+          cp.push_back(cos[mm][k].as<uint32_t>() + (size_t)b * cols_[K.col_ids[k]].limbs * rn);
+        op.push_back(Q.as<uint32_t>() + ((size_t)mm * B_ + b) * 6 * rn);
+      }
+    }
     // (g w_{rho n}^i)^n - 1 = g^n w_rho^i - 1
     std::vector<uint32_t> zh(rho);
     const uint32_t gn = kb_pow(M(MULT_GEN), n);
     for (int i = 0; i < rho; ++i) zh[i] = kb_inv(kb_sub(kb_mul(gn, kb_pow(omega(rho_log), i)), KB_R));
-    std::vector<uint32_t*> op(B_);
-    for (int b = 0; b < B_; ++b) op[b] = Q.as<uint32_t>() + (size_t)b * 6 * rn;
+    // challenge operands of every member, [member][b][slot][6]
+    const uint32_t* chal = nullptr;
+    if (!K0.chal_ids.empty()) {
+      keep_.push_back(dev((size_t)MB * K0.chal_ids.size() * 6));
+      for (size_t mm = 0; mm < NM; ++mm) {
+        const DagKernel& K = *plan_.relation_kernel(mods[mm]);
+        launch_chal_compact(chal_dev_.as<uint32_t>(), (int)d_.cols.size(), stage(K.chal_ids), (int)K.chal_ids.size(),
+                            keep_.back().as<uint32_t>() + mm * (size_t)B_ * K.chal_ids.size() * 6, B_, s_);
+      }
+      chal = keep_.back().as<uint32_t>();
+    }
     DagLaunch a{};
     a.cols = stage(cp);
-    a.chal = chal_operands(K.chal_ids);
+    a.chal = chal;
     a.rowscale = stage(zh);
     a.out = const_cast<uint32_t* const*>(stage(op));
     a.col_stride = rn;
     a.out_stride = rn;
     a.logn = logn;
     a.ncos = rho;
-    a.batch = B_;
+    a.batch = MB;
     double leaf_words = 0;
-    for (auto c : K.col_ids) leaf_words += cols_[c].limbs;
-    TimeIt tm(ctx_, s_, "dag_quotient", (double)B_ * rn * (leaf_words + 6) * 4, 0);
-    dag_kernel_launch(K, a, s_);
+    for (auto c : K0.col_ids) leaf_words += cols_[c].limbs;
+    TimeIt tm(ctx_, s_, "dag_quotient", (double)MB * rn * (leaf_words + 6) * 4, 0);
+    dag_kernel_launch(K0, a, s_);
   }
-  for (auto& c : cos) keep_.push_back(std::move(c));
+  for (auto& cm : cos)
+    for (auto& c : cm) keep_.push_back(std::move(c));
   // 3. per-coset inverse transforms with the w_{rho n}^{-a i} / (rho n) twiddle
-  DevBuf Y = dev((size_t)B_ * 6 * rn);
+  DevBuf Y = dev((size_t)MB * 6 * rn);
   jobs.clear();
-  for (int b = 0; b < B_; ++b)
+  for (int mb = 0; mb < MB; ++mb)
     for (int l = 0; l < 6; ++l)
       for (int i = 0; i < rho; ++i) {
         NttJob j{};
-        size_t off = ((size_t)b * 6 + l) * rn + (size_t)i * n;
+        size_t off = ((size_t)mb * 6 + l) * rn + (size_t)i * n;
         j.in = Q.as<uint32_t>() + off;
         j.post = tb_.quot_post(logn, i, rho_log);
         j.out[0] = Y.as<uint32_t>() + off;
@@ -1964,31 +2011,34 @@ void Job::quotient(size_t mi) {
     TimeIt tm(ctx_, s_, "ntt_quot_back", jobs.size() * 2.0 * n * 4, 0);
     ntt(c2, jobs);
   }
-  // 4. rho-point combination -> chunk coefficients [b][t][l][a]
-  DevBuf coef = dev((size_t)B_ * rho * 6 * n);
+  // 4. rho-point combination -> chunk coefficients [mb][t][l][a]
+  DevBuf coef = dev((size_t)MB * rho * 6 * n);
   {
     std::vector<uint32_t> wr(rho);
     for (int k = 0; k < rho; ++k) wr[k] = kb_inv(kb_pow(omega(rho_log), k));
-    launch_quot_combine(Y.as<uint32_t>(), coef.as<uint32_t>(), tb_.ginv_pow(logn + rho_log), stage(wr), rho, n, B_, s_);
+    launch_quot_combine(Y.as<uint32_t>(), coef.as<uint32_t>(), tb_.ginv_pow(logn + rho_log), stage(wr), rho, n, MB, s_);
   }
   // 5. chunks: LDE (4 bit-reversed blocks) + Lagrange values on H
-  std::vector<uint32_t> chunk_col(rho, PLAN_NONE);
-  for (size_t c = 0; c < d_.cols.size(); ++c)
-    if (d_.cols[c].src == SRC_AIR && d_.cols[c].module == mi && d_.cols[c].a0 < (uint32_t)rho) chunk_col[d_.cols[c].a0] = c;
   jobs.clear();
-  for (int t = 0; t < rho; ++t) {
-    if (chunk_col[t] == PLAN_NONE) throw std::runtime_error("quotient: missing chunk column");
-    ColBuf& cb = cols_[chunk_col[t]];
-    if (cb.limbs != 6 || !cb.committed) throw std::runtime_error("quotient: chunk column must be a committed E6 column");
-    for (int b = 0; b < B_; ++b)
-      for (int l = 0; l < 6; ++l) {
-        NttJob j{};
-        j.in = coef.as<uint32_t>() + (((size_t)b * rho + t) * 6 + l) * n;
-        for (int o = 0; o < 4; ++o) j.out[o] = cb.lde(b, l) + (size_t)o * n;
-        j.out[4] = cb.lag(b, l);
-        jobs.push_back(j);
-      }
-    cb.lde_ready = true;
+  for (size_t mm = 0; mm < NM; ++mm) {
+    const size_t mi = mods[mm];
+    std::vector<uint32_t> chunk_col(rho, PLAN_NONE);
+    for (size_t c = 0; c < d_.cols.size(); ++c)
+      if (d_.cols[c].src == SRC_AIR && d_.cols[c].module == mi && d_.cols[c].a0 < (uint32_t)rho) chunk_col[d_.cols[c].a0] = c;
+    for (int t = 0; t < rho; ++t) {
+      if (chunk_col[t] == PLAN_NONE) throw std::runtime_error("quotient: missing chunk column");
+      ColBuf& cb = cols_[chunk_col[t]];
+      if (cb.limbs != 6 || !cb.committed) throw std::runtime_error("quotient: chunk column must be a committed E6 column");
+      for (int b = 0; b < B_; ++b)
+        for (int l = 0; l < 6; ++l) {
+          NttJob j{};
+          j.in = coef.as<uint32_t>() + (((mm * B_ + b) * rho + t) * 6 + l) * n;
+          for (int o = 0; o < 4; ++o) j.out[o] = cb.lde(b, l) + (size_t)o * n;
+          j.out[4] = cb.lag(b, l);
+          jobs.push_back(j);
+        }
+      cb.lde_ready = true;
+    }
   }
   NttCfg c3{};
   c3.kind = NTT_KIND_CHUNKS;
@@ -2077,8 +2127,26 @@ void Job::run() {
   flush_err_flags();  // the generated columns' flag when there are no rounds
   phase("execute-steps", t0);
   // ComputeAIRQuotients (prover.go:461-561)
-  for (size_t mi = 0; mi < d_.modules.size(); ++mi)
-    if (d_.modules[mi].n_chunks) quotient(mi);
+  {
+    std::vector<std::vector<size_t>> classes;
+    for (size_t mi = 0; mi < d_.modules.size(); ++mi) {
+      if (!d_.modules[mi].n_chunks) continue;
+      const DagKernel& K = *plan_.relation_kernel(mi);
+      bool placed = false;
+      for (auto& c : classes) {
+        const size_t m0 = c[0];
+        const DagKernel& K0 = *plan_.relation_kernel(m0);
+        if (d_.modules[m0].N == d_.modules[mi].N && d_.modules[m0].n_chunks == d_.modules[mi].n_chunks &&
+            K0.code == K.code && K0.col_ids.size() == K.col_ids.size() && K0.chal_ids.size() == K.chal_ids.size()) {
+          c.push_back(mi);
+          placed = true;
+          break;
+        }
+      }
+      if (!placed) classes.push_back({mi});
+    }
+    for (auto& c : classes) quotient(c);
+  }
   trace("quotient");
   for (size_t t = 0; t < d_.trees.size(); ++t) {
     if (d_.trees[t].kind != 1) continue;
@@ -2109,28 +2177,30 @@ void Job::open() {
   const int nq = (int)d_.num_queries;
   // ---- claimed values; zeta is the device digest of __zeta
   const uint32_t* zdig = fs_dig_.as<uint32_t>() + d_.rounds.size() * (size_t)B_ * 8;
-  std::vector<DevBuf> W(nm);
-  std::vector<uint32_t> wmods, wsizes, wninv;
+  // Lagrange weights L_i(zeta) depend on zeta and the size only: one vector
+  // per distinct module size, shared by the modules of that size.
+  std::map<uint32_t, DevBuf> W;  // by module size: [b][6][N]
+  std::vector<uint32_t> wsizes, wninv;
   for (size_t mi = 0; mi < nm; ++mi) {
     bool used = false;
     for (auto& t : d_.trees)
       for (auto& g : t.groups) used |= g.module == mi;
-    if (!used) continue;
-    wmods.push_back((uint32_t)mi);
-    wsizes.push_back(d_.modules[mi].N);
-    wninv.push_back(kb_inv(M(d_.modules[mi].N)));
+    const uint32_t n = d_.modules[mi].N;
+    if (!used || std::find(wsizes.begin(), wsizes.end(), n) != wsizes.end()) continue;
+    wsizes.push_back(n);
+    wninv.push_back(kb_inv(M(n)));
   }
-  DevBuf zb = dev((size_t)B_ * 6), cmb = dev(wmods.size() * (size_t)B_ * 6 + 6);
-  launch_zeta_consts(zdig, stage(wsizes), stage(wninv), (int)wmods.size(), zb.as<uint32_t>(), cmb.as<uint32_t>(), B_,
+  DevBuf zb = dev((size_t)B_ * 6), cmb = dev(wsizes.size() * (size_t)B_ * 6 + 6);
+  launch_zeta_consts(zdig, stage(wsizes), stage(wninv), (int)wsizes.size(), zb.as<uint32_t>(), cmb.as<uint32_t>(), B_,
                      s_);
-  for (size_t j = 0; j < wmods.size(); ++j) {
-    const ModuleInfo& m = d_.modules[wmods[j]];
-    DevBuf& w = W[wmods[j]];
-    w = dev((size_t)B_ * 6 * m.N);
+  for (size_t j = 0; j < wsizes.size(); ++j) {
+    const uint32_t n = wsizes[j];
+    DevBuf& w = W[n];
+    w = dev((size_t)B_ * 6 * n);
     std::vector<uint32_t*> wp(B_);
-    for (int b = 0; b < B_; ++b) wp[b] = w.as<uint32_t>() + (size_t)b * 6 * m.N;
-    launch_lagrange_weights(zb.as<uint32_t>(), cmb.as<uint32_t>() + j * (size_t)B_ * 6, tb_.wpow(m.log_n),
-                            const_cast<uint32_t* const*>(stage(wp)), m.N, B_, s_);
+    for (int b = 0; b < B_; ++b) wp[b] = w.as<uint32_t>() + (size_t)b * 6 * n;
+    launch_lagrange_weights(zb.as<uint32_t>(), cmb.as<uint32_t>() + j * (size_t)B_ * 6, tb_.wpow(ilog2(n)),
+                            const_cast<uint32_t* const*>(stage(wp)), n, B_, s_);
   }
   keep_.push_back(std::move(zb));
   keep_.push_back(std::move(cmb));
@@ -2159,17 +2229,18 @@ void Job::open() {
     std::vector<ClaimItem> items;
     double bytes = 0;  // algorithmic: W once per set, each distinct polynomial once
   };
+  // one set per (instance, module size): the claims of every module of that
+  // size share its weights
   std::map<uint32_t, SizeClaims> cl;
   for (int b = 0; b < B_; ++b)
-    for (size_t mi = 0; mi < nm; ++mi) {
-      if (!W[mi].bytes()) continue;
-      const uint32_t n = d_.modules[mi].N;
+    for (auto& wn : W) {
+      const uint32_t n = wn.first;
       SizeClaims& sc = cl[n];
-      ClaimSet set{W[mi].as<uint32_t>() + (size_t)b * 6 * n, (int32_t)sc.items.size(), 0};
+      ClaimSet set{wn.second.as<uint32_t>() + (size_t)b * 6 * n, (int32_t)sc.items.size(), 0};
       std::set<uint32_t> polys;
       for (size_t k = 0; k < ncv; ++k) {
         const CV& v = cvs[k];
-        if (d_.cols[v.col].module != mi) continue;
+        if (d_.modules[d_.cols[v.col].module].N != n) continue;
         ClaimItem it;
         it.poly = cols_[v.col].lag(b, 0);
         it.out = cvd.as<uint32_t>() + ((size_t)b * ncv + k) * 6;
@@ -2226,8 +2297,18 @@ void Job::open() {
   }
   trace("claimed values + alpha");
   uint32_t* alpha_h = nullptr;  // pinned: [b][8] digests, then the claimed values
-  // ---- DEEP quotient per class, RS-encoded (deep.go:71-266)
+  // ---- DEEP quotient per class, RS-encoded (deep.go:71-266).  With several
+  // classes, classes of one size and bundle structure run as one launch of
+  // (class, instance) pairs (below the loop).
   std::vector<Layer> level(L);
+  struct DeepPending {
+    size_t ci;
+    std::vector<DeepBundle> bundles;
+    std::vector<int32_t> ptl;
+    std::vector<uint32_t> pt_shift, bcols;
+    std::vector<E6> scale, consts;  // [b][bundle], [b][point]
+  };
+  std::vector<DeepPending> pending;
   for (size_t ci = 0; ci < L; ++ci) {
     const size_t mi = classes[ci];
     const ModuleInfo& m = d_.modules[mi];
@@ -2342,8 +2423,9 @@ void Job::open() {
           a = e6_mul(a, alpha[b]);
         }
       }
-      scale_d = reinterpret_cast<const uint32_t*>(stg().put(scale.data(), scale.size() * sizeof(E6), s_));
-      consts_d = reinterpret_cast<const uint32_t*>(stg().put(consts.data(), consts.size() * sizeof(E6), s_));
+      pending.push_back(DeepPending{ci, bundles, ptl, pt_shift, bcols, std::move(scale), std::move(consts)});
+      keep_.push_back(std::move(pts));
+      continue;
     }
     Layer dq;
     dq.len = m.N;
@@ -2392,6 +2474,97 @@ void Job::open() {
     }
     keep_.push_back(std::move(dq.buf));
     keep_.push_back(std::move(pts));
+  }
+  // the deferred classes, grouped by size and structure
+  {
+    auto same = [&](const DeepPending& a, const DeepPending& b) {
+      if (d_.modules[classes[a.ci]].N != d_.modules[classes[b.ci]].N || a.pt_shift != b.pt_shift ||
+          a.ptl != b.ptl || a.bundles.size() != b.bundles.size() || a.bcols.size() != b.bcols.size())
+        return false;
+      for (size_t i = 0; i < a.bundles.size(); ++i)
+        if (a.bundles[i].ext != b.bundles[i].ext || a.bundles[i].col != b.bundles[i].col ||
+            a.bundles[i].first_pt != b.bundles[i].first_pt || a.bundles[i].n_pts != b.bundles[i].n_pts)
+          return false;
+      return true;
+    };
+    std::vector<char> done(pending.size(), 0);
+    for (size_t g0 = 0; g0 < pending.size(); ++g0) {
+      if (done[g0]) continue;
+      std::vector<size_t> grp;
+      for (size_t k = g0; k < pending.size(); ++k)
+        if (!done[k] && same(pending[g0], pending[k])) {
+          grp.push_back(k);
+          done[k] = 1;
+        }
+      const DeepPending& p0 = pending[grp[0]];
+      const ModuleInfo& m = d_.modules[classes[p0.ci]];
+      const int np = (int)p0.pt_shift.size();
+      const int MB = (int)grp.size() * B_;  // mb = member * B + b
+      std::vector<uint32_t> wsh(np);
+      for (int p = 0; p < np; ++p) wsh[p] = kb_pow(omega(m.log_n), p0.pt_shift[p]);
+      DevBuf pts = dev((size_t)B_ * np * 6 + 6);  // shared by the members: indexed by instance
+      launch_zeta_points(zdig, stage(wsh), np, pts.as<uint32_t>(), B_, s_);
+      std::vector<const uint32_t*> cp;
+      std::vector<E6> scale, consts;
+      for (size_t k : grp) {
+        const DeepPending& pk = pending[k];
+        for (int b = 0; b < B_; ++b)
+          for (auto c : pk.bcols) cp.push_back(cols_[c].lag(b, 0));
+        scale.insert(scale.end(), pk.scale.begin(), pk.scale.end());
+        consts.insert(consts.end(), pk.consts.begin(), pk.consts.end());
+      }
+      Layer dq;
+      dq.len = m.N;
+      dq.buf = dev((size_t)MB * 6 * m.N);
+      std::vector<uint32_t*> op(MB);
+      for (int mb = 0; mb < MB; ++mb) op[mb] = dq.ptr(mb);
+      DeepDesc dd{};
+      dd.n = m.N;
+      dd.n_bundles = (int)p0.bundles.size();
+      dd.n_points = np;
+      dd.bundles = stage(p0.bundles);
+      dd.pt_list = stage(p0.ptl);
+      dd.cols = stage(cp);
+      dd.n_cols = (int)p0.bcols.size();
+      dd.scale = reinterpret_cast<const uint32_t*>(stg().put(scale.data(), scale.size() * sizeof(E6), s_));
+      dd.consts = reinterpret_cast<const uint32_t*>(stg().put(consts.data(), consts.size() * sizeof(E6), s_));
+      dd.points = pts.as<uint32_t>();
+      dd.points_mod = B_;
+      dd.wpow = tb_.wpow(m.log_n);
+      dd.out = const_cast<uint32_t* const*>(stage(op));
+      {
+        TimeIt tm(ctx_, s_, "deep", (double)MB * m.N * (p0.bcols.size() + 6) * 4, 0);
+        launch_deep(dd, MB, s_);
+      }
+      std::vector<NttJob> jobs;
+      for (size_t mm = 0; mm < grp.size(); ++mm) {
+        Layer& lv = level[pending[grp[mm]].ci];
+        lv.len = 4 * m.N;
+        lv.buf = dev((size_t)B_ * 6 * 4 * m.N);
+        for (int b = 0; b < B_; ++b)
+          for (int l = 0; l < 6; ++l) {
+            NttJob j{};
+            j.in = dq.ptr((int)mm * B_ + b) + (size_t)l * m.N;
+            for (int o = 0; o < 4; ++o) j.out[o] = lv.ptr(b) + (size_t)l * 4 * m.N + (size_t)o * m.N;
+            jobs.push_back(j);
+          }
+      }
+      NttCfg cfg{};
+      cfg.kind = NTT_KIND_DEEP_LDE;
+      cfg.logn = m.log_n;
+      cfg.inverse_first = 1;
+      cfg.n_out = 4;
+      for (int o = 0; o < 4; ++o) {
+        cfg.pre[o] = tb_.lde_pre(m.log_n, o, RATE_LOG, true);
+        cfg.bitrev_out[o] = 1;
+      }
+      {
+        TimeIt tm(ctx_, s_, "ntt_deep_lde", jobs.size() * 5.0 * m.N * 4, 0);
+        ntt(cfg, jobs);
+      }
+      keep_.push_back(std::move(dq.buf));
+      keep_.push_back(std::move(pts));
+    }
   }
   trace("alpha + deep");
   // ---- FRI (fri.go:606-736)
@@ -3041,7 +3214,7 @@ void Job::open() {
             wait / B_ * 1e3, assemble / B_ * 1e3);
   }
   trace("queries, gathers, proof assembly");
-  for (auto& x : W) keep_.push_back(std::move(x));
+  for (auto& x : W) keep_.push_back(std::move(x.second));
   for (auto& x : level) keep_.push_back(std::move(x.buf));
   for (auto& x : layers) keep_.push_back(std::move(x.buf));
   keep_.push_back(std::move(running.buf));
