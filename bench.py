@@ -139,6 +139,7 @@ def main():
     host_same = all(np.array_equal(a, b) for a, b in zip(host_words, batch_words))
 
     identical = all(np.array_equal(a, b) for a, b in zip(plain_words, batch_words))
+    graphs = prover.graph_stats(ctx, items)  # recordings, replays, eager fallbacks of the plan
 
     # kernel timing pass (untimed): HIP events around every launch of one batched step
     ctx.set_timing(True)
@@ -220,6 +221,7 @@ def main():
         "bit_exact_batched_vs_plain": identical and host_same,
         "batched_host_inputs_proofs_per_s": round(n_proofs / host_s, 3),
         "bit_exact_vs_oracle": oracle_ok,
+        "graphs": graphs,
         "poseidon2_perms_per_s": round(p2_rate, 1),
         "dominant_kernel": dom,
         "kernels": kernels,

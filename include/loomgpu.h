@@ -175,6 +175,11 @@ int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning);
  * shape replay the graph.  Proofs are the same either way.  At most 4 shapes
  * per plan are kept (least recently used dropped). */
 int lg_ctx_set_graphs(lg_ctx* ctx, int on);
+/* Graph use of a plan so far (4 counts): device bytes held by its shape
+ * buckets, proves that recorded a graph, proves that replayed one, and proves
+ * that ran again eagerly because a recording or a replay failed (a replay
+ * fails when a table it would stage differs from the recorded one). */
+int lg_plan_graph_stats(const lg_plan* plan, uint64_t* out4);
 
 /* Per-kernel device timing with HIP events on the context's stream (off by
  * default).  lg_ctx_timing_report writes one text line per kernel name,

@@ -287,6 +287,17 @@ int lg_ctx_set_graphs(lg_ctx* ctx, int on) {
   });
 }
 
+int lg_plan_graph_stats(const lg_plan* plan, uint64_t* out4) {
+  return guarded([&] {
+    if (!plan || !out4) throw std::runtime_error("lg_plan_graph_stats: null argument");
+    out4[0] = plan->p->graph_bytes();
+    out4[1] = plan->p->graph_records.load();
+    out4[2] = plan->p->graph_replays.load();
+    out4[3] = plan->p->graph_fallbacks.load();
+    return 0;
+  });
+}
+
 int lg_ctx_set_timing(lg_ctx* ctx, int on) {
   return guarded([&] {
     if (!ctx) throw std::runtime_error("lg_ctx_set_timing: null ctx");

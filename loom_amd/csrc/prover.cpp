@@ -272,6 +272,7 @@ Staging::~Staging() {
 void* Staging::put(const void* src, size_t n, hipStream_t) {
   size_t off = (off_ + 255) & ~size_t(255);
   if (off + n > cap_) throw std::runtime_error("staging area exhausted");
+  if (n && verify_ && std::memcmp(host_ + off, src, n) != 0) changed_ = true;
   if (n) std::memcpy(host_ + off, src, n);
   off_ = off + n;
   return dev_.as<uint8_t>() + off;
@@ -879,10 +880,17 @@ class Job {
     if (mode_ == REPLAY) {
       LG_HIP(hipGraphLaunch(gb_->exec, s_));
       set_launch_suppression(true);
+      stg().set_verify(true);
     }
   }
   void region_end() {
     if (!in_region_) return;
+    if (mode_ == REPLAY) {
+      stg().set_verify(false);
+      // a staged table that differs from the recorded one means the graph ran
+      // on stale addresses or values: fail (the caller proves again eagerly)
+      if (stg().changed()) throw std::runtime_error("graph bucket: staged tables changed on replay");
+    }
     if (mode_ == CAPTURE) {
       stg().flush_sync(ctx_.side_stream());
       hipGraph_t g = nullptr;
@@ -1157,6 +1165,29 @@ void Job::upload_inputs() {
     perm_ptr_.assign(d_.perms.size(), std::vector<const uint32_t*>(B_));
     for (size_t p = 0; p < d_.perms.size(); ++p)
       for (int b = 0; b < B_; ++b) perm_ptr_[p][b] = in_[b].perms[p];
+    if (gb_ && !d_.perms.empty()) {
+      // A recorded graph reads the addresses it was recorded with, and the
+      // permutations are read inside it (k_gencol): copy them into the
+      // bucket's own buffer, whose address every prove of the bucket shares.
+      size_t words = 0;
+      for (auto& pm : d_.perms) words += (size_t)pm.len * B_;
+      in_arena_ = dev(words);
+      std::map<uint32_t, std::pair<std::vector<const uint32_t*>, std::vector<uint32_t*>>> by_len;
+      size_t off = 0;
+      for (size_t p = 0; p < d_.perms.size(); ++p)
+        for (int b = 0; b < B_; ++b) {
+          uint32_t* dst = in_arena_.as<uint32_t>() + off;
+          off += d_.perms[p].len;
+          auto& e = by_len[d_.perms[p].len];
+          e.first.push_back(in_[b].perms[p]);
+          e.second.push_back(dst);
+          perm_ptr_[p][b] = dst;
+        }
+      for (auto& kv : by_len)
+        if (kv.first)
+          launch_copy_cols(stage(kv.second.first), const_cast<uint32_t* const*>(stage(kv.second.second)),
+                           (int)kv.first, (int)kv.second.first.size(), s_);
+    }
     std::vector<std::vector<const uint32_t*>> pv(npub, std::vector<const uint32_t*>(B_));
     for (size_t i = 0; i < npub; ++i)
       for (int b = 0; b < B_; ++b) pv[i][b] = in_[b].columns[nin + i];
@@ -3269,20 +3300,20 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   // broken and the same prove runs again eagerly.
   auto run_job = [&](const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& pcb,
                      int lane, GraphBucket* gb, int mode) {
-    if (mode == Job::CAPTURE) {
+    if (mode == Job::CAPTURE || mode == Job::REPLAY) {
+      // a failed recording or replay breaks the bucket; the prove runs again eagerly
       try {
-        Job job(*this, in, out, pcb, lane, gb, Job::CAPTURE);
+        Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode);
         job.run();
+        ++(mode == Job::CAPTURE ? graph_records : graph_replays);
         return;
       } catch (const std::exception&) {
         gb->broken = true;
         out.assign(in.size(), ProveOutput());
+        ++graph_fallbacks;
       }
-      Job job(*this, in, out, pcb, lane);
-      job.run();
-      return;
     }
-    Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode);
+    Job job(*this, in, out, pcb, lane);
     job.run();
   };
   if (lanes <= 1) {

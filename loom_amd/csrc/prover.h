@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <functional>
 #include <map>
@@ -87,6 +88,14 @@ class Staging {
   void assume_flushed();
   // flush() on `side` (a stream no graph records), waited for.
   void flush_sync(hipStream_t side);
+  // While verifying, put() notes whether any appended bytes differ from what
+  // the area already held at that offset (a replayed graph reads the bytes
+  // recorded with it, so they must not change).
+  void set_verify(bool on) {
+    verify_ = on;
+    changed_ = false;
+  }
+  bool changed() const { return changed_; }
   template <class T>
   const T* put(const std::vector<T>& v, hipStream_t s) {
     return static_cast<const T*>(put(v.data(), v.size() * sizeof(T), s));
@@ -96,6 +105,7 @@ class Staging {
   uint8_t* host_ = nullptr;
   DevBuf dev_;
   size_t cap_ = 0, off_ = 0, flushed_ = 0;
+  bool verify_ = false, changed_ = false;
 };
 
 // Optional per-kernel timing with HIP events on the launching stream: every
@@ -292,6 +302,9 @@ class Plan {
   size_t graph_bytes();
   // Drops every graph bucket (frees its buffers and graph).
   void drop_graphs();
+  // Proves of this plan that recorded a graph, replayed one, or ran again
+  // eagerly after a failed recording or replay.
+  std::atomic<uint64_t> graph_records{0}, graph_replays{0}, graph_fallbacks{0};
 };
 
 }  // namespace lg

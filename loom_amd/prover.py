@@ -289,6 +289,15 @@ def autotune(ctx, items, dev_inputs=None, reps=3):
     return {"tuning": dict(zip(TUNING_FIELDS, list(t)[:5])), "trials": t[5], "start_ms": ms[0], "best_ms": ms[1]}
 
 
+def graph_stats(ctx, items):
+    """{"bytes", "records", "replays", "fallbacks"} of the plan of these
+    prepared instances (lg_plan_graph_stats)."""
+    ph = ctx.plan_handle(items[0].words, items[0].setup)
+    v = (ctypes.c_uint64 * 4)()
+    _lib.check(_lib.lib().lg_plan_graph_stats(ph, v), "lg_plan_graph_stats")
+    return dict(zip(("bytes", "records", "replays", "fallbacks"), list(v)))
+
+
 def set_tuning(ctx, items, tuning):
     ph = ctx.plan_handle(items[0].words, items[0].setup)
     v = (ctypes.c_int32 * 5)(*[int(tuning[f]) for f in TUNING_FIELDS])

@@ -373,8 +373,10 @@ def test_graph_replay_equals_eager():
     """HIP graphs per shape bucket: the same shape proved repeatedly with new
     witnesses (eager, then recorded, then replayed) gives the proofs and
     transcripts of a context without graphs -- one lane (batch 3), two lanes
-    (batch 16), and a plan with range lookups whose replayed batch holds an
-    invalid instance that fails alone."""
+    (batch 16), one instance per call from device inputs held at different
+    addresses at once, and a plan with range lookups whose replayed batch
+    holds an invalid instance that fails alone.  No replay may fall back to
+    an eager prove."""
     import torch
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
@@ -383,6 +385,7 @@ def test_graph_replay_equals_eager():
     e = prover.Context(0)
     e.set_graphs(False)
     try:
+        held = []  # device inputs stay alive, so every call's addresses differ
         for B, logn in ((3, 9), (16, 8)):
             for rep in range(4):
                 log("graphs: batch %d, 2^%d rows, call %d" % (B, logn, rep))
@@ -394,9 +397,26 @@ def test_graph_replay_equals_eager():
                     assert np.array_equal(a[k], b[k]), (B, rep, k)
                     assert np.array_equal(ta[k], tb[k]), (B, rep, k)
                 # the same shape from device inputs is a bucket of its own
-                d = prover.prove_prepared(g, items, dev_inputs=prover.DeviceInputs(items))
+                held.append(prover.DeviceInputs(items))
+                d = prover.prove_prepared(g, items, dev_inputs=held[-1])
                 for k in range(B):
                     assert np.array_equal(d[k], b[k]), (B, rep, k)
+            st = prover.graph_stats(g, items)
+            log("graphs: batch %d stats %s" % (B, st))
+            # per lane and input kind: call 0 eager, call 1 records, calls 2-3 replay
+            assert st["fallbacks"] == 0 and st["records"] >= 2 and st["replays"] == 2 * st["records"], st
+        # one instance per call, each from its own device inputs (the plain path)
+        items = [prover.Prepared(*workloads.plonk_instance(8, 500 + k), 24) for k in range(5)]
+        dev = [prover.DeviceInputs([it]) for it in items]
+        want = [prover.prove_prepared(e, [it])[0] for it in items]
+        for rep in range(3):
+            log("graphs: one instance per call from device inputs, pass %d" % rep)
+            for k in range(5):
+                got = prover.prove_prepared(g, [items[k]], dev_inputs=dev[k])[0]
+                assert np.array_equal(got, want[k]), (rep, k)
+        st = prover.graph_stats(g, items)
+        log("graphs: single-instance stats %s" % st)
+        assert st["fallbacks"] == 0 and st["replays"] >= 12, st
         for rep in range(4):
             log("graphs: range lookups, call %d" % rep)
             items = [prover.Prepared(*workloads.uneven_range_instance([8], 70 + 5 * rep + k), 16) for k in range(4)]
