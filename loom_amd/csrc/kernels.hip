@@ -32,6 +32,29 @@ bool prelaunch() {
 
 __device__ __forceinline__ uint32_t brev(uint32_t x, int bits) { return __brev(x) >> (32 - bits); }
 
+// A pointer read from a job table is a generic pointer to the compiler, and
+// accesses through it become flat_* instructions: they count against the LDS
+// counter too, so every LDS wait also waits for them.  Every buffer here is
+// device global memory; accessing it through this view gives global_*
+// instructions (with the SGPR-base form where the pointer is uniform).
+template <class T>
+using gmem = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gmem<T> gbl(T* p) {
+  return (gmem<T>)p;
+}
+// Element i of a global buffer addressed by a 32-bit byte offset (i * sizeof
+// T < 2^32): with a uniform base this is the SGPR-base form, one VGPR per
+// address instead of a 64-bit pair.
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T& gat(gmem<T> base, uint32_t i) {
+  using C = __attribute__((address_space(1))) char*;
+  return *(gmem<T>)((C)base + i * (uint32_t)sizeof(T));
+}
+// 16-byte stores through gat (a builtin vector type: HIP's uint4 has no
+// assignment for an object in the global address space)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 // ------------------------------------------------------------------ NTT
 // Radix-2 decimation-in-time on LDS data already in bit-reversed order;
 // leaves natural order.  tw[k] = w_n^k for k < n/2.
@@ -180,11 +203,11 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
   uint32_t x[16];
   if (cfg.inverse_first) {
     const NttRoots Ri = ntt16_roots(tw_inv, L);
-    const uint32_t* __restrict__ in = jp->in;
+    const gmem<const uint32_t> __restrict__ in = gbl(jp->in);
 #pragma unroll
     for (int k = 0; k < 16; ++k) {
       const uint32_t i = t + (uint32_t)k * q0;
-      sh[lds_swz(brev(i, L))] = in[i];
+      sh[lds_swz(brev(i, L))] = gat(in, i);
     }
     __syncthreads();
     if (r) {
@@ -208,30 +231,30 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
         __syncthreads();
       }
     }
-    const uint32_t* __restrict__ post = jp->post;
+    const gmem<const uint32_t> __restrict__ post = gbl(jp->post);
     if (post) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = kb_mul(x[k], post[t + (uint32_t)k * q0]);
+      for (int k = 0; k < 16; ++k) x[k] = kb_mul(x[k], gat(post, t + (uint32_t)k * q0));
     }
     if (cfg.n_out == 0) {
-      uint32_t* __restrict__ out = jp->out[0];
+      const gmem<uint32_t> __restrict__ out = gbl(jp->out[0]);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) out[t + (uint32_t)k * q0] = x[k];
+      for (int k = 0; k < 16; ++k) gat(out, t + (uint32_t)k * q0) = x[k];
       return;
     }
     __syncthreads();  // the last inverse pass read LDS; the forward passes write it
   } else {
-    const uint32_t* __restrict__ in = jp->in;
+    const gmem<const uint32_t> __restrict__ in = gbl(jp->in);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = in[t + (uint32_t)k * q0];
+    for (int k = 0; k < 16; ++k) x[k] = gat(in, t + (uint32_t)k * q0);
   }
   const NttRoots Rf = ntt16_roots(tw_fwd, L);
   for (int o = 0; o < cfg.n_out; ++o) {
     uint32_t y[16];
-    const uint32_t* __restrict__ pre = pres[o];
+    const gmem<const uint32_t> __restrict__ pre = gbl(pres[o]);
     if (pre) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) y[k] = kb_mul(x[k], pre[t + (uint32_t)k * q0]);
+      for (int k = 0; k < 16; ++k) y[k] = kb_mul(x[k], gat(pre, t + (uint32_t)k * q0));
     } else {
 #pragma unroll
       for (int k = 0; k < 16; ++k) y[k] = x[k];
@@ -257,14 +280,15 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
       dif_small_r(y, r, Rf);
     }
     // y[k] is now the bit-reversed-order value at position 16 t + k
-    uint32_t* __restrict__ out = jp->out[o];
-    if (((cfg.bitrev_mask >> o) & 1) && !(reinterpret_cast<uintptr_t>(out) & 15)) {
-      uint4* o4 = reinterpret_cast<uint4*>(out + 16 * t);
+    const gmem<uint32_t> __restrict__ out = gbl(jp->out[o]);
+    if (((cfg.bitrev_mask >> o) & 1) && !(reinterpret_cast<uintptr_t>(jp->out[o]) & 15)) {
+      const gmem<u32x4> o4 = (gmem<u32x4>)out;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) o4[k] = make_uint4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
+      for (int k = 0; k < 4; ++k)
+        gat(o4, 4 * t + (uint32_t)k) = u32x4{y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]};
     } else if ((cfg.bitrev_mask >> o) & 1) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) out[16 * t + k] = y[k];
+      for (int k = 0; k < 16; ++k) gat(out, 16 * t + (uint32_t)k) = y[k];
     } else {
       __syncthreads();  // other threads may still read LDS of this output's last pass
 #pragma unroll
@@ -273,7 +297,7 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
 #pragma unroll
       for (int k = 0; k < 16; ++k) {
         const uint32_t i = t + (uint32_t)k * q0;
-        out[i] = sh[lds_swz(brev(i, L))];
+        gat(out, i) = sh[lds_swz(brev(i, L))];
       }
     }
     __syncthreads();  // before the next output writes LDS
@@ -391,23 +415,23 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_a(const BigJob* __restrict__ 
   const uint32_t c = threadIdx.x & (BIG_C - 1), tt = threadIdx.x >> 3;
   const uint32_t j1 = xcd_group(blockIdx.x, gridDim.x) * BIG_C + c;
   const uint32_t qc = n2 >> 4;
-  const uint32_t* __restrict__ in = job->in;
-  const uint32_t* __restrict__ pre = job->pre;
+  const gmem<const uint32_t> __restrict__ in = gbl(job->in);
+  const gmem<const uint32_t> __restrict__ pre = gbl(job->pre);
   uint32_t y[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
-    const size_t gi = (size_t)j1 + (size_t)n1 * (tt + (uint32_t)k * qc);
-    const uint32_t v = in[gi];
-    y[k] = pre ? kb_mul(v, pre[gi]) : v;
+    const uint32_t gi = j1 + n1 * (tt + (uint32_t)k * qc);  // < n <= 2^22
+    const uint32_t v = gat(in, gi);
+    y[k] = pre ? kb_mul(v, gat(pre, gi)) : v;
   }
   const NttRoots R = ntt16_roots(tw_a, b);
   col_dif(y, sh + c, BIG_C, b, tt, tw_a, R);
-  uint32_t* __restrict__ tmp = job->tmp;
+  const gmem<uint32_t> __restrict__ tmp = gbl(job->tmp);
 #pragma unroll
   for (int k = 0; k < 16; ++k) {
     const uint32_t k2 = brev(16 * tt + k, b);
     const uint32_t e = j1 * k2;
-    tmp[(size_t)k2 * n1 + j1] = kb_mul(y[k], kb_mul(wlo[e & 2047u], whi[e >> 11]));
+    gat(tmp, k2 * n1 + j1) = kb_mul(y[k], kb_mul(wlo[e & 2047u], whi[e >> 11]));
   }
 }
 
@@ -423,35 +447,36 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ 
   const uint32_t tpc = n1 >> 4;  // threads per row
   const uint32_t tt = threadIdx.x % tpc, c = threadIdx.x / tpc;
   const uint32_t k2 = xcd_group(blockIdx.x, gridDim.x) * BIG_C + c;
-  const uint32_t* __restrict__ tmp = job->tmp + (size_t)k2 * n1;
+  const gmem<const uint32_t> __restrict__ tmp = gbl(job->tmp);
   uint32_t y[16];
 #pragma unroll
-  for (int k = 0; k < 16; ++k) y[k] = tmp[tt + (uint32_t)k * tpc];
+  for (int k = 0; k < 16; ++k) y[k] = gat(tmp, k2 * n1 + tt + (uint32_t)k * tpc);
   const NttRoots R = ntt16_roots(tw_b, a);
   col_dif(y, sh + c * n1, 1, a, tt, tw_b, R);
-  uint32_t* __restrict__ out = job->out;
-  const uint32_t* __restrict__ post = job->post;
+  const gmem<uint32_t> __restrict__ out = gbl(job->out);
+  const gmem<const uint32_t> __restrict__ post = gbl(job->post);
   const uint32_t k2r = brev(k2, b);
   if (post) {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) y[k] = kb_mul(y[k], post[k2 + (size_t)n2 * brev(16 * tt + k, a)]);
+    for (int k = 0; k < 16; ++k) y[k] = kb_mul(y[k], gat(post, k2 + n2 * brev(16 * tt + k, a)));
   }
   if (bitrev_out) {
     // the 16 values of a thread are contiguous there (positions 16 tt + k =
     // brev_a(k1)): four 16-byte stores per thread instead of sixteen strided
     // dword stores
-    uint32_t* o = out + 16 * tt + (size_t)n1 * k2r;
-    if (!(reinterpret_cast<uintptr_t>(o) & 15)) {
-      uint4* o4 = reinterpret_cast<uint4*>(o);
+    const uint32_t o = 16 * tt + n1 * k2r;
+    if (!(reinterpret_cast<uintptr_t>(job->out) & 15)) {
+      const gmem<u32x4> o4 = (gmem<u32x4>)out;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) o4[k] = make_uint4(y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]);
+      for (int k = 0; k < 4; ++k)
+        gat(o4, o / 4 + (uint32_t)k) = u32x4{y[4 * k], y[4 * k + 1], y[4 * k + 2], y[4 * k + 3]};
     } else {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) o[k] = y[k];
+      for (int k = 0; k < 16; ++k) gat(out, o + (uint32_t)k) = y[k];
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < 16; ++k) out[k2 + (size_t)n2 * brev(16 * tt + k, a)] = y[k];
+    for (int k = 0; k < 16; ++k) gat(out, k2 + n2 * brev(16 * tt + k, a)) = y[k];
   }
 }
 
