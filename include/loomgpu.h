@@ -278,6 +278,31 @@ int lg_probe_issue_rate(lg_ctx* ctx, double* out5);
  * column after the other) to 4n bit-reversed evaluations each. */
 int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t* out);
 
+/* verifier.Verify (verifier/verifier.go:479-533) for n proofs of this plan.
+ * The host decodes each proof, replays its transcript and checks the logup
+ * buses and every module's relation at zeta; the device hashes every opened
+ * row and walks every Merkle path of the batch (committed trees with their
+ * level injections, the FRI levels tree and layers) while the host checks the
+ * DEEP bridge and the FRI folds (internal/fri/verify.go:60-157,
+ * internal/fri/fri.go:751-1534).
+ *   proofs[i], lens[i]: proof words (the grammar above);
+ *   setup_roots: the verification key, 8 Montgomery words per setup tree (the
+ *     root lg_plan_set_setup returned), n_setup_roots of them (0 and NULL
+ *     without setup columns);
+ *   public_values: NULL without public inputs, else n * n_public pointers:
+ *     [i * n_public + j] -> the entry values of public input j of proof i
+ *     (Montgomery, as lg_prove takes them);
+ *   skip_fri: nonzero stops after the AIR check (verifier.SkipFRI, :52-57);
+ *   status[i]: 0 valid, else the failed check: 1 proof format or shape,
+ *     2 logup bus, 3 AIR identity at zeta, 4 opening of a committed tree,
+ *     5 DEEP bridge, 6 FRI (layer opening, fold or final polynomial).
+ * Returns 0 once every proof has its verdict, whatever the verdicts. */
+int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens, const uint32_t* setup_roots,
+              size_t n_setup_roots, const uint32_t* const* public_values, int skip_fri, int32_t* status);
+/* The message of proof i's failed check in this plan's last lg_verify ("" when
+ * it passed); NUL-terminated into buf, *len = bytes needed. */
+int lg_plan_verify_message(const lg_plan* plan, int i, char* buf, size_t cap, size_t* len);
+
 #ifdef __cplusplus
 }
 #endif

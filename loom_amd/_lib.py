@@ -30,6 +30,9 @@ EXPORTS = {
     "lg_plan_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "lg_ctx_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_plan_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),
+    "lg_verify": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, sizep, u32p, ctypes.c_size_t, u32pp, ctypes.c_int,
+                                 ctypes.POINTER(ctypes.c_int32)]),
+    "lg_plan_verify_message": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, sizep]),
     "lg_ctx_set_timing": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_ctx_timing_report": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t, sizep]),
     "lg_plan_last_phases": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)]),

@@ -12,7 +12,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libloomgpu.so")
-SOURCES = ["kernels.hip", "noalloc.hip", "prover.cpp", "dagjit.cpp", "server.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "noalloc.hip", "prover.cpp", "verifier.cpp", "dagjit.cpp", "server.cpp", "capi.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LOOMGPU_ARCH", "gfx950")
 

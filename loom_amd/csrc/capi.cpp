@@ -15,6 +15,7 @@
 #include "pool.h"
 #include "prover.h"
 #include "server.h"
+#include "verifier.h"
 
 struct lg_ctx {
   lg::Context c;
@@ -29,6 +30,7 @@ struct lg_plan {
   lg_ctx* ctx;
   std::unique_ptr<lg::Plan> p;
   double phases[3] = {0, 0, 0};
+  std::vector<std::string> verify_msgs;  // of the last lg_verify
 };
 
 namespace {
@@ -283,6 +285,46 @@ int lg_ctx_set_graphs(lg_ctx* ctx, int on) {
     if (!ctx) throw std::runtime_error("lg_ctx_set_graphs: null ctx");
     std::lock_guard<std::mutex> lock(ctx->mu);
     ctx->c.graphs = on != 0;
+    return 0;
+  });
+}
+
+int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens, const uint32_t* setup_roots,
+              size_t n_setup_roots, const uint32_t* const* public_values, int skip_fri, int32_t* status) {
+  return guarded([&] {
+    if (!plan || n < 0 || (n && (!proofs || !lens || !status)) || (n_setup_roots && !setup_roots))
+      throw std::runtime_error("lg_verify: bad arguments");
+    const size_t npub = plan->p->data().publics.size();
+    if (npub && n && !public_values) throw std::runtime_error("lg_verify: the plan has public inputs");
+    std::vector<lg::VerifyInput> in(n);
+    for (int i = 0; i < n; ++i) {
+      in[i].words = proofs[i];
+      in[i].len = lens[i];
+      for (size_t j = 0; j < npub; ++j) in[i].publics.push_back(public_values[(size_t)i * npub + j]);
+    }
+    std::vector<lg::VerifyResult> out;
+    std::lock_guard<std::mutex> lock(plan->ctx->mu);
+    lg::verify_batch(*plan->p, in, setup_roots, n_setup_roots, skip_fri != 0, out);
+    plan->verify_msgs.assign(n, std::string());
+    for (int i = 0; i < n; ++i) {
+      status[i] = out[i].status;
+      plan->verify_msgs[i] = out[i].message;
+    }
+    return 0;
+  });
+}
+
+int lg_plan_verify_message(const lg_plan* plan, int i, char* buf, size_t cap, size_t* len) {
+  return guarded([&] {
+    if (!plan || i < 0 || (size_t)i >= plan->verify_msgs.size() || !len)
+      throw std::runtime_error("lg_plan_verify_message: bad arguments");
+    const std::string& m = plan->verify_msgs[i];
+    *len = m.size() + 1;
+    if (buf && cap) {
+      const size_t k = std::min(cap - 1, m.size());
+      std::memcpy(buf, m.data(), k);
+      buf[k] = 0;
+    }
     return 0;
   });
 }

@@ -13,7 +13,7 @@ namespace lg {
 constexpr uint32_t PLAN_MAGIC = 0x31504C4Cu;
 constexpr uint32_t PLAN_NONE = 0xFFFFFFFFu;
 
-constexpr uint32_t PLAN_VERSION = 3;
+constexpr uint32_t PLAN_VERSION = 4;
 
 enum ColSrc : uint32_t {
   SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE,
@@ -97,6 +97,9 @@ struct PlanData {
   std::vector<PublicCol> publics;  // sorted by name
   std::vector<TreeInfo> trees;
   uint32_t n_setup = 0;            // setup columns (SRC_SETUP, a0 = 0 .. n_setup-1)
+  // logup buses: the exposed totals (indices into `exposed`) that must sum to
+  // zero (the verifier's checkLogupBus); the prover does not read them
+  std::vector<std::vector<uint32_t>> buses;
 };
 
 class WordReader {
@@ -389,6 +392,17 @@ inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
     }
   }
   for (auto& m : p.modules) m.relation = read_dag(r, nc);
+  uint32_t nbus = r.next();
+  if (nbus > 4096) throw std::runtime_error("plan: too many buses");
+  p.buses.resize(nbus);
+  for (auto& bus : p.buses) {
+    uint32_t k = r.next();
+    if (k == 0 || k > 4096) throw std::runtime_error("plan: bad bus size");
+    for (uint32_t j = 0; j < k; ++j) {
+      bus.push_back(r.next());
+      if (bus.back() >= nx) throw std::runtime_error("plan: bus member out of range");
+    }
+  }
   if (r.pos() != r.size()) throw std::runtime_error("plan: trailing words");
   validate_plan(p);
   return p;

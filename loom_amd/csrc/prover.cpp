@@ -19,35 +19,12 @@
 #include "kernels.h"
 #include "ntt16.h"
 #include "pool.h"
+#include "protocol.h"
 #include "transcript.h"
 
 namespace lg {
 
 namespace {
-
-constexpr int RATE_LOG = 2;
-constexpr int FRI_LOG_N = 24;  // RATE * 2^MAX_LOG_MODULE_SIZE (internal/constants/const.go:24-32)
-constexpr int FRI_LOG_D = 22;
-constexpr int FRI_ROUNDS = FRI_LOG_D;
-constexpr uint32_t TAG_HASH = 0x48415348u;
-constexpr uint32_t TAG_SIZE = 0x53495a45u;
-constexpr uint32_t TAG_EXTP = 0x45585450u;
-constexpr uint32_t MULT_GEN = 3;  // fft.GeneratorFullMultiplicativeGroup (canonical)
-
-uint32_t M(uint32_t x) { return kb_to_mont(x); }
-
-uint32_t omega_max() {
-  static const uint32_t w = kb_pow(M(MULT_GEN), (KB_P - 1) >> 24);
-  return w;
-}
-
-uint32_t omega(int logn) { return kb_pow(omega_max(), 1ull << (24 - logn)); }
-
-uint32_t brev_host(uint32_t x, int bits) {
-  uint32_t r = 0;
-  for (int i = 0; i < bits; ++i) r |= ((x >> i) & 1u) << (bits - 1 - i);
-  return r;
-}
 
 std::vector<uint32_t> powers(uint32_t g, size_t n, uint32_t scale = KB_R) {
   std::vector<uint32_t> v(n);
@@ -62,8 +39,6 @@ std::vector<uint32_t> powers(uint32_t g, size_t n, uint32_t scale = KB_R) {
 double now_s() {
   return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
-
-std::string challenge_name(size_t r) { return "challenge@loom_" + std::to_string(r); }
 
 }  // namespace
 
@@ -1258,7 +1233,6 @@ void Job::public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_de
   pi_words_.assign(B_, std::vector<uint32_t>());
   if (!npub) return;
   const size_t nin = d_.inputs.size();
-  constexpr uint32_t TAG_PUBI = 0x50554249u;
   for (size_t i = 0; i < npub; ++i) {
     const PublicCol& pc = d_.publics[i];
     const ColBuf& cb = cols_[pc.col];
@@ -1284,21 +1258,10 @@ void Job::public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_de
     }
   std::vector<uint32_t> all;
   for (int b = 0; b < B_; ++b) {
+    std::vector<const uint32_t*> vals(npub);
+    for (size_t i = 0; i < npub; ++i) vals[i] = hv[i * B_ + b].data();
     std::vector<uint32_t>& w = pi_words_[b];
-    w.push_back(M(TAG_PUBI));
-    w.push_back(M((uint32_t)npub));
-    for (size_t i = 0; i < npub; ++i) {
-      const PublicCol& pc = d_.publics[i];
-      auto a = string_to_elements(TAG_PUBI, pc.name), m = string_to_elements(TAG_PUBI, pc.module_name);
-      w.insert(w.end(), a.begin(), a.end());
-      w.insert(w.end(), m.begin(), m.end());
-      w.push_back(M((uint32_t)pc.rows.size()));
-      for (size_t e = 0; e < pc.rows.size(); ++e) {
-        w.push_back(M(pc.rows[e]));
-        w.push_back(0u);  // field.Base
-        w.push_back(hv[i * B_ + b][e]);
-      }
-    }
+    w = public_input_words(d_, vals);
     if (b && w.size() != pi_words_[0].size()) throw std::runtime_error("public inputs: instance shapes differ");
     all.insert(all.end(), w.begin(), w.end());
   }

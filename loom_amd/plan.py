@@ -32,6 +32,8 @@ Word stream (u32, little endian):
             nb, base cols..., ne, ext cols..., shift lists (base polys then
             ext polys; each: count, shifts...)
   per module: relation dag
+  n_buses, then per logup bus: n_members, exposed-value indices (into the
+            sorted exposed list; the totals that must sum to zero)
   dag := n_nodes, root, then 4 words per node: op, field, a, b
 """
 import numpy as np
@@ -41,7 +43,7 @@ from . import board as B
 from . import protocol as PR
 
 MAGIC = 0x31504C4C
-VERSION = 3
+VERSION = 4
 NONE = 0xFFFFFFFF
 
 SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE, SRC_SETUP, SRC_PUBLIC = range(10)
@@ -266,6 +268,10 @@ class Plan:
         for mn in self.modules:
             m = prog.modules[mn]
             self._dag_words(m.relation, m.N, w)
+        w.append(len(prog.logup_bus))
+        for bus in prog.logup_bus:
+            w.append(len(bus))
+            w += [exposed.index(nm) for nm in bus]
         return np.array(w, dtype=np.uint32)
 
 

@@ -289,6 +289,51 @@ def autotune(ctx, items, dev_inputs=None, reps=3):
     return {"tuning": dict(zip(TUNING_FIELDS, list(t)[:5])), "trials": t[5], "start_ms": ms[0], "best_ms": ms[1]}
 
 
+VERIFY_CHECKS = {0: "ok", 1: "format", 2: "logup bus", 3: "AIR identity", 4: "opening", 5: "DEEP", 6: "FRI"}
+
+
+def verify(ctx, words, proofs, setup_roots=(), publics=None, skip_fri=False):
+    """lg_verify: checks proofs of the plan `words` in one batch; returns
+    [(status, message)] per proof (status 0: valid; VERIFY_CHECKS names the
+    others).  setup_roots: the verification key, canonical 8-value roots;
+    publics: per proof, the public-input value arrays in plan order
+    (Montgomery)."""
+    ph = ctx.plan_handle(words)
+    arrs = [np.ascontiguousarray(p, dtype=np.uint32) for p in proofs]
+    n = len(arrs)
+    ptrs = (_lib.u32p * max(1, n))(*[a.ctypes.data_as(_lib.u32p) for a in arrs])
+    lens = (ctypes.c_size_t * max(1, n))(*[a.size for a in arrs])
+    key = np.ascontiguousarray(to_mont(np.asarray(setup_roots, dtype=np.uint64).reshape(-1)), dtype=np.uint32)
+    pubs = None
+    keep = []
+    if publics is not None:
+        flat = [np.ascontiguousarray(v, dtype=np.uint32) for pp in publics for v in pp]
+        keep = flat
+        pubs = (_lib.u32p * max(1, len(flat)))(*[v.ctypes.data_as(_lib.u32p) for v in flat])
+    st = (ctypes.c_int32 * max(1, n))()
+    _lib.check(_lib.lib().lg_verify(ph, n, ptrs, lens, key.ctypes.data_as(_lib.u32p) if key.size else None,
+                                    key.size // 8, pubs, 1 if skip_fri else 0, st), "lg_verify")
+    out = []
+    for i in range(n):
+        msg = ""
+        if st[i]:
+            ln = ctypes.c_size_t()
+            buf = ctypes.create_string_buffer(512)
+            _lib.check(_lib.lib().lg_plan_verify_message(ph, i, buf, 512, ctypes.byref(ln)), "lg_plan_verify_message")
+            msg = buf.value.decode()
+        out.append((int(st[i]), msg))
+    del keep
+    return out
+
+
+def verify_prepared(ctx, items, proofs, setup_roots=(), skip_fri=False):
+    """verify() for the proofs of these prepared instances (one plan): the
+    public-input values are taken from the instances."""
+    npub = len(items[0].plan.public)
+    publics = [it.columns[len(it.columns) - npub:] for it in items] if npub else None
+    return verify(ctx, items[0].words, proofs, setup_roots, publics, skip_fri)
+
+
 def graph_stats(ctx, items):
     """{"bytes", "records", "replays", "fallbacks"} of the plan of these
     prepared instances (lg_plan_graph_stats)."""

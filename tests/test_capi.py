@@ -124,6 +124,11 @@ def _plan_offsets(w):
                 i += 1 + k
     for _ in range(nm):
         i = dag(i)
+    nbus = w[i]; i += 1
+    off["buses"] = []
+    for _ in range(nbus):
+        off["buses"] += list(range(i + 1, i + 1 + w[i]))
+        i += 1 + w[i]
     assert i == len(w)
     return off
 
