@@ -177,6 +177,18 @@ def main():
                      "permutations_per_launch": lperms / lc, "avg_launch_us": round(lms / lc * 1e3, 3)}
     probe = {k: round(v, 2) for k, v in probe.items()}
 
+    # batched verification of the batch's proofs (lg_verify), untimed for the
+    # headline: proofs/s and every verdict
+    for _ in range(2):
+        vres = prover.verify_prepared(ctx, items, batch_words)
+    vt = []
+    for _ in range(5):
+        a = time.perf_counter()
+        vres = prover.verify_prepared(ctx, items, batch_words)
+        vt.append(time.perf_counter() - a)
+    verify = {"proofs_per_s": round(B / float(np.median(vt)), 1), "ms_per_batch": round(float(np.median(vt)) * 1e3, 3),
+              "all_valid": all(r == (0, "") for r in vres)}
+
     # CPU baseline: the oracle on a bounded sample (one proof of instance 0)
     cpu = None
     oracle_ok = None
@@ -192,6 +204,10 @@ def main():
                          % (logn, nq, dt)}
         if not args.no_oracle_check:
             oracle_ok = bool(np.array_equal(enc, batch_words[0]))
+        from oracle import verifier as OV
+        a = time.perf_counter()
+        OV.verify(inst[0][0], nq, batch_words[0])
+        verify["cpu_oracle_s_per_proof"] = round(time.perf_counter() - a, 3)
 
     n_proofs = B * args.steps
     value = n_proofs / batch_s
@@ -222,6 +238,7 @@ def main():
         "batched_host_inputs_proofs_per_s": round(n_proofs / host_s, 3),
         "bit_exact_vs_oracle": oracle_ok,
         "graphs": graphs,
+        "verify": verify,
         "poseidon2_perms_per_s": round(p2_rate, 1),
         "dominant_kernel": dom,
         "kernels": kernels,

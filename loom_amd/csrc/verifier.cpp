@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 
@@ -779,6 +782,8 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
   if (S.n_setup_trees && !setup_roots) throw std::runtime_error("verify: no verification key");
   Context& ctx = *plan.ctx();
   LG_HIP(hipSetDevice(ctx.device));
+  static const bool trace = getenv("LOOMGPU_TRACE") != nullptr;
+  auto now = [] { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); };
   // proofs in chunks whose words (plus the key) fit one device buffer
   constexpr size_t CHUNK_WORDS = (size_t)1 << 27;
   size_t b0 = 0;
@@ -798,6 +803,7 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
     }
     std::vector<ProofJobs> J(nb);
     std::vector<char> alive(nb, 0);
+    const double t0 = now();
     // 1. decode, replay, buses, AIR; the device jobs of the live proofs
     parallel_for((int)nb, [&](int i) {
       VerifyResult& res = out[b0 + i];
@@ -827,6 +833,7 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
         res.message = f.msg;
       }
     });
+    const double t1 = now();
     // 2. merge the device jobs
     std::vector<VMsg> msgs;
     std::vector<VInj> inj;
@@ -851,35 +858,41 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
     }
     std::vector<uint32_t> flags(paths.size(), 0);
     hipStream_t s = ctx.stream;
-    DevBuf wbuf, mbuf, ibuf, pbuf, dbuf, fbuf;
+    DevBuf abuf, dbuf, fbuf;
     if (!paths.empty()) {
-      wbuf = ctx.pool.get(words * 4);
-      if (n_setup_roots)
-        LG_HIP(hipMemcpyAsync(wbuf.as<uint32_t>(), setup_roots, 32 * n_setup_roots, hipMemcpyHostToDevice, s));
-      for (size_t i = 0; i < nb; ++i)
-        if (alive[i])
-          LG_HIP(hipMemcpyAsync(wbuf.as<uint32_t>() + base[i], in[b0 + i].words, in[b0 + i].len * 4,
-                                hipMemcpyHostToDevice, s));
-      mbuf = ctx.pool.get(std::max<size_t>(1, msgs.size()) * sizeof(VMsg));
-      ibuf = ctx.pool.get(std::max<size_t>(1, inj.size()) * sizeof(VInj));
-      pbuf = ctx.pool.get(paths.size() * sizeof(VPath));
+      // one pinned staging area and one copy: the key and the proof words,
+      // then the job tables
+      auto up = [](size_t x) { return (x + 255) & ~size_t(255); };
+      const size_t moff = up(words * 4), ioff = up(moff + msgs.size() * sizeof(VMsg)),
+                   poff = up(ioff + inj.size() * sizeof(VInj)), total = poff + paths.size() * sizeof(VPath);
+      uint8_t* h = reinterpret_cast<uint8_t*>(ctx.pinned_in(total, 0));
+      uint32_t* hw = reinterpret_cast<uint32_t*>(h);
+      if (n_setup_roots) std::memcpy(hw, setup_roots, 32 * n_setup_roots);
+      parallel_for((int)nb, [&](int i) {
+        if (alive[i]) std::memcpy(hw + base[i], in[b0 + i].words, in[b0 + i].len * 4);
+      });
+      std::memcpy(h + moff, msgs.data(), msgs.size() * sizeof(VMsg));
+      std::memcpy(h + ioff, inj.data(), inj.size() * sizeof(VInj));
+      std::memcpy(h + poff, paths.data(), paths.size() * sizeof(VPath));
+      abuf = ctx.pool.get(total);
       dbuf = ctx.pool.get(std::max<size_t>(1, msgs.size()) * 32);
       fbuf = ctx.pool.get(paths.size() * 4);
-      LG_HIP(hipMemcpyAsync(mbuf.as<void>(), msgs.data(), msgs.size() * sizeof(VMsg), hipMemcpyHostToDevice, s));
-      if (!inj.empty())
-        LG_HIP(hipMemcpyAsync(ibuf.as<void>(), inj.data(), inj.size() * sizeof(VInj), hipMemcpyHostToDevice, s));
-      LG_HIP(hipMemcpyAsync(pbuf.as<void>(), paths.data(), paths.size() * sizeof(VPath), hipMemcpyHostToDevice, s));
+      LG_HIP(hipMemcpyAsync(abuf.as<void>(), h, total, hipMemcpyHostToDevice, s));
+      uint8_t* dev = abuf.as<uint8_t>();
+      const uint32_t* wdev = reinterpret_cast<const uint32_t*>(dev);
       if (!msgs.empty()) {
         hipLaunchKernelGGL(k_verify_leaves, dim3((unsigned)((msgs.size() + 255) / 256)), dim3(256), 0, s,
-                           mbuf.as<const VMsg>(), (int)msgs.size(), wbuf.as<const uint32_t>(), dbuf.as<uint32_t>());
+                           reinterpret_cast<const VMsg*>(dev + moff), (int)msgs.size(), wdev, dbuf.as<uint32_t>());
         LG_HIP(hipGetLastError());
       }
       hipLaunchKernelGGL(k_verify_paths, dim3((unsigned)((paths.size() + 255) / 256)), dim3(256), 0, s,
-                         pbuf.as<const VPath>(), (int)paths.size(), ibuf.as<const VInj>(), wbuf.as<const uint32_t>(),
-                         dbuf.as<const uint32_t>(), fbuf.as<uint32_t>());
+                         reinterpret_cast<const VPath*>(dev + poff), (int)paths.size(),
+                         reinterpret_cast<const VInj*>(dev + ioff), wdev, dbuf.as<const uint32_t>(),
+                         fbuf.as<uint32_t>());
       LG_HIP(hipGetLastError());
       LG_HIP(hipMemcpyAsync(flags.data(), fbuf.as<void>(), flags.size() * 4, hipMemcpyDeviceToHost, s));
     }
+    const double t2 = now();
     // 3. FRI folds and the DEEP bridge on the host while the device hashes
     std::vector<Fail> fri_fail(nb, Fail{VERIFY_OK, ""}), deep_fail(nb, Fail{VERIFY_OK, ""});
     parallel_for((int)nb, [&](int i) {
@@ -895,8 +908,14 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
         deep_fail[i] = f;
       }
     });
+    const double t3 = now();
     LG_HIP(hipStreamSynchronize(s));
-    for (auto* b : {&wbuf, &mbuf, &ibuf, &pbuf, &dbuf, &fbuf})
+    if (trace)
+      fprintf(stderr,
+              "[loomgpu] verify %zu proofs: decode/replay/AIR %.3f ms, jobs+upload %.3f ms, host FRI/DEEP %.3f ms, "
+              "device wait %.3f ms (%zu leaves, %zu paths)\n",
+              nb, (t1 - t0) * 1e3, (t2 - t1) * 1e3, (t3 - t2) * 1e3, (now() - t3) * 1e3, msgs.size(), paths.size());
+    for (auto* b : {&abuf, &dbuf, &fbuf})
       if (b->bytes()) ctx.pool.put(std::move(*b));
     // 4. verdicts in the reference's order of checks (fri/verify.go:127-155):
     //    the FRI proof (layer openings and folds), the point samplings, the bridge
