@@ -28,6 +28,7 @@
  *   - WithNumFriQueries: the plan's num_queries.
  *   - WithHashBackend: Poseidon2 only (hash id "poseidon2").
  *   - WithPhaseCallback: lg_plan_last_phases after the call.
+ *   - SkipFRI: lg_plan_set_options(plan, LG_PROVE_SKIP_FRI).
  *   - setup.ProvingKey: lg_plan_set_setup commits the setup columns once per
  *     plan and keeps them resident (base-field setup columns).
  *   - public.Inputs: base-field entries at rows the plan lists; each
@@ -35,7 +36,7 @@
  *   - Refused (a plan or call that needs them is rejected with an error):
  *     extension-field setup columns or public-input entries,
  *     WithTranscript and WithNewTranscriptHasher (the
- *     transcript always starts fresh with the Poseidon2 sponge), SkipFRI,
+ *     transcript always starts fresh with the Poseidon2 sponge),
  *     WithFriOption (no proof-of-work grinding: the PoW list is empty), other
  *     hash backends, lookups other than width-1 selector-free range lookups.
  *
@@ -164,6 +165,15 @@ int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, co
  *               chosen configuration's median prove time. */
 int lg_plan_autotune(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
                      int device_inputs, int reps, int32_t* tuning_out, double* ms_out);
+/* Prove options for the proves of this plan that follow (flags):
+ *   LG_PROVE_SKIP_FRI  prover.SkipFRI (prover/prover.go:78-83, 626-653,
+ *     691-700): the proof carries the exposed values, commitments and claimed
+ *     values; the rest of the opening is empty (levels root 0, no levels, no
+ *     roots, an empty final polynomial "1, 0", no queries, no proofs of work,
+ *     no samplings).  lg_verify with skip_fri checks such proofs.
+ * Other flags are refused. */
+#define LG_PROVE_SKIP_FRI 1u
+int lg_plan_set_options(lg_plan* plan, uint32_t flags);
 /* Sets the launch choices for proves of `batch` instances (5 ints, as above). */
 int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning);
 

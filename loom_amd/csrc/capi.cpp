@@ -262,6 +262,16 @@ int lg_plan_autotune(lg_plan* plan, int batch, const uint32_t* const* columns, c
   });
 }
 
+int lg_plan_set_options(lg_plan* plan, uint32_t flags) {
+  return guarded([&] {
+    if (!plan) throw std::runtime_error("lg_plan_set_options: null plan");
+    if (flags & ~lg::PROVE_SKIP_FRI) throw std::runtime_error("lg_plan_set_options: unknown option");
+    std::lock_guard<std::mutex> lock(plan->ctx->mu);
+    plan->p->set_options(flags);
+    return 0;
+  });
+}
+
 int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning) {
   return guarded([&] {
     if (!plan || batch <= 0 || !tuning) throw std::runtime_error("lg_plan_set_tuning: bad arguments");

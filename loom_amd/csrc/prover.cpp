@@ -611,7 +611,7 @@ GraphBucket::~GraphBucket() {
 GraphBucket* Plan::take_bucket(int batch, int lane, bool device_inputs, int* mode) {
   *mode = 0;
   static const bool diag = getenv("LOOMGPU_TRACE") || getenv("LOOMGPU_DEBUG_DIR");
-  if (!graph_eligible_ || !ctx_->graphs || ctx_->timer.on || diag) return nullptr;
+  if (!graph_eligible_ || !ctx_->graphs || ctx_->timer.on || diag || (options_ & PROVE_SKIP_FRI)) return nullptr;
   std::lock_guard<std::mutex> lk(gmu_);
   auto key = std::make_tuple(batch, lane, device_inputs);
   auto it = buckets_.find(key);
@@ -755,7 +755,7 @@ class Job {
       int lane, GraphBucket* gb = nullptr, Mode mode = EAGER)
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
         B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER),
-        tune_(plan.tuning(plan.tuning_batch())) {
+        tune_(plan.tuning(plan.tuning_batch())), skip_fri_((plan.options() & PROVE_SKIP_FRI) != 0) {
     if (out.size() != in.size()) throw std::runtime_error("prove: output slots do not match the inputs");
     if (gb_) {
       if (!gb_->stg) gb_->stg = std::make_unique<Staging>(64u << 20);
@@ -802,6 +802,48 @@ class Job {
   }
 
   void run();
+  // The proof words up to the opening: hash id, exposed values, commitments
+  // and claimed values (cv: instance b's values in canonical order).
+  template <class W>
+  void emit_head(W& w, int b, const E6* cv) const {
+    const size_t nt = d_.trees.size();
+    auto put_string = [&](const std::string& x) {
+      w.push_back((uint32_t)x.size());
+      for (size_t i = 0; i < x.size(); i += 4) {
+        uint32_t v = 0;
+        for (size_t k = 0; k < 4 && i + k < x.size(); ++k) v |= (uint32_t)(uint8_t)x[i + k] << (8 * k);
+        w.push_back(v);
+      }
+    };
+    put_string("poseidon2");
+    w.push_back((uint32_t)d_.exposed.size());  // exposed values, sorted by name
+    for (auto& x : d_.exposed) {
+      put_string(x.name);
+      w.push_back(1);  // entries
+      w.push_back(0);  // index
+      w.insert(w.end(), totals_[b][x.col].v, totals_[b][x.col].v + 6);
+    }
+    // commitments: the trace and AIR roots; setup roots are the verification key's
+    w.push_back((uint32_t)(nt - (d_.n_setup ? 1 : 0)));
+    for (size_t t = 0; t < nt; ++t)
+      if (d_.trees[t].kind != TREE_SETUP) w.insert(w.end(), roots_[t][b].v, roots_[t][b].v + 8);
+    // claimed values
+    w.push_back((uint32_t)nt);
+    size_t k = 0;
+    for (size_t t = 0; t < nt; ++t) {
+      w.push_back((uint32_t)d_.trees[t].groups.size());
+      for (auto& g : d_.trees[t].groups) {
+        for (int rail = 0; rail < 2; ++rail) {
+          const auto& sh = rail ? g.ext_shifts : g.base_shifts;
+          w.push_back((uint32_t)sh.size());
+          for (auto& ss : sh) {
+            w.push_back((uint32_t)ss.size());
+            for (size_t q = 0; q < ss.size(); ++q, ++k) w.insert(w.end(), cv[k].v, cv[k].v + 6);
+          }
+        }
+      }
+    }
+  }
   // setup.Setup on the device: uploads, RS-encodes and commits the setup
   // columns (this Job has one instance) into `out`, which then owns them.
   void commit_setup(const uint32_t* const* host_cols, SetupData& out);
@@ -973,6 +1015,7 @@ class Job {
   GraphBucket* gb_;
   Mode mode_;
   Tuning tune_;  // the plan's launch choices for this prove's batch
+  const bool skip_fri_;  // prover.SkipFRI: claimed values, no opening
   bool in_region_ = false;
   size_t next_buf_ = 0, n_pins_ = 0;
   // the fold rounds before the level enters, chained on the device
@@ -2259,6 +2302,42 @@ void Job::open() {
     launch_claims(stage(kv.second.sets), (int)kv.second.sets.size(), stage(kv.second.items),
                   (int)kv.second.items.size(), kv.first, part, s_, tune_.claims_rows);
   }
+  if (skip_fri_) {
+    // prover.SkipFRI (prover/prover.go:626-653, 691-700): the claimed values
+    // and nothing of the opening after them
+    std::vector<E6> cv((size_t)B_ * ncv);
+    region_end();
+    LG_HIP(hipMemcpyAsync(cv.data(), cvd.as<void>(), cv.size() * sizeof(E6), hipMemcpyDeviceToHost, s_));
+    sync();
+    replay_device_fs();
+    keep_.push_back(std::move(cvd));
+    for (auto& x : W) keep_.push_back(std::move(x.second));
+    parallel_for(B_, [&](int b) {
+      if (!inst_err_[b].empty()) {
+        out_[b].error = inst_err_[b];
+        out_[b].proof_len = 0;
+        return;
+      }
+      auto emit = [&](auto& w) {
+        emit_head(w, b, cv.data() + (size_t)b * ncv);
+        // levels root (zero), no levels, no roots, an empty extension-field
+        // final polynomial, no queries, no proofs of work, no samplings
+        const uint32_t tail[] = {0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 0u, 1u, 0u, 0u, 0u, 0u};
+        w.insert(w.end(), tail, tail + 15);
+      };
+      if (in_[b].proof_dst) {
+        SpanWriter w{in_[b].proof_dst, in_[b].proof_cap};
+        emit(w);
+        out_[b].proof_len = w.n;
+        out_[b].direct = true;
+      } else {
+        emit(out_[b].proof);
+        out_[b].proof_len = out_[b].proof.size();
+      }
+      out_[b].transcript = ts_[b].log_words();
+    });
+    return;
+  }
   // ---- alpha_DEEP: bind in class (module) order, then tree, group, base, ext
   std::vector<int32_t> bind_order;  // claimed-value indexes in binding order
   for (size_t mi = 0; mi < nm; ++mi)
@@ -3052,48 +3131,7 @@ void Job::open() {
     }
    };
    auto emit = [&](auto& w) {
-    const std::string hid = "poseidon2";
-    w.push_back((uint32_t)hid.size());
-    for (size_t i = 0; i < hid.size(); i += 4) {
-      uint32_t x = 0;
-      for (size_t k = 0; k < 4 && i + k < hid.size(); ++k) x |= (uint32_t)(uint8_t)hid[i + k] << (8 * k);
-      w.push_back(x);
-    }
-    w.push_back((uint32_t)d_.exposed.size());  // exposed values, sorted by name
-    for (auto& x : d_.exposed) {
-      w.push_back((uint32_t)x.name.size());
-      for (size_t i = 0; i < x.name.size(); i += 4) {
-        uint32_t v = 0;
-        for (size_t k = 0; k < 4 && i + k < x.name.size(); ++k) v |= (uint32_t)(uint8_t)x.name[i + k] << (8 * k);
-        w.push_back(v);
-      }
-      w.push_back(1);  // entries
-      w.push_back(0);  // index
-      w.insert(w.end(), totals_[b][x.col].v, totals_[b][x.col].v + 6);
-    }
-    // commitments: the trace and AIR roots; setup roots are the verification key's
-    w.push_back((uint32_t)(nt - (d_.n_setup ? 1 : 0)));
-    for (size_t t = 0; t < nt; ++t)
-      if (d_.trees[t].kind != TREE_SETUP) w.insert(w.end(), roots_[t][b].v, roots_[t][b].v + 8);
-    // claimed values
-    w.push_back((uint32_t)nt);
-    size_t k = 0;
-    for (size_t t = 0; t < nt; ++t) {
-      w.push_back((uint32_t)d_.trees[t].groups.size());
-      for (auto& g : d_.trees[t].groups) {
-        for (int rail = 0; rail < 2; ++rail) {
-          const auto& sh = rail ? g.ext_shifts : g.base_shifts;
-          w.push_back((uint32_t)sh.size());
-          for (auto& ss : sh) {
-            w.push_back((uint32_t)ss.size());
-            for (size_t q = 0; q < ss.size(); ++q, ++k) {
-              const E6& v = cvals[(size_t)b * ncv + k];
-              w.insert(w.end(), v.v, v.v + 6);
-            }
-          }
-        }
-      }
-    }
+    emit_head(w, b, cvals.data() + (size_t)b * ncv);
     const uint32_t* rec0 = gh + (size_t)b * nq * stride;
     // FRI proof
     w.insert(w.end(), levels_root[b].v, levels_root[b].v + 8);

@@ -190,6 +190,10 @@ struct Context {
   hipStream_t side_ = nullptr;
 };
 
+// Prove options: SkipFRI (prover.SkipFRI, prover/prover.go:78-83): the
+// proof carries the claimed values and no opening.
+constexpr uint32_t PROVE_SKIP_FRI = 1u;
+
 // Per-instance caller data.
 struct InstanceInput {
   std::vector<const uint32_t*> columns;  // plan input order, Montgomery, N_module words each
@@ -293,6 +297,7 @@ class Plan {
   std::mutex gmu_;
   std::map<std::tuple<int, int, bool>, std::unique_ptr<GraphBucket>> buckets_;  // (batch, lane, device inputs)
   uint64_t tick_ = 0;
+  uint32_t options_ = 0;
   GraphBucket* take_bucket(int batch, int lane, bool device_inputs, int* mode);
   void give_bucket(GraphBucket* b);
 
@@ -302,6 +307,9 @@ class Plan {
   size_t graph_bytes();
   // Drops every graph bucket (frees its buffers and graph).
   void drop_graphs();
+  // Prove options (PROVE_* flags), for the proves that follow.
+  uint32_t options() const { return options_; }
+  void set_options(uint32_t flags) { options_ = flags; }
   // Proves of this plan that recorded a graph, replayed one, or ran again
   // eagerly after a failed recording or replay.
   std::atomic<uint64_t> graph_records{0}, graph_replays{0}, graph_fallbacks{0};

@@ -304,7 +304,10 @@ Shape make_shape(const PlanData& d) {
   return S;
 }
 
-Decoded decode(const Shape& S, const uint32_t* w, size_t n) {
+// With skip_fri only the words up to the claimed values are read (the
+// opening of a SkipFRI proof is empty, and a SkipFRI verifier does not read
+// a full proof's).
+Decoded decode(const Shape& S, const uint32_t* w, size_t n, bool skip_fri) {
   const PlanData& d = *S.d;
   Reader r(w, n);
   Decoded D;
@@ -339,6 +342,7 @@ Decoded decode(const Shape& S, const uint32_t* w, size_t n) {
       }
     }
   }
+  if (skip_fri) return D;
   const size_t nl = S.lv_mod.size();
   D.levels_root = r.skip(8);
   need(r.count(4096, "levels") == nl - 1, VERIFY_FORMAT, "level query count");
@@ -414,7 +418,8 @@ struct ProofCtx {
   std::vector<const uint32_t*> roots;  // per tree
 };
 
-void replay(const Shape& S, const Plan& plan, const VerifyInput& in, const uint32_t* key, ProofCtx& P) {
+void replay(const Shape& S, const Plan& plan, const VerifyInput& in, const uint32_t* key, bool skip_fri,
+            ProofCtx& P) {
   const PlanData& d = *S.d;
   const uint32_t* w = P.w;
   Transcript ts(plan.schedule());
@@ -449,6 +454,7 @@ void replay(const Shape& S, const Plan& plan, const VerifyInput& in, const uint3
   }
   for (size_t t : S.air_trees) ts.bind("__zeta", P.roots[t], 8);
   P.R.zeta = e6_of(ts.compute("__zeta"));
+  if (skip_fri) return;  // nothing of the opening is replayed
   for (size_t cl : S.classes)
     for (size_t t = 0; t < S.nt; ++t)
       for (size_t g = 0; g < d.trees[t].groups.size(); ++g) {
@@ -811,7 +817,7 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
         ProofCtx& p = P[i];
         p.w = in[b0 + i].words;
         if (!p.w) fail(VERIFY_FORMAT, "no proof words");
-        p.D = decode(S, p.w, in[b0 + i].len);
+        p.D = decode(S, p.w, in[b0 + i].len, skip_fri);
         std::vector<uint32_t> root_off(S.nt);
         p.roots.resize(S.nt);
         for (size_t t = 0; t < S.nt; ++t) {
@@ -823,7 +829,7 @@ void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t
             root_off[t] = base[i] + p.D.commitments[t - S.n_setup_trees];
           }
         }
-        replay(S, plan, in[b0 + i], setup_roots, p);
+        replay(S, plan, in[b0 + i], setup_roots, skip_fri, p);
         check_buses_and_air(S, in[b0 + i], p);
         if (skip_fri) return;
         build_jobs(S, p, base[i], root_off, J[i]);

@@ -343,6 +343,13 @@ def graph_stats(ctx, items):
     return dict(zip(("bytes", "records", "replays", "fallbacks"), list(v)))
 
 
+def set_options(ctx, items, skip_fri=False):
+    """Prove options of the plan of these prepared instances
+    (lg_plan_set_options): skip_fri is prover.SkipFRI."""
+    ph = ctx.plan_handle(items[0].words, items[0].setup)
+    _lib.check(_lib.lib().lg_plan_set_options(ph, 1 if skip_fri else 0), "lg_plan_set_options")
+
+
 def set_tuning(ctx, items, tuning):
     ph = ctx.plan_handle(items[0].words, items[0].setup)
     v = (ctypes.c_int32 * 5)(*[int(tuning[f]) for f in TUNING_FIELDS])

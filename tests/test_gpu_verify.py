@@ -16,7 +16,9 @@ bridge):
   a FRI layer value -> FRI;
   a truncated proof -> format;
 and every mutated proof is rejected by the oracle's checker too, while the
-untouched proofs of the same batch stay valid.
+untouched proofs of the same batch stay valid.  prover.SkipFRI proofs carry
+the full proof's words up to the claimed values and an empty opening; the
+SkipFRI verifier accepts them.
 """
 import sys
 
@@ -101,6 +103,7 @@ class Offsets:
                         polys.append([self.skip(6) for _ in range(self.u())])
                 groups.append(polys)
             self.claimed.append(groups)
+        self.head_end = self.i  # the words up to the claimed values
         self.skip(8)
         for _ in range(self.u()):
             for _ in range(self.u()):
@@ -247,3 +250,31 @@ def test_verify_full_size_batch(ctx):
     log("verify: 40 x 2^14")
     res = prover.verify_prepared(ctx, items, proofs)
     assert res == [(OK, "")] * 40
+
+
+@pytest.mark.gpu
+def test_prove_skip_fri(ctx):
+    """prover.SkipFRI (prover/prover.go:626-653, 691-700): the same words as
+    the full proof up to the claimed values (the transcript up to zeta is the
+    same), then the empty opening; verifier.SkipFRI accepts it, the full
+    verifier does not, and proofs are full again once the option is cleared."""
+    from loom_amd import prover, workloads
+    nq = 16
+    insts = [workloads.plonk_instance(8, 900 + k) for k in range(3)]
+    items = [prover.Prepared(p, t, nq) for p, t in insts]
+    full = [f.copy() for f in prover.prove_prepared(ctx, items)]
+    prover.set_options(ctx, items, skip_fri=True)
+    try:
+        short = [f.copy() for f in prover.prove_prepared(ctx, items)]
+    finally:
+        prover.set_options(ctx, items, skip_fri=False)
+    empty = [0] * 8 + [0, 0, 1, 0, 0, 0, 0]
+    for f, sh in zip(full, short):
+        h = Offsets(f).head_end
+        assert np.array_equal(sh[:h], f[:h])
+        assert [int(x) for x in sh[h:]] == empty
+    assert prover.verify_prepared(ctx, items, short, skip_fri=True) == [(OK, "")] * 3
+    assert all(st == FORMAT for st, _ in prover.verify_prepared(ctx, items, short))
+    again = prover.prove_prepared(ctx, items)
+    for f, g in zip(full, again):
+        assert np.array_equal(f, g)
