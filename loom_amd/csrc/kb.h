@@ -64,7 +64,9 @@ LG_HD uint32_t kb_neg(uint32_t a) { return a ? KB_P - a : 0u; }
 // (-p^-1 = 0x7effffff), t + m p is divisible by 2^32 and below
 // 2^62 + 2^63, and (t + m p) / 2^32 < p^2/2^32 + p < 2p: one 64-bit
 // multiply-add and one conditional subtraction (on gfx950 a v_mad_u64_u32 with
-// the accumulator, then v_add + v_min).
+// the accumulator, then v_add + v_min).  The result is also reduced for
+// a < 2p, b < p: t + m p < 2p^2 + 2^32 p < 2^64 and the quotient is below
+// 2p^2/2^32 + p < 2p.
 LG_HD uint32_t kb_mul(uint32_t a, uint32_t b) {
   const uint64_t t = (uint64_t)a * b;
   const uint32_t m = (uint32_t)t * 0x7effffffu;

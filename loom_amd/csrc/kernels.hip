@@ -51,6 +51,15 @@ __device__ __forceinline__ __attribute__((address_space(1))) T& gat(gmem<T> base
   using C = __attribute__((address_space(1))) char*;
   return *(gmem<T>)((C)base + i * (uint32_t)sizeof(T));
 }
+// A uniform pointer made opaque in an SGPR pair: the compiler cannot
+// re-associate base + k * stride with the per-thread offset, so an access
+// gat(sreg(base + k * stride), t) stays a global access with that SGPR base
+// and the one shared VGPR offset (instead of a 64-bit VGPR address per k).
+template <class T>
+__device__ __forceinline__ gmem<T> sreg(gmem<T> p) {
+  asm("" : "+s"(p));
+  return p;
+}
 // 16-byte stores through gat (a builtin vector type: HIP's uint4 has no
 // assignment for an object in the global address space)
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
@@ -150,18 +159,6 @@ __global__ __launch_bounds__(1024) void k_ntt_multi(const NttJob* __restrict__ j
   }
 }
 
-// LDS word index of element a in k_ntt16: a ^ f(a >> 6) with
-// f(h) = ((h << 2) ^ h ^ (h >> 5)) mod 64, a bijection on [0, n) for n >= 64
-// (only the bank bits change, as a function of the others).  The kernel's
-// access patterns (bit-reversed scatter, 16 consecutive words per thread,
-// pass strides q = n/16 .. 1) then meet at most 2 words per bank where the
-// plain index meets up to 64 (found by exhaustive search over XOR forms for
-// 2^8..2^14 with 32 and 64 banks).
-__device__ __forceinline__ uint32_t lds_swz(uint32_t a) {
-  const uint32_t h = a >> 6;
-  return a ^ (((h << 2) ^ h ^ (h >> 5)) & 63u);
-}
-
 // Radix-16 form of k_ntt_multi for 2^8 <= n <= 2^14 (same job and argument
 // semantics): n/16 threads, 16 values per thread, four radix-2 stages per
 // pass in registers (ntt16.h), one LDS exchange and barrier per pass instead
@@ -175,8 +172,10 @@ __device__ __forceinline__ uint32_t lds_swz(uint32_t a) {
 // the host builds with ntt16_ladder: four 16-byte loads instead of 14 field
 // products per pass.
 __device__ __forceinline__ Ladder load_ladder(const uint32_t* __restrict__ lad, int logm, uint32_t b) {
-  const uint4* p = reinterpret_cast<const uint4*>(lad + 16 * ((1u << (logm - 4)) - 1u + b));
-  const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+  // uniform base of the size's block, per-thread 32-bit offset: global loads
+  // with an SGPR base and one VGPR offset
+  const gmem<const u32x4> p = (gmem<const u32x4>)gbl(lad + 16 * ((1u << (logm - 4)) - 1u));
+  const u32x4 v0 = gat(p, 4 * b), v1 = gat(p, 4 * b + 1), v2 = gat(p, 4 * b + 2), v3 = gat(p, 4 * b + 3);
   Ladder L;
   L.a[0] = v0.x; L.a[1] = v0.y; L.a[2] = v0.z; L.a[3] = v0.w;
   L.a[4] = v1.x; L.a[5] = v1.y; L.a[6] = v1.z; L.a[7] = v1.w;
@@ -186,67 +185,83 @@ __device__ __forceinline__ Ladder load_ladder(const uint32_t* __restrict__ lad, 
   return L;
 }
 
-template <int KIND>
+// 4-bit reversal of a value index (compile-time after unrolling).
+__device__ __forceinline__ constexpr uint32_t brev4(int k) {
+  return (uint32_t)(((k & 1) << 3) | ((k & 2) << 1) | ((k & 4) >> 1) | ((k & 8) >> 3));
+}
+
+// The size L is a template parameter so that every pass unrolls with its
+// stride q = 2^lq known: the LDS index of value k of a pass is then
+// lds_swz(base) ^ lds_swz(k << lq) (ntt16.h), one XOR with a constant per
+// access, and the scattered loads and stores of the bit-reversed copies are
+// lds_swz(brev(t) << 4) ^ brev4(k) likewise.
+template <int KIND, int L>
 __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs, NttArgs cfg,
                                                 const uint32_t* const* __restrict__ pres,
                                                 const uint32_t* __restrict__ tw_fwd,
                                                 const uint32_t* __restrict__ tw_inv,
                                                 const uint32_t* __restrict__ lad_fwd,
                                                 const uint32_t* __restrict__ lad_inv) {
+  static_assert(L >= 8 && L <= 14, "k_ntt16 covers 2^8 .. 2^14");
   extern __shared__ uint32_t sh[];
   const NttJob* __restrict__ jp = jobs + blockIdx.x;
-  const int L = cfg.logn;
-  const uint32_t n = 1u << L;
-  const uint32_t q0 = n >> 4;
+  constexpr uint32_t n = 1u << L;
+  constexpr uint32_t q0 = n >> 4;
+  constexpr int P = L >> 2, r = L & 3;
   const uint32_t t = threadIdx.x;
-  const int P = L >> 2, r = L & 3;
+  // Global accesses of value k go through the uniform pointer base + k q0
+  // (scalar arithmetic) with the one per-thread offset t, so the 16 offsets
+  // take no vector registers.
+  // LDS index bases: the bit-reversed scatter (value k of thread t at
+  // brev(t + k q0) = brev_{L-4}(t) << 4 | brev4(k)) and the contiguous
+  // 16-value row of the r-stage pass (16 t + k)
+  const uint32_t s_br = lds_swz(brev(t, L - 4) << 4);
+  const uint32_t s_row = lds_swz(16u * t);
   uint32_t x[16];
   if (cfg.inverse_first) {
     const NttRoots Ri = ntt16_roots(tw_inv, L);
     const gmem<const uint32_t> __restrict__ in = gbl(jp->in);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-      const uint32_t i = t + (uint32_t)k * q0;
-      sh[lds_swz(brev(i, L))] = gat(in, i);
-    }
+    for (int k = 0; k < 16; ++k) sh[s_br ^ brev4(k)] = gat(sreg(in + k * q0), t);
     __syncthreads();
-    if (r) {
+    if constexpr (r != 0) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = sh[lds_swz(16 * t + k)];
-      dit_small_r(x, r, Ri);
+      for (int k = 0; k < 16; ++k) x[k] = sh[s_row ^ (uint32_t)k];
+      dit_small<r>(x, Ri);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sh[lds_swz(16 * t + k)] = x[k];
+      for (int k = 0; k < 16; ++k) sh[s_row ^ (uint32_t)k] = x[k];
       __syncthreads();
     }
-    for (int p = 0; p < P; ++p) {
-      const int logm = r + 4 * (p + 1);
-      const uint32_t q = 1u << (logm - 4);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = sh[lds_swz(ntt16_pos(t, q, k))];
-      const Ladder Ld = load_ladder(lad_inv, logm, t % q);
+    for (int p = 0; p < P; ++p) {
+      const int lq = r + 4 * p;  // pass sub-size 2^(lq + 4)
+      const uint32_t sb = lds_swz(ntt16_base(t, lq));
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = sh[sb ^ lds_swz((uint32_t)k << lq)];
+      const Ladder Ld = load_ladder(lad_inv, lq + 4, t & ((1u << lq) - 1u));
       dit16(x, Ld);
       if (p < P - 1) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) sh[lds_swz(ntt16_pos(t, q, k))] = x[k];
+        for (int k = 0; k < 16; ++k) sh[sb ^ lds_swz((uint32_t)k << lq)] = x[k];
         __syncthreads();
       }
     }
     const gmem<const uint32_t> __restrict__ post = gbl(jp->post);
     if (post) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) x[k] = kb_mul(x[k], gat(post, t + (uint32_t)k * q0));
+      for (int k = 0; k < 16; ++k) x[k] = kb_mul(x[k], gat(sreg(post + k * q0), t));
     }
     if (cfg.n_out == 0) {
       const gmem<uint32_t> __restrict__ out = gbl(jp->out[0]);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) gat(out, t + (uint32_t)k * q0) = x[k];
+      for (int k = 0; k < 16; ++k) gat(sreg(out + k * q0), t) = x[k];
       return;
     }
     __syncthreads();  // the last inverse pass read LDS; the forward passes write it
   } else {
     const gmem<const uint32_t> __restrict__ in = gbl(jp->in);
 #pragma unroll
-    for (int k = 0; k < 16; ++k) x[k] = gat(in, t + (uint32_t)k * q0);
+    for (int k = 0; k < 16; ++k) x[k] = gat(sreg(in + k * q0), t);
   }
   const NttRoots Rf = ntt16_roots(tw_fwd, L);
   for (int o = 0; o < cfg.n_out; ++o) {
@@ -254,30 +269,36 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
     const gmem<const uint32_t> __restrict__ pre = gbl(pres[o]);
     if (pre) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) y[k] = kb_mul(x[k], gat(pre, t + (uint32_t)k * q0));
+      for (int k = 0; k < 16; ++k) y[k] = kb_mul(x[k], gat(sreg(pre + k * q0), t));
     } else {
 #pragma unroll
       for (int k = 0; k < 16; ++k) y[k] = x[k];
     }
+#pragma unroll
     for (int p = 0; p < P; ++p) {
-      const int logm = L - 4 * p;
-      const uint32_t q = 1u << (logm - 4);
+      const int lq = L - 4 * p - 4;
+      const uint32_t sb = lds_swz(ntt16_base(t, lq));
       if (p > 0) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) y[k] = sh[lds_swz(ntt16_pos(t, q, k))];
+        for (int k = 0; k < 16; ++k) y[k] = sh[sb ^ lds_swz((uint32_t)k << lq)];
       }
-      const Ladder Ld = load_ladder(lad_fwd, logm, t % q);
+      // the ladders do not depend on the output; the empty asm keeps the
+      // compiler from hoisting all passes' ladders (45 registers) out of the
+      // output loop, which spills
+      uint32_t lb = t & ((1u << lq) - 1u);
+      asm volatile("" : "+v"(lb));
+      const Ladder Ld = load_ladder(lad_fwd, lq + 4, lb);
       dif16(y, Ld);
       if (p < P - 1 || r) {
 #pragma unroll
-        for (int k = 0; k < 16; ++k) sh[lds_swz(ntt16_pos(t, q, k))] = y[k];
+        for (int k = 0; k < 16; ++k) sh[sb ^ lds_swz((uint32_t)k << lq)] = y[k];
         __syncthreads();
       }
     }
-    if (r) {
+    if constexpr (r != 0) {
 #pragma unroll
-      for (int k = 0; k < 16; ++k) y[k] = sh[lds_swz(16 * t + k)];
-      dif_small_r(y, r, Rf);
+      for (int k = 0; k < 16; ++k) y[k] = sh[s_row ^ (uint32_t)k];
+      dif_small<r>(y, Rf);
     }
     // y[k] is now the bit-reversed-order value at position 16 t + k
     const gmem<uint32_t> __restrict__ out = gbl(jp->out[o]);
@@ -292,15 +313,38 @@ __global__ __launch_bounds__(1024) void k_ntt16(const NttJob* __restrict__ jobs,
     } else {
       __syncthreads();  // other threads may still read LDS of this output's last pass
 #pragma unroll
-      for (int k = 0; k < 16; ++k) sh[lds_swz(16 * t + k)] = y[k];
+      for (int k = 0; k < 16; ++k) sh[s_row ^ (uint32_t)k] = y[k];
       __syncthreads();
+      // the scatter base again, from an opaque copy of t: hoisted out of the
+      // output loop, its 16 indices spill
+      uint32_t tt = t;
+      asm volatile("" : "+v"(tt));
+      const uint32_t s_out = lds_swz(brev(tt, L - 4) << 4);
 #pragma unroll
-      for (int k = 0; k < 16; ++k) {
-        const uint32_t i = t + (uint32_t)k * q0;
-        gat(out, i) = sh[lds_swz(brev(i, L))];
-      }
+      for (int k = 0; k < 16; ++k) gat(sreg(out + k * q0), t) = sh[s_out ^ brev4(k)];
     }
     __syncthreads();  // before the next output writes LDS
+  }
+}
+
+template <int KIND>
+static void launch_ntt16(int logn, dim3 g, dim3 b, size_t lds, hipStream_t s, const NttJob* jobs, NttArgs a,
+                         const uint32_t* const* pres, const uint32_t* tw_fwd, const uint32_t* tw_inv,
+                         const uint32_t* lad_fwd, const uint32_t* lad_inv) {
+  switch (logn) {
+#define LG_NTT16_CASE(LL)                                                                                    \
+  case LL:                                                                                                   \
+    hipLaunchKernelGGL((k_ntt16<KIND, LL>), g, b, lds, s, jobs, a, pres, tw_fwd, tw_inv, lad_fwd, lad_inv); \
+    break;
+    LG_NTT16_CASE(8)
+    LG_NTT16_CASE(9)
+    LG_NTT16_CASE(10)
+    LG_NTT16_CASE(11)
+    LG_NTT16_CASE(12)
+    LG_NTT16_CASE(13)
+    LG_NTT16_CASE(14)
+#undef LG_NTT16_CASE
+    default: throw std::runtime_error("launch_ntt: unsupported radix-16 size");
   }
 }
 
@@ -320,12 +364,13 @@ void launch_ntt(const NttJob* jobs_dev, int n_jobs, const NttCfg& cfg, const uin
   const size_t lds = n * sizeof(uint32_t);
   if (cfg.logn >= 8) {
     const dim3 g(n_jobs), b(n / 16);
+    const int L = cfg.logn;
     switch (cfg.kind) {
-      case NTT_KIND_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
-      case NTT_KIND_QUOT_COSET: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_COSET>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
-      case NTT_KIND_QUOT_BACK: hipLaunchKernelGGL(k_ntt16<NTT_KIND_QUOT_BACK>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
-      case NTT_KIND_CHUNKS: hipLaunchKernelGGL(k_ntt16<NTT_KIND_CHUNKS>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
-      case NTT_KIND_DEEP_LDE: hipLaunchKernelGGL(k_ntt16<NTT_KIND_DEEP_LDE>, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_LDE: launch_ntt16<NTT_KIND_LDE>(L, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_QUOT_COSET: launch_ntt16<NTT_KIND_QUOT_COSET>(L, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_QUOT_BACK: launch_ntt16<NTT_KIND_QUOT_BACK>(L, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_CHUNKS: launch_ntt16<NTT_KIND_CHUNKS>(L, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
+      case NTT_KIND_DEEP_LDE: launch_ntt16<NTT_KIND_DEEP_LDE>(L, g, b, lds, s, jobs_dev, a, pres_dev, tw_fwd, tw_inv, lad_fwd, lad_inv); break;
       default: throw std::runtime_error("launch_ntt: unknown kind");
     }
     LG_HIP(hipGetLastError());

@@ -27,6 +27,29 @@ struct NttRoots {
 // Position of value k of thread t in a full pass on sub-size m = 16 q.
 LG_HD uint32_t ntt16_pos(uint32_t t, uint32_t q, int k) { return (t / q) * 16u * q + (t % q) + (uint32_t)k * q; }
 
+// LDS word index of element a in the LDS-resident NTT: a ^ f(a >> 6) with
+// f(h) = ((h << 2) ^ h ^ (h >> 5)) mod 64, a bijection on [0, n) for n >= 64
+// (only the bank bits change, as a function of the others).  The kernel's
+// access patterns (bit-reversed scatter, 16 consecutive words per thread,
+// pass strides q = n/16 .. 1) then meet at most 2 words per bank where the
+// plain index meets up to 64 (found by exhaustive search over XOR forms for
+// 2^8..2^14 with 32 and 64 banks).
+//
+// The map is linear over GF(2) (XORs of shifted copies, masked), so for a
+// and b with disjoint bits lds_swz(a | b) = lds_swz(a) ^ lds_swz(b).  Every
+// access pattern of a pass is a per-thread base OR-ed with k << lq (value
+// index k < 16, fixed lq), so its LDS index is lds_swz(base) ^ lds_swz(k <<
+// lq): one XOR with a compile-time constant once the pass is unrolled.
+LG_HD constexpr uint32_t lds_swz(uint32_t a) {
+  return a ^ ((((a >> 6) << 2) ^ (a >> 6) ^ (a >> 11)) & 63u);
+}
+
+// The base of thread t in a full pass with q = 2^lq: ntt16_pos(t, q, k) =
+// ntt16_base(t, lq) | (k << lq), the two parts with disjoint bits.
+LG_HD uint32_t ntt16_base(uint32_t t, int lq) {
+  return ((t >> lq) << (lq + 4)) | (t & ((1u << lq) - 1u));
+}
+
 // Twiddle ladder of a full pass from W = w_m^b.
 struct Ladder {
   uint32_t a[8];  // W * w16^k  (size-m stage)
@@ -50,10 +73,13 @@ LG_HD Ladder ntt16_ladder(uint32_t W, const NttRoots& R) {
   return L;
 }
 
-// DIF butterfly: (u, v) -> (u + v, (u - v) w).
+// DIF butterfly: (u, v) -> (u + v, (u - v) w).  The difference enters the
+// product unreduced as u + (p - v) in [1, 2p): kb_mul's result is reduced for
+// a first operand below 2p (twiddle w < p), which saves the difference's
+// correction.
 LG_HD void dif_bf(uint32_t& u, uint32_t& v, uint32_t w) {
   const uint32_t s = kb_add(u, v);
-  v = kb_mul(kb_sub(u, v), w);
+  v = kb_mul(u + (KB_P - v), w);
   u = s;
 }
 LG_HD void dif_bf1(uint32_t& u, uint32_t& v) {

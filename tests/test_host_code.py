@@ -165,3 +165,23 @@ def test_transcript_forms_match_oracle():
         want.append([int(v) for v in t.compute(n)])
     assert got == want
 
+
+
+def test_lds_index_decomposition():
+    """The radix-16 NTT kernel computes each LDS index of a pass as
+    lds_swz(base) ^ lds_swz(k << lq) (and the scatter and row indices
+    likewise); the host check compares that with lds_swz of the full index for
+    every size, pass, thread and value, and checks the map is a bijection."""
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    d = tempfile.mkdtemp()
+    try:
+        exe = os.path.join(d, "ntt16_swizzle_check")
+        subprocess.run([gxx, "-O2", "-std=c++17", "-I" + CSRC, os.path.join(HERE, "host", "ntt16_swizzle_check.cpp"),
+                        "-o", exe], check=True)
+        r = subprocess.run([exe], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout
+        assert r.stdout.split()[-1] == "0"
+    finally:
+        shutil.rmtree(d)
