@@ -67,16 +67,16 @@ LG_HD void p2_external(uint32_t* s) {
 
 template <>
 struct P2Consts<24> {
-  static LG_HD uint32_t full(int r, int i) { return P2_W24_FULL_RK[r][i]; }
-  static LG_HD uint32_t partial(int r) { return P2_W24_PARTIAL_RK[r]; }
-  static LG_HD uint32_t diag(int i) { return P2_W24_DIAG[i]; }
+  static LG_HD constexpr uint32_t full(int r, int i) { return P2_W24_FULL_RK[r][i]; }
+  static LG_HD constexpr uint32_t partial(int r) { return P2_W24_PARTIAL_RK[r]; }
+  static LG_HD constexpr uint32_t diag(int i) { return P2_W24_DIAG[i]; }
 };
 
 template <>
 struct P2Consts<16> {
-  static LG_HD uint32_t full(int r, int i) { return P2_W16_FULL_RK[r][i]; }
-  static LG_HD uint32_t partial(int r) { return P2_W16_PARTIAL_RK[r]; }
-  static LG_HD uint32_t diag(int i) { return P2_W16_DIAG[i]; }
+  static LG_HD constexpr uint32_t full(int r, int i) { return P2_W16_FULL_RK[r][i]; }
+  static LG_HD constexpr uint32_t partial(int r) { return P2_W16_PARTIAL_RK[r]; }
+  static LG_HD constexpr uint32_t diag(int i) { return P2_W16_DIAG[i]; }
 };
 
 // x^3 for x < p.  The square skips its final correction: it is below 2p, and
@@ -199,6 +199,114 @@ LG_HD void p2_partial_round(uint32_t* s, int r) {
   p2_diag_all<W, 0>(s, p2_state_sum<W>(s));
 }
 
+// Device form of the 21 partial rounds: the internal layer without a
+// reduction per lane.  Round r keeps lane values u_i = sigma_r s_i (mod p),
+// below 2p, for the field scale sigma_r = 2^(-8r) (s: the Montgomery state).
+// The layer s_i <- S + d_i s_i becomes, times 2^24,
+//   T_i = 2^24 sum_j u_j + e_i u_i,   e_i = 2^24 d_i,
+// where every diagonal entry here is a small integer or +-2^-k (k <= 24), so
+// e_i is an integer up to 2^26 (a negative one multiplies 2p - u_i instead);
+// T_i < 48p 2^24 + 2^26 2p < 2^62, and one Montgomery reduction
+// (T_i + m p) / 2^32, m = T_i (-p^-1) mod 2^32, gives the next lane below
+// (2^62 + 2^63) / 2^32 < 2p with scale sigma_r 2^24 2^-32 = sigma_(r+1): two
+// multiply-adds per lane instead of a modular addition and a reduced product.
+// Lane 0's S-box works on the canonical scaled value plus the scaled key and
+// scales its cube back (times sigma_r^-2).  After the last round one product
+// by 2^168 restores the scale and reduces.
+constexpr uint32_t p2_mulmod(uint32_t a, uint32_t b) { return (uint32_t)((uint64_t)a * b % KB_P); }
+
+template <int W>
+struct P2Scaled {
+  // kind of each diagonal entry as a multiple of 2^-24: e = 2^24 d as a
+  // signed integer (0 if d is neither a small integer nor +-2^-k)
+  static constexpr int64_t e(int i) {
+    const DiagOp op = p2_diag_op(P2Diag<W>::canon(i));
+    const int64_t m = op.kind == 1 ? ((int64_t)op.c << 24) : (op.kind == 2 ? ((int64_t)1 << (24 - op.c)) : 0);
+    return op.neg ? -m : m;
+  }
+  static constexpr bool ok() {
+    for (int i = 0; i < W; ++i)
+      if (e(i) == 0) return false;
+    return true;
+  }
+  // sigma_r times the partial-round key r (Montgomery form is linear)
+  static constexpr uint32_t rk(int r) {
+    return p2_mulmod(P2Consts<W>::partial(r), p2_pow_canon(p2_pow_canon(2, KB_P - 2), 8u * (uint32_t)r));
+  }
+  // Montgomery form of sigma_r^-2 = 2^(16r)
+  static constexpr uint32_t cfix(int r) { return p2_mulmod(p2_pow_canon(2, 16u * (uint32_t)r), KB_R); }
+  // Montgomery form of sigma_21^-1 = 2^168
+  static constexpr uint32_t cend() { return p2_mulmod(p2_pow_canon(2, 168), KB_R); }
+};
+
+// x * C + acc as one 32 x 32 + 64-bit multiply-add (v_mad_u64_u32): on the
+// device the constant passes through an empty asm, so that a power of two
+// does not become a zero extension, a 64-bit shift and a 64-bit add.
+template <uint32_t C>
+LG_HD uint64_t p2_mad_c(uint32_t x, uint64_t acc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t c = C;
+  asm("" : "+s"(c));
+  return (uint64_t)x * c + acc;
+#else
+  return (uint64_t)x * C + acc;
+#endif
+}
+
+// Sum of the W lane values (each below 2p) in 64 bits, one multiply-add by an
+// opaque 1 per lane: plain additions of zero-extended values cost a register
+// move and a 64-bit add each.  The asm input I keeps the W multipliers
+// distinct, so that they are not merged and the sum not re-factored.
+template <int W, int I>
+LG_HD uint64_t p2_scaled_sum(const uint32_t* u, uint64_t acc) {
+  if constexpr (I < W) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t one = 1;
+    asm("" : "+s"(one) : "i"(I));
+    acc = (uint64_t)u[I] * one + acc;
+#else
+    acc += u[I];
+#endif
+    return p2_scaled_sum<W, I + 1>(u, acc);
+  }
+  return acc;
+}
+
+template <int W, int I>
+LG_HD void p2_scaled_lane(uint32_t* u, uint64_t s24) {
+  if constexpr (I < W) {
+    constexpr int64_t e = P2Scaled<W>::e(I);
+    const uint32_t x = e < 0 ? 2u * KB_P - u[I] : u[I];
+    const uint64_t t = p2_mad_c<(uint32_t)(e < 0 ? -e : e)>(x, s24);
+    const uint32_t m = (uint32_t)t * 0x7effffffu;
+    u[I] = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
+    p2_scaled_lane<W, I + 1>(u, s24);
+  }
+}
+
+// Round R of the scaled partial rounds (the round is a template parameter so
+// that its key and scale constants are compile-time literals).
+template <int W, int R>
+LG_HD void p2_partial_scaled(uint32_t* u) {
+  if constexpr (R < 21) {
+    constexpr uint32_t rk = P2Scaled<W>::rk(R), cfix = P2Scaled<W>::cfix(R);
+    uint32_t x0 = kb_umin(u[0], u[0] - KB_P);
+    x0 = kb_add(x0, rk);
+    u[0] = kb_mul(kb_cube(x0), cfix);
+    p2_scaled_lane<W, 0>(u, p2_scaled_sum<W, 0>(u, 0) << 24);
+    p2_partial_scaled<W, R + 1>(u);
+  }
+}
+
+template <int W>
+LG_HD void p2_partial_rounds_scaled(uint32_t* u) {
+  static_assert(P2Scaled<W>::ok(), "diagonal entries must be small integers or +-2^-k");
+  p2_partial_scaled<W, 0>(u);
+  constexpr uint32_t cend = P2Scaled<W>::cend();
+#pragma unroll
+  for (int i = 0; i < W; ++i) u[i] = kb_mul(u[i], cend);
+}
+
 #if !defined(__HIP_DEVICE_COMPILE__)
 // Host form of the partial rounds.  The internal layer is linear with small
 // diagonal entries, so lanes 1..W-1 are carried as signed 64-bit integers
@@ -263,8 +371,11 @@ inline void p2_partial_rounds_host(uint32_t* s) {
 }
 #endif
 
-template <int W>
-LG_HD void p2_permute(uint32_t* s) {
+// The permutation with the partial rounds in their device form (DEV) or
+// their host form; p2_permute picks the one of the compilation target (the
+// host build of the device form is for tests).
+template <int W, bool DEV>
+LG_HD void p2_permute_form(uint32_t* s) {
   // full round r = keys, S-boxes, external layer; the keys of a full round
   // that follows an external layer are folded into that layer's reduction
   p2_external<W, 0>(s);
@@ -275,10 +386,14 @@ LG_HD void p2_permute(uint32_t* s) {
   p2_cube_all<W>(s);
   p2_external<W>(s);
 #if defined(__HIP_DEVICE_COMPILE__)
+  if constexpr (P2Scaled<W>::ok()) p2_partial_rounds_scaled<W>(s);
+  else {
 #pragma unroll
-  for (int r = 0; r < 21; ++r) p2_partial_round<W>(s, r);
+    for (int r = 0; r < 21; ++r) p2_partial_round<W>(s, r);
+  }
 #else
-  p2_partial_rounds_host<W>(s);
+  if constexpr (DEV) p2_partial_rounds_scaled<W>(s);
+  else p2_partial_rounds_host<W>(s);
 #endif
 #pragma unroll
   for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], P2Consts<W>::full(3, i));
@@ -288,6 +403,15 @@ LG_HD void p2_permute(uint32_t* s) {
   p2_external<W, 5>(s);
   p2_cube_all<W>(s);
   p2_external<W>(s);
+}
+
+template <int W>
+LG_HD void p2_permute(uint32_t* s) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  p2_permute_form<W, true>(s);
+#else
+  p2_permute_form<W, false>(s);
+#endif
 }
 
 // node = P([NODE, 0 x 7, L, R])[0..8]; l, r, out: 8 Montgomery elements each.

@@ -1,5 +1,6 @@
 // Host build of the device field and Poseidon2 code (kb.h, p2.h): reads
-// states from stdin, prints the permuted states, and checks the specialised
+// states from stdin, prints each permuted state twice (the host form of the
+// partial rounds, then their device form), and checks the specialised
 // divisions by powers of two against Montgomery products.
 #include <cstdio>
 #include <cstdlib>
@@ -34,8 +35,12 @@ int main() {
       if (scanf("%u", &v) != 1) return 1;
       s[i] = kb_to_mont(v);
     }
+    uint32_t d[24];
+    for (int i = 0; i < 24; ++i) d[i] = s[i];
     p2_permute<24>(s);
+    p2_permute_form<24, true>(d);  // the device form of the partial rounds
     for (int i = 0; i < 24; ++i) printf("%u%c", kb_from_mont(s[i]), i == 23 ? '\n' : ' ');
+    for (int i = 0; i < 24; ++i) printf("%u%c", kb_from_mont(d[i]), i == 23 ? '\n' : ' ');
   }
   return bad ? 2 : 0;
 }

@@ -28,6 +28,8 @@ def p2_bin():
 
 
 def test_div_pow2_and_permutation_match_oracle(p2_bin):
+    """Both forms of the partial rounds (the host's lazy signed 64-bit lanes and
+    the device's scaled lanes) give the oracle's permutation."""
     from oracle import poseidon2 as p2
     rng = np.random.default_rng(5)
     P = 2**31 - 2**24 + 1
@@ -49,7 +51,10 @@ def test_div_pow2_and_permutation_match_oracle(p2_bin):
     got = np.array([[int(x) for x in ln.split()] for ln in lines[1:]], dtype=np.uint64)
     perm = p2.Permutation(24)
     want = perm(states.T.copy()).T
-    assert np.array_equal(got, want)
+    # host form of the partial rounds, then the device form (scaled lanes,
+    # one Montgomery reduction per lane and round)
+    assert np.array_equal(got[0::2], want)
+    assert np.array_equal(got[1::2], want)
 
 
 @pytest.fixture(scope="module")
