@@ -66,6 +66,37 @@ struct BigJob {
 };
 void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw_a,
                     const uint32_t* tw_b, const uint32_t* wlo, const uint32_t* whi, int bitrev_out, hipStream_t s);
+// The two passes alone (pass A: column transforms of size 2^b, twiddle,
+// transpose into tmp; pass B: row transforms of size 2^a of tmp into out).
+void launch_ntt_pass_a(const BigJob* jobs, int n_jobs, int a, int b, const uint32_t* tw_a, const uint32_t* wlo,
+                       const uint32_t* whi, hipStream_t s);
+void launch_ntt_pass_b(const BigJob* jobs, int n_jobs, int a, int b, const uint32_t* tw_b, int bitrev_out,
+                       hipStream_t s);
+
+// Fused middle pass of a large LDE (n = 2^(a+b) >= 2^15, a = ceil(log n / 2)):
+// the inverse transform's row pass and the first pass of every coset's
+// forward transform, with the factors swapped.  For 8 rows k2 of tmp
+// (tmp[k2 2^a + j1], the inverse's pass-A output) a workgroup takes the
+// inverse DFT over j1 (giving the coefficients c[k2 + 2^b k1]), and for each
+// output o: scales them by pre_o (geometric: pre_o[k2] * blk_o[k1 position]),
+// takes the forward DFT over k1, multiplies by w_n^(k2 i1) and stores
+// u_o[i1 2^b + k2], the input of pass B with the factors swapped.
+struct LdeMidJob {
+  const uint32_t* tmp;               // 2^(a+b) words
+  uint32_t* u[NTT_MAX_OUT];          // per output, 2^(a+b) words
+};
+struct LdeMidArgs {
+  int a, b, n_out;
+  const uint32_t* pre[NTT_MAX_OUT];  // per output: pre_o[k] for k < 2^b (its first 2^b entries)
+  const uint32_t* blk[NTT_MAX_OUT];  // per output: blk_o[p] = s_o^(2^b brev_a(p)), p < 2^a
+  const uint32_t* lad_inv;           // radix-16 pass ladders (Tables::ntt_ladders)
+  const uint32_t* lad_fwd;
+  const uint32_t* tw_inv;            // w_{2^a}^-e, e < 2^(a-1) (r-stage roots)
+  const uint32_t* tw_fwd;            // w_{2^a}^e
+  const uint32_t* wlo;               // w_n^e, e < 2048
+  const uint32_t* whi;               // w_n^(2048 e), e < n / 2048
+};
+void launch_lde_mid(const LdeMidJob* jobs, int n_jobs, const LdeMidArgs& args, hipStream_t s);
 
 // Leaf hashing: for instance b (grid y) and pair leaf k, hashes the message
 // [LEAF, hdr1, hdr2, ...] whose element m >= 3 is column msgmap[m] >> 1 at row

@@ -525,6 +525,153 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ 
   }
 }
 
+void launch_ntt_pass_a(const BigJob* jobs, int n_jobs, int a, int b, const uint32_t* tw_a, const uint32_t* wlo,
+                       const uint32_t* whi, hipStream_t s) {
+  if (n_jobs == 0) return;
+  if (a < 7 || a > 11 || b < 4 || b > 11) throw std::runtime_error("launch_ntt_pass_a: unsupported size");
+  if (!prelaunch()) return;
+  hipLaunchKernelGGL(k_ntt_pass_a, dim3((1 << a) / BIG_C, n_jobs), dim3(BIG_C * ((1 << b) >> 4)),
+                     (size_t)BIG_C * (1 << b) * 4, s, jobs, a, b, tw_a, wlo, whi);
+  LG_HIP(hipGetLastError());
+}
+
+void launch_ntt_pass_b(const BigJob* jobs, int n_jobs, int a, int b, const uint32_t* tw_b, int bitrev_out,
+                       hipStream_t s) {
+  if (n_jobs == 0) return;
+  if (a < 4 || a > 11 || b < 7 || b > 11) throw std::runtime_error("launch_ntt_pass_b: unsupported size");
+  if (!prelaunch()) return;
+  hipLaunchKernelGGL(k_ntt_pass_b, dim3((1 << b) / BIG_C, n_jobs), dim3(BIG_C * ((1 << a) >> 4)),
+                     (size_t)BIG_C * (1 << a) * 4, s, jobs, a, b, tw_b, bitrev_out);
+  LG_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ fused LDE middle pass
+// LDS index of position i of row c (8 interleaved rows): the low 3 position
+// bits flipped by bits 4-6 (conflict-free for the pass patterns, as in pass
+// A).  The map is linear over GF(2), so a pass's index is a per-thread base
+// XOR a constant per value.
+__device__ __forceinline__ constexpr uint32_t idx8(uint32_t i) { return i ^ ((i >> 4) & 7u); }
+
+template <int A>
+__global__ __launch_bounds__(1024) void k_lde_mid(const LdeMidJob* __restrict__ jobs, LdeMidArgs g) {
+  static_assert(A >= 8 && A <= 11, "row sizes 2^8 .. 2^11");
+  extern __shared__ uint32_t sh[];
+  constexpr uint32_t n1 = 1u << A, q0 = n1 >> 4;
+  constexpr int P = A >> 2, R = A & 3;
+  const LdeMidJob* job = jobs + blockIdx.y;
+  const uint32_t c = threadIdx.x & (BIG_C - 1), tt = threadIdx.x >> 3;
+  const uint32_t n2 = 1u << g.b;
+  const uint32_t k2 = xcd_group(blockIdx.x, gridDim.x) * BIG_C + c;
+  const uint32_t s_row = (idx8(16u * tt) << 3) | c;  // position 16 tt + k: s_row ^ (k << 3)
+  uint32_t x[16];
+  {
+    const gmem<const uint32_t> tmp = gbl(job->tmp);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) x[k] = gat(sreg(tmp + k * q0), k2 * n1 + tt);
+  }
+  // inverse DFT over j1 (decimation in frequency): natural positions
+  // tt + k q0 in, bit-reversed positions 16 tt + k out, i.e. x[k] =
+  // c[k2 + n2 brev_A(16 tt + k)] (times n: 1/n is in the pre-scale)
+  {
+    const NttRoots Ri = ntt16_roots(g.tw_inv, A);
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int lq = A - 4 * p - 4;
+      const uint32_t sb = (idx8(ntt16_base(tt, lq)) << 3) | c;
+      if (p > 0) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) x[k] = sh[sb ^ (idx8((uint32_t)k << lq) << 3)];
+      }
+      const Ladder Ld = load_ladder(g.lad_inv, lq + 4, tt & ((1u << lq) - 1u));
+      dif16(x, Ld);
+      if (p < P - 1 || R) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh[sb ^ (idx8((uint32_t)k << lq) << 3)] = x[k];
+        __syncthreads();
+      }
+    }
+    if constexpr (R != 0) {
+#pragma unroll
+      for (int k = 0; k < 16; ++k) x[k] = sh[s_row ^ ((uint32_t)k << 3)];
+      dif_small<R>(x, Ri);
+    }
+  }
+  // twiddles w_n^(k2 i1) for i1 = tt + k q0: w_n^(k2 tt) times (w_n^(k2 q0))^k
+  const uint32_t e0 = k2 * tt, e1 = k2 * q0;  // < n
+  const uint32_t tw0 = kb_mul(g.wlo[e0 & 2047u], g.whi[e0 >> 11]);
+  const uint32_t stp = kb_mul(g.wlo[e1 & 2047u], g.whi[e1 >> 11]);
+  const NttRoots Rf = ntt16_roots(g.tw_fwd, A);
+  for (int o = 0; o < g.n_out; ++o) {
+    uint32_t y[16];
+    {
+      // pre-scale of coefficient k2 + n2 k1: blk_o at the value's position
+      // (16 contiguous words per thread), the row factor pre_o[k2] later
+      const gmem<const u32x4> b4 = (gmem<const u32x4>)gbl(g.blk[o]);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const u32x4 v = gat(b4, 4 * tt + (uint32_t)k);
+        y[4 * k] = kb_mul(x[4 * k], v.x);
+        y[4 * k + 1] = kb_mul(x[4 * k + 1], v.y);
+        y[4 * k + 2] = kb_mul(x[4 * k + 2], v.z);
+        y[4 * k + 3] = kb_mul(x[4 * k + 3], v.w);
+      }
+    }
+    __syncthreads();  // the previous LDS reads (inverse or previous output) are done
+    // forward DFT over k1 (decimation in time): bit-reversed positions in,
+    // natural positions tt + k q0 out
+    if constexpr (R != 0) {
+      dit_small<R>(y, Rf);
+#pragma unroll
+      for (int k = 0; k < 16; ++k) sh[s_row ^ ((uint32_t)k << 3)] = y[k];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+      const int lq = R + 4 * p;
+      const uint32_t sb = (idx8(ntt16_base(tt, lq)) << 3) | c;
+      if (p > 0 || R != 0) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) y[k] = sh[sb ^ (idx8((uint32_t)k << lq) << 3)];
+      }
+      // not hoisted out of the output loop (the ladders of all passes would
+      // stay live)
+      uint32_t lb = tt & ((1u << lq) - 1u);
+      asm volatile("" : "+v"(lb));
+      const Ladder Ld = load_ladder(g.lad_fwd, lq + 4, lb);
+      dit16(y, Ld);
+      if (p < P - 1) {
+#pragma unroll
+        for (int k = 0; k < 16; ++k) sh[sb ^ (idx8((uint32_t)k << lq) << 3)] = y[k];
+        __syncthreads();
+      }
+    }
+    // times pre_o[k2] w_n^(k2 i1), transposed: u_o[i1 n2 + k2]
+    uint32_t t = kb_mul(tw0, g.pre[o][k2]);
+    const gmem<uint32_t> u = gbl(job->u[o]);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+      gat(sreg(u + k * q0 * n2), tt * n2 + k2) = kb_mul(y[k], t);
+      t = kb_mul(t, stp);
+    }
+  }
+}
+
+void launch_lde_mid(const LdeMidJob* jobs, int n_jobs, const LdeMidArgs& args, hipStream_t s) {
+  if (n_jobs == 0) return;
+  if (args.a < 8 || args.a > 11 || args.b < 7 || args.b > args.a || args.n_out < 1 || args.n_out > NTT_MAX_OUT)
+    throw std::runtime_error("launch_lde_mid: unsupported shape");
+  if (!prelaunch()) return;
+  const dim3 grid((1u << args.b) / BIG_C, n_jobs), block(BIG_C * ((1u << args.a) >> 4));
+  const size_t lds = (size_t)BIG_C * (1u << args.a) * 4;
+  switch (args.a) {
+    case 8: hipLaunchKernelGGL(k_lde_mid<8>, grid, block, lds, s, jobs, args); break;
+    case 9: hipLaunchKernelGGL(k_lde_mid<9>, grid, block, lds, s, jobs, args); break;
+    case 10: hipLaunchKernelGGL(k_lde_mid<10>, grid, block, lds, s, jobs, args); break;
+    default: hipLaunchKernelGGL(k_lde_mid<11>, grid, block, lds, s, jobs, args); break;
+  }
+  LG_HIP(hipGetLastError());
+}
+
 void launch_ntt_big(const BigJob* jobs_a, const BigJob* jobs_b, int n_jobs, int logn, const uint32_t* tw_a,
                     const uint32_t* tw_b, const uint32_t* wlo, const uint32_t* whi, int bitrev_out, hipStream_t s) {
   if (n_jobs == 0) return;

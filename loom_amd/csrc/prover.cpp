@@ -105,19 +105,42 @@ const uint32_t* Tables::wpow_inv(int logn) {
   return get("wpi" + std::to_string(logn), [&] { return powers(kb_inv(omega(logn)), 1u << logn); });
 }
 
-const uint32_t* Tables::lde_pre(int logn, int b, int rate_log, bool from_values) {
-  return get("lde" + std::to_string(logn) + "_" + std::to_string(b) + "_" + std::to_string(from_values), [&] {
-    uint32_t s = kb_pow(omega(logn + rate_log), brev_host(b, rate_log));
-    uint32_t sc = from_values ? kb_inv(M(1u << logn)) : KB_R;
-    return powers(s, 1u << logn, sc);
+const uint32_t* Tables::geo(const std::string& key, uint32_t s, size_t n, uint32_t sc) {
+  const uint32_t* t = get(key, [&] { return powers(s, n, sc); });
+  std::lock_guard<std::mutex> lock(geo_mu_);
+  geo_[t] = s;
+  return t;
+}
+
+bool Tables::geometric(const uint32_t* t, uint32_t* s) {
+  std::lock_guard<std::mutex> lock(geo_mu_);
+  auto it = geo_.find(t);
+  if (it == geo_.end()) return false;
+  *s = it->second;
+  return true;
+}
+
+const uint32_t* Tables::geo_blocks(uint32_t s, int a, int b) {
+  return get("gblk" + std::to_string(s) + "_" + std::to_string(a) + "_" + std::to_string(b), [&] {
+    const uint32_t sb = kb_pow(s, 1ull << b);  // s^(2^b)
+    const std::vector<uint32_t> pw = powers(sb, 1u << a);
+    std::vector<uint32_t> v(1u << a);
+    for (uint32_t p = 0; p < (1u << a); ++p) v[p] = pw[brev_host(p, a)];
+    return v;
   });
 }
 
+const uint32_t* Tables::lde_pre(int logn, int b, int rate_log, bool from_values) {
+  const uint32_t s = kb_pow(omega(logn + rate_log), brev_host(b, rate_log));
+  const uint32_t sc = from_values ? kb_inv(M(1u << logn)) : KB_R;
+  return geo("lde" + std::to_string(logn) + "_" + std::to_string(b) + "_" + std::to_string(from_values), s,
+             (size_t)1 << logn, sc);
+}
+
 const uint32_t* Tables::quot_pre(int logn, int i, int rho_log) {
-  return get("qpre" + std::to_string(logn) + "_" + std::to_string(i) + "_" + std::to_string(rho_log), [&] {
-    uint32_t s = kb_mul(M(MULT_GEN), kb_pow(omega(logn + rho_log), i));
-    return powers(s, 1u << logn, kb_inv(M(1u << logn)));
-  });
+  const uint32_t s = kb_mul(M(MULT_GEN), kb_pow(omega(logn + rho_log), i));
+  return geo("qpre" + std::to_string(logn) + "_" + std::to_string(i) + "_" + std::to_string(rho_log), s,
+             (size_t)1 << logn, kb_inv(M(1u << logn)));
 }
 
 const uint32_t* Tables::quot_post(int logn, int i, int rho_log) {
@@ -286,6 +309,13 @@ hipStream_t Context::side_stream() {
   return side_;
 }
 
+hipEvent_t Context::lane_event(int lane) {
+  std::lock_guard<std::mutex> lock(lane_mu_);
+  LaneBufs& L = lane_.at(lane);
+  if (!L.event) LG_HIP(hipEventCreateWithFlags(&L.event, hipEventDisableTiming));
+  return L.event;
+}
+
 hipStream_t Context::lane_stream(int lane) {
   if (lane == 0) return stream;
   std::lock_guard<std::mutex> lock(lane_mu_);
@@ -348,6 +378,7 @@ Context::~Context() {
       (void)hipStreamSynchronize(L.stream);
       (void)hipStreamDestroy(L.stream);
     }
+    if (L.event) (void)hipEventDestroy(L.event);
   }
 }
 
@@ -742,6 +773,32 @@ struct Layer {  // an E6 vector of len elements per instance, 6 limb arrays
 
 struct FsMsgBuilder;
 
+// Orders the device phases of a batch's lanes: lane l's device phase starts
+// on the GPU once lane l-1's has ended, so that lane l-1's host work (query
+// chains, gathers, proof assembly) runs while lane l's kernels do.  A lane
+// signals after it has enqueued its device phase (the event then marks its
+// end on the GPU) or when it fails before that (no event: waiting on it is a
+// no-op).
+struct LaneGate {
+  std::mutex mu;
+  std::condition_variable cv;
+  std::vector<hipEvent_t> ev;  // per lane, recorded after its device phase
+  std::vector<char> done;      // per lane: signalled
+  void signal(int lane, hipStream_t s, bool record) {
+    if (record) LG_HIP(hipEventRecord(ev[lane], s));
+    std::lock_guard<std::mutex> lk(mu);
+    done[lane] = 1;
+    cv.notify_all();
+  }
+  void wait(int lane, hipStream_t s) {
+    if (lane == 0) return;
+    std::unique_lock<std::mutex> lk(mu);
+    cv.wait(lk, [&] { return done[lane - 1] != 0; });
+    lk.unlock();
+    LG_HIP(hipStreamWaitEvent(s, ev[lane - 1], 0));
+  }
+};
+
 class Job {
  public:
   // With a graph bucket the Job's device buffers, staging area and pinned
@@ -752,9 +809,9 @@ class Job {
   // graph.
   enum Mode { EAGER, CAPTURE, REPLAY };
   Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb,
-      int lane, GraphBucket* gb = nullptr, Mode mode = EAGER)
+      int lane, GraphBucket* gb = nullptr, Mode mode = EAGER, LaneGate* gate = nullptr)
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
-        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER),
+        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER), gate_(gate),
         tune_(plan.tuning(plan.tuning_batch())), skip_fri_((plan.options() & PROVE_SKIP_FRI) != 0) {
     if (out.size() != in.size()) throw std::runtime_error("prove: output slots do not match the inputs");
     if (gb_) {
@@ -775,6 +832,7 @@ class Job {
     ctx_.timer.launch_point(s_);
   }
   ~Job() {
+    if (gate_ && !gate_signaled_) gate_->signal(lane_, s_, false);  // failed before its device phase ended
     if (in_region_) {  // an error inside the device phase: leave capture / suppression
       set_launch_suppression(false);
       if (mode_ == CAPTURE) {
@@ -892,6 +950,7 @@ class Job {
   // plan, the batch size and the bucket's addresses).
   void region_begin() {
     stg().flush(s_);
+    if (gate_) gate_->wait(lane_, s_);  // outside any capture: the previous lane's device phase first
     in_region_ = true;
     if (mode_ == CAPTURE) LG_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
     if (mode_ == REPLAY) {
@@ -922,6 +981,10 @@ class Job {
     }
     in_region_ = false;
     if (mode_ == CAPTURE) LG_HIP(hipGraphLaunch(gb_->exec, s_));
+    if (gate_) {
+      gate_->signal(lane_, s_, true);
+      gate_signaled_ = true;
+    }
   }
   template <class T>
   const T* stage(const std::vector<T>& v) { return stg().put(v, s_); }
@@ -1013,6 +1076,8 @@ class Job {
   hipStream_t s_;
   std::unique_ptr<Staging> stg_;
   GraphBucket* gb_;
+  LaneGate* gate_ = nullptr;
+  bool gate_signaled_ = false;
   Mode mode_;
   Tuning tune_;  // the plan's launch choices for this prove's batch
   const bool skip_fri_;  // prover.SkipFRI: claimed values, no opening
@@ -1377,11 +1442,66 @@ void run_ntt(Tables& tb, const NttCfg& cfg, const std::vector<NttJob>& jobs, hip
                tb.ntt_ladders(true), s);
     return;
   }
+  const size_t n = (size_t)1 << logn;
+  const int la = (logn + 1) / 2, lb = logn - la;  // sub-transform sizes 2^la (pass B) and 2^lb (pass A)
+  // Fused form of an inverse followed by forward outputs (the LDE, the
+  // quotient cosets, the DEEP LDE), when every pre-scale is a geometric table
+  // and no job has a post-scale: the inverse's pass A, one middle pass (the
+  // inverse's row transform and every output's first pass, factors swapped:
+  // launch_lde_mid) and every output's pass B: three passes per job instead
+  // of 2 (1 + n_out).  Scratch per job: n words for the pass-A output and n
+  // per output.
+  bool fuse = cfg.inverse_first && cfg.n_out >= 1;
+  uint32_t geo_s[NTT_MAX_OUT] = {0};
+  for (int o = 0; fuse && o < cfg.n_out; ++o)
+    fuse = cfg.pre[o] && tb.geometric(cfg.pre[o], &geo_s[o]) && cfg.bitrev_out[o] == cfg.bitrev_out[0];
+  for (size_t k = 0; fuse && k < jobs.size(); ++k) fuse = jobs[k].post == nullptr;
+  if (fuse) {
+    const size_t per_job = n * (1 + (size_t)cfg.n_out);
+    const size_t max_words = (size_t)256 << 20;
+    const size_t G = std::min(jobs.size(), std::max<size_t>(1, max_words / per_job));
+    uint32_t* sp = scratch(G * per_job);
+    LdeMidArgs ma{};
+    ma.a = la;
+    ma.b = lb;
+    ma.n_out = cfg.n_out;
+    for (int o = 0; o < cfg.n_out; ++o) {
+      ma.pre[o] = cfg.pre[o];
+      ma.blk[o] = tb.geo_blocks(geo_s[o], la, lb);
+    }
+    ma.lad_inv = tb.ntt_ladders(true);
+    ma.lad_fwd = tb.ntt_ladders(false);
+    ma.tw_inv = tb.tw_inv(la);
+    ma.tw_fwd = tb.tw_fwd(la);
+    ma.wlo = tb.wpow(logn);
+    ma.whi = tb.wpow(logn - 11);
+    for (size_t j0 = 0; j0 < jobs.size(); j0 += G) {
+      const size_t g = std::min(G, jobs.size() - j0);
+      auto tmp = [&](size_t k) { return sp + k * per_job; };
+      auto ubuf = [&](size_t k, int o) { return sp + k * per_job + n * (1 + (size_t)o); };
+      std::vector<BigJob> ja(g), jb(g * cfg.n_out);
+      std::vector<LdeMidJob> jm(g);
+      for (size_t k = 0; k < g; ++k) {
+        const NttJob& J = jobs[j0 + k];
+        ja[k] = BigJob{J.in, nullptr, tmp(k), nullptr, nullptr};
+        jm[k].tmp = tmp(k);
+        for (int o = 0; o < cfg.n_out; ++o) {
+          jm[k].u[o] = ubuf(k, o);
+          jb[k * cfg.n_out + o] = BigJob{nullptr, J.out[o], ubuf(k, o), nullptr, nullptr};
+        }
+      }
+      const BigJob* da = static_cast<const BigJob*>(stage(ja.data(), g * sizeof(BigJob)));
+      launch_ntt_pass_a(da, (int)g, la, lb, tb.tw_inv(lb), tb.wpow_inv(logn), tb.wpow_inv(logn - 11), s);
+      const LdeMidJob* dm = static_cast<const LdeMidJob*>(stage(jm.data(), g * sizeof(LdeMidJob)));
+      launch_lde_mid(dm, (int)g, ma, s);
+      const BigJob* db = static_cast<const BigJob*>(stage(jb.data(), jb.size() * sizeof(BigJob)));
+      launch_ntt_pass_b(db, (int)jb.size(), lb, la, tb.tw_fwd(lb), cfg.bitrev_out[0], s);
+    }
+    return;
+  }
   // Four-step form: every transform is two launches over a group of jobs;
   // per job an n-word transpose buffer, plus an n-word buffer for the
   // inverse result when forward outputs follow.  Groups bound the scratch.
-  const size_t n = (size_t)1 << logn;
-  const int la = (logn + 1) / 2, lb = logn - la;  // sub-transform sizes 2^la (pass B) and 2^lb (pass A)
   const bool need_x = cfg.inverse_first && cfg.n_out > 0;
   const size_t per_job = n * (need_x ? 2 : 1);
   const size_t max_words = (size_t)256 << 20;
@@ -3299,12 +3419,21 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   // A recording can fail for reasons outside the prove (another thread's
   // legacy-stream work invalidates a capture): the bucket is then marked
   // broken and the same prove runs again eagerly.
+  static const bool stagger = getenv("LOOMGPU_STAGGER") && atoi(getenv("LOOMGPU_STAGGER")) != 0;
+  static const bool shared_pool = getenv("LOOMGPU_POOL") && std::string(getenv("LOOMGPU_POOL")) == "shared";
+  LaneGate gate;
+  LaneGate* gp = (stagger && lanes > 1) ? &gate : nullptr;
+  if (gp) {
+    gate.ev.resize(lanes);
+    gate.done.assign(lanes, 0);
+    for (int l = 0; l < lanes; ++l) gate.ev[l] = ctx_->lane_event(l);
+  }
   auto run_job = [&](const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& pcb,
                      int lane, GraphBucket* gb, int mode) {
     if (mode == Job::CAPTURE || mode == Job::REPLAY) {
       // a failed recording or replay breaks the bucket; the prove runs again eagerly
       try {
-        Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode);
+        Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode, gp);
         job.run();
         ++(mode == Job::CAPTURE ? graph_records : graph_replays);
         return;
@@ -3314,7 +3443,7 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
         ++graph_fallbacks;
       }
     }
-    Job job(*this, in, out, pcb, lane);
+    Job job(*this, in, out, pcb, lane, nullptr, Job::EAGER, gp);
     job.run();
   };
   if (lanes <= 1) {
@@ -3338,7 +3467,7 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   std::vector<std::exception_ptr> err(lanes);
   auto run_lane = [&](int l) {
     try {
-      current_pool() = &ThreadPool::for_lane(l, lanes);
+      current_pool() = shared_pool ? &ThreadPool::get() : &ThreadPool::for_lane(l, lanes);
       int mode = 0;
       BucketHold hold{this, take_bucket((int)part[l].size(), l, inputs[0].device, &mode)};
       run_job(part[l], out[l], l == 0 ? cb : PhaseCallback(), l, hold.b, mode);  // phases: lane 0's

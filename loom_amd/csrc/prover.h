@@ -37,11 +37,20 @@ class Tables {
   // (4 <= logm <= 14) and b < m/16, the 15 words of ntt16_ladder(w_m^b)
   // (w_m^-b when inverse) at 16 * (m/16 - 1 + b), padded to 16.
   const uint32_t* ntt_ladders(bool inverse);
+  // The geometric scale tables (lde_pre, quot_pre: t[k] = sc s^k): their
+  // ratio s, by device address; false for any other table.
+  bool geometric(const uint32_t* t, uint32_t* s);
+  // blk[p] = s^(2^b brev_a(p)), p < 2^a: the block factors of a geometric
+  // scale table in the fused four-step LDE (run_ntt).
+  const uint32_t* geo_blocks(uint32_t s, int a, int b);
 
  private:
   const uint32_t* get(const std::string& key, const std::function<std::vector<uint32_t>()>& make);
+  const uint32_t* geo(const std::string& key, uint32_t s, size_t n, uint32_t sc);
   std::mutex mu_;
   std::map<std::string, DevBuf> bufs_;
+  std::mutex geo_mu_;
+  std::map<const uint32_t*, uint32_t> geo_;
   hipStream_t stream_ = nullptr;
 };
 
@@ -159,6 +168,8 @@ struct Context {
   int lanes = 2;
   bool lanes_forced = false;  // LOOMGPU_LANES set: overrides the tuned lane count
   hipStream_t lane_stream(int lane);
+  // an event of the lane (no timing), for ordering lanes' device phases
+  hipEvent_t lane_event(int lane);
   // a non-blocking stream for one-off copies beside the lanes' work
   hipStream_t side_stream();
   // Pinned host buffers per lane, reused across proves.
@@ -175,6 +186,7 @@ struct Context {
  private:
   struct LaneBufs {
     hipStream_t stream = nullptr;  // owned for lanes >= 1
+    hipEvent_t event = nullptr;
     uint32_t* pinned = nullptr;
     size_t pinned_cap = 0;
     uint32_t* pinned_in = nullptr;

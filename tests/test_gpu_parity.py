@@ -63,18 +63,21 @@ def canon(a):
     return F.from_mont(np.asarray(a, dtype=np.uint32))
 
 
-@pytest.mark.parametrize("logn", [1, 4, 10, 14, 15, 16, 17])
+# 2^15 .. 2^21 take the fused four-step LDE: every row size of its middle
+# pass (2^8 .. 2^11) and both factor splits (a = b and a = b + 1)
+@pytest.mark.parametrize("logn", [1, 4, 10, 14, 15, 16, 17, 18, 19, 20, 21])
 def test_rs_encode(torch, ctx, logn):
     from loom_amd import _lib
     from oracle import ntt
     log("rs_encode logn=%d" % logn)
     n = 1 << logn
+    ncol = 3 if logn <= 17 else 1
     rng = np.random.default_rng(logn)
-    cols = rng.integers(0, P, size=(3, n), dtype=np.uint64)
+    cols = rng.integers(0, P, size=(ncol, n), dtype=np.uint64)
     din = dev_u32(torch, mont(cols).reshape(-1))
-    dout = torch.zeros(3 * 4 * n, dtype=torch.int32, device="cuda")
-    _lib.check(_lib.lib().lg_rs_encode(ctx.h, ptr(din), n, 3, ptr(dout)), "lg_rs_encode")
-    got = canon(host_u32(dout)).reshape(3, 4 * n)
+    dout = torch.zeros(ncol * 4 * n, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().lg_rs_encode(ctx.h, ptr(din), n, ncol, ptr(dout)), "lg_rs_encode")
+    got = canon(host_u32(dout)).reshape(ncol, 4 * n)
     want = ntt.encode(cols)
     assert np.array_equal(got, want)
 
