@@ -490,14 +490,21 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ 
   const BigJob* job = jobs + blockIdx.y;
   const uint32_t n1 = 1u << a, n2 = 1u << b;
   const uint32_t tpc = n1 >> 4;  // threads per row
-  const uint32_t tt = threadIdx.x % tpc, c = threadIdx.x / tpc;
+  // Bit-reversed output: contiguous rows (a thread's 16 outputs are
+  // contiguous).  Natural output: the 8 rows interleaved, row index fastest
+  // across threads, so that each store instruction writes 32-byte runs (8
+  // adjacent k2) instead of one dword per lane.
+  const bool il = !bitrev_out;
+  const uint32_t tt = il ? threadIdx.x >> 3 : threadIdx.x % tpc;
+  const uint32_t c = il ? threadIdx.x & (BIG_C - 1) : threadIdx.x / tpc;
   const uint32_t k2 = xcd_group(blockIdx.x, gridDim.x) * BIG_C + c;
   const gmem<const uint32_t> __restrict__ tmp = gbl(job->tmp);
   uint32_t y[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) y[k] = gat(tmp, k2 * n1 + tt + (uint32_t)k * tpc);
   const NttRoots R = ntt16_roots(tw_b, a);
-  col_dif(y, sh + c * n1, 1, a, tt, tw_b, R);
+  if (il) col_dif(y, sh + c, BIG_C, a, tt, tw_b, R);
+  else col_dif(y, sh + c * n1, 1, a, tt, tw_b, R);
   const gmem<uint32_t> __restrict__ out = gbl(job->out);
   const gmem<const uint32_t> __restrict__ post = gbl(job->post);
   const uint32_t k2r = brev(k2, b);

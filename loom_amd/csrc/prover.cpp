@@ -309,13 +309,6 @@ hipStream_t Context::side_stream() {
   return side_;
 }
 
-hipEvent_t Context::lane_event(int lane) {
-  std::lock_guard<std::mutex> lock(lane_mu_);
-  LaneBufs& L = lane_.at(lane);
-  if (!L.event) LG_HIP(hipEventCreateWithFlags(&L.event, hipEventDisableTiming));
-  return L.event;
-}
-
 hipStream_t Context::lane_stream(int lane) {
   if (lane == 0) return stream;
   std::lock_guard<std::mutex> lock(lane_mu_);
@@ -378,7 +371,6 @@ Context::~Context() {
       (void)hipStreamSynchronize(L.stream);
       (void)hipStreamDestroy(L.stream);
     }
-    if (L.event) (void)hipEventDestroy(L.event);
   }
 }
 
@@ -773,32 +765,6 @@ struct Layer {  // an E6 vector of len elements per instance, 6 limb arrays
 
 struct FsMsgBuilder;
 
-// Orders the device phases of a batch's lanes: lane l's device phase starts
-// on the GPU once lane l-1's has ended, so that lane l-1's host work (query
-// chains, gathers, proof assembly) runs while lane l's kernels do.  A lane
-// signals after it has enqueued its device phase (the event then marks its
-// end on the GPU) or when it fails before that (no event: waiting on it is a
-// no-op).
-struct LaneGate {
-  std::mutex mu;
-  std::condition_variable cv;
-  std::vector<hipEvent_t> ev;  // per lane, recorded after its device phase
-  std::vector<char> done;      // per lane: signalled
-  void signal(int lane, hipStream_t s, bool record) {
-    if (record) LG_HIP(hipEventRecord(ev[lane], s));
-    std::lock_guard<std::mutex> lk(mu);
-    done[lane] = 1;
-    cv.notify_all();
-  }
-  void wait(int lane, hipStream_t s) {
-    if (lane == 0) return;
-    std::unique_lock<std::mutex> lk(mu);
-    cv.wait(lk, [&] { return done[lane - 1] != 0; });
-    lk.unlock();
-    LG_HIP(hipStreamWaitEvent(s, ev[lane - 1], 0));
-  }
-};
-
 class Job {
  public:
   // With a graph bucket the Job's device buffers, staging area and pinned
@@ -809,9 +775,9 @@ class Job {
   // graph.
   enum Mode { EAGER, CAPTURE, REPLAY };
   Job(Plan& plan, const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& cb,
-      int lane, GraphBucket* gb = nullptr, Mode mode = EAGER, LaneGate* gate = nullptr)
+      int lane, GraphBucket* gb = nullptr, Mode mode = EAGER)
       : plan_(plan), d_(plan.data()), ctx_(*plan.ctx()), tb_(ctx_.tables), in_(in), out_(out), cb_(cb),
-        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER), gate_(gate),
+        B_((int)in.size()), lane_(lane), s_(ctx_.lane_stream(lane)), gb_(gb), mode_(gb ? mode : EAGER),
         tune_(plan.tuning(plan.tuning_batch())), skip_fri_((plan.options() & PROVE_SKIP_FRI) != 0) {
     if (out.size() != in.size()) throw std::runtime_error("prove: output slots do not match the inputs");
     if (gb_) {
@@ -832,7 +798,6 @@ class Job {
     ctx_.timer.launch_point(s_);
   }
   ~Job() {
-    if (gate_ && !gate_signaled_) gate_->signal(lane_, s_, false);  // failed before its device phase ended
     if (in_region_) {  // an error inside the device phase: leave capture / suppression
       set_launch_suppression(false);
       if (mode_ == CAPTURE) {
@@ -950,7 +915,6 @@ class Job {
   // plan, the batch size and the bucket's addresses).
   void region_begin() {
     stg().flush(s_);
-    if (gate_) gate_->wait(lane_, s_);  // outside any capture: the previous lane's device phase first
     in_region_ = true;
     if (mode_ == CAPTURE) LG_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
     if (mode_ == REPLAY) {
@@ -981,10 +945,6 @@ class Job {
     }
     in_region_ = false;
     if (mode_ == CAPTURE) LG_HIP(hipGraphLaunch(gb_->exec, s_));
-    if (gate_) {
-      gate_->signal(lane_, s_, true);
-      gate_signaled_ = true;
-    }
   }
   template <class T>
   const T* stage(const std::vector<T>& v) { return stg().put(v, s_); }
@@ -1076,8 +1036,6 @@ class Job {
   hipStream_t s_;
   std::unique_ptr<Staging> stg_;
   GraphBucket* gb_;
-  LaneGate* gate_ = nullptr;
-  bool gate_signaled_ = false;
   Mode mode_;
   Tuning tune_;  // the plan's launch choices for this prove's batch
   const bool skip_fri_;  // prover.SkipFRI: claimed values, no opening
@@ -3419,21 +3377,12 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   // A recording can fail for reasons outside the prove (another thread's
   // legacy-stream work invalidates a capture): the bucket is then marked
   // broken and the same prove runs again eagerly.
-  static const bool stagger = getenv("LOOMGPU_STAGGER") && atoi(getenv("LOOMGPU_STAGGER")) != 0;
-  static const bool shared_pool = getenv("LOOMGPU_POOL") && std::string(getenv("LOOMGPU_POOL")) == "shared";
-  LaneGate gate;
-  LaneGate* gp = (stagger && lanes > 1) ? &gate : nullptr;
-  if (gp) {
-    gate.ev.resize(lanes);
-    gate.done.assign(lanes, 0);
-    for (int l = 0; l < lanes; ++l) gate.ev[l] = ctx_->lane_event(l);
-  }
   auto run_job = [&](const std::vector<InstanceInput>& in, std::vector<ProveOutput>& out, const PhaseCallback& pcb,
                      int lane, GraphBucket* gb, int mode) {
     if (mode == Job::CAPTURE || mode == Job::REPLAY) {
       // a failed recording or replay breaks the bucket; the prove runs again eagerly
       try {
-        Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode, gp);
+        Job job(*this, in, out, pcb, lane, gb, (Job::Mode)mode);
         job.run();
         ++(mode == Job::CAPTURE ? graph_records : graph_replays);
         return;
@@ -3443,7 +3392,7 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
         ++graph_fallbacks;
       }
     }
-    Job job(*this, in, out, pcb, lane, nullptr, Job::EAGER, gp);
+    Job job(*this, in, out, pcb, lane);
     job.run();
   };
   if (lanes <= 1) {
@@ -3467,7 +3416,7 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
   std::vector<std::exception_ptr> err(lanes);
   auto run_lane = [&](int l) {
     try {
-      current_pool() = shared_pool ? &ThreadPool::get() : &ThreadPool::for_lane(l, lanes);
+      current_pool() = &ThreadPool::for_lane(l, lanes);
       int mode = 0;
       BucketHold hold{this, take_bucket((int)part[l].size(), l, inputs[0].device, &mode)};
       run_job(part[l], out[l], l == 0 ? cb : PhaseCallback(), l, hold.b, mode);  // phases: lane 0's

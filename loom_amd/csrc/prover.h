@@ -168,8 +168,6 @@ struct Context {
   int lanes = 2;
   bool lanes_forced = false;  // LOOMGPU_LANES set: overrides the tuned lane count
   hipStream_t lane_stream(int lane);
-  // an event of the lane (no timing), for ordering lanes' device phases
-  hipEvent_t lane_event(int lane);
   // a non-blocking stream for one-off copies beside the lanes' work
   hipStream_t side_stream();
   // Pinned host buffers per lane, reused across proves.
@@ -186,7 +184,6 @@ struct Context {
  private:
   struct LaneBufs {
     hipStream_t stream = nullptr;  // owned for lanes >= 1
-    hipEvent_t event = nullptr;
     uint32_t* pinned = nullptr;
     size_t pinned_cap = 0;
     uint32_t* pinned_in = nullptr;
