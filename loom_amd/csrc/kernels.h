@@ -159,9 +159,10 @@ void launch_grand_product(const uint32_t* const* num, const uint32_t* const* den
 // board/logup.go:138-186), batch on grid y, per-instance pointer arrays.
 // (The range table column itself is a generated column: launch_gencol.)
 void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s);
-// counts[b][v] += 1 for every canonical value v < bound of src[b][0..n), err[b] += 1 for the others
-void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
-                      int batch, hipStream_t s);
+// counts[b][v] += 1 for every canonical value v < bound of src[b][0..n), *err[b] += 1 for the others
+// (several batch entries may share a count table and an error word: atomic additions)
+void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound,
+                      uint32_t* const* err, int batch, hipStream_t s);
 void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s);
 // acc = (first ? 0 : acc) + m / e (E6 columns, limb stride n)
 void launch_logup_acc(const uint32_t* const* e, const uint32_t* const* m, uint32_t* const* acc, int n, int first,

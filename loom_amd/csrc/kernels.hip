@@ -1341,15 +1341,15 @@ void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s
 }
 
 // Counts the canonical values of src[b][0..n) (Montgomery words) below bound
-// into counts[b]; values at or above bound count into err[b].
+// into counts[b]; values at or above bound count into *err[b].
 __global__ void k_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound,
-                            uint32_t* err) {
+                            uint32_t* const* err) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const int b = blockIdx.y;
   const uint32_t v = kb_from_mont(src[b][j]);
   if (v < bound) atomicAdd(&counts[b][v], 1u);
-  else atomicAdd(&err[b], 1u);
+  else atomicAdd(err[b], 1u);
 }
 
 // Tables up to HIST_LDS_BINS entries: each workgroup counts HIST_ROWS rows into
@@ -1359,7 +1359,7 @@ constexpr int HIST_LDS_BINS = 4096;
 constexpr int HIST_ROWS = 4096;
 
 __global__ __launch_bounds__(256) void k_histogram_lds(const uint32_t* const* src, int n, uint32_t* const* counts,
-                                                       uint32_t bound, uint32_t* err) {
+                                                       uint32_t bound, uint32_t* const* err) {
   __shared__ uint32_t h[HIST_LDS_BINS];
   __shared__ uint32_t e;
   const int b = blockIdx.y;
@@ -1378,11 +1378,11 @@ __global__ __launch_bounds__(256) void k_histogram_lds(const uint32_t* const* sr
   uint32_t* c = counts[b];
   for (uint32_t i = threadIdx.x; i < bound; i += blockDim.x)
     if (h[i]) atomicAdd(&c[i], h[i]);
-  if (threadIdx.x == 0 && e) atomicAdd(&err[b], e);
+  if (threadIdx.x == 0 && e) atomicAdd(err[b], e);
 }
 
-void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound, uint32_t* err,
-                      int batch, hipStream_t s) {
+void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound,
+                      uint32_t* const* err, int batch, hipStream_t s) {
   if (n <= 0 || batch <= 0) return;
   if (!prelaunch()) return;
   if (bound <= (uint32_t)HIST_LDS_BINS) {

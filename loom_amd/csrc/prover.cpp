@@ -989,10 +989,16 @@ class Job {
   void public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_dev);
   void gencol();
   void run_steps(size_t r);
-  Layer eval_step_dag(size_t r, size_t s, size_t k, uint32_t mi);
-  void step_grand_product(size_t r, size_t s);
+  // One step DAG of round r: DAG k of step s, on module mi.
+  struct StepDag {
+    size_t s, k;
+    uint32_t mi;
+  };
+  Layer eval_step_dags(size_t r, const std::vector<StepDag>& items);
+  bool same_dag(size_t r, const StepDag& a, const StepDag& b) const;
+  void step_grand_products(size_t r, const std::vector<size_t>& steps);
   void step_multiplicity(size_t r, size_t s);
-  void step_logup(size_t r, size_t s);
+  void step_logups(size_t r, const std::vector<size_t>& steps);
   void finish_round_steps(size_t r);
   void flush_err_flags();
   // Round and __zeta challenges on the device (k_fs_challenge); the host
@@ -1056,7 +1062,7 @@ class Job {
   std::vector<DevBuf> keep_;
   std::vector<Transcript> ts_;
   std::vector<std::vector<E6>> totals_;  // [b][col] exposed logup totals (host, after the replay)
-  std::vector<std::pair<uint32_t, DevBuf>> totals_dev_;  // (total column, B x 6 words) of the current round
+  std::vector<std::pair<uint32_t, const uint32_t*>> totals_dev_;  // (total column, B x 6 words) of the current round
   // Per-instance error flags (B words each) raised by kernels of the current
   // round, with the message an instance with a set flag fails with; copied
   // back after the round's commit (finish_round_steps).
@@ -1733,66 +1739,132 @@ void Job::commit_tree(size_t t) {
 
 // Evaluates input DAG k of step (r, s) on the rows of module mi (Lagrange
 // values), into an E6 layer.
-Layer Job::eval_step_dag(size_t r, size_t s, size_t k, uint32_t mi) {
-  const DagKernel& K = *plan_.step_kernel(r, s, k);
-  const ModuleInfo& m = d_.modules[mi];
+// Step DAGs of one launch: items that share the kernel code, the module size
+// and the slot counts, with (item, instance) pairs as the launch batch; the
+// output Layer holds item i's instance b at batch index i B + b.
+Layer Job::eval_step_dags(size_t r, const std::vector<StepDag>& it) {
+  const DagKernel& K0 = *plan_.step_kernel(r, it[0].s, it[0].k);
+  const ModuleInfo& m = d_.modules[it[0].mi];
+  const size_t NI = it.size(), nchal = K0.chal_ids.size();
   Layer out;
   out.len = m.N;
-  out.buf = dev((size_t)B_ * 6 * m.N);
+  out.buf = dev(NI * B_ * 6 * m.N);
   std::vector<const uint32_t*> cp;
-  for (int b = 0; b < B_; ++b)
-    for (auto c : K.col_ids) cp.push_back(cols_[c].lag(b, 0));
-  std::vector<uint32_t*> op(B_);
-  for (int b = 0; b < B_; ++b) op[b] = out.ptr(b);
+  std::vector<uint32_t*> op;
+  uint32_t* chal = nullptr;
+  if (nchal) {
+    keep_.push_back(dev(NI * B_ * nchal * 6));
+    chal = keep_.back().as<uint32_t>();
+  }
+  for (size_t i = 0; i < NI; ++i) {
+    const DagKernel& K = *plan_.step_kernel(r, it[i].s, it[i].k);
+    for (int b = 0; b < B_; ++b) {
+      for (auto c : K.col_ids) cp.push_back(cols_[c].lag(b, 0));
+      op.push_back(out.ptr((int)(i * B_ + b)));
+    }
+    if (nchal)  // challenge operands of item i: [b][slot][6]
+      launch_chal_compact(chal_dev_.as<uint32_t>(), (int)d_.cols.size(), stage(K.chal_ids), (int)nchal,
+                          chal + i * B_ * nchal * 6, B_, s_);
+  }
   DagLaunch a{};
   a.cols = stage(cp);
-  a.chal = chal_operands(K.chal_ids);
+  a.chal = chal;
   a.rowscale = nullptr;
   a.out = const_cast<uint32_t* const*>(stage(op));
   a.col_stride = m.N;
   a.out_stride = m.N;
   a.logn = m.log_n;
   a.ncos = 1;
-  a.batch = B_;
-  dag_kernel_launch(K, a, s_);
+  a.batch = (int)(NI * B_);
+  dag_kernel_launch(K0, a, s_);
   return out;
 }
 
+bool Job::same_dag(size_t r, const StepDag& a, const StepDag& b) const {
+  const DagKernel& A = *plan_.step_kernel(r, a.s, a.k);
+  const DagKernel& B = *plan_.step_kernel(r, b.s, b.k);
+  return d_.modules[a.mi].N == d_.modules[b.mi].N && A.code == B.code && A.col_ids.size() == B.col_ids.size() &&
+         A.chal_ids.size() == B.chal_ids.size();
+}
+
+// The steps of a round are independent (each writes its own columns); steps
+// of one kind whose modules have one size and whose DAGs share their code
+// run as one class: each phase is one launch for the whole class, the
+// (step, instance) pairs its batch.
 void Job::run_steps(size_t r) {
   const Round& rd = d_.rounds[r];
+  std::vector<std::vector<size_t>> gp, lu;
+  auto place = [&](std::vector<std::vector<size_t>>& cls, size_t s, size_t ndag) {
+    for (auto& c : cls) {
+      bool ok = true;
+      for (size_t k = 0; ok && k < ndag; ++k)
+        ok = same_dag(r, {c[0], k, rd.steps[c[0]].module}, {s, k, rd.steps[s].module});
+      if (ok) {
+        c.push_back(s);
+        return;
+      }
+    }
+    cls.push_back({s});
+  };
   for (size_t s = 0; s < rd.steps.size(); ++s) {
-    switch (rd.steps[s].kind) {
-      case STEP_GRAND_PRODUCT: step_grand_product(r, s); break;
+    const Step& st = rd.steps[s];
+    switch (st.kind) {
+      case STEP_GRAND_PRODUCT: place(gp, s, 2); break;
       case STEP_LOOKUP_MULTIPLICITY: step_multiplicity(r, s); break;
-      case STEP_CYCLIC_LOGUP: step_logup(r, s); break;
+      case STEP_CYCLIC_LOGUP: {
+        bool placed = false;
+        for (auto& c : lu)
+          if (rd.steps[c[0]].params[0] == st.params[0]) {
+            bool ok = true;
+            for (size_t k = 0; ok && k < 2 * (size_t)st.params[0]; ++k)
+              ok = same_dag(r, {c[0], k, rd.steps[c[0]].module}, {s, k, st.module});
+            if (ok) {
+              c.push_back(s);
+              placed = true;
+              break;
+            }
+          }
+        if (!placed) lu.push_back({s});
+        break;
+      }
       default: throw std::runtime_error("unsupported prover step");
     }
   }
+  for (auto& c : gp) step_grand_products(r, c);
+  for (auto& c : lu) step_logups(r, c);
 }
 
-// board.GrandProductStep (board/step.go:301-327).
-void Job::step_grand_product(size_t r, size_t s) {
-  const Step& st = d_.rounds[r].steps[s];
-  const uint32_t n = d_.modules[st.module].N;
-  Layer nd[2] = {eval_step_dag(r, s, 0, st.module), eval_step_dag(r, s, 1, st.module)};
-  std::vector<const uint32_t*> np(B_), dp(B_);
-  std::vector<uint32_t*> gp(B_), rp(B_);
+// board.GrandProductStep (board/step.go:301-327) for a class of steps.
+void Job::step_grand_products(size_t r, const std::vector<size_t>& steps) {
+  const size_t NI = steps.size();
+  const uint32_t n = d_.modules[d_.rounds[r].steps[steps[0]].module].N;
+  std::vector<StepDag> num, den;
+  for (size_t s : steps) {
+    num.push_back({s, 0, d_.rounds[r].steps[s].module});
+    den.push_back({s, 1, d_.rounds[r].steps[s].module});
+  }
+  Layer nd[2] = {eval_step_dags(r, num), eval_step_dags(r, den)};
+  const int NB = (int)(NI * B_);
+  std::vector<const uint32_t*> np(NB), dp(NB);
+  std::vector<uint32_t*> gpp(NB), rp(NB);
   Layer ratio;
   ratio.len = n;
-  ratio.buf = dev((size_t)B_ * 6 * n);
-  for (int b = 0; b < B_; ++b) {
-    np[b] = nd[0].ptr(b);
-    dp[b] = nd[1].ptr(b);
-    gp[b] = cols_[st.out_col].lag(b, 0);
-    rp[b] = ratio.ptr(b);
-  }
+  ratio.buf = dev((size_t)NB * 6 * n);
+  for (size_t i = 0; i < NI; ++i)
+    for (int b = 0; b < B_; ++b) {
+      const int x = (int)(i * B_ + b);
+      np[x] = nd[0].ptr(x);
+      dp[x] = nd[1].ptr(x);
+      gpp[x] = cols_[d_.rounds[r].steps[steps[i]].out_col].lag(b, 0);
+      rp[x] = ratio.ptr(x);
+    }
   uint32_t* scan_tmp = nullptr;
-  if (scan_tmp_words((int)n, B_)) {
-    keep_.push_back(dev(scan_tmp_words((int)n, B_)));
+  if (scan_tmp_words((int)n, NB)) {
+    keep_.push_back(dev(scan_tmp_words((int)n, NB)));
     scan_tmp = keep_.back().as<uint32_t>();
   }
   launch_grand_product(stage(np), stage(dp), const_cast<uint32_t* const*>(stage(rp)),
-                       const_cast<uint32_t* const*>(stage(gp)), n, B_, scan_tmp, s_);
+                       const_cast<uint32_t* const*>(stage(gpp)), n, NB, scan_tmp, s_);
   keep_.push_back(std::move(ratio.buf));
   keep_.push_back(std::move(nd[0].buf));
   keep_.push_back(std::move(nd[1].buf));
@@ -1802,6 +1874,8 @@ void Job::step_grand_product(size_t r, size_t s) {
 // the target row holding v is row v, so the multiplicity column is the
 // histogram of the source values; a value outside the table is an error
 // (poly/lookup_multiplicity.go:90-96), reported after the round's commit.
+// Sources of one size and DAG code are evaluated and counted in one launch
+// each (the counts are atomic additions, so their order does not matter).
 void Job::step_multiplicity(size_t r, size_t s) {
   const Step& st = d_.rounds[r].steps[s];
   const uint32_t ns = st.params[0];
@@ -1816,55 +1890,85 @@ void Job::step_multiplicity(size_t r, size_t s) {
   uint32_t* const* cnt_d = const_cast<uint32_t* const*>(stage(cnt));
   launch_fill(cnt_d, (int)mc.n, 0u, B_, s_);
   launch_fill(const_cast<uint32_t* const*>(stage(errp)), 1, 0u, B_, s_);
+  std::vector<std::vector<StepDag>> cls;
   for (uint32_t j = 0; j < ns; ++j) {
-    const uint32_t mj = st.params[5 + j];
-    Layer src = eval_step_dag(r, s, j, mj);
-    std::vector<const uint32_t*> sp(B_);
-    for (int b = 0; b < B_; ++b) sp[b] = src.ptr(b);  // limb 0: the base value
-    launch_histogram(stage(sp), (int)d_.modules[mj].N, cnt_d, bound, err.as<uint32_t>(), B_, s_);
+    const StepDag it{s, j, st.params[5 + j]};
+    bool placed = false;
+    for (auto& c : cls)
+      if (same_dag(r, c[0], it)) {
+        c.push_back(it);
+        placed = true;
+        break;
+      }
+    if (!placed) cls.push_back({it});
+  }
+  for (auto& c : cls) {
+    Layer src = eval_step_dags(r, c);
+    const int NB = (int)(c.size() * B_);
+    std::vector<const uint32_t*> sp(NB);
+    std::vector<uint32_t*> cp(NB), ep(NB);
+    for (int x = 0; x < NB; ++x) {  // (source, instance) pair x counts into instance x % B's table and flag
+      sp[x] = src.ptr(x);  // limb 0: the base value
+      cp[x] = cnt[x % B_];
+      ep[x] = errp[x % B_];
+    }
+    launch_histogram(stage(sp), (int)d_.modules[c[0].mi].N, const_cast<uint32_t* const*>(stage(cp)), bound,
+                     const_cast<uint32_t* const*>(stage(ep)), NB, s_);
     keep_.push_back(std::move(src.buf));
   }
   launch_counts_to_field(cnt_d, (int)mc.n, B_, s_);
   pending_err_.push_back({"lookup: a source value is not in the range table", std::move(err)});
 }
 
-// board.CyclicLogUpStep (board/logup.go:138-186): L_i = sum_{r<=i} v_r - (i+1) T/N
-// with v = sum_j M_j / E_j; the totals T come back with the round's roots.
-void Job::step_logup(size_t r, size_t s) {
-  const Step& st = d_.rounds[r].steps[s];
-  const ModuleInfo& m = d_.modules[st.module];
-  const uint32_t nt = st.params[0];
+// board.CyclicLogUpStep (board/logup.go:138-186) for a class of steps:
+// L_i = sum_{r<=i} v_r - (i+1) T/N with v = sum_j M_j / E_j; the totals T
+// come back with the round's roots.
+void Job::step_logups(size_t r, const std::vector<size_t>& steps) {
+  const size_t NI = steps.size();
+  const Step& st0 = d_.rounds[r].steps[steps[0]];
+  const ModuleInfo& m = d_.modules[st0.module];
+  const uint32_t nt = st0.params[0];
+  const int NB = (int)(NI * B_);
   Layer acc;
   acc.len = m.N;
-  acc.buf = dev((size_t)B_ * 6 * m.N);
-  std::vector<uint32_t*> ap(B_), lp(B_);
-  for (int b = 0; b < B_; ++b) {
-    ap[b] = acc.ptr(b);
-    lp[b] = cols_[st.outs[0]].lag(b, 0);
-  }
+  acc.buf = dev((size_t)NB * 6 * m.N);
+  std::vector<uint32_t*> ap(NB), lp(NB);
+  for (size_t i = 0; i < NI; ++i)
+    for (int b = 0; b < B_; ++b) {
+      const int x = (int)(i * B_ + b);
+      ap[x] = acc.ptr(x);
+      lp[x] = cols_[d_.rounds[r].steps[steps[i]].outs[0]].lag(b, 0);
+    }
   uint32_t* const* ap_d = const_cast<uint32_t* const*>(stage(ap));
   for (uint32_t j = 0; j < nt; ++j) {
-    Layer e = eval_step_dag(r, s, 2 * j, st.module);
-    Layer mm = eval_step_dag(r, s, 2 * j + 1, st.module);
-    std::vector<const uint32_t*> ep(B_), mp(B_);
-    for (int b = 0; b < B_; ++b) {
-      ep[b] = e.ptr(b);
-      mp[b] = mm.ptr(b);
+    std::vector<StepDag> ei, mi;
+    for (size_t s : steps) {
+      ei.push_back({s, 2 * (size_t)j, d_.rounds[r].steps[s].module});
+      mi.push_back({s, 2 * (size_t)j + 1, d_.rounds[r].steps[s].module});
     }
-    launch_logup_acc(stage(ep), stage(mp), ap_d, (int)m.N, j == 0 ? 1 : 0, B_, s_);
+    Layer e = eval_step_dags(r, ei);
+    Layer mm = eval_step_dags(r, mi);
+    std::vector<const uint32_t*> ep(NB), mp(NB);
+    for (int x = 0; x < NB; ++x) {
+      ep[x] = e.ptr(x);
+      mp[x] = mm.ptr(x);
+    }
+    launch_logup_acc(stage(ep), stage(mp), ap_d, (int)m.N, j == 0 ? 1 : 0, NB, s_);
     keep_.push_back(std::move(e.buf));
     keep_.push_back(std::move(mm.buf));
   }
-  DevBuf tot = dev((size_t)B_ * 6);
+  DevBuf tot = dev((size_t)NB * 6);
   uint32_t* scan_tmp = nullptr;
-  if (scan_tmp_words((int)m.N, B_)) {
-    keep_.push_back(dev(scan_tmp_words((int)m.N, B_)));
+  if (scan_tmp_words((int)m.N, NB)) {
+    keep_.push_back(dev(scan_tmp_words((int)m.N, NB)));
     scan_tmp = keep_.back().as<uint32_t>();
   }
   launch_cyclic_logup_finish(ap_d, const_cast<uint32_t* const*>(stage(lp)), tot.as<uint32_t>(), (int)m.N,
-                             kb_inv(M(m.N)), B_, scan_tmp, s_);
+                             kb_inv(M(m.N)), NB, scan_tmp, s_);
   keep_.push_back(std::move(acc.buf));
-  totals_dev_.push_back({st.outs[1], std::move(tot)});
+  for (size_t i = 0; i < NI; ++i)  // step i's totals: [b][6] at i B 6
+    totals_dev_.push_back({d_.rounds[r].steps[steps[i]].outs[1], tot.as<uint32_t>() + i * B_ * 6});
+  keep_.push_back(std::move(tot));
 }
 
 // After round r's commitment: lookup errors, then the logup totals of the
@@ -1884,13 +1988,10 @@ void Job::finish_round_steps(size_t r) {
   for (auto& kv : totals_dev_) {
     const uint32_t col = kv.first;
     const uint32_t ninv = kb_inv(M(d_.modules[d_.cols[col].module].N));
-    launch_scale_into(kv.second.as<uint32_t>(), ninv, chal_dev_.as<uint32_t>(), (int)d_.cols.size(), (int)col, B_,
-                      s_);
-    copy_async(fs_h_ + fs_tot_off_ + (size_t)col * B_ * 6, kv.second.as<void>(), (size_t)B_ * 24,
-               hipMemcpyDeviceToHost);
-    total_ptr_[col] = kv.second.as<uint32_t>();
+    launch_scale_into(kv.second, ninv, chal_dev_.as<uint32_t>(), (int)d_.cols.size(), (int)col, B_, s_);
+    copy_async(fs_h_ + fs_tot_off_ + (size_t)col * B_ * 6, kv.second, (size_t)B_ * 24, hipMemcpyDeviceToHost);
+    total_ptr_[col] = kv.second;
     total_cols_.push_back(col);
-    keep_.push_back(std::move(kv.second));
   }
   totals_dev_.clear();
   const Round& rd = d_.rounds[r];

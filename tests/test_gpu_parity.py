@@ -265,6 +265,18 @@ def test_prove_range_lookups(ctx, log_ns, nq):
     assert np.array_equal(words, enc)
 
 
+def test_prove_range_lookups_repeated_sizes(ctx):
+    """Modules of one size run their prover steps as one class (one launch per
+    phase for the grand products, the multiplicity sources and the logups)."""
+    from loom_amd import workloads, prover
+    log("prove range lookups, repeated sizes (7, 5, 7, 7, 5)")
+    prog, tr = workloads.uneven_range_instance([7, 5, 7, 7, 5], 9)
+    enc, tw = _oracle_words(prog, tr, 12)
+    words, trans = prover.prove(ctx, prog, tr, 12, want_transcript=True)
+    assert np.array_equal(trans, tw), "transcript digests differ"
+    assert np.array_equal(words, enc)
+
+
 def test_prove_range_lookups_batched(ctx):
     from loom_amd import workloads, prover
     log("prove range lookups batched 3 x (6, 8)")
