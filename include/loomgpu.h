@@ -29,6 +29,7 @@
  *   - WithHashBackend: Poseidon2 only (hash id "poseidon2").
  *   - WithPhaseCallback: lg_plan_last_phases after the call.
  *   - SkipFRI: lg_plan_set_options(plan, LG_PROVE_SKIP_FRI).
+ *   - WithFriOption(fri.WithGrinding(bits)): lg_plan_set_grinding.
  *   - setup.ProvingKey: lg_plan_set_setup commits the setup columns once per
  *     plan and keeps them resident (base-field setup columns).
  *   - public.Inputs: base-field entries at rows the plan lists; each
@@ -37,8 +38,8 @@
  *     extension-field setup columns or public-input entries,
  *     WithTranscript and WithNewTranscriptHasher (the
  *     transcript always starts fresh with the Poseidon2 sponge),
- *     WithFriOption (no proof-of-work grinding: the PoW list is empty), other
- *     hash backends, lookups other than width-1 selector-free range lookups.
+ *     WithFriOption other than grinding, other hash backends, lookups other
+ *     than width-1 selector-free range lookups.
  *
  * Plan word stream: see loom_amd/plan.py (module sizes, column table, rounds
  * and prover steps, commitment trees with opening shifts, folded constraint
@@ -63,7 +64,10 @@
  *     #roots, roots (8 words each; one per folded layer)
  *     1 (the final polynomial is extension-field), #final, E6 per value
  *     #queries, then per query: #layers { opening }
- *     #proofs of work (0: no grinding)
+ *     #proofs of work (0 without grinding), then per fold challenge in
+ *       ascending byte order of its name ("fri_fold_0", "fri_fold_1",
+ *       "fri_fold_10", ...): name (byte length, then bytes packed 4 per
+ *       word), NbBits, Salt (Montgomery)
  *   samplings (PointSamplings): #queries { #trees { row(lo), row(hi), path,
  *     #injected groups { row(lo), row(hi) } } }
  *   opening := 1 (extension-field layer), row index, P (E6), Q (E6), path:
@@ -174,6 +178,12 @@ int lg_plan_autotune(lg_plan* plan, int batch, const uint32_t* const* columns, c
  * Other flags are refused. */
 #define LG_PROVE_SKIP_FRI 1u
 int lg_plan_set_options(lg_plan* plan, uint32_t flags);
+/* prover.WithFriOption(fri.WithGrinding(bits)) (prover/prover.go:62-68,
+ * internal/fri/fri.go:71-78, 1053-1090): every FRI fold challenge of later
+ * proves (and the challenges lg_verify replays) carries a proof of work of
+ * `bits` bits (0: off, the default; at most 31).  The salt search runs on the
+ * device; the proof lists one (name, bits, salt) per fold challenge. */
+int lg_plan_set_grinding(lg_plan* plan, int bits);
 /* Sets the launch choices for proves of `batch` instances (5 ints, as above). */
 int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning);
 

@@ -28,6 +28,7 @@ EXPORTS = {
     "lg_plan_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "lg_plan_set_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
+    "lg_plan_set_grinding": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_plan_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "lg_ctx_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_plan_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),

@@ -272,6 +272,15 @@ int lg_plan_set_options(lg_plan* plan, uint32_t flags) {
   });
 }
 
+int lg_plan_set_grinding(lg_plan* plan, int bits) {
+  return guarded([&] {
+    if (!plan || bits < 0 || bits > 31) throw std::runtime_error("lg_plan_set_grinding: bad arguments");
+    std::lock_guard<std::mutex> lock(plan->ctx->mu);
+    plan->p->set_grinding((uint32_t)bits);
+    return 0;
+  });
+}
+
 int lg_plan_set_tuning(lg_plan* plan, int batch, const int32_t* tuning) {
   return guarded([&] {
     if (!plan || batch <= 0 || !tuning) throw std::runtime_error("lg_plan_set_tuning: bad arguments");

@@ -296,6 +296,17 @@ struct FsSeg {
 // seg_dev: nseg segments in device memory; len: their total length.
 void launch_fs_challenge(const FsSeg* seg_dev, int nseg, int len, uint32_t* dig, uint32_t* e6, int e6_stride,
                          int batch, hipStream_t s);
+// "FSPW": the tag before a proof of work's bit count and salt
+// (internal/fiat-shamir/transcript.go:36)
+constexpr uint32_t TAG_FSPW_WORD = 0x46535057u;
+// Proof-of-work salt search of one challenge per instance: the message is
+// the segments followed by [FSPW, bits, salt]; found[b] (set to ~0 by the
+// caller) receives the smallest salt below p whose digest has `bits` zero
+// low bits (~0 if none), and trailer[3 b ..] the message's last three words
+// for that salt (Montgomery).  `workgroups` workgroups of 256 salts each
+// search one instance.
+void launch_fs_grind(const FsSeg* seg_dev, int nseg, int len, uint32_t bits, uint32_t* found, uint32_t* trailer,
+                     int workgroups, int batch, hipStream_t s);
 // out[b][k] = chal[b][ids[k]] for the nch E6 challenge operands of a DAG.
 void launch_chal_compact(const uint32_t* chal, int ncols, const uint32_t* ids, int nch, uint32_t* out, int batch,
                          hipStream_t s);

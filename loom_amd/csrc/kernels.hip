@@ -1981,6 +1981,74 @@ void launch_fs_challenge(const FsSeg* seg_dev, int nseg, int len, uint32_t* dig,
   LG_HIP(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ proof of work
+// The salt search of one transcript challenge per instance (grindChallenge,
+// transcript.go:317-332): the message is the segments (len words) followed by
+// [FSPW, bits, salt]; found[b] ends as the smallest salt whose digest has
+// `bits` zero low bits (hasZeroGrindingBits, :349-365, one 31-bit element for
+// bits <= 31), or keeps ~0 if no salt below p has.  Workgroup w of instance b
+// tries the 256-salt blocks w, w + W, w + 2W, ... in increasing order, one
+// salt per thread, and stops once its block starts above found[b]: every
+// salt below the final found[b] has then been tried by its workgroup.  The
+// sponge state after the message's full blocks is the same for every salt;
+// each trial overwrites the final block (the salt last) and permutes once.
+__global__ __launch_bounds__(256) void k_fs_grind(const FsSeg* __restrict__ seg, int nseg, int len, uint32_t bits,
+                                                  uint32_t* found) {
+  const int b = blockIdx.y;
+  const uint64_t step = (uint64_t)gridDim.x * 256u;
+  const int total = len + 3, nfull = (total - 1) / 16, f0 = 16 * nfull, nlast = total - f0;
+  const uint32_t tagw = kb_to_mont(TAG_FSPW_WORD), bitsw = kb_to_mont(bits);
+  FsCursor cur;
+  auto word = [&](int i) -> uint32_t { return i < len ? cur.word(seg, nseg, b, i) : (i == len ? tagw : bitsw); };
+  uint32_t pre[24];
+#pragma unroll
+  for (int i = 0; i < 24; ++i) pre[i] = 0u;
+  for (int blk = 0; blk < nfull; ++blk) {
+#pragma unroll
+    for (int i = 0; i < 16; ++i) pre[i] = word(16 * blk + i);
+    p2_permute<24>(pre);
+  }
+  // the final block's words before the salt (lanes nlast - 1 .. 15 keep the state)
+#pragma unroll
+  for (int i = 0; i < 15; ++i)
+    if (i < nlast - 1) pre[i] = word(f0 + i);
+  const uint32_t mask = bits >= 31 ? 0x7fffffffu : ((1u << bits) - 1u);
+  for (uint64_t base = (uint64_t)blockIdx.x * 256u; base < KB_P; base += step) {
+    if (base > *(volatile uint32_t*)&found[b]) break;
+    const uint32_t salt = (uint32_t)base + threadIdx.x;
+    if (salt >= KB_P) continue;
+    const uint32_t sw = kb_to_mont(salt);
+    uint32_t st[24];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) st[i] = (i == nlast - 1) ? sw : pre[i];
+    p2_permute<24>(st);
+    if ((kb_from_mont(st[0]) & mask) == 0u) atomicMin(&found[b], salt);
+  }
+}
+
+// The trailer [FSPW, bits, salt] of each instance's message (Montgomery).
+__global__ void k_grind_trailer(const uint32_t* __restrict__ found, uint32_t bits, uint32_t* __restrict__ tr,
+                                int batch) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= batch) return;
+  const uint32_t salt = found[b];
+  tr[3 * b] = kb_to_mont(TAG_FSPW_WORD);
+  tr[3 * b + 1] = kb_to_mont(bits);
+  tr[3 * b + 2] = kb_to_mont(salt < KB_P ? salt : 0u);
+}
+
+void launch_fs_grind(const FsSeg* seg_dev, int nseg, int len, uint32_t bits, uint32_t* found, uint32_t* trailer,
+                     int workgroups, int batch, hipStream_t s) {
+  if (nseg < 1 || len < 1 || bits < 1 || bits > 31 || workgroups < 1 || batch < 1)
+    throw std::runtime_error("proof of work: bad arguments");
+  if (!prelaunch()) return;
+  hipLaunchKernelGGL(k_fs_grind, dim3(workgroups, batch), dim3(256), 0, s, seg_dev, nseg, len, bits, found);
+  LG_HIP(hipGetLastError());
+  if (!prelaunch()) return;
+  hipLaunchKernelGGL(k_grind_trailer, dim3((batch + 63) / 64), dim3(64), 0, s, found, bits, trailer, batch);
+  LG_HIP(hipGetLastError());
+}
+
 // out[b][k] = chal[b][ids[k]] (E6 slots): the challenge operands of one DAG.
 __global__ void k_chal_compact(const uint32_t* __restrict__ chal, int ncols, const uint32_t* __restrict__ ids, int nch,
                                uint32_t* __restrict__ out) {

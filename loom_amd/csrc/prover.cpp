@@ -448,6 +448,12 @@ Tuning Plan::tuning(int batch) const {
   return it == tuned_.end() ? Tuning() : it->second;
 }
 
+void Plan::set_grinding(uint32_t bits) {
+  if (bits > 31) throw std::runtime_error("grinding: at most 31 bits (one KoalaBear salt)");
+  grinding_ = bits;
+  drop_graphs();  // recorded graphs hold the fold-challenge launches
+}
+
 void Plan::set_tuning(int batch, const Tuning& t) {
   tuned_[batch] = t;
   drop_graphs();  // recorded graphs hold the previous launch choices
@@ -821,6 +827,7 @@ class Job {
     for (auto& x : keep_) ctx_.pool.put(std::move(x));
     ctx_.pool.put(std::move(in_arena_));
     ctx_.pool.put(std::move(pre_fold_.dev));
+    ctx_.pool.put(std::move(fri_salts_));
     if (stg_) ctx_.give_staging(std::move(stg_));
   }
 
@@ -1025,6 +1032,11 @@ class Job {
     uint32_t *dig, *alpha, *roots;
   };
   bool tree_levels(TreeBuf& tb, uint32_t* const* tp_d, const RootChallenge* chal = nullptr);
+  // A challenge with a proof of work (fri.WithGrinding): the device salt
+  // search over mb || [FSPW, bits, salt], then the digest (dig[b], and its
+  // first six words at e6[6 b] when e6 is given); salts[b] gets the salt.
+  void grind_challenge(FsMsgBuilder mb, uint32_t* dig, uint32_t* e6, uint32_t* salts);
+
   // AIR quotients of modules of one class (same size, chunk count and
   // relation kernel code), proved together as (module, instance) pairs.
   void quotient(const std::vector<size_t>& mods);
@@ -1033,6 +1045,8 @@ class Job {
   Plan& plan_;
   const PlanData& d_;
   Context& ctx_;
+  const uint32_t grind_bits_ = plan_.grinding();
+  DevBuf fri_salts_;  // [fold round][b] salts of the fold challenges (with grinding)
   Tables& tb_;
   const std::vector<InstanceInput>& in_;
   std::vector<ProveOutput>& out_;
@@ -2018,6 +2032,19 @@ struct FsMsgBuilder {
   }
 };
 
+void Job::grind_challenge(FsMsgBuilder mb, uint32_t* dig, uint32_t* e6, uint32_t* salts) {
+  std::vector<uint32_t*> sp(B_);
+  for (int b = 0; b < B_; ++b) sp[b] = salts + b;
+  launch_fill(const_cast<uint32_t* const*>(stage(sp)), 1, 0xFFFFFFFFu, B_, s_);
+  // about one sweep of 256-salt blocks for the expected 2^bits trials
+  const int wg = (int)std::min<uint64_t>(256, std::max<uint64_t>(1, (1ull << grind_bits_) / 256));
+  keep_.push_back(dev((size_t)B_ * 3));
+  uint32_t* tr = keep_.back().as<uint32_t>();
+  launch_fs_grind(stage(mb.seg), (int)mb.seg.size(), mb.len, grind_bits_, salts, tr, wg, B_, s_);
+  mb.add(tr, 3, 3);
+  launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, dig, e6, 6, B_, s_);
+}
+
 // The first challenge's bindings: the common words, then each instance's
 // public-input words.
 void Job::init_segments(FsMsgBuilder& mb) {
@@ -2391,6 +2418,7 @@ void Job::open() {
   const size_t nt = d_.trees.size();
   const size_t nm = d_.modules.size();
   const int nq = (int)d_.num_queries;
+  if (grind_bits_ && !skip_fri_) fri_salts_ = dev((size_t)FRI_ROUNDS * B_);
   // ---- claimed values; zeta is the device digest of __zeta
   const uint32_t* zdig = fs_dig_.as<uint32_t>() + d_.rounds.size() * (size_t)B_ * 8;
   // Lagrange weights L_i(zeta) depend on zeta and the size only: one vector
@@ -2646,7 +2674,8 @@ void Job::open() {
           mb.add(prev, 8, 8);
           mb.add(z, 8, 0);
           uint32_t* out = pre_fold_.dev.as<uint32_t>() + (size_t)j * B_ * 8;
-          launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, out, nullptr, 0, B_, s_);
+          if (grind_bits_) grind_challenge(mb, out, nullptr, fri_salts_.as<uint32_t>() + (size_t)j * B_);
+          else launch_fs_challenge(stage(mb.seg), (int)mb.seg.size(), mb.len, out, nullptr, 0, B_, s_);
           prev = out;
         }
         pre_fold_.prev = prev;
@@ -2843,6 +2872,7 @@ void Job::open() {
   // With one DEEP class the host replays the claimed values, alpha_DEEP and
   // the fold rounds before the level after the FRI commit phase (below):
   // until then nothing the device does waits for the host.
+  const uint32_t* salts_h = nullptr;  // [fold round][b] proof-of-work salts, once the device phase is done
   auto replay_alpha = [&] {
     replay_device_fs();
     const uint32_t* cv_h = alpha_h + (size_t)B_ * 8;
@@ -2854,6 +2884,7 @@ void Job::open() {
       static const uint32_t zero[8] = {0};
       for (int j = 0; j < pre_fold_.n; ++j) {
         ts_[b].bind_at(fold_pos_ + j, zero, 8);
+        if (grind_bits_) ts_[b].set_pow(fold_pos_ + j, grind_bits_, salts_h[(size_t)j * B_ + b]);
         ts_[b].check_at(fold_pos_ + j, pre_h + ((size_t)j * B_ + b) * 8);
       }
     });
@@ -2874,6 +2905,7 @@ void Job::open() {
         const Digest& dg = ts_[b].compute(nmg);
         for (int k = 0; k < 6; ++k) gamma[l][b].v[k] = dg.v[k];
       }
+      if (grind_bits_) return;  // the fold challenges carry proofs of work: found on the device
       for (int j = 0; j < first; ++j) {
         const std::string nf = "fri_fold_" + std::to_string(j);
         uint32_t zero[8] = {0};
@@ -2948,6 +2980,17 @@ void Job::open() {
   const uint32_t* prev_dig = nullptr;
   const uint32_t* alpha_first = nullptr;
   const std::string first_name = "fri_fold_" + std::to_string(first);
+  // a fold challenge with a proof of work: the message prefix, the previous
+  // digest and the bound root (node 1 of each instance's node array)
+  auto grind_fold = [&](int j, const uint32_t* prev, const TreeBuf& tb) {
+    const std::vector<uint32_t>& pf = plan_.schedule()->prefix[fold_pos_ + j];
+    FsMsgBuilder mb;
+    mb.add(stage(pf), (uint32_t)pf.size(), 0);
+    mb.add(prev, 8, 8);
+    mb.add(tb.node_ptr(0) + 8, 8, (uint32_t)(16 * tb.nleaves));
+    grind_challenge(mb, chal_dig.as<uint32_t>() + (size_t)j * fb, chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6,
+                    fri_salts_.as<uint32_t>() + (size_t)j * B_);
+  };
   if (!first_on_host) {
     // the previous digest is the device's (alpha_DEEP or the last zero-bound
     // fold round, chained above)
@@ -2955,11 +2998,39 @@ void Job::open() {
     for (int b = 0; b < B_; ++b) tp[b] = levels_tree.node_ptr(b);
     uint32_t* dig_f = chal_dig.as<uint32_t>() + (size_t)first * fb;
     uint32_t* al_f = chal_alpha.as<uint32_t>() + (size_t)first * B_ * 6;
-    const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, first_name);
-    launch_fri_challenge(pre.data(), (int)pre.size(), pre_fold_.prev, const_cast<uint32_t* const*>(stage(tp)), dig_f,
-                         al_f, root_dev.as<uint32_t>() + (size_t)first * fb, B_, s_);
+    if (grind_bits_) {
+      launch_copy_roots(const_cast<uint32_t* const*>(stage(tp)), root_dev.as<uint32_t>() + (size_t)first * fb, B_, s_);
+      grind_fold(first, pre_fold_.prev, levels_tree);
+    } else {
+      const std::vector<uint32_t> pre = string_to_elements(TAG_FSID, first_name);
+      launch_fri_challenge(pre.data(), (int)pre.size(), pre_fold_.prev, const_cast<uint32_t* const*>(stage(tp)),
+                           dig_f, al_f, root_dev.as<uint32_t>() + (size_t)first * fb, B_, s_);
+    }
     prev_dig = dig_f;
     alpha_first = al_f;
+  } else if (grind_bits_) {
+    // the zero-bound rounds and round `first` chain from the last level
+    // gamma's digest (the host's), each with its proof of work
+    std::vector<uint32_t> pv((size_t)B_ * 8);
+    for (int b = 0; b < B_; ++b) {
+      const Digest& d = ts_[b].previous_value("fri_fold_0");
+      std::copy(d.v, d.v + 8, pv.begin() + (size_t)8 * b);
+    }
+    const uint32_t* prev = stage(pv);
+    static const std::vector<uint32_t> zero8(8, 0u);
+    const uint32_t* z = stage(zero8);
+    for (int j = 0; j < first; ++j) {
+      const std::vector<uint32_t>& pf = plan_.schedule()->prefix[fold_pos_ + j];
+      FsMsgBuilder mb;
+      mb.add(stage(pf), (uint32_t)pf.size(), 0);
+      mb.add(prev, 8, 8);
+      mb.add(z, 8, 0);
+      grind_challenge(mb, chal_dig.as<uint32_t>() + (size_t)j * fb, nullptr, fri_salts_.as<uint32_t>() + (size_t)j * B_);
+      prev = chal_dig.as<uint32_t>() + (size_t)j * fb;
+    }
+    grind_fold(first, prev, levels_tree);
+    prev_dig = chal_dig.as<uint32_t>() + (size_t)first * fb;
+    alpha_first = chal_alpha.as<uint32_t>() + (size_t)first * B_ * 6;
   } else {
     std::vector<uint32_t> dg((size_t)B_ * 8), al((size_t)B_ * 6);
     parallel_for(B_, [&](int b) {
@@ -2991,9 +3062,14 @@ void Job::open() {
       uint32_t* al_j = chal_alpha.as<uint32_t>() + (size_t)j * B_ * 6;
       const RootChallenge rc{string_to_elements(TAG_FSID, "fri_fold_" + std::to_string(j)), prev_dig, dig_j, al_j,
                              root_dev.as<uint32_t>() + (size_t)j * fb};
-      if (!tree_levels(tb, tree_d[j], &rc))
+      if (grind_bits_) {
+        tree_levels(tb, tree_d[j]);
+        launch_copy_roots(tree_d[j], rc.roots, B_, s_);
+        grind_fold(j, prev_dig, tb);
+      } else if (!tree_levels(tb, tree_d[j], &rc)) {
         launch_fri_challenge(rc.prefix.data(), (int)rc.prefix.size(), prev_dig, tree_d[j], dig_j, al_j, rc.roots, B_,
                              s_);
+      }
       prev_dig = dig_j;
       alpha_j = al_j;
     }
@@ -3015,12 +3091,17 @@ void Job::open() {
   // final polynomial (running after the last round): 4 E6 values
   const uint32_t flen = running.len;
   const size_t fw_words = (size_t)B_ * 6 * flen;
-  uint32_t* fri_h = pin((2 * (size_t)FRI_ROUNDS * fb + fw_words) * 4);  // digests, roots, final polynomial
+  // digests, roots, final polynomial, proof-of-work salts
+  const size_t salt_words = grind_bits_ ? (size_t)FRI_ROUNDS * B_ : 0;
+  uint32_t* fri_h = pin((2 * (size_t)FRI_ROUNDS * fb + fw_words + salt_words) * 4);
   copy_async(fri_h, chal_dig.as<void>(), (size_t)FRI_ROUNDS * fb * 4, hipMemcpyDeviceToHost);
   copy_async(fri_h + FRI_ROUNDS * fb, root_dev.as<void>(), (size_t)FRI_ROUNDS * fb * 4, hipMemcpyDeviceToHost);
   copy_async(fri_h + 2 * FRI_ROUNDS * fb, running.buf.as<void>(), fw_words * 4, hipMemcpyDeviceToHost);
+  if (salt_words)
+    copy_async(fri_h + 2 * FRI_ROUNDS * fb + fw_words, fri_salts_.as<void>(), salt_words * 4, hipMemcpyDeviceToHost);
   region_end();  // the device phase of the prove ends here
   sync();
+  if (grind_bits_) salts_h = fri_h + 2 * FRI_ROUNDS * fb + fw_words;
   if (deep_on_device) replay_alpha();
   std::vector<uint32_t> final_w(fri_h + 2 * FRI_ROUNDS * fb, fri_h + 2 * FRI_ROUNDS * fb + fw_words);
   {
@@ -3242,14 +3323,33 @@ void Job::open() {
   // challenge itself, checking the device's digests (each query task does
   // its instance before the query chain).
   const size_t fold0 = ts_.empty() ? 0 : ts_[0].index(first_name);  // fri_fold_j at fold0 + j - first
+  // with grinding, fold round j's message ends with its proof of work (the
+  // device's salt); the transcript checks the digest's zero bits
+  auto pow_at = [&](int b, int j, size_t p) {
+    if (grind_bits_) ts_[b].set_pow(p, grind_bits_, salts_h[(size_t)j * B_ + b]);
+  };
+  std::vector<int> fold_name_order(FRI_ROUNDS);
+  for (int j = 0; j < FRI_ROUNDS; ++j) fold_name_order[j] = j;
+  std::sort(fold_name_order.begin(), fold_name_order.end(),
+            [](int a, int b) { return "fri_fold_" + std::to_string(a) < "fri_fold_" + std::to_string(b); });
   auto replay_fri = [&](int b) {
-    if (!first_on_host) {
+    if (!first_on_host || grind_bits_) {
+      if (first_on_host) {  // the zero-bound rounds were found on the device too
+        static const uint32_t zero[8] = {0};
+        for (int j = 0; j < first; ++j) {
+          ts_[b].bind_at(fold_pos_ + j, zero, 8);
+          pow_at(b, j, fold_pos_ + j);
+          ts_[b].check_at(fold_pos_ + j, &fri_dig[(size_t)j * fb + (size_t)8 * b]);
+        }
+      }
       ts_[b].bind_at(fold0, levels_root[b].v, 8);
+      pow_at(b, first, fold0);
       ts_[b].check_at(fold0, &fri_dig[(size_t)first * fb + (size_t)8 * b]);
     }
     for (int j = first + 1; j < FRI_ROUNDS; ++j) {
       const size_t p = fold0 + (size_t)(j - first);
       ts_[b].bind_at(p, fri_roots[j - first - 1][b].v, 8);
+      pow_at(b, j, p);
       ts_[b].check_at(p, &fri_dig[(size_t)j * fb + (size_t)8 * b]);
     }
   };
@@ -3354,7 +3454,24 @@ void Job::open() {
         w.push_back(0);  // layer r0's injection leaves are dropped (fri.go:719); others have none
       }
     }
-    w.push_back(0);  // proofs of work (no grinding)
+    // proofs of work (fri.Proof.PoW, recordFoldProofsOfWork, fri.go:1075-1090):
+    // one per fold challenge, in ascending byte order of the names
+    if (grind_bits_) {
+      w.push_back((uint32_t)FRI_ROUNDS);
+      for (int j : fold_name_order) {
+        const std::string nm = "fri_fold_" + std::to_string(j);
+        w.push_back((uint32_t)nm.size());
+        for (size_t i = 0; i < nm.size(); i += 4) {
+          uint32_t v = 0;
+          for (size_t k = 0; k < 4 && i + k < nm.size(); ++k) v |= (uint32_t)(uint8_t)nm[i + k] << (8 * k);
+          w.push_back(v);
+        }
+        w.push_back(grind_bits_);
+        w.push_back(kb_to_mont(salts_h[(size_t)j * B_ + b]));
+      }
+    } else {
+      w.push_back(0);
+    }
     w.push_back((uint32_t)nq);
     for (int q = 0; q < nq; ++q) {
       const uint32_t* rec = rec0 + (size_t)q * stride;

@@ -307,6 +307,7 @@ class Plan {
   std::map<std::tuple<int, int, bool>, std::unique_ptr<GraphBucket>> buckets_;  // (batch, lane, device inputs)
   uint64_t tick_ = 0;
   uint32_t options_ = 0;
+  uint32_t grinding_ = 0;
   GraphBucket* take_bucket(int batch, int lane, bool device_inputs, int* mode);
   void give_bucket(GraphBucket* b);
 
@@ -319,6 +320,10 @@ class Plan {
   // Prove options (PROVE_* flags), for the proves that follow.
   uint32_t options() const { return options_; }
   void set_options(uint32_t flags) { options_ = flags; }
+  // Proof-of-work bits of every FRI fold challenge (fri.WithGrinding,
+  // internal/fri/fri.go:71-78); 0: off, the default.
+  uint32_t grinding() const { return grinding_; }
+  void set_grinding(uint32_t bits);
   // Proves of this plan that recorded a graph, replayed one, or ran again
   // eagerly after a failed recording or replay.
   std::atomic<uint64_t> graph_records{0}, graph_replays{0}, graph_fallbacks{0};

@@ -16,6 +16,7 @@
 namespace lg {
 
 constexpr uint32_t TAG_FSID = 0x46534944u;
+constexpr uint32_t TAG_FSPW = 0x46535057u;  // "FSPW": before a proof of work's bit count and salt
 
 // hash.StringToElements (internal/hash/hash.go:36-53), Montgomery form.
 inline std::vector<uint32_t> string_to_elements(uint32_t tag, const std::string& s) {
@@ -58,6 +59,7 @@ class Transcript {
     bindings_.resize(n);
     values_.resize(n);
     computed_.assign(n, false);
+    pow_.assign(n, {0u, 0u});
   }
   void new_challenge(const std::string& name) {
     if (fixed_) throw std::runtime_error("transcript: the challenge schedule is fixed: " + name);
@@ -65,6 +67,24 @@ class Transcript {
     bindings_.emplace_back();
     values_.emplace_back();
     computed_.push_back(false);
+    pow_.push_back({0u, 0u});
+  }
+  // A proof of work for the challenge at p (ComputeChallenge WithGrinding,
+  // transcript.go:216-332): its message ends with [FSPW, bits, salt] (salt
+  // canonical, below p) and its digest must have `bits` zero low bits
+  // (hasZeroGrindingBits, :349-365); computing it otherwise is an error.
+  void set_pow(size_t p, uint32_t bits, uint32_t salt) {
+    if (p >= computed_.size() || computed_[p]) throw std::runtime_error("transcript: bad proof-of-work position");
+    if (bits < 1 || bits > 31 || salt >= KB_P) throw std::runtime_error("transcript: invalid proof of work");
+    pow_[p] = {bits, salt};
+  }
+  static bool zero_grinding_bits(const uint32_t* digest, uint32_t bits) {
+    for (int i = 0; i < 8 && bits > 0; ++i) {
+      const uint32_t k = bits < 31 ? bits : 31;
+      if (kb_from_mont(digest[i]) & ((1u << k) - 1u)) return false;
+      bits -= k;
+    }
+    return true;
   }
   void bind(const std::string& name, const uint32_t* e, size_t n) {
     size_t p = at(name);
@@ -81,7 +101,12 @@ class Transcript {
       msg.insert(msg.end(), values_[p - 1].v, values_[p - 1].v + 8);
     }
     msg.insert(msg.end(), bindings_[p].begin(), bindings_[p].end());
+    if (pow_[p].first) {
+      const uint32_t t[3] = {kb_to_mont(TAG_FSPW), kb_to_mont(pow_[p].first), kb_to_mont(pow_[p].second)};
+      msg.insert(msg.end(), t, t + 3);
+    }
     hash(msg.data(), msg.size(), values_[p].v);
+    check_pow(p);
     computed_[p] = true;
     log_.push_back(p);
     return values_[p];
@@ -108,7 +133,7 @@ class Transcript {
     if (p != 0 && !computed_[p - 1])
       throw std::runtime_error("transcript: previous challenge not computed: " + sch().names[p]);
     uint32_t small[64];
-    const size_t len = prefix.size() + (p != 0 ? 8 : 0) + bindings_[p].size();
+    const size_t len = prefix.size() + (p != 0 ? 8 : 0) + bindings_[p].size() + (pow_[p].first ? 3 : 0);
     std::vector<uint32_t> big;
     uint32_t* msg = small;
     if (len > 64) {
@@ -120,7 +145,13 @@ class Transcript {
     if (p != 0)
       for (int i = 0; i < 8; ++i) msg[o++] = values_[p - 1].v[i];
     for (uint32_t x : bindings_[p]) msg[o++] = x;
+    if (pow_[p].first) {
+      msg[o++] = kb_to_mont(TAG_FSPW);
+      msg[o++] = kb_to_mont(pow_[p].first);
+      msg[o++] = kb_to_mont(pow_[p].second);
+    }
     hash(msg, len, values_[p].v);
+    check_pow(p);
     computed_[p] = true;
     log_.push_back(p);
     return values_[p];
@@ -175,6 +206,10 @@ class Transcript {
     if (hasher_) hasher_(msg, len, out, hasher_user_);
     else p2_sponge_host(msg, len, out);
   }
+  void check_pow(size_t p) const {
+    if (pow_[p].first && !zero_grinding_bits(values_[p].v, pow_[p].first))
+      throw std::runtime_error("transcript: invalid proof of work for " + sch().names[p]);
+  }
   Hasher hasher_ = nullptr;
   void* hasher_user_ = nullptr;
   size_t at(const std::string& name) const {
@@ -188,6 +223,7 @@ class Transcript {
   std::vector<std::vector<uint32_t>> bindings_;
   std::vector<Digest> values_;
   std::vector<bool> computed_;
+  std::vector<std::pair<uint32_t, uint32_t>> pow_;  // (bits, salt) per position, bits 0: none
   std::vector<size_t> log_;
 };
 

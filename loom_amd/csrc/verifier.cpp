@@ -149,6 +149,9 @@ struct Fail {
 void need(bool c, int status, const char* m) {
   if (!c) fail(status, m);
 }
+void need(bool c, int status, const std::string& m) {
+  if (!c) fail(status, m);
+}
 
 struct RowRef {
   uint32_t base = 0, nb = 0, ext = 0, ne = 0;  // word offsets in the proof
@@ -177,6 +180,7 @@ struct Decoded {
   uint32_t final_off = 0, n_final = 0;
   std::vector<std::vector<Opening>> layers;  // [query][layer]
   std::vector<std::vector<Sampling>> samp;   // [query][tree]
+  std::vector<std::pair<uint32_t, uint32_t>> pow;  // per fold round: (bits, salt canonical)
 };
 
 class Reader {
@@ -259,6 +263,7 @@ struct Shape {
   std::vector<size_t> air_trees;
   uint32_t nq = 0;
   std::vector<std::vector<uint32_t>> chunk_col;  // per module: its quotient chunk columns
+  uint32_t grinding = 0;                         // proof-of-work bits of the fold challenges
 };
 
 Shape make_shape(const PlanData& d) {
@@ -368,7 +373,25 @@ Decoded decode(const Shape& S, const uint32_t* w, size_t n, bool skip_fri) {
     need(r.count(64, "layers") == n_layers, VERIFY_FORMAT, "query layer count");
     for (uint32_t i = 0; i < n_layers; ++i) D.layers[k].push_back(r.opening());
   }
-  need(r.u() == 0, VERIFY_FORMAT, "proofs of work: none expected (grinding off)");
+  {
+    // fri.Proof.PoW: with grinding one entry per fold challenge, by name
+    const uint32_t npow = r.count(64, "proofs of work");
+    need(npow == (S.grinding ? (uint32_t)FRI_ROUNDS : 0u), VERIFY_FORMAT,
+         S.grinding ? "proofs of work: one per fold challenge" : "proofs of work: none expected (grinding off)");
+    D.pow.assign(FRI_ROUNDS, {0u, 0u});
+    std::vector<char> seen(FRI_ROUNDS, 0);
+    for (uint32_t k = 0; k < npow; ++k) {
+      const std::string nm = r.str();
+      int j = -1;
+      for (int x = 0; x < FRI_ROUNDS; ++x)
+        if (nm == "fri_fold_" + std::to_string(x)) j = x;
+      need(j >= 0 && !seen[j], VERIFY_FORMAT, "proof of work: unknown or repeated challenge " + nm);
+      seen[j] = 1;
+      const uint32_t bits = r.u(), salt = kb_from_mont(r.u());
+      need(bits == S.grinding, VERIFY_FORMAT, "proof of work: bit count of " + nm);
+      D.pow[j] = {bits, salt};
+    }
+  }
   need(r.count(1u << 16, "queries") == S.nq, VERIFY_FORMAT, "sampling count");
   D.samp.resize(S.nq);
   for (uint32_t k = 0; k < S.nq; ++k) {
@@ -476,7 +499,19 @@ void replay(const Shape& S, const Plan& plan, const VerifyInput& in, const uint3
     const std::string nm = "fri_fold_" + std::to_string(j);
     if (j < S.first) ts.bind(nm, zeros, 8);
     else ts.bind(nm, w + (j == S.first ? P.D.levels_root : P.D.fri_roots[j - S.first - 1]), 8);
-    P.R.fold[j] = e6_of(ts.compute(nm));
+    if (S.grinding) {
+      // computeVerifierFoldChallenge (fri.go:1060-1073): the proof's salt, whose
+      // digest must have the zero bits
+      if (P.D.pow[j].second >= KB_P) fail(VERIFY_FORMAT, "proof of work: salt out of range for " + nm);
+      ts.set_pow(ts.index(nm), P.D.pow[j].first, P.D.pow[j].second);
+      try {
+        P.R.fold[j] = e6_of(ts.compute(nm));
+      } catch (const std::runtime_error&) {
+        fail(VERIFY_FRI, "invalid proof of work for " + nm);
+      }
+    } else {
+      P.R.fold[j] = e6_of(ts.compute(nm));
+    }
   }
   std::vector<uint32_t> fp{M(TAG_EXTP), M(P.D.n_final)};
   fp.insert(fp.end(), w + P.D.final_off, w + P.D.final_off + 6 * P.D.n_final);
@@ -779,7 +814,8 @@ void build_jobs(const Shape& S, const ProofCtx& P, uint32_t base, const std::vec
 void verify_batch(Plan& plan, const std::vector<VerifyInput>& in, const uint32_t* setup_roots, size_t n_setup_roots,
                   bool skip_fri, std::vector<VerifyResult>& out) {
   const PlanData& d = plan.data();
-  const Shape S = make_shape(d);
+  Shape S = make_shape(d);
+  S.grinding = plan.grinding();
   out.assign(in.size(), VerifyResult());
   if (in.empty()) return;
   if (n_setup_roots != S.n_setup_trees)

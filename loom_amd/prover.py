@@ -350,6 +350,14 @@ def set_options(ctx, items, skip_fri=False):
     _lib.check(_lib.lib().lg_plan_set_options(ph, 1 if skip_fri else 0), "lg_plan_set_options")
 
 
+def set_grinding(ctx, items, bits):
+    """FRI proof-of-work bits of the plan of these prepared instances
+    (lg_plan_set_grinding; prover.WithFriOption(fri.WithGrinding(bits))):
+    later proves and verifications of the plan use them; 0 turns it off."""
+    ph = ctx.plan_handle(items[0].words, items[0].setup)
+    _lib.check(_lib.lib().lg_plan_set_grinding(ph, int(bits)), "lg_plan_set_grinding")
+
+
 def set_tuning(ctx, items, tuning):
     ph = ctx.plan_handle(items[0].words, items[0].setup)
     v = (ctypes.c_int32 * 5)(*[int(tuning[f]) for f in TUNING_FIELDS])

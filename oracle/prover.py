@@ -125,13 +125,16 @@ def as_e6_rows(v, m):
 
 # ------------------------------------------------------------------ prover
 class Prover:
-    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None):
+    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None, grinding=0):
         """trace: column name -> canonical values (the witness, and the setup
         columns of the proving key); public_inputs: name -> (module,
-        [(index, value), ...]) base-field entries."""
+        [(index, value), ...]) base-field entries; grinding: proof-of-work
+        bits of every FRI fold challenge (fri.WithGrinding, fri.go:71-78;
+        0: off, the default)."""
         self.prog = program
         self.layout = PR.build_layout(program)
         self.nq = num_queries
+        self.grinding = int(grinding)
         self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items()}
         self.pi = dict(public_inputs or {})
         for name, (mod, entries) in self.pi.items():
@@ -152,6 +155,15 @@ class Prover:
     # -- transcript helpers
     def compute(self, name):
         v = self.ts.compute(name)
+        self.log.append((name, list(v)))
+        return v
+
+    def compute_fold(self, name):
+        """computeProverFoldChallenge (fri.go:1053-1058): with grinding the
+        challenge carries a proof of work."""
+        if not self.grinding:
+            return self.compute(name)
+        v = self.ts.compute_grind(name, self.grinding)
         self.log.append((name, list(v)))
         return v
 
@@ -513,7 +525,7 @@ class Prover:
             gammas[l] = np.array(self.compute(nm)[:6], dtype=np.uint64)
         for j in range(first):
             self.ts.bind("fri_fold_%d" % j, [0] * 8)
-            self.compute("fri_fold_%d" % j)
+            self.compute_fold("fri_fold_%d" % j)
         running = levels[0][1].copy()
         layers = {}
         trees = {}
@@ -531,7 +543,7 @@ class Prover:
             trees[j] = tree
             root = tree.root()
             self.ts.bind("fri_fold_%d" % j, root)
-            a = np.array(self.compute("fri_fold_%d" % j)[:6], dtype=np.uint64)
+            a = np.array(self.compute_fold("fri_fold_%d" % j)[:6], dtype=np.uint64)
             if j > first:
                 roots.append(root)
             running = self.fold(running, a, FRI_N >> j)
@@ -560,8 +572,10 @@ class Prover:
                 row = s >> jl
                 lo = row & ~1
                 level_queries[l - 1].append(dict(row=row, p=levels[l][1][lo], q=levels[l][1][lo + 1]))
+        # recordFoldProofsOfWork (fri.go:1075-1090): one per fold challenge
+        pow_ = {"fri_fold_%d" % j: self.ts.pow["fri_fold_%d" % j] for j in range(FRI_ROUNDS)} if self.grinding else {}
         return dict(levels_root=ltree.root(), level_queries=level_queries, roots=roots,
-                    final=final, queries=queries, positions=positions)
+                    final=final, queries=queries, positions=positions, pow=pow_)
 
     @staticmethod
     def fold(layer, alpha, n):
@@ -692,7 +706,14 @@ def encode_proof(pf):
             e6v(lay["p"])
             e6v(lay["q"])
             path(lay["path"])
-    u(0)  # proofs of work (grinding off)
+    # proofs of work (fri.Proof.PoW), sorted by name: name, NbBits, Salt
+    pw = fr.get("pow", {})
+    u(len(pw))
+    for name in sorted(pw, key=lambda x: x.encode()):
+        bits, salt = pw[name]
+        string(name)
+        u(bits)
+        out.append(_m(salt))
     u(len(pf["samplings"]))
     for row in pf["samplings"]:
         u(len(row))
@@ -707,9 +728,9 @@ def encode_proof(pf):
     return np.array(out, dtype=np.uint32)
 
 
-def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None):
+def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None, grinding=0):
     """Returns (proof dict, canonical u32 encoding, transcript log)."""
-    pv = Prover(program, trace, num_queries, public_inputs)
+    pv = Prover(program, trace, num_queries, public_inputs, grinding)
     pf = pv.prove()
     return pf, encode_proof(pf), pv.log
 

@@ -129,7 +129,12 @@ def decode_proof(words):
             layers.append(dict(row=r.u(), p=r.e6(), q=r.e6(), path=r.path()))
         queries.append(layers)
     fr["queries"] = queries
-    check(r.u() == 0, "proofs of work: none expected (grinding off)")
+    pw = {}
+    for _ in range(r.u()):  # name, NbBits, Salt (fri.Proof.PoW, sorted by name)
+        name = r.string()
+        check(name not in pw, "proof of work listed twice")
+        pw[name] = (r.u(), r.f())
+    fr["pow"] = pw
     pf["fri"] = fr
     samplings = []
     for _ in range(r.u()):
@@ -209,12 +214,13 @@ def _e(v):
 
 
 # ------------------------------------------------------------------ verify
-def verify(program, num_queries, words, check_queries=None, setup_roots=(), public_inputs=None):
+def verify(program, num_queries, words, check_queries=None, setup_roots=(), public_inputs=None, grinding=0):
     """Checks proof words against the program; returns the decoded proof.
     check_queries bounds how many queries have their openings checked (all by
     default); the transcript replay always covers every query.  setup_roots:
     the verification key (setup.VerificationKey.Roots); public_inputs: name ->
-    (module, [(index, value), ...])."""
+    (module, [(index, value), ...]); grinding: the FRI parameters' proof-of-work
+    bits (every fold challenge then carries a proof of work, fri.go:1060-1073)."""
     pf = decode_proof(words)
     lay = PR.build_layout(program)
     prog = program
@@ -349,15 +355,31 @@ def verify(program, num_queries, words, check_queries=None, setup_roots=(), publ
         if l == 1:
             ts.bind(nm, fr["levels_root"])
         gammas[l] = _e(ts.compute(nm)[:6])
+    pw = fr["pow"]
+    if grinding:
+        check(sorted(pw) == sorted("fri_fold_%d" % j for j in range(FRI_ROUNDS)), "proofs of work: one per fold")
+    else:
+        check(not pw, "proofs of work: none expected (grinding off)")
+
+    def fold_challenge(name):
+        if not grinding:
+            return ts.compute(name)
+        bits, salt = pw[name]
+        check(bits == grinding, "proof of work: bit count")
+        try:
+            return ts.compute_with_pow(name, bits, salt)
+        except ValueError:
+            check(False, "invalid proof of work for " + name)
+
     for j in range(first):
         ts.bind("fri_fold_%d" % j, [0] * 8)
-        ts.compute("fri_fold_%d" % j)
+        fold_challenge("fri_fold_%d" % j)
     check(len(fr["roots"]) == FRI_ROUNDS - first - 1, "FRI root count")
     alphas = {}
     for j in range(first, FRI_ROUNDS):
         root = fr["levels_root"] if j == first else fr["roots"][j - first - 1]
         ts.bind("fri_fold_%d" % j, root)
-        alphas[j] = _e(ts.compute("fri_fold_%d" % j)[:6])
+        alphas[j] = _e(fold_challenge("fri_fold_%d" % j)[:6])
     final = [_e(v) for v in fr["final"]]
     check(len(final) == FRI_N >> FRI_ROUNDS, "final polynomial length")
     ts.bind("fri_query_0", [EXTP_TAG, len(final)] + [int(x) for v in final for x in v])

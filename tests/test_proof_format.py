@@ -90,7 +90,7 @@ class HeaderDecoder:
         assert self.u() == 1, "final polynomial: extension-field flag"
         fri["final"] = [self.e6() for _ in range(self.u())]
         fri["queries"] = [[self.opening() for _ in range(self.u())] for _ in range(self.u())]
-        fri["pow"] = self.u()
+        fri["pow"] = [(self.bytes_string(), self.u(), self.field()) for _ in range(self.u())]
         pf["fri"] = fri
         samp = []
         for _ in range(self.u()):
@@ -126,7 +126,7 @@ def _check_common(pf, prog, nq):
             assert [len(v) for v in base] == [len(s) for s in g.base_shifts]
             assert [len(v) for v in ext] == [len(s) for s in g.ext_shifts]
     fri = pf["fri"]
-    assert fri["pow"] == 0
+    assert fri["pow"] == []
     assert len(fri["queries"]) == nq and len(pf["samplings"]) == nq
     sizes = sorted({prog.modules[m].N for m in prog.modules}, reverse=True)
     assert len(fri["level_queries"]) == len(sizes) - 1
@@ -182,6 +182,23 @@ def test_decode_single_module_proof():
     assert pf["exposed"] == []
 
 
+def test_proofs_of_work_by_the_header_grammar():
+    """With grinding the list holds one (name, bits, salt) per fold challenge
+    in ascending byte order of the names, and the decoder consumes every word."""
+    from loom_amd import workloads
+    from oracle import prover as OP
+    prog, tr = workloads.plonk_instance(5, 4)
+    pf, enc, _ = OP.prove(prog, tr, 4, grinding=3)
+    d = HeaderDecoder(enc)
+    got = d.proof()["fri"]["pow"]
+    assert d.i == len(d.w)
+    names = [n.decode() for n, _, _ in got]
+    assert names == sorted(("fri_fold_%d" % j for j in range(22)), key=lambda x: x.encode())
+    R_inv = pow(2**32, P - 2, P)
+    for n, bits, salt in got:
+        assert bits == 3 and salt * R_inv % P == pf["fri"]["pow"][n.decode()][1]
+
+
 def test_decoders_agree():
     """The header decoder and the oracle verifier's decoder read the same
     values (two readers of one grammar)."""
@@ -207,7 +224,9 @@ def test_header_states_this_grammar():
                    "#entries, then per entry: Idx, ExtValue() as E6",
                    "#commitments, roots (8 words each; trace rounds then AIR)",
                    "#levels - 1, then per level l >= 1: #queries { opening }",
-                   "#proofs of work (0: no grinding)",
+                   "#proofs of work (0 without grinding), then per fold challenge in ascending byte order of its "
+                   "name (\"fri_fold_0\", \"fri_fold_1\", \"fri_fold_10\", ...): name (byte length, then bytes "
+                   "packed 4 per word), NbBits, Salt (Montgomery)",
                    "opening := 1 (extension-field layer), row index, P (E6), Q (E6), path",
                    "path := leaf index, #siblings, siblings (8 words each, leaf to root), "
                    "#injection leaves, leaves (8 words each)",
