@@ -34,10 +34,12 @@
  *     plan and keeps them resident (base-field setup columns).
  *   - public.Inputs: base-field entries at rows the plan lists; each
  *     instance passes the values after its input columns.
+ *   - WithTranscript: a fresh running transcript is lg_prove (its computed
+ *     challenges come back in `transcripts`); one with computed challenges is
+ *     lg_prove_chained (its last digest).
  *   - Refused (a plan or call that needs them is rejected with an error):
  *     extension-field setup columns or public-input entries,
- *     WithTranscript and WithNewTranscriptHasher (the
- *     transcript always starts fresh with the Poseidon2 sponge),
+ *     WithNewTranscriptHasher (the transcript hasher is the Poseidon2 sponge),
  *     WithFriOption other than grinding, other hash backends, lookups other
  *     than width-1 selector-free range lookups.
  *
@@ -153,6 +155,19 @@ int lg_prove(lg_plan* plan, int batch, const uint32_t* const* columns, const uin
 int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
                     uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
                     uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens);
+
+/* prover.WithTranscript(fs) (prover/prover.go:54-60, 196-205) with a running
+ * transcript whose last challenge is computed: the prove's challenges are
+ * registered after the caller's, so its first challenge's message carries
+ * that challenge's digest after its name, as every later one carries its
+ * predecessor's (internal/fiat-shamir/transcript.go:285-315).  fs_prev:
+ * batch x 8 words (Montgomery), instance b's digest at 8 b.  device_inputs: 0
+ * as lg_prove, 1 as lg_prove_device.  (A fresh running transcript is lg_prove
+ * itself; the challenges a prove computed come back in `transcripts`.) */
+int lg_prove_chained(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                     int device_inputs, const uint32_t* fs_prev, uint32_t* const* proofs, const size_t* proof_caps,
+                     size_t* proof_lens, uint32_t* const* transcripts, const size_t* transcript_caps,
+                     size_t* transcript_lens);
 
 /* Autotuner (no counterpart in the reference).  Launch choices that do not
  * change the proofs -- proving lanes, the Merkle width from which levels run
@@ -319,6 +334,11 @@ int lg_rs_encode(lg_ctx* ctx, const uint32_t* in, size_t n, int n_cols, uint32_t
  * Returns 0 once every proof has its verdict, whatever the verdicts. */
 int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens, const uint32_t* setup_roots,
               size_t n_setup_roots, const uint32_t* const* public_values, int skip_fri, int32_t* status);
+/* verifier.WithTranscript (verifier/verifier.go:60-65, 167-176) for proofs made
+ * with lg_prove_chained: fs_prev as there, n x 8 words. */
+int lg_verify_chained(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens,
+                      const uint32_t* setup_roots, size_t n_setup_roots, const uint32_t* const* public_values,
+                      int skip_fri, const uint32_t* fs_prev, int32_t* status);
 /* The message of proof i's failed check in this plan's last lg_verify ("" when
  * it passed); NUL-terminated into buf, *len = bytes needed. */
 int lg_plan_verify_message(const lg_plan* plan, int i, char* buf, size_t cap, size_t* len);

@@ -29,6 +29,10 @@ EXPORTS = {
                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
     "lg_plan_set_options": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint32]),
     "lg_plan_set_grinding": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
+    "lg_prove_chained": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, u32pp, ctypes.c_int, u32p, u32pp,
+                                        sizep, sizep, u32pp, sizep, sizep]),
+    "lg_verify_chained": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, u32pp, sizep, u32p, ctypes.c_size_t, u32pp,
+                                         ctypes.c_int, u32p, ctypes.POINTER(ctypes.c_int32)]),
     "lg_plan_set_tuning": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_int32)]),
     "lg_ctx_set_graphs": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int]),
     "lg_plan_graph_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_uint64)]),

@@ -148,7 +148,8 @@ int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms) 
 
 static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
                       bool device, uint32_t* const* proofs, const size_t* proof_caps, size_t* proof_lens,
-                      uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens) {
+                      uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens,
+                      const uint32_t* fs_prev = nullptr) {
   return guarded([&] {
     if (!plan || batch <= 0 || !proofs || !proof_caps || !proof_lens) throw std::runtime_error("lg_prove: bad arguments");
     const size_t nc = plan->p->num_inputs(), np = plan->p->num_perms();
@@ -166,6 +167,10 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
       in[b].device = device;
       in[b].proof_dst = proofs[b];
       in[b].proof_cap = proof_caps[b];
+      if (fs_prev) {
+        in[b].chained = true;
+        std::memcpy(in[b].fs_prev, fs_prev + (size_t)8 * b, 32);
+      }
     }
     std::vector<lg::ProveOutput> out;
     {
@@ -220,6 +225,18 @@ int lg_prove_device(lg_plan* plan, int batch, const uint32_t* const* columns, co
                     uint32_t* const* transcripts, const size_t* transcript_caps, size_t* transcript_lens) {
   return prove_impl(plan, batch, columns, perms, true, proofs, proof_caps, proof_lens, transcripts, transcript_caps,
                     transcript_lens);
+}
+
+int lg_prove_chained(lg_plan* plan, int batch, const uint32_t* const* columns, const uint32_t* const* perms,
+                     int device_inputs, const uint32_t* fs_prev, uint32_t* const* proofs, const size_t* proof_caps,
+                     size_t* proof_lens, uint32_t* const* transcripts, const size_t* transcript_caps,
+                     size_t* transcript_lens) {
+  if (!fs_prev) {
+    g_err = "lg_prove_chained: null fs_prev";
+    return -1;
+  }
+  return prove_impl(plan, batch, columns, perms, device_inputs != 0, proofs, proof_caps, proof_lens, transcripts,
+                    transcript_caps, transcript_lens, fs_prev);
 }
 
 static std::vector<lg::InstanceInput> instance_inputs(lg_plan* plan, int batch, const uint32_t* const* columns,
@@ -308,8 +325,9 @@ int lg_ctx_set_graphs(lg_ctx* ctx, int on) {
   });
 }
 
-int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens, const uint32_t* setup_roots,
-              size_t n_setup_roots, const uint32_t* const* public_values, int skip_fri, int32_t* status) {
+static int verify_impl(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens,
+                       const uint32_t* setup_roots, size_t n_setup_roots, const uint32_t* const* public_values,
+                       int skip_fri, const uint32_t* fs_prev, int32_t* status) {
   return guarded([&] {
     if (!plan || n < 0 || (n && (!proofs || !lens || !status)) || (n_setup_roots && !setup_roots))
       throw std::runtime_error("lg_verify: bad arguments");
@@ -320,6 +338,10 @@ int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t*
       in[i].words = proofs[i];
       in[i].len = lens[i];
       for (size_t j = 0; j < npub; ++j) in[i].publics.push_back(public_values[(size_t)i * npub + j]);
+      if (fs_prev) {
+        in[i].chained = true;
+        std::memcpy(in[i].fs_prev, fs_prev + (size_t)8 * i, 32);
+      }
     }
     std::vector<lg::VerifyResult> out;
     std::lock_guard<std::mutex> lock(plan->ctx->mu);
@@ -331,6 +353,21 @@ int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t*
     }
     return 0;
   });
+}
+
+int lg_verify(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens, const uint32_t* setup_roots,
+              size_t n_setup_roots, const uint32_t* const* public_values, int skip_fri, int32_t* status) {
+  return verify_impl(plan, n, proofs, lens, setup_roots, n_setup_roots, public_values, skip_fri, nullptr, status);
+}
+
+int lg_verify_chained(lg_plan* plan, int n, const uint32_t* const* proofs, const size_t* lens,
+                      const uint32_t* setup_roots, size_t n_setup_roots, const uint32_t* const* public_values,
+                      int skip_fri, const uint32_t* fs_prev, int32_t* status) {
+  if (!fs_prev) {
+    g_err = "lg_verify_chained: null fs_prev";
+    return -1;
+  }
+  return verify_impl(plan, n, proofs, lens, setup_roots, n_setup_roots, public_values, skip_fri, fs_prev, status);
 }
 
 int lg_plan_verify_message(const lg_plan* plan, int i, char* buf, size_t cap, size_t* len) {

@@ -1046,6 +1046,7 @@ class Job {
   const PlanData& d_;
   Context& ctx_;
   const uint32_t grind_bits_ = plan_.grinding();
+  const uint32_t* chain_dev_ = nullptr;  // [b][8] digests of the callers' running transcripts
   DevBuf fri_salts_;  // [fold round][b] salts of the fold challenges (with grinding)
   Tables& tb_;
   const std::vector<InstanceInput>& in_;
@@ -2048,6 +2049,7 @@ void Job::grind_challenge(FsMsgBuilder mb, uint32_t* dig, uint32_t* e6, uint32_t
 // The first challenge's bindings: the common words, then each instance's
 // public-input words.
 void Job::init_segments(FsMsgBuilder& mb) {
+  if (chain_dev_) mb.add(chain_dev_, 8, 8);  // a running transcript's last digest precedes the bindings
   mb.add(stage(fs_init_), (uint32_t)fs_init_.size(), 0);
   if (!pi_words_.empty() && !pi_words_[0].empty())
     mb.add(pi_dev_, (uint32_t)pi_words_[0].size(), (uint32_t)pi_words_[0].size());
@@ -2315,6 +2317,18 @@ void Job::run() {
   totals_.assign(B_, std::vector<E6>(d_.cols.size(), e6_zero()));
   ts_.assign(B_, Transcript(plan_.schedule()));
   inst_err_.assign(B_, std::string());
+  if (in_[0].chained) {  // prover.WithTranscript: every instance follows its caller's transcript
+    std::vector<uint32_t> pv((size_t)B_ * 8);
+    for (int b = 0; b < B_; ++b) {
+      if (!in_[b].chained) throw std::runtime_error("prove: running transcripts for some instances of a batch only");
+      ts_[b].chain_from(in_[b].fs_prev);
+      std::copy(in_[b].fs_prev, in_[b].fs_prev + 8, pv.begin() + (size_t)8 * b);
+    }
+    chain_dev_ = stage(pv);
+  } else {
+    for (int b = 0; b < B_; ++b)
+      if (in_[b].chained) throw std::runtime_error("prove: running transcripts for some instances of a batch only");
+  }
   const size_t nr = d_.rounds.size();
   // the first challenge binds the hash backend id and the module sizes
   // (prover/prover.go:214-220)

@@ -210,6 +210,10 @@ struct InstanceInput {
   bool device = false;                   // pointers are device memory of the context's device (else host)
   uint32_t* proof_dst = nullptr;         // optional caller buffer for the proof words
   size_t proof_cap = 0;                  // its capacity in words
+  // prover.WithTranscript with a running transcript: the digest of its last
+  // (computed) challenge, which the prove's first challenge follows
+  bool chained = false;
+  uint32_t fs_prev[8] = {0};
 };
 
 struct ProveOutput {

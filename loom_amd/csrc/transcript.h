@@ -78,6 +78,15 @@ class Transcript {
     if (bits < 1 || bits > 31 || salt >= KB_P) throw std::runtime_error("transcript: invalid proof of work");
     pow_[p] = {bits, salt};
   }
+  // prover.WithTranscript / verifier.WithTranscript with a running transcript
+  // whose last challenge is computed: this transcript's challenges follow it,
+  // so the first one's message carries that digest after its name, as every
+  // later challenge carries its predecessor's (transcript.go:285-315).
+  void chain_from(const uint32_t* digest) {
+    if (!computed_.empty() && computed_[0]) throw std::runtime_error("transcript: chained after a computation");
+    chained_ = true;
+    std::copy(digest, digest + 8, prev0_.v);
+  }
   static bool zero_grinding_bits(const uint32_t* digest, uint32_t bits) {
     for (int i = 0; i < 8 && bits > 0; ++i) {
       const uint32_t k = bits < 31 ? bits : 31;
@@ -99,6 +108,8 @@ class Transcript {
     if (p != 0) {
       if (!computed_[p - 1]) throw std::runtime_error("transcript: previous challenge not computed: " + name);
       msg.insert(msg.end(), values_[p - 1].v, values_[p - 1].v + 8);
+    } else if (chained_) {
+      msg.insert(msg.end(), prev0_.v, prev0_.v + 8);
     }
     msg.insert(msg.end(), bindings_[p].begin(), bindings_[p].end());
     if (pow_[p].first) {
@@ -133,7 +144,7 @@ class Transcript {
     if (p != 0 && !computed_[p - 1])
       throw std::runtime_error("transcript: previous challenge not computed: " + sch().names[p]);
     uint32_t small[64];
-    const size_t len = prefix.size() + (p != 0 ? 8 : 0) + bindings_[p].size() + (pow_[p].first ? 3 : 0);
+    const size_t len = prefix.size() + (p != 0 || chained_ ? 8 : 0) + bindings_[p].size() + (pow_[p].first ? 3 : 0);
     std::vector<uint32_t> big;
     uint32_t* msg = small;
     if (len > 64) {
@@ -144,6 +155,8 @@ class Transcript {
     for (uint32_t x : prefix) msg[o++] = x;
     if (p != 0)
       for (int i = 0; i < 8; ++i) msg[o++] = values_[p - 1].v[i];
+    else if (chained_)
+      for (int i = 0; i < 8; ++i) msg[o++] = prev0_.v[i];
     for (uint32_t x : bindings_[p]) msg[o++] = x;
     if (pow_[p].first) {
       msg[o++] = kb_to_mont(TAG_FSPW);
@@ -160,6 +173,7 @@ class Transcript {
   // value" its message starts with); it must have been computed.
   const Digest& previous_value(const std::string& name) const {
     const size_t p = at(name);
+    if (p == 0 && chained_) return prev0_;
     if (p == 0 || !computed_[p - 1]) throw std::runtime_error("transcript: no computed predecessor of " + name);
     return values_[p - 1];
   }
@@ -224,6 +238,8 @@ class Transcript {
   std::vector<Digest> values_;
   std::vector<bool> computed_;
   std::vector<std::pair<uint32_t, uint32_t>> pow_;  // (bits, salt) per position, bits 0: none
+  bool chained_ = false;  // the first challenge follows a caller's computed one
+  Digest prev0_{};
   std::vector<size_t> log_;
 };
 

@@ -446,6 +446,7 @@ void replay(const Shape& S, const Plan& plan, const VerifyInput& in, const uint3
   const PlanData& d = *S.d;
   const uint32_t* w = P.w;
   Transcript ts(plan.schedule());
+  if (in.chained) ts.chain_from(in.fs_prev);
   const size_t nr = d.rounds.size();
   const std::string init = nr ? challenge_name(0) : "__zeta";
   ts.bind(init, string_to_elements(TAG_HASH, "poseidon2"));

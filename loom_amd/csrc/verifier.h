@@ -30,6 +30,10 @@ struct VerifyInput {
   const uint32_t* words = nullptr;  // canonical proof words
   size_t len = 0;
   std::vector<const uint32_t*> publics;  // per public-input column of the plan: its entry values (Montgomery)
+  // verifier.WithTranscript with a running transcript: its last digest, which
+  // the proof's first challenge follows (as in the prove)
+  bool chained = false;
+  uint32_t fs_prev[8] = {0};
 };
 
 struct VerifyResult {

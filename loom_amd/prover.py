@@ -214,7 +214,8 @@ def _pointer_tables(items, dev_inputs):
     return col_ptrs, perm_ptrs
 
 
-def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None, allow_failures=False):
+def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None, allow_failures=False,
+                   fs_prev=None):
     """Proves prepared instances sharing one plan shape in one batched call.
     With `dev_inputs` (a DeviceInputs of these items) the inputs are read
     from device memory (lg_prove_device), else from host memory (lg_prove).
@@ -222,7 +223,11 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
     An instance with an invalid witness (a lookup value outside its table, a
     permutation entry outside its cells) fails alone: by default the call
     raises LoomGpuError naming it; with `allow_failures` its result is None
-    and the other instances' proofs are returned."""
+    and the other instances' proofs are returned.
+
+    fs_prev: per instance the last digest (8 canonical values) of a running
+    transcript the proof's challenges follow (prover.WithTranscript,
+    lg_prove_chained); None for fresh transcripts."""
     if not items:
         return []
     words = items[0].words
@@ -240,6 +245,14 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
     B = len(items)
     col_ptrs, perm_ptrs = _pointer_tables(items, dev_inputs)
     entry = _lib.lib().lg_prove if dev_inputs is None else _lib.lib().lg_prove_device
+    if fs_prev is not None:
+        prev = np.ascontiguousarray(to_mont(np.asarray(fs_prev, dtype=np.uint64).reshape(-1)), dtype=np.uint32)
+        if prev.size != 8 * len(items):
+            raise ValueError("fs_prev: 8 values per instance")
+        dev_flag = 0 if dev_inputs is None else 1
+
+        def entry(ph_, B_, cp_, pp_, *rest, _p=prev, _d=dev_flag):
+            return _lib.lib().lg_prove_chained(ph_, B_, cp_, pp_, _d, _p.ctypes.data_as(_lib.u32p), *rest)
     cap = _proof_cap.get(words.tobytes(), 1 << 21)
     while True:
         outs = _out_pool.take(cap, B)
@@ -292,12 +305,13 @@ def autotune(ctx, items, dev_inputs=None, reps=3):
 VERIFY_CHECKS = {0: "ok", 1: "format", 2: "logup bus", 3: "AIR identity", 4: "opening", 5: "DEEP", 6: "FRI"}
 
 
-def verify(ctx, words, proofs, setup_roots=(), publics=None, skip_fri=False):
+def verify(ctx, words, proofs, setup_roots=(), publics=None, skip_fri=False, fs_prev=None):
     """lg_verify: checks proofs of the plan `words` in one batch; returns
     [(status, message)] per proof (status 0: valid; VERIFY_CHECKS names the
     others).  setup_roots: the verification key, canonical 8-value roots;
     publics: per proof, the public-input value arrays in plan order
-    (Montgomery)."""
+    (Montgomery); fs_prev: per proof the running transcript's last digest
+    (lg_verify_chained), None for fresh transcripts."""
     ph = ctx.plan_handle(words)
     arrs = [np.ascontiguousarray(p, dtype=np.uint32) for p in proofs]
     n = len(arrs)
@@ -311,8 +325,16 @@ def verify(ctx, words, proofs, setup_roots=(), publics=None, skip_fri=False):
         keep = flat
         pubs = (_lib.u32p * max(1, len(flat)))(*[v.ctypes.data_as(_lib.u32p) for v in flat])
     st = (ctypes.c_int32 * max(1, n))()
-    _lib.check(_lib.lib().lg_verify(ph, n, ptrs, lens, key.ctypes.data_as(_lib.u32p) if key.size else None,
-                                    key.size // 8, pubs, 1 if skip_fri else 0, st), "lg_verify")
+    keyp = key.ctypes.data_as(_lib.u32p) if key.size else None
+    if fs_prev is None:
+        _lib.check(_lib.lib().lg_verify(ph, n, ptrs, lens, keyp, key.size // 8, pubs, 1 if skip_fri else 0, st),
+                   "lg_verify")
+    else:
+        prev = np.ascontiguousarray(to_mont(np.asarray(fs_prev, dtype=np.uint64).reshape(-1)), dtype=np.uint32)
+        if prev.size != 8 * n:
+            raise ValueError("fs_prev: 8 values per proof")
+        _lib.check(_lib.lib().lg_verify_chained(ph, n, ptrs, lens, keyp, key.size // 8, pubs, 1 if skip_fri else 0,
+                                                prev.ctypes.data_as(_lib.u32p), st), "lg_verify_chained")
     out = []
     for i in range(n):
         msg = ""
@@ -326,12 +348,12 @@ def verify(ctx, words, proofs, setup_roots=(), publics=None, skip_fri=False):
     return out
 
 
-def verify_prepared(ctx, items, proofs, setup_roots=(), skip_fri=False):
+def verify_prepared(ctx, items, proofs, setup_roots=(), skip_fri=False, fs_prev=None):
     """verify() for the proofs of these prepared instances (one plan): the
     public-input values are taken from the instances."""
     npub = len(items[0].plan.public)
     publics = [it.columns[len(it.columns) - npub:] for it in items] if npub else None
-    return verify(ctx, items[0].words, proofs, setup_roots, publics, skip_fri)
+    return verify(ctx, items[0].words, proofs, setup_roots, publics, skip_fri, fs_prev)
 
 
 def graph_stats(ctx, items):

@@ -125,16 +125,20 @@ def as_e6_rows(v, m):
 
 # ------------------------------------------------------------------ prover
 class Prover:
-    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None, grinding=0):
+    def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None, grinding=0,
+                 fs_prev=None):
         """trace: column name -> canonical values (the witness, and the setup
         columns of the proving key); public_inputs: name -> (module,
         [(index, value), ...]) base-field entries; grinding: proof-of-work
         bits of every FRI fold challenge (fri.WithGrinding, fri.go:71-78;
-        0: off, the default)."""
+        0: off, the default); fs_prev: the last digest of a running
+        transcript the proof's challenges follow (prover.WithTranscript,
+        prover.go:54-60, 196-205), None for a fresh one."""
         self.prog = program
         self.layout = PR.build_layout(program)
         self.nq = num_queries
         self.grinding = int(grinding)
+        self.fs_prev = fs_prev
         self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items()}
         self.pi = dict(public_inputs or {})
         for name, (mod, entries) in self.pi.items():
@@ -387,6 +391,8 @@ class Prover:
         nr = len(prog.rounds)
         names = [B.challenge_name(r) for r in range(nr)] + ["__zeta"]
         self.ts = Transcript(names)
+        if self.fs_prev is not None:
+            self.ts.chain_from(self.fs_prev)
         init = names[0]
         self.ts.bind(init, string_to_elements(HASH_TAG, "poseidon2"))
         self.ts.bind(init, PR.module_sizes_transcript(prog))
@@ -728,9 +734,9 @@ def encode_proof(pf):
     return np.array(out, dtype=np.uint32)
 
 
-def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None, grinding=0):
+def prove(program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None, grinding=0, fs_prev=None):
     """Returns (proof dict, canonical u32 encoding, transcript log)."""
-    pv = Prover(program, trace, num_queries, public_inputs, grinding)
+    pv = Prover(program, trace, num_queries, public_inputs, grinding, fs_prev)
     pf = pv.prove()
     return pf, encode_proof(pf), pv.log
 

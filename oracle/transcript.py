@@ -52,6 +52,7 @@ class Transcript:
         self.bindings = {}
         self.values = {}
         self.pow = {}  # name -> (bits, salt) of challenges computed with grinding
+        self.prev0 = None  # the running transcript's last digest (chain_from)
         self.hasher = hasher
         for n in names:
             self.new_challenge(n)
@@ -70,6 +71,14 @@ class Transcript:
             raise ValueError("challenge already computed: " + name)
         self.bindings[name].extend(int(e) for e in elems)
 
+    def chain_from(self, digest):
+        """A running transcript (prover/verifier.WithTranscript) whose last
+        challenge is computed: this transcript's challenges follow it, so the
+        first one's message carries that digest (8 canonical elements)."""
+        if any(n in self.values for n in self.order):
+            raise ValueError("chained after a computation")
+        self.prev0 = [int(x) for x in digest]
+
     def message(self, name):
         """The exact element sequence ``compute`` hashes."""
         p = self.pos[name]
@@ -79,6 +88,8 @@ class Transcript:
             if prev not in self.values:
                 raise ValueError("previous challenge not computed")
             msg += self.values[prev]
+        elif self.prev0 is not None:
+            msg += self.prev0
         msg += self.bindings[name]
         return msg
 

@@ -214,13 +214,16 @@ def _e(v):
 
 
 # ------------------------------------------------------------------ verify
-def verify(program, num_queries, words, check_queries=None, setup_roots=(), public_inputs=None, grinding=0):
+def verify(program, num_queries, words, check_queries=None, setup_roots=(), public_inputs=None, grinding=0,
+           fs_prev=None):
     """Checks proof words against the program; returns the decoded proof.
     check_queries bounds how many queries have their openings checked (all by
     default); the transcript replay always covers every query.  setup_roots:
     the verification key (setup.VerificationKey.Roots); public_inputs: name ->
     (module, [(index, value), ...]); grinding: the FRI parameters' proof-of-work
-    bits (every fold challenge then carries a proof of work, fri.go:1060-1073)."""
+    bits (every fold challenge then carries a proof of work, fri.go:1060-1073);
+    fs_prev: the running transcript's last digest (verifier.WithTranscript,
+    verifier.go:60-65), None for a fresh one."""
     pf = decode_proof(words)
     lay = PR.build_layout(program)
     prog = program
@@ -240,6 +243,8 @@ def verify(program, num_queries, words, check_queries=None, setup_roots=(), publ
     nr = len(prog.rounds)
     names = [B.challenge_name(r) for r in range(nr)] + ["__zeta"]
     ts = Transcript(names)
+    if fs_prev is not None:
+        ts.chain_from(fs_prev)
     ts.bind(names[0], string_to_elements(HASH_TAG, "poseidon2"))
     ts.bind(names[0], PR.module_sizes_transcript(prog))
     for r in setup_roots:
