@@ -2348,7 +2348,9 @@ void Job::run() {
   chal_dev_ = dev((size_t)B_ * d_.cols.size() * 6);
   fs_dig_ = dev((nr + 1) * (size_t)B_ * 8);
   fold_pos_ = plan_.schedule()->pos.at("fri_fold_0");
+  mark("setup done");
   region_begin();  // the device phase: from the generated columns to the FRI commit phase
+  mark("device phase launched");
   {
     // zeroed with a fill kernel (a recorded graph replays it like any launch)
     std::vector<uint32_t*> cp{chal_dev_.as<uint32_t>()};
@@ -2415,9 +2417,11 @@ void Job::run() {
   phase("compute-air-quotients", t0);
   open();
   phase("pcs-open", t0);
-  if (!marks_.empty() && lane_ == 0) {
-    std::string line = "[loomgpu] marks (ms):";
+  if (!marks_.empty()) {
+    std::string line = "[loomgpu] lane " + std::to_string(lane_) + " marks (ms):";
     char buf[96];
+    snprintf(buf, sizeof buf, " t0 %.6f s;", t0_run_);
+    line += buf;
     for (auto& m : marks_) {
       snprintf(buf, sizeof buf, " %s %.3f;", m.first.c_str(), (m.second - t0_run_) * 1e3);
       line += buf;
@@ -3114,9 +3118,12 @@ void Job::open() {
   if (salt_words)
     copy_async(fri_h + 2 * FRI_ROUNDS * fb + fw_words, fri_salts_.as<void>(), salt_words * 4, hipMemcpyDeviceToHost);
   region_end();  // the device phase of the prove ends here
+  mark("device phase enqueued");
   sync();
+  mark("device phase done");
   if (grind_bits_) salts_h = fri_h + 2 * FRI_ROUNDS * fb + fw_words;
   if (deep_on_device) replay_alpha();
+  mark("transcripts replayed");
   std::vector<uint32_t> final_w(fri_h + 2 * FRI_ROUNDS * fb, fri_h + 2 * FRI_ROUNDS * fb + fw_words);
   {
     std::vector<uint32_t> dg(fri_h, fri_h + FRI_ROUNDS * fb), rt(fri_h + FRI_ROUNDS * fb, fri_h + 2 * FRI_ROUNDS * fb);
@@ -3537,10 +3544,12 @@ void Job::open() {
   // Query chains first (tasks are handed out in index order), then the
   // assemblies, which start as soon as threads free up and their chunk is
   // back, overlapping the last query chains.
+  this->mark("query phase start");
   parallel_for(2 * B_, [&](int t) {
     if (t < B_) query_task(t);
     else assemble_task(t - B_);
   });
+  this->mark("proofs assembled");
   if (getenv("LOOMGPU_TRACE")) {
     auto last = [&](const std::vector<double>& v) { return (*std::max_element(v.begin(), v.end()) - q_t0) * 1e3; };
     double chain = 0, wait = 0, assemble = 0;

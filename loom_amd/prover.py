@@ -138,6 +138,11 @@ class Prepared:
             self.columns.append(to_mont([v for _, v in sorted((int(i), int(v)) for i, v in public_inputs[name][1])]))
         self.perms = self.plan.perm_arrays()
         self.setup = [to_mont(trace[name]) for name in self.plan.setup]
+        # the plan's identity and the inputs' host addresses, computed once:
+        # per call they would cost more than the batched prove's host work
+        self.words_key = self.words.tobytes()
+        self._col_addrs = [c.ctypes.data for c in self.columns]
+        self._perm_addrs = [np.ascontiguousarray(q).ctypes.data for q in self.perms]
 
 
 _proof_cap = {}
@@ -192,26 +197,30 @@ class DeviceInputs:
         self.perms = [[torch.from_numpy(np.ascontiguousarray(s).view(np.int32)).to(dev) for s in it.perms]
                       for it in items]
         torch.cuda.synchronize(dev)
+        self.tables = _address_tables([[c.data_ptr() for c in cs] for cs in self.columns],
+                                      [[q.data_ptr() for q in ps] for ps in self.perms])
+
+
+class _AddressTable:
+    """A C array of addresses passed as uint32_t* const* (the array stays
+    referenced by the object)."""
+
+    def __init__(self, addrs):
+        self.arr = (ctypes.c_void_p * max(1, len(addrs)))(*addrs)
+        self.ptr = ctypes.cast(self.arr, _lib.u32pp)
+
+
+def _address_tables(col_addrs, perm_addrs):
+    return (_AddressTable([a for row in col_addrs for a in row]),
+            _AddressTable([a for row in perm_addrs for a in row]))
 
 
 def _pointer_tables(items, dev_inputs):
-    B = len(items)
-    nc = len(items[0].columns)
-    npm = len(items[0].perms)
-    col_ptrs = (_lib.u32p * max(1, B * nc))()
-    perm_ptrs = (_lib.u32p * max(1, B * npm))()
-    for b, it in enumerate(items):
-        for k in range(nc):
-            if dev_inputs is None:
-                col_ptrs[b * nc + k] = it.columns[k].ctypes.data_as(_lib.u32p)
-            else:
-                col_ptrs[b * nc + k] = ctypes.cast(dev_inputs.columns[b][k].data_ptr(), _lib.u32p)
-        for k in range(npm):
-            if dev_inputs is None:
-                perm_ptrs[b * npm + k] = it.perms[k].ctypes.data_as(_lib.u32p)
-            else:
-                perm_ptrs[b * npm + k] = ctypes.cast(dev_inputs.perms[b][k].data_ptr(), _lib.u32p)
-    return col_ptrs, perm_ptrs
+    """(columns, permutations) tables of the call: instance-major, plan
+    input order."""
+    if dev_inputs is not None:
+        return dev_inputs.tables
+    return _address_tables([it._col_addrs for it in items], [it._perm_addrs for it in items])
 
 
 def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None, allow_failures=False,
@@ -231,8 +240,9 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
     if not items:
         return []
     words = items[0].words
+    key = items[0].words_key
     for it in items[1:]:
-        if it.words.shape != words.shape or not np.array_equal(it.words, words):
+        if it.words_key != key:
             raise ValueError("batched instances must share one plan (same program shape)")
     if dev_inputs is not None and len(dev_inputs.items) != len(items):
         raise ValueError("dev_inputs does not match items")
@@ -243,7 +253,8 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
                 raise ValueError("batched instances must share one proving key (setup columns)")
     ph = ctx.plan_handle(words, items[0].setup)
     B = len(items)
-    col_ptrs, perm_ptrs = _pointer_tables(items, dev_inputs)
+    col_tab, perm_tab = _pointer_tables(items, dev_inputs)
+    col_ptrs, perm_ptrs = col_tab.ptr, perm_tab.ptr
     entry = _lib.lib().lg_prove if dev_inputs is None else _lib.lib().lg_prove_device
     if fs_prev is not None:
         prev = np.ascontiguousarray(to_mont(np.asarray(fs_prev, dtype=np.uint64).reshape(-1)), dtype=np.uint32)
@@ -253,11 +264,12 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
 
         def entry(ph_, B_, cp_, pp_, *rest, _p=prev, _d=dev_flag):
             return _lib.lib().lg_prove_chained(ph_, B_, cp_, pp_, _d, _p.ctypes.data_as(_lib.u32p), *rest)
-    cap = _proof_cap.get(words.tobytes(), 1 << 21)
+    cap = _proof_cap.get(key, 1 << 21)
     while True:
         outs = _out_pool.take(cap, B)
         touts = [np.zeros(1 << 16, dtype=np.uint32) for _ in range(B)] if want_transcript else None
-        out_ptrs = (_lib.u32p * B)(*[o.ctypes.data_as(_lib.u32p) for o in outs])
+        out_tab = _AddressTable([o.ctypes.data for o in outs])
+        out_ptrs = out_tab.ptr
         caps = (ctypes.c_size_t * B)(*([cap] * B))
         lens = (ctypes.c_size_t * B)()
         if want_transcript:
@@ -273,7 +285,7 @@ def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=No
         if rc == -3 and allow_failures:
             rc = 0
         _lib.check(rc, "lg_prove")
-        _proof_cap[words.tobytes()] = max(cap if max(lens) == 0 else max(lens), 1)
+        _proof_cap[key] = max(cap if max(lens) == 0 else max(lens), 1)
         break
     if phases is not None:
         arr = (ctypes.c_double * 3)()
@@ -294,7 +306,8 @@ def autotune(ctx, items, dev_inputs=None, reps=3):
     The choice applies to later proves of this batch size on this context."""
     words = items[0].words
     ph = ctx.plan_handle(words, items[0].setup)
-    col_ptrs, perm_ptrs = _pointer_tables(items, dev_inputs)
+    col_tab, perm_tab = _pointer_tables(items, dev_inputs)
+    col_ptrs, perm_ptrs = col_tab.ptr, perm_tab.ptr
     t = (ctypes.c_int32 * 6)()
     ms = (ctypes.c_double * 2)()
     _lib.check(_lib.lib().lg_plan_autotune(ph, len(items), col_ptrs, perm_ptrs, 0 if dev_inputs is None else 1, reps,
