@@ -186,7 +186,12 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
     for (int b = 0; b < batch; ++b)
       if (!out[b].error.empty()) failed += (failed.empty() ? "" : "; ") + std::string("instance ") + std::to_string(b) +
                                            ": " + out[b].error;
-    lg::parallel_for(batch, [&](int b) {
+    // copies only for proofs assembled into the library's own buffers and
+    // for transcripts; otherwise the loop is too short for the thread pool
+    bool copies = false;
+    for (int b = 0; b < batch; ++b)
+      if (out[b].error.empty() && (!out[b].direct || (transcripts && transcripts[b]))) copies = true;
+    auto finish = [&](int b) {
       proof_lens[b] = out[b].proof_len;
       if (!out[b].error.empty()) {
         proof_lens[b] = 0;
@@ -202,7 +207,10 @@ static int prove_impl(lg_plan* plan, int batch, const uint32_t* const* columns, 
         if (!transcript_caps || out[b].transcript.size() > transcript_caps[b]) rc = -2;
         else std::memcpy(transcripts[b], out[b].transcript.data(), out[b].transcript.size() * 4);
       }
-    });
+    };
+    if (copies) lg::parallel_for(batch, finish);
+    else
+      for (int b = 0; b < batch; ++b) finish(b);
     if (!failed.empty()) {
       // the other instances' proofs are complete (unless a buffer was short)
       g_err = "lg_prove: " + failed;

@@ -484,7 +484,10 @@ __global__ __launch_bounds__(1024) void k_ntt_pass_a(const BigJob* __restrict__ 
 // tmp[k2 n1 + .] (radix-16 DIF), giving X[k2 + n2 k1] (times post, natural
 // index), stored at that natural index or at its bit reversal
 // brev_a(k1) + n1 brev_b(k2).
-__global__ __launch_bounds__(1024) void k_ntt_pass_b(const BigJob* __restrict__ jobs, int a, int b,
+// 60 VGPRs at 8 waves per SIMD (no spills): two 1024-thread workgroups per
+// CU (LDS 2 x 64 KB at n1 = 2^11), so one workgroup's LDS phases overlap the
+// other's arithmetic; at the default 66 VGPRs only one fits
+__global__ __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(8))) void k_ntt_pass_b(const BigJob* __restrict__ jobs, int a, int b,
                                                      const uint32_t* __restrict__ tw_b, int bitrev_out) {
   extern __shared__ uint32_t sh[];
   const BigJob* job = jobs + blockIdx.y;

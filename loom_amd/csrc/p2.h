@@ -26,6 +26,44 @@ constexpr uint32_t TAG_NODE_MONT = 0x6425c2feu;  // Montgomery form of TAG_NODE
 template <int W>
 struct P2Consts;
 
+// acc + x (a 32-bit term into a 64-bit sum).  On the device it is one
+// v_mad_u64_u32 with an opaque multiplier 1 (a plain 64-bit addition of a
+// zero-extended value costs a register move and an add); the asm input I
+// keeps the multipliers of one sum distinct, so that the compiler neither
+// merges them nor re-factors the sum.
+template <int I>
+LG_HD uint64_t p2_acc(uint32_t x, uint64_t acc) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t one = 1;
+  asm("" : "+s"(one) : "i"(I));
+  return (uint64_t)x * one + acc;
+#else
+  return acc + x;
+#endif
+}
+template <int I>
+LG_HD uint64_t p2_acc2(uint32_t x, uint64_t acc) {  // acc + 2 x
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t two = 2;
+  asm("" : "+s"(two) : "i"(I));
+  return (uint64_t)x * two + acc;
+#else
+  return acc + 2 * (uint64_t)x;
+#endif
+}
+
+// v mod p below 2p for v < 2^37: v = hi 2^31 + lo and 2^31 = 2^24 - 1
+// (mod p), so r = lo + hi (2^24 - 1) < 2^31 + 2^30 (hi < 64); on the device
+// the product and sum are one multiply-add.
+LG_HD uint32_t p2_fold31(uint64_t v) {
+  const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul24(hi, 0xffffffu) + lo;
+#else
+  return lo + (hi << 24) - hi;
+#endif
+}
+
 // External layer; with RK >= 0 the round keys of full round RK (which the
 // next S-boxes take) are added before the reduction.  One reduction per
 // output: the M4 outputs (coefficient sum 7) and the chunk-wise sums are
@@ -33,20 +71,27 @@ struct P2Consts;
 // (plus a key) is below 50p < 50 * 2^31; with v = hi 2^31 + lo and
 // 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is below 2^31 + 50 * 2^24 < 2p,
 // and one unsigned minimum finishes.
+// M4 of chunks C.. in 64 bits (the chunk index is a template parameter so
+// that every multiply-add has its own opaque multiplier).
+template <int W, int C>
+LG_HD void p2_m4_chunks(const uint32_t* s, uint64_t* y) {
+  if constexpr (C < W / 4) {
+    const uint32_t x0 = s[4 * C], x1 = s[4 * C + 1], x2 = s[4 * C + 2], x3 = s[4 * C + 3];
+    const uint32_t a = x0 + x1, b = x2 + x3;
+    const uint64_t t0123 = p2_acc<8 * C>(b, p2_acc<8 * C + 1>(a, 0));
+    const uint64_t t01123 = p2_acc<8 * C + 2>(x1, t0123), t01233 = p2_acc<8 * C + 3>(x3, t0123);
+    y[4 * C + 0] = p2_acc<8 * C + 4>(a, t01123);
+    y[4 * C + 1] = p2_acc2<8 * C + 5>(x2, t01123);
+    y[4 * C + 2] = p2_acc<8 * C + 6>(b, t01233);
+    y[4 * C + 3] = p2_acc2<8 * C + 7>(x0, t01233);
+    p2_m4_chunks<W, C + 1>(s, y);
+  }
+}
+
 template <int W, int RK = -1>
 LG_HD void p2_external(uint32_t* s) {
   uint64_t y[W];
-#pragma unroll
-  for (int c = 0; c < W / 4; ++c) {
-    const uint32_t x0 = s[4 * c], x1 = s[4 * c + 1], x2 = s[4 * c + 2], x3 = s[4 * c + 3];
-    const uint32_t a = x0 + x1, b = x2 + x3;
-    const uint64_t t0123 = (uint64_t)a + b;
-    const uint64_t t01123 = t0123 + x1, t01233 = t0123 + x3;
-    y[4 * c + 0] = t01123 + a;
-    y[4 * c + 1] = t01123 + 2 * (uint64_t)x2;
-    y[4 * c + 2] = t01233 + b;
-    y[4 * c + 3] = t01233 + 2 * (uint64_t)x0;
-  }
+  p2_m4_chunks<W, 0>(s, y);
   uint64_t sum[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -59,8 +104,7 @@ LG_HD void p2_external(uint32_t* s) {
   for (int i = 0; i < W; ++i) {
     uint64_t v = y[i] + sum[i & 3];
     if constexpr (RK >= 0) v += P2Consts<W>::full(RK, i);
-    const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
-    const uint32_t r = lo + (hi << 24) - hi;
+    const uint32_t r = p2_fold31(v);
     s[i] = kb_umin(r, r - KB_P);
   }
 }
@@ -188,8 +232,7 @@ LG_HD uint32_t p2_state_sum(const uint32_t* s) {
   uint64_t acc = 0;
 #pragma unroll
   for (int i = 0; i < W; i += 2) acc += (uint64_t)(s[i] + s[i + 1]);
-  const uint32_t lo = (uint32_t)acc & 0x7fffffffu, hi = (uint32_t)(acc >> 31);
-  const uint32_t r = lo + (hi << 24) - hi;
+  const uint32_t r = p2_fold31(acc);
   return kb_umin(r, r - KB_P);
 }
 
@@ -487,8 +530,7 @@ __device__ __forceinline__ uint32_t row_sum(uint32_t v) {
 __device__ __forceinline__ uint32_t quad_m4(uint32_t x) {
   const uint32_t x1 = dpp<DPP_QUAD_ROT1>(x), x2 = dpp<DPP_QUAD_ROT2>(x), x3 = dpp<DPP_QUAD_ROT3>(x);
   const uint64_t v = 2 * (uint64_t)(x + x1) + x1 + (uint64_t)(x2 + x3);
-  const uint32_t lo = (uint32_t)v & 0x7fffffffu, hi = (uint32_t)(v >> 31);
-  const uint32_t r = lo + (hi << 24) - hi;
+  const uint32_t r = p2_fold31(v);
   return kb_umin(r, r - KB_P);
 }
 

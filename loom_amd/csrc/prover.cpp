@@ -814,9 +814,11 @@ class Job {
       if (gb_) gb_->broken = true;
     }
     set_prelaunch_hook(nullptr, nullptr);
+    const double t_in = marks_.empty() ? 0.0 : now_s();
     // after an error work may still be queued on the stream: drain it before
     // the buffers go back to the shared pool
     (void)hipStreamSynchronize(s_);
+    const double t_sync = marks_.empty() ? 0.0 : now_s();
     ctx_.pool.put(std::move(chal_dev_));
     ctx_.pool.put(std::move(fs_dig_));
     for (auto& c : cols_) ctx_.pool.put(std::move(c.buf));
@@ -829,6 +831,9 @@ class Job {
     ctx_.pool.put(std::move(pre_fold_.dev));
     ctx_.pool.put(std::move(fri_salts_));
     if (stg_) ctx_.give_staging(std::move(stg_));
+    if (!marks_.empty())
+      fprintf(stderr, "[loomgpu] lane %d job end (ms): entered %.3f, synced %.3f, buffers returned %.3f\n", lane_,
+              (t_in - t0_run_) * 1e3, (t_sync - t0_run_) * 1e3, (now_s() - t0_run_) * 1e3);
   }
 
   void run();
@@ -3655,14 +3660,20 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
     out[l].assign(part[l].size(), ProveOutput());  // a Job writes its outputs in place
   }
   std::vector<std::exception_ptr> err(lanes);
+  // host timestamps of the lanes' ends (LOOMGPU_MARKS)
+  static const bool marks_on = getenv("LOOMGPU_MARKS") != nullptr;
+  const double t_call = marks_on ? now_s() : 0.0;
+  std::vector<double> t_job(lanes), t_sync(lanes);
   auto run_lane = [&](int l) {
     try {
       current_pool() = &ThreadPool::for_lane(l, lanes);
       int mode = 0;
       BucketHold hold{this, take_bucket((int)part[l].size(), l, inputs[0].device, &mode)};
       run_job(part[l], out[l], l == 0 ? cb : PhaseCallback(), l, hold.b, mode);  // phases: lane 0's
+      if (marks_on) t_job[l] = now_s();
       current_pool() = nullptr;
       LG_HIP(hipStreamSynchronize(ctx_->lane_stream(l)));
+      if (marks_on) t_sync[l] = now_s();
     } catch (...) {
       current_pool() = nullptr;
       err[l] = std::current_exception();
@@ -3676,6 +3687,18 @@ void Plan::prove(const std::vector<InstanceInput>& inputs, std::vector<ProveOutp
     if (e) std::rethrow_exception(e);
   for (int l = 0; l < lanes; ++l)
     for (int b = 0; b < (int)out[l].size(); ++b) outputs[first[l] + b] = std::move(out[l][b]);
+  if (marks_on) {
+    std::string line = "[loomgpu] prove marks (ms): t0 " + std::to_string(t_call) + " s;";
+    char buf[96];
+    for (int l = 0; l < lanes; ++l) {
+      snprintf(buf, sizeof buf, " lane %d job returned %.3f, synced %.3f;", l, (t_job[l] - t_call) * 1e3,
+               (t_sync[l] - t_call) * 1e3);
+      line += buf;
+    }
+    snprintf(buf, sizeof buf, " outputs moved %.3f;", (now_s() - t_call) * 1e3);
+    line += buf;
+    fprintf(stderr, "%s\n", line.c_str());
+  }
 }
 
 }  // namespace lg

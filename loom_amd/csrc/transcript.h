@@ -4,6 +4,7 @@
 // (except for the first challenge) || its bindings.  Grinding is not used
 // (the reference's default).  All elements are Montgomery-form.
 #pragma once
+#include <algorithm>
 #include <cstdint>
 #include <map>
 #include <memory>
@@ -33,6 +34,31 @@ inline std::vector<uint32_t> string_to_elements(uint32_t tag, const std::string&
 
 struct Digest {
   uint32_t v[8];
+};
+
+// The bindings of one challenge: up to 8 words inline (a digest, the common
+// case of the FRI query chain), longer runs on the heap.  A proof's hundreds
+// of short bindings then cost no allocation to build or to free.
+class Words {
+ public:
+  void append(const uint32_t* e, size_t n) {
+    if (heap_.empty() && n_ + n <= kInline) {
+      std::copy(e, e + n, inl_ + n_);
+    } else {
+      if (heap_.empty()) heap_.assign(inl_, inl_ + n_);
+      heap_.insert(heap_.end(), e, e + n);
+    }
+    n_ += n;
+  }
+  size_t size() const { return n_; }
+  const uint32_t* begin() const { return heap_.empty() ? inl_ : heap_.data(); }
+  const uint32_t* end() const { return begin() + n_; }
+
+ private:
+  static constexpr size_t kInline = 8;
+  size_t n_ = 0;
+  uint32_t inl_[kInline] = {};
+  std::vector<uint32_t> heap_;
 };
 
 // Challenge names in registration order with their message prefixes
@@ -98,7 +124,7 @@ class Transcript {
   void bind(const std::string& name, const uint32_t* e, size_t n) {
     size_t p = at(name);
     if (computed_[p]) throw std::runtime_error("transcript: challenge already computed: " + name);
-    bindings_[p].insert(bindings_[p].end(), e, e + n);
+    bindings_[p].append(e, n);
   }
   void bind(const std::string& name, const std::vector<uint32_t>& e) { bind(name, e.data(), e.size()); }
   const Digest& compute(const std::string& name) {
@@ -135,7 +161,7 @@ class Transcript {
   size_t index(const std::string& name) const { return at(name); }
   void bind_at(size_t p, const uint32_t* e, size_t n) {
     if (p >= computed_.size() || computed_[p]) throw std::runtime_error("transcript: bad bind position");
-    bindings_[p].insert(bindings_[p].end(), e, e + n);
+    bindings_[p].append(e, n);
   }
   const Digest& compute_at(size_t p) {
     if (p >= computed_.size()) throw std::runtime_error("transcript: bad challenge position");
@@ -234,7 +260,7 @@ class Transcript {
   }
   ChallengeSchedule own_;                           // challenges registered one by one
   std::shared_ptr<const ChallengeSchedule> fixed_;  // or a shared schedule
-  std::vector<std::vector<uint32_t>> bindings_;
+  std::vector<Words> bindings_;
   std::vector<Digest> values_;
   std::vector<bool> computed_;
   std::vector<std::pair<uint32_t, uint32_t>> pow_;  // (bits, salt) per position, bits 0: none

@@ -138,11 +138,24 @@ class Prepared:
             self.columns.append(to_mont([v for _, v in sorted((int(i), int(v)) for i, v in public_inputs[name][1])]))
         self.perms = self.plan.perm_arrays()
         self.setup = [to_mont(trace[name]) for name in self.plan.setup]
-        # the plan's identity and the inputs' host addresses, computed once:
-        # per call they would cost more than the batched prove's host work
+        # the plan's identity, computed once (per call it would cost more than
+        # the batched prove's host work)
         self.words_key = self.words.tobytes()
-        self._col_addrs = [c.ctypes.data for c in self.columns]
-        self._perm_addrs = [np.ascontiguousarray(q).ctypes.data for q in self.perms]
+        self._addr_arrays = None
+
+    def host_addresses(self):
+        """Host addresses of the input columns and permutations.  Cached
+        while .columns and .perms hold the same array objects (the cache keeps
+        them referenced); replacing an array recomputes them."""
+        cache = self._addr_arrays
+        if (cache is None or len(cache[0]) != len(self.columns) or len(cache[1]) != len(self.perms)
+                or any(a is not b for a, b in zip(cache[0], self.columns))
+                or any(a is not b for a, b in zip(cache[1], self.perms))):
+            cols, perms = list(self.columns), list(self.perms)
+            perm_c = [np.ascontiguousarray(q) for q in perms]  # referenced with the addresses
+            cache = (cols, perms, [c.ctypes.data for c in cols], [q.ctypes.data for q in perm_c], perm_c)
+            self._addr_arrays = cache
+        return cache[2], cache[3]
 
 
 _proof_cap = {}
@@ -219,8 +232,15 @@ def _pointer_tables(items, dev_inputs):
     """(columns, permutations) tables of the call: instance-major, plan
     input order."""
     if dev_inputs is not None:
-        return dev_inputs.tables
-    return _address_tables([it._col_addrs for it in items], [it._perm_addrs for it in items])
+        # a DeviceInputs keeps its tables; any object with per-instance lists
+        # of device tensors (.columns, .perms) also serves
+        tabs = getattr(dev_inputs, "tables", None)
+        if tabs is not None:
+            return tabs
+        return _address_tables([[c.data_ptr() for c in cs] for cs in dev_inputs.columns],
+                               [[q.data_ptr() for q in ps] for ps in dev_inputs.perms])
+    addrs = [it.host_addresses() for it in items]
+    return _address_tables([a[0] for a in addrs], [a[1] for a in addrs])
 
 
 def prove_prepared(ctx, items, want_transcript=False, phases=None, dev_inputs=None, allow_failures=False,

@@ -180,3 +180,21 @@ def test_malformed_plans_are_rejected():
         rc = _lib.lib().lg_plan_bytes_per_instance(ww.ctypes.data_as(_lib.u32p), ww.size, ctypes.byref(v))
         assert rc == -1, name
         assert "plan" in _lib.lib().lg_last_error().decode(), name
+
+
+def test_host_addresses_follow_replaced_arrays():
+    """The Python binding caches an instance's input addresses; replacing an
+    input array (not only writing into it) must give the new array's address,
+    and the cached arrays stay referenced while their addresses are in use."""
+    from loom_amd import workloads, prover
+    it = prover.Prepared(*workloads.plonk_instance(5, 1), 8)
+    cols, perms = it.host_addresses()
+    assert cols == [c.ctypes.data for c in it.columns]
+    assert perms == [q.ctypes.data for q in it.perms]
+    assert it.host_addresses() == (cols, perms)
+    it.perms = [q.copy() for q in it.perms]
+    cols2, perms2 = it.host_addresses()
+    assert cols2 == cols
+    assert perms2 == [q.ctypes.data for q in it.perms] and perms2 != perms
+    it.columns[0] = it.columns[0].copy()
+    assert it.host_addresses()[0][0] == it.columns[0].ctypes.data
