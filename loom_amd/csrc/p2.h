@@ -64,34 +64,47 @@ LG_HD uint32_t p2_fold31(uint64_t v) {
 #endif
 }
 
-// External layer; with RK >= 0 the round keys of full round RK (which the
-// next S-boxes take) are added before the reduction.  One reduction per
-// output: the M4 outputs (coefficient sum 7) and the chunk-wise sums are
-// formed in 64 bits from 32-bit pair sums (x0 + x1 < 2p fits), so every output
-// (plus a key) is below 50p < 50 * 2^31; with v = hi 2^31 + lo and
-// 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is below 2^31 + 50 * 2^24 < 2p,
-// and one unsigned minimum finishes.
 // M4 of chunks C.. in 64 bits (the chunk index is a template parameter so
-// that every multiply-add has its own opaque multiplier).
-template <int W, int C>
+// that every multiply-add has its own opaque multiplier).  Inputs below p
+// (WIDE false) take 32-bit pair sums x0 + x1 < 2p; inputs below 2p (WIDE:
+// the unreduced S-box outputs) enter the 64-bit sums one by one.
+template <int W, int C, bool WIDE>
 LG_HD void p2_m4_chunks(const uint32_t* s, uint64_t* y) {
   if constexpr (C < W / 4) {
     const uint32_t x0 = s[4 * C], x1 = s[4 * C + 1], x2 = s[4 * C + 2], x3 = s[4 * C + 3];
-    const uint32_t a = x0 + x1, b = x2 + x3;
-    const uint64_t t0123 = p2_acc<8 * C>(b, p2_acc<8 * C + 1>(a, 0));
-    const uint64_t t01123 = p2_acc<8 * C + 2>(x1, t0123), t01233 = p2_acc<8 * C + 3>(x3, t0123);
-    y[4 * C + 0] = p2_acc<8 * C + 4>(a, t01123);
-    y[4 * C + 1] = p2_acc2<8 * C + 5>(x2, t01123);
-    y[4 * C + 2] = p2_acc<8 * C + 6>(b, t01233);
-    y[4 * C + 3] = p2_acc2<8 * C + 7>(x0, t01233);
-    p2_m4_chunks<W, C + 1>(s, y);
+    constexpr int K = 12 * C;
+    if constexpr (!WIDE) {
+      const uint32_t a = x0 + x1, b = x2 + x3;
+      const uint64_t t0123 = p2_acc<K>(b, p2_acc<K + 1>(a, 0));
+      const uint64_t t01123 = p2_acc<K + 2>(x1, t0123), t01233 = p2_acc<K + 3>(x3, t0123);
+      y[4 * C + 0] = p2_acc<K + 4>(a, t01123);
+      y[4 * C + 1] = p2_acc2<K + 5>(x2, t01123);
+      y[4 * C + 2] = p2_acc<K + 6>(b, t01233);
+      y[4 * C + 3] = p2_acc2<K + 7>(x0, t01233);
+    } else {
+      const uint64_t t01 = p2_acc<K>(x1, p2_acc<K + 1>(x0, 0));
+      const uint64_t t0123 = p2_acc<K + 2>(x3, p2_acc<K + 3>(x2, t01));
+      const uint64_t t01123 = p2_acc<K + 4>(x1, t0123), t01233 = p2_acc<K + 5>(x3, t0123);
+      y[4 * C + 0] = p2_acc<K + 6>(x1, p2_acc<K + 7>(x0, t01123));
+      y[4 * C + 1] = p2_acc2<K + 8>(x2, t01123);
+      y[4 * C + 2] = p2_acc<K + 9>(x3, p2_acc<K + 10>(x2, t01233));
+      y[4 * C + 3] = p2_acc2<K + 11>(x0, t01233);
+    }
+    p2_m4_chunks<W, C + 1, WIDE>(s, y);
   }
 }
 
-template <int W, int RK = -1>
+// External layer; with RK >= 0 the round keys of full round RK (which the
+// next S-boxes take) are added before the reduction.  One reduction per
+// output: the M4 outputs (coefficient sum 7) and the chunk-wise sums are
+// formed in 64 bits, so every output (plus a key) is below 50p < 50 * 2^31
+// for inputs below p, or 99p for inputs below 2p (WIDE); with v = hi 2^31 +
+// lo and 2^31 = 2^24 - 1 (mod p), lo + hi 2^24 - hi is below 2^31 + 99 * 2^24
+// < 2p, and one unsigned minimum gives the output below p.
+template <int W, int RK = -1, bool WIDE = false>
 LG_HD void p2_external(uint32_t* s) {
   uint64_t y[W];
-  p2_m4_chunks<W, 0>(s, y);
+  p2_m4_chunks<W, 0, WIDE>(s, y);
   uint64_t sum[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -148,11 +161,29 @@ LG_HD uint32_t kb_cube(uint32_t x) {
 #endif
 }
 
-// S-boxes of a full round whose keys are already in the state.
+// x^3 below 2p for x < p: the cube without kb_mul's final correction (the
+// product of sq < 2p and x < p is exact and its Montgomery quotient is below
+// 2p).  Its consumers take operands below 2p: the external layer's wide form
+// and the first operand of a Montgomery product.  The host returns x^3 < p.
+LG_HD uint32_t kb_cube_lazy(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  const uint64_t t = (uint64_t)x * x;
+  const uint32_t m = (uint32_t)t * 0x7effffffu;
+  const uint32_t sq = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
+  const uint64_t u = (uint64_t)sq * x;
+  const uint32_t n = (uint32_t)u * 0x7effffffu;
+  return (uint32_t)((u + (uint64_t)n * KB_P) >> 32);
+#else
+  return kb_cube(x);
+#endif
+}
+
+// S-boxes of a full round whose keys are already in the state; outputs
+// below 2p (for the wide external layer).
 template <int W>
 LG_HD void p2_cube_all(uint32_t* s) {
 #pragma unroll
-  for (int i = 0; i < W; ++i) s[i] = kb_cube(s[i]);
+  for (int i = 0; i < W; ++i) s[i] = kb_cube_lazy(s[i]);
 }
 
 // The internal diagonal entries are small integers and (negated) inverse
@@ -335,7 +366,11 @@ LG_HD void p2_partial_scaled(uint32_t* u) {
     constexpr uint32_t rk = P2Scaled<W>::rk(R), cfix = P2Scaled<W>::cfix(R);
     uint32_t x0 = kb_umin(u[0], u[0] - KB_P);
     x0 = kb_add(x0, rk);
-    u[0] = kb_mul(kb_cube(x0), cfix);
+    // cube below 2p times cfix < p: the Montgomery quotient, below 2p, is a
+    // lane value as it stands
+    const uint64_t t = (uint64_t)kb_cube_lazy(x0) * cfix;
+    const uint32_t m = (uint32_t)t * 0x7effffffu;
+    u[0] = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
     p2_scaled_lane<W, 0>(u, p2_scaled_sum<W, 0>(u, 0) << 24);
     p2_partial_scaled<W, R + 1>(u);
   }
@@ -423,11 +458,11 @@ LG_HD void p2_permute_form(uint32_t* s) {
   // that follows an external layer are folded into that layer's reduction
   p2_external<W, 0>(s);
   p2_cube_all<W>(s);
-  p2_external<W, 1>(s);
+  p2_external<W, 1, true>(s);
   p2_cube_all<W>(s);
-  p2_external<W, 2>(s);
+  p2_external<W, 2, true>(s);
   p2_cube_all<W>(s);
-  p2_external<W>(s);
+  p2_external<W, -1, true>(s);
 #if defined(__HIP_DEVICE_COMPILE__)
   if constexpr (P2Scaled<W>::ok()) p2_partial_rounds_scaled<W>(s);
   else {
@@ -441,11 +476,11 @@ LG_HD void p2_permute_form(uint32_t* s) {
 #pragma unroll
   for (int i = 0; i < W; ++i) s[i] = kb_add(s[i], P2Consts<W>::full(3, i));
   p2_cube_all<W>(s);
-  p2_external<W, 4>(s);
+  p2_external<W, 4, true>(s);
   p2_cube_all<W>(s);
-  p2_external<W, 5>(s);
+  p2_external<W, 5, true>(s);
   p2_cube_all<W>(s);
-  p2_external<W>(s);
+  p2_external<W, -1, true>(s);
 }
 
 template <int W>

@@ -132,6 +132,43 @@ def test_leaf_hash(torch, ctx, nb, ne, logrows):
     assert np.array_equal(got, want)
 
 
+def test_leaf_hash_extreme_values(torch, ctx):
+    """Leaf sponges of rows at the ends of the field (all p - 1, zeros,
+    alternating, p - 2 and 1): the device permutation's lazy ranges (S-box
+    outputs below 2p into the 64-bit external layer, scaled partial-round
+    lanes below 2p) meet their largest first-round sums; equal to the oracle."""
+    from loom_amd import _lib
+    from oracle import poseidon2 as p2
+    log("leaf_hash extreme values")
+    rows = 64
+    pats = [np.full(rows, P - 1, dtype=np.uint64), np.zeros(rows, dtype=np.uint64),
+            np.where(np.arange(rows) % 2 == 0, P - 1, 0).astype(np.uint64),
+            np.full(rows, P - 2, dtype=np.uint64), np.ones(rows, dtype=np.uint64)]
+    base = [pats[k % len(pats)].copy() for k in range(16)]
+    base[3][::3] = P - 1
+    ext = [np.full((rows, 6), P - 1, dtype=np.uint64)]
+    db = [dev_u32(torch, mont(c)) for c in base]
+    de = [dev_u32(torch, mont(c.T.copy()).reshape(-1)) for c in ext]
+    bp = (_lib.u32p * len(db))(*[ptr(t) for t in db])
+    ep = (_lib.u32p * len(de))(*[ptr(t) for t in de])
+    dig = torch.zeros(rows // 2 * 8, dtype=torch.int32, device="cuda")
+    _lib.check(_lib.lib().lg_hash_leaf_pairs(ctx.h, bp, len(db), ep, len(de), rows, ptr(dig)), "lg_hash_leaf_pairs")
+    got = canon(host_u32(dig)).reshape(rows // 2, 8).T
+    assert np.array_equal(got, p2.hash_leaf_pairs(base, ext))
+    # Merkle nodes over all-(p - 1) and all-zero leaves
+    for fill in (P - 1, 0):
+        n = 64
+        leaves = np.full((8, n), fill, dtype=np.uint64)
+        nodes = np.zeros((2 * n, 8), dtype=np.uint32)
+        nodes[n:] = mont(leaves.T)
+        dn = dev_u32(torch, nodes.reshape(-1))
+        ip = (_lib.u32p * 1)()
+        iw = (ctypes.c_size_t * 1)()
+        _lib.check(_lib.lib().lg_merkle_build(ctx.h, ptr(dn), n, ip, iw, 0), "lg_merkle_build")
+        from oracle.merkle import Tree
+        assert np.array_equal(canon(host_u32(dn)).reshape(2 * n, 8)[1:], Tree(leaves, []).nodes.T[1:])
+
+
 @pytest.mark.parametrize("logn,inj", [(0, []), (3, []), (5, [32, 8, 8, 1]), (10, [1024, 64]), (15, [])])
 def test_merkle(torch, ctx, logn, inj):
     from loom_amd import _lib
