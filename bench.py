@@ -91,8 +91,8 @@ def main():
     tuning = None
     if not args.no_tune:
         print("[bench] autotuning", file=sys.stderr, flush=True)
-        tb = prover.autotune(ctx, items, dev_inputs=dev_all, reps=3)
-        tp = prover.autotune(ctx, items[:1], dev_inputs=dev_one[0], reps=3)
+        tb = prover.autotune(ctx, items, dev_inputs=dev_all, reps=7)
+        tp = prover.autotune(ctx, items[:1], dev_inputs=dev_one[0], reps=7)
         tuning = {"batched": tb, "plain": tp}
     # warmup (plan compile, buffer pools, graph recording) for both paths
     for _ in range(max(1, args.warmup)):

@@ -470,14 +470,22 @@ Tuning Plan::autotune(const std::vector<InstanceInput>& inputs, int reps, double
   }
   // Trials run as production proves do: with graphs on, a trial's first
   // prove runs eagerly and its second records the graph, the timed ones
-  // replay it.
+  // replay it; the proofs go to caller-style buffers (allocated once, after
+  // the first prove gives their length), as lg_prove writes them.
   drop_graphs();
   int n_trials = 0;
+  std::vector<std::vector<uint32_t>> sink(B);
   auto time_it = [&](const Tuning& t) {
     tuned_[B] = t;
     drop_graphs();
     std::vector<ProveOutput> out;
     prove(in, out);  // warm-up: tables, buffers
+    for (int b = 0; b < B; ++b) {
+      if (in[b].proof_dst && in[b].proof_cap >= out[b].proof_len) continue;
+      sink[b].assign(out[b].proof_len + 1024, 0u);
+      in[b].proof_dst = sink[b].data();
+      in[b].proof_cap = sink[b].size();
+    }
     if (ctx_->graphs) prove(in, out);  // records the graph
     std::vector<double> v;
     for (int r = 0; r < reps; ++r) {
