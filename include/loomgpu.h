@@ -40,8 +40,11 @@
  *   - Refused (a plan or call that needs them is rejected with an error):
  *     extension-field setup columns or public-input entries,
  *     WithNewTranscriptHasher (the transcript hasher is the Poseidon2 sponge),
- *     WithFriOption other than grinding, other hash backends, lookups other
- *     than width-1 selector-free range lookups.
+ *     WithFriOption other than grinding, other hash backends.
+ *   - Lookups: every arguments/lookup.go form (Range, Lookup, LookupTuple,
+ *     LookupUnion(Tuple), CLookup(Tuple), CLookupUnion(Tuple)): tuples of up
+ *     to 8 base-field expressions, optional selectors, targets in several
+ *     modules; a source tuple in no selected target fails its instance.
  *
  * Plan word stream: see loom_amd/plan.py (module sizes, column table, rounds
  * and prover steps, commitment trees with opening shifts, folded constraint

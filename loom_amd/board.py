@@ -222,11 +222,14 @@ class Builder:
         self._grow(r)
         self.rounds[r].fshook = hook
 
-    def add_lookup_multiplicity_step(self, r, sel_s, sel_t, S, T, output, source_modules, target_module):
+    def add_lookup_multiplicity_step(self, r, sel_s, sel_t, S, T, output, source_modules, target_modules):
         """board.AddLookupMultiplicityStep (board/board.go:235-249): one
-        output column per target, named chunk_name(output, k)."""
+        output column per target, named chunk_name(output, k), in the target's
+        module; the step's module is the first target's."""
         ctx = dict(n_sources=len(S), n_targets=len(T), width=len(S[0]), has_sel_s=sel_s is not None,
-                   has_sel_t=sel_t is not None, source_modules=list(source_modules))
+                   has_sel_t=sel_t is not None, source_modules=list(source_modules),
+                   target_modules=list(target_modules))
+        target_module = target_modules[0]
         ins = list(sel_s or []) + list(sel_t or [])
         for t in S:
             ins += list(t)
@@ -343,34 +346,108 @@ def range_check(b, S, bound):
     lookup(b, S, T)
 
 
+@dataclass
+class Table:
+    """board.Table: a tuple of expressions of one module."""
+    module: str
+    cols: list
+
+
+def _tables_of(cols):
+    """columnsToTables (lookup.go:256-262)."""
+    return [Table(c.module, [c.expr]) for c in cols]
+
+
+def _check_widths(S, T):
+    """checkWidths (lookup.go:246-254)."""
+    w = len(S[0].cols)
+    for t in list(S) + list(T):
+        if len(t.cols) != w:
+            raise ValueError("inconsistent width, expected %d, got %d" % (w, len(t.cols)))
+
+
+def _fold_tables(tables):
+    """foldTables (lookup.go:232-244): each table folded into one column with
+    alpha = Coin(RoundFold); width-1 tables as they are."""
+    alpha = Coin(ROUND_FOLD)
+    return [Column(t.module, t.cols[0] if len(t.cols) == 1 else E.fold(alpha, t.cols)) for t in tables]
+
+
 def lookup(b, S, T):
-    """arguments.Lookup -> LookupUnionTuple (lookup.go:60-80) for width-1
-    tables: multiplicities staged at round 0, one logup per module at round 2."""
-    mult = _stage_lookup(b, None, None, [[S]], [[T]])
+    """arguments.Lookup (lookup.go:111-113): S in T for two columns."""
+    lookup_union(b, [S], [T])
+
+
+def lookup_union(b, S, T):
+    """arguments.LookupUnion (lookup.go:54-56): {S[0], S[1], ..} in {T[0], T[1], ..}."""
+    lookup_union_tuple(b, _tables_of(S), _tables_of(T))
+
+
+def lookup_tuple(b, S, T):
+    """arguments.LookupTuple (lookup.go:116-121): row(S) in row(T), tables of
+    equal width."""
+    if len(S.cols) != len(T.cols):
+        raise ValueError("[LookupTuple] S and T must have equal size, got %d and %d" % (len(S.cols), len(T.cols)))
+    lookup_union_tuple(b, [S], [T])
+
+
+def lookup_union_tuple(b, S, T):
+    """arguments.LookupUnionTuple (lookup.go:62-80): multiplicities staged at
+    RoundFold on the raw tuples, logups at RoundRunningSums with the tuples
+    folded by alpha; every source counts 1, every target -multiplicity."""
+    _check_widths(S, T)
+    mult = _stage_lookup(b, None, None, S, T)
     one = E.Const(1)
-    mult_t = [E.Col(chunk_name(mult, 0))]
-    _add_logups(b, [S], [T], [one], mult_t)
+    mult_t = [E.Col(chunk_name(mult, i)) for i in range(len(T))]
+    _add_logups(b, _fold_tables(S), _fold_tables(T), [one] * len(S), mult_t)
+
+
+def c_lookup(b, S, T, sel_s, sel_t):
+    """arguments.CLookup (lookup.go:123-125)."""
+    c_lookup_union(b, [sel_s], [sel_t], [S], [T])
+
+
+def c_lookup_tuple(b, S, T, sel_s, sel_t):
+    """arguments.CLookupTuple (lookup.go:129-131)."""
+    c_lookup_union_tuple(b, [sel_s], [sel_t], [S], [T])
+
+
+def c_lookup_union(b, sel_s, sel_t, S, T):
+    """arguments.CLookupUnion (lookup.go:83-85)."""
+    c_lookup_union_tuple(b, sel_s, sel_t, _tables_of(S), _tables_of(T))
+
+
+def c_lookup_union_tuple(b, sel_s, sel_t, S, T):
+    """arguments.CLookupUnionTuple (lookup.go:89-109): {row(S[i]) | selS[i] != 0}
+    in {row(T[i]) | selT[i] != 0}; sources count selS, targets
+    -multiplicity * selT."""
+    if len(sel_s) != len(S):
+        raise ValueError("selS has %d chunks, but S has %d chunks" % (len(sel_s), len(S)))
+    if len(sel_t) != len(T):
+        raise ValueError("selT has %d chunks, but T has %d chunks" % (len(sel_t), len(T)))
+    _check_widths(S, T)
+    mult = _stage_lookup(b, list(sel_s), list(sel_t), S, T)
+    mult_t = [E.Col(chunk_name(mult, i)) * sel_t[i] for i in range(len(T))]
+    _add_logups(b, _fold_tables(S), _fold_tables(T), list(sel_s), mult_t)
 
 
 def _stage_lookup(b, sel_s, sel_t, S, T):
-    """arguments.stageLookup (lookup.go:130-148); S, T: lists of tables, each a
-    list of Columns."""
+    """arguments.stageLookup (lookup.go:136-154); S, T: lists of Tables."""
     mult = b.fresh_name("Mult_")
     for tables in (S, T):
         for t in tables:
-            b.stage_leaves(ROUND_FOLD, *[c.expr for c in t])
+            b.stage_leaves(ROUND_FOLD, *t.cols)
     b.stage_leaves(ROUND_FOLD, *(sel_s or []))
     b.stage_leaves(ROUND_FOLD, *(sel_t or []))
-    b.add_lookup_multiplicity_step(ROUND_FOLD, sel_s, sel_t, [[c.expr for c in t] for t in S],
-                                   [[c.expr for c in t] for t in T], mult,
-                                   [t[0].module for t in S], T[0][0].module)
+    b.add_lookup_multiplicity_step(ROUND_FOLD, sel_s, sel_t, [list(t.cols) for t in S], [list(t.cols) for t in T],
+                                   mult, [t.module for t in S], [t.module for t in T])
     for i in range(len(T)):
         b.stage_columns(ROUND_FOLD, chunk_name(mult, i))
     return mult
 
 
 def _add_logups(b, S, T, num_s, num_t):
-    """arguments.addLogups (lookup.go:157-178): sources contribute
+    """arguments.addLogups (lookup.go:163-184): sources contribute
     num_s/(s - gamma), targets -num_t/(t - gamma); totals go on a bus."""
     gamma = Coin(ROUND_LOG_DERIVATIVE)
     minus_one = E.Const(E.P - 1)
@@ -384,7 +461,7 @@ def _add_logups(b, S, T, num_s, num_t):
 
 
 def add_logup_columns(b, prefix, terms, max_degree=None):
-    """arguments.AddLogupColumnsWithMaxDegree (lookup.go:195-230): terms
+    """arguments.AddLogupColumnsWithMaxDegree (lookup.go:197-230): terms
     grouped by module in order of first appearance, batched into columns of at
     most MAX_LOGUP_BATCH terms within the degree limit."""
     order = []

@@ -164,6 +164,37 @@ void launch_fill(uint32_t* const* p, int n, uint32_t v, int batch, hipStream_t s
 void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts, uint32_t bound,
                       uint32_t* const* err, int batch, hipStream_t s);
 void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s);
+
+// General lookup multiplicities (poly/lookup_multiplicity.go:37-100): tuples
+// of up to LK_MAX_W base-field columns with optional selectors, several
+// sources and targets.  Per instance, a hash table keyed by tuple, whose
+// slots hold a representative target row (rows are compared word by word
+// through the tables, so a slot is claimed by one compare-and-swap): the
+// first selected target row of each tuple (targets in order, then rows),
+// the selected source weight of each tuple (a 64-bit sum of canonical
+// selector values, or of ones), and the multiplicity weight / selT written
+// at the first row.  Source tuples found in no target are summed in a second
+// table (with source selectors) or flag the instance's error word at once.
+constexpr int LK_MAX_W = 8;
+struct LkTable {  // one source or target table of one instance
+  const uint32_t* col[LK_MAX_W];  // tuple columns (Montgomery, base field)
+  const uint32_t* sel;            // selector column, or null (all ones)
+  uint32_t* mult;                 // targets: the multiplicity column
+  uint32_t n, off;                // rows; first global row index (targets, then sources, counted apart)
+};
+struct LkJob {  // one instance
+  uint32_t* slot_row;             // cap: representative target row + 1, 0 empty
+  uint32_t* slot_first;           // cap: first selected target row (init ~0)
+  unsigned long long* slot_w;     // cap: summed source weight (init 0)
+  uint32_t* miss_row;             // miss_cap: representative source row + 1 (null: no selectors)
+  unsigned long long* miss_w;     // miss_cap
+  uint32_t* err;                  // error word
+  uint32_t cap_log, miss_cap_log;
+};
+// tabs: per instance nt targets then ns sources (batch x (nt + ns)); counts
+// of target and source rows per instance: total_t, total_s
+void launch_lookup_general(const LkTable* tabs, const LkJob* jobs, int nt, int ns, int w, uint32_t total_t,
+                           uint32_t total_s, uint32_t max_cap, uint32_t max_miss_cap, int batch, hipStream_t s);
 // acc = (first ? 0 : acc) + m / e (E6 columns, limb stride n)
 void launch_logup_acc(const uint32_t* const* e, const uint32_t* const* m, uint32_t* const* acc, int n, int first,
                       int batch, hipStream_t s);

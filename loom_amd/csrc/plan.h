@@ -13,7 +13,7 @@ namespace lg {
 constexpr uint32_t PLAN_MAGIC = 0x31504C4Cu;
 constexpr uint32_t PLAN_NONE = 0xFFFFFFFFu;
 
-constexpr uint32_t PLAN_VERSION = 4;
+constexpr uint32_t PLAN_VERSION = 5;
 
 enum ColSrc : uint32_t {
   SRC_INPUT = 0, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE,

@@ -599,23 +599,25 @@ Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_
           if (st.ins.size() != 2 || st.outs.size() != 1) throw std::runtime_error("grand product step: bad shape");
           break;
         case STEP_LOOKUP_MULTIPLICITY: {
-          // Supported: width-1 lookups without selectors into one generated
-          // range column of the step's module (arguments.Range).
+          // params: n_sources, n_targets, width, has_sel_s, has_sel_t, module
+          // per source, module per target; ins: selS, selT, source tuples,
+          // target tuples (board/step.go:257-295)
           if (st.params.size() < 5) throw std::runtime_error("lookup step: bad parameters");
-          const uint32_t ns = st.params[0];
-          if (st.params[1] != 1 || st.params[2] != 1 || st.params[3] || st.params[4] ||
-              st.params.size() != 5 + ns || st.ins.size() != ns + 1 || st.outs.size() != 1)
-            throw std::runtime_error("lookup step: only width-1 lookups without selectors into a range table are supported");
-          for (uint32_t j = 0; j < ns; ++j)
-            if (st.params[5 + j] >= d_.modules.size()) throw std::runtime_error("lookup step: source module out of range");
-          const Dag& t = st.ins[ns];
-          if (t.nodes.size() != 1 || t.nodes[0].op != OP_COL || t.nodes[0].b != 0 ||
-              d_.cols[t.nodes[0].a].src != SRC_RANGE || d_.cols[t.nodes[0].a].module != st.module ||
-              d_.cols[t.nodes[0].a].a0 > d_.modules[st.module].N)
-            throw std::runtime_error("lookup step: the target must be a range column of the step's module");
-          if (d_.cols[st.outs[0]].field != 0 || d_.cols[st.outs[0]].module != st.module)
-            throw std::runtime_error("lookup step: bad multiplicity column");
-          n_kernels = ns;
+          const uint32_t ns = st.params[0], nt = st.params[1], w = st.params[2];
+          const bool hs = st.params[3] != 0, ht = st.params[4] != 0;
+          if (ns < 1 || nt < 1 || w < 1 || w > (uint32_t)LK_MAX_W || st.params.size() != 5 + ns + nt)
+            throw std::runtime_error("lookup step: bad shape (at most " + std::to_string(LK_MAX_W) + " columns per tuple)");
+          if (st.ins.size() != (hs ? ns : 0) + (ht ? nt : 0) + (size_t)(ns + nt) * w || st.outs.size() != nt)
+            throw std::runtime_error("lookup step: bad inputs or outputs");
+          for (uint32_t j = 0; j < ns + nt; ++j)
+            if (st.params[5 + j] >= d_.modules.size()) throw std::runtime_error("lookup step: module out of range");
+          for (uint32_t k = 0; k < nt; ++k)
+            if (d_.cols[st.outs[k]].field != 0 || d_.cols[st.outs[k]].module != st.params[5 + ns + k])
+              throw std::runtime_error("lookup step: bad multiplicity column");
+          // tuples and selectors are base-field values (evalTable, poly/lookup_multiplicity.go:112-139)
+          for (auto& d : st.ins)
+            if (d.nodes.empty() || d.root >= d.nodes.size() || d.nodes[d.root].field != 0)
+              throw std::runtime_error("lookup step: tuple entries and selectors must be base-field expressions");
           break;
         }
         case STEP_CYCLIC_LOGUP:
@@ -1018,6 +1020,7 @@ class Job {
   bool same_dag(size_t r, const StepDag& a, const StepDag& b) const;
   void step_grand_products(size_t r, const std::vector<size_t>& steps);
   void step_multiplicity(size_t r, size_t s);
+  void step_multiplicity_general(size_t r, size_t s);
   void step_logups(size_t r, const std::vector<size_t>& steps);
   void finish_round_steps(size_t r);
   void flush_err_flags();
@@ -1904,8 +1907,24 @@ void Job::step_grand_products(size_t r, const std::vector<size_t>& steps) {
 // (poly/lookup_multiplicity.go:90-96), reported after the round's commit.
 // Sources of one size and DAG code are evaluated and counted in one launch
 // each (the counts are atomic additions, so their order does not matter).
+// The range shape of arguments.Range (lookup.go:34-51): width 1, no
+// selectors, one target that is a generated range column [0, bound) of the
+// step's module, where the first matching row of value v is row v.
+static bool range_lookup(const PlanData& d, const Step& st) {
+  if (st.params[1] != 1 || st.params[2] != 1 || st.params[3] || st.params[4]) return false;
+  const uint32_t ns = st.params[0];
+  const Dag& t = st.ins[ns];
+  return t.nodes.size() == 1 && t.nodes[0].op == OP_COL && t.nodes[0].b == 0 && d.cols[t.nodes[0].a].src == SRC_RANGE &&
+         d.cols[t.nodes[0].a].module == st.module && st.params[5 + ns] == st.module &&
+         d.cols[t.nodes[0].a].a0 <= d.modules[st.module].N;
+}
+
 void Job::step_multiplicity(size_t r, size_t s) {
   const Step& st = d_.rounds[r].steps[s];
+  if (!range_lookup(d_, st)) {
+    step_multiplicity_general(r, s);
+    return;
+  }
   const uint32_t ns = st.params[0];
   const uint32_t bound = d_.cols[st.ins[ns].nodes[0].a].a0;
   const ColBuf& mc = cols_[st.out_col];
@@ -1946,6 +1965,107 @@ void Job::step_multiplicity(size_t r, size_t s) {
   }
   launch_counts_to_field(cnt_d, (int)mc.n, B_, s_);
   pending_err_.push_back({"lookup: a source value is not in the range table", std::move(err)});
+}
+
+// poly.BuildLookupMultiplicities (poly/lookup_multiplicity.go:37-100) for
+// any shape: every selector and tuple entry evaluated on the device, then the
+// hash-table kernels of launch_lookup_general per instance.
+void Job::step_multiplicity_general(size_t r, size_t s) {
+  const Step& st = d_.rounds[r].steps[s];
+  const uint32_t ns = st.params[0], nt = st.params[1], w = st.params[2];
+  const bool hs = st.params[3] != 0, ht = st.params[4] != 0;
+  auto mod_of = [&](uint32_t x) { return st.params[5 + x]; };  // x < ns: source, else target x - ns
+  // input DAG index of each operand
+  const size_t i_sel_s = 0, i_sel_t = hs ? ns : 0, i_src = i_sel_t + (ht ? nt : 0), i_tgt = i_src + (size_t)ns * w;
+  std::vector<Layer> val(st.ins.size());
+  for (size_t k = 0; k < st.ins.size(); ++k) {
+    uint32_t mi;
+    if (k < i_sel_t) mi = mod_of((uint32_t)k);
+    else if (k < i_src) mi = mod_of(ns + (uint32_t)(k - i_sel_t));
+    else if (k < i_tgt) mi = mod_of((uint32_t)((k - i_src) / w));
+    else mi = mod_of(ns + (uint32_t)((k - i_tgt) / w));
+    val[k] = eval_step_dags(r, {StepDag{s, k, mi}});
+  }
+  uint32_t total_t = 0, total_s = 0;
+  for (uint32_t k = 0; k < nt; ++k) total_t += d_.modules[mod_of(ns + k)].N;
+  for (uint32_t j = 0; j < ns; ++j) total_s += d_.modules[mod_of(j)].N;
+  auto cap_log_of = [](uint64_t rows) {
+    int l = 10;
+    while ((1ull << l) < 2 * rows) ++l;
+    return (uint32_t)l;
+  };
+  const uint32_t cap_log = cap_log_of(total_t), miss_log = hs ? cap_log_of(total_s) : 0;
+  const size_t cap = (size_t)1 << cap_log, mcap = hs ? (size_t)1 << miss_log : 0;
+  DevBuf row = dev(B_ * cap), first = dev(B_ * cap), wsum = dev(B_ * cap * 2);
+  DevBuf mrow = hs ? dev(B_ * mcap) : DevBuf(), mw = hs ? dev(B_ * mcap * 2) : DevBuf();
+  DevBuf err = dev((size_t)B_);
+  std::vector<uint32_t*> fill0, fill1;
+  std::vector<LkTable> tabs((size_t)B_ * (nt + ns));
+  std::vector<LkJob> jobs(B_);
+  for (int b = 0; b < B_; ++b) {
+    LkJob& J = jobs[b];
+    J.slot_row = row.as<uint32_t>() + b * cap;
+    J.slot_first = first.as<uint32_t>() + b * cap;
+    J.slot_w = reinterpret_cast<unsigned long long*>(wsum.as<uint32_t>() + b * cap * 2);
+    J.miss_row = hs ? mrow.as<uint32_t>() + b * mcap : nullptr;
+    J.miss_w = hs ? reinterpret_cast<unsigned long long*>(mw.as<uint32_t>() + b * mcap * 2) : nullptr;
+    J.err = err.as<uint32_t>() + b;
+    J.cap_log = cap_log;
+    J.miss_cap_log = miss_log;
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < nt; ++k) {  // targets
+      LkTable& t = tabs[(size_t)b * (nt + ns) + k];
+      for (uint32_t c = 0; c < w; ++c) t.col[c] = val[i_tgt + (size_t)k * w + c].ptr(b);
+      t.sel = ht ? val[i_sel_t + k].ptr(b) : nullptr;
+      t.mult = cols_[st.outs[k]].lag(b, 0);
+      t.n = d_.modules[mod_of(ns + k)].N;
+      t.off = off;
+      off += t.n;
+    }
+    off = 0;
+    for (uint32_t j = 0; j < ns; ++j) {  // sources
+      LkTable& t = tabs[(size_t)b * (nt + ns) + nt + j];
+      for (uint32_t c = 0; c < w; ++c) t.col[c] = val[i_src + (size_t)j * w + c].ptr(b);
+      t.sel = hs ? val[i_sel_s + j].ptr(b) : nullptr;
+      t.mult = nullptr;
+      t.n = d_.modules[mod_of(j)].N;
+      t.off = off;
+      off += t.n;
+    }
+    fill0.push_back(J.slot_row);
+    fill1.push_back(J.slot_first);
+  }
+  // cleared here by fill kernels (no memset inside a recorded graph)
+  launch_fill(const_cast<uint32_t* const*>(stage(fill0)), (int)cap, 0u, B_, s_);
+  launch_fill(const_cast<uint32_t* const*>(stage(fill1)), (int)cap, ~0u, B_, s_);
+  {
+    std::vector<uint32_t*> z;
+    for (int b = 0; b < B_; ++b) z.push_back(wsum.as<uint32_t>() + b * cap * 2);
+    launch_fill(const_cast<uint32_t* const*>(stage(z)), (int)(cap * 2), 0u, B_, s_);
+    if (hs) {
+      std::vector<uint32_t*> zr, zw;
+      for (int b = 0; b < B_; ++b) {
+        zr.push_back(mrow.as<uint32_t>() + b * mcap);
+        zw.push_back(mw.as<uint32_t>() + b * mcap * 2);
+      }
+      launch_fill(const_cast<uint32_t* const*>(stage(zr)), (int)mcap, 0u, B_, s_);
+      launch_fill(const_cast<uint32_t* const*>(stage(zw)), (int)(mcap * 2), 0u, B_, s_);
+    }
+    std::vector<uint32_t*> ze;
+    for (int b = 0; b < B_; ++b) ze.push_back(err.as<uint32_t>() + b);
+    launch_fill(const_cast<uint32_t* const*>(stage(ze)), 1, 0u, B_, s_);
+  }
+  launch_lookup_general(stage(tabs), stage(jobs), (int)nt, (int)ns, (int)w, total_t, total_s, (uint32_t)cap,
+                        (uint32_t)mcap, B_, s_);
+  for (auto& v : val) keep_.push_back(std::move(v.buf));
+  keep_.push_back(std::move(row));
+  keep_.push_back(std::move(first));
+  keep_.push_back(std::move(wsum));
+  if (hs) {
+    keep_.push_back(std::move(mrow));
+    keep_.push_back(std::move(mw));
+  }
+  pending_err_.push_back({"lookup: a source tuple is not in any target", std::move(err)});
 }
 
 // board.CyclicLogUpStep (board/logup.go:138-186) for a class of steps:

@@ -20,8 +20,8 @@ Word stream (u32, little endian):
             n_steps, per step: kind, module, n_outs, out cols...,
             n_params, params..., n_ins, dag...
             (params: lookup multiplicity -> n_sources, n_targets, width,
-            has_sel_s, has_sel_t, source module per source; cyclic logup ->
-            n_terms; grand product -> none)
+            has_sel_s, has_sel_t, module per source, module per target;
+            cyclic logup -> n_terms; grand product -> none)
   n_exposed, then per exposed value (sorted by name): column, name
             (byte length, bytes packed 4 per word, little endian)
   n_public, then per public-input column (sorted by name): column, module,
@@ -43,7 +43,7 @@ from . import board as B
 from . import protocol as PR
 
 MAGIC = 0x31504C4C
-VERSION = 4
+VERSION = 5
 NONE = 0xFFFFFFFF
 
 SRC_INPUT, SRC_LAGRANGE, SRC_ID, SRC_SIGMA, SRC_STEP, SRC_CHALLENGE, SRC_AIR, SRC_RANGE, SRC_SETUP, SRC_PUBLIC = range(10)
@@ -117,8 +117,11 @@ class Plan:
         for r, rd in enumerate(prog.rounds):
             self._add(B.challenge_name(r), E.EXT, 0, SRC_CHALLENGE, r)
             for s, st in enumerate(rd.steps):
-                for o in st.outs:
-                    step_out[o] = (self.mod_idx[st.module], r, s)
+                # a multiplicity column lives in its target's module
+                mods = (st.ctx.get("target_modules") if st.kind == B.LOOKUP_MULTIPLICITY else None) or \
+                    [st.module] * len(st.outs)
+                for o, om in zip(st.outs, mods):
+                    step_out[o] = (self.mod_idx[om], r, s)
         fields = prog.column_fields
 
         def reg(name):
@@ -224,11 +227,14 @@ class Plan:
                 sizes = [n] * len(st.ins)
                 if st.kind == B.LOOKUP_MULTIPLICITY:
                     c = st.ctx
+                    tmods = c.get("target_modules") or [st.module] * c["n_targets"]
                     params = [c["n_sources"], c["n_targets"], c["width"], int(c["has_sel_s"]), int(c["has_sel_t"])]
                     params += [self.mod_idx[x] for x in c["source_modules"]]
+                    params += [self.mod_idx[x] for x in tmods]
                     ns = [prog.modules[x].N for x in c["source_modules"]]
-                    sizes = (ns if c["has_sel_s"] else []) + ([n] * c["n_targets"] if c["has_sel_t"] else [])
-                    sizes += [x for x in ns for _ in range(c["width"])] + [n] * (c["n_targets"] * c["width"])
+                    nt = [prog.modules[x].N for x in tmods]
+                    sizes = (ns if c["has_sel_s"] else []) + (nt if c["has_sel_t"] else [])
+                    sizes += [x for x in ns for _ in range(c["width"])] + [x for x in nt for _ in range(c["width"])]
                 elif st.kind == B.CYCLIC_LOGUP:
                     params = [st.ctx["n_terms"]]
                 else:

@@ -264,3 +264,88 @@ def plonk_setup_instance(log_n, setup_seed, witness_seed, n_public=0):
     B.copy_constraint(b, "plonk", [E.Col("L"), E.Col("R"), E.Col("O")], B.PermutationGen(sigma, "plonk.S"))
     public = {"PI": ("plonk", pi_entries)} if n_public else {}
     return B.compile_program(b), trace, public
+
+
+def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
+    """Programs with the general lookups of arguments/lookup.go (tuples,
+    selectors, unions), seeded with PCG64(seed).  kind:
+    * "tuple":  LookupTuple of (A, B) of module "main" (2^log_n rows) into
+      (X, Y) of module "tab" (2^log_t rows); the table repeats some pairs, so
+      the first matching row takes the weight;
+    * "cond":   CLookupTuple with selectors: source rows with selector 0 hold
+      pairs outside the table, others selector 1 or 2; target selectors 0 on
+      some rows (a later copy of that pair then takes its weight), else
+      values 1..5; two unselected-in-target source rows of one pair whose
+      selectors sum to 0 mod p (no error: their weight cancels);
+    * "union":  LookupUnion of columns of two source modules (2^log_n and
+      2^(log_n-1) rows) into columns of two target modules (2^log_t and
+      2^(log_t-1) rows).
+    bad: one source row takes a value in no target (the prove fails)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n, nt = 1 << log_n, 1 << log_t
+    b = B.Builder()
+    tr = {}
+
+    def module(name, rows):
+        m = B.Module(name, rows)
+        b.add_module(m)
+        return m
+
+    if kind in ("tuple", "cond"):
+        module("main", n)
+        module("tab", nt)
+        X = rng.integers(0, P, size=nt, dtype=np.uint64)
+        Y = rng.integers(0, P, size=nt, dtype=np.uint64)
+        dup = 8 + rng.choice(nt - 9, size=nt // 8, replace=False)
+        X[dup[1:]], Y[dup[1:]] = X[dup[0]], Y[dup[0]]  # one pair on several rows
+        X[nt - 1], Y[nt - 1] = X[3], Y[3]
+        if kind == "cond":
+            X[5] = (X[5] + 1) % P  # a pair on no selected target row (below)
+        rows = np.array([i for i in range(nt) if not (kind == "cond" and i == 5)])
+        r = rows[rng.integers(0, len(rows), size=n)]
+        A, Bv = X[r].copy(), Y[r].copy()
+        tr.update(X=X, Y=Y, A=A, B=Bv)
+        S = B.Table("main", [E.Col("A"), E.Col("B")])
+        T = B.Table("tab", [E.Col("X"), E.Col("Y")])
+        if kind == "tuple":
+            B.lookup_tuple(b, S, T)
+        else:
+            SA = rng.choice(np.array([0, 1, 2], dtype=np.uint64), size=n, p=[0.2, 0.6, 0.2])
+            off = SA == 0
+            A[off] = rng.integers(0, P, size=int(off.sum()), dtype=np.uint64)  # junk, unselected
+            ST = rng.integers(1, 6, size=nt).astype(np.uint64)
+            ST[3] = 0  # row 3's pair is also on row nt-1, which takes its weight
+            # row 5's pair is on no selected target row: two source rows of it
+            # whose selectors cancel
+            ST[5] = 0
+            A[0], Bv[0], SA[0] = X[5], Y[5], 1
+            A[1], Bv[1], SA[1] = X[5], Y[5], P - 1
+            tr.update(SA=SA, ST=ST)
+            B.c_lookup_tuple(b, S, T, E.Col("SA"), E.Col("ST"))
+        if bad:
+            A[n - 1] = (X[0] + 12345) % P
+            Bv[n - 1] = (Y[0] + 777) % P
+            if kind == "cond":
+                tr["SA"][n - 1] = 1
+    elif kind == "union":
+        module("src0", n)
+        module("src1", n // 2)
+        module("tab0", nt)
+        module("tab1", nt // 2)
+        T0 = rng.integers(0, P, size=nt, dtype=np.uint64)
+        T1 = rng.integers(0, P, size=nt // 2, dtype=np.uint64)
+        T1[2] = T0[7]  # a value in both targets: tab0's row takes it
+        pool = np.concatenate([T0, T1])
+        S0 = pool[rng.integers(0, len(pool), size=n)]
+        S1 = pool[rng.integers(0, len(pool), size=n // 2)]
+        S1[0] = T1[2]
+        if bad:
+            S1[-1] = (int(T0.max()) + 1) % P
+            while S1[-1] in set(pool.tolist()):
+                S1[-1] = (S1[-1] + 1) % P
+        tr.update(S0=S0, S1=S1, T0=T0, T1=T1)
+        B.lookup_union(b, [B.Column("src0", E.Col("S0")), B.Column("src1", E.Col("S1"))],
+                       [B.Column("tab0", E.Col("T0")), B.Column("tab1", E.Col("T1"))])
+    else:
+        raise ValueError(kind)
+    return B.compile_program(b), tr

@@ -268,9 +268,10 @@ class Prover:
                     order.append(key)
                     weights[key] = 0
                 weights[key] = (weights[key] + int(sel[i])) % P
-        nt = self.prog.modules[st.module].N
+        tmods = c.get("target_modules") or [st.module] * c["n_targets"]
         assigned = set()
         for k, tup in enumerate(T):
+            nt = self.prog.modules[tmods[k]].N
             cols = [self._eval_base(e, nt) for e in tup]
             sel = self._eval_base(sel_t[k], nt) if sel_t else np.ones(nt, dtype=np.uint64)
             res = np.zeros(nt, dtype=np.uint64)

@@ -189,7 +189,7 @@ def test_base_only_slot_stability():
     mul = B.chunk_name("mul", 0)
     m.assert_zero(E.Col("z") + E.Col(mul) + E.Col("s") + E.Col("t") + E.Setup("pz") + E.Setup("pa") + E.Setup("pm"))
     b.add_module(m)
-    b.add_lookup_multiplicity_step(0, None, None, [[E.Col("s")]], [[E.Col("t")]], "mul", ["m"], "m")
+    b.add_lookup_multiplicity_step(0, None, None, [[E.Col("s")]], [[E.Col("t")]], "mul", ["m"], ["m"])
     lay = PR.build_layout(B.compile_program(b))
     want = {"pa": (0, 0, 0, E.BASE), "pm": (0, 0, 1, E.BASE), "pz": (0, 0, 2, E.BASE),
             mul: (1, 0, 0, E.BASE), "s": (1, 0, 1, E.BASE), "t": (1, 0, 2, E.BASE), "z": (1, 0, 3, E.BASE),
