@@ -32,13 +32,15 @@
  *   - WithFriOption(fri.WithGrinding(bits)): lg_plan_set_grinding.
  *   - setup.ProvingKey: lg_plan_set_setup commits the setup columns once per
  *     plan and keeps them resident (base-field setup columns).
- *   - public.Inputs: base-field entries at rows the plan lists; each
+ *   - public.Inputs: entries at rows the plan lists, base-field values or
+ *     (for inputs declared with expr.PublicInputExt) E6 values; each
  *     instance passes the values after its input columns.
  *   - WithTranscript: a fresh running transcript is lg_prove (its computed
  *     challenges come back in `transcripts`); one with computed challenges is
  *     lg_prove_chained (its last digest).
  *   - Refused (a plan or call that needs them is rejected with an error):
- *     extension-field setup columns or public-input entries,
+ *     extension-field setup columns, extension entries of a public input
+ *     declared in the base field,
  *     WithNewTranscriptHasher (the transcript hasher is the Poseidon2 sponge),
  *     WithFriOption other than grinding, other hash backends.
  *   - Lookups: every arguments/lookup.go form (Range, Lookup, LookupTuple,
@@ -113,8 +115,9 @@ int lg_plan_check(const uint32_t* words, size_t n_words, size_t* code_bytes);
 
 /* Number of input arrays / permutations an instance must provide: the trace
  * input columns (N words each, plan input order), then for each public-input
- * column of the plan (public.Inputs, name order) its entry values (one word
- * per entry row the plan lists, ascending rows). */
+ * column of the plan (public.Inputs, name order) its entry values in
+ * ascending row order: one word per base entry, six (the E6 limbs B0.A0 ..
+ * B2.A1, Montgomery) per extension entry, as the plan marks each entry. */
 int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms);
 
 /* setup.Setup (setup/setup.go:68-124) for a plan with setup columns

@@ -521,6 +521,9 @@ class Program:
     setup_columns: list = field(default_factory=list)
     # public-input columns referenced by the relations: (name, module), by name
     public_columns: list = field(default_factory=list)
+    # public-input columns declared in the extension field (stored with six
+    # limbs; their entries may be E6 values)
+    public_ext: list = field(default_factory=list)
 
 
 def _column_modules(modules):
@@ -588,6 +591,7 @@ def compile_program(b):
     # (collectSetupColumns, board/program.go:138-165); public-input leaves
     setup = {}
     public = {}
+    public_ext = set()  # public inputs declared in the extension field (expr.PublicInputExt)
     for mn, m in b.modules.items():
         for rel in m.relations:
             for leaf in E.leaves_full(rel):
@@ -598,6 +602,8 @@ def compile_program(b):
                     if public.get(leaf.name, mn) != mn:
                         raise ValueError("public input %r used by two modules" % leaf.name)
                     public[leaf.name] = mn
+                    if leaf.field == E.EXT:
+                        public_ext.add(leaf.name)
     setup_columns = [setup[k] for k in sorted(setup)]
     for c in setup_columns:
         setf(c.name, c.field)
@@ -610,4 +616,4 @@ def compile_program(b):
         dag = E.to_dag(rel, fields)
         mods[k] = CompiledModule(m.name, m.N, rel, dag, E.degree(rel), list(m.gencol))
     return Program(mods, rounds, fields, [list(x) for x in b.logup_bus], setup_columns,
-                   sorted(public.items()))
+                   sorted(public.items()), sorted(public_ext))

@@ -83,7 +83,14 @@ struct TreeInfo {
 struct PublicCol {
   uint32_t col, module;
   std::string name, module_name;
-  std::vector<uint32_t> rows;  // ascending
+  std::vector<uint32_t> rows;    // ascending
+  std::vector<uint32_t> fields;  // per entry: 0 base (one value word), 1 extension (six)
+  // value words an instance passes for this input (entries in row order)
+  size_t words() const {
+    size_t w = 0;
+    for (auto f : fields) w += f ? 6 : 1;
+    return w;
+  }
 };
 
 struct PlanData {
@@ -198,8 +205,10 @@ inline void validate_plan(const PlanData& p) {
         if (c.field != 0 || c.a0 >= p.n_setup) bad("setup column index out of range (base-field setup columns only)");
         break;
       case SRC_PUBLIC:
-        if (c.field != 0 || c.a0 >= p.publics.size() || p.publics[c.a0].col != (uint32_t)(&c - &p.cols[0]))
+        if (c.field > 1 || c.a0 >= p.publics.size() || p.publics[c.a0].col != (uint32_t)(&c - &p.cols[0]))
           bad("public-input column does not match its record");
+        for (auto f : p.publics[c.a0].fields)
+          if (f > 1 || (f == 1 && c.field != 1)) bad("public-input entry field (extension entries need an extension column)");
         break;
       default:
         bad("unknown column source");
@@ -366,6 +375,7 @@ inline PlanData parse_plan(const uint32_t* words, size_t n_words) {
     uint32_t k = r.next();
     if (k > (1u << 22)) throw std::runtime_error("plan: too many public-input entries");
     for (uint32_t j = 0; j < k; ++j) pc.rows.push_back(r.next());
+    for (uint32_t j = 0; j < k; ++j) pc.fields.push_back(r.next());
   }
   for (auto& c : p.cols) p.n_setup += c.src == SRC_SETUP;
   uint32_t nt = r.next();

@@ -61,10 +61,13 @@ inline std::vector<uint32_t> public_input_words(const PlanData& d, const std::ve
     w.insert(w.end(), a.begin(), a.end());
     w.insert(w.end(), m.begin(), m.end());
     w.push_back(M((uint32_t)pc.rows.size()));
+    size_t o = 0;  // entry e's value words in vals[i]
     for (size_t e = 0; e < pc.rows.size(); ++e) {
       w.push_back(M(pc.rows[e]));
-      w.push_back(0u);  // field.Base
-      w.push_back(vals[i][e]);
+      w.push_back(M(pc.fields[e]));  // field.Base 0, field.Ext 1
+      const size_t k = pc.fields[e] ? 6 : 1;
+      w.insert(w.end(), vals[i] + o, vals[i] + o + k);
+      o += k;
     }
   }
   return w;

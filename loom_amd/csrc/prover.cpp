@@ -1289,7 +1289,7 @@ void Job::upload_inputs() {
   for (size_t i = 0; i < npub; ++i)
     for (int b = 0; b < B_; ++b) {
       pub_off[i * B_ + b] = words;
-      words += d_.publics[i].rows.size();
+      words += d_.publics[i].words();
     }
   uint32_t* h = ctx_.pinned_in(words * 4, lane_);
   parallel_for((int)(B_ * (nin + d_.perms.size())), [&](int i) {
@@ -1300,7 +1300,7 @@ void Job::upload_inputs() {
   });
   for (size_t i = 0; i < npub; ++i)
     for (int b = 0; b < B_; ++b)
-      std::memcpy(h + pub_off[i * B_ + b], in_[b].columns[nin + i], d_.publics[i].rows.size() * 4);
+      std::memcpy(h + pub_off[i * B_ + b], in_[b].columns[nin + i], d_.publics[i].words() * 4);
   in_arena_ = dev(words);
   LG_HIP(hipMemcpyAsync(in_arena_.as<void>(), h, words * 4, hipMemcpyHostToDevice, s_));
   // columns of one module size share a scatter launch
@@ -1342,15 +1342,20 @@ void Job::public_inputs(const std::vector<std::vector<const uint32_t*>>& vals_de
     std::vector<uint32_t*> dst(B_);
     for (int b = 0; b < B_; ++b) dst[b] = cb.lag(b, 0);
     uint32_t* const* dst_d = const_cast<uint32_t* const*>(stage(dst));
-    launch_fill(dst_d, (int)cb.n, 0u, B_, s_);
-    if (!pc.rows.empty())
-      launch_scatter_rows(stage(vals_dev[i]), stage(pc.rows), (int)pc.rows.size(), dst_d, B_, s_);
+    // every limb of the column is zero but the entries; value word j goes to
+    // limb l of its row (limb arrays of n words each; a base entry in an
+    // extension column keeps its other limbs zero: the lift)
+    launch_fill(dst_d, (int)(cb.n * cb.limbs), 0u, B_, s_);
+    std::vector<uint32_t> at;
+    for (size_t e = 0; e < pc.rows.size(); ++e)
+      for (uint32_t l = 0; l < (pc.fields[e] ? 6u : 1u); ++l) at.push_back(l * cb.n + pc.rows[e]);
+    if (!at.empty()) launch_scatter_rows(stage(vals_dev[i]), stage(at), (int)at.size(), dst_d, B_, s_);
   }
   // the values on the host: the caller's arrays, or a copy of the device ones
   std::vector<std::vector<uint32_t>> hv(npub * B_);
   for (size_t i = 0; i < npub; ++i)
     for (int b = 0; b < B_; ++b) {
-      const size_t k = d_.publics[i].rows.size();
+      const size_t k = d_.publics[i].words();
       hv[i * B_ + b].resize(k);
       if (!k) continue;
       if (in_[b].device) {

@@ -586,8 +586,18 @@ void check_buses_and_air(const Shape& S, const VerifyInput& in, ProofCtx& P) {
           } else if (c.src == SRC_PUBLIC) {  // sum of v_e L_{row_e}(zeta) (verifier/hookFns.go:42-64)
             const PublicCol& pc = d.publics.at(c.a0);
             E6 acc = e6_zero();
-            for (size_t e = 0; e < pc.rows.size(); ++e)
-              acc = e6_add(acc, lagrange_at(zeta, zn1, n_inv, m.log_n, pc.rows[e] % m.N, in.publics[c.a0][e]));
+            const uint32_t* pv = in.publics[c.a0];
+            for (size_t e = 0; e < pc.rows.size(); ++e) {
+              if (pc.fields[e]) {  // an extension entry: six limbs
+                E6 val;
+                for (int l = 0; l < 6; ++l) val.v[l] = pv[l];
+                acc = e6_add(acc, e6_mul(lagrange_at(zeta, zn1, n_inv, m.log_n, pc.rows[e] % m.N, KB_R), val));
+                pv += 6;
+              } else {
+                acc = e6_add(acc, lagrange_at(zeta, zn1, n_inv, m.log_n, pc.rows[e] % m.N, *pv));
+                pv += 1;
+              }
+            }
             v[i] = acc;
           } else {
             auto it = val.find({x.a, x.b});

@@ -126,6 +126,13 @@ def PublicInput(name):
     return Leaf(VERIFIER, name, 0, BASE, PUBLIC_INPUT_HOOK)
 
 
+def PublicInputExt(name):
+    """A public-input column declared in the extension field
+    (expr.PublicInputExt, ast.go:234-239): its entries may be E6 values
+    (public.Entry.SetExt); base entries are lifted."""
+    return Leaf(VERIFIER, name, 0, EXT, PUBLIC_INPUT_HOOK)
+
+
 def Challenge(name):
     return Leaf(CHALLENGE, name, 0, EXT)
 

@@ -25,9 +25,10 @@ Word stream (u32, little endian):
   n_exposed, then per exposed value (sorted by name): column, name
             (byte length, bytes packed 4 per word, little endian)
   n_public, then per public-input column (sorted by name): column, module,
-            name, module name (strings as above), n_entries, entry rows...
-            (the caller passes each instance's entry values, n_entries words,
-            after its input columns)
+            name, module name (strings as above), n_entries, entry rows...,
+            entry fields... (0 base, 1 extension; the caller passes each
+            instance's entry values after its input columns: one word per
+            base entry, six per extension entry, in row order)
   n_trees, then per tree: kind (0 trace, 1 AIR, 2 setup), n_groups, per group: module, N,
             nb, base cols..., ne, ext cols..., shift lists (base polys then
             ext polys; each: count, shifts...)
@@ -56,7 +57,10 @@ class Plan:
         statement's public.Inputs fix which rows are public; their values are
         per-instance inputs)."""
         self.program = program
-        self.public_rows = {k: [int(i) for i in v] for k, v in (public_rows or {}).items()}
+        # entry rows of each public input, each a row (field.Base entry) or a
+        # (row, field) pair (field 1: an extension entry)
+        self.public_rows = {k: sorted((int(e[0]), int(e[1])) if isinstance(e, (tuple, list)) else (int(e), 0)
+                                      for e in v) for k, v in (public_rows or {}).items()}
         self.layout = PR.build_layout(program)
         self.num_queries = num_queries
         self.modules = sorted(program.modules)
@@ -91,7 +95,10 @@ class Plan:
         for name, mn in getattr(prog, "public_columns", []):
             if name not in self.public_rows:
                 raise KeyError("public input %r has no entry rows" % name)
-            self._add(name, E.BASE, self.mod_idx[mn], SRC_PUBLIC, len(self.public))
+            f = E.EXT if name in getattr(prog, "public_ext", []) else E.BASE
+            if f == E.BASE and any(fe for _, fe in self.public_rows[name]):
+                raise ValueError("public input %r has extension entries: declare it with PublicInputExt" % name)
+            self._add(name, f, self.mod_idx[mn], SRC_PUBLIC, len(self.public))
             self.public.append(name)
         gen = {}
         for mn in self.modules:
@@ -260,9 +267,9 @@ class Plan:
             w += [self.col_idx[name], self.mod_idx[mn]]
             string(name)
             string(mn)
-            rows = sorted(self.public_rows[name])
-            w.append(len(rows))
-            w += rows
+            ents = self.public_rows[name]
+            w.append(len(ents))
+            w += [r for r, _ in ents] + [f for _, f in ents]
         w.append(len(lay.trees))
         for t in lay.trees:
             w += [{"trace": 0, "air": 1, "setup": 2}[t.kind], len(t.groups)]

@@ -10,6 +10,8 @@ opening shifts (AIR chunks: [0]).
 """
 from dataclasses import dataclass, field
 
+import numpy as np
+
 from . import expr as E
 from .board import challenge_name
 
@@ -143,19 +145,36 @@ def _string_elements(tag, s):
     return [tag, len(b)] + [int.from_bytes(b[i:i + 3], "little") for i in range(0, len(b), 3)]
 
 
+def public_entry_field(v):
+    """field.Base (0) for an int value, field.Ext (1) for 6 limbs."""
+    return 0 if np.isscalar(v) or np.ndim(v) == 0 else 1
+
+
+def public_entry_words(v):
+    """An entry's value elements (appendEntryValueElements,
+    public/public_inputs.go:96-101): the base value, or the 6 E6 limbs."""
+    if public_entry_field(v) == 0:
+        return [int(v) % E.P]
+    vals = [int(x) % E.P for x in v]
+    if len(vals) != 6:
+        raise ValueError("an extension public-input entry has 6 limbs")
+    return vals
+
+
 def public_input_elements(public_inputs):
-    """public.Inputs.TranscriptElements (public/public_inputs.go:50-74) for
-    base-field entries: [PUBI, #inputs], then per input in name order the
-    tagged name and module, #entries and (index, field 0, value) per entry in
-    index order.  public_inputs: name -> (module, [(index, value), ...])."""
+    """public.Inputs.TranscriptElements (public/public_inputs.go:50-74):
+    [PUBI, #inputs], then per input in name order the tagged name and module,
+    #entries and (index, field, value elements) per entry in index order
+    (field 0: one base value; field 1: six E6 limbs).  public_inputs: name ->
+    (module, [(index, value), ...]), value an int or 6 limbs."""
     out = [PUBI_TAG, len(public_inputs)]
     for name in sorted(public_inputs):
         module, entries = public_inputs[name]
         out += _string_elements(PUBI_TAG, name) + _string_elements(PUBI_TAG, module)
-        ents = sorted((int(i), int(v) % E.P) for i, v in entries)
+        ents = sorted(((int(i), v) for i, v in entries), key=lambda e: e[0])
         out.append(len(ents))
         for i, v in ents:
-            out += [i, 0, v]
+            out += [i, public_entry_field(v)] + public_entry_words(v)
     return out
 
 

@@ -131,11 +131,13 @@ class Prepared:
     def __init__(self, program, trace, num_queries=PR.NUM_QUERIES_DEFAULT, public_inputs=None):
         public_inputs = dict(public_inputs or {})
         self.public_inputs = public_inputs
-        self.plan = Plan(program, num_queries, {k: [i for i, _ in v[1]] for k, v in public_inputs.items()})
+        self.plan = Plan(program, num_queries, {k: [(int(i), PR.public_entry_field(v)) for i, v in e[1]]
+                                                for k, e in public_inputs.items()})
         self.words = self.plan.words()
         self.columns = [to_mont(trace[name]) for name in self.plan.input_names()]
         for name in self.plan.public:
-            self.columns.append(to_mont([v for _, v in sorted((int(i), int(v)) for i, v in public_inputs[name][1])]))
+            ents = sorted(((int(i), v) for i, v in public_inputs[name][1]), key=lambda e: e[0])
+            self.columns.append(to_mont([x for _, v in ents for x in PR.public_entry_words(v)]))
         self.perms = self.plan.perm_arrays()
         self.setup = [to_mont(trace[name]) for name in self.plan.setup]
         # the plan's identity, computed once (per call it would cost more than

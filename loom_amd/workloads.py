@@ -349,3 +349,37 @@ def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
     else:
         raise ValueError(kind)
     return B.compile_program(b), tr
+
+
+def ext_public_instance(log_n, seed, bad=False):
+    """A module with a base public input "PB" (expr.PublicInput, base
+    entries) and an extension public input "PX" (expr.PublicInputExt) whose
+    entries are E6 values except one base entry (lifted), used in relations
+    that hold on every row: PB * Q = 0 and PX * Q = 0 with the base column Q
+    zero at every public row and random elsewhere, and A * B - C = 0.
+    Returns (program, trace, public_inputs); public_inputs: name -> (module,
+    [(row, value)]), value an int or 6 limbs; the rows depend on log_n only.
+    bad: Q is non-zero at one public row of PX (the relation fails there)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = 1 << log_n
+    b = B.Builder()
+    m = B.Module("pub", n)
+    b.add_module(m)
+    m.assert_zero(E.Col("A") * E.Col("B") - E.Col("C"))
+    m.assert_zero(E.PublicInput("PB") * E.Col("Q"))
+    m.assert_zero(E.PublicInputExt("PX") * E.Col("Q"))
+    A = rng.integers(0, P, size=n, dtype=np.uint64)
+    Bv = rng.integers(0, P, size=n, dtype=np.uint64)
+    C = A * Bv % np.uint64(P)
+    Q = rng.integers(1, P, size=n, dtype=np.uint64)
+    # the public rows are part of the statement's shape (the plan): they
+    # depend on log_n only, so statements of one size batch together
+    rows = sorted(int(x) for x in np.random.Generator(np.random.PCG64(log_n)).choice(n, size=min(n, 6), replace=False))
+    rb, rx = rows[:2], rows[2:]
+    Q[rows] = 0
+    if bad:
+        Q[rx[0]] = 5
+    pb = [(r, int(rng.integers(0, P))) for r in rb]
+    px = [(r, [int(x) for x in rng.integers(0, P, size=6)]) for r in rx[:-1]] + [(rx[-1], int(rng.integers(0, P)))]
+    tr = dict(A=A, B=Bv, C=C, Q=Q)
+    return B.compile_program(b), tr, {"PB": ("pub", pb), "PX": ("pub", px)}

@@ -141,13 +141,23 @@ class Prover:
         self.fs_prev = fs_prev
         self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items()}
         self.pi = dict(public_inputs or {})
+        self.pi_ext = {}
         for name, (mod, entries) in self.pi.items():
-            col = np.zeros(program.modules[mod].N, dtype=np.uint64)
-            for i, v in entries:
-                col[int(i)] = int(v) % P
-            self.base[name] = col
+            if name not in getattr(program, "public_ext", []):
+                if any(PR.public_entry_field(v) for _, v in entries):
+                    raise ValueError("public input %r has extension entries: declare it with PublicInputExt" % name)
+                col = np.zeros(program.modules[mod].N, dtype=np.uint64)
+                for i, v in entries:
+                    col[int(i)] = int(v) % P
+                self.base[name] = col
+            else:  # an extension column: base entries lifted (prover.go:255-268)
+                col = np.zeros((program.modules[mod].N, 6), dtype=np.uint64)
+                for i, v in entries:
+                    w = PR.public_entry_words(v)
+                    col[int(i), :len(w)] = w
+                self.pi_ext[name] = col
         self.setup_roots = []
-        self.ext = {}
+        self.ext = dict(self.pi_ext)
         self.air_ext = {}
         self.committed = [None] * len(self.layout.trees)
         self.commitments = []

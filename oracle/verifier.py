@@ -312,9 +312,12 @@ def verify(program, num_queries, words, check_queries=None, setup_roots=(), publ
                 acc = F.e6_zero()
                 for i, v in entries:
                     wi = pow(w, int(i) % n, P)
-                    num = F.e6_scale(zn1, wi * F.inv(n) % P * (int(v) % P) % P)
+                    num = F.e6_scale(zn1, wi * F.inv(n) % P)
+                    val = np.zeros(6, dtype=np.uint64)
+                    ws = PR.public_entry_words(v)
+                    val[:len(ws)] = ws
                     den = F.e6_sub(zeta, F.e6_from_base(np.array([wi], dtype=np.uint64))[0])
-                    acc = F.e6_add(acc, F.e6_mul(num, F.e6_inv(den)))
+                    acc = F.e6_add(acc, F.e6_mul(F.e6_mul(num, val), F.e6_inv(den)))
                 return acc[None, :]
             check(x.type in (E.COMMITTED, E.SETUP), "unsupported leaf in a relation")
             return vals[(x.name, x.shift % n)][None, :]
