@@ -31,7 +31,7 @@
  *   - SkipFRI: lg_plan_set_options(plan, LG_PROVE_SKIP_FRI).
  *   - WithFriOption(fri.WithGrinding(bits)): lg_plan_set_grinding.
  *   - setup.ProvingKey: lg_plan_set_setup commits the setup columns once per
- *     plan and keeps them resident (base-field setup columns).
+ *     plan and keeps them resident (expr.Setup and expr.ExtSetup columns).
  *   - public.Inputs: entries at rows the plan lists, base-field values or
  *     (for inputs declared with expr.PublicInputExt) E6 values; each
  *     instance passes the values after its input columns.
@@ -39,8 +39,7 @@
  *     challenges come back in `transcripts`); one with computed challenges is
  *     lg_prove_chained (its last digest).
  *   - Refused (a plan or call that needs them is rejected with an error):
- *     extension-field setup columns, extension entries of a public input
- *     declared in the base field,
+ *     extension entries of a public input declared in the base field,
  *     WithNewTranscriptHasher (the transcript hasher is the Poseidon2 sponge),
  *     WithFriOption other than grinding, other hash backends.
  *   - Lookups: every arguments/lookup.go form (Range, Lookup, LookupTuple,
@@ -122,8 +121,10 @@ int lg_plan_num_inputs(const lg_plan* plan, size_t* n_columns, size_t* n_perms);
 
 /* setup.Setup (setup/setup.go:68-124) for a plan with setup columns
  * (expr.Setup leaves, e.g. PLONK selectors, plonk_gnark_utils.go:217-219):
- * setup_columns[k] is setup column k in plan setup order (host memory, N words
- * of its module, Montgomery).  The columns are RS-encoded and committed into
+ * setup_columns[k] is setup column k in plan setup order (host memory,
+ * Montgomery): N words of its module, or for an extension column
+ * (expr.ExtSetup) its six limb arrays of N words one after another (limb
+ * order B0.A0 .. B2.A1).  The columns are RS-encoded and committed into
  * the setup tree (one group per module) once; columns, LDE and tree stay in
  * device memory for every later prove of the plan (the proving key), which
  * binds the root to its first challenge and opens the tree at the queries.

@@ -202,7 +202,7 @@ inline void validate_plan(const PlanData& p) {
         if (c.field != 0 || c.a0 > N) bad("range table larger than its module");
         break;
       case SRC_SETUP:
-        if (c.field != 0 || c.a0 >= p.n_setup) bad("setup column index out of range (base-field setup columns only)");
+        if (c.field > 1 || c.a0 >= p.n_setup) bad("setup column index out of range");
         break;
       case SRC_PUBLIC:
         if (c.field > 1 || c.a0 >= p.publics.size() || p.publics[c.a0].col != (uint32_t)(&c - &p.cols[0]))
@@ -249,15 +249,15 @@ inline void validate_plan(const PlanData& p) {
     if (t.kind == TREE_SETUP && ti != 0) bad("the setup tree must come first");
     if (t.groups.empty()) bad("tree without groups");
     for (auto& g : t.groups)
-      for (auto c : g.base)
-        if (c < nc && ((p.cols[c].src == SRC_SETUP) != (t.kind == TREE_SETUP)))
-          bad("setup columns belong to the setup tree only");
+      for (int rail = 0; rail < 2; ++rail)
+        for (auto c : rail ? g.ext : g.base)
+          if (c < nc && ((p.cols[c].src == SRC_SETUP) != (t.kind == TREE_SETUP)))
+            bad("setup columns belong to the setup tree only");
     if (t.kind == TREE_SETUP)
-      for (auto& g : t.groups) {
-        if (!g.ext.empty()) bad("extension-field setup columns are not supported");
-        for (auto c : g.base)
-          if (c < nc && p.cols[c].a0 < p.n_setup) ++setup_seen[p.cols[c].a0];
-      }
+      for (auto& g : t.groups)
+        for (int rail = 0; rail < 2; ++rail)
+          for (auto c : rail ? g.ext : g.base)
+            if (c < nc && p.cols[c].a0 < p.n_setup) ++setup_seen[p.cols[c].a0];
   }
   for (auto v : setup_seen)
     if (v != 1) bad("every setup column must be committed once in the setup tree");

@@ -1168,15 +1168,18 @@ void Job::commit_setup(const uint32_t* const* host_cols, SetupData& out) {
     const ColInfo& ci = d_.cols[c];
     if (ci.src != SRC_SETUP) continue;
     ColBuf& cb = cols_[c];
-    cb.limbs = 1;
+    cb.limbs = ci.field ? 6 : 1;  // an extension column: 6 limb arrays of N words (ExtSetup)
     cb.n = d_.modules[ci.module].N;
     cb.committed = true;
-    cb.blk = (size_t)5 * cb.n;
+    cb.blk = (size_t)5 * cb.limbs * cb.n;
     out.cols[ci.a0].alloc(cb.blk * 4);
     cb.buf = DevBuf::view(out.cols[ci.a0].as<void>(), cb.blk * 4);
-    LG_HIP(hipMemcpyAsync(cb.lag(0, 0), host_cols[ci.a0], (size_t)cb.n * 4, hipMemcpyHostToDevice, s_));
+    LG_HIP(hipMemcpyAsync(cb.lag(0, 0), host_cols[ci.a0], (size_t)cb.limbs * cb.n * 4, hipMemcpyHostToDevice, s_));
   }
-  for (auto& g : ti.groups) ids.insert(ids.end(), g.base.begin(), g.base.end());
+  for (auto& g : ti.groups) {
+    ids.insert(ids.end(), g.base.begin(), g.base.end());
+    ids.insert(ids.end(), g.ext.begin(), g.ext.end());
+  }
   lde_columns(ids);
   std::vector<std::vector<const uint32_t*>> gcols;
   std::vector<std::pair<int, int>> shape;
@@ -1184,8 +1187,10 @@ void Job::commit_setup(const uint32_t* const* host_cols, SetupData& out) {
   for (auto& g : ti.groups) {
     std::vector<const uint32_t*> cp;
     for (auto c : g.base) cp.push_back(cols_[c].lde(0, 0));
+    for (auto c : g.ext)
+      for (int l = 0; l < 6; ++l) cp.push_back(cols_[c].lde(0, l));
     gcols.push_back(cp);
-    shape.push_back({(int)g.base.size(), 0});
+    shape.push_back({(int)g.base.size(), (int)g.ext.size()});
     rows.push_back((int)g.N * 4);
   }
   trees_.resize(d_.trees.size());

@@ -120,6 +120,12 @@ def Setup(name, shift=0):
     return Leaf(SETUP, name, shift, BASE)
 
 
+def ExtSetup(name, shift=0):
+    """A precommitted setup column of the extension field (expr.ExtSetup,
+    ast.go:215-217): its values are E6."""
+    return Leaf(SETUP, name, shift, EXT)
+
+
 def PublicInput(name):
     """A public-input column (expr.PublicInput, ast.go:230-232): a verifier
     column the prover materialises from the statement's entries."""

@@ -88,8 +88,6 @@ class Plan:
             for g in t.groups:
                 for name in g.base + g.ext:
                     f = prog.column_fields.get(name, E.BASE)
-                    if f != E.BASE:
-                        raise NotImplementedError("extension-field setup column %r" % name)
                     self._add(name, f, self.mod_idx[g.module], SRC_SETUP, len(self.setup))
                     self.setup.append(name)
         for name, mn in getattr(prog, "public_columns", []):

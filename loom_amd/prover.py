@@ -139,7 +139,10 @@ class Prepared:
             ents = sorted(((int(i), v) for i, v in public_inputs[name][1]), key=lambda e: e[0])
             self.columns.append(to_mont([x for _, v in ents for x in PR.public_entry_words(v)]))
         self.perms = self.plan.perm_arrays()
-        self.setup = [to_mont(trace[name]) for name in self.plan.setup]
+        # setup columns: N words, or for an extension column (N x 6 limbs) the
+        # 6 limb arrays one after another (lg_plan_set_setup)
+        self.setup = [to_mont(np.asarray(trace[name]).T.reshape(-1) if np.ndim(trace[name]) == 2 else trace[name])
+                      for name in self.plan.setup]
         # the plan's identity, computed once (per call it would cost more than
         # the batched prove's host work)
         self.words_key = self.words.tobytes()

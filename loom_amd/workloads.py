@@ -383,3 +383,33 @@ def ext_public_instance(log_n, seed, bad=False):
     px = [(r, [int(x) for x in rng.integers(0, P, size=6)]) for r in rx[:-1]] + [(rx[-1], int(rng.integers(0, P)))]
     tr = dict(A=A, B=Bv, C=C, Q=Q)
     return B.compile_program(b), tr, {"PB": ("pub", pb), "PX": ("pub", px)}
+
+
+def ext_setup_instance(log_n, setup_seed, witness_seed):
+    """A module with a base setup column S1 (expr.Setup) and an extension
+    setup column SX (expr.ExtSetup, N x 6 limbs), both committed once in the
+    setup tree: relations A*B - C = 0, S1*A - D = 0 and SX*Z = 0, with SX
+    non-zero on a few rows and the witness column Z zero there and random
+    elsewhere.  The setup columns depend on setup_seed only (one proving key
+    for many witnesses).  Returns (program, trace) with the setup columns in
+    the trace (SX as N x 6 limbs)."""
+    n = 1 << log_n
+    rs = np.random.Generator(np.random.PCG64(setup_seed))
+    rw = np.random.Generator(np.random.PCG64(witness_seed))
+    b = B.Builder()
+    m = B.Module("ks", n)
+    b.add_module(m)
+    m.assert_zero(E.Col("A") * E.Col("B") - E.Col("C"))
+    m.assert_zero(E.Setup("S1") * E.Col("A") - E.Col("D"))
+    m.assert_zero(E.ExtSetup("SX") * E.Col("Z"))
+    S1 = rs.integers(0, P, size=n, dtype=np.uint64)
+    SX = np.zeros((n, 6), dtype=np.uint64)
+    rows = rs.choice(n, size=max(1, n // 8), replace=False)
+    SX[rows] = rs.integers(0, P, size=(len(rows), 6), dtype=np.uint64)
+    A = rw.integers(0, P, size=n, dtype=np.uint64)
+    Bv = rw.integers(0, P, size=n, dtype=np.uint64)
+    Z = rw.integers(1, P, size=n, dtype=np.uint64)
+    Z[rows] = 0
+    pp = np.uint64(P)
+    tr = dict(A=A, B=Bv, C=A * Bv % pp, D=S1 * A % pp, Z=Z, S1=S1, SX=SX)
+    return B.compile_program(b), tr

@@ -139,7 +139,10 @@ class Prover:
         self.nq = num_queries
         self.grinding = int(grinding)
         self.fs_prev = fs_prev
-        self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items()}
+        # base columns: N values; extension columns (ExtSetup setup columns):
+        # N x 6 limbs
+        self.base = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items() if np.ndim(v) == 1}
+        trace_ext = {k: np.asarray(v, dtype=np.uint64) % U64(P) for k, v in trace.items() if np.ndim(v) == 2}
         self.pi = dict(public_inputs or {})
         self.pi_ext = {}
         for name, (mod, entries) in self.pi.items():
@@ -158,6 +161,7 @@ class Prover:
                 self.pi_ext[name] = col
         self.setup_roots = []
         self.ext = dict(self.pi_ext)
+        self.ext.update(trace_ext)
         self.air_ext = {}
         self.committed = [None] * len(self.layout.trees)
         self.commitments = []
