@@ -41,7 +41,8 @@
  *   - Refused (a plan or call that needs them is rejected with an error):
  *     extension entries of a public input declared in the base field,
  *     WithNewTranscriptHasher (the transcript hasher is the Poseidon2 sponge),
- *     WithFriOption other than grinding, other hash backends.
+ *     other hash backends.  (fri.WoFullDomainAllocation, the other FRI
+ *     option, changes only the reference's memory use: nothing to carry.)
  *   - Lookups: every arguments/lookup.go form (Range, Lookup, LookupTuple,
  *     LookupUnion(Tuple), CLookup(Tuple), CLookupUnion(Tuple)): tuples of up
  *     to 8 base-field expressions, optional selectors, targets in several
