@@ -569,6 +569,13 @@ __device__ __forceinline__ uint32_t quad_m4(uint32_t x) {
   return kb_umin(r, r - KB_P);
 }
 
+// (t + m p) / 2^32 for t < 2^63 + 2^57, reduced below p (t's quotient is below 2p)
+__device__ __forceinline__ uint32_t coop_redc(uint64_t t) {
+  const uint32_t m = (uint32_t)t * 0x7effffffu;
+  const uint32_t r = (uint32_t)((t + (uint64_t)m * KB_P) >> 32);
+  return kb_umin(r, r - KB_P);
+}
+
 struct P2Coop16 {
   uint32_t rka[6], rkb[6];  // full-round keys of elements l and l + 12 (zeros on lanes 12..15)
   uint32_t da, db;          // internal diagonal of elements l and l + 12
@@ -612,8 +619,12 @@ struct P2Coop16 {
       a = (c & l0mask) | (a & ~l0mask);
       // inactive lanes: a = b = 0, da = db = 0 and the sum masked to 0
       const uint32_t s = row_sum(kb_add(a, b)) & amask;
-      a = kb_add(s, kb_mul(a, da));
-      b = kb_add(s, kb_mul(b, db));
+      // s + x d as one Montgomery reduction of x d + s R (R = 2^32 mod p, so
+      // the quotient is x d R^-1 + s): x d < p^2, s R < 2^56, and the sum
+      // plus m p stays below 2^64; the quotient is below 2p
+      const uint64_t sr = (uint64_t)s * KB_R;
+      a = coop_redc((uint64_t)a * da + sr);
+      b = coop_redc((uint64_t)b * db + sr);
     }
 #pragma unroll
     for (int r = 3; r < 6; ++r) {
