@@ -277,6 +277,8 @@ def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
       some rows (a later copy of that pair then takes its weight), else
       values 1..5; two unselected-in-target source rows of one pair whose
       selectors sum to 0 mod p (no error: their weight cancels);
+    * "self":   as "tuple" with the table in module "main" itself (2^log_n
+      rows, one module);
     * "union":  LookupUnion of columns of two source modules (2^log_n and
       2^(log_n-1) rows) into columns of two target modules (2^log_t and
       2^(log_t-1) rows).
@@ -291,9 +293,12 @@ def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
         b.add_module(m)
         return m
 
-    if kind in ("tuple", "cond"):
+    if kind == "self":  # the table in the sources' own module (one module: one DEEP class)
+        nt = n
+    if kind in ("tuple", "cond", "self"):
         module("main", n)
-        module("tab", nt)
+        if kind != "self":
+            module("tab", nt)
         X = rng.integers(0, P, size=nt, dtype=np.uint64)
         Y = rng.integers(0, P, size=nt, dtype=np.uint64)
         dup = 8 + rng.choice(nt - 9, size=nt // 8, replace=False)
@@ -306,8 +311,8 @@ def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
         A, Bv = X[r].copy(), Y[r].copy()
         tr.update(X=X, Y=Y, A=A, B=Bv)
         S = B.Table("main", [E.Col("A"), E.Col("B")])
-        T = B.Table("tab", [E.Col("X"), E.Col("Y")])
-        if kind == "tuple":
+        T = B.Table("main" if kind == "self" else "tab", [E.Col("X"), E.Col("Y")])
+        if kind in ("tuple", "self"):
             B.lookup_tuple(b, S, T)
         else:
             SA = rng.choice(np.array([0, 1, 2], dtype=np.uint64), size=n, p=[0.2, 0.6, 0.2])

@@ -70,3 +70,20 @@ def test_missing_tuple_fails_alone(ctx, kind):
     assert res[1] is None
     for k, (prog, tr) in ((0, good[0]), (2, good[1])):
         assert np.array_equal(res[k], _oracle(prog, tr, 8))
+
+
+def test_general_lookup_graph_replay(ctx):
+    """A tuple lookup within one module (one DEEP class): the device phase
+    with the general multiplicity kernels is recorded into a HIP graph on the
+    second call and replayed on the third; every call equals the oracle."""
+    from loom_amd import workloads, prover
+    inst = [workloads.lookup_instance("self", 70 + s, log_n=7) for s in range(2)]
+    items = [prover.Prepared(p, t, 12) for p, t in inst]
+    want = [_oracle(p, t, 12) for p, t in inst]
+    for rep in range(3):
+        log("general lookups, one size class, call %d (eager, recorded, replayed)" % rep)
+        got = prover.prove_prepared(ctx, items)
+        for k in range(2):
+            assert np.array_equal(got[k], want[k]), (rep, k)
+    st = prover.graph_stats(ctx, items)
+    assert st["fallbacks"] == 0 and st["replays"] >= 1, st
