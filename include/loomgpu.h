@@ -232,8 +232,8 @@ int lg_ctx_set_timing(lg_ctx* ctx, int on);
 int lg_ctx_timing_report(lg_ctx* ctx, char* buf, size_t cap, size_t* len);
 
 /* Phase timings of the last lg_prove on this plan (seconds): execute-steps,
- * compute-air-quotients, pcs-open (the phases prover.WithPhaseCallback
- * reports, prover.go:682-716). */
+ * compute-air-quotients, pcs-open (pcs-claimed-values-only under SkipFRI):
+ * the phases prover.WithPhaseCallback reports, prover.go:682-716. */
 int lg_plan_last_phases(const lg_plan* plan, double* out3);
 
 /* ---- serving ---- */

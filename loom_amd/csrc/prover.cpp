@@ -828,6 +828,8 @@ class Job {
     // after an error work may still be queued on the stream: drain it before
     // the buffers go back to the shared pool
     (void)hipStreamSynchronize(s_);
+    for (hipEvent_t e : ph_ev_)
+      if (e) (void)hipEventDestroy(e);
     const double t_sync = marks_.empty() ? 0.0 : now_s();
     ctx_.pool.put(std::move(chal_dev_));
     ctx_.pool.put(std::move(fs_dig_));
@@ -982,15 +984,43 @@ class Job {
   }
   std::vector<std::pair<std::string, double>> marks_;
   double t0_run_ = 0;
+  // Phases that end inside the device region do not wait for the device: an
+  // eager prove records an event on its stream there and reports the device
+  // time between the events once the first phase outside the region has
+  // synchronised; that phase gets the rest of the wall time. A recorded or
+  // replayed region reports 0 for its phases (the whole region lands in the
+  // next phase).
+  hipEvent_t ph_ev_[3] = {nullptr, nullptr, nullptr};
+  std::vector<const char*> ph_pending_;
+  void phase_start() {
+    if (!cb_ || mode_ != EAGER) return;
+    LG_HIP(hipEventCreate(&ph_ev_[0]));
+    LG_HIP(hipEventRecord(ph_ev_[0], s_));
+  }
   void phase(const char* name, double& t0) {
-    if (cb_ && in_region_) {  // a recorded device phase is timed as a whole (pcs-open)
-      cb_(name, 0.0);
+    if (cb_ && in_region_) {
+      if (mode_ == EAGER && ph_ev_[0] && ph_pending_.size() < 2) {
+        hipEvent_t& e = ph_ev_[1 + ph_pending_.size()];
+        LG_HIP(hipEventCreate(&e));
+        LG_HIP(hipEventRecord(e, s_));
+        ph_pending_.push_back(name);
+      } else {
+        cb_(name, 0.0);
+      }
       return;
     }
     if (cb_) {
       sync();
       double t = now_s();
-      cb_(name, t - t0);
+      double dev = 0;
+      for (size_t i = 0; i < ph_pending_.size(); ++i) {
+        float ms = 0;
+        LG_HIP(hipEventElapsedTime(&ms, ph_ev_[i], ph_ev_[i + 1]));
+        cb_(ph_pending_[i], ms * 1e-3);
+        dev += ms * 1e-3;
+      }
+      ph_pending_.clear();
+      cb_(name, std::max(0.0, t - t0 - dev));
       t0 = t;
     }
   }
@@ -2452,6 +2482,7 @@ void Job::run() {
   double t0 = now_s();
   t0_run_ = t0;
   LG_HIP(hipSetDevice(ctx_.device));
+  phase_start();
   alloc_columns();
   upload_inputs();
   trace("upload");
@@ -2559,7 +2590,8 @@ void Job::run() {
   trace("air commit + zeta");
   phase("compute-air-quotients", t0);
   open();
-  phase("pcs-open", t0);
+  // prover.go:692-716: SkipFRI reports the claimed-values-only opening
+  phase(skip_fri_ ? "pcs-claimed-values-only" : "pcs-open", t0);
   if (!marks_.empty()) {
     std::string line = "[loomgpu] lane " + std::to_string(lane_) + " marks (ms):";
     char buf[96];

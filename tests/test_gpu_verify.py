@@ -278,3 +278,29 @@ def test_prove_skip_fri(ctx):
     again = prover.prove_prepared(ctx, items)
     for f, g in zip(full, again):
         assert np.array_equal(f, g)
+
+
+@pytest.mark.gpu
+def test_phase_timings(ctx):
+    """prover.WithPhaseCallback (prover/prover.go:99-108, 676-716): an eager
+    prove of modules of different sizes (no recorded graph) reports three
+    positive phase durations, execute-steps, compute-air-quotients and
+    pcs-open, whose sum stays within the call's wall time; under SkipFRI the
+    third slot is pcs-claimed-values-only and the proof is still the same."""
+    import time
+    from loom_amd import prover, workloads
+    log("phase timings: uneven (9, 11, 8), 16 queries")
+    prog, tr = workloads.uneven_instance([9, 11, 8], 77)
+    items = [prover.Prepared(prog, tr, 16)]
+    for skip in (False, True):
+        prover.set_options(ctx, items, skip_fri=skip)
+        try:
+            ph = []
+            t0 = time.perf_counter()
+            w = prover.prove_prepared(ctx, items, phases=ph)[0]
+            wall = time.perf_counter() - t0
+        finally:
+            prover.set_options(ctx, items, skip_fri=False)
+        assert w is not None and len(ph) == 3
+        assert all(x > 0 for x in ph), ph
+        assert sum(ph) <= wall, (ph, wall)
