@@ -166,7 +166,7 @@ void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts
 void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s);
 
 // General lookup multiplicities (poly/lookup_multiplicity.go:37-100): tuples
-// of up to LK_MAX_W base-field columns with optional selectors, several
+// of any number of base-field columns with optional selectors, several
 // sources and targets.  Per instance, a hash table keyed by tuple, whose
 // slots hold a representative target row (rows are compared word by word
 // through the tables, so a slot is claimed by one compare-and-swap): the
@@ -175,9 +175,8 @@ void launch_counts_to_field(uint32_t* const* c, int n, int batch, hipStream_t s)
 // selector values, or of ones), and the multiplicity weight / selT written
 // at the first row.  Source tuples found in no target are summed in a second
 // table (with source selectors) or flag the instance's error word at once.
-constexpr int LK_MAX_W = 8;
 struct LkTable {  // one source or target table of one instance
-  const uint32_t* col[LK_MAX_W];  // tuple columns (Montgomery, base field)
+  const uint32_t* const* col;     // w tuple columns (Montgomery, base field), a device array
   const uint32_t* sel;            // selector column, or null (all ones)
   uint32_t* mult;                 // targets: the multiplicity column
   uint32_t n, off;                // rows; first global row index (targets, then sources, counted apart)

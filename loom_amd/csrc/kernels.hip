@@ -1399,10 +1399,12 @@ void launch_histogram(const uint32_t* const* src, int n, uint32_t* const* counts
 }
 
 // ---- general lookup multiplicities
-__device__ __forceinline__ uint32_t lk_hash(const uint32_t* t, int w) {
+// Tuples are read where they lie (row i of table t, one word per column), so
+// their width is not bounded by registers.
+__device__ __forceinline__ uint32_t lk_hash(const LkTable& t, uint32_t i, int w) {
   uint32_t h = 0x9e3779b9u;
   for (int c = 0; c < w; ++c) {
-    h = (h ^ t[c]) * 0x85ebca6bu;
+    h = (h ^ t.col[c][i]) * 0x85ebca6bu;
     h ^= h >> 13;
   }
   h *= 0xc2b2ae35u;
@@ -1416,26 +1418,24 @@ __device__ __forceinline__ int lk_table_of(const LkTable* T, int n, uint32_t g) 
   return k;
 }
 
-__device__ __forceinline__ void lk_row(const LkTable& t, uint32_t i, int w, uint32_t* x) {
-  for (int c = 0; c < w; ++c) x[c] = t.col[c][i];
-}
-
-__device__ __forceinline__ bool lk_row_equal(const LkTable* T, int n, uint32_t g, const uint32_t* x, int w) {
+// row g of tables T (n of them) equals row qi of table q
+__device__ __forceinline__ bool lk_row_equal(const LkTable* T, int n, uint32_t g, const LkTable& q, uint32_t qi, int w) {
   const LkTable& t = T[lk_table_of(T, n, g)];
   const uint32_t i = g - t.off;
   for (int c = 0; c < w; ++c)
-    if (t.col[c][i] != x[c]) return false;
+    if (t.col[c][i] != q.col[c][qi]) return false;
   return true;
 }
 
-// Finds (or with claim, claims) the slot of tuple x among rows of tables T:
-// slot_row holds a representative row + 1, immutable once set, whose words
-// stay readable in its table.  The tables hold at most half as many rows as
-// slots, so a probe meets an empty slot or its tuple within cap steps.
-__device__ __forceinline__ int lk_probe(uint32_t* slot_row, uint32_t cap_log, const LkTable* T, int n, const uint32_t* x,
-                                        int w, uint32_t g_claim, bool claim) {
+// Finds (or with claim, claims) the slot of the tuple at row qi of table q
+// among rows of tables T: slot_row holds a representative row + 1, immutable
+// once set, whose words stay readable in its table.  The tables hold at most
+// half as many rows as slots, so a probe meets an empty slot or its tuple
+// within cap steps.
+__device__ __forceinline__ int lk_probe(uint32_t* slot_row, uint32_t cap_log, const LkTable* T, int n, const LkTable& q,
+                                        uint32_t qi, int w, uint32_t g_claim, bool claim) {
   const uint32_t mask = (1u << cap_log) - 1u;
-  uint32_t slot = lk_hash(x, w) & mask;
+  uint32_t slot = lk_hash(q, qi, w) & mask;
   for (uint32_t it = 0; it <= mask; ++it) {
     uint32_t cur = __hip_atomic_load(&slot_row[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (cur == 0u) {
@@ -1443,7 +1443,7 @@ __device__ __forceinline__ int lk_probe(uint32_t* slot_row, uint32_t cap_log, co
       cur = atomicCAS(&slot_row[slot], 0u, g_claim + 1u);
       if (cur == 0u) return (int)slot;
     }
-    if (lk_row_equal(T, n, cur - 1u, x, w)) return (int)slot;
+    if (lk_row_equal(T, n, cur - 1u, q, qi, w)) return (int)slot;
     slot = (slot + 1u) & mask;
   }
   return -2;  // not reached while the table is at most half full
@@ -1458,9 +1458,7 @@ __global__ __launch_bounds__(256) void k_lookup_targets(const LkTable* __restric
   const LkTable& t = T[lk_table_of(T, nt, g)];
   const uint32_t i = g - t.off;
   if (i >= t.n || (t.sel && t.sel[i] == 0u)) return;
-  uint32_t x[LK_MAX_W];
-  lk_row(t, i, w, x);
-  const int slot = lk_probe(J.slot_row, J.cap_log, T, nt, x, w, g, true);
+  const int slot = lk_probe(J.slot_row, J.cap_log, T, nt, t, i, w, g, true);
   if (slot < 0) {
     atomicOr(J.err, 2u);
     return;
@@ -1480,9 +1478,7 @@ __global__ __launch_bounds__(256) void k_lookup_sources(const LkTable* __restric
   if (i >= src.n) return;
   const uint32_t wt = src.sel ? kb_from_mont(src.sel[i]) : 1u;
   if (wt == 0u) return;
-  uint32_t x[LK_MAX_W];
-  lk_row(src, i, w, x);
-  const int slot = lk_probe(J.slot_row, J.cap_log, T, nt, x, w, 0u, false);
+  const int slot = lk_probe(J.slot_row, J.cap_log, T, nt, src, i, w, 0u, false);
   if (slot >= 0) {
     atomicAdd(&J.slot_w[slot], (unsigned long long)wt);
     return;
@@ -1491,7 +1487,7 @@ __global__ __launch_bounds__(256) void k_lookup_sources(const LkTable* __restric
     atomicOr(J.err, 1u);
     return;
   }
-  const int ms = lk_probe(J.miss_row, J.miss_cap_log, S, ns, x, w, g, true);
+  const int ms = lk_probe(J.miss_row, J.miss_cap_log, S, ns, src, i, w, g, true);
   if (ms < 0) {
     atomicOr(J.err, 2u);
     return;
@@ -1510,9 +1506,7 @@ __global__ __launch_bounds__(256) void k_lookup_assign(const LkTable* __restrict
   if (i >= t.n) return;
   uint32_t m = 0u;
   if (!t.sel || t.sel[i] != 0u) {
-    uint32_t x[LK_MAX_W];
-    lk_row(t, i, w, x);
-    const int slot = lk_probe(J.slot_row, J.cap_log, T, nt, x, w, 0u, false);
+    const int slot = lk_probe(J.slot_row, J.cap_log, T, nt, t, i, w, 0u, false);
     if (slot >= 0 && J.slot_first[slot] == g) {
       m = kb_to_mont((uint32_t)(J.slot_w[slot] % KB_P));
       if (t.sel) m = kb_mul(m, kb_inv(t.sel[i]));
@@ -1531,7 +1525,7 @@ __global__ __launch_bounds__(256) void k_lookup_missing(const LkJob* __restrict_
 void launch_lookup_general(const LkTable* tabs, const LkJob* jobs, int nt, int ns, int w, uint32_t total_t,
                            uint32_t total_s, uint32_t max_cap, uint32_t max_miss_cap, int batch, hipStream_t s) {
   if (batch <= 0) return;
-  if (w < 1 || w > LK_MAX_W || nt < 1 || ns < 1) throw std::runtime_error("launch_lookup_general: bad shape");
+  if (w < 1 || nt < 1 || ns < 1) throw std::runtime_error("launch_lookup_general: bad shape");
   if (!prelaunch()) return;
   const int T = 256;
   if (total_t) {

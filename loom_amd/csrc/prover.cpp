@@ -605,8 +605,8 @@ Plan::Plan(Context* ctx, const uint32_t* words, size_t n) : ctx_(ctx), d_(parse_
           if (st.params.size() < 5) throw std::runtime_error("lookup step: bad parameters");
           const uint32_t ns = st.params[0], nt = st.params[1], w = st.params[2];
           const bool hs = st.params[3] != 0, ht = st.params[4] != 0;
-          if (ns < 1 || nt < 1 || w < 1 || w > (uint32_t)LK_MAX_W || st.params.size() != 5 + ns + nt)
-            throw std::runtime_error("lookup step: bad shape (at most " + std::to_string(LK_MAX_W) + " columns per tuple)");
+          if (ns < 1 || nt < 1 || w < 1 || st.params.size() != 5 + ns + nt)
+            throw std::runtime_error("lookup step: bad shape");
           if (st.ins.size() != (hs ? ns : 0) + (ht ? nt : 0) + (size_t)(ns + nt) * w || st.outs.size() != nt)
             throw std::runtime_error("lookup step: bad inputs or outputs");
           for (uint32_t j = 0; j < ns + nt; ++j)
@@ -2042,6 +2042,16 @@ void Job::step_multiplicity_general(size_t r, size_t s) {
   std::vector<uint32_t*> fill0, fill1;
   std::vector<LkTable> tabs((size_t)B_ * (nt + ns));
   std::vector<LkJob> jobs(B_);
+  // the tuple columns of every table, w per table, in tabs' order
+  std::vector<const uint32_t*> colp(tabs.size() * w);
+  for (int b = 0; b < B_; ++b) {
+    const size_t t0 = (size_t)b * (nt + ns);
+    for (uint32_t k = 0; k < nt; ++k)
+      for (uint32_t c = 0; c < w; ++c) colp[(t0 + k) * w + c] = val[i_tgt + (size_t)k * w + c].ptr(b);
+    for (uint32_t j = 0; j < ns; ++j)
+      for (uint32_t c = 0; c < w; ++c) colp[(t0 + nt + j) * w + c] = val[i_src + (size_t)j * w + c].ptr(b);
+  }
+  const uint32_t* const* colp_d = stage(colp);
   for (int b = 0; b < B_; ++b) {
     LkJob& J = jobs[b];
     J.slot_row = row.as<uint32_t>() + b * cap;
@@ -2055,7 +2065,7 @@ void Job::step_multiplicity_general(size_t r, size_t s) {
     uint32_t off = 0;
     for (uint32_t k = 0; k < nt; ++k) {  // targets
       LkTable& t = tabs[(size_t)b * (nt + ns) + k];
-      for (uint32_t c = 0; c < w; ++c) t.col[c] = val[i_tgt + (size_t)k * w + c].ptr(b);
+      t.col = colp_d + ((size_t)b * (nt + ns) + k) * w;
       t.sel = ht ? val[i_sel_t + k].ptr(b) : nullptr;
       t.mult = cols_[st.outs[k]].lag(b, 0);
       t.n = d_.modules[mod_of(ns + k)].N;
@@ -2065,7 +2075,7 @@ void Job::step_multiplicity_general(size_t r, size_t s) {
     off = 0;
     for (uint32_t j = 0; j < ns; ++j) {  // sources
       LkTable& t = tabs[(size_t)b * (nt + ns) + nt + j];
-      for (uint32_t c = 0; c < w; ++c) t.col[c] = val[i_src + (size_t)j * w + c].ptr(b);
+      t.col = colp_d + ((size_t)b * (nt + ns) + nt + j) * w;
       t.sel = hs ? val[i_sel_s + j].ptr(b) : nullptr;
       t.mult = nullptr;
       t.n = d_.modules[mod_of(j)].N;

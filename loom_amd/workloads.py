@@ -281,7 +281,9 @@ def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
       rows, one module);
     * "union":  LookupUnion of columns of two source modules (2^log_n and
       2^(log_n-1) rows) into columns of two target modules (2^log_t and
-      2^(log_t-1) rows).
+      2^(log_t-1) rows);
+    * "wide":   LookupTuple of 11 columns of "main" into 11 columns of "tab",
+      two table rows of which differ in their last column only.
     bad: one source row takes a value in no target (the prove fails)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     n, nt = 1 << log_n, 1 << log_t
@@ -351,6 +353,25 @@ def lookup_instance(kind, seed, log_n=8, log_t=6, bad=False):
         tr.update(S0=S0, S1=S1, T0=T0, T1=T1)
         B.lookup_union(b, [B.Column("src0", E.Col("S0")), B.Column("src1", E.Col("S1"))],
                        [B.Column("tab0", E.Col("T0")), B.Column("tab1", E.Col("T1"))])
+    elif kind == "wide":
+        WD = 11
+        module("main", n)
+        module("tab", nt)
+        X = rng.integers(0, P, size=(WD, nt), dtype=np.uint64)
+        dup = 8 + rng.choice(nt - 9, size=nt // 8, replace=False)
+        X[:, dup[1:]] = X[:, dup[0]][:, None]
+        X[:, 5] = X[:, 4]
+        X[WD - 1, 5] = (X[WD - 1, 4] + 1) % P  # rows 4 and 5 differ in the last column only
+        r = rng.integers(0, nt, size=n)
+        r[0], r[1] = 4, 5
+        A = X[:, r].copy()
+        if bad:  # row 4's tuple but for its last column
+            A[:, n - 1] = X[:, 4]
+            A[WD - 1, n - 1] = (X[WD - 1, 4] + 2) % P
+        for c in range(WD):
+            tr["X%d" % c], tr["A%d" % c] = X[c].copy(), A[c].copy()
+        B.lookup_tuple(b, B.Table("main", [E.Col("A%d" % c) for c in range(WD)]),
+                       B.Table("tab", [E.Col("X%d" % c) for c in range(WD)]))
     else:
         raise ValueError(kind)
     return B.compile_program(b), tr

@@ -10,7 +10,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-KINDS = ["tuple", "cond", "union"]
+KINDS = ["tuple", "cond", "union", "wide"]
 
 
 def log(msg):
@@ -47,7 +47,7 @@ def test_general_lookup_proofs_equal_oracle(ctx, kind):
     assert all(s == 0 for s, _ in st), st
 
 
-@pytest.mark.parametrize("kind,log_n,log_t", [("tuple", 13, 12), ("cond", 12, 10), ("union", 14, 8)])
+@pytest.mark.parametrize("kind,log_n,log_t", [("tuple", 13, 12), ("cond", 12, 10), ("union", 14, 8), ("wide", 12, 10)])
 def test_general_lookup_larger(ctx, kind, log_n, log_t):
     """Hash tables of 2^13..2^15 slots, many sources per target tuple."""
     from loom_amd import workloads, prover

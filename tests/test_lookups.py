@@ -12,7 +12,7 @@ import pytest
 from loom_amd import workloads, board as B, expr as E
 
 P = E.P
-KINDS = ["tuple", "cond", "union", "self"]
+KINDS = ["tuple", "cond", "union", "self", "wide"]
 
 
 def _oracle_prover(prog, tr, nq=8):
@@ -28,6 +28,10 @@ def _brute_force(tr, kind):
     if kind in ("tuple", "cond", "self"):
         srcs = [(list(zip(tr["A"].tolist(), tr["B"].tolist())), tr.get("SA"))]
         tgts = [(list(zip(tr["X"].tolist(), tr["Y"].tolist())), tr.get("ST"))]
+    elif kind == "wide":
+        nc = sum(1 for k in tr if k.startswith("A"))
+        srcs = [(list(zip(*[tr["A%d" % c].tolist() for c in range(nc)])), None)]
+        tgts = [(list(zip(*[tr["X%d" % c].tolist() for c in range(nc)])), None)]
     else:
         srcs = [([(v,) for v in tr["S0"].tolist()], None), ([(v,) for v in tr["S1"].tolist()], None)]
         tgts = [([(v,) for v in tr["T0"].tolist()], None), ([(v,) for v in tr["T1"].tolist()], None)]
