@@ -93,9 +93,11 @@ class Plan:
         for name, mn in getattr(prog, "public_columns", []):
             if name not in self.public_rows:
                 raise KeyError("public input %r has no entry rows" % name)
-            f = E.EXT if name in getattr(prog, "public_ext", []) else E.BASE
-            if f == E.BASE and any(fe for _, fe in self.public_rows[name]):
-                raise ValueError("public input %r has extension entries: declare it with PublicInputExt" % name)
+            # the column is materialised in the extension field when it is
+            # declared there or any entry is an extension value (the join of
+            # the entries' fields, prover.go:241-268); base entries are lifted
+            ext = name in getattr(prog, "public_ext", []) or any(fe for _, fe in self.public_rows[name])
+            f = E.EXT if ext else E.BASE
             self._add(name, f, self.mod_idx[mn], SRC_PUBLIC, len(self.public))
             self.public.append(name)
         gen = {}

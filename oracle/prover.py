@@ -146,9 +146,9 @@ class Prover:
         self.pi = dict(public_inputs or {})
         self.pi_ext = {}
         for name, (mod, entries) in self.pi.items():
-            if name not in getattr(program, "public_ext", []):
-                if any(PR.public_entry_field(v) for _, v in entries):
-                    raise ValueError("public input %r has extension entries: declare it with PublicInputExt" % name)
+            # prover.go:241-268: an extension column when any entry is an
+            # extension value (or the input is declared PublicInputExt)
+            if name not in getattr(program, "public_ext", []) and not any(PR.public_entry_field(v) for _, v in entries):
                 col = np.zeros(program.modules[mod].N, dtype=np.uint64)
                 for i, v in entries:
                     col[int(i)] = int(v) % P
