@@ -1744,19 +1744,29 @@ __global__ __launch_bounds__(256) void k_claims(const ClaimSet* __restrict__ set
         }
       }
     }
+    // the wave sums of all 6 cnt words of the group (independent lane
+    // exchanges, so their latencies overlap), one barrier, then thread k
+    // adds the four wave sums of word k
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
 #pragma unroll
     for (int c = 0; c < CLAIMS_GROUP; ++c) {
       if (c < cnt) {
-        const int item = set.first + g0 + c;
 #pragma unroll
         for (int l = 0; l < 6; ++l) {
-          const uint32_t v = block_sum4(acc[c].v[l], sh[c * 6 + l]);
-          if (threadIdx.x == 0) {
-            if (S == 1) items[item].out[l] = v;
-            else part[((size_t)item * S + blockIdx.y) * 6 + l] = v;
-          }
+          uint32_t v = acc[c].v[l];
+#pragma unroll
+          for (int o = 32; o > 0; o >>= 1) v = kb_add(v, (uint32_t)__shfl_xor((int)v, o, 64));
+          if (lane == 0) sh[c * 6 + l][wv] = v;
         }
       }
+    }
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < cnt * 6) {
+      const int item = set.first + g0 + k / 6, l = k % 6;
+      const uint32_t v = kb_add(kb_add(sh[k][0], sh[k][1]), kb_add(sh[k][2], sh[k][3]));
+      if (S == 1) items[item].out[l] = v;
+      else part[((size_t)item * S + blockIdx.y) * 6 + l] = v;
     }
     __syncthreads();  // sh is reused by the next group
   }
